@@ -1,0 +1,169 @@
+/*
+ * mgp.h -- C ABI of the MI355X pressure-projection library (libmgp.so).
+ *
+ * The library replaces the kernels behind MGLET-base's Fortran module
+ * `pressuresolver_mod` (src/flow/pressuresolver_mod.F90:45, PUBLIC
+ * init_pressuresolver / mgpoisl / finish_pressuresolver) together with the
+ * coefficient setup of `ib%giteig` (src/ib/noib_mod.F90:81-250) and the
+ * intra-rank ghost-layer copies `conn` / `connect`
+ * (src/core/connect/conn1_mod.F90:31-129, connect2_mod.F90:75-138).
+ * The Fortran side binds these symbols with ISO_C_BINDING BIND(C)
+ * (mglet-base_amd/fortran/mgp_pressuresolver_mod.F90).
+ *
+ * Conventions (mirroring src/core/mglet_precision.h:12-39 with
+ * _MGLET_DOUBLE_PRECISION_ and 32-bit integers):
+ *   - every entry point returns an int status, 0 = success (the reference's
+ *     C helpers report errors through an integer, e.g.
+ *     src/core/jsoncpp_wrapper.h:10-33); mgp_last_error() explains it;
+ *   - 3-D fields use the reference layout: one flat array per field, block
+ *     igrid starting at ip3[igrid-1] (1-based, as ip3d in
+ *     src/core/pointers_mod.F90:48-92), dims kk x jj x ii, k fastest,
+ *     two ghost layers per side; 1-D metric fields start at ip1x/y/z;
+ *   - the caller owns every array it passes; the library owns its device
+ *     state and never frees caller memory.
+ */
+#ifndef MGP_H
+#define MGP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double mgletreal;
+typedef int32_t mgletint;
+
+enum {
+    MGP_OK = 0,
+    MGP_ERR_ARG = 1,       /* invalid argument or configuration            */
+    MGP_ERR_HIP = 2,       /* HIP runtime error (message in last_error)    */
+    MGP_ERR_TOPOLOGY = 3,  /* unsupported or inconsistent block topology   */
+    MGP_ERR_NONFINITE = 4, /* residual norm became NaN/Inf                 */
+    MGP_ERR_STATE = 5      /* call out of order (e.g. solve before init)   */
+};
+
+/* Block topology: GRIDINFO subset (src/core/gridio_mod.F90:15-37),
+ * itypboconds(1, iface, igrid) (src/core/grids_mod.F90:253-290) and the
+ * pointer offsets (src/core/pointers_mod.F90:23-103), all per grid. */
+typedef struct mgp_grids {
+    mgletint ngrid;
+    mgletint minlevel, maxlevel;
+    const mgletint *kk, *jj, *ii;   /* [ngrid] dims incl. ghost layers     */
+    const mgletint *level;          /* [ngrid]                              */
+    const mgletint *nbrgrid;        /* [ngrid*26] neighbour ids, 0 = none   */
+    const mgletint *itypbc;         /* [ngrid*6] first BC code per face     */
+    const int64_t *ip3;             /* [ngrid] 1-based 3-D offsets          */
+    const int64_t *ip1x, *ip1y, *ip1z; /* [ngrid] 1-based 1-D offsets       */
+} mgp_grids;
+
+/* /flow/pressuresolver keys (src/flow/pressuresolver_mod.F90:57-142) plus
+ * the build's own switches. */
+typedef struct mgp_config {
+    mgletint ityp;        /* 0 "sip" (hyperplane order), 1 "sor", 2 "classic" */
+    mgletint ninner;      /* inner iterations per outer iteration (>= 0)    */
+    mgletint nouter;      /* maximum outer iterations (>= 0)                */
+    mgletint nouter_min;  /* outer iterations before the exit test (>= 0)   */
+    mgletreal epcorr;     /* exit when maxrhs / (rho/dt) < epcorr           */
+    mgletreal omega;      /* SOR relaxation factor, 0 < omega < 2           */
+    mgletreal rho;        /* density (src/flow/flowcore_mod.F90:43-98)      */
+    mgletint ibtype;      /* 0 "noib", 1 "ghostcell" (divcal x BP, masked   */
+                          /*   velocity correction)                          */
+    mgletint precision;   /* 0 all FP64; 1 FP32 inner sweeps inside the     */
+                          /*   FP64 defect-correction (outer) loop          */
+    mgletreal eps_rel;    /* > 0: exit when maxrhs <= eps_rel * maxrhs at   */
+                          /*   iteration 0 instead of the epcorr test       */
+    mgletint sor_all_levels; /* 1: "sor" also relaxes the coarsest level    */
+    mgletint check_every; /* outer iterations per captured graph (>= 1)     */
+} mgp_config;
+
+typedef struct mgp_stats {
+    mgletint ipc;         /* inner iterations run (reference `ipc`)         */
+    mgletint nouter_done; /* outer iterations run                           */
+    mgletint converged;   /* 1 if the exit test fired before nouter          */
+    mgletreal maxrhs0;    /* level-weighted max |RHS| before iterating      */
+    mgletreal maxrhs;     /* the same after the last outer iteration        */
+    double solve_ms;      /* device time of the outer-iteration loop        */
+} mgp_stats;
+
+typedef struct mgp_handle mgp_handle;
+
+/* init_pressuresolver (src/flow/pressuresolver_mod.F90:47-145) together
+ * with ib%giteig (src/ib/noib_mod.F90:81-250): builds the connection
+ * tables, the Poisson coefficients and the SIP factors on the GPU.
+ * dx..ddz are the 1-D metric fields DX, DY, DZ, DDX, DDY, DDZ; bp is the
+ * 3-D BP field (1 fluid, 0 blocked) or NULL for all-fluid.  Host memory. */
+int mgp_init(const mgp_grids *grids, const mgp_config *cfg,
+             const mgletreal *dx, const mgletreal *dy, const mgletreal *dz,
+             const mgletreal *ddx, const mgletreal *ddy, const mgletreal *ddz,
+             const mgletreal *bp, mgp_handle **out);
+
+/* Same as mgp_init with the BP field given as an int8 mask per cell of the
+ * 3-D layout: 1 = blocked (BP = 0), 0 = fluid (BP = 1). */
+int mgp_init_mask(const mgp_grids *grids, const mgp_config *cfg,
+                  const mgletreal *dx, const mgletreal *dy,
+                  const mgletreal *dz, const mgletreal *ddx,
+                  const mgletreal *ddy, const mgletreal *ddz,
+                  const int8_t *blocked, mgp_handle **out);
+
+/* mgpoisl(u, v, w, p, dt, ittot, irk) (src/flow/pressuresolver_mod.F90:
+ * 160-357): projection of host arrays u, v, w, p in place. */
+int mgp_mgpoisl(mgp_handle *h, mgletreal *u, mgletreal *v, mgletreal *w,
+                mgletreal *p, mgletreal dt, mgletint ittot, mgletint irk,
+                mgp_stats *stats);
+
+/* The same with u, v, w, p already resident in device memory. */
+int mgp_mgpoisl_device(mgp_handle *h, mgletreal *u, mgletreal *v,
+                       mgletreal *w, mgletreal *p, mgletreal dt,
+                       mgletint ittot, mgletint irk, mgp_stats *stats);
+
+/* finish_pressuresolver (src/flow/pressuresolver_mod.F90:148-157). */
+int mgp_finish(mgp_handle *h);
+
+/* Text of the last error on this thread ("" if none). */
+const char *mgp_last_error(void);
+
+/* Change ninner / nouter / nouter_min / epcorr / eps_rel / check_every of
+ * an initialised handle (the factors are kept). */
+int mgp_set_iterations(mgp_handle *h, mgletint ninner, mgletint nouter,
+                       mgletint nouter_min, mgletreal epcorr,
+                       mgletreal eps_rel, mgletint check_every);
+
+/* Use `stream` (a hipStream_t, default priority) for all work of h;
+ * NULL restores the handle's own stream. */
+int mgp_set_stream(mgp_handle *h, void *stream);
+
+/* Copy a library field to host memory in the reference layout.  Names:
+ * "GSAP", "BP", "SIPLW", "SIPLS", "SIPLB", "SIPLPR", "SIPUE", "SIPUN",
+ * "SIPUT" (the factors in hyperplane access order, as
+ * src/flow/sip_hyperplane_mod.F90:224-230 stores them, or spatial order
+ * for ityp 2), "DP", "RHS" (state after the last mgpoisl).  1-D fields
+ * "GSAW", "GSAE", "GSAS", "GSAN", "GSAB", "GSAT" use the 1-D layout. */
+int mgp_get_field(mgp_handle *h, const char *name, mgletreal *out);
+
+/* Ghost-layer exchange of device fields in the reference layout
+ * (conn / connect: src/core/connect/conn1_mod.F90:31-129,
+ * connect2_mod.F90:75-138).  level < 0: all levels; layers 1 or 2;
+ * corners 0/1; forward 0, 1 or -1. */
+int mgp_connect_device(mgp_handle *h, mgletint level, mgletint layers,
+                       mgletint corners, mgletint forward,
+                       mgletreal **fields, mgletint nfield);
+
+/* ib%divcal (src/ib/noib_mod.F90:253-354): div = fak * div(u, v, w) over
+ * the interior of device fields (x BP for ibtype 1). */
+int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
+                      const mgletreal *v, const mgletreal *w,
+                      mgletreal fak);
+
+/* Times `niter` inner SIP iterations on the current RHS with HIP events
+ * around each kernel; writes the mean kernel durations in ms:
+ * ms[0] forward sweep, ms[1] backward sweep, ms[2] one outer update. */
+int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms);
+
+/* Number of interior cells and bytes of device memory held by h. */
+int mgp_info(mgp_handle *h, int64_t *ncells, int64_t *device_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MGP_H */

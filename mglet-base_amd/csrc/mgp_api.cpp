@@ -1,0 +1,1132 @@
+// mgp_api.cpp -- host orchestration and the C ABI of libmgp.so.
+//
+// Mirrors the Fortran interface of src/flow/pressuresolver_mod.F90:
+// mgp_init ~ init_pressuresolver (+ ib%giteig), mgp_mgpoisl ~ mgpoisl,
+// mgp_finish ~ finish_pressuresolver.  All work is issued on one HIP
+// stream per handle; the outer-iteration loop is captured into a HIP graph
+// and its exit test runs on the device, so the host only polls a flag every
+// `check_every` outer iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/mgp.h"
+#include "mgp_launch.h"
+
+using namespace mgp;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+struct HipError {
+    std::string msg;
+};
+
+void hip_check(hipError_t e, const char *what, int line) {
+    if (e != hipSuccess) {
+        char buf[512];
+        snprintf(buf, sizeof buf, "%s failed (line %d): %s", what, line,
+                 hipGetErrorString(e));
+        throw HipError{buf};
+    }
+}
+#define HC(x) hip_check((x), #x, __LINE__)
+
+// device buffer owned by a handle
+struct DBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    void alloc(size_t bytes) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        n = bytes;
+        if (bytes) HC(hipMalloc(&p, bytes));
+    }
+    template <typename T> T *as() const { return (T *)p; }
+};
+
+// Faces and directions (src/core/connect/connect_core_mod.F90:37-165): the
+// face numbering is part of the data format.
+const int FACELIST[26][4] = {
+    {1, 1, 0, 0}, {1, 2, 0, 0}, {1, 3, 0, 0}, {1, 4, 0, 0}, {1, 5, 0, 0},
+    {1, 6, 0, 0}, {2, 1, 3, 0}, {2, 1, 4, 0}, {2, 1, 5, 0}, {2, 1, 6, 0},
+    {2, 2, 3, 0}, {2, 2, 4, 0}, {2, 2, 5, 0}, {2, 2, 6, 0}, {2, 3, 5, 0},
+    {2, 3, 6, 0}, {2, 4, 5, 0}, {2, 4, 6, 0}, {3, 1, 3, 5}, {3, 1, 3, 6},
+    {3, 1, 4, 5}, {3, 1, 4, 6}, {3, 2, 3, 5}, {3, 2, 3, 6}, {3, 2, 4, 5},
+    {3, 2, 4, 6}};
+const int FACENBR[26] = {2,  1,  4,  3,  6,  5,  12, 11, 14, 13, 8,  7,  10,
+                         9,  18, 17, 16, 15, 26, 25, 24, 23, 22, 21, 20, 19};
+// neighbour search order per receiving direction and the face it is sent
+// from (rescue_dir / rescue_nbr, connect_core_mod.F90:76-151)
+const int RESCUE_DIR[26][7] = {
+    {1}, {2}, {3}, {4}, {5}, {6}, {7, 1, 3}, {8, 4, 1}, {9, 1, 5},
+    {10, 6, 1}, {11, 2, 3}, {12, 2, 4}, {13, 2, 5}, {14, 2, 6}, {15, 3, 5},
+    {16, 6, 3}, {17, 4, 5}, {18, 4, 6}, {19, 7, 9, 15, 1, 3, 5},
+    {20, 7, 10, 16, 1, 3, 6}, {21, 8, 9, 17, 1, 4, 5},
+    {22, 8, 10, 18, 1, 4, 6}, {23, 11, 13, 15, 2, 3, 5},
+    {24, 11, 14, 16, 2, 3, 6}, {25, 12, 13, 17, 2, 4, 5},
+    {26, 12, 14, 18, 2, 4, 6}};
+const int RESCUE_NBR[26][7] = {
+    {2}, {1}, {4}, {3}, {6}, {5}, {12, 11, 8}, {11, 7, 12}, {14, 13, 10},
+    {13, 9, 14}, {8, 7, 12}, {7, 8, 11}, {10, 9, 14}, {9, 10, 13},
+    {18, 17, 16}, {17, 15, 18}, {16, 15, 18}, {15, 16, 17},
+    {26, 25, 24, 22, 23, 21, 20}, {25, 26, 23, 21, 24, 22, 19},
+    {24, 23, 26, 20, 25, 19, 22}, {23, 24, 25, 19, 26, 20, 21},
+    {22, 21, 20, 26, 19, 25, 24}, {21, 22, 19, 25, 20, 26, 23},
+    {20, 19, 22, 24, 21, 23, 26}, {19, 20, 21, 23, 22, 24, 25}};
+
+bool is_con(int c) { return c == 7 || c == 19; }
+bool is_wall_lo(int c) { return c == 2 || c == 5 || c == 6 || c == 19; }
+bool is_wall_hi(int c) { return c == 2 || c == 5 || c == 6; }
+
+struct Conn {
+    int recv, send, frecv, fsend;  // 1-based grid ids, face numbers
+};
+
+struct Box {
+    int is, ie, js, je, ks, ke;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+struct mgp_handle {
+    // configuration and topology (host)
+    mgp_config cfg{};
+    int ngrid = 0, minlevel = 0, maxlevel = 0;
+    std::vector<int> kk, jj, ii, level, nbr, bc;
+    std::vector<int64_t> ip3, ip1x, ip1y, ip1z;  // 0-based
+    int64_t n3d = 0, n1x = 0, n1y = 0, n1z = 0, ncell = 0, nface = 0;
+    std::vector<Shape> shapes;
+    std::vector<Block> blocks;
+    std::vector<Conn> conns;
+    std::map<int, std::vector<int>> level_blocks;  // level -> block indices
+    bool masked = false;
+    int max_ni = 0, max_nj = 0, max_nk = 0;
+
+    hipStream_t own_stream = nullptr, stream = nullptr;
+
+    // device topology
+    DBuf d_blocks, d_shapes, d_cells, d_pstart, d_slot, d_blist_all;
+    std::map<int, DBuf> d_blist_lvl;
+    // 1-D fields (FP64 and working precision)
+    DBuf d_dx, d_dy, d_dz, d_rdx, d_rdy, d_rdz, d_rddx, d_rddy, d_rddz;
+    DBuf d_raw[6], d_c64[6], d_cT[6];
+    // per-cell WO arrays
+    DBuf d_ap64, d_apT, d_f[7], d_hilf, d_rhs64, d_rhsT, d_res, d_dp64;
+    DBuf d_bpw, d_bpf, d_bpsp;
+    DBuf d_hface, d_rhi[2], d_rlo;
+    // reference-layout work arrays
+    DBuf d_u, d_v, d_w, d_p, d_dpsp;
+    // scalars and history
+    DBuf d_scal, d_hist, d_never;
+    // exchange tasks: key (level, layers, corners, forward)
+    struct TaskSet {
+        DBuf d;
+        std::vector<int> stage_start;  // per stage offsets (+ end)
+    };
+    std::map<std::tuple<int, int, int, int>, std::unique_ptr<TaskSet>> tasks;
+    // captured outer-iteration graphs per level
+    hipGraphExec_t graph = nullptr;
+    int graph_key[4] = {-1, -1, -1, -1};
+    bool have_solution = false;
+
+    ~mgp_handle() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (own_stream) (void)hipStreamDestroy(own_stream);
+    }
+
+    bool fp32() const { return cfg.precision == 1; }
+    size_t tsz() const { return fp32() ? 4 : 8; }
+
+    Topo topo(int lvl) const {
+        Topo t;
+        t.blocks = d_blocks.as<Block>();
+        t.shapes = d_shapes.as<Shape>();
+        t.cells = d_cells.as<uint32_t>();
+        t.pstart = d_pstart.as<int>();
+        t.slot_of = d_slot.as<int>();
+        t.max_ni = max_ni;
+        t.max_nj = max_nj;
+        t.max_nk = max_nk;
+        t.maxlevel = maxlevel;
+        if (lvl < 0) {
+            t.blist = d_blist_all.as<int>();
+            t.nlist = ngrid;
+        } else {
+            auto it = d_blist_lvl.find(lvl);
+            t.blist = it->second.as<int>();
+            t.nlist = (int)level_blocks.at(lvl).size();
+        }
+        return t;
+    }
+
+    Mask mask() const {
+        Mask m;
+        m.bp = masked ? d_bpw.as<uint8_t>() : nullptr;
+        m.bpf = masked ? d_bpf.as<uint8_t>() : nullptr;
+        return m;
+    }
+
+    double *scal() const { return d_scal.as<double>(); }
+    Outer outer() const {
+        Outer o;
+        double *s = scal();
+        o.rhs64 = d_rhs64.as<double>();
+        o.dp64 = d_dp64.as<double>();
+        o.ap64 = d_ap64.as<double>();
+        o.aw = d_c64[0].as<double>();
+        o.ae = d_c64[1].as<double>();
+        o.as = d_c64[2].as<double>();
+        o.an = d_c64[3].as<double>();
+        o.ab = d_c64[4].as<double>();
+        o.at = d_c64[5].as<double>();
+        o.gmax = (unsigned long long *)(s + 0);
+        o.maxrhs0 = s + 1;
+        o.maxrhs = s + 2;
+        int *iv = (int *)(s + 3);
+        o.ctr = iv + 0;
+        o.done = iv + 1;
+        o.converged = iv + 2;
+        o.nonfinite = iv + 3;
+        return o;
+    }
+
+    template <typename T>
+    Fields<T> fields() const {
+        Fields<T> F;
+        F.hilf = d_hilf.as<T>();
+        F.rhs = fp32() ? d_rhsT.as<T>() : (T *)d_rhs64.as<T>();
+        F.res = d_res.as<T>();
+        F.ap = fp32() ? d_apT.as<T>() : (T *)d_ap64.as<T>();
+        for (int q = 0; q < 7; ++q) F.f[q] = d_f[q].as<T>();
+        const T **c[6] = {&F.aw, &F.ae, &F.as, &F.an, &F.ab, &F.at};
+        for (int q = 0; q < 6; ++q)
+            *c[q] = fp32() ? d_cT[q].as<T>() : d_c64[q].as<T>();
+        F.hface = d_hface.as<T>();
+        F.rhi[0] = d_rhi[0].as<T>();
+        F.rhi[1] = d_rhi[1].as<T>();
+        F.rlo = d_rlo.as<T>();
+        return F;
+    }
+};
+
+namespace {
+
+// ------------------------------------------------------------------
+// Connection tables: src/core/connect/connect_core_mod.F90:167-433.
+void get_nbrs(const mgp_handle &h, int g, int iface, int &ng, int &nf) {
+    ng = nf = 0;
+    const int *nb = &h.nbr[26 * (g - 1)];
+    for (int r = 0; r < 7; ++r) {
+        const int dir = RESCUE_DIR[iface - 1][r];
+        if (dir == 0) break;
+        if (nb[dir - 1] > 0) {
+            ng = nb[dir - 1];
+            nf = RESCUE_NBR[iface - 1][r];
+            if (nf > 18) {
+                const int *bcn = &h.bc[6 * (ng - 1)];
+                bool any = false;
+                for (int a = 1; a <= 3; ++a)
+                    any = any || is_con(bcn[FACELIST[nf - 1][a] - 1]);
+                if (!any) {
+                    ng = nf = 0;
+                    continue;
+                }
+            }
+            break;
+        }
+    }
+}
+
+void build_conns(mgp_handle &h) {
+    h.conns.clear();
+    for (int g = 1; g <= h.ngrid; ++g) {
+        const int *bc = &h.bc[6 * (g - 1)];
+        for (int f = 1; f <= 26; ++f) {
+            bool want = false;
+            for (int a = 1; a <= FACELIST[f - 1][0]; ++a)
+                want = want || is_con(bc[FACELIST[f - 1][a] - 1]);
+            if (!want) continue;
+            int ng, nf;
+            get_nbrs(h, g, f, ng, nf);
+            if (ng == 0) continue;
+            h.conns.push_back({g, ng, f, nf});
+        }
+    }
+}
+
+// start_and_stop(_face) / corr_start_stop, connect_core_mod.F90:454-602
+void ss_face(const mgp_handle &h, int g, int f, Box &b, int np, bool ng) {
+    const int kk = h.kk[g - 1], jj = h.jj[g - 1], ii = h.ii[g - 1];
+    switch (f) {
+    case 1: b.is = 3 - np; b.ie = 2; break;
+    case 2: b.is = ii - 1; b.ie = ii - (2 - np); break;
+    case 3: b.js = 3 - np; b.je = 2; break;
+    case 4: b.js = jj - 1; b.je = jj - (2 - np); break;
+    case 5: b.ks = 3 - np; b.ke = 2; break;
+    case 6: b.ks = kk - 1; b.ke = kk - (2 - np); break;
+    }
+    if (ng) {
+        switch (f) {
+        case 1: b.is += np; b.ie += np; break;
+        case 2: b.is -= np; b.ie -= np; break;
+        case 3: b.js += np; b.je += np; break;
+        case 4: b.js -= np; b.je -= np; break;
+        case 5: b.ks += np; b.ke += np; break;
+        case 6: b.ks -= np; b.ke -= np; break;
+        }
+    }
+}
+
+Box ss(const mgp_handle &h, int g, int f, int np, bool ng) {
+    Box b{3, h.ii[g - 1] - 2, 3, h.jj[g - 1] - 2, 3, h.kk[g - 1] - 2};
+    for (int a = 1; a <= FACELIST[f - 1][0]; ++a)
+        ss_face(h, g, FACELIST[f - 1][a], b, np, ng);
+    return b;
+}
+
+bool overlap(const Box &a, const Box &b) {
+    return a.is <= b.ie && b.is <= a.ie && a.js <= b.je && b.js <= a.je &&
+           a.ks <= b.ke && b.ks <= a.ke;
+}
+
+// Copy tasks of one exchange, grouped into stages that keep the
+// reference's sequential order wherever two copies touch the same cells
+// (only possible when a source box reaches into a ghost layer).
+mgp_handle::TaskSet &task_set(mgp_handle &h, int level, int layers,
+                              int corners, int forward) {
+    auto key = std::make_tuple(level, layers, corners, forward);
+    auto it = h.tasks.find(key);
+    if (it != h.tasks.end()) return *it->second;
+    const int minl = level >= 0 ? level : h.minlevel;
+    const int maxl = level >= 0 ? level : h.maxlevel;
+    struct T2 {
+        CopyTask t;
+        int sg, dg;
+        Box sb, db;
+    };
+    std::vector<T2> list;
+    for (const Conn &c : h.conns) {
+        const int fr = c.frecv;
+        const int lvl = h.level[c.recv - 1];
+        if (lvl > maxl || lvl < minl) continue;          // decide()
+        if (!corners && fr > 6) continue;
+        if (forward == 1 && (fr == 2 || fr == 4 || fr == 6)) continue;
+        if (forward == -1 && (fr == 1 || fr == 3 || fr == 5)) continue;
+        Box sb = ss(h, c.send, FACENBR[fr - 1], layers, true);
+        if (fr > 6)
+            for (int a = 1; a <= FACELIST[c.fsend - 1][0]; ++a)
+                for (int q = 1; q <= FACELIST[fr - 1][0]; ++q)
+                    if (FACELIST[c.fsend - 1][a] == FACELIST[fr - 1][q])
+                        ss_face(h, c.send, FACELIST[c.fsend - 1][a], sb,
+                                layers, false);
+        Box db = ss(h, c.recv, fr, layers, false);
+        const int ni = db.ie - db.is + 1, nj = db.je - db.js + 1,
+                  nk = db.ke - db.ks + 1;
+        if ((sb.ie - sb.is + 1) * (sb.je - sb.js + 1) * (sb.ke - sb.ks + 1) !=
+            ni * nj * nk)
+            throw HipError{"inconsistent connection box sizes"};
+        const int64_t kks = h.kk[c.send - 1], jjs = h.jj[c.send - 1];
+        const int64_t kkd = h.kk[c.recv - 1], jjd = h.jj[c.recv - 1];
+        CopyTask t;
+        t.src = h.ip3[c.send - 1] + (sb.ks - 1) + (sb.js - 1) * kks +
+                (sb.is - 1) * kks * jjs;
+        t.dst = h.ip3[c.recv - 1] + (db.ks - 1) + (db.js - 1) * kkd +
+                (db.is - 1) * kkd * jjd;
+        t.ni = ni; t.nj = nj; t.nk = nk;
+        t.s_sj = (int)kks; t.s_si = (int)(kks * jjs);
+        t.d_sj = (int)kkd; t.d_si = (int)(kkd * jjd);
+        list.push_back({t, c.send, c.recv, sb, db});
+    }
+    // staging
+    std::vector<int> stage(list.size(), 0);
+    std::vector<std::vector<int>> wr(h.ngrid + 1), rdl(h.ngrid + 1);
+    int nst = list.empty() ? 0 : 1;
+    for (size_t q = 0; q < list.size(); ++q) {
+        int st = 0;
+        for (int u : wr[list[q].sg])
+            if (overlap(list[u].db, list[q].sb)) st = std::max(st, stage[u] + 1);
+        for (int u : rdl[list[q].dg])
+            if (overlap(list[u].sb, list[q].db)) st = std::max(st, stage[u] + 1);
+        for (int u : wr[list[q].dg])
+            if (overlap(list[u].db, list[q].db)) st = std::max(st, stage[u] + 1);
+        stage[q] = st;
+        nst = std::max(nst, st + 1);
+        wr[list[q].dg].push_back((int)q);
+        rdl[list[q].sg].push_back((int)q);
+    }
+    std::vector<CopyTask> sorted;
+    auto ts = std::make_unique<mgp_handle::TaskSet>();
+    for (int s = 0; s < nst; ++s) {
+        ts->stage_start.push_back((int)sorted.size());
+        for (size_t q = 0; q < list.size(); ++q)
+            if (stage[q] == s) sorted.push_back(list[q].t);
+    }
+    ts->stage_start.push_back((int)sorted.size());
+    ts->d.alloc(sorted.size() * sizeof(CopyTask));
+    if (!sorted.empty())
+        HC(hipMemcpy(ts->d.p, sorted.data(), sorted.size() * sizeof(CopyTask),
+                     hipMemcpyHostToDevice));
+    auto &ref = *ts;
+    h.tasks[key] = std::move(ts);
+    return ref;
+}
+
+void exchange(mgp_handle &h, int level, int layers, int corners, int forward,
+              double *const *fields, int nfield) {
+    auto &ts = task_set(h, level, layers, corners, forward);
+    for (size_t s = 0; s + 1 < ts.stage_start.size(); ++s) {
+        const int a = ts.stage_start[s], b = ts.stage_start[s + 1];
+        for (int f0 = 0; f0 < nfield; f0 += 6)
+            launch_copy(ts.d.as<CopyTask>() + a, b - a, fields + f0,
+                        std::min(6, nfield - f0), h.stream);
+    }
+}
+
+// ------------------------------------------------------------------
+// Shape tables: hyperplane order of src/flow/sip_hyperplane_mod.F90:
+// 269-361 (planes m = i+j+k, ascending linear index inside a plane, i.e.
+// i ascending then j ascending).
+void build_shapes(mgp_handle &h, std::vector<uint32_t> &cells,
+                  std::vector<int> &pstart, std::vector<int> &slot) {
+    std::map<std::tuple<int, int, int>, int> idx;
+    h.blocks.resize(h.ngrid);
+    int64_t wo = 0, fc = 0;
+    for (int g = 0; g < h.ngrid; ++g) {
+        const int ni = h.ii[g] - 4, nj = h.jj[g] - 4, nk = h.kk[g] - 4;
+        auto key = std::make_tuple(ni, nj, nk);
+        auto it = idx.find(key);
+        int sid;
+        if (it == idx.end()) {
+            Shape S;
+            S.ni = ni; S.nj = nj; S.nk = nk;
+            S.np = ni + nj + nk - 2;
+            S.ncell = ni * nj * nk;
+            S.cell_off = (int64_t)cells.size();
+            S.pstart_off = (int64_t)pstart.size();
+            S.slot_off = (int64_t)slot.size();
+            S.fsize = 2 * (nk * nj + nk * ni + nj * ni);
+            slot.resize(slot.size() + S.ncell);
+            int s = 0;
+            for (int p = 0; p < S.np; ++p) {
+                pstart.push_back(s);
+                for (int i = 0; i < ni; ++i)
+                    for (int j = 0; j < nj; ++j) {
+                        const int k = p - i - j;
+                        if (k < 0 || k >= nk) continue;
+                        cells.push_back((uint32_t)i | ((uint32_t)j << 10) |
+                                        ((uint32_t)k << 20));
+                        slot[S.slot_off + k + nk * (j + nj * i)] = s;
+                        ++s;
+                    }
+            }
+            pstart.push_back(s);
+            sid = (int)h.shapes.size();
+            h.shapes.push_back(S);
+            idx[key] = sid;
+        } else {
+            sid = it->second;
+        }
+        Block &B = h.blocks[g];
+        B.shape = sid;
+        B.ni = ni; B.nj = nj; B.nk = nk;
+        B.level = h.level[g];
+        B.flags = 0;
+        for (int f = 0; f < 6; ++f) {
+            B.bc[f] = h.bc[6 * g + f];
+            const int ng = h.nbr[26 * g + f];
+            B.nbr[f] = (is_con(B.bc[f]) && ng > 0) ? ng - 1 : -1;
+        }
+        B.wo = wo;
+        B.sp = h.ip3[g];
+        B.fc = fc;
+        B.o1x = h.ip1x[g]; B.o1y = h.ip1y[g]; B.o1z = h.ip1z[g];
+        wo += (int64_t)ni * nj * nk;
+        fc += h.shapes[sid].fsize;
+        h.max_ni = std::max(h.max_ni, ni);
+        h.max_nj = std::max(h.max_nj, nj);
+        h.max_nk = std::max(h.max_nk, nk);
+    }
+    h.ncell = wo;
+    h.nface = fc;
+}
+
+template <typename T>
+void upload(DBuf &d, const std::vector<T> &v) {
+    d.alloc(v.size() * sizeof(T));
+    if (!v.empty())
+        HC(hipMemcpy(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+void upload_raw(DBuf &d, const void *src, size_t bytes) {
+    d.alloc(bytes);
+    if (bytes) HC(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
+}
+
+int check_cfg(const mgp_config &c) {
+    if (c.ityp < 0 || c.ityp > 2) return fail(MGP_ERR_ARG, "invalid pressure solver type %d", c.ityp);
+    if (c.ninner < 0) return fail(MGP_ERR_ARG, "Invalid number of ninner: %d", c.ninner);
+    if (c.nouter < 0) return fail(MGP_ERR_ARG, "Invalid number of nouter: %d", c.nouter);
+    if (c.nouter_min < 0) return fail(MGP_ERR_ARG, "Invalid number of nouter_min: %d", c.nouter_min);
+    if (c.epcorr < 0.0) return fail(MGP_ERR_ARG, "Invalid epcorr: %g", c.epcorr);
+    if (c.ityp == 1 && (c.omega <= 0.0 || c.omega >= 2.0))
+        return fail(MGP_ERR_ARG, "Invalid omega: %g", c.omega);
+    if (c.precision < 0 || c.precision > 1) return fail(MGP_ERR_ARG, "invalid precision %d", c.precision);
+    if (c.ibtype < 0 || c.ibtype > 1) return fail(MGP_ERR_ARG, "invalid ibtype %d", c.ibtype);
+    if (!(c.rho > 0.0)) return fail(MGP_ERR_ARG, "invalid rho %g", c.rho);
+    return MGP_OK;
+}
+
+// ------------------------------------------------------------------
+template <typename T>
+void setup_factors(mgp_handle &h) {
+    Topo t = h.topo(-1);
+    T *f[7];
+    for (int q = 0; q < 7; ++q) f[q] = h.d_f[q].as<T>();
+    launch_lu<T>(t, h.d_ap64.as<double>(), h.d_c64[0].as<double>(),
+                 h.d_c64[1].as<double>(), h.d_c64[2].as<double>(),
+                 h.d_c64[3].as<double>(), h.d_c64[4].as<double>(),
+                 h.d_c64[5].as<double>(), h.mask(), f, h.stream);
+    HC(hipGetLastError());
+}
+
+int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
+                const double *dy, const double *dz, const double *ddx,
+                const double *ddy, const double *ddz, const double *bp,
+                const int8_t *blocked, mgp_handle **out) {
+    if (!G || !cfg || !out || !dx || !dy || !dz || !ddx || !ddy || !ddz)
+        return fail(MGP_ERR_ARG, "null argument");
+    *out = nullptr;
+    int rc = check_cfg(*cfg);
+    if (rc) return rc;
+    if (G->ngrid <= 0) return fail(MGP_ERR_TOPOLOGY, "no grids");
+    auto h = std::make_unique<mgp_handle>();
+    h->cfg = *cfg;
+    if (h->cfg.check_every < 1) h->cfg.check_every = 1;
+    h->ngrid = G->ngrid;
+    h->minlevel = G->minlevel;
+    h->maxlevel = G->maxlevel;
+    const int n = G->ngrid;
+    h->kk.assign(G->kk, G->kk + n);
+    h->jj.assign(G->jj, G->jj + n);
+    h->ii.assign(G->ii, G->ii + n);
+    h->level.assign(G->level, G->level + n);
+    h->nbr.assign(G->nbrgrid, G->nbrgrid + 26 * n);
+    h->bc.assign(G->itypbc, G->itypbc + 6 * n);
+    for (int g = 0; g < n; ++g) {
+        h->ip3.push_back(G->ip3[g] - 1);
+        h->ip1x.push_back(G->ip1x[g] - 1);
+        h->ip1y.push_back(G->ip1y[g] - 1);
+        h->ip1z.push_back(G->ip1z[g] - 1);
+        const int ni = h->ii[g] - 4, nj = h->jj[g] - 4, nk = h->kk[g] - 4;
+        if (ni < 2 || nj < 2 || nk < 2 || ni >= kMaxN || nj >= kMaxN || nk >= kMaxN)
+            return fail(MGP_ERR_TOPOLOGY, "grid %d: unsupported dims", g + 1);
+        if (h->level[g] < h->minlevel || h->level[g] > h->maxlevel)
+            return fail(MGP_ERR_TOPOLOGY, "grid %d: level out of range", g + 1);
+        for (int f = 0; f < 6; ++f)
+            if (h->bc[6 * g + f] == 8)
+                return fail(MGP_ERR_TOPOLOGY,
+                            "grid %d face %d: PAR (parent-coupled) faces are "
+                            "not supported by this build", g + 1, f + 1);
+        h->level_blocks[h->level[g]].push_back(g);
+        if (h->level[g] != h->level[0])
+            return fail(MGP_ERR_TOPOLOGY,
+                        "grids on more than one level (ctof/ftoc/parent "
+                        "coupling) are not supported by this build");
+        h->n3d = std::max<int64_t>(h->n3d, h->ip3[g] + (int64_t)h->ii[g] * h->jj[g] * h->kk[g]);
+        h->n1x = std::max<int64_t>(h->n1x, h->ip1x[g] + h->ii[g]);
+        h->n1y = std::max<int64_t>(h->n1y, h->ip1y[g] + h->jj[g]);
+        h->n1z = std::max<int64_t>(h->n1z, h->ip1z[g] + h->kk[g]);
+    }
+    // a CON face must see a neighbour of matching face size
+    for (int g = 0; g < n; ++g)
+        for (int f = 0; f < 6; ++f) {
+            const int ng = h->nbr[26 * g + f];
+            if (!is_con(h->bc[6 * g + f]) || ng <= 0) continue;
+            if (ng > n) return fail(MGP_ERR_TOPOLOGY, "grid %d: neighbour id %d out of range", g + 1, ng);
+            const int a = ng - 1;
+            const bool ok = (f < 2) ? (h->jj[a] == h->jj[g] && h->kk[a] == h->kk[g])
+                          : (f < 4) ? (h->ii[a] == h->ii[g] && h->kk[a] == h->kk[g])
+                                    : (h->ii[a] == h->ii[g] && h->jj[a] == h->jj[g]);
+            if (!ok || h->level[a] != h->level[g])
+                return fail(MGP_ERR_TOPOLOGY, "grid %d face %d: neighbour %d does not match", g + 1, f + 1, ng);
+        }
+    try {
+        HC(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking));
+        h->stream = h->own_stream;
+        std::vector<uint32_t> cells;
+        std::vector<int> pstart, slot;
+        build_shapes(*h, cells, pstart, slot);
+        build_conns(*h);
+
+        // BP: per WO cell, own ghost layer per face, reference layout
+        std::vector<double> bpd;
+        if (bp || blocked) {
+            bpd.resize(h->n3d);
+            for (int64_t c = 0; c < h->n3d; ++c)
+                bpd[c] = bp ? bp[c] : (blocked[c] ? 0.0 : 1.0);
+        }
+        std::vector<uint8_t> bpw, bpf;
+        if (!bpd.empty()) {
+            bpw.resize(h->ncell);
+            bpf.resize(h->nface);
+            for (int g = 0; g < n; ++g) {
+                Block &B = h->blocks[g];
+                const Shape &S = h->shapes[B.shape];
+                const int kk = h->kk[g], jj = h->jj[g];
+                auto at = [&](int i, int j, int k) {  // 0-based interior, ghosts -1/N
+                    return bpd[h->ip3[g] + (k + 2) + (int64_t)(j + 2) * kk +
+                               (int64_t)(i + 2) * kk * jj];
+                };
+                bool all1 = true;
+                for (int i = -1; i <= B.ni; ++i)
+                    for (int j = -1; j <= B.nj; ++j)
+                        for (int k = -1; k <= B.nk; ++k) {
+                            const int out = (i < 0 || i >= B.ni) + (j < 0 || j >= B.nj) + (k < 0 || k >= B.nk);
+                            if (out > 1) continue;
+                            const double v = at(i, j, k);
+                            if (v != 0.0 && v != 1.0)
+                                throw HipError{"BP must be 0 or 1"};
+                            if (v != 1.0) all1 = false;
+                        }
+                if (all1) B.flags |= 1;
+                for (int i = 0; i < B.ni; ++i)
+                    for (int j = 0; j < B.nj; ++j)
+                        for (int k = 0; k < B.nk; ++k)
+                            bpw[B.wo + slot[S.slot_off + k + B.nk * (j + B.nj * i)]] = (uint8_t)at(i, j, k);
+                uint8_t *f = &bpf[B.fc];
+                for (int j = 0; j < B.nj; ++j)
+                    for (int k = 0; k < B.nk; ++k) {
+                        f[face_base(0, B.ni, B.nj, B.nk) + k + B.nk * j] = (uint8_t)at(-1, j, k);
+                        f[face_base(1, B.ni, B.nj, B.nk) + k + B.nk * j] = (uint8_t)at(B.ni, j, k);
+                    }
+                for (int i = 0; i < B.ni; ++i)
+                    for (int k = 0; k < B.nk; ++k) {
+                        f[face_base(2, B.ni, B.nj, B.nk) + k + B.nk * i] = (uint8_t)at(i, -1, k);
+                        f[face_base(3, B.ni, B.nj, B.nk) + k + B.nk * i] = (uint8_t)at(i, B.nj, k);
+                    }
+                for (int i = 0; i < B.ni; ++i)
+                    for (int j = 0; j < B.nj; ++j) {
+                        f[face_base(4, B.ni, B.nj, B.nk) + j + B.nj * i] = (uint8_t)at(i, j, -1);
+                        f[face_base(5, B.ni, B.nj, B.nk) + j + B.nj * i] = (uint8_t)at(i, j, B.nk);
+                    }
+                if (!all1) h->masked = true;
+            }
+        } else {
+            for (auto &B : h->blocks) B.flags |= 1;
+        }
+
+        upload(h->d_blocks, h->blocks);
+        upload(h->d_shapes, h->shapes);
+        upload(h->d_cells, cells);
+        upload(h->d_pstart, pstart);
+        upload(h->d_slot, slot);
+        std::vector<int> all(n);
+        for (int g = 0; g < n; ++g) all[g] = g;
+        upload(h->d_blist_all, all);
+        for (auto &kv : h->level_blocks) upload(h->d_blist_lvl[kv.first], kv.second);
+        if (h->masked) {
+            upload(h->d_bpw, bpw);
+            upload(h->d_bpf, bpf);
+        }
+        if (!bpd.empty() && (h->cfg.ibtype == 1)) upload(h->d_bpsp, bpd);
+
+        // 1-D metrics and coefficients (src/ib/noib_mod.F90:141-152,
+        // fold-in zeroing :178-244; reciprocals src/core/corefields_mod.F90:
+        // 154-173 with divide0)
+        auto recip = [](const double *d, int64_t m) {
+            std::vector<double> r(m);
+            for (int64_t q = 0; q < m; ++q) r[q] = d[q] == 0.0 ? 0.0 : 1.0 / d[q];
+            return r;
+        };
+        upload_raw(h->d_dx, dx, h->n1x * 8);
+        upload_raw(h->d_dy, dy, h->n1y * 8);
+        upload_raw(h->d_dz, dz, h->n1z * 8);
+        upload(h->d_rdx, recip(dx, h->n1x));
+        upload(h->d_rdy, recip(dy, h->n1y));
+        upload(h->d_rdz, recip(dz, h->n1z));
+        upload(h->d_rddx, recip(ddx, h->n1x));
+        upload(h->d_rddy, recip(ddy, h->n1y));
+        upload(h->d_rddz, recip(ddz, h->n1z));
+        std::vector<double> raw[6], fold[6];
+        raw[0].assign(h->n1x, 0.0); raw[1].assign(h->n1x, 0.0);
+        raw[2].assign(h->n1y, 0.0); raw[3].assign(h->n1y, 0.0);
+        raw[4].assign(h->n1z, 0.0); raw[5].assign(h->n1z, 0.0);
+        for (int g = 0; g < n; ++g) {
+            const double *d[3] = {dx + h->ip1x[g] - 1, dy + h->ip1y[g] - 1, dz + h->ip1z[g] - 1};
+            double *lo[3] = {raw[0].data() + h->ip1x[g] - 1, raw[2].data() + h->ip1y[g] - 1, raw[4].data() + h->ip1z[g] - 1};
+            double *hi[3] = {raw[1].data() + h->ip1x[g] - 1, raw[3].data() + h->ip1y[g] - 1, raw[5].data() + h->ip1z[g] - 1};
+            const int nn[3] = {h->ii[g], h->jj[g], h->kk[g]};
+            for (int a = 0; a < 3; ++a)
+                for (int q = 3; q <= nn[a] - 2; ++q) {
+                    hi[a][q] = 2.0 / ((d[a][q - 1] + d[a][q]) * d[a][q]);
+                    lo[a][q] = 2.0 / ((d[a][q - 1] + d[a][q]) * d[a][q - 1]);
+                }
+        }
+        for (int q = 0; q < 6; ++q) fold[q] = raw[q];
+        for (int g = 0; g < n; ++g) {
+            const int *bc = &h->bc[6 * g];
+            if (is_wall_lo(bc[0])) fold[0][h->ip1x[g] + 2] = 0.0;
+            if (is_wall_hi(bc[1])) fold[1][h->ip1x[g] + h->ii[g] - 3] = 0.0;
+            if (is_wall_lo(bc[2])) fold[2][h->ip1y[g] + 2] = 0.0;
+            if (is_wall_hi(bc[3])) fold[3][h->ip1y[g] + h->jj[g] - 3] = 0.0;
+            if (is_wall_lo(bc[4])) fold[4][h->ip1z[g] + 2] = 0.0;
+            if (is_wall_hi(bc[5])) fold[5][h->ip1z[g] + h->kk[g] - 3] = 0.0;
+        }
+        for (int q = 0; q < 6; ++q) {
+            upload(h->d_raw[q], raw[q]);
+            upload(h->d_c64[q], fold[q]);
+            if (h->fp32()) {
+                std::vector<float> fq(fold[q].begin(), fold[q].end());
+                upload(h->d_cT[q], fq);
+            }
+        }
+
+        // per-cell arrays
+        const size_t tsz = h->tsz();
+        h->d_ap64.alloc(h->ncell * 8);
+        for (int q = 0; q < 7; ++q) h->d_f[q].alloc(h->ncell * tsz);
+        h->d_hilf.alloc(h->ncell * tsz);
+        h->d_res.alloc(h->ncell * tsz);
+        h->d_rhs64.alloc(h->ncell * 8);
+        h->d_dp64.alloc(h->ncell * 8);
+        if (h->fp32()) {
+            h->d_apT.alloc(h->ncell * 4);
+            h->d_rhsT.alloc(h->ncell * 4);
+        }
+        h->d_hface.alloc(h->nface * tsz);
+        h->d_rhi[0].alloc(h->nface * tsz);
+        h->d_rhi[1].alloc(h->nface * tsz);
+        h->d_rlo.alloc(h->nface * tsz);
+        h->d_scal.alloc(64);
+        h->d_never.alloc(64);
+        HC(hipMemsetAsync(h->d_never.p, 0, 64, h->stream));
+        h->d_hist.alloc(std::max(1, h->cfg.nouter) * 8);
+
+        // GSAP, then the SIP factors
+        Topo t = h->topo(-1);
+        launch_ap(t, h->d_dx.as<double>(), h->d_dy.as<double>(),
+                  h->d_dz.as<double>(), h->d_raw[0].as<double>(),
+                  h->d_raw[1].as<double>(), h->d_raw[2].as<double>(),
+                  h->d_raw[3].as<double>(), h->d_raw[4].as<double>(),
+                  h->d_raw[5].as<double>(), h->mask(), h->d_ap64.as<double>(),
+                  h->stream);
+        HC(hipGetLastError());
+        if (h->fp32()) {
+            setup_factors<float>(*h);
+            launch_convert<float>(h->d_ap64.as<double>(), h->d_apT.as<float>(), h->ncell, h->stream);
+        } else {
+            setup_factors<double>(*h);
+        }
+        HC(hipStreamSynchronize(h->stream));
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    *out = h.release();
+    g_err.clear();
+    return MGP_OK;
+}
+
+template <typename T>
+int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
+          double dt, mgp_stats *st) {
+    const mgp_config &c = h.cfg;
+    Outer o = h.outer();
+    const double prefak = c.rho / dt;
+    const hipStream_t s = h.stream;
+    Topo tall = h.topo(-1);
+    T *rhsT = h.fp32() ? h.d_rhsT.as<T>() : nullptr;
+    // pool fields start at zero (src/core/fieldpool_mod.F90:113-122)
+    launch_fill<double>(h.d_dp64.as<double>(), h.ncell, 0.0, s);
+    launch_fill<T>(h.d_hilf.as<T>(), h.ncell, T(0), s);
+    launch_fill<T>(h.d_hface.as<T>(), h.nface, T(0), s);
+    launch_fill<T>(h.d_rhi[0].as<T>(), h.nface, T(0), s);
+    launch_fill<T>(h.d_rhi[1].as<T>(), h.nface, T(0), s);
+    launch_fill<T>(h.d_rlo.as<T>(), h.nface, T(0), s);
+    launch_divcal<T>(tall, u, v, w, h.d_rddx.as<double>(), h.d_rddy.as<double>(),
+                     h.d_rddz.as<double>(), c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr,
+                     prefak, h.d_rhs64.as<double>(), rhsT, s);
+    launch_maxabs(tall, h.d_rhs64.as<double>(), o, s);
+    HC(hipGetLastError());
+
+    hipEvent_t e0, e1;
+    HC(hipEventCreate(&e0));
+    HC(hipEventCreate(&e1));
+    HC(hipEventRecord(e0, s));
+    if (c.nouter > 0) {
+        // chunk of outer iterations that returns the RES buffer parity
+        int chunk = std::max(1, c.check_every);
+        if (((c.ninner - 1) & 1) && (chunk & 1)) ++chunk;
+        int launched = 0;
+        const bool classic = c.ityp == 2;
+        auto enqueue_chunk = [&]() {
+            int rd = 0;
+            Fields<T> F = h.fields<T>();
+            Mask m = h.mask();
+            for (int it = 0; it < chunk; ++it) {
+                for (auto &kv : h.level_blocks) {
+                    Topo t = h.topo(kv.first);
+                    for (int il = 1; il <= c.ninner; ++il) {
+                        const int last = il == c.ninner;
+                        launch_forward<T>(t, F, m, rd, !last, il == 1, classic, o.done, s);
+                        launch_backward<T>(t, F, il == 1, classic, o.done, s);
+                        if (!last) rd ^= 1;
+                    }
+                }
+                launch_outer<T>(tall, F, m, o, rhsT, s);
+                launch_check(o, c.nouter, c.nouter_min, c.epcorr, c.eps_rel,
+                             prefak, h.d_hist.as<double>(), c.nouter, s);
+            }
+        };
+        // capture one chunk into a graph; streams that cannot be captured
+        // (the legacy null stream) get the same launches directly
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (s != nullptr && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            enqueue_chunk();
+            HC(hipStreamEndCapture(s, &graph));
+            HC(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        } else {
+            (void)hipGetLastError();
+        }
+        int hdone = 0;
+        while (launched < c.nouter) {
+            if (exec) HC(hipGraphLaunch(exec, s));
+            else enqueue_chunk();
+            launched += chunk;
+            if (launched >= c.nouter) break;
+            HC(hipMemcpyAsync(&hdone, o.done, sizeof(int), hipMemcpyDeviceToHost, s));
+            HC(hipStreamSynchronize(s));
+            if (hdone) break;
+        }
+        if (exec) HC(hipGraphExecDestroy(exec));
+        if (graph) HC(hipGraphDestroy(graph));
+    }
+    HC(hipEventRecord(e1, s));
+
+    // DP -> reference layout with the face ghosts of the exchange
+    // (src/flow/pressuresolver_mod.F90:278 accumulates HILF including the
+    // ghosts that conn(...) copied from the neighbours), then mgpcorr and
+    // the post-projection exchange (:333-348).
+    if (h.d_dpsp.n < (size_t)h.n3d * 8) h.d_dpsp.alloc((size_t)h.n3d * 8);
+    launch_fill<double>(h.d_dpsp.as<double>(), h.n3d, 0.0, s);
+    launch_wo_to_sp<double>(tall, h.d_dp64.as<double>(), h.d_dpsp.as<double>(), s);
+    double *fd[1] = {h.d_dpsp.as<double>()};
+    exchange(h, -1, 1, 0, 0, fd, 1);
+    launch_mgpcorr(tall, u, v, w, p, h.d_dpsp.as<double>(), h.d_rdx.as<double>(),
+                   h.d_rdy.as<double>(), h.d_rdz.as<double>(), dt / c.rho,
+                   c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr, s);
+    double *fs[4] = {u, v, w, p};
+    for (int l = h.minlevel; l <= h.maxlevel; ++l)
+        if (h.level_blocks.count(l)) exchange(h, l, 2, 1, 0, fs, 4);
+    HC(hipGetLastError());
+
+    double sc[3];
+    int iv[4];
+    HC(hipMemcpyAsync(sc, h.d_scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HC(hipMemcpyAsync(iv, (char *)h.d_scal.p + 24, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HC(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HC(hipEventElapsedTime(&ms, e0, e1));
+    HC(hipEventDestroy(e0));
+    HC(hipEventDestroy(e1));
+    h.have_solution = true;
+    if (st) {
+        st->nouter_done = iv[0];
+        st->ipc = iv[0] * c.ninner;
+        st->converged = iv[2];
+        st->maxrhs0 = sc[1];
+        st->maxrhs = sc[2];
+        st->solve_ms = ms;
+    }
+    if (iv[3]) return fail(MGP_ERR_NONFINITE, "residual became NaN or Inf");
+    return MGP_OK;
+}
+
+int solve_any(mgp_handle *h, double *u, double *v, double *w, double *p,
+              double dt, mgp_stats *st) {
+    if (!(dt > 0.0) && !(dt < 0.0)) return fail(MGP_ERR_ARG, "dt must be non-zero");
+    try {
+        int rc = h->fp32() ? solve<float>(*h, u, v, w, p, dt, st)
+                           : solve<double>(*h, u, v, w, p, dt, st);
+        if (rc == MGP_OK) g_err.clear();
+        return rc;
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+}
+
+}  // namespace
+
+// =====================================================================
+extern "C" {
+
+int mgp_init(const mgp_grids *grids, const mgp_config *cfg,
+             const mgletreal *dx, const mgletreal *dy, const mgletreal *dz,
+             const mgletreal *ddx, const mgletreal *ddy, const mgletreal *ddz,
+             const mgletreal *bp, mgp_handle **out) {
+    return init_common(grids, cfg, dx, dy, dz, ddx, ddy, ddz, bp, nullptr, out);
+}
+
+int mgp_init_mask(const mgp_grids *grids, const mgp_config *cfg,
+                  const mgletreal *dx, const mgletreal *dy,
+                  const mgletreal *dz, const mgletreal *ddx,
+                  const mgletreal *ddy, const mgletreal *ddz,
+                  const int8_t *blocked, mgp_handle **out) {
+    return init_common(grids, cfg, dx, dy, dz, ddx, ddy, ddz, nullptr,
+                       blocked, out);
+}
+
+int mgp_mgpoisl_device(mgp_handle *h, mgletreal *u, mgletreal *v,
+                       mgletreal *w, mgletreal *p, mgletreal dt,
+                       mgletint ittot, mgletint irk, mgp_stats *stats) {
+    (void)ittot;
+    (void)irk;
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (!u || !v || !w || !p) return fail(MGP_ERR_ARG, "null field");
+    return solve_any(h, u, v, w, p, dt, stats);
+}
+
+int mgp_mgpoisl(mgp_handle *h, mgletreal *u, mgletreal *v, mgletreal *w,
+                mgletreal *p, mgletreal dt, mgletint ittot, mgletint irk,
+                mgp_stats *stats) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (!u || !v || !w || !p) return fail(MGP_ERR_ARG, "null field");
+    try {
+        const size_t bytes = (size_t)h->n3d * 8;
+        DBuf *d[4] = {&h->d_u, &h->d_v, &h->d_w, &h->d_p};
+        double *hp[4] = {u, v, w, p};
+        for (int q = 0; q < 4; ++q) {
+            if (d[q]->n < bytes) d[q]->alloc(bytes);
+            HC(hipMemcpyAsync(d[q]->p, hp[q], bytes, hipMemcpyHostToDevice, h->stream));
+        }
+        int rc = solve_any(h, h->d_u.as<double>(), h->d_v.as<double>(),
+                           h->d_w.as<double>(), h->d_p.as<double>(), dt, stats);
+        for (int q = 0; q < 4; ++q)
+            HC(hipMemcpyAsync(hp[q], d[q]->p, bytes, hipMemcpyDeviceToHost, h->stream));
+        HC(hipStreamSynchronize(h->stream));
+        return rc;
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+}
+
+int mgp_finish(mgp_handle *h) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    (void)hipStreamSynchronize(h->stream);
+    delete h;
+    g_err.clear();
+    return MGP_OK;
+}
+
+const char *mgp_last_error(void) { return g_err.c_str(); }
+
+int mgp_set_iterations(mgp_handle *h, mgletint ninner, mgletint nouter,
+                       mgletint nouter_min, mgletreal epcorr,
+                       mgletreal eps_rel, mgletint check_every) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    mgp_config c = h->cfg;
+    c.ninner = ninner;
+    c.nouter = nouter;
+    c.nouter_min = nouter_min;
+    c.epcorr = epcorr;
+    c.eps_rel = eps_rel;
+    c.check_every = check_every < 1 ? 1 : check_every;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    h->cfg = c;
+    try {
+        if (h->d_hist.n < (size_t)std::max(1, nouter) * 8)
+            h->d_hist.alloc((size_t)std::max(1, nouter) * 8);
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    return MGP_OK;
+}
+
+int mgp_set_stream(mgp_handle *h, void *stream) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    h->stream = stream ? (hipStream_t)stream : h->own_stream;
+    return MGP_OK;
+}
+
+int mgp_get_field(mgp_handle *h, const char *name, mgletreal *out) {
+    if (!h || !name || !out) return fail(MGP_ERR_ARG, "null argument");
+    const std::string nm(name);
+    try {
+        const hipStream_t s = h->stream;
+        const char *c1[6] = {"GSAW", "GSAE", "GSAS", "GSAN", "GSAB", "GSAT"};
+        for (int q = 0; q < 6; ++q)
+            if (nm == c1[q]) {
+                HC(hipMemcpyAsync(out, h->d_c64[q].p, h->d_c64[q].n, hipMemcpyDeviceToHost, s));
+                HC(hipStreamSynchronize(s));
+                return MGP_OK;
+            }
+        DBuf tmp;
+        tmp.alloc((size_t)h->n3d * 8);
+        launch_fill<double>(tmp.as<double>(), h->n3d, 0.0, s);
+        Topo t = h->topo(-1);
+        const char *fn[7] = {"SIPLW", "SIPLS", "SIPLB", "SIPLPR", "SIPUE", "SIPUN", "SIPUT"};
+        bool found = false;
+        for (int q = 0; q < 7; ++q)
+            if (nm == fn[q]) {
+                if (h->fp32())
+                    launch_to_ref_factors<float>(t, h->d_f[q].as<float>(), tmp.as<double>(), h->cfg.ityp == 2, s);
+                else
+                    launch_to_ref_factors<double>(t, h->d_f[q].as<double>(), tmp.as<double>(), h->cfg.ityp == 2, s);
+                found = true;
+            }
+        if (nm == "GSAP") {
+            launch_wo_to_sp<double>(t, h->d_ap64.as<double>(), tmp.as<double>(), s);
+            found = true;
+        } else if (nm == "DP" || nm == "RHS") {
+            if (!h->have_solution) return fail(MGP_ERR_STATE, "no mgpoisl call yet");
+            launch_wo_to_sp<double>(t, nm == "DP" ? h->d_dp64.as<double>() : h->d_rhs64.as<double>(), tmp.as<double>(), s);
+            found = true;
+        } else if (nm == "BP") {
+            if (h->masked) {
+                launch_fill<double>(tmp.as<double>(), h->n3d, 1.0, s);
+                launch_wo_to_sp<uint8_t>(t, h->d_bpw.as<uint8_t>(), tmp.as<double>(), s);
+            } else {
+                launch_fill<double>(tmp.as<double>(), h->n3d, 1.0, s);
+            }
+            found = true;
+        }
+        if (!found) return fail(MGP_ERR_ARG, "unknown field %s", name);
+        HC(hipMemcpyAsync(out, tmp.p, (size_t)h->n3d * 8, hipMemcpyDeviceToHost, s));
+        HC(hipStreamSynchronize(s));
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    return MGP_OK;
+}
+
+int mgp_connect_device(mgp_handle *h, mgletint level, mgletint layers,
+                       mgletint corners, mgletint forward,
+                       mgletreal **fields, mgletint nfield) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (layers < 1 || layers > 2) return fail(MGP_ERR_ARG, "layers must be 1 or 2");
+    if (forward != 0 && corners) return fail(MGP_ERR_ARG, "forward with corners");
+    if (forward < -1 || forward > 1) return fail(MGP_ERR_ARG, "forward must be -1, 0 or 1");
+    if (!fields || nfield < 1) return fail(MGP_ERR_ARG, "no fields");
+    try {
+        exchange(*h, level, layers, corners, forward, fields, nfield);
+        HC(hipGetLastError());
+        HC(hipStreamSynchronize(h->stream));
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    return MGP_OK;
+}
+
+int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
+                      const mgletreal *v, const mgletreal *w, mgletreal fak) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (!div || !u || !v || !w) return fail(MGP_ERR_ARG, "null field");
+    try {
+        DBuf tmp;
+        tmp.alloc((size_t)h->ncell * 8);
+        Topo t = h->topo(-1);
+        launch_divcal<double>(t, u, v, w, h->d_rddx.as<double>(), h->d_rddy.as<double>(),
+                              h->d_rddz.as<double>(), h->cfg.ibtype == 1 ? h->d_bpsp.as<double>() : nullptr,
+                              fak, tmp.as<double>(), nullptr, h->stream);
+        launch_wo_to_sp<double>(t, tmp.as<double>(), div, h->stream);
+        HC(hipGetLastError());
+        HC(hipStreamSynchronize(h->stream));
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    return MGP_OK;
+}
+
+int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms) {
+    if (!h || !ms || niter < 1) return fail(MGP_ERR_ARG, "bad argument");
+    try {
+        const hipStream_t s = h->stream;
+        hipEvent_t ev[4];
+        for (auto &e : ev) HC(hipEventCreate(&e));
+        double acc[3] = {0, 0, 0};
+        Outer o = h->outer();
+        Outer onever = o;
+        onever.done = (int *)h->d_never.p;
+        const int *never = (const int *)h->d_never.p;
+        const bool classic = h->cfg.ityp == 2;
+        Mask m = h->mask();
+        Topo tall = h->topo(-1);
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            Fields<T> F = h->fields<T>();
+            int rd = 0;
+            for (int it = 0; it < niter; ++it) {
+                float t0 = 0, t1 = 0;
+                HC(hipEventRecord(ev[0], s));
+                launch_forward<T>(tall, F, m, rd, 1, 0, classic, never, s);
+                HC(hipEventRecord(ev[1], s));
+                launch_backward<T>(tall, F, 0, classic, never, s);
+                HC(hipEventRecord(ev[2], s));
+                HC(hipEventSynchronize(ev[2]));
+                HC(hipEventElapsedTime(&t0, ev[0], ev[1]));
+                HC(hipEventElapsedTime(&t1, ev[1], ev[2]));
+                acc[0] += t0;
+                acc[1] += t1;
+                rd ^= 1;
+            }
+            float t2 = 0;
+            HC(hipEventRecord(ev[0], s));
+            launch_outer<T>(tall, F, m, onever, h->fp32() ? h->d_rhsT.as<T>() : nullptr, s);
+            HC(hipEventRecord(ev[3], s));
+            HC(hipEventSynchronize(ev[3]));
+            HC(hipEventElapsedTime(&t2, ev[0], ev[3]));
+            acc[2] = t2;
+        };
+        if (h->fp32()) run(float{});
+        else run(double{});
+        HC(hipGetLastError());
+        ms[0] = acc[0] / niter;
+        ms[1] = acc[1] / niter;
+        ms[2] = acc[2];
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    return MGP_OK;
+}
+
+int mgp_info(mgp_handle *h, int64_t *ncells, int64_t *device_bytes) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (ncells) *ncells = h->ncell;
+    if (device_bytes) {
+        size_t tot = 0;
+        const DBuf *bs[] = {&h->d_blocks, &h->d_shapes, &h->d_cells, &h->d_pstart, &h->d_slot,
+                            &h->d_ap64, &h->d_apT, &h->d_hilf, &h->d_rhs64, &h->d_rhsT, &h->d_res,
+                            &h->d_dp64, &h->d_bpw, &h->d_bpf, &h->d_bpsp, &h->d_hface, &h->d_rhi[0],
+                            &h->d_rhi[1], &h->d_rlo, &h->d_u, &h->d_v, &h->d_w, &h->d_p, &h->d_dpsp};
+        for (auto *b : bs) tot += b->n;
+        for (auto &b : h->d_f) tot += b.n;
+        *device_bytes = (int64_t)tot;
+    }
+    return MGP_OK;
+}
+
+}  // extern "C"

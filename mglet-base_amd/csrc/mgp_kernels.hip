@@ -1,0 +1,1045 @@
+// mgp_kernels.hip -- gfx950 kernels of the pressure projection.
+//
+// Design (per block, one workgroup of 256 lanes = four wave64s):
+//   * SIP sweeps walk the i+j+k hyperplanes of the block (the reference's
+//     dependency order, src/flow/sip_hyperplane_mod.F90:450-569).  Every
+//     per-cell array is stored hyperplane-major (WO layout, see
+//     mgp_internal.h), so each step reads one contiguous run per array.
+//   * The recurrence state lives in LDS as a 2-D "pencil" array R[j][i]
+//     holding the latest value along each k-pencil: cell (k, j, i) of plane
+//     p reads (k-1, j, i), (k, j-1, i), (k, j, i-1) from R of plane p-1.
+//     R is double-buffered, so one barrier per plane suffices.
+//   * The Laplacian fused into the forward sweep reads a 4-slot ring of
+//     hyperplanes of HILF, also stored per pencil ([j][i]) in LDS.
+//   * Ghost values come from face buffers written by the neighbouring
+//     blocks' previous kernels; the reference's halo copies of RES and DP
+//     (src/flow/pressuresolver_mod.F90:414, 453-457) become reads of those
+//     buffers, with the same lagging.
+// All kernels are compiled with -ffp-contract=off: each expression is
+// evaluated in the reference's association order without FMA, which makes
+// the FP64 path bitwise equal to the CPU oracle.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "mgp_launch.h"
+
+namespace mgp {
+
+#define ALFA ((double)0.92f)
+#define TINY ((double)1.0e-20f)
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ void unpack(uint32_t c, int &i, int &j, int &k) {
+    i = c & 1023;
+    j = (c >> 10) & 1023;
+    k = c >> 20;
+}
+
+template <typename T>
+__device__ __forceinline__ const T *nbr_face(const T *base,
+                                             const Block *blocks, int nb,
+                                             int f, int ni, int nj, int nk) {
+    return nb >= 0 ? base + blocks[nb].fc + face_base(f, ni, nj, nk)
+                   : nullptr;
+}
+
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(T *p, int64_t n, T v) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        p[e] = v;
+}
+
+template <typename T>
+void launch_fill(T *p, int64_t n, T v, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    k_fill<T><<<(unsigned)nb, 256, 0, s>>>(p, n, v);
+}
+void launch_fill_u8(uint8_t *p, int64_t n, uint8_t v, hipStream_t s) {
+    launch_fill<uint8_t>(p, n, v, s);
+}
+template void launch_fill<double>(double *, int64_t, double, hipStream_t);
+template void launch_fill<float>(float *, int64_t, float, hipStream_t);
+template void launch_fill<int>(int *, int64_t, int, hipStream_t);
+
+template <typename T>
+__global__ void k_convert(const double *src, T *dst, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        dst[e] = (T)src[e];
+}
+template <typename T>
+void launch_convert(const double *src, T *dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    k_convert<T><<<(unsigned)nb, 256, 0, s>>>(src, dst, n);
+}
+template void launch_convert<float>(const double *, float *, int64_t, hipStream_t);
+template void launch_convert<double>(const double *, double *, int64_t, hipStream_t);
+
+// BP of cell (i, j, k) of a block (0-based interior indices, one of them
+// possibly -1 or N for the ghost layer).
+__device__ __forceinline__ double bp_at(const Block &B, const int *slot_of,
+                                        Mask m, int i, int j, int k) {
+    if (B.flags & 1) return 1.0;
+    const int ni = B.ni, nj = B.nj, nk = B.nk;
+    const uint8_t *f = m.bpf + B.fc;
+    if (i < 0) return f[face_base(0, ni, nj, nk) + k + nk * j];
+    if (i >= ni) return f[face_base(1, ni, nj, nk) + k + nk * j];
+    if (j < 0) return f[face_base(2, ni, nj, nk) + k + nk * i];
+    if (j >= nj) return f[face_base(3, ni, nj, nk) + k + nk * i];
+    if (k < 0) return f[face_base(4, ni, nj, nk) + j + nj * i];
+    if (k >= nk) return f[face_base(5, ni, nj, nk) + j + nj * i];
+    return m.bp[B.wo + slot_of[k + nk * (j + nj * i)]];
+}
+
+// ---------------------------------------------------------------------
+// GSAP: src/ib/noib_mod.F90:154-244.  One thread per WO cell.
+__global__ void k_ap(Topo t, const double *dx_, const double *dy_,
+                     const double *dz_, const double *aw_, const double *ae_,
+                     const double *as_, const double *an_, const double *ab_,
+                     const double *at_, Mask m, double *ap) {
+    const int b = t.blist[blockIdx.y];
+    const Block B = t.blocks[b];
+    const Shape S = t.shapes[B.shape];
+    const int *slot_of = t.slot_of + S.slot_off;
+    const uint32_t *cells = t.cells + S.cell_off;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S.ncell;
+         s += gridDim.x * blockDim.x) {
+        int i, j, k;
+        unpack(cells[s], i, j, k);
+        // 1-based reference index r = i + 3; dx(r) is dx_[o1x + r - 1]
+        const double *dx = dx_ + B.o1x + 2, *dy = dy_ + B.o1y + 2,
+                     *dz = dz_ + B.o1z + 2;
+        const double *aw = aw_ + B.o1x + 2, *ae = ae_ + B.o1x + 2;
+        const double *as = as_ + B.o1y + 2, *an = an_ + B.o1y + 2;
+        const double *ab = ab_ + B.o1z + 2, *at = at_ + B.o1z + 2;
+        double a = -(2.0 / (dx[i - 1] * dx[i])) - 2.0 / (dy[j - 1] * dy[j]) -
+                   2.0 / (dz[k - 1] * dz[k]);
+        const double c = bp_at(B, slot_of, m, i, j, k);
+        const double bw = bp_at(B, slot_of, m, i - 1, j, k);
+        const double be = bp_at(B, slot_of, m, i + 1, j, k);
+        const double bs = bp_at(B, slot_of, m, i, j - 1, k);
+        const double bn = bp_at(B, slot_of, m, i, j + 1, k);
+        const double bb = bp_at(B, slot_of, m, i, j, k - 1);
+        const double bt = bp_at(B, slot_of, m, i, j, k + 1);
+        a = a + aw[i] * (1.0 - bw * c);
+        a = a + ae[i] * (1.0 - c * be);
+        a = a + as[j] * (1.0 - bs * c);
+        a = a + an[j] * (1.0 - c * bn);
+        a = a + ab[k] * (1.0 - bb * c);
+        a = a + at[k] * (1.0 - c * bt);
+        const int *bc = B.bc;
+        if (i == 0 && (bc[0] == 2 || bc[0] == 5 || bc[0] == 6 || bc[0] == 19))
+            a = a + aw[0] * (bw * c);
+        if (i == B.ni - 1 && (bc[1] == 2 || bc[1] == 5 || bc[1] == 6))
+            a = a + ae[i] * (c * be);
+        if (j == 0 && (bc[2] == 2 || bc[2] == 5 || bc[2] == 6 || bc[2] == 19))
+            a = a + as[0] * (bs * c);
+        if (j == B.nj - 1 && (bc[3] == 2 || bc[3] == 5 || bc[3] == 6))
+            a = a + an[j] * (c * bn);
+        if (k == 0 && (bc[4] == 2 || bc[4] == 5 || bc[4] == 6 || bc[4] == 19))
+            a = a + ab[0] * (bb * c);
+        if (k == B.nk - 1 && (bc[5] == 2 || bc[5] == 5 || bc[5] == 6))
+            a = a + at[k] * (c * bt);
+        ap[B.wo + s] = a;
+    }
+}
+
+void launch_ap(const Topo &t, const double *dx, const double *dy,
+               const double *dz, const double *aw, const double *ae,
+               const double *as, const double *an, const double *ab,
+               const double *at, Mask m, double *ap, hipStream_t s) {
+    const int maxc = t.max_ni * t.max_nj * t.max_nk;
+    dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
+    k_ap<<<grid, kThreads, 0, s>>>(t, dx, dy, dz, aw, ae, as, an, ab, at, m,
+                                   ap);
+}
+
+// ---------------------------------------------------------------------
+// SIP incomplete LU in hyperplane order: src/flow/sip_hyperplane_mod.F90:
+// 106-257 (identical values to src/flow/sip_classic_mod.F90:17-127).
+// The upper factors of plane p-1 are kept per pencil in LDS.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_lu(
+    Topo t, const double *ap, const double *aw_, const double *ae_,
+    const double *as_, const double *an_, const double *ab_,
+    const double *at_, Mask m, T *lw_o, T *ls_o, T *lb_o, T *lpr_o, T *ue_o,
+    T *un_o, T *ut_o) {
+    extern __shared__ double lds_lu[];
+    const int b = t.blist[blockIdx.x];
+    const Block B = t.blocks[b];
+    const Shape S = t.shapes[B.shape];
+    const int NI = B.ni, NJ = B.nj;
+    const int RW = NJ + 1, RS = (NJ + 1) * (NI + 1);
+    double *U = lds_lu;  // [2][3][RS]
+    const uint32_t *cells = t.cells + S.cell_off;
+    const int *ps = t.pstart + S.pstart_off;
+    const int *slot_of = t.slot_of + S.slot_off;
+    const double *aw = aw_ + B.o1x + 2, *ae = ae_ + B.o1x + 2;
+    const double *as = as_ + B.o1y + 2, *an = an_ + B.o1y + 2;
+    const double *ab = ab_ + B.o1z + 2, *at = at_ + B.o1z + 2;
+    for (int e = threadIdx.x; e < 6 * RS; e += blockDim.x) U[e] = 0.0;
+    __syncthreads();
+    for (int p = 0; p < S.np; ++p) {
+        const double *pe = U + ((p - 1) & 1) * 3 * RS;
+        const double *pn = pe + RS, *pt = pn + RS;
+        double *ce = U + (p & 1) * 3 * RS;
+        double *cn = ce + RS, *ct = cn + RS;
+        const int cnt = ps[p + 1] - ps[p];
+        for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+            const int s = ps[p] + q;
+            int i, j, k;
+            unpack(cells[s], i, j, k);
+            const int c0 = (j + 1) + RW * (i + 1);
+            const int cim = c0 - RW, cjm = c0 - 1;
+            const double e_im = pe[cim], n_im = pn[cim], t_im = pt[cim];
+            const double e_jm = pe[cjm], n_jm = pn[cjm], t_jm = pt[cjm];
+            const double e_km = pe[c0], n_km = pn[c0], t_km = pt[c0];
+            double bu_m = 1.0, bv_m = 1.0, bw_m = 1.0, bu_c = 1.0, bv_c = 1.0,
+                   bw_c = 1.0;
+            if (!(B.flags & 1)) {
+                const double c = bp_at(B, slot_of, m, i, j, k);
+                bu_m = bp_at(B, slot_of, m, i - 1, j, k) * c;
+                bv_m = bp_at(B, slot_of, m, i, j - 1, k) * c;
+                bw_m = bp_at(B, slot_of, m, i, j, k - 1) * c;
+                bu_c = c * bp_at(B, slot_of, m, i + 1, j, k);
+                bv_c = c * bp_at(B, slot_of, m, i, j + 1, k);
+                bw_c = c * bp_at(B, slot_of, m, i, j, k + 1);
+            }
+            double lw = aw[i] * bu_m / (1.0 + ALFA * (n_im + t_im));
+            double ls = as[j] * bv_m / (1.0 + ALFA * (e_jm + t_jm));
+            double lb = ab[k] * bw_m / (1.0 + ALFA * (n_km + e_km));
+            const double p1 = ALFA * (lb * e_km + ls * e_jm);
+            const double p2 = ALFA * (lb * n_km + lw * n_im);
+            const double p3 = ALFA * (lw * t_im + ls * t_jm);
+            const double lpr = 1.0 / (ap[B.wo + s] + p1 + p2 + p3 -
+                                      lb * t_km - lw * e_im - ls * n_jm +
+                                      TINY);
+            const double ue = (ae[i] * bu_c - p1) * lpr;
+            const double un = (an[j] * bv_c - p2) * lpr;
+            const double ut = (at[k] * bw_c - p3) * lpr;
+            lw = lw * lpr;
+            ls = ls * lpr;
+            lb = lb * lpr;
+            ce[c0] = ue;
+            cn[c0] = un;
+            ct[c0] = ut;
+            const int64_t o = B.wo + s;
+            lw_o[o] = (T)lw;
+            ls_o[o] = (T)ls;
+            lb_o[o] = (T)lb;
+            lpr_o[o] = (T)lpr;
+            ue_o[o] = (T)ue;
+            un_o[o] = (T)un;
+            ut_o[o] = (T)ut;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+void launch_lu(const Topo &t, const double *ap, const double *aw,
+               const double *ae, const double *as, const double *an,
+               const double *ab, const double *at, Mask m, T *const f[7],
+               hipStream_t s) {
+    const size_t lds = 6 * (size_t)(t.max_nj + 1) * (t.max_ni + 1) *
+                       sizeof(double);
+    k_lu<T><<<t.nlist, kThreads, lds, s>>>(t, ap, aw, ae, as, an, ab, at, m,
+                                           f[0], f[1], f[2], f[3], f[4], f[5],
+                                           f[6]);
+}
+template void launch_lu<double>(const Topo &, const double *, const double *,
+                                const double *, const double *, const double *,
+                                const double *, const double *, Mask,
+                                double *const[7], hipStream_t);
+template void launch_lu<float>(const Topo &, const double *, const double *,
+                               const double *, const double *, const double *,
+                               const double *, const double *, Mask,
+                               float *const[7], hipStream_t);
+
+// ---------------------------------------------------------------------
+// RHS = fak * div(u, v, w): src/ib/noib_mod.F90:304-354 (noib) and the
+// ghostcell variant src/ib/gc_mod.F90:278-347 (+ fak*SDIV with SDIV = 0,
+// then x BP).  One thread per WO cell; u, v, w in the reference layout.
+template <typename T>
+__global__ void k_divcal(Topo t, const double *u, const double *v,
+                         const double *w, const double *rddx_,
+                         const double *rddy_, const double *rddz_,
+                         const double *bp_sp, double fak, double *rhs64,
+                         T *rhs_t) {
+    const int b = t.blist[blockIdx.y];
+    const Block B = t.blocks[b];
+    const Shape S = t.shapes[B.shape];
+    const uint32_t *cells = t.cells + S.cell_off;
+    const int64_t kk = B.nk + 4, jj = B.nj + 4;
+    const double *rddx = rddx_ + B.o1x + 2, *rddy = rddy_ + B.o1y + 2,
+                 *rddz = rddz_ + B.o1z + 2;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S.ncell;
+         s += gridDim.x * blockDim.x) {
+        int i, j, k;
+        unpack(cells[s], i, j, k);
+        const int64_t c = B.sp + (k + 2) + (j + 2) * kk + (i + 2) * kk * jj;
+        double d = fak * ((u[c] - u[c - kk * jj]) * rddx[i] +
+                          (v[c] - v[c - kk]) * rddy[j] +
+                          (w[c] - w[c - 1]) * rddz[k]);
+        if (bp_sp) {
+            d = d + fak * 0.0;
+            d = bp_sp[c] * d;
+        }
+        rhs64[B.wo + s] = d;
+        if (rhs_t) rhs_t[B.wo + s] = (T)d;
+    }
+}
+
+template <typename T>
+void launch_divcal(const Topo &t, const double *u, const double *v,
+                   const double *w, const double *rddx, const double *rddy,
+                   const double *rddz, const double *bp_sp, double fak,
+                   double *rhs64, T *rhs_t, hipStream_t s) {
+    const int maxc = t.max_ni * t.max_nj * t.max_nk;
+    dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
+    k_divcal<T><<<grid, kThreads, 0, s>>>(t, u, v, w, rddx, rddy, rddz,
+                                          bp_sp, fak, rhs64, rhs_t);
+}
+template void launch_divcal<double>(const Topo &, const double *,
+                                    const double *, const double *,
+                                    const double *, const double *,
+                                    const double *, const double *, double,
+                                    double *, double *, hipStream_t);
+template void launch_divcal<float>(const Topo &, const double *,
+                                   const double *, const double *,
+                                   const double *, const double *,
+                                   const double *, const double *, double,
+                                   double *, float *, hipStream_t);
+
+// ---------------------------------------------------------------------
+// Block-wide max of non-negative doubles folded into a global maximum.
+// Non-negative IEEE doubles order like their bit patterns.
+__device__ void block_max_to_global(double m, unsigned long long *g) {
+    __shared__ double red[kThreads / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_down(m, off, 64);
+        m = (o > m || m != m) ? o : m;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
+            r = (red[q] > r || r != r) ? red[q] : r;
+        atomicMax(g, (unsigned long long)__double_as_longlong(r));
+    }
+}
+
+// max |RHS| over interior cells with the level weight 2^(maxlevel-level):
+// src/flow/pressuresolver_mod.F90:623-689.
+__global__ void k_maxabs(Topo t, const double *rhs64, Outer o) {
+    const int b = t.blist[blockIdx.y];
+    const Block B = t.blocks[b];
+    const Shape S = t.shapes[B.shape];
+    double m = 0.0;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S.ncell;
+         s += gridDim.x * blockDim.x) {
+        const double a = fabs(rhs64[B.wo + s]);
+        m = (a > m || a != a) ? a : m;
+    }
+    m = m * ldexp(1.0, t.maxlevel - B.level);
+    block_max_to_global(m, o.gmax);
+}
+
+__global__ void k_maxabs_store(Outer o) {
+    const double m = __longlong_as_double((long long)*o.gmax);
+    *o.maxrhs0 = m;
+    *o.maxrhs = m;
+    *o.gmax = 0ull;
+    *o.ctr = 0;
+    *o.converged = 0;
+    *o.nonfinite = (m != m || m == INFINITY) ? 1 : 0;
+    *o.done = *o.nonfinite;
+}
+
+void launch_maxabs(const Topo &t, const double *rhs64, Outer o,
+                   hipStream_t s) {
+    const int maxc = t.max_ni * t.max_nj * t.max_nk;
+    dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
+    launch_fill<unsigned long long>(o.gmax, 1, 0ull, s);
+    k_maxabs<<<grid, kThreads, 0, s>>>(t, rhs64, o);
+    k_maxabs_store<<<1, 1, 0, s>>>(o);
+}
+template void launch_fill<unsigned long long>(unsigned long long *, int64_t,
+                                              unsigned long long, hipStream_t);
+
+// ---------------------------------------------------------------------
+// Plane-window helpers.  H holds 4 hyperplanes of a field, each stored per
+// pencil [j][i] with a one-cell border: index (j+1) + (NJ+2)(i+1).  Plane q
+// (q = -1 .. NP) goes to slot (q+1)&3.  Ghost entries (one index out of
+// range by one) come from face buffers.
+template <typename T>
+struct Window {
+    const T *zlo, *zhi, *ylo, *yhi, *xlo, *xhi;  // neighbour faces or null
+};
+
+template <typename T>
+__device__ void fill_plane(T *Hq, int q, const Block &B, const Shape &S,
+                           const uint32_t *cells, const int *ps,
+                           const T *field, const Window<T> &w) {
+    const int NI = B.ni, NJ = B.nj, NK = B.nk, HW = NJ + 2;
+    if (q >= 0 && q < S.np) {
+        const int cnt = ps[q + 1] - ps[q];
+        for (int x = threadIdx.x; x < cnt; x += blockDim.x) {
+            const int s = ps[q] + x;
+            int i, j, k;
+            unpack(cells[s], i, j, k);
+            Hq[(j + 1) + HW * (i + 1)] = field[B.wo + s];
+        }
+    }
+    const int ng = 4 * NI + 2 * NJ;
+    for (int e = threadIdx.x; e < ng; e += blockDim.x) {
+        int i, j, k, idx = -1;
+        T v = T(0);
+        if (e < NI) {                      // k = -1
+            i = e; j = q + 1 - i;
+            if (j >= 0 && j < NJ) { idx = (j + 1) + HW * (i + 1); if (w.zlo) v = w.zlo[j + NJ * i]; }
+        } else if (e < 2 * NI) {           // k = NK
+            i = e - NI; j = q - NK - i;
+            if (j >= 0 && j < NJ) { idx = (j + 1) + HW * (i + 1); if (w.zhi) v = w.zhi[j + NJ * i]; }
+        } else if (e < 3 * NI) {           // j = -1
+            i = e - 2 * NI; k = q + 1 - i;
+            if (k >= 0 && k < NK) { idx = HW * (i + 1); if (w.ylo) v = w.ylo[k + NK * i]; }
+        } else if (e < 4 * NI) {           // j = NJ
+            i = e - 3 * NI; k = q - NJ - i;
+            if (k >= 0 && k < NK) { idx = (NJ + 1) + HW * (i + 1); if (w.yhi) v = w.yhi[k + NK * i]; }
+        } else if (e < 4 * NI + NJ) {      // i = -1
+            j = e - 4 * NI; k = q + 1 - j;
+            if (k >= 0 && k < NK) { idx = j + 1; if (w.xlo) v = w.xlo[k + NK * j]; }
+        } else {                           // i = NI
+            j = e - 4 * NI - NJ; k = q - NI - j;
+            if (k >= 0 && k < NK) { idx = (j + 1) + HW * (NI + 1); if (w.xhi) v = w.xhi[k + NK * j]; }
+        }
+        if (idx >= 0) Hq[idx] = v;
+    }
+}
+
+// Same for the BP mask (own ghost layer, face layout).
+__device__ void fill_plane_bp(uint8_t *Hq, int q, const Block &B,
+                              const Shape &S, const uint32_t *cells,
+                              const int *ps, Mask m) {
+    const int NI = B.ni, NJ = B.nj, NK = B.nk, HW = NJ + 2;
+    const uint8_t *f = m.bpf + B.fc;
+    if (q >= 0 && q < S.np) {
+        const int cnt = ps[q + 1] - ps[q];
+        for (int x = threadIdx.x; x < cnt; x += blockDim.x) {
+            const int s = ps[q] + x;
+            int i, j, k;
+            unpack(cells[s], i, j, k);
+            Hq[(j + 1) + HW * (i + 1)] = m.bp[B.wo + s];
+        }
+    }
+    const int ng = 4 * NI + 2 * NJ;
+    for (int e = threadIdx.x; e < ng; e += blockDim.x) {
+        int i, j, k, idx = -1;
+        uint8_t v = 0;
+        if (e < NI) {
+            i = e; j = q + 1 - i;
+            if (j >= 0 && j < NJ) { idx = (j + 1) + HW * (i + 1); v = f[face_base(4, NI, NJ, NK) + j + NJ * i]; }
+        } else if (e < 2 * NI) {
+            i = e - NI; j = q - NK - i;
+            if (j >= 0 && j < NJ) { idx = (j + 1) + HW * (i + 1); v = f[face_base(5, NI, NJ, NK) + j + NJ * i]; }
+        } else if (e < 3 * NI) {
+            i = e - 2 * NI; k = q + 1 - i;
+            if (k >= 0 && k < NK) { idx = HW * (i + 1); v = f[face_base(2, NI, NJ, NK) + k + NK * i]; }
+        } else if (e < 4 * NI) {
+            i = e - 3 * NI; k = q - NJ - i;
+            if (k >= 0 && k < NK) { idx = (NJ + 1) + HW * (i + 1); v = f[face_base(3, NI, NJ, NK) + k + NK * i]; }
+        } else if (e < 4 * NI + NJ) {
+            j = e - 4 * NI; k = q + 1 - j;
+            if (k >= 0 && k < NK) { idx = j + 1; v = f[face_base(0, NI, NJ, NK) + k + NK * j]; }
+        } else {
+            j = e - 4 * NI - NJ; k = q - NI - j;
+            if (k >= 0 && k < NK) { idx = (j + 1) + HW * (NI + 1); v = f[face_base(1, NI, NJ, NK) + k + NK * j]; }
+        }
+        if (idx >= 0) Hq[idx] = v;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ Window<T> hilf_window(const T *hface,
+                                                 const Block *blocks,
+                                                 const Block &B) {
+    Window<T> w;
+    const int ni = B.ni, nj = B.nj, nk = B.nk;
+    w.xlo = nbr_face(hface, blocks, B.nbr[0], 1, ni, nj, nk);
+    w.xhi = nbr_face(hface, blocks, B.nbr[1], 0, ni, nj, nk);
+    w.ylo = nbr_face(hface, blocks, B.nbr[2], 3, ni, nj, nk);
+    w.yhi = nbr_face(hface, blocks, B.nbr[3], 2, ni, nj, nk);
+    w.zlo = nbr_face(hface, blocks, B.nbr[4], 5, ni, nj, nk);
+    w.zhi = nbr_face(hface, blocks, B.nbr[5], 4, ni, nj, nk);
+    return w;
+}
+
+// res = -(A phi) at one cell, src/flow/laplacephi_mod.F90:149-162, with
+// phi and BP read from the plane windows of planes p-1, p, p+1.
+template <typename T, typename A, bool MASKED>
+__device__ __forceinline__ A laplace_cell(const T *Hm, const T *H0,
+                                          const T *Hp, const uint8_t *Bm,
+                                          const uint8_t *B0,
+                                          const uint8_t *Bp, int c0, int HW,
+                                          A aw, A ae, A as, A an, A ab, A at,
+                                          A ap) {
+    const A pw = (A)Hm[c0 - HW], pe = (A)Hp[c0 + HW];
+    const A psv = (A)Hm[c0 - 1], pn = (A)Hp[c0 + 1];
+    const A pb = (A)Hm[c0], pt = (A)Hp[c0], pc = (A)H0[c0];
+    A r;
+    if (MASKED) {
+        const A bc = (A)B0[c0];
+        const A bw = (A)Bm[c0 - HW], be = (A)Bp[c0 + HW];
+        const A bs = (A)Bm[c0 - 1], bn = (A)Bp[c0 + 1];
+        const A bb = (A)Bm[c0], bt = (A)Bp[c0];
+        r = -(aw * pw * bw * bc);
+        r = r - ae * pe * bc * be;
+        r = r - as * psv * bs * bc;
+        r = r - an * pn * bc * bn;
+        r = r - ab * pb * bb * bc;
+        r = r - at * pt * bc * bt;
+    } else {
+        r = -(aw * pw);
+        r = r - ae * pe;
+        r = r - as * psv;
+        r = r - an * pn;
+        r = r - ab * pb;
+        r = r - at * pt;
+    }
+    r = r - ap * pc;
+    return r;
+}
+
+// ---------------------------------------------------------------------
+// Fused inner-iteration first half: RES = -(A HILF) (laplacephi_level),
+// then sipiter1 (src/flow/sip_hyperplane_mod.F90:450-505, classic order
+// src/flow/sip_classic_mod.F90:143-166).  Low-side ghosts of RES are the
+// neighbours' upper planes of the previous exchange (rhi[rd]); the upper
+// planes of this sweep go to rhi[1-rd] unless this is the last inner
+// iteration (the reference's conn(..., forward=-1) then skips faces
+// 1/3/5), the lower planes always go to rlo for the backward sweep.
+template <typename T, bool CLASSIC, bool MASKED>
+__global__ __launch_bounds__(kThreads) void k_forward(
+    Topo t, Fields<T> F, Mask m, int rd, int write_hi, int hilf_zero,
+    const int *done) {
+    if (*done) return;
+    extern __shared__ __align__(16) unsigned char lds_fw[];
+    const int b = t.blist[blockIdx.x];
+    const Block B = t.blocks[b];
+    const Shape S = t.shapes[B.shape];
+    const int NI = B.ni, NJ = B.nj, NK = B.nk, NP = S.np;
+    const int RW = NJ + 1, RS = (NJ + 1) * (NI + 1);
+    const int HW = NJ + 2, HS = (NJ + 2) * (NI + 2);
+    const int mRS = (t.max_nj + 1) * (t.max_ni + 1);
+    const int mHS = (t.max_nj + 2) * (t.max_ni + 2);
+    T *R = (T *)lds_fw;
+    T *H = R + 2 * mRS;
+    uint8_t *BH = (uint8_t *)(H + 4 * mHS);
+    const uint32_t *cells = t.cells + S.cell_off;
+    const int *ps = t.pstart + S.pstart_off;
+    const int64_t wo = B.wo;
+    const T *rh = F.rhi[rd];
+    T *rw = write_hi ? F.rhi[rd ^ 1] : nullptr;
+    const T *rzlo = nbr_face(rh, t.blocks, B.nbr[4], 5, NI, NJ, NK);
+    const T *rylo = nbr_face(rh, t.blocks, B.nbr[2], 3, NI, NJ, NK);
+    const T *rxlo = nbr_face(rh, t.blocks, B.nbr[0], 1, NI, NJ, NK);
+    const Window<T> hw = hilf_window(F.hface, t.blocks, B);
+    const T *aw = F.aw + B.o1x + 2, *ae = F.ae + B.o1x + 2;
+    const T *as = F.as + B.o1y + 2, *an = F.an + B.o1y + 2;
+    const T *ab = F.ab + B.o1z + 2, *at = F.at + B.o1z + 2;
+    const T *lw = F.f[0] + wo, *ls = F.f[1] + wo, *lb = F.f[2] + wo,
+            *lpr = F.f[3] + wo;
+    const T *rhs = F.rhs + wo, *apv = F.ap + wo;
+    T *res = F.res + wo;
+    const int64_t fc = B.fc;
+    const int fb0 = face_base(0, NI, NJ, NK), fb1 = face_base(1, NI, NJ, NK),
+              fb2 = face_base(2, NI, NJ, NK), fb3 = face_base(3, NI, NJ, NK),
+              fb4 = face_base(4, NI, NJ, NK), fb5 = face_base(5, NI, NJ, NK);
+
+    // pencils start from the (lagged) ghost below k = 0
+    for (int e = threadIdx.x; e < NJ * NI; e += blockDim.x) {
+        const int j = e % NJ, i = e / NJ;
+        const T v = rzlo ? rzlo[j + NJ * i] : T(0);
+        R[(j + 1) + RW * (i + 1)] = v;
+        R[RS + (j + 1) + RW * (i + 1)] = v;
+    }
+    // borders j = -1 / i = -1 read at step pp from R[(pp-1)&1]
+    auto fill_border = [&](T *Rb, int pp) {
+        for (int e = threadIdx.x; e < NI + NJ; e += blockDim.x) {
+            if (e < NI) {
+                const int i = e, k = pp - i;
+                if (k >= 0 && k < NK)
+                    Rb[RW * (i + 1)] = rylo ? rylo[k + NK * i] : T(0);
+            } else {
+                const int j = e - NI, k = pp - j;
+                if (k >= 0 && k < NK) Rb[j + 1] = rxlo ? rxlo[k + NK * j] : T(0);
+            }
+        }
+    };
+    fill_border(R + RS, 0);
+    const bool lap = !hilf_zero;
+    const bool masked = MASKED && !(B.flags & 1);
+    if (lap) {
+        for (int q = -1; q <= 1; ++q) {
+            fill_plane(H + ((q + 1) & 3) * mHS, q, B, S, cells, ps, F.hilf, hw);
+            if (masked) fill_plane_bp(BH + ((q + 1) & 3) * mHS, q, B, S, cells, ps, m);
+        }
+    }
+    __syncthreads();
+    (void)HS;
+
+    for (int p = 0; p < NP; ++p) {
+        const T *Rp = R + ((p - 1) & 1) * RS;
+        T *Rc = R + (p & 1) * RS;
+        const T *Hm = H + (p & 3) * mHS, *H0 = H + ((p + 1) & 3) * mHS,
+                *Hp = H + ((p + 2) & 3) * mHS;
+        const uint8_t *Bm = BH + (p & 3) * mHS, *B0 = BH + ((p + 1) & 3) * mHS,
+                      *Bp = BH + ((p + 2) & 3) * mHS;
+        const int cnt = ps[p + 1] - ps[p];
+        for (int x = threadIdx.x; x < cnt; x += blockDim.x) {
+            const int s = ps[p] + x;
+            int i, j, k;
+            unpack(cells[s], i, j, k);
+            T r = T(0);
+            if (lap) {
+                const int c0 = (j + 1) + HW * (i + 1);
+                if (masked)
+                    r = laplace_cell<T, T, true>(Hm, H0, Hp, Bm, B0, Bp, c0, HW,
+                                                 aw[i], ae[i], as[j], an[j],
+                                                 ab[k], at[k], apv[s]);
+                else
+                    r = laplace_cell<T, T, false>(Hm, H0, Hp, Bm, B0, Bp, c0,
+                                                  HW, aw[i], ae[i], as[j],
+                                                  an[j], ab[k], at[k], apv[s]);
+            }
+            const int c1 = (j + 1) + RW * (i + 1);
+            r = (rhs[s] + r) * lpr[s];
+            if (CLASSIC) {
+                r = r - lw[s] * Rp[c1 - RW];
+                r = r - ls[s] * Rp[c1 - 1];
+                r = r - lb[s] * Rp[c1];
+            } else {
+                r = r - lb[s] * Rp[c1];
+                r = r - ls[s] * Rp[c1 - 1];
+                r = r - lw[s] * Rp[c1 - RW];
+            }
+            Rc[c1] = r;
+            res[s] = r;
+            if (i == 0) F.rlo[fc + fb0 + k + NK * j] = r;
+            if (j == 0) F.rlo[fc + fb2 + k + NK * i] = r;
+            if (k == 0) F.rlo[fc + fb4 + j + NJ * i] = r;
+            if (rw) {
+                if (i == NI - 1) rw[fc + fb1 + k + NK * j] = r;
+                if (j == NJ - 1) rw[fc + fb3 + k + NK * i] = r;
+                if (k == NK - 1) rw[fc + fb5 + j + NJ * i] = r;
+            }
+        }
+        fill_border(Rc, p + 1);
+        if (lap && p + 2 <= NP) {
+            fill_plane(H + ((p + 3) & 3) * mHS, p + 2, B, S, cells, ps, F.hilf, hw);
+            if (masked) fill_plane_bp(BH + ((p + 3) & 3) * mHS, p + 2, B, S, cells, ps, m);
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+void launch_forward(const Topo &t, const Fields<T> &f, Mask m, int rd,
+                    int write_hi, int hilf_zero, int classic, const int *done,
+                    hipStream_t s) {
+    const size_t mRS = (size_t)(t.max_nj + 1) * (t.max_ni + 1);
+    const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
+    const bool masked = m.bp != nullptr;
+    const size_t lds = 2 * mRS * sizeof(T) + 4 * mHS * sizeof(T) +
+                       (masked ? 4 * mHS : 0) + 16;
+    if (classic) {
+        if (masked)
+            k_forward<T, true, true><<<t.nlist, kThreads, lds, s>>>(t, f, m, rd, write_hi, hilf_zero, done);
+        else
+            k_forward<T, true, false><<<t.nlist, kThreads, lds, s>>>(t, f, m, rd, write_hi, hilf_zero, done);
+    } else {
+        if (masked)
+            k_forward<T, false, true><<<t.nlist, kThreads, lds, s>>>(t, f, m, rd, write_hi, hilf_zero, done);
+        else
+            k_forward<T, false, false><<<t.nlist, kThreads, lds, s>>>(t, f, m, rd, write_hi, hilf_zero, done);
+    }
+}
+template void launch_forward<double>(const Topo &, const Fields<double> &,
+                                     Mask, int, int, int, int, const int *,
+                                     hipStream_t);
+template void launch_forward<float>(const Topo &, const Fields<float> &, Mask,
+                                    int, int, int, int, const int *,
+                                    hipStream_t);
+
+// ---------------------------------------------------------------------
+// Inner-iteration second half: sipiter2 (src/flow/sip_hyperplane_mod.F90:
+// 508-569; classic order src/flow/sip_classic_mod.F90:169-198), then
+// HILF += RES.  Upper ghosts of RES are the neighbours' lower planes of
+// this iteration's forward sweep.  Also writes HILF's boundary planes,
+// which stand in for the reference's conn(ilevel, 1, s1=dp) (:414).
+template <typename T, bool CLASSIC>
+__global__ __launch_bounds__(kThreads) void k_backward(Topo t, Fields<T> F,
+                                                       int hilf_zero,
+                                                       const int *done) {
+    if (*done) return;
+    extern __shared__ __align__(16) unsigned char lds_bw[];
+    const int b = t.blist[blockIdx.x];
+    const Block B = t.blocks[b];
+    const Shape S = t.shapes[B.shape];
+    const int NI = B.ni, NJ = B.nj, NK = B.nk, NP = S.np;
+    const int RW = NJ + 1, RS = (NJ + 1) * (NI + 1);
+    T *R = (T *)lds_bw;
+    const uint32_t *cells = t.cells + S.cell_off;
+    const int *ps = t.pstart + S.pstart_off;
+    const int64_t wo = B.wo;
+    const T *rzhi = nbr_face((const T *)F.rlo, t.blocks, B.nbr[5], 4, NI, NJ, NK);
+    const T *ryhi = nbr_face((const T *)F.rlo, t.blocks, B.nbr[3], 2, NI, NJ, NK);
+    const T *rxhi = nbr_face((const T *)F.rlo, t.blocks, B.nbr[1], 0, NI, NJ, NK);
+    const T *ue = F.f[4] + wo, *un = F.f[5] + wo, *ut = F.f[6] + wo;
+    const T *res = F.res + wo;
+    T *hilf = F.hilf + wo;
+    const int64_t fc = B.fc;
+    const int fb0 = face_base(0, NI, NJ, NK), fb1 = face_base(1, NI, NJ, NK),
+              fb2 = face_base(2, NI, NJ, NK), fb3 = face_base(3, NI, NJ, NK),
+              fb4 = face_base(4, NI, NJ, NK), fb5 = face_base(5, NI, NJ, NK);
+
+    for (int e = threadIdx.x; e < NJ * NI; e += blockDim.x) {
+        const int j = e % NJ, i = e / NJ;
+        const T v = rzhi ? rzhi[j + NJ * i] : T(0);
+        R[j + RW * i] = v;
+        R[RS + j + RW * i] = v;
+    }
+    // borders j = NJ / i = NI read at step pp from R[(pp+1)&1]
+    auto fill_border = [&](T *Rb, int pp) {
+        for (int e = threadIdx.x; e < NI + NJ; e += blockDim.x) {
+            if (e < NI) {
+                const int i = e, k = pp - (NJ - 1) - i;
+                if (k >= 0 && k < NK)
+                    Rb[NJ + RW * i] = ryhi ? ryhi[k + NK * i] : T(0);
+            } else {
+                const int j = e - NI, k = pp - j - (NI - 1);
+                if (k >= 0 && k < NK)
+                    Rb[j + RW * NI] = rxhi ? rxhi[k + NK * j] : T(0);
+            }
+        }
+    };
+    fill_border(R + (NP & 1) * RS, NP - 1);
+    __syncthreads();
+
+    for (int p = NP - 1; p >= 0; --p) {
+        const T *Rp = R + ((p + 1) & 1) * RS;
+        T *Rc = R + (p & 1) * RS;
+        const int cnt = ps[p + 1] - ps[p];
+        for (int x = threadIdx.x; x < cnt; x += blockDim.x) {
+            const int s = ps[p] + x;
+            int i, j, k;
+            unpack(cells[s], i, j, k);
+            const int c1 = j + RW * i;
+            T r = res[s];
+            if (CLASSIC) {
+                r = r - un[s] * Rp[c1 + 1];
+                r = r - ue[s] * Rp[c1 + RW];
+                r = r - ut[s] * Rp[c1];
+            } else {
+                r = r - ut[s] * Rp[c1];
+                r = r - un[s] * Rp[c1 + 1];
+                r = r - ue[s] * Rp[c1 + RW];
+            }
+            Rc[c1] = r;
+            const T h = (hilf_zero ? T(0) : hilf[s]) + r;
+            hilf[s] = h;
+            if (i == 0) F.hface[fc + fb0 + k + NK * j] = h;
+            if (i == NI - 1) F.hface[fc + fb1 + k + NK * j] = h;
+            if (j == 0) F.hface[fc + fb2 + k + NK * i] = h;
+            if (j == NJ - 1) F.hface[fc + fb3 + k + NK * i] = h;
+            if (k == 0) F.hface[fc + fb4 + j + NJ * i] = h;
+            if (k == NK - 1) F.hface[fc + fb5 + j + NJ * i] = h;
+        }
+        if (p > 0) fill_border(Rc, p - 1);
+        __syncthreads();
+    }
+}
+
+template <typename T>
+void launch_backward(const Topo &t, const Fields<T> &f, int hilf_zero,
+                     int classic, const int *done, hipStream_t s) {
+    const size_t mRS = (size_t)(t.max_nj + 1) * (t.max_ni + 1);
+    const size_t lds = 2 * mRS * sizeof(T) + 16;
+    if (classic)
+        k_backward<T, true><<<t.nlist, kThreads, lds, s>>>(t, f, hilf_zero, done);
+    else
+        k_backward<T, false><<<t.nlist, kThreads, lds, s>>>(t, f, hilf_zero, done);
+}
+template void launch_backward<double>(const Topo &, const Fields<double> &,
+                                      int, int, const int *, hipStream_t);
+template void launch_backward<float>(const Topo &, const Fields<float> &, int,
+                                     int, const int *, hipStream_t);
+
+// ---------------------------------------------------------------------
+// FP64 outer update (src/flow/pressuresolver_mod.F90:258-279): RES =
+// -(A HILF) with the FP64 operator, RHS += RES, max |RHS|, DP += HILF.
+// In the mixed-precision mode the sweeps' RHS copy is refreshed here.
+template <typename T, bool MASKED>
+__global__ __launch_bounds__(kThreads) void k_outer(Topo t, Fields<T> F,
+                                                    Mask m, Outer o,
+                                                    T *rhs_t) {
+    if (*o.done) return;
+    extern __shared__ __align__(16) unsigned char lds_ou[];
+    const int b = t.blist[blockIdx.x];
+    const Block B = t.blocks[b];
+    const Shape S = t.shapes[B.shape];
+    const int NJ = B.nj, NP = S.np, HW = NJ + 2;
+    const int mHS = (t.max_nj + 2) * (t.max_ni + 2);
+    T *H = (T *)lds_ou;
+    uint8_t *BH = (uint8_t *)(H + 4 * mHS);
+    const uint32_t *cells = t.cells + S.cell_off;
+    const int *ps = t.pstart + S.pstart_off;
+    const int64_t wo = B.wo;
+    const Window<T> hw = hilf_window(F.hface, t.blocks, B);
+    const double *aw = o.aw + B.o1x + 2, *ae = o.ae + B.o1x + 2;
+    const double *as = o.as + B.o1y + 2, *an = o.an + B.o1y + 2;
+    const double *ab = o.ab + B.o1z + 2, *at = o.at + B.o1z + 2;
+    const bool masked = MASKED && !(B.flags & 1);
+    double mx = 0.0;
+    for (int q = -1; q <= 1; ++q) {
+        fill_plane(H + ((q + 1) & 3) * mHS, q, B, S, cells, ps, (const T *)F.hilf, hw);
+        if (masked) fill_plane_bp(BH + ((q + 1) & 3) * mHS, q, B, S, cells, ps, m);
+    }
+    __syncthreads();
+    for (int p = 0; p < NP; ++p) {
+        const T *Hm = H + (p & 3) * mHS, *H0 = H + ((p + 1) & 3) * mHS,
+                *Hp = H + ((p + 2) & 3) * mHS;
+        const uint8_t *Bm = BH + (p & 3) * mHS, *B0 = BH + ((p + 1) & 3) * mHS,
+                      *Bp = BH + ((p + 2) & 3) * mHS;
+        const int cnt = ps[p + 1] - ps[p];
+        for (int x = threadIdx.x; x < cnt; x += blockDim.x) {
+            const int s = ps[p] + x;
+            int i, j, k;
+            unpack(cells[s], i, j, k);
+            const int c0 = (j + 1) + HW * (i + 1);
+            double r;
+            if (masked)
+                r = laplace_cell<T, double, true>(Hm, H0, Hp, Bm, B0, Bp, c0, HW,
+                                                  aw[i], ae[i], as[j], an[j],
+                                                  ab[k], at[k], o.ap64[wo + s]);
+            else
+                r = laplace_cell<T, double, false>(Hm, H0, Hp, Bm, B0, Bp, c0, HW,
+                                                   aw[i], ae[i], as[j], an[j],
+                                                   ab[k], at[k], o.ap64[wo + s]);
+            const double rn = o.rhs64[wo + s] + r;
+            o.rhs64[wo + s] = rn;
+            if (rhs_t) rhs_t[wo + s] = (T)rn;
+            const double a = fabs(rn);
+            mx = (a > mx || a != a) ? a : mx;
+            o.dp64[wo + s] = o.dp64[wo + s] + (double)H0[c0];
+        }
+        if (p + 2 <= NP) {
+            fill_plane(H + ((p + 3) & 3) * mHS, p + 2, B, S, cells, ps, (const T *)F.hilf, hw);
+            if (masked) fill_plane_bp(BH + ((p + 3) & 3) * mHS, p + 2, B, S, cells, ps, m);
+        }
+        __syncthreads();
+    }
+    mx = mx * ldexp(1.0, t.maxlevel - B.level);
+    block_max_to_global(mx, o.gmax);
+}
+
+template <typename T>
+void launch_outer(const Topo &t, const Fields<T> &f, Mask m, Outer o,
+                  T *rhs_t, hipStream_t s) {
+    const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
+    const bool masked = m.bp != nullptr;
+    const size_t lds = 4 * mHS * sizeof(T) + (masked ? 4 * mHS : 0) + 16;
+    if (masked)
+        k_outer<T, true><<<t.nlist, kThreads, lds, s>>>(t, f, m, o, rhs_t);
+    else
+        k_outer<T, false><<<t.nlist, kThreads, lds, s>>>(t, f, m, o, rhs_t);
+}
+template void launch_outer<double>(const Topo &, const Fields<double> &, Mask,
+                                   Outer, double *, hipStream_t);
+template void launch_outer<float>(const Topo &, const Fields<float> &, Mask,
+                                  Outer, float *, hipStream_t);
+
+// Exit test of the outer loop (src/flow/pressuresolver_mod.F90:274-301).
+__global__ void k_check(Outer o, int nouter, int nouter_min, double epcorr,
+                        double eps_rel, double prefak, double *hist,
+                        int hist_cap) {
+    if (*o.done) return;
+    const double m = __longlong_as_double((long long)*o.gmax);
+    *o.gmax = 0ull;
+    const int it = *o.ctr + 1;
+    *o.ctr = it;
+    *o.maxrhs = m;
+    if (hist && it <= hist_cap) hist[it - 1] = m;
+    const bool bad = (m != m) || m == INFINITY;
+    bool conv = false;
+    if (it >= nouter_min)
+        conv = eps_rel > 0.0 ? (m <= eps_rel * *o.maxrhs0)
+                             : (m / prefak < epcorr);
+    if (conv) *o.converged = 1;
+    if (bad) *o.nonfinite = 1;
+    if (conv || bad || it >= nouter) *o.done = 1;
+}
+
+void launch_check(Outer o, int nouter, int nouter_min, double epcorr,
+                  double eps_rel, double prefak, double *hist, int hist_cap,
+                  hipStream_t s) {
+    k_check<<<1, 1, 0, s>>>(o, nouter, nouter_min, epcorr, eps_rel, prefak,
+                            hist, hist_cap);
+}
+
+// ---------------------------------------------------------------------
+// WO -> reference layout (interior only; ghosts are left as they are).
+template <typename T>
+__global__ void k_wo_to_sp(Topo t, const T *src, double *dst) {
+    const int b = t.blist[blockIdx.y];
+    const Block B = t.blocks[b];
+    const Shape S = t.shapes[B.shape];
+    const uint32_t *cells = t.cells + S.cell_off;
+    const int64_t kk = B.nk + 4, jj = B.nj + 4;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S.ncell;
+         s += gridDim.x * blockDim.x) {
+        int i, j, k;
+        unpack(cells[s], i, j, k);
+        dst[B.sp + (k + 2) + (j + 2) * kk + (i + 2) * kk * jj] =
+            (double)src[B.wo + s];
+    }
+}
+template <typename T>
+void launch_wo_to_sp(const Topo &t, const T *src, double *dst,
+                     hipStream_t s) {
+    const int maxc = t.max_ni * t.max_nj * t.max_nk;
+    dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
+    k_wo_to_sp<T><<<grid, kThreads, 0, s>>>(t, src, dst);
+}
+template void launch_wo_to_sp<double>(const Topo &, const double *, double *,
+                                      hipStream_t);
+template void launch_wo_to_sp<float>(const Topo &, const float *, double *,
+                                     hipStream_t);
+template void launch_wo_to_sp<uint8_t>(const Topo &, const uint8_t *,
+                                       double *, hipStream_t);
+
+// Factors back to the reference's storage: hyperplane access order in the
+// first ncell entries of the block (src/flow/sip_hyperplane_mod.F90:
+// 224-230) or spatial order for the classic solver.
+template <typename T>
+__global__ void k_to_ref_factors(Topo t, const T *wo, double *ref,
+                                 int spatial) {
+    const int b = t.blist[blockIdx.y];
+    const Block B = t.blocks[b];
+    const Shape S = t.shapes[B.shape];
+    const uint32_t *cells = t.cells + S.cell_off;
+    const int64_t kk = B.nk + 4, jj = B.nj + 4;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S.ncell;
+         s += gridDim.x * blockDim.x) {
+        int i, j, k;
+        unpack(cells[s], i, j, k);
+        const int64_t d = spatial ? (k + 2) + (j + 2) * kk + (i + 2) * kk * jj
+                                  : (int64_t)s;
+        ref[B.sp + d] = (double)wo[B.wo + s];
+    }
+}
+template <typename T>
+void launch_to_ref_factors(const Topo &t, const T *wo, double *ref,
+                           int spatial, hipStream_t s) {
+    const int maxc = t.max_ni * t.max_nj * t.max_nk;
+    dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
+    k_to_ref_factors<T><<<grid, kThreads, 0, s>>>(t, wo, ref, spatial);
+}
+template void launch_to_ref_factors<double>(const Topo &, const double *,
+                                            double *, int, hipStream_t);
+template void launch_to_ref_factors<float>(const Topo &, const float *,
+                                           double *, int, hipStream_t);
+
+// ---------------------------------------------------------------------
+// Velocity and pressure correction (src/flow/pressuresolver_mod.F90:
+// 787-879) on the reference layout; one thread per stored cell.
+__global__ void k_mgpcorr(Topo t, double *u_, double *v_, double *w_,
+                          double *p_, const double *dp_, const double *rdx_,
+                          const double *rdy_, const double *rdz_,
+                          double rfak, const double *bp_) {
+    const int b = t.blist[blockIdx.y];
+    const Block B = t.blocks[b];
+    const int kk = B.nk + 4, jj = B.nj + 4, ii = B.ni + 4;
+    const int64_t n = (int64_t)kk * jj * ii;
+    double *u = u_ + B.sp, *v = v_ + B.sp, *w = w_ + B.sp, *p = p_ + B.sp;
+    const double *dp = dp_ + B.sp;
+    const double *bp = bp_ ? bp_ + B.sp : nullptr;
+    const double *rdx = rdx_ + B.o1x - 1, *rdy = rdy_ + B.o1y - 1,
+                 *rdz = rdz_ + B.o1z - 1;  // 1-based
+    const int64_t sj = kk, si = (int64_t)kk * jj;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(e % kk) + 1;
+        const int j = (int)((e / kk) % jj) + 1;
+        const int i = (int)(e / si) + 1;
+        if (i >= 2 && i <= ii - 1 && j >= 2 && j <= jj - 1 && k >= 2 &&
+            k <= kk - 1)
+            p[e] = bp ? p[e] + dp[e] * bp[e] : p[e] + dp[e];
+        if (i >= 2 && i <= ii - 2 && j >= 3 && j <= jj - 2 && k >= 3 &&
+            k <= kk - 2)
+            u[e] = bp ? u[e] + (dp[e] - dp[e + si]) * bp[e] * bp[e + si] *
+                                   rdx[i] * rfak
+                      : u[e] + (dp[e] - dp[e + si]) * rdx[i] * rfak;
+        if (i >= 3 && i <= ii - 2 && j >= 2 && j <= jj - 2 && k >= 3 &&
+            k <= kk - 2)
+            v[e] = bp ? v[e] + (dp[e] - dp[e + sj]) * bp[e] * bp[e + sj] *
+                                   rdy[j] * rfak
+                      : v[e] + (dp[e] - dp[e + sj]) * rdy[j] * rfak;
+        if (i >= 3 && i <= ii - 2 && j >= 3 && j <= jj - 2 && k >= 2 &&
+            k <= kk - 2)
+            w[e] = bp ? w[e] + (dp[e] - dp[e + 1]) * bp[e] * bp[e + 1] *
+                                   rdz[k] * rfak
+                      : w[e] + (dp[e] - dp[e + 1]) * rdz[k] * rfak;
+    }
+}
+
+void launch_mgpcorr(const Topo &t, double *u, double *v, double *w,
+                    double *p, const double *dp, const double *rdx,
+                    const double *rdy, const double *rdz, double rfak,
+                    const double *bp, hipStream_t s) {
+    const int maxc = (t.max_ni + 4) * (t.max_nj + 4) * (t.max_nk + 4);
+    dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
+    k_mgpcorr<<<grid, kThreads, 0, s>>>(t, u, v, w, p, dp, rdx, rdy, rdz,
+                                        rfak, bp);
+}
+
+// ---------------------------------------------------------------------
+// Ghost-layer box copies (connect_self_single, src/core/connect/
+// conn1_mod.F90:674-704): one workgroup per task, all fields.
+struct FieldList {
+    double *f[6];
+    int n;
+};
+
+__global__ void k_copy(const CopyTask *tasks, FieldList fl) {
+    const CopyTask T = tasks[blockIdx.x];
+    const int n = T.ni * T.nj * T.nk;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int k = e % T.nk, j = (e / T.nk) % T.nj, i = e / (T.nk * T.nj);
+        const int64_t so = T.src + k + (int64_t)j * T.s_sj + (int64_t)i * T.s_si;
+        const int64_t d = T.dst + k + (int64_t)j * T.d_sj + (int64_t)i * T.d_si;
+        for (int f = 0; f < fl.n; ++f) fl.f[f][d] = fl.f[f][so];
+    }
+}
+
+void launch_copy(const CopyTask *tasks, int ntask, double *const *fields,
+                 int nfield, hipStream_t s) {
+    if (ntask <= 0) return;
+    FieldList fl;
+    fl.n = nfield;
+    for (int f = 0; f < nfield && f < 6; ++f) fl.f[f] = fields[f];
+    k_copy<<<ntask, 128, 0, s>>>(tasks, fl);
+}
+
+}  // namespace mgp

@@ -1,0 +1,117 @@
+// mgp_launch.h -- host-side launchers of the gfx950 kernels in
+// mgp_kernels.hip, called by the orchestration in mgp_api.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mgp_internal.h"
+
+namespace mgp {
+
+// Read-only per-level device view of the topology and shape tables.
+struct Topo {
+    const Block *blocks;   // [nblocks]
+    const int *blist;      // block indices of the launch
+    int nlist;             // number of blocks in the launch
+    const Shape *shapes;
+    const uint32_t *cells; // packed (i | j<<10 | k<<20) per WO slot
+    const int *pstart;     // plane starts per shape
+    const int *slot_of;    // spatial (k + NK(j + NJ i)) -> WO slot
+    int max_ni, max_nj, max_nk;
+    int maxlevel;
+};
+
+// Field set of one precision T (double or float) in WO layout.
+template <typename T>
+struct Fields {
+    T *hilf;                    // correction HILF
+    T *rhs;                     // right-hand side read by the sweeps
+    T *res;                     // forward-sweep result RES
+    const T *ap;                // GSAP
+    const T *f[7];              // LW LS LB LPR UE UN UT
+    const T *aw, *ae, *as, *an, *ab, *at;  // 1-D coefficients (ref. layout)
+    T *hface;                   // HILF boundary planes (own faces)
+    T *rhi[2];                  // RES upper planes, double-buffered
+    T *rlo;                     // RES lower planes
+};
+
+// Shared state of the FP64 outer (defect-correction) loop.
+struct Outer {
+    double *rhs64;       // RHS in FP64 (WO)
+    double *dp64;        // DP in FP64 (WO)
+    const double *ap64;  // GSAP in FP64 (WO)
+    const double *aw, *ae, *as, *an, *ab, *at;  // FP64 1-D coefficients
+    unsigned long long *gmax;  // running max |RHS| (bit pattern)
+    double *maxrhs0;     // max |RHS| before the first iteration
+    double *maxrhs;      // max |RHS| after the last outer iteration
+    int *ctr;            // outer iterations done
+    int *done;           // 1: stop (converged, nouter reached or non-finite)
+    int *converged;      // 1: exit test fired
+    int *nonfinite;      // 1: NaN/Inf residual seen
+};
+
+struct Mask {
+    const uint8_t *bp;   // BP per WO cell (1 fluid, 0 blocked)
+    const uint8_t *bpf;  // BP of the block's own ghost layer, face layout
+};
+
+void launch_fill_u8(uint8_t *p, int64_t n, uint8_t v, hipStream_t s);
+template <typename T>
+void launch_fill(T *p, int64_t n, T v, hipStream_t s);
+
+void launch_ap(const Topo &t, const double *dx, const double *dy,
+               const double *dz, const double *aw_raw, const double *ae_raw,
+               const double *as_raw, const double *an_raw,
+               const double *ab_raw, const double *at_raw, Mask m,
+               double *ap, hipStream_t s);
+
+template <typename T>
+void launch_lu(const Topo &t, const double *ap, const double *aw,
+               const double *ae, const double *as, const double *an,
+               const double *ab, const double *at, Mask m, T *const f[7],
+               hipStream_t s);
+
+template <typename T>
+void launch_convert(const double *src, T *dst, int64_t n, hipStream_t s);
+
+template <typename T>
+void launch_divcal(const Topo &t, const double *u, const double *v,
+                   const double *w, const double *rddx, const double *rddy,
+                   const double *rddz, const double *bp_sp, double fak,
+                   double *rhs64, T *rhs_t, hipStream_t s);
+
+void launch_maxabs(const Topo &t, const double *rhs64, Outer o, hipStream_t s);
+
+template <typename T>
+void launch_forward(const Topo &t, const Fields<T> &f, Mask m, int rd,
+                    int write_hi, int hilf_zero, int classic, const int *done,
+                    hipStream_t s);
+
+template <typename T>
+void launch_backward(const Topo &t, const Fields<T> &f, int hilf_zero,
+                     int classic, const int *done, hipStream_t s);
+
+template <typename T>
+void launch_outer(const Topo &t, const Fields<T> &f, Mask m, Outer o,
+                  T *rhs_t, hipStream_t s);
+
+void launch_check(Outer o, int nouter, int nouter_min, double epcorr,
+                  double eps_rel, double prefak, double *hist, int hist_cap,
+                  hipStream_t s);
+
+template <typename T>
+void launch_wo_to_sp(const Topo &t, const T *src, double *dst,
+                     hipStream_t s);
+
+void launch_mgpcorr(const Topo &t, double *u, double *v, double *w,
+                    double *p, const double *dp, const double *rdx,
+                    const double *rdy, const double *rdz, double rfak,
+                    const double *bp_sp, hipStream_t s);
+
+void launch_copy(const CopyTask *tasks, int ntask, double *const *fields,
+                 int nfield, hipStream_t s);
+
+template <typename T>
+void launch_to_ref_factors(const Topo &t, const T *wo, double *ref,
+                           int spatial, hipStream_t s);
+
+}  // namespace mgp

@@ -1,0 +1,152 @@
+"""mgp -- Python front end of the MI355X pressure-projection library.
+
+`PressureSolver` mirrors the Fortran module `pressuresolver_mod`
+(src/flow/pressuresolver_mod.F90:45): construction = init_pressuresolver
+(+ ib%giteig), `mgpoisl` = mgpoisl, `close` = finish_pressuresolver.  The
+keyword arguments are the `/flow/pressuresolver` keys with the reference's
+defaults and validation (:57-142).  Fields are flat float64 arrays in the
+reference layout (see mgp.grids): numpy arrays are copied through host
+memory, torch CUDA tensors are used in place.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import MgpConfig, MgpError, MgpGrids, MgpStats
+from .grids import Grids, uniform_box  # noqa: F401
+
+__all__ = ["PressureSolver", "MgpError", "Grids", "uniform_box"]
+
+_TYPES = {"sip": 0, "sor": 1, "classic": 2}
+_IB = {"noib": 0, "ghostcell": 1}
+_PREC = {"fp64": 0, "fp32": 1}
+
+
+def _ptr(a, ct=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class PressureSolver:
+    def __init__(self, grids: Grids, type="sip", ninner=5, nouter=9, nouter_min=0,
+                 epcorr=0.0, omega=1.1, rho=1.0, ib="noib", precision="fp64",
+                 eps_rel=0.0, check_every=4, bp=None, blocked=None):
+        self.lib = _lib.load()
+        self.grids = g = grids
+        t = type.lower()
+        if t not in _TYPES:
+            raise MgpError(f"Invalid pressure solver type: {type}")
+        self._keep = [g.kk, g.jj, g.ii, g.level, g.nbr, g.bc,
+                      g.ip3 + 1, g.ip1x + 1, g.ip1y + 1, g.ip1z + 1]
+        k = self._keep
+        self._grids = MgpGrids(g.ngrid, g.minlevel, g.maxlevel,
+                               _ptr(k[0], C.c_int32), _ptr(k[1], C.c_int32),
+                               _ptr(k[2], C.c_int32), _ptr(k[3], C.c_int32),
+                               _ptr(k[4], C.c_int32), _ptr(k[5], C.c_int32),
+                               _ptr(k[6], C.c_int64), _ptr(k[7], C.c_int64),
+                               _ptr(k[8], C.c_int64), _ptr(k[9], C.c_int64))
+        self.cfg = MgpConfig(_TYPES[t], ninner, nouter, nouter_min, epcorr, omega, rho,
+                             _IB[ib], _PREC[precision], eps_rel, 0, check_every)
+        h = C.c_void_p()
+        if blocked is not None:
+            blocked = np.ascontiguousarray(blocked, dtype=np.int8)
+            rc = self.lib.mgp_init_mask(C.byref(self._grids), C.byref(self.cfg), _ptr(g.dx),
+                                        _ptr(g.dy), _ptr(g.dz), _ptr(g.ddx), _ptr(g.ddy),
+                                        _ptr(g.ddz), _ptr(blocked, C.c_int8), C.byref(h))
+        else:
+            bpa = None if bp is None else np.ascontiguousarray(bp, dtype=np.float64)
+            rc = self.lib.mgp_init(C.byref(self._grids), C.byref(self.cfg), _ptr(g.dx),
+                                   _ptr(g.dy), _ptr(g.dz), _ptr(g.ddx), _ptr(g.ddy),
+                                   _ptr(g.ddz), _ptr(bpa) if bpa is not None else None,
+                                   C.byref(h))
+        _lib.check(rc, "mgp_init")
+        self.h = h
+
+    # ------------------------------------------------------------------
+    def set_iterations(self, ninner=None, nouter=None, nouter_min=None, epcorr=None,
+                       eps_rel=None, check_every=None):
+        c = self.cfg
+        c.ninner = c.ninner if ninner is None else ninner
+        c.nouter = c.nouter if nouter is None else nouter
+        c.nouter_min = c.nouter_min if nouter_min is None else nouter_min
+        c.epcorr = c.epcorr if epcorr is None else epcorr
+        c.eps_rel = c.eps_rel if eps_rel is None else eps_rel
+        c.check_every = c.check_every if check_every is None else check_every
+        _lib.check(self.lib.mgp_set_iterations(self.h, c.ninner, c.nouter, c.nouter_min,
+                                               c.epcorr, c.eps_rel, c.check_every),
+                   "mgp_set_iterations")
+
+    def mgpoisl(self, u, v, w, p, dt, ittot=0, irk=0):
+        """Project (u, v, w) and update p in place; returns the stats."""
+        st = MgpStats()
+        if isinstance(u, np.ndarray):
+            for a in (u, v, w, p):
+                if a.dtype != np.float64 or not a.flags.c_contiguous or a.size != self.grids.n3d:
+                    raise MgpError("fields must be contiguous float64 of the 3-D length")
+            rc = self.lib.mgp_mgpoisl(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(p), dt, ittot,
+                                      irk, C.byref(st))
+        else:
+            import torch
+            for a in (u, v, w, p):
+                if (a.dtype != torch.float64 or not a.is_cuda or not a.is_contiguous()
+                        or a.numel() != self.grids.n3d):
+                    raise MgpError("fields must be contiguous CUDA float64 tensors")
+            self._bind_stream()
+            rc = self.lib.mgp_mgpoisl_device(self.h, u.data_ptr(), v.data_ptr(), w.data_ptr(),
+                                             p.data_ptr(), dt, ittot, irk, C.byref(st))
+        _lib.check(rc, "mgp_mgpoisl")
+        return dict(ipc=st.ipc, nouter_done=st.nouter_done, converged=bool(st.converged),
+                    maxrhs0=st.maxrhs0, maxrhs=st.maxrhs, solve_ms=st.solve_ms)
+
+    def _bind_stream(self):
+        # the library works on its own stream and synchronises it before
+        # returning; torch work queued before the call must be complete
+        import torch
+        torch.cuda.current_stream().synchronize()
+
+    def get_field(self, name):
+        g = self.grids
+        n = {"GSAW": g.n1x, "GSAE": g.n1x, "GSAS": g.n1y, "GSAN": g.n1y,
+             "GSAB": g.n1z, "GSAT": g.n1z}.get(name, g.n3d)
+        out = np.zeros(n)
+        _lib.check(self.lib.mgp_get_field(self.h, name.encode(), _ptr(out)), "mgp_get_field")
+        return out
+
+    def connect(self, fields, level=-1, layers=1, corners=False, forward=0):
+        """Ghost-layer exchange of CUDA float64 tensors (reference layout)."""
+        self._bind_stream()
+        arr = (C.c_void_p * len(fields))(*[f.data_ptr() for f in fields])
+        _lib.check(self.lib.mgp_connect_device(self.h, level, layers, int(corners), forward,
+                                               arr, len(fields)), "mgp_connect_device")
+
+    def divcal(self, u, v, w, fak=1.0):
+        import torch
+        self._bind_stream()
+        div = torch.zeros_like(u)
+        _lib.check(self.lib.mgp_divcal_device(self.h, div.data_ptr(), u.data_ptr(),
+                                              v.data_ptr(), w.data_ptr(), fak),
+                   "mgp_divcal_device")
+        return div
+
+    def time_kernels(self, niter=10):
+        ms = np.zeros(3)
+        _lib.check(self.lib.mgp_time_kernels(self.h, niter, _ptr(ms)), "mgp_time_kernels")
+        return dict(forward_ms=ms[0], backward_ms=ms[1], outer_ms=ms[2])
+
+    def info(self):
+        a, b = C.c_int64(), C.c_int64()
+        _lib.check(self.lib.mgp_info(self.h, C.byref(a), C.byref(b)), "mgp_info")
+        return dict(ncells=a.value, device_bytes=b.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.check(self.lib.mgp_finish(self.h), "mgp_finish")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

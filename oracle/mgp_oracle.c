@@ -1,0 +1,1062 @@
+/*
+ * mgp_oracle.c -- CPU restatement (FP64) of MGLET-base's pressure-projection
+ * hot path.  TEST INFRASTRUCTURE ONLY: the checker for the HIP library and
+ * the timed CPU baseline.  The product path never links or calls this file.
+ *
+ * Each function cites the reference routine it restates.  The restatement
+ * keeps the reference's loop bounds, evaluation order and literals:
+ *   - `alfa = 0.92` and `1.0e-20` are default-real (real32) literals in the
+ *     reference, so in an FP64 build they hold (double)0.92f and
+ *     (double)1.0e-20f (src/flow/sip_hyperplane_mod.F90:123, 202;
+ *     src/flow/sip_classic_mod.F90:26, 82);
+ *   - Fortran evaluates `a - b*c - d*e` left to right; so does C here, and
+ *     the file is built with -ffp-contract=off (no FMA contraction).
+ * Parity status of each function is recorded in the tests that pin it.
+ */
+#include "mgp_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ALFA ((double)0.92f)
+#define TINY ((double)1.0e-20f)
+
+/* 1-based (k, j, i) -> 0-based offset inside one block */
+#define IX(k, j, i) ((int64_t)((k)-1) + (int64_t)((j)-1) * kk + \
+                     (int64_t)((i)-1) * kk * jj)
+
+#define GDIMS(t, g)                 \
+    const int kk = (t)->kk[(g)-1];  \
+    const int jj = (t)->jj[(g)-1];  \
+    const int ii = (t)->ii[(g)-1];  \
+    (void)kk; (void)jj; (void)ii
+
+/* ------------------------------------------------------------------ */
+/* Face / edge / corner tables: src/core/connect/connect_core_mod.F90:37-165.
+ * These tables are part of the data format (face numbering) and must be
+ * the same as the reference's. */
+static const int FACELIST[26][4] = {
+    {1, 1, 0, 0}, {1, 2, 0, 0}, {1, 3, 0, 0}, {1, 4, 0, 0}, {1, 5, 0, 0},
+    {1, 6, 0, 0}, {2, 1, 3, 0}, {2, 1, 4, 0}, {2, 1, 5, 0}, {2, 1, 6, 0},
+    {2, 2, 3, 0}, {2, 2, 4, 0}, {2, 2, 5, 0}, {2, 2, 6, 0}, {2, 3, 5, 0},
+    {2, 3, 6, 0}, {2, 4, 5, 0}, {2, 4, 6, 0}, {3, 1, 3, 5}, {3, 1, 3, 6},
+    {3, 1, 4, 5}, {3, 1, 4, 6}, {3, 2, 3, 5}, {3, 2, 3, 6}, {3, 2, 4, 5},
+    {3, 2, 4, 6}};
+static const int FACENBR[26] = {2,  1,  4,  3,  6,  5,  12, 11, 14,
+                                13, 8,  7,  10, 9,  18, 17, 16, 15,
+                                26, 25, 24, 23, 22, 21, 20, 19};
+static const int RESCUE_DIR[26][7] = {
+    {1, 0, 0, 0, 0, 0, 0},      {2, 0, 0, 0, 0, 0, 0},
+    {3, 0, 0, 0, 0, 0, 0},      {4, 0, 0, 0, 0, 0, 0},
+    {5, 0, 0, 0, 0, 0, 0},      {6, 0, 0, 0, 0, 0, 0},
+    {7, 1, 3, 0, 0, 0, 0},      {8, 4, 1, 0, 0, 0, 0},
+    {9, 1, 5, 0, 0, 0, 0},      {10, 6, 1, 0, 0, 0, 0},
+    {11, 2, 3, 0, 0, 0, 0},     {12, 2, 4, 0, 0, 0, 0},
+    {13, 2, 5, 0, 0, 0, 0},     {14, 2, 6, 0, 0, 0, 0},
+    {15, 3, 5, 0, 0, 0, 0},     {16, 6, 3, 0, 0, 0, 0},
+    {17, 4, 5, 0, 0, 0, 0},     {18, 4, 6, 0, 0, 0, 0},
+    {19, 7, 9, 15, 1, 3, 5},    {20, 7, 10, 16, 1, 3, 6},
+    {21, 8, 9, 17, 1, 4, 5},    {22, 8, 10, 18, 1, 4, 6},
+    {23, 11, 13, 15, 2, 3, 5},  {24, 11, 14, 16, 2, 3, 6},
+    {25, 12, 13, 17, 2, 4, 5},  {26, 12, 14, 18, 2, 4, 6}};
+static const int RESCUE_NBR[26][7] = {
+    {2, 0, 0, 0, 0, 0, 0},        {1, 0, 0, 0, 0, 0, 0},
+    {4, 0, 0, 0, 0, 0, 0},        {3, 0, 0, 0, 0, 0, 0},
+    {6, 0, 0, 0, 0, 0, 0},        {5, 0, 0, 0, 0, 0, 0},
+    {12, 11, 8, 0, 0, 0, 0},      {11, 7, 12, 0, 0, 0, 0},
+    {14, 13, 10, 0, 0, 0, 0},     {13, 9, 14, 0, 0, 0, 0},
+    {8, 7, 12, 0, 0, 0, 0},       {7, 8, 11, 0, 0, 0, 0},
+    {10, 9, 14, 0, 0, 0, 0},      {9, 10, 13, 0, 0, 0, 0},
+    {18, 17, 16, 0, 0, 0, 0},     {17, 15, 18, 0, 0, 0, 0},
+    {16, 15, 18, 0, 0, 0, 0},     {15, 16, 17, 0, 0, 0, 0},
+    {26, 25, 24, 22, 23, 21, 20}, {25, 26, 23, 21, 24, 22, 19},
+    {24, 23, 26, 20, 25, 19, 22}, {23, 24, 25, 19, 26, 20, 21},
+    {22, 21, 20, 26, 19, 25, 24}, {21, 22, 19, 25, 20, 26, 23},
+    {20, 19, 22, 24, 21, 23, 26}, {19, 20, 21, 23, 22, 24, 25}};
+
+static int is_con(int code) { return code == 7 || code == 19; }
+
+/* ------------------------------------------------------------------ */
+/* src/ib/noib_mod.F90:81-250 (giteig) */
+void og_giteig(const og_topo *t, const double *dx_, const double *dy_,
+               const double *dz_, const double *bp_, double *aw_,
+               double *ae_, double *as_, double *an_, double *ab_,
+               double *at_, double *ap_)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        GDIMS(t, g);
+        const double *dx = dx_ + t->ip1x[g - 1] - 1; /* 1-based */
+        const double *dy = dy_ + t->ip1y[g - 1] - 1;
+        const double *dz = dz_ + t->ip1z[g - 1] - 1;
+        double *aw = aw_ + t->ip1x[g - 1] - 1, *ae = ae_ + t->ip1x[g - 1] - 1;
+        double *as = as_ + t->ip1y[g - 1] - 1, *an = an_ + t->ip1y[g - 1] - 1;
+        double *ab = ab_ + t->ip1z[g - 1] - 1, *at = at_ + t->ip1z[g - 1] - 1;
+        const double *bp = bp_ + t->ip3[g - 1];
+        double *ap = ap_ + t->ip3[g - 1];
+        const int32_t *bc = t->bc + 6 * (g - 1);
+
+        for (int i = 3; i <= ii - 2; ++i) {
+            ae[i] = 2.0 / ((dx[i - 1] + dx[i]) * dx[i]);
+            aw[i] = 2.0 / ((dx[i - 1] + dx[i]) * dx[i - 1]);
+        }
+        for (int j = 3; j <= jj - 2; ++j) {
+            an[j] = 2.0 / ((dy[j - 1] + dy[j]) * dy[j]);
+            as[j] = 2.0 / ((dy[j - 1] + dy[j]) * dy[j - 1]);
+        }
+        for (int k = 3; k <= kk - 2; ++k) {
+            at[k] = 2.0 / ((dz[k - 1] + dz[k]) * dz[k]);
+            ab[k] = 2.0 / ((dz[k - 1] + dz[k]) * dz[k - 1]);
+        }
+        for (int i = 3; i <= ii - 2; ++i)
+            for (int j = 3; j <= jj - 2; ++j)
+                for (int k = 3; k <= kk - 2; ++k)
+                    ap[IX(k, j, i)] = -(2.0 / (dx[i - 1] * dx[i]))
+                                      - 2.0 / (dy[j - 1] * dy[j])
+                                      - 2.0 / (dz[k - 1] * dz[k]);
+        for (int i = 3; i <= ii - 2; ++i)
+            for (int j = 3; j <= jj - 2; ++j)
+                for (int k = 3; k <= kk - 2; ++k) {
+                    const double c = bp[IX(k, j, i)];
+                    double a = ap[IX(k, j, i)];
+                    a = a + aw[i] * (1.0 - bp[IX(k, j, i - 1)] * c);
+                    a = a + ae[i] * (1.0 - c * bp[IX(k, j, i + 1)]);
+                    a = a + as[j] * (1.0 - bp[IX(k, j - 1, i)] * c);
+                    a = a + an[j] * (1.0 - c * bp[IX(k, j + 1, i)]);
+                    a = a + ab[k] * (1.0 - bp[IX(k - 1, j, i)] * c);
+                    a = a + at[k] * (1.0 - c * bp[IX(k + 1, j, i)]);
+                    ap[IX(k, j, i)] = a;
+                }
+        const int nfro = bc[0], nbac = bc[1], nrgt = bc[2], nlft = bc[3],
+                  nbot = bc[4], ntop = bc[5];
+        if (nfro == 2 || nfro == 5 || nfro == 6 || nfro == 19) {
+            for (int j = 3; j <= jj - 2; ++j)
+                for (int k = 3; k <= kk - 2; ++k)
+                    ap[IX(k, j, 3)] = ap[IX(k, j, 3)]
+                        + aw[3] * (bp[IX(k, j, 2)] * bp[IX(k, j, 3)]);
+            aw[3] = 0.0;
+        }
+        if (nbac == 2 || nbac == 5 || nbac == 6) {
+            for (int j = 3; j <= jj - 2; ++j)
+                for (int k = 3; k <= kk - 2; ++k)
+                    ap[IX(k, j, ii - 2)] = ap[IX(k, j, ii - 2)]
+                        + ae[ii - 2] * (bp[IX(k, j, ii - 2)] * bp[IX(k, j, ii - 1)]);
+            ae[ii - 2] = 0.0;
+        }
+        if (nrgt == 2 || nrgt == 5 || nrgt == 6 || nrgt == 19) {
+            for (int i = 3; i <= ii - 2; ++i)
+                for (int k = 3; k <= kk - 2; ++k)
+                    ap[IX(k, 3, i)] = ap[IX(k, 3, i)]
+                        + as[3] * (bp[IX(k, 2, i)] * bp[IX(k, 3, i)]);
+            as[3] = 0.0;
+        }
+        if (nlft == 2 || nlft == 5 || nlft == 6) {
+            for (int i = 3; i <= ii - 2; ++i)
+                for (int k = 3; k <= kk - 2; ++k)
+                    ap[IX(k, jj - 2, i)] = ap[IX(k, jj - 2, i)]
+                        + an[jj - 2] * (bp[IX(k, jj - 2, i)] * bp[IX(k, jj - 1, i)]);
+            an[jj - 2] = 0.0;
+        }
+        if (nbot == 2 || nbot == 5 || nbot == 6 || nbot == 19) {
+            for (int i = 3; i <= ii - 2; ++i)
+                for (int j = 3; j <= jj - 2; ++j)
+                    ap[IX(3, j, i)] = ap[IX(3, j, i)]
+                        + ab[3] * (bp[IX(2, j, i)] * bp[IX(3, j, i)]);
+            ab[3] = 0.0;
+        }
+        if (ntop == 2 || ntop == 5 || ntop == 6) {
+            for (int i = 3; i <= ii - 2; ++i)
+                for (int j = 3; j <= jj - 2; ++j)
+                    ap[IX(kk - 2, j, i)] = ap[IX(kk - 2, j, i)]
+                        + at[kk - 2] * (bp[IX(kk - 2, j, i)] * bp[IX(kk - 1, j, i)]);
+            at[kk - 2] = 0.0;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/flow/sip_hyperplane_mod.F90:269-447: MIP(m) plane offsets and IDX
+ * (1-based linear index, ascending inside each plane).  mip/idx are full
+ * 3-D intfields; this block's MIP uses entries 1..ii+jj+kk of its region. */
+static int cmp_i64(const void *a, const void *b)
+{
+    const int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+    return (x > y) - (x < y);
+}
+
+int og_hp_tables(const og_topo *t, int64_t *mip_, int64_t *idx_)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        GDIMS(t, g);
+        int64_t *mip = mip_ + t->ip3[g - 1] - 1; /* 1-based */
+        int64_t *idx = idx_ + t->ip3[g - 1] - 1;
+        const int n3dmin = 9, n3dmax = (ii - 2) + (jj - 2) + (kk - 2);
+        for (int m = 1; m <= ii + jj + kk; ++m) mip[m] = 0;
+        for (int64_t s = 1; s <= (int64_t)ii * jj * kk; ++s) idx[s] = -1;
+        for (int m = n3dmin; m <= n3dmax; ++m) {
+            int ic1 = 0;
+            const int64_t lm = mip[m];
+            const int nkmin = (m - (ii - 2) - (jj - 2) > 3) ? m - (ii - 2) - (jj - 2) : 3;
+            const int nkmax = (m - 6 < kk - 2) ? m - 6 : kk - 2;
+            for (int k = nkmin; k <= nkmax; ++k) {
+                const int nimin = (m - (jj - 2) - k > 3) ? m - (jj - 2) - k : 3;
+                const int nimax = (m - k - 3 < ii - 2) ? m - k - 3 : ii - 2;
+                for (int i = nimin; i <= nimax; ++i) {
+                    const int j = m - k - i;
+                    if (i < 3 || i > ii - 2 || j < 3 || j > jj - 2 ||
+                        k < 3 || k > kk - 2)
+                        return -1;
+                    ++ic1;
+                    idx[lm + ic1] = (int64_t)k + (int64_t)(j - 1) * kk
+                                    + (int64_t)(i - 1) * kk * jj;
+                }
+            }
+            mip[m + 1] = mip[m] + ic1;
+        }
+        /* sip_hyperplane_sort_grid: ascending inside each plane */
+        for (int m = n3dmin; m <= n3dmax; ++m) {
+            const int64_t len = mip[m + 1] - mip[m];
+            qsort(&idx[mip[m] + 1], (size_t)len, sizeof(int64_t), cmp_i64);
+            for (int64_t s = mip[m] + 1; s < mip[m + 1]; ++s)
+                if (idx[s + 1] <= idx[s]) return -2;
+        }
+        /* sip_hyperplane_check_grid */
+        int64_t sum = 0;
+        char *seen = calloc((size_t)ii * jj * kk, 1);
+        for (int m = n3dmin; m <= n3dmax; ++m)
+            for (int64_t s = mip[m] + 1; s <= mip[m + 1]; ++s) {
+                const int64_t l = idx[s] - 1;
+                const int k = (int)(l % kk) + 1;
+                const int j = (int)((l / kk) % jj) + 1;
+                const int i = (int)(l / ((int64_t)kk * jj)) + 1;
+                if (k + j + i != m) { free(seen); return -3; }
+                if (seen[l]++) { free(seen); return -4; }
+                ++sum;
+            }
+        free(seen);
+        if (sum != (int64_t)(kk - 4) * (jj - 4) * (ii - 4)) return -5;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* src/flow/sip_hyperplane_mod.F90:106-257 */
+void og_lu_hp(const og_topo *t, const double *aw_, const double *ae_,
+              const double *as_, const double *an_, const double *ab_,
+              const double *at_, const double *ap_, const double *bp_,
+              const int64_t *mip_, const int64_t *idx_, double *lw_,
+              double *ls_, double *lb_, double *lpr_, double *ue_,
+              double *un_, double *ut_)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        GDIMS(t, g);
+        const int64_t n = (int64_t)kk * jj * ii;
+        const int64_t o3 = t->ip3[g - 1];
+        const double *aw = aw_ + t->ip1x[g - 1] - 1, *ae = ae_ + t->ip1x[g - 1] - 1;
+        const double *as = as_ + t->ip1y[g - 1] - 1, *an = an_ + t->ip1y[g - 1] - 1;
+        const double *ab = ab_ + t->ip1z[g - 1] - 1, *at = at_ + t->ip1z[g - 1] - 1;
+        const double *ap = ap_ + o3, *bp = bp_ + o3;
+        const int64_t *mip = mip_ + o3 - 1, *idxs = idx_ + o3 - 1;
+        double *lw = lw_ + o3 - 1, *ls = ls_ + o3 - 1, *lb = lb_ + o3 - 1,
+               *lpr = lpr_ + o3 - 1, *ue = ue_ + o3 - 1, *un = un_ + o3 - 1,
+               *ut = ut_ + o3 - 1;
+        /* tmp arrays, 1-based, zero-initialised (:137-143) */
+        double *tw = calloc((size_t)n + 1, sizeof(double));
+        double *ts = calloc((size_t)n + 1, sizeof(double));
+        double *tb = calloc((size_t)n + 1, sizeof(double));
+        double *tp = calloc((size_t)n + 1, sizeof(double));
+        double *te = calloc((size_t)n + 1, sizeof(double));
+        double *tn = calloc((size_t)n + 1, sizeof(double));
+        double *tt = calloc((size_t)n + 1, sizeof(double));
+        const int n3dmin = 9, n3dmax = (ii - 2) + (jj - 2) + (kk - 2);
+        for (int m = n3dmin; m <= n3dmax; ++m) {
+            const int64_t lm = mip[m], lp = mip[m + 1] - lm;
+            for (int64_t ip = 1; ip <= lp; ++ip) {
+                const int64_t iacc = lm + ip;
+                const int64_t id = idxs[iacc];
+                const int64_t km = id - 1, jm = id - kk,
+                              im = id - (int64_t)kk * jj;
+                const int64_t l = id - 1;
+                const int k = (int)(l % kk) + 1;
+                const int j = (int)((l / kk) % jj) + 1;
+                const int i = (int)(l / ((int64_t)kk * jj)) + 1;
+                const double bu_m = bp[IX(k, j, i - 1)] * bp[IX(k, j, i)];
+                const double bv_m = bp[IX(k, j - 1, i)] * bp[IX(k, j, i)];
+                const double bw_m = bp[IX(k - 1, j, i)] * bp[IX(k, j, i)];
+                const double bu_c = bp[IX(k, j, i)] * bp[IX(k, j, i + 1)];
+                const double bv_c = bp[IX(k, j, i)] * bp[IX(k, j + 1, i)];
+                const double bw_c = bp[IX(k, j, i)] * bp[IX(k + 1, j, i)];
+                tw[id] = aw[i] * bu_m / (1.0 + ALFA * (tn[im] + tt[im]));
+                ts[id] = as[j] * bv_m / (1.0 + ALFA * (te[jm] + tt[jm]));
+                tb[id] = ab[k] * bw_m / (1.0 + ALFA * (tn[km] + te[km]));
+                const double p1 = ALFA * (tb[id] * te[km] + ts[id] * te[jm]);
+                const double p2 = ALFA * (tb[id] * tn[km] + tw[id] * tn[im]);
+                const double p3 = ALFA * (tw[id] * tt[im] + ts[id] * tt[jm]);
+                tp[id] = 1.0 / (ap[IX(k, j, i)] + p1 + p2 + p3
+                                - tb[id] * tt[km] - tw[id] * te[im]
+                                - ts[id] * tn[jm] + TINY);
+                te[id] = (ae[i] * bu_c - p1) * tp[id];
+                tn[id] = (an[j] * bv_c - p2) * tp[id];
+                tt[id] = (at[k] * bw_c - p3) * tp[id];
+                tw[id] = tw[id] * tp[id];
+                ts[id] = ts[id] * tp[id];
+                tb[id] = tb[id] * tp[id];
+                lw[iacc] = tw[id]; ls[iacc] = ts[id]; lb[iacc] = tb[id];
+                lpr[iacc] = tp[id]; ue[iacc] = te[id]; un[iacc] = tn[id];
+                ut[iacc] = tt[id];
+            }
+        }
+        free(tw); free(ts); free(tb); free(tp); free(te); free(tn); free(tt);
+    }
+}
+
+/* src/flow/sip_classic_mod.F90:17-127 (factor fields start zeroed) */
+void og_lu_classic(const og_topo *t, const double *aw_, const double *ae_,
+                   const double *as_, const double *an_, const double *ab_,
+                   const double *at_, const double *ap_, const double *bp_,
+                   double *lw_, double *ls_, double *lb_, double *lpr_,
+                   double *ue_, double *un_, double *ut_)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        GDIMS(t, g);
+        const int64_t o3 = t->ip3[g - 1];
+        const double *aw = aw_ + t->ip1x[g - 1] - 1, *ae = ae_ + t->ip1x[g - 1] - 1;
+        const double *as = as_ + t->ip1y[g - 1] - 1, *an = an_ + t->ip1y[g - 1] - 1;
+        const double *ab = ab_ + t->ip1z[g - 1] - 1, *at = at_ + t->ip1z[g - 1] - 1;
+        const double *ap = ap_ + o3, *bp = bp_ + o3;
+        double *lw = lw_ + o3, *ls = ls_ + o3, *lb = lb_ + o3,
+               *lpr = lpr_ + o3, *ue = ue_ + o3, *un = un_ + o3,
+               *ut = ut_ + o3;
+        for (int i = 3; i <= ii - 2; ++i)
+            for (int j = 3; j <= jj - 2; ++j)
+                for (int k = 3; k <= kk - 2; ++k) {
+                    const int64_t c = IX(k, j, i), im = IX(k, j, i - 1),
+                                  jm = IX(k, j - 1, i), km = IX(k - 1, j, i);
+                    const double bu_m = bp[im] * bp[c];
+                    const double bv_m = bp[jm] * bp[c];
+                    const double bw_m = bp[km] * bp[c];
+                    lw[c] = aw[i] * bu_m / (1.0 + ALFA * (un[im] + ut[im]));
+                    ls[c] = as[j] * bv_m / (1.0 + ALFA * (ue[jm] + ut[jm]));
+                    lb[c] = ab[k] * bw_m / (1.0 + ALFA * (un[km] + ue[km]));
+                    const double p1 = ALFA * (lb[c] * ue[km] + ls[c] * ue[jm]);
+                    const double p2 = ALFA * (lb[c] * un[km] + lw[c] * un[im]);
+                    const double p3 = ALFA * (lw[c] * ut[im] + ls[c] * ut[jm]);
+                    lpr[c] = 1.0 / (ap[c] + p1 + p2 + p3 - lb[c] * ut[km]
+                                    - lw[c] * ue[im] - ls[c] * un[jm] + TINY);
+                    ue[c] = (ae[i] * (bp[c] * bp[IX(k, j, i + 1)]) - p1) * lpr[c];
+                    un[c] = (an[j] * (bp[c] * bp[IX(k, j + 1, i)]) - p2) * lpr[c];
+                    ut[c] = (at[k] * (bp[c] * bp[IX(k + 1, j, i)]) - p3) * lpr[c];
+                }
+        for (int i = 3; i <= ii - 2; ++i)
+            for (int j = 3; j <= jj - 2; ++j) {
+                for (int k = 3; k <= kk - 2; ++k) lw[IX(k, j, i)] *= lpr[IX(k, j, i)];
+                for (int k = 3; k <= kk - 2; ++k) ls[IX(k, j, i)] *= lpr[IX(k, j, i)];
+                for (int k = 3; k <= kk - 2; ++k) lb[IX(k, j, i)] *= lpr[IX(k, j, i)];
+            }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/flow/laplacephi_mod.F90:134-165 */
+static void laplacephi_grid(int kk, int jj, int ii, double *res,
+                            const double *phi, const double *aw,
+                            const double *ae, const double *an,
+                            const double *as, const double *at,
+                            const double *ab, const double *ap,
+                            const double *bp)
+{
+    for (int i = 3; i <= ii - 2; ++i)
+        for (int j = 3; j <= jj - 2; ++j)
+            for (int k = 3; k <= kk - 2; ++k) {
+                const int64_t c = IX(k, j, i);
+                double r = -(aw[i] * phi[IX(k, j, i - 1)] * bp[IX(k, j, i - 1)] * bp[c]);
+                r = r - ae[i] * phi[IX(k, j, i + 1)] * bp[c] * bp[IX(k, j, i + 1)];
+                r = r - as[j] * phi[IX(k, j - 1, i)] * bp[IX(k, j - 1, i)] * bp[c];
+                r = r - an[j] * phi[IX(k, j + 1, i)] * bp[c] * bp[IX(k, j + 1, i)];
+                r = r - ab[k] * phi[IX(k - 1, j, i)] * bp[IX(k - 1, j, i)] * bp[c];
+                r = r - at[k] * phi[IX(k + 1, j, i)] * bp[c] * bp[IX(k + 1, j, i)];
+                r = r - ap[c] * phi[c];
+                res[c] = r;
+            }
+}
+
+void og_laplacephi(const og_topo *t, int level, double *res, const double *phi,
+                   const double *aw, const double *ae, const double *as,
+                   const double *an, const double *ab, const double *at,
+                   const double *ap, const double *bp)
+{
+#pragma omp parallel for schedule(dynamic)
+    for (int g = 1; g <= t->ngrid; ++g) {
+        if (level >= 0 && t->level[g - 1] != level) continue;
+        GDIMS(t, g);
+        const int64_t o3 = t->ip3[g - 1];
+        laplacephi_grid(kk, jj, ii, res + o3, phi + o3,
+                        aw + t->ip1x[g - 1] - 1, ae + t->ip1x[g - 1] - 1,
+                        an + t->ip1y[g - 1] - 1, as + t->ip1y[g - 1] - 1,
+                        at + t->ip1z[g - 1] - 1, ab + t->ip1z[g - 1] - 1,
+                        ap + o3, bp + o3);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/flow/sip_hyperplane_mod.F90:450-505 */
+void og_sipiter1_hp(const og_topo *t, int level, const double *rhs_,
+                    double *res_, const double *lw_, const double *ls_,
+                    const double *lb_, const double *lpr_, const int64_t *mip_,
+                    const int64_t *idx_)
+{
+#pragma omp parallel for schedule(dynamic)
+    for (int g = 1; g <= t->ngrid; ++g) {
+        if (level >= 0 && t->level[g - 1] != level) continue;
+        GDIMS(t, g);
+        const int64_t o = t->ip3[g - 1] - 1; /* 1-based views */
+        const double *rhs = rhs_ + o, *lw = lw_ + o, *ls = ls_ + o,
+                     *lb = lb_ + o, *lpr = lpr_ + o;
+        double *res = res_ + o;
+        const int64_t *mip = mip_ + o, *idxs = idx_ + o;
+        const int n3dmin = 9, n3dmax = (ii - 2) + (jj - 2) + (kk - 2);
+        for (int m = n3dmin; m <= n3dmax; ++m) {
+            const int64_t lm = mip[m], lp = mip[m + 1] - lm;
+            for (int64_t ip = 1; ip <= lp; ++ip) {
+                const int64_t iacc = lm + ip;
+                const int64_t id = idxs[iacc];
+                double r = (rhs[id] + res[id]) * lpr[iacc];
+                r = r - lb[iacc] * res[id - 1] - ls[iacc] * res[id - kk]
+                      - lw[iacc] * res[id - (int64_t)kk * jj];
+                res[id] = r;
+            }
+        }
+    }
+}
+
+/* src/flow/sip_hyperplane_mod.F90:508-569 */
+void og_sipiter2_hp(const og_topo *t, int level, double *phi_, double *res_,
+                    const double *ue_, const double *un_, const double *ut_,
+                    const int64_t *mip_, const int64_t *idx_)
+{
+#pragma omp parallel for schedule(dynamic)
+    for (int g = 1; g <= t->ngrid; ++g) {
+        if (level >= 0 && t->level[g - 1] != level) continue;
+        GDIMS(t, g);
+        const int64_t o = t->ip3[g - 1] - 1;
+        const double *ue = ue_ + o, *un = un_ + o, *ut = ut_ + o;
+        double *res = res_ + o, *phi = phi_ + o;
+        const int64_t *mip = mip_ + o, *idxs = idx_ + o;
+        const int n3dmin = 9, n3dmax = (ii - 2) + (jj - 2) + (kk - 2);
+        for (int m = n3dmax; m >= n3dmin; --m) {
+            const int64_t lm = mip[m], lp = mip[m + 1] - lm;
+            for (int64_t ip = 1; ip <= lp; ++ip) {
+                const int64_t iacc = lm + ip;
+                const int64_t id = idxs[iacc];
+                double r = res[id];
+                r = r - ut[iacc] * res[id + 1] - un[iacc] * res[id + kk]
+                      - ue[iacc] * res[id + (int64_t)kk * jj];
+                res[id] = r;
+            }
+        }
+        for (int i = 3; i <= ii - 2; ++i)
+            for (int j = 3; j <= jj - 2; ++j)
+                for (int k = 3; k <= kk - 2; ++k) {
+                    const int64_t id = (int64_t)k + (int64_t)(j - 1) * kk
+                                       + (int64_t)(i - 1) * kk * jj;
+                    phi[id] = phi[id] + res[id];
+                }
+    }
+}
+
+/* src/flow/sip_classic_mod.F90:143-166 */
+void og_sipiter1_cl(const og_topo *t, int level, const double *rhs_,
+                    double *res_, const double *lw_, const double *ls_,
+                    const double *lb_, const double *lpr_)
+{
+#pragma omp parallel for schedule(dynamic)
+    for (int g = 1; g <= t->ngrid; ++g) {
+        if (level >= 0 && t->level[g - 1] != level) continue;
+        GDIMS(t, g);
+        const int64_t o = t->ip3[g - 1];
+        const double *rhs = rhs_ + o, *lw = lw_ + o, *ls = ls_ + o,
+                     *lb = lb_ + o, *lpr = lpr_ + o;
+        double *res = res_ + o;
+        for (int i = 3; i <= ii - 2; ++i)
+            for (int j = 3; j <= jj - 2; ++j) {
+                for (int k = 3; k <= kk - 2; ++k) {
+                    const int64_t c = IX(k, j, i);
+                    double r = (rhs[c] + res[c]) * lpr[c];
+                    r = r - lw[c] * res[IX(k, j, i - 1)]
+                          - ls[c] * res[IX(k, j - 1, i)];
+                    res[c] = r;
+                }
+                for (int k = 3; k <= kk - 2; ++k) {
+                    const int64_t c = IX(k, j, i);
+                    res[c] = res[c] - lb[c] * res[IX(k - 1, j, i)];
+                }
+            }
+    }
+}
+
+/* src/flow/sip_classic_mod.F90:169-198 */
+void og_sipiter2_cl(const og_topo *t, int level, double *phi_, double *res_,
+                    const double *ue_, const double *un_, const double *ut_)
+{
+#pragma omp parallel for schedule(dynamic)
+    for (int g = 1; g <= t->ngrid; ++g) {
+        if (level >= 0 && t->level[g - 1] != level) continue;
+        GDIMS(t, g);
+        const int64_t o = t->ip3[g - 1];
+        const double *ue = ue_ + o, *un = un_ + o, *ut = ut_ + o;
+        double *res = res_ + o, *phi = phi_ + o;
+        for (int i = ii - 2; i >= 3; --i)
+            for (int j = jj - 2; j >= 3; --j) {
+                for (int k = 3; k <= kk - 2; ++k) {
+                    const int64_t c = IX(k, j, i);
+                    res[c] = res[c] - un[c] * res[IX(k, j + 1, i)]
+                                    - ue[c] * res[IX(k, j, i + 1)];
+                }
+                for (int k = kk - 2; k >= 3; --k) {
+                    const int64_t c = IX(k, j, i);
+                    res[c] = res[c] - ut[c] * res[IX(k + 1, j, i)];
+                }
+            }
+        for (int i = 3; i <= ii - 2; ++i)
+            for (int j = 3; j <= jj - 2; ++j)
+                for (int k = 3; k <= kk - 2; ++k) {
+                    const int64_t c = IX(k, j, i);
+                    phi[c] = phi[c] + res[c];
+                }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/flow/sor_mod.F90:18-49 (divide0: src/core/simdfunctions_mod.F90:70-82) */
+void og_sor_init(const og_topo *t, const double *ap_, double *rap_)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        GDIMS(t, g);
+        const double *ap = ap_ + t->ip3[g - 1];
+        double *rap = rap_ + t->ip3[g - 1];
+        for (int i = 3; i <= ii - 2; ++i)
+            for (int j = 3; j <= jj - 2; ++j)
+                for (int k = 3; k <= kk - 2; ++k) {
+                    const double b = ap[IX(k, j, i)];
+                    rap[IX(k, j, i)] = (b == 0.0) ? 0.0 : 1.0 / b;
+                }
+    }
+}
+
+/* src/flow/sor_mod.F90:102-203 */
+void og_relax(const og_topo *t, int level, double *dp_, const double *rhs_,
+              const double *aw_, const double *ae_, const double *as_,
+              const double *an_, const double *ab_, const double *at_,
+              const double *rap_, const double *bp_, double omg)
+{
+#pragma omp parallel for schedule(dynamic)
+    for (int g = 1; g <= t->ngrid; ++g) {
+        if (level >= 0 && t->level[g - 1] != level) continue;
+        GDIMS(t, g);
+        const int64_t o3 = t->ip3[g - 1];
+        double *dp = dp_ + o3;
+        const double *rhs = rhs_ + o3, *rap = rap_ + o3;
+        const double *bp = bp_ ? bp_ + o3 : NULL;
+        const double *gw = aw_ + t->ip1x[g - 1] - 1, *ge = ae_ + t->ip1x[g - 1] - 1;
+        const double *gs = as_ + t->ip1y[g - 1] - 1, *gn = an_ + t->ip1y[g - 1] - 1;
+        const double *gb = ab_ + t->ip1z[g - 1] - 1, *gt = at_ + t->ip1z[g - 1] - 1;
+        for (int rb = 0; rb <= 1; ++rb)
+            for (int i = 3; i <= ii - 2; ++i)
+                for (int j = 3; j <= jj - 2; ++j) {
+                    int kstart, kstop;
+                    if ((i + j) % 2 == rb) { kstart = 4; kstop = kk - 2; }
+                    else { kstart = 3; kstop = kk - 3; }
+                    for (int k = kstart; k <= kstop; k += 2) {
+                        const int64_t c = IX(k, j, i);
+                        double res;
+                        if (bp) {
+                            const double aw = bp[IX(k, j, i - 1)] * bp[c] * gw[i];
+                            const double ae = bp[c] * bp[IX(k, j, i + 1)] * ge[i];
+                            const double as = bp[IX(k, j - 1, i)] * bp[c] * gs[j];
+                            const double an = bp[c] * bp[IX(k, j + 1, i)] * gn[j];
+                            const double ab = bp[IX(k - 1, j, i)] * bp[c] * gb[k];
+                            const double at = bp[c] * bp[IX(k + 1, j, i)] * gt[k];
+                            res = (aw * dp[IX(k, j, i - 1)] + ae * dp[IX(k, j, i + 1)]
+                                   + as * dp[IX(k, j - 1, i)] + an * dp[IX(k, j + 1, i)]
+                                   + ab * dp[IX(k - 1, j, i)] + at * dp[IX(k + 1, j, i)]
+                                   - rhs[c]) * rap[c];
+                        } else {
+                            res = (gw[i] * dp[IX(k, j, i - 1)] + ge[i] * dp[IX(k, j, i + 1)]
+                                   + gs[j] * dp[IX(k, j - 1, i)] + gn[j] * dp[IX(k, j + 1, i)]
+                                   + gb[k] * dp[IX(k - 1, j, i)] + gt[k] * dp[IX(k + 1, j, i)]
+                                   - rhs[c]) * rap[c];
+                        }
+                        dp[c] = (1.0 - omg) * dp[c] - omg * res;
+                    }
+                }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/ib/noib_mod.F90:304-354 */
+void og_divcal(const og_topo *t, double *div_, const double *u_,
+               const double *v_, const double *w_, const double *rddx_,
+               const double *rddy_, const double *rddz_, double fak,
+               const double *bp_, const double *sdiv_)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        GDIMS(t, g);
+        const int64_t o3 = t->ip3[g - 1];
+        double *div = div_ + o3;
+        const double *u = u_ + o3, *v = v_ + o3, *w = w_ + o3;
+        const double *rddx = rddx_ + t->ip1x[g - 1] - 1;
+        const double *rddy = rddy_ + t->ip1y[g - 1] - 1;
+        const double *rddz = rddz_ + t->ip1z[g - 1] - 1;
+        for (int i = 3; i <= ii - 2; ++i)
+            for (int j = 3; j <= jj - 2; ++j)
+                for (int k = 3; k <= kk - 2; ++k) {
+                    const int64_t c = IX(k, j, i);
+                    div[c] = fak * ((u[c] - u[IX(k, j, i - 1)]) * rddx[i]
+                                    + (v[c] - v[IX(k, j - 1, i)]) * rddy[j]
+                                    + (w[c] - w[IX(k - 1, j, i)]) * rddz[k]);
+                }
+        if (sdiv_) {
+            const double *sdiv = sdiv_ + o3;
+            for (int i = 3; i <= ii - 2; ++i)
+                for (int j = 3; j <= jj - 2; ++j)
+                    for (int k = 3; k <= kk - 2; ++k)
+                        div[IX(k, j, i)] = div[IX(k, j, i)] + fak * sdiv[IX(k, j, i)];
+        }
+        if (bp_) {
+            const double *bp = bp_ + o3;
+            for (int64_t c = 0; c < (int64_t)ii * jj * kk; ++c)
+                div[c] = bp[c] * div[c];
+        }
+    }
+}
+
+/* src/flow/pressuresolver_mod.F90:787-879 */
+void og_mgpcorr(const og_topo *t, double *u_, double *v_, double *w_,
+                double *p_, const double *dp_, const double *rdx_,
+                const double *rdy_, const double *rdz_, double rfak,
+                const double *bp_)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        GDIMS(t, g);
+        const int64_t o3 = t->ip3[g - 1];
+        double *u = u_ + o3, *v = v_ + o3, *w = w_ + o3, *p = p_ + o3;
+        const double *dp = dp_ + o3;
+        const double *rdx = rdx_ + t->ip1x[g - 1] - 1;
+        const double *rdy = rdy_ + t->ip1y[g - 1] - 1;
+        const double *rdz = rdz_ + t->ip1z[g - 1] - 1;
+        if (bp_) {
+            const double *bp = bp_ + o3;
+            for (int i = 2; i <= ii - 1; ++i)
+                for (int j = 2; j <= jj - 1; ++j)
+                    for (int k = 2; k <= kk - 1; ++k)
+                        p[IX(k, j, i)] = p[IX(k, j, i)] + dp[IX(k, j, i)] * bp[IX(k, j, i)];
+            for (int i = 2; i <= ii - 2; ++i)
+                for (int j = 3; j <= jj - 2; ++j)
+                    for (int k = 3; k <= kk - 2; ++k)
+                        u[IX(k, j, i)] = u[IX(k, j, i)]
+                            + (dp[IX(k, j, i)] - dp[IX(k, j, i + 1)])
+                              * bp[IX(k, j, i)] * bp[IX(k, j, i + 1)] * rdx[i] * rfak;
+            for (int i = 3; i <= ii - 2; ++i)
+                for (int j = 2; j <= jj - 2; ++j)
+                    for (int k = 3; k <= kk - 2; ++k)
+                        v[IX(k, j, i)] = v[IX(k, j, i)]
+                            + (dp[IX(k, j, i)] - dp[IX(k, j + 1, i)])
+                              * bp[IX(k, j, i)] * bp[IX(k, j + 1, i)] * rdy[j] * rfak;
+            for (int i = 3; i <= ii - 2; ++i)
+                for (int j = 3; j <= jj - 2; ++j)
+                    for (int k = 2; k <= kk - 2; ++k)
+                        w[IX(k, j, i)] = w[IX(k, j, i)]
+                            + (dp[IX(k, j, i)] - dp[IX(k + 1, j, i)])
+                              * bp[IX(k, j, i)] * bp[IX(k + 1, j, i)] * rdz[k] * rfak;
+        } else {
+            for (int i = 2; i <= ii - 1; ++i)
+                for (int j = 2; j <= jj - 1; ++j)
+                    for (int k = 2; k <= kk - 1; ++k)
+                        p[IX(k, j, i)] = p[IX(k, j, i)] + dp[IX(k, j, i)];
+            for (int i = 2; i <= ii - 2; ++i)
+                for (int j = 3; j <= jj - 2; ++j)
+                    for (int k = 3; k <= kk - 2; ++k)
+                        u[IX(k, j, i)] = u[IX(k, j, i)]
+                            + (dp[IX(k, j, i)] - dp[IX(k, j, i + 1)]) * rdx[i] * rfak;
+            for (int i = 3; i <= ii - 2; ++i)
+                for (int j = 2; j <= jj - 2; ++j)
+                    for (int k = 3; k <= kk - 2; ++k)
+                        v[IX(k, j, i)] = v[IX(k, j, i)]
+                            + (dp[IX(k, j, i)] - dp[IX(k, j + 1, i)]) * rdy[j] * rfak;
+            for (int i = 3; i <= ii - 2; ++i)
+                for (int j = 3; j <= jj - 2; ++j)
+                    for (int k = 2; k <= kk - 2; ++k)
+                        w[IX(k, j, i)] = w[IX(k, j, i)]
+                            + (dp[IX(k, j, i)] - dp[IX(k + 1, j, i)]) * rdz[k] * rfak;
+        }
+    }
+}
+
+/* src/flow/pressuresolver_mod.F90:623-689 */
+double og_maxabscal(const og_topo *t, const double *phi_, double *maxlvl)
+{
+    double maxabs = 0.0;
+    if (maxlvl)
+        for (int l = t->minlevel; l <= t->maxlevel; ++l)
+            maxlvl[l - t->minlevel] = 0.0;
+    for (int g = 1; g <= t->ngrid; ++g) {
+        GDIMS(t, g);
+        const double *phi = phi_ + t->ip3[g - 1];
+        double m = 0.0;
+        for (int i = 3; i <= ii - 2; ++i)
+            for (int j = 3; j <= jj - 2; ++j)
+                for (int k = 3; k <= kk - 2; ++k) {
+                    const double a = fabs(phi[IX(k, j, i)]);
+                    m = (a > m) ? a : m;
+                }
+        const double s = ldexp(1.0, t->maxlevel - t->level[g - 1]);
+        const double ms = fabs(m * s);
+        maxabs = (ms > maxabs) ? ms : maxabs;
+        if (maxlvl) {
+            double *ml = &maxlvl[t->level[g - 1] - t->minlevel];
+            *ml = (m > *ml) ? m : *ml;
+        }
+    }
+    return maxabs;
+}
+
+/* src/flow/pressuresolver_mod.F90:722-742 */
+void og_rescal(const og_topo *t, double *rhs_, const double *res_)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        GDIMS(t, g);
+        double *rhs = rhs_ + t->ip3[g - 1];
+        const double *res = res_ + t->ip3[g - 1];
+        for (int i = 3; i <= ii - 2; ++i)
+            for (int j = 3; j <= jj - 2; ++j)
+                for (int k = 3; k <= kk - 2; ++k)
+                    rhs[IX(k, j, i)] = rhs[IX(k, j, i)] + res[IX(k, j, i)];
+    }
+}
+
+/* src/flow/pressuresolver_mod.F90:882-908 */
+void og_accumulate(int64_t n, double *dp, const double *hilf)
+{
+    for (int64_t c = 0; c < n; ++c) dp[c] = dp[c] + hilf[c];
+}
+
+/* ------------------------------------------------------------------ */
+/* src/core/connect/connect_core_mod.F90:373-433 */
+static void get_nbrs(const og_topo *t, int iface, const int32_t *nbrs,
+                     int *nbrgrid, int *nbrface)
+{
+    *nbrgrid = 0;
+    *nbrface = 0;
+    for (int r = 0; r < 7; ++r) {
+        const int dir = RESCUE_DIR[iface - 1][r];
+        if (dir == 0) break;
+        if (nbrs[dir - 1] > 0) {
+            *nbrgrid = nbrs[dir - 1];
+            *nbrface = RESCUE_NBR[iface - 1][r];
+            if (*nbrface > 18) {
+                const int32_t *bcn = t->bc + 6 * (*nbrgrid - 1);
+                const int f1 = FACELIST[*nbrface - 1][1];
+                const int f2 = FACELIST[*nbrface - 1][2];
+                const int f3 = FACELIST[*nbrface - 1][3];
+                if (!is_con(bcn[f1 - 1]) && !is_con(bcn[f2 - 1]) &&
+                    !is_con(bcn[f3 - 1])) {
+                    *nbrgrid = 0;
+                    *nbrface = 0;
+                    continue;
+                }
+            }
+            break;
+        }
+    }
+}
+
+/* src/core/connect/connect_core_mod.F90:167-370 (one rank: the sort by
+ * sending rank is a stable no-op, sendconns == recvconns). */
+int og_connect_build(const og_topo *t, og_conn *out, int cap)
+{
+    int n = 0;
+    for (int g = 1; g <= t->ngrid; ++g) {
+        const int32_t *nb = t->nbr + 26 * (g - 1);
+        const int32_t *bc = t->bc + 6 * (g - 1);
+        for (int iface = 1; iface <= 26; ++iface) {
+            int want;
+            if (iface <= 6) {
+                want = is_con(bc[iface - 1]);
+            } else if (iface <= 18) {
+                want = is_con(bc[FACELIST[iface - 1][1] - 1]) ||
+                       is_con(bc[FACELIST[iface - 1][2] - 1]);
+            } else {
+                want = is_con(bc[FACELIST[iface - 1][1] - 1]) ||
+                       is_con(bc[FACELIST[iface - 1][2] - 1]) ||
+                       is_con(bc[FACELIST[iface - 1][3] - 1]);
+            }
+            if (!want) continue;
+            int ng, nf;
+            get_nbrs(t, iface, nb, &ng, &nf);
+            if (ng == 0) continue;
+            if (n >= cap) return -1;
+            out[n].recv_grid = g;
+            out[n].send_grid = ng;
+            out[n].recv_face = iface;
+            out[n].send_face = nf;
+            ++n;
+        }
+    }
+    return n;
+}
+
+/* src/core/connect/connect_core_mod.F90:475-548 */
+static void start_and_stop_face(const og_topo *t, int g, int iface, int *is,
+                                int *ie, int *js, int *je, int *ks, int *ke,
+                                int nplane, int nghost)
+{
+    GDIMS(t, g);
+    switch (iface) {
+    case 1: *is = 3 - nplane; *ie = 2; break;
+    case 2: *is = ii - 1; *ie = ii - (2 - nplane); break;
+    case 3: *js = 3 - nplane; *je = 2; break;
+    case 4: *js = jj - 1; *je = jj - (2 - nplane); break;
+    case 5: *ks = 3 - nplane; *ke = 2; break;
+    case 6: *ks = kk - 1; *ke = kk - (2 - nplane); break;
+    }
+    if (nghost) {
+        switch (iface) {
+        case 1: *is += nplane; *ie += nplane; break;
+        case 2: *is -= nplane; *ie -= nplane; break;
+        case 3: *js += nplane; *je += nplane; break;
+        case 4: *js -= nplane; *je -= nplane; break;
+        case 5: *ks += nplane; *ke += nplane; break;
+        case 6: *ks -= nplane; *ke -= nplane; break;
+        }
+    }
+}
+
+/* src/core/connect/connect_core_mod.F90:454-472 */
+static void start_and_stop(const og_topo *t, int g, int iface, int *is,
+                           int *ie, int *js, int *je, int *ks, int *ke,
+                           int nplane, int nghost)
+{
+    GDIMS(t, g);
+    *is = 3; *ie = ii - 2; *js = 3; *je = jj - 2; *ks = 3; *ke = kk - 2;
+    for (int f = 1; f <= FACELIST[iface - 1][0]; ++f)
+        start_and_stop_face(t, g, FACELIST[iface - 1][f], is, ie, js, je, ks,
+                            ke, nplane, nghost);
+}
+
+/* src/core/connect/connect_core_mod.F90:575-602 */
+static void corr_start_stop(const og_topo *t, int g, int fsend, int frecv,
+                            int *is, int *ie, int *js, int *je, int *ks,
+                            int *ke, int nplane)
+{
+    if (frecv <= 6) return;
+    for (int a = 1; a <= FACELIST[fsend - 1][0]; ++a)
+        for (int b = 1; b <= FACELIST[frecv - 1][0]; ++b)
+            if (FACELIST[fsend - 1][a] == FACELIST[frecv - 1][b])
+                start_and_stop_face(t, g, FACELIST[fsend - 1][a], is, ie, js,
+                                    je, ks, ke, nplane, 0);
+}
+
+/* src/core/connect/connect_core_mod.F90:640-682 */
+static int decide(const og_topo *t, const og_conn *c, int vertices, int fwd,
+                  int minlvl, int maxlvl)
+{
+    int ex = 1;
+    const int fr = c->recv_face;
+    const int lvl = t->level[c->recv_grid - 1];
+    if (lvl > maxlvl || lvl < minlvl) ex = 0;
+    if (!vertices && fr > 6) ex = 0;
+    if (fwd == 1 && (fr == 2 || fr == 4 || fr == 6)) ex = 0;
+    else if (fwd == -1 && (fr == 1 || fr == 3 || fr == 5)) ex = 0;
+    return ex;
+}
+
+/* src/core/connect/conn1_mod.F90:31-129 (one rank), connect_self :561-704 */
+int og_exchange(const og_topo *t, const og_conn *conns, int nconn, int level,
+            int layers, int corners, int forward, double **fields, int nfield)
+{
+    if (layers < 1 || layers > 2) return -1;
+    if (forward != 0 && corners) return -2;
+    const int minl = level >= 0 ? level : t->minlevel;
+    const int maxl = level >= 0 ? level : t->maxlevel;
+    for (int s = 0; s < nconn; ++s) {
+        const og_conn *c = &conns[s];
+        if (!decide(t, c, corners, forward, minl, maxl)) continue;
+        const int gd = c->recv_grid, gs = c->send_grid;
+        int is, ie, js, je, ks, ke, isd, ied, jsd, jed, ksd, ked;
+        start_and_stop(t, gs, FACENBR[c->recv_face - 1], &is, &ie, &js, &je,
+                       &ks, &ke, layers, 1);
+        corr_start_stop(t, gs, c->send_face, c->recv_face, &is, &ie, &js, &je,
+                        &ks, &ke, layers);
+        start_and_stop(t, gd, c->recv_face, &isd, &ied, &jsd, &jed, &ksd,
+                       &ked, layers, 0);
+        if ((ie - is + 1) * (je - js + 1) * (ke - ks + 1) !=
+            (ied - isd + 1) * (jed - jsd + 1) * (ked - ksd + 1))
+            return -3;
+        const int koff = ks - ksd, joff = js - jsd, ioff = is - isd;
+        const int kks = t->kk[gs - 1], jjs = t->jj[gs - 1];
+        const int kkd = t->kk[gd - 1], jjd = t->jj[gd - 1];
+        for (int f = 0; f < nfield; ++f) {
+            const double *src = fields[f] + t->ip3[gs - 1];
+            double *dst = fields[f] + t->ip3[gd - 1];
+            for (int i = isd; i <= ied; ++i)
+                for (int j = jsd; j <= jed; ++j)
+                    for (int k = ksd; k <= ked; ++k)
+                        dst[(k - 1) + (int64_t)(j - 1) * kkd +
+                            (int64_t)(i - 1) * kkd * jjd] =
+                            src[(k + koff - 1) + (int64_t)(j + joff - 1) * kks +
+                                (int64_t)(i + ioff - 1) * kks * jjs];
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+static int64_t field_len(const og_topo *t)
+{
+    const int g = t->ngrid;
+    return t->ip3[g - 1] + (int64_t)t->kk[g - 1] * t->jj[g - 1] * t->ii[g - 1];
+}
+
+/* src/flow/pressuresolver_mod.F90:423-465 (sip) and :361-420 (mgpoisit) */
+static void mgpoisit(const og_topo *t, const og_conn *c, int nconn,
+                     const og_params *prm, int ilevel, double *dp,
+                     const double *rhs, double *res, const double *bp,
+                     const double *aw, const double *ae, const double *as,
+                     const double *an, const double *ab, const double *at,
+                     const double *ap, const double *lw, const double *ls,
+                     const double *lb, const double *lpr, const double *ue,
+                     const double *un, const double *ut, const int64_t *mip,
+                     const int64_t *idx, const double *rap)
+{
+    for (int iloop = 1; iloop <= prm->ninner; ++iloop) {
+        /* bound_pressure: no PAR faces on the supported topologies */
+        const int use_sor = prm->ityp == 1 &&
+                            (ilevel > t->minlevel || prm->sor_all_levels);
+        if (use_sor) {
+            og_relax(t, ilevel, dp, rhs, aw, ae, as, an, ab, at, rap,
+                     prm->ibtype ? bp : NULL, prm->omega);
+        } else {
+            og_laplacephi(t, ilevel, res, dp, aw, ae, as, an, ab, at, ap, bp);
+            if (prm->ityp == 2)
+                og_sipiter1_cl(t, ilevel, rhs, res, lw, ls, lb, lpr);
+            else
+                og_sipiter1_hp(t, ilevel, rhs, res, lw, ls, lb, lpr, mip, idx);
+            double *fr[1] = {res};
+            og_exchange(t, c, nconn, ilevel, 1, 0,
+                    iloop < prm->ninner ? 0 : -1, fr, 1);
+            if (prm->ityp == 2)
+                og_sipiter2_cl(t, ilevel, dp, res, ue, un, ut);
+            else
+                og_sipiter2_hp(t, ilevel, dp, res, ue, un, ut, mip, idx);
+        }
+        double *fd[1] = {dp};
+        og_exchange(t, c, nconn, ilevel, 1, 0, 0, fd, 1);
+    }
+}
+
+/* src/flow/pressuresolver_mod.F90:160-357 */
+int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
+               const og_params *prm, double *u, double *v, double *w,
+               double *p, double dt, const double *bp,
+               const double *aw, const double *ae, const double *as,
+               const double *an, const double *ab, const double *at,
+               const double *ap, const double *lw, const double *ls,
+               const double *lb, const double *lpr, const double *ue,
+               const double *un, const double *ut, const int64_t *mip,
+               const int64_t *idx, const double *rap,
+               const double *rdx, const double *rdy, const double *rdz,
+               const double *rddx, const double *rddy, const double *rddz,
+               double *dp, double *hilf, double *rhs, double *res,
+               og_stats *st)
+{
+    const int64_t n = field_len(t);
+    memset(dp, 0, (size_t)n * sizeof(double));
+    memset(hilf, 0, (size_t)n * sizeof(double));
+    memset(rhs, 0, (size_t)n * sizeof(double));
+    memset(res, 0, (size_t)n * sizeof(double));
+
+    const double prefak = prm->rho / dt;
+    if (prm->ibtype == 0) {
+        og_divcal(t, rhs, u, v, w, rddx, rddy, rddz, prefak, NULL, NULL);
+    } else {
+        /* ghostcell divcal (src/ib/gc_mod.F90:278-347) passes SDIV, which
+         * is zero without STL stencils, then multiplies by BP */
+        double *sdiv = calloc((size_t)n, sizeof(double));
+        og_divcal(t, rhs, u, v, w, rddx, rddy, rddz, prefak, bp, sdiv);
+        free(sdiv);
+    }
+    st->maxrhs0 = og_maxabscal(t, rhs, NULL);
+
+    int ipc = 0, ipcount;
+    double maxrhs = st->maxrhs0;
+    for (ipcount = 1; ipcount <= prm->nouter; ++ipcount) {
+        for (int l = t->minlevel; l <= t->maxlevel; ++l)
+            mgpoisit(t, c, nconn, prm, l, hilf, rhs, res, bp, aw, ae, as, an,
+                     ab, at, ap, lw, ls, lb, lpr, ue, un, ut, mip, idx, rap);
+        double *fh[1] = {hilf};
+        og_exchange(t, c, nconn, -1, 1, 0, 0, fh, 1);
+        og_laplacephi(t, -1, res, hilf, aw, ae, as, an, ab, at, ap, bp);
+        og_rescal(t, rhs, res);
+        maxrhs = og_maxabscal(t, rhs, NULL);
+        og_accumulate(n, dp, hilf);
+        memset(hilf, 0, (size_t)n * sizeof(double));
+        ipc += prm->ninner;
+        if (st->history) st->history[ipcount - 1] = maxrhs;
+        if (ipcount >= prm->nouter_min) {
+            int done;
+            if (prm->eps_rel > 0.0)
+                done = maxrhs <= prm->eps_rel * st->maxrhs0;
+            else
+                done = maxrhs / prefak < prm->epcorr;
+            if (done) { ++ipcount; break; }
+        }
+    }
+    st->ipc = ipc;
+    st->nouter_done = ipcount - 1;
+    st->maxrhs = maxrhs;
+
+    og_mgpcorr(t, u, v, w, p, dp, rdx, rdy, rdz, dt / prm->rho,
+               prm->ibtype ? bp : NULL);
+    double *fs[4] = {u, v, w, p};
+    for (int l = t->minlevel; l <= t->maxlevel; ++l)
+        og_exchange(t, c, nconn, l, 2, 1, 0, fs, 4);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+double og_time_sip(const og_topo *t, const og_conn *c, int nconn, int niter,
+                   int nthreads, const double *rhs, double *hilf, double *res,
+                   const double *aw, const double *ae, const double *as,
+                   const double *an, const double *ab, const double *at,
+                   const double *ap, const double *bp, const double *lw,
+                   const double *ls, const double *lb, const double *lpr,
+                   const double *ue, const double *un, const double *ut,
+                   const int64_t *mip, const int64_t *idx)
+{
+#ifdef _OPENMP
+    const int saved = omp_get_max_threads();
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    struct timespec a, b;
+    clock_gettime(CLOCK_MONOTONIC, &a);
+    for (int it = 1; it <= niter; ++it) {
+        og_laplacephi(t, -1, res, hilf, aw, ae, as, an, ab, at, ap, bp);
+        og_sipiter1_hp(t, -1, rhs, res, lw, ls, lb, lpr, mip, idx);
+        double *fr[1] = {res};
+        og_exchange(t, c, nconn, -1, 1, 0, 0, fr, 1);
+        og_sipiter2_hp(t, -1, hilf, res, ue, un, ut, mip, idx);
+        double *fh[1] = {hilf};
+        og_exchange(t, c, nconn, -1, 1, 0, 0, fh, 1);
+    }
+    clock_gettime(CLOCK_MONOTONIC, &b);
+#ifdef _OPENMP
+    omp_set_num_threads(saved);
+#endif
+    return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
+}

@@ -1,0 +1,39 @@
+"""Seeded inputs shared by the tests: topologies and staggered velocity
+fields in the reference layout (see mgp.grids)."""
+import numpy as np
+
+from mgp.grids import NOS, uniform_box
+
+
+def topology(name):
+    if name == "self12":        # one self-periodic block (TaylorGreen topology)
+        return uniform_box((1, 1, 1), (12, 12, 12), length=(2 * np.pi,) * 3)
+    if name == "box2":          # 2x2x2 periodic blocks of 8^3
+        return uniform_box((2, 2, 2), (8, 8, 8))
+    if name == "aniso":         # uneven block dims, walls in y, periodic x/z
+        return uniform_box((2, 2, 3), (8, 6, 10), length=(1.0, 0.5, 1.5),
+                           periodic=(True, False, True), wall=NOS)
+    if name == "walls":         # closed box
+        return uniform_box((2, 2, 2), (6, 8, 6), periodic=(False, False, False))
+    if name == "c2":            # 8x8x8 periodic blocks of 24^3 (192^3)
+        return uniform_box((8, 8, 8), (24, 24, 24))
+    raise KeyError(name)
+
+
+def velocity(g, o, seed, amp=1.0):
+    """U(-amp, amp) staggered velocity on interior cells, ghosts filled by
+    the oracle's 2-layer exchange with corners (as the caller's connect
+    does before mgpoisl)."""
+    rng = np.random.default_rng(seed)
+    inner = g.interior_mask()
+    u, v, w = (np.where(inner, rng.uniform(-amp, amp, g.n3d), 0.0) for _ in range(3))
+    o.exchange([u, v, w], layers=2, corners=True)
+    return u, v, w
+
+
+def blocked_mask(g, seed, frac=0.1):
+    """int8 mask (1 = blocked) on interior cells, ghosts from the exchange."""
+    rng = np.random.default_rng(seed)
+    inner = g.interior_mask()
+    m = np.where(inner, (rng.random(g.n3d) < frac).astype(np.float64), 0.0)
+    return m

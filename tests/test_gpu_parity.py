@@ -1,0 +1,118 @@
+"""GPU parity: libmgp (through its C ABI) against the CPU oracle.
+
+FP64 results must be bitwise equal: both sides evaluate every expression in
+the reference's order without FMA, and the GPU sweeps follow the same
+dependency order (hyperplanes), so only the storage layout differs.
+"""
+import numpy as np
+import pytest
+
+from cases import blocked_mask, topology, velocity
+
+pytestmark = pytest.mark.gpu
+
+TOPOS = ["self12", "box2", "aniso", "walls"]
+
+
+def _solver(g, **kw):
+    import mgp
+    return mgp.PressureSolver(g, **kw)
+
+
+def _run_both(oracle_mod, name, seed, ityp=0, ninner=5, nouter=3, ibtype=0,
+              frac=0.0, eps_rel=0.0, precision="fp64", nouter_min=0):
+    g = topology(name)
+    o = oracle_mod.Oracle(g)
+    bp = None
+    if frac > 0:
+        m = blocked_mask(g, seed + 100, frac)
+        bp = 1.0 - m
+        o.exchange([bp], layers=2, corners=True)
+    st = o.setup(bp=bp, ityp=ityp)
+    u, v, w = velocity(g, o, seed)
+    p = np.zeros(g.n3d)
+    ins = [a.copy() for a in (u, v, w, p)]
+    so = o.mgpoisl(st, u, v, w, p, 1e-3, ityp=ityp, ninner=ninner, nouter=nouter,
+                   nouter_min=nouter_min, eps_rel=eps_rel, ibtype=ibtype)
+    s = _solver(g, type={0: "sip", 1: "sor", 2: "classic"}[ityp], ninner=ninner,
+                nouter=nouter, nouter_min=nouter_min, eps_rel=eps_rel,
+                ib={0: "noib", 1: "ghostcell"}[ibtype], precision=precision, bp=bp)
+    gu, gv, gw, gp = ins
+    sg = s.mgpoisl(gu, gv, gw, gp, 1e-3)
+    return g, o, st, so, s, sg, (u, v, w, p), (gu, gv, gw, gp)
+
+
+@pytest.mark.parametrize("name", TOPOS)
+def test_coefficients_bitwise(oracle_mod, name):
+    g = topology(name)
+    o = oracle_mod.Oracle(g)
+    st = o.setup()
+    s = _solver(g)
+    assert np.array_equal(s.get_field("GSAP"), st["coef"]["ap"])
+    for k in ("GSAW", "GSAE", "GSAS", "GSAN", "GSAB", "GSAT"):
+        assert np.array_equal(s.get_field(k), st["coef"][k[2:].lower()]), k
+    for k, f in (("SIPLW", "lw"), ("SIPLS", "ls"), ("SIPLB", "lb"), ("SIPLPR", "lpr"),
+                 ("SIPUE", "ue"), ("SIPUN", "un"), ("SIPUT", "ut")):
+        assert np.array_equal(s.get_field(k), st["fac"][f]), k
+
+
+@pytest.mark.parametrize("name", TOPOS)
+@pytest.mark.parametrize("ityp", [0, 2])
+def test_mgpoisl_fixed_iterations_bitwise(oracle_mod, name, ityp):
+    g, o, st, so, s, sg, ref, got = _run_both(oracle_mod, name, seed=2, ityp=ityp)
+    assert sg["ipc"] == so["ipc"] and sg["nouter_done"] == so["nouter_done"]
+    assert sg["maxrhs0"] == so["maxrhs0"]
+    assert sg["maxrhs"] == so["maxrhs"]
+    for a, b, nm in zip(ref, got, "uvwp"):
+        assert np.array_equal(a, b), nm
+    assert np.array_equal(s.get_field("DP"), so["dp"] * g.interior_mask())
+
+
+@pytest.mark.parametrize("ninner", [1, 2, 4])
+def test_ninner_variants_bitwise(oracle_mod, ninner):
+    g, o, st, so, s, sg, ref, got = _run_both(oracle_mod, "box2", seed=3, ninner=ninner,
+                                              nouter=4)
+    for a, b, nm in zip(ref, got, "uvwp"):
+        assert np.array_equal(a, b), nm
+
+
+def test_masked_ghostcell_bitwise(oracle_mod):
+    g, o, st, so, s, sg, ref, got = _run_both(oracle_mod, "box2", seed=4, ibtype=1,
+                                              frac=0.15)
+    assert np.array_equal(s.get_field("GSAP"), st["coef"]["ap"])
+    for a, b, nm in zip(ref, got, "uvwp"):
+        assert np.array_equal(a, b), nm
+
+
+def test_converged_fp64_and_fp32(oracle_mod):
+    g, o, st, so, s, sg, ref, got = _run_both(oracle_mod, "box2", seed=5, nouter=400,
+                                              eps_rel=1e-10)
+    assert so["maxrhs"] <= 1e-10 * so["maxrhs0"]
+    assert sg["nouter_done"] == so["nouter_done"]
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
+    # FP32 sweeps inside the FP64 outer loop reach the same residual
+    s32 = _solver(g, nouter=400, eps_rel=1e-10, precision="fp32")
+    u, v, w = velocity(g, o, 5)
+    p = np.zeros(g.n3d)
+    r32 = s32.mgpoisl(u, v, w, p, 1e-3)
+    assert r32["converged"] and r32["maxrhs"] <= 1e-10 * r32["maxrhs0"]
+    inner = g.interior_mask()
+    pr = ref[3][inner] - ref[3][inner].mean()
+    pq = p[inner] - p[inner].mean()
+    assert np.abs(pq - pr).max() <= 1e-8 * np.abs(pr).max()
+
+
+def test_connect_identity(oracle_mod):
+    import torch
+    g = topology("aniso")
+    o = oracle_mod.Oracle(g)
+    s = _solver(g)
+    for layers, corners, fwd in ((1, False, 0), (1, False, 1), (1, False, -1),
+                                 (2, True, 0), (2, False, 0)):
+        a = np.arange(g.n3d, dtype=np.float64) * g.interior_mask()
+        b = a.copy()
+        o.exchange([a], layers=layers, corners=corners, forward=fwd)
+        t = torch.from_numpy(b).cuda()
+        s.connect([t], layers=layers, corners=corners, forward=fwd)
+        assert np.array_equal(t.cpu().numpy(), a), (layers, corners, fwd)
