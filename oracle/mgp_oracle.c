@@ -23,7 +23,10 @@
 #include <omp.h>
 #endif
 
-#define ALFA ((double)0.92f)
+/* alfa is the reference constant; tests may change it to pin the
+ * factorisation against closed forms (alfa = 0 gives ILU(0)). */
+static double ALFA = (double)0.92f;
+void og_set_alfa(double a) { ALFA = a; }
 #define TINY ((double)1.0e-20f)
 
 /* 1-based (k, j, i) -> 0-based offset inside one block */
@@ -711,11 +714,12 @@ double og_maxabscal(const og_topo *t, const double *phi_, double *maxlvl)
             for (int j = 3; j <= jj - 2; ++j)
                 for (int k = 3; k <= kk - 2; ++k) {
                     const double a = fabs(phi[IX(k, j, i)]);
-                    m = (a > m) ? a : m;
+                    /* NaN propagates so that a diverging solve is seen */
+                    m = (a > m || a != a) ? a : m;
                 }
         const double s = ldexp(1.0, t->maxlevel - t->level[g - 1]);
         const double ms = fabs(m * s);
-        maxabs = (ms > maxabs) ? ms : maxabs;
+        maxabs = (ms > maxabs || ms != ms) ? ms : maxabs;
         if (maxlvl) {
             double *ml = &maxlvl[t->level[g - 1] - t->minlevel];
             *ml = (m > *ml) ? m : *ml;

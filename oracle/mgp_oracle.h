@@ -70,6 +70,9 @@ void og_giteig(const og_topo *t, const double *dx, const double *dy,
 /* src/flow/sip_hyperplane_mod.F90:269-447 (MIP / IDX, 1-based values) */
 int og_hp_tables(const og_topo *t, int64_t *mip, int64_t *idx);
 
+/* Test hook: change SIP alfa (reference value (double)0.92f). */
+void og_set_alfa(double a);
+
 /* src/flow/sip_hyperplane_mod.F90:106-257 (factors in access order) */
 void og_lu_hp(const og_topo *t, const double *aw, const double *ae,
               const double *as, const double *an, const double *ab,

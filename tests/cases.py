@@ -20,13 +20,31 @@ def topology(name):
     raise KeyError(name)
 
 
+def apply_walls(g, u, v, w):
+    """Zero normal velocity on wall faces (what the caller's bound_flow does
+    for NOS walls, src/flow/bound_flow_mod.F90): u at the -x wall lives at
+    ghost index i=2, at the +x wall at the last interior index ii-2."""
+    for b in range(1, g.ngrid + 1):
+        bc = g.bc[b - 1]
+        for f, (arr, ax) in enumerate(((u, 0), (u, 0), (v, 1), (v, 1), (w, 2), (w, 2))):
+            if bc[f] in (7, 19):
+                continue
+            view = g.block_view(arr, b)
+            n = view.shape[ax]
+            idx = 1 if f % 2 == 0 else n - 3
+            sl = [slice(None)] * 3
+            sl[ax] = idx
+            view[tuple(sl)] = 0.0
+
+
 def velocity(g, o, seed, amp=1.0):
-    """U(-amp, amp) staggered velocity on interior cells, ghosts filled by
-    the oracle's 2-layer exchange with corners (as the caller's connect
-    does before mgpoisl)."""
+    """U(-amp, amp) staggered velocity on interior cells, zero normal
+    velocity on walls, ghosts filled by the oracle's 2-layer exchange with
+    corners (as the caller's connect does before mgpoisl)."""
     rng = np.random.default_rng(seed)
     inner = g.interior_mask()
     u, v, w = (np.where(inner, rng.uniform(-amp, amp, g.n3d), 0.0) for _ in range(3))
+    apply_walls(g, u, v, w)
     o.exchange([u, v, w], layers=2, corners=True)
     return u, v, w
 

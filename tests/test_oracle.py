@@ -1,0 +1,378 @@
+"""Pin the CPU oracle before trusting it.
+
+The reference's own tests hold no values for this path (its cases only check
+exit codes), so each oracle function is pinned here by what the mathematics
+fixes: closed forms, library routines on assembled matrices, brute force on
+tiny inputs and exact invariants.  A plausible slip (wrong neighbour, wrong
+sign, transposed coefficient, off-by-one plane) fails at least one test.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from cases import apply_walls, topology, velocity
+from mgp.grids import NOS, uniform_box
+
+
+def _ids(g):
+    """interior index map: (g, i, j, k) 0-based interior -> flat 3-D index"""
+    out = []
+    for b in range(1, g.ngrid + 1):
+        ii, jj, kk = g.ii[b - 1], g.jj[b - 1], g.kk[b - 1]
+        base = g.ip3[b - 1]
+        for i in range(ii - 4):
+            for j in range(jj - 4):
+                for k in range(kk - 4):
+                    out.append(base + (k + 2) + (j + 2) * kk + (i + 2) * kk * jj)
+    return np.array(out)
+
+
+# ---------------------------------------------------------------- giteig
+def test_giteig_uniform_closed_form(oracle_mod):
+    g = uniform_box((2, 1, 1), (4, 4, 4), length=(2.0, 1.0, 1.0))   # h = 1/4
+    o = oracle_mod.Oracle(g)
+    c = o.giteig(g.ones())
+    inner = g.interior_mask()
+    assert np.all(c["ap"][inner] == -6.0 * 16.0)
+    for k in ("aw", "ae", "as", "an", "ab", "at"):
+        nz = c[k][c[k] != 0]
+        assert np.all(nz == 16.0), k
+
+
+def test_giteig_wall_fold_in_and_blocked_cells(oracle_mod):
+    g = uniform_box((1, 1, 1), (4, 4, 4), periodic=(False, True, True), wall=NOS)
+    o = oracle_mod.Oracle(g)
+    bp = g.ones()
+    v = g.block_view(bp, 1)
+    v[3, 3, 3] = 0.0                         # one blocked cell (interior i=j=k=2)
+    c = o.giteig(bp)
+    a = g.block_view(c["ap"], 1)
+    h2 = 16.0
+    # low/high x walls: coefficient folded into AP and zeroed (noib_mod :178-198)
+    assert c["aw"][2] == 0.0 and c["ae"][g.ii[0] - 3] == 0.0
+    assert a[2, 4, 4] == -6 * h2 + h2        # wall neighbour folded back
+    # blocked cell: all six links cut -> AP = -6/h^2 + 6/h^2 = 0 exactly
+    assert a[3, 3, 3] == 0.0
+    # fluid neighbour of the blocked cell loses one link
+    assert a[4, 3, 3] == -6 * h2 + h2
+
+
+# ---------------------------------------------------------------- tables
+def test_hp_tables_brute_force(oracle_mod):
+    g = uniform_box((1, 1, 1), (5, 3, 4))
+    mip, idx = oracle_mod.Oracle(g).hp_tables()
+    ii, jj, kk = g.ii[0], g.jj[0], g.kk[0]
+    pos = 0
+    for m in range(9, (ii - 2) + (jj - 2) + (kk - 2) + 1):
+        cells = sorted(k + (j - 1) * kk + (i - 1) * kk * jj
+                       for i in range(3, ii - 1) for j in range(3, jj - 1)
+                       for k in range(3, kk - 1) if i + j + k == m)
+        assert mip[m - 1] == pos                       # MIP(m), 1-based m
+        assert list(idx[pos:pos + len(cells)]) == cells
+        pos += len(cells)
+    assert pos == 5 * 3 * 4
+
+
+# ---------------------------------------------------------------- LU
+def _assemble(g, o, c, lo, f):
+    """A, and the unit triangular factors of the sweeps, for a 1-block grid
+    whose faces are all walls (no ghost coupling)."""
+    ids = _ids(g)
+    n = len(ids)
+    ii, jj, kk = g.ii[0], g.jj[0], g.kk[0]
+    ni, nj, nk = ii - 4, jj - 4, kk - 4
+    num = lambda i, j, k: k + nk * (j + nj * i)
+    A = sp.lil_matrix((n, n))
+    Lu = sp.lil_matrix((n, n))
+    Uu = sp.lil_matrix((n, n))
+    lpr = np.zeros(n)
+    mip, idx = o.hp_tables()
+    acc = {int(idx[s]) - 1: s for s in range(n)}        # linear -> access index
+    ap = g.block_view(c["ap"], 1)
+    for i in range(ni):
+        for j in range(nj):
+            for k in range(nk):
+                r = num(i, j, k)
+                lin = (k + 2) + (j + 2) * kk + (i + 2) * kk * jj
+                s = acc[lin]
+                A[r, r] = ap[i + 2, j + 2, k + 2]
+                lpr[r] = f["lpr"][s]
+                nb = ((i - 1, j, k, c["aw"][i + 2], "lw", Lu), (i + 1, j, k, c["ae"][i + 2], "ue", Uu),
+                      (i, j - 1, k, c["as"][j + 2], "ls", Lu), (i, j + 1, k, c["an"][j + 2], "un", Uu),
+                      (i, j, k - 1, c["ab"][k + 2], "lb", Lu), (i, j, k + 1, c["at"][k + 2], "ut", Uu))
+                for a, b, d, coef, fk, M in nb:
+                    if 0 <= a < ni and 0 <= b < nj and 0 <= d < nk:
+                        A[r, num(a, b, d)] = coef
+                        M[r, num(a, b, d)] = f[fk][s]
+    I = sp.identity(n, format="csr")
+    return A.tocsr(), (I + Lu).tocsr(), (I + Uu).tocsr(), lpr, ids
+
+
+@pytest.fixture
+def closed_box():
+    return uniform_box((1, 1, 1), (4, 3, 5), length=(1.0, 0.75, 1.25),
+                       periodic=(False, False, False))
+
+
+def test_lu_alfa0_is_ilu0(oracle_mod, closed_box):
+    """alfa = 0: L U equals A on the 7-point pattern (textbook ILU(0))."""
+    g = closed_box
+    o = oracle_mod.Oracle(g)
+    lib = oracle_mod.lib()
+    lib.og_set_alfa(C.c_double(0.0))
+    try:
+        c = o.giteig(g.ones())
+        mip, idx = o.hp_tables()
+        f = o.lu_hp(c, g.ones(), mip, idx)
+    finally:
+        lib.og_set_alfa(C.c_double(float(np.float32(0.92))))
+    A, L1, U1, lpr, _ = _assemble(g, o, c, None, f)
+    M = (sp.diags(1.0 / lpr) @ L1 @ U1).toarray()
+    Ad = A.toarray()
+    pat = Ad != 0
+    assert np.allclose(M[pat], Ad[pat], rtol=1e-12, atol=0)
+    assert np.abs(M[~pat]).max() > 0            # fill-in exists off the pattern
+
+
+def test_lu_alfa1_row_sums(oracle_mod, closed_box):
+    """alfa = 1: Stone's compensation makes M and A agree on constants."""
+    g = closed_box
+    o = oracle_mod.Oracle(g)
+    lib = oracle_mod.lib()
+    lib.og_set_alfa(C.c_double(1.0))
+    try:
+        c = o.giteig(g.ones())
+        mip, idx = o.hp_tables()
+        f = o.lu_hp(c, g.ones(), mip, idx)
+    finally:
+        lib.og_set_alfa(C.c_double(float(np.float32(0.92))))
+    A, L1, U1, lpr, _ = _assemble(g, o, c, None, f)
+    M = sp.diags(1.0 / lpr) @ L1 @ U1
+    one = np.ones(A.shape[0])
+    assert np.allclose(M @ one, A @ one, rtol=0, atol=1e-9 * np.abs(A).max())
+
+
+def test_lu_classic_equals_hyperplane(oracle_mod):
+    """Both restatements produce identical factors; only storage differs."""
+    g = topology("aniso")
+    o = oracle_mod.Oracle(g)
+    c = o.giteig(g.ones())
+    mip, idx = o.hp_tables()
+    fh = o.lu_hp(c, g.ones(), mip, idx)
+    fc = o.lu_classic(c, g.ones())
+    for b in range(1, g.ngrid + 1):
+        base = g.ip3[b - 1]
+        n = (g.ii[b - 1] - 4) * (g.jj[b - 1] - 4) * (g.kk[b - 1] - 4)
+        lin = idx[base:base + n] - 1 + base
+        for k in fh:
+            assert np.array_equal(fh[k][base:base + n], fc[k][lin]), k
+
+
+def test_sweeps_equal_triangular_solves(oracle_mod, closed_box):
+    g = closed_box
+    o = oracle_mod.Oracle(g)
+    st = o.setup()
+    A, L1, U1, lpr, ids = _assemble(g, o, st["coef"], None, st["fac"])
+    rng = np.random.default_rng(11)
+    rhs = g.zeros()
+    rhs[ids] = rng.standard_normal(len(ids))
+    for ityp in (0, 2):
+        f = st["fac"] if ityp == 0 else o.lu_classic(st["coef"], st["bp"])
+        hilf, res = g.zeros(), g.zeros()
+        o.laplacephi(res, hilf, st["coef"], st["bp"])
+        if ityp == 0:
+            o.sipiter1_hp(rhs, res, f, st["mip"], st["idx"])
+            o.sipiter2_hp(hilf, res, f, st["mip"], st["idx"])
+        else:
+            o.sipiter1_cl(rhs, res, f)
+            o.sipiter2_cl(hilf, res, f)
+        y = spla.spsolve_triangular(L1, lpr * rhs[ids], lower=True)
+        x = spla.spsolve_triangular(U1, y, lower=False)
+        assert np.allclose(hilf[ids], x, rtol=1e-13, atol=1e-13 * np.abs(x).max()), ityp
+
+
+# ---------------------------------------------------------------- stencils
+def test_laplacian_eigenfunction(oracle_mod):
+    g = uniform_box((1, 1, 1), (8, 6, 10), length=(2.0, 1.5, 2.5))
+    o = oracle_mod.Oracle(g)
+    st = o.setup()
+    m = (1, 2, 3)
+    L = (2.0, 1.5, 2.5)
+    h = [L[a] / n for a, n in enumerate((8, 6, 10))]
+    phi = g.zeros()
+    v = g.block_view(phi, 1)
+    X = g.x[:g.ii[0]][:, None, None]
+    Y = g.y[:g.jj[0]][None, :, None]
+    Z = g.z[:g.kk[0]][None, None, :]
+    v[...] = (np.sin(2 * np.pi * m[0] * X / L[0]) * np.cos(2 * np.pi * m[1] * Y / L[1])
+              * np.sin(2 * np.pi * m[2] * Z / L[2]))
+    lam = sum(-4.0 / h[a] ** 2 * np.sin(np.pi * m[a] * h[a] / L[a]) ** 2 for a in range(3))
+    res = g.zeros()
+    o.laplacephi(res, phi, st["coef"], st["bp"])
+    inner = g.interior_mask()
+    assert np.allclose(res[inner], -lam * phi[inner], atol=1e-11 * abs(lam))
+
+
+def test_divcal_discrete_taylor_green(oracle_mod):
+    """The staggered Taylor-Green field is exactly divergence free."""
+    g = uniform_box((2, 2, 2), (8, 8, 8), length=(2 * np.pi,) * 3)
+    o = oracle_mod.Oracle(g)
+    u, v, w = g.zeros(), g.zeros(), g.zeros()
+    for b in range(1, g.ngrid + 1):
+        sx = slice(g.ip1x[b - 1], g.ip1x[b - 1] + g.ii[b - 1])
+        sy = slice(g.ip1y[b - 1], g.ip1y[b - 1] + g.jj[b - 1])
+        sz = slice(g.ip1z[b - 1], g.ip1z[b - 1] + g.kk[b - 1])
+        xs, ys, zs = g.xstag[sx][:, None, None], g.ystag[sy][None, :, None], g.zstag[sz][None, None, :]
+        xc, yc, zc = g.x[sx][:, None, None], g.y[sy][None, :, None], g.z[sz][None, None, :]
+        g.block_view(u, b)[...] = np.sin(xs) * np.cos(yc) * np.cos(zc)
+        g.block_view(v, b)[...] = -np.cos(xc) * np.sin(ys) * np.cos(zc)
+    div = g.zeros()
+    o.divcal(div, u, v, w, 1.0)
+    assert np.abs(div).max() < 1e-13
+
+
+def test_divergence_gradient_adjoint(oracle_mod):
+    """<D u, p> = -<u, G p> on a periodic uniform grid (volume weights equal),
+    which pins the index shifts of divcal against those of mgpcorr."""
+    g = topology("box2")
+    o = oracle_mod.Oracle(g)
+    u, v, w = velocity(g, o, 21)
+    rng = np.random.default_rng(22)
+    inner = g.interior_mask()
+    p = np.where(inner, rng.standard_normal(g.n3d), 0.0)
+    o.exchange([p], layers=1)
+    div = g.zeros()
+    o.divcal(div, u, v, w, 1.0)
+    gu, gv, gw, gp = g.zeros(), g.zeros(), g.zeros(), g.zeros()
+    o.mgpcorr(gu, gv, gw, gp, p, -1.0)      # gu = (p_{i+1} - p_i) / dx
+    lhs = np.dot(div[inner], p[inner])
+    rhs = -(np.dot(u[inner], gu[inner]) + np.dot(v[inner], gv[inner]) +
+            np.dot(w[inner], gw[inner]))
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+# ---------------------------------------------------------------- halo
+def _global_ids(g, nb, n):
+    a = g.zeros()
+    gid = np.zeros(g.n3d, dtype=np.int64)
+    for b in range(1, g.ngrid + 1):
+        bx, by, bz = (b - 1) % nb[0], ((b - 1) // nb[0]) % nb[1], (b - 1) // (nb[0] * nb[1])
+        v = g.block_view(a, b)
+        for i in range(n[0]):
+            for j in range(n[1]):
+                for k in range(n[2]):
+                    X, Y, Z = bx * n[0] + i, by * n[1] + j, bz * n[2] + k
+                    v[i + 2, j + 2, k + 2] = 1 + X + nb[0] * n[0] * (Y + nb[1] * n[1] * Z)
+    return a
+
+
+@pytest.mark.parametrize("layers,corners", [(1, False), (2, False), (2, True)])
+def test_exchange_identity_periodic(oracle_mod, layers, corners):
+    nb, n = (3, 2, 2), (4, 3, 5)
+    g = uniform_box(nb, n)
+    o = oracle_mod.Oracle(g)
+    a = _global_ids(g, nb, n)
+    o.exchange([a], layers=layers, corners=corners)
+    N = [nb[d] * n[d] for d in range(3)]
+    for b in range(1, g.ngrid + 1):
+        bx, by, bz = (b - 1) % nb[0], ((b - 1) // nb[0]) % nb[1], (b - 1) // (nb[0] * nb[1])
+        v = g.block_view(a, b)
+        for i in range(-2, n[0] + 2):
+            for j in range(-2, n[1] + 2):
+                for k in range(-2, n[2] + 2):
+                    out = [(c < 0 or c >= n[d]) for d, c in enumerate((i, j, k))]
+                    dist = [max(-c, c - n[d] + 1, 0) for d, c in enumerate((i, j, k))]
+                    if not any(out):
+                        continue
+                    got = v[i + 2, j + 2, k + 2]
+                    reach = max(dist) <= layers and (corners or sum(out) == 1)
+                    if not reach:
+                        assert got == 0.0
+                        continue
+                    X = (bx * n[0] + i) % N[0]
+                    Y = (by * n[1] + j) % N[1]
+                    Z = (bz * n[2] + k) % N[2]
+                    assert got == 1 + X + N[0] * (Y + N[1] * Z), (b, i, j, k)
+
+
+def test_exchange_forward_flags(oracle_mod):
+    """forward=1 fills only the low-side ghosts (faces 1/3/5), forward=-1
+    only the high-side ghosts (connect_core_mod.F90:672-681)."""
+    nb, n = (2, 2, 2), (3, 3, 3)
+    g = uniform_box(nb, n)
+    o = oracle_mod.Oracle(g)
+    for fwd, lo_filled in ((1, True), (-1, False)):
+        a = _global_ids(g, nb, n)
+        o.exchange([a], layers=1, forward=fwd)
+        v = g.block_view(a, 1)
+        assert (v[1, 3, 3] != 0) == lo_filled and (v[-2, 3, 3] != 0) == (not lo_filled)
+        assert (v[3, 1, 3] != 0) == lo_filled and (v[3, 3, -2] != 0) == (not lo_filled)
+
+
+# ---------------------------------------------------------------- projection
+def test_projection_annihilates_gradients(oracle_mod):
+    """u = -G phi is removed entirely and dt*DP recovers -phi (mod const)."""
+    g = topology("box2")
+    o = oracle_mod.Oracle(g)
+    st = o.setup()
+    rng = np.random.default_rng(31)
+    inner = g.interior_mask()
+    phi = np.where(inner, rng.standard_normal(g.n3d), 0.0)
+    o.exchange([phi], layers=1)
+    u, v, w, q = g.zeros(), g.zeros(), g.zeros(), g.zeros()
+    o.mgpcorr(u, v, w, q, phi, 1.0)          # u = (phi_i - phi_{i+1}) / dx
+    o.exchange([u, v, w], layers=2, corners=True)
+    p = g.zeros()
+    dt = 1e-2
+    r = o.mgpoisl(st, u, v, w, p, dt, nouter=400, eps_rel=1e-12)
+    assert r["maxrhs"] <= 1e-12 * r["maxrhs0"]
+    for a in (u, v, w):
+        assert np.abs(a[inner]).max() < 1e-9
+    dp = r["dp"][inner] * dt            # u_new = u - dt G dp = -G (phi + dt dp)
+    assert np.allclose(dp - dp.mean(), -(phi[inner] - phi[inner].mean()), atol=1e-9)
+
+
+def test_projection_idempotent(oracle_mod):
+    g = topology("aniso")
+    o = oracle_mod.Oracle(g)
+    st = o.setup()
+    u, v, w = velocity(g, o, 41)
+    p = g.zeros()
+    o.mgpoisl(st, u, v, w, p, 1e-3, nouter=400, eps_rel=1e-12)
+    apply_walls(g, u, v, w)                  # the caller's bound_flow
+    u1, v1, w1 = u.copy(), v.copy(), w.copy()
+    p2 = g.zeros()
+    o.mgpoisl(st, u, v, w, p2, 1e-3, nouter=400, eps_rel=1e-12)
+    inner = g.interior_mask()
+    scale = np.abs(u1[inner]).max()
+    for a, b in ((u, u1), (v, v1), (w, w1)):
+        assert np.abs(a[inner] - b[inner]).max() < 1e-9 * scale
+
+
+def test_sor_fixed_point_and_solution(oracle_mod):
+    """SOR leaves the exact solution unchanged, and SOR and SIP converge to
+    the same pressure (sor_mod.F90:102-203 against sip)."""
+    g = topology("walls")
+    o = oracle_mod.Oracle(g)
+    st = o.setup()
+    u, v, w = velocity(g, o, 51)
+    ps, pr = g.zeros(), g.zeros()
+    us = [a.copy() for a in (u, v, w)]
+    r1 = o.mgpoisl(st, *us, ps, 1e-3, nouter=2000, eps_rel=1e-11)
+    ur = [a.copy() for a in (u, v, w)]
+    r2 = o.mgpoisl(st, *ur, pr, 1e-3, ityp=1, omega=1.2, sor_all_levels=1,
+                   nouter=5000, eps_rel=1e-11)
+    assert r1["maxrhs"] <= 1e-11 * r1["maxrhs0"] and r2["maxrhs"] <= 1e-11 * r2["maxrhs0"]
+    inner = g.interior_mask()
+    a, b = ps[inner] - ps[inner].mean(), pr[inner] - pr[inner].mean()
+    assert np.abs(a - b).max() < 1e-8 * np.abs(a).max()
+    # fixed point: relax on the converged DP with its own RHS
+    rhs = g.zeros()
+    o.divcal(rhs, u, v, w, 1e3)
+    dp = r1["dp"].copy()
+    before = dp.copy()
+    o.relax(dp, rhs, st["coef"], st["rap"], 1.3)
+    assert np.abs(dp[inner] - before[inner]).max() < 1e-7 * np.abs(before[inner]).max()
