@@ -1,0 +1,255 @@
+"""Benchmark of the MI355X pressure projection (one JSON line on stdout).
+
+Workload (default): 32768 blocks of 24^3 cells in a 32x32x32 periodic block
+layout (768^3 = 452,984,832 cells), Gaussian-random staggered velocity with a
+k^-5/3 energy spectrum at rms 1e-3 (seed 4, generated on the GPU), SIP
+pressure solve.  A step is one outer (defect-correction) iteration of the
+projection: `ninner` SIP inner iterations over every cell plus the FP64
+residual/RHS/pressure update.  `value` = interior cells x inner iterations /
+wall time of the K timed steps (GCell-updates/s) for the FP32-refined form;
+the all-FP64 form is timed the same way and reported beside it.
+
+The line also carries:
+  roofline     -- dominant kernel (forward sweep) timed with HIP events on
+                  the library's stream; algorithmic bytes per launch from the
+                  per-cell figures in `ALG_BYTES` below;
+  cpu_baseline -- the oracle (C restatement of the reference CPU path) timed
+                  on this host on a bounded sample of the same workload;
+  tts          -- time to a 1e-10 relative residual (FP32-refined and FP64)
+                  for the configuration named in `tts.config`.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT), str(ROOT / "mglet-base_amd")]
+
+# algorithmic bytes per cell per launch (all-fluid blocks read no mask)
+#   forward sweep: HILF, RHS, AP, LW, LS, LB, LPR            (7 reads)
+#   backward sweep: UE, UN, UT reads, HILF write             (4 accesses)
+ALG_BYTES = {"fp32": {"forward": 28, "backward": 16},
+             "fp64": {"forward": 56, "backward": 32}}
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_velocity(g, nb, n, seed, rms=1e-3, spectrum=True):
+    """Staggered velocity in the reference layout as CUDA tensors."""
+    import torch
+    N = [nb[a] * n[a] for a in range(3)]
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    comps = []
+    for c in range(3):
+        if spectrum:
+            kx = torch.fft.fftfreq(N[0], d=1.0 / N[0], device="cuda")
+            ky = torch.fft.fftfreq(N[1], d=1.0 / N[1], device="cuda")
+            kz = torch.fft.rfftfreq(N[2], d=1.0 / N[2], device="cuda")
+            k2 = kx[:, None, None] ** 2 + ky[None, :, None] ** 2 + kz[None, None, :] ** 2
+            amp = torch.where(k2 > 0, k2.clamp(min=1.0) ** (-11.0 / 12.0), torch.zeros_like(k2))
+            del k2
+            re = torch.randn(amp.shape, generator=gen, device="cuda", dtype=torch.float64)
+            im = torch.randn(amp.shape, generator=gen, device="cuda", dtype=torch.float64)
+            spec = torch.complex(re * amp, im * amp)
+            del re, im, amp
+            f = torch.fft.irfftn(spec, s=N)
+            del spec
+        else:
+            f = torch.rand(N, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+        comps.append(f)
+    scale = rms / torch.sqrt(sum((f * f).mean() for f in comps) / 3)
+    out = []
+    for f in comps:
+        f = f * scale
+        blk = (f.reshape(nb[0], n[0], nb[1], n[1], nb[2], n[2])
+               .permute(4, 2, 0, 1, 3, 5).reshape(-1, n[0], n[1], n[2]))
+        full = torch.zeros((blk.shape[0], n[0] + 4, n[1] + 4, n[2] + 4), dtype=torch.float64,
+                           device="cuda")
+        full[:, 2:-2, 2:-2, 2:-2] = blk
+        out.append(full.reshape(-1))
+        del f, blk
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_steps(solver, fields0, steps, ninner):
+    """mgpoisl with nouter = steps on fresh copies; returns (wall s, stats)."""
+    import torch
+    u, v, w = (f.clone() for f in fields0)
+    p = torch.zeros_like(u)
+    solver.set_iterations(ninner=ninner, nouter=steps, nouter_min=steps + 1, eps_rel=0.0,
+                          check_every=max(1, steps))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = solver.mgpoisl(u, v, w, p, 1e-3)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    del u, v, w, p
+    return t1 - t0, st
+
+
+def time_to_solution(solver, fields0, ninner, eps, cap):
+    import torch
+    u, v, w = (f.clone() for f in fields0)
+    p = torch.zeros_like(u)
+    solver.set_iterations(ninner=ninner, nouter=cap, nouter_min=0, eps_rel=eps, check_every=8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = solver.mgpoisl(u, v, w, p, 1e-3)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return t1 - t0, st, p
+
+
+def cpu_baseline(target_s=12.0):
+    """Oracle SIP inner iterations (hyperplane order) on a 4x4x4-block
+    sample of 24^3 blocks, all host threads; GCell-updates/s."""
+    import numpy as np
+    import oracle
+    from mgp.grids import uniform_box
+    g = uniform_box((4, 4, 4), (24, 24, 24))
+    o = oracle.Oracle(g)
+    st = o.setup()
+    rng = np.random.default_rng(4)
+    rhs = np.where(g.interior_mask(), rng.standard_normal(g.n3d), 0.0)
+    threads = oracle.cpu_count()
+    t1 = o.time_sip(st, rhs, 1, threads)
+    niter = max(2, int(target_s / max(t1, 1e-3)))
+    t = o.time_sip(st, rhs, niter, threads)
+    return dict(value=round(g.ncells * niter / t / 1e9, 4), unit="GCell-updates/s",
+                cores=threads, kind="port",
+                sample=f"{g.ngrid} blocks of 24^3 ({g.ncells} cells), {niter} hyperplane-SIP "
+                       f"inner iterations (laplacephi+sipiter1+conn+sipiter2+conn), "
+                       f"OpenMP over blocks, {t:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8, help="timed outer iterations")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--ninner", type=int, default=5)
+    ap.add_argument("--blocks", type=int, nargs=3, default=[32, 32, 32])
+    ap.add_argument("--n", type=int, default=24)
+    ap.add_argument("--tts", choices=["none", "c2", "main"], default="c2",
+                    help="time-to-1e-10 on the 8x8x8-block case (c2), the main case, or skip")
+    ap.add_argument("--tts-cap", type=int, default=4000, help="max outer iterations")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp32,fp64")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (every block of the projection "
+              "lives on one MI355X); multi-GPU sharding is described in DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import mgp
+    from mgp.grids import uniform_box
+
+    nb = tuple(args.blocks)
+    n = (args.n,) * 3
+    log(f"[bench] building topology {nb} blocks of {n}")
+    g = uniform_box(nb, n)
+    log(f"[bench] generating velocity ({g.ncells} cells)")
+    fields0 = make_velocity(g, nb, n, seed=4)
+    forms = args.forms.split(",")
+    res = {}
+    filled = False
+    for form in forms:
+        log(f"[bench] init {form}")
+        s = mgp.PressureSolver(g, ninner=args.ninner, nouter=1, precision=form, check_every=4)
+        if not filled:      # the caller's connect: ghosts of the input velocity
+            s.connect(fields0, layers=2, corners=True)
+            filled = True
+        if args.warmup > 0:
+            run_steps(s, fields0, args.warmup, args.ninner)
+        log(f"[bench] timing {form}: {args.steps} outer x {args.ninner} inner")
+        wall, st = run_steps(s, fields0, args.steps, args.ninner)
+        kt = s.time_kernels(niter=3)
+        res[form] = dict(wall=wall, stats=st, kernels=kt, info=s.info())
+        log(f"[bench] {form}: {wall:.3f} s, device loop {st['solve_ms']:.1f} ms, kernels {kt}")
+        s.close()
+        del s
+        torch.cuda.empty_cache()
+
+    cells = g.ncells
+    main_form = "fp32" if "fp32" in res else forms[0]
+    r = res[main_form]
+    upd = cells * args.ninner * args.steps
+    value = upd / r["wall"] / 1e9
+    fw_ms = r["kernels"]["forward_ms"]
+    alg = ALG_BYTES[main_form]["forward"] * cells
+    achieved = alg / (fw_ms * 1e-3) / 1e9
+    line = {
+        "metric": "GCell-updates/s + time-to-1e-10 residual, FP32-refined vs FP64, 1 MI355X",
+        "value": round(value, 3),
+        "unit": "GCell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 inner sweeps, f64 outer" if main_form == "fp32" else "f64",
+        "data": "synthetic (seeded k^-5/3 Gaussian velocity, rms 1e-3)",
+        "config": {"workload": f"SIP projection, {g.ngrid} blocks of {n[0]}^3 "
+                               f"({nb[0]}x{nb[1]}x{nb[2]} periodic), {cells} cells",
+                   "ninner": args.ninner, "step": f"1 outer iteration = {args.ninner} SIP "
+                                                   "inner iterations + FP64 update"},
+        "forms": {f: {"gcell_updates_per_s": round(cells * args.ninner * args.steps
+                                                   / res[f]["wall"] / 1e9, 3),
+                      "ms_per_step": round(res[f]["wall"] / args.steps * 1e3, 3),
+                      "kernel_ms": {k: round(v, 4) for k, v in res[f]["kernels"].items()},
+                      "device_gb": round(res[f]["info"]["device_bytes"] / 1e9, 2)}
+                  for f in res},
+        "roofline": {"bound": "hbm", "kernel": f"k_forward ({main_form})",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "alg_bytes_per_cell": ALG_BYTES[main_form]["forward"]},
+    }
+    if "fp32" in res and "fp64" in res:
+        line["fp32_over_fp64_throughput"] = round(res["fp64"]["wall"] / res["fp32"]["wall"], 3)
+
+    if args.tts != "none":
+        tnb = (8, 8, 8) if args.tts == "c2" else nb
+        tg = uniform_box(tnb, n) if args.tts == "c2" else g
+        tf = make_velocity(tg, tnb, n, seed=4) if args.tts == "c2" else fields0
+        tts = {"config": f"{tg.ngrid} blocks of {n[0]}^3 ({tnb[0]}x{tnb[1]}x{tnb[2]} periodic)",
+               "eps_rel": 1e-10}
+        ps = {}
+        for form in forms:
+            s = mgp.PressureSolver(tg, ninner=args.ninner, nouter=1, precision=form)
+            if form == forms[0] and args.tts == "c2":
+                s.connect(tf, layers=2, corners=True)
+            time_to_solution(s, tf, args.ninner, 1e-10, 2)        # warm
+            t, st, p = time_to_solution(s, tf, args.ninner, 1e-10, args.tts_cap)
+            ps[form] = p
+            tts[form] = {"seconds": round(t, 4), "inner_iterations": st["ipc"],
+                         "converged": st["converged"],
+                         "rel_residual": st["maxrhs"] / st["maxrhs0"]}
+            log(f"[bench] tts {form}: {t:.3f} s, {st['ipc']} inner, conv {st['converged']}")
+            s.close()
+        if "fp32" in ps and "fp64" in ps:
+            a, b = ps["fp32"], ps["fp64"]
+            a = a - a.mean()
+            b = b - b.mean()
+            tts["fp32_pressure_err_vs_fp64"] = float((a - b).abs().max() / b.abs().max())
+            tts["speedup_fp32_over_fp64"] = round(tts["fp64"]["seconds"] / tts["fp32"]["seconds"], 3)
+        line["tts"] = tts
+
+    if not args.no_cpu:
+        log("[bench] cpu baseline")
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,118 @@
+! mgp_bindc_mod -- ISO_C_BINDING interface of libmgp.so (include/mgp.h).
+!
+! The derived types mirror the C structs field by field (BIND(C) gives
+! them the C layout); the functions are the C entry points.  Arrays are
+! passed as C_LOC of the caller's contiguous Fortran arrays, the way the
+! reference already binds zlib's crc32 (src/core/checksum_mod.F90:18-24).
+MODULE mgp_bindc_mod
+    USE, INTRINSIC :: ISO_C_BINDING
+    IMPLICIT NONE
+    PRIVATE
+
+    INTEGER(C_INT), PARAMETER, PUBLIC :: MGP_OK = 0
+
+    ! struct mgp_grids (GRIDINFO subset, src/core/gridio_mod.F90:15-37)
+    TYPE, BIND(C), PUBLIC :: mgp_grids_t
+        INTEGER(C_INT) :: ngrid = 0, minlevel = 0, maxlevel = 0
+        TYPE(C_PTR) :: kk = C_NULL_PTR, jj = C_NULL_PTR, ii = C_NULL_PTR
+        TYPE(C_PTR) :: level = C_NULL_PTR, nbrgrid = C_NULL_PTR
+        TYPE(C_PTR) :: itypbc = C_NULL_PTR
+        TYPE(C_PTR) :: ip3 = C_NULL_PTR, ip1x = C_NULL_PTR
+        TYPE(C_PTR) :: ip1y = C_NULL_PTR, ip1z = C_NULL_PTR
+    END TYPE mgp_grids_t
+
+    ! struct mgp_config (/flow/pressuresolver keys,
+    ! src/flow/pressuresolver_mod.F90:57-142)
+    TYPE, BIND(C), PUBLIC :: mgp_config_t
+        INTEGER(C_INT) :: ityp = 0, ninner = 5, nouter = 9, nouter_min = 0
+        REAL(C_DOUBLE) :: epcorr = 0.0_C_DOUBLE, omega = 1.1_C_DOUBLE
+        REAL(C_DOUBLE) :: rho = 1.0_C_DOUBLE
+        INTEGER(C_INT) :: ibtype = 0, precision = 0
+        REAL(C_DOUBLE) :: eps_rel = 0.0_C_DOUBLE
+        INTEGER(C_INT) :: sor_all_levels = 0, check_every = 4
+    END TYPE mgp_config_t
+
+    TYPE, BIND(C), PUBLIC :: mgp_stats_t
+        INTEGER(C_INT) :: ipc = 0, nouter_done = 0, converged = 0
+        REAL(C_DOUBLE) :: maxrhs0 = 0.0_C_DOUBLE, maxrhs = 0.0_C_DOUBLE
+        REAL(C_DOUBLE) :: solve_ms = 0.0_C_DOUBLE
+    END TYPE mgp_stats_t
+
+    PUBLIC :: mgp_init, mgp_init_mask, mgp_mgpoisl, mgp_mgpoisl_device, &
+        mgp_finish, mgp_last_error, mgp_set_iterations, mgp_error_message
+
+    INTERFACE
+        INTEGER(C_INT) FUNCTION mgp_init(grids, cfg, dx, dy, dz, ddx, ddy, &
+                ddz, bp, handle) BIND(C, name="mgp_init")
+            IMPORT :: C_INT, C_PTR, mgp_grids_t, mgp_config_t
+            TYPE(mgp_grids_t), INTENT(in) :: grids
+            TYPE(mgp_config_t), INTENT(in) :: cfg
+            TYPE(C_PTR), VALUE :: dx, dy, dz, ddx, ddy, ddz, bp
+            TYPE(C_PTR), INTENT(out) :: handle
+        END FUNCTION mgp_init
+
+        INTEGER(C_INT) FUNCTION mgp_init_mask(grids, cfg, dx, dy, dz, ddx, &
+                ddy, ddz, blocked, handle) BIND(C, name="mgp_init_mask")
+            IMPORT :: C_INT, C_PTR, mgp_grids_t, mgp_config_t
+            TYPE(mgp_grids_t), INTENT(in) :: grids
+            TYPE(mgp_config_t), INTENT(in) :: cfg
+            TYPE(C_PTR), VALUE :: dx, dy, dz, ddx, ddy, ddz, blocked
+            TYPE(C_PTR), INTENT(out) :: handle
+        END FUNCTION mgp_init_mask
+
+        INTEGER(C_INT) FUNCTION mgp_mgpoisl(handle, u, v, w, p, dt, ittot, &
+                irk, stats) BIND(C, name="mgp_mgpoisl")
+            IMPORT :: C_INT, C_PTR, C_DOUBLE, mgp_stats_t
+            TYPE(C_PTR), VALUE :: handle, u, v, w, p
+            REAL(C_DOUBLE), VALUE :: dt
+            INTEGER(C_INT), VALUE :: ittot, irk
+            TYPE(mgp_stats_t), INTENT(out) :: stats
+        END FUNCTION mgp_mgpoisl
+
+        INTEGER(C_INT) FUNCTION mgp_mgpoisl_device(handle, u, v, w, p, dt, &
+                ittot, irk, stats) BIND(C, name="mgp_mgpoisl_device")
+            IMPORT :: C_INT, C_PTR, C_DOUBLE, mgp_stats_t
+            TYPE(C_PTR), VALUE :: handle, u, v, w, p
+            REAL(C_DOUBLE), VALUE :: dt
+            INTEGER(C_INT), VALUE :: ittot, irk
+            TYPE(mgp_stats_t), INTENT(out) :: stats
+        END FUNCTION mgp_mgpoisl_device
+
+        INTEGER(C_INT) FUNCTION mgp_finish(handle) BIND(C, name="mgp_finish")
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: handle
+        END FUNCTION mgp_finish
+
+        TYPE(C_PTR) FUNCTION mgp_last_error() BIND(C, name="mgp_last_error")
+            IMPORT :: C_PTR
+        END FUNCTION mgp_last_error
+
+        INTEGER(C_INT) FUNCTION mgp_set_iterations(handle, ninner, nouter, &
+                nouter_min, epcorr, eps_rel, check_every) &
+                BIND(C, name="mgp_set_iterations")
+            IMPORT :: C_INT, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: handle
+            INTEGER(C_INT), VALUE :: ninner, nouter, nouter_min, check_every
+            REAL(C_DOUBLE), VALUE :: epcorr, eps_rel
+        END FUNCTION mgp_set_iterations
+    END INTERFACE
+
+CONTAINS
+
+    ! Text of mgp_last_error() as a Fortran string
+    FUNCTION mgp_error_message() RESULT(msg)
+        CHARACTER(len=512) :: msg
+        CHARACTER(kind=C_CHAR), POINTER :: cs(:)
+        INTEGER :: n
+        TYPE(C_PTR) :: p
+
+        msg = ""
+        p = mgp_last_error()
+        IF (.NOT. C_ASSOCIATED(p)) RETURN
+        CALL C_F_POINTER(p, cs, [512])
+        DO n = 1, 512
+            IF (cs(n) == C_NULL_CHAR) EXIT
+            msg(n:n) = cs(n)
+        END DO
+    END FUNCTION mgp_error_message
+END MODULE mgp_bindc_mod

@@ -59,6 +59,13 @@ def load():
     if not LIB_PATH.exists():
         raise MgpError(f"{LIB_PATH} is not built; run `make -C mglet-base_amd` "
                        "or __graft_entry__.build()")
+    # One HIP runtime per process: when torch is present its runtime is
+    # loaded first, and libmgp's libamdhip64.so.7 dependency resolves to the
+    # already-loaded copy (same soname) instead of mapping a second runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(LIB_PATH))
     vp = C.c_void_p
     sig = {

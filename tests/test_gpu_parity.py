@@ -116,3 +116,17 @@ def test_connect_identity(oracle_mod):
         t = torch.from_numpy(b).cuda()
         s.connect([t], layers=layers, corners=corners, forward=fwd)
         assert np.array_equal(t.cpu().numpy(), a), (layers, corners, fwd)
+
+
+def test_single_hip_runtime():
+    """torch and libmgp share one HIP runtime in the process."""
+    import torch
+    import mgp
+    g = topology("box2")
+    s = _solver(g)
+    x = torch.ones(4, device="cuda")
+    assert float(x.sum()) == 4.0
+    maps = open("/proc/self/maps").read()
+    libs = {ln.split()[-1] for ln in maps.splitlines() if "libamdhip64" in ln}
+    assert len(libs) == 1, libs
+    s.close()
