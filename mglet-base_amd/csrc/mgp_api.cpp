@@ -122,7 +122,7 @@ struct mgp_handle {
     std::vector<Conn> conns;
     std::map<int, std::vector<int>> level_blocks;  // level -> block indices
     bool masked = false;
-    int max_ni = 0, max_nj = 0, max_nk = 0;
+    int max_ni = 0, max_nj = 0, max_nk = 0, max_plane = 0;
 
     hipStream_t own_stream = nullptr, stream = nullptr;
 
@@ -169,6 +169,7 @@ struct mgp_handle {
         t.max_ni = max_ni;
         t.max_nj = max_nj;
         t.max_nk = max_nk;
+        t.max_plane = max_plane;
         t.maxlevel = maxlevel;
         if (lvl < 0) {
             t.blist = d_blist_all.as<int>();
@@ -442,6 +443,19 @@ void build_shapes(mgp_handle &h, std::vector<uint32_t> &cells,
                     }
             }
             pstart.push_back(s);
+            // face_index must be a bijection onto [0, a*b) for each face
+            const int fa[3][2] = {{nk, nj}, {nk, ni}, {nj, ni}};
+            for (auto &ab : fa) {
+                std::vector<char> seen((size_t)ab[0] * ab[1], 0);
+                for (int u = 0; u < ab[0]; ++u)
+                    for (int v = 0; v < ab[1]; ++v) {
+                        const int q = face_index(u, v, ab[0], ab[1]);
+                        if (q < 0 || q >= ab[0] * ab[1] || seen[q]++)
+                            throw HipError{"face index table inconsistent"};
+                    }
+            }
+            for (int p = 0; p < S.np; ++p)
+                h.max_plane = std::max(h.max_plane, pstart[S.pstart_off + p + 1] - pstart[S.pstart_off + p]);
             sid = (int)h.shapes.size();
             h.shapes.push_back(S);
             idx[key] = sid;
@@ -540,8 +554,10 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
         h->ip1y.push_back(G->ip1y[g] - 1);
         h->ip1z.push_back(G->ip1z[g] - 1);
         const int ni = h->ii[g] - 4, nj = h->jj[g] - 4, nk = h->kk[g] - 4;
-        if (ni < 2 || nj < 2 || nk < 2 || ni >= kMaxN || nj >= kMaxN || nk >= kMaxN)
-            return fail(MGP_ERR_TOPOLOGY, "grid %d: unsupported dims", g + 1);
+        if (ni < 2 || nj < 2 || nk < 2 || 4 * ni + 2 * nj > 256 || ni + nj + nk - 2 > 512 ||
+            ni * nj > 1024)
+            return fail(MGP_ERR_TOPOLOGY, "grid %d: interior %dx%dx%d outside the supported "
+                        "block sizes (4*ni + 2*nj <= 256, ni*nj <= 1024)", g + 1, ni, nj, nk);
         if (h->level[g] < h->minlevel || h->level[g] > h->maxlevel)
             return fail(MGP_ERR_TOPOLOGY, "grid %d: level out of range", g + 1);
         for (int f = 0; f < 6; ++f)
@@ -618,18 +634,18 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
                 uint8_t *f = &bpf[B.fc];
                 for (int j = 0; j < B.nj; ++j)
                     for (int k = 0; k < B.nk; ++k) {
-                        f[face_base(0, B.ni, B.nj, B.nk) + k + B.nk * j] = (uint8_t)at(-1, j, k);
-                        f[face_base(1, B.ni, B.nj, B.nk) + k + B.nk * j] = (uint8_t)at(B.ni, j, k);
+                        f[face_base(0, B.ni, B.nj, B.nk) + fidx_x(k, j, B.nj, B.nk)] = (uint8_t)at(-1, j, k);
+                        f[face_base(1, B.ni, B.nj, B.nk) + fidx_x(k, j, B.nj, B.nk)] = (uint8_t)at(B.ni, j, k);
                     }
                 for (int i = 0; i < B.ni; ++i)
                     for (int k = 0; k < B.nk; ++k) {
-                        f[face_base(2, B.ni, B.nj, B.nk) + k + B.nk * i] = (uint8_t)at(i, -1, k);
-                        f[face_base(3, B.ni, B.nj, B.nk) + k + B.nk * i] = (uint8_t)at(i, B.nj, k);
+                        f[face_base(2, B.ni, B.nj, B.nk) + fidx_y(k, i, B.ni, B.nk)] = (uint8_t)at(i, -1, k);
+                        f[face_base(3, B.ni, B.nj, B.nk) + fidx_y(k, i, B.ni, B.nk)] = (uint8_t)at(i, B.nj, k);
                     }
                 for (int i = 0; i < B.ni; ++i)
                     for (int j = 0; j < B.nj; ++j) {
-                        f[face_base(4, B.ni, B.nj, B.nk) + j + B.nj * i] = (uint8_t)at(i, j, -1);
-                        f[face_base(5, B.ni, B.nj, B.nk) + j + B.nj * i] = (uint8_t)at(i, j, B.nk);
+                        f[face_base(4, B.ni, B.nj, B.nk) + fidx_z(j, i, B.ni, B.nj)] = (uint8_t)at(i, j, -1);
+                        f[face_base(5, B.ni, B.nj, B.nk) + fidx_z(j, i, B.ni, B.nj)] = (uint8_t)at(i, j, B.nk);
                     }
                 if (!all1) h->masked = true;
             }

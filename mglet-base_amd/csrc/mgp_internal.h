@@ -7,9 +7,9 @@
 // by i then j -- the same order as the reference's SIP access index
 // `iacc` (src/flow/sip_hyperplane_mod.F90:269-361).  A sweep step over one
 // hyperplane therefore reads a contiguous run of every array.  Ghost
-// values live in per-block face buffers:
-//   face 0/1 (-x/+x) indexed k + NK*j, face 2/3 (-y/+y) k + NK*i,
-//   face 4/5 (-z/+z) j + NJ*i.
+// values live in per-block face buffers: face 0/1 (-x/+x) hold (k, j),
+// face 2/3 (-y/+y) (k, i), face 4/5 (-z/+z) (j, i), each anti-diagonal-
+// major (see face_index).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -43,6 +43,28 @@ struct Block {
     int64_t fc;        // offset of the block's face buffers
     int64_t o1x, o1y, o1z; // offsets of the block in 1-D fields
 };
+
+// Position of face cell (u, v), u in [0, a), v in [0, b), inside one face
+// buffer.  Faces are stored anti-diagonal-major: diagonal d = u + v, then v
+// ascending.  A sweep step over hyperplane p touches exactly one diagonal
+// of each face, so those accesses are contiguous.
+__host__ __device__ inline int diag_start(int d, int a, int b) {
+    const int m = a < b ? a : b, M = a < b ? b : a;
+    if (d <= m) return d * (d + 1) / 2;
+    const int base1 = m * (m + 1) / 2;
+    if (d <= M) return base1 + (d - m) * m;
+    const int base2 = base1 + (M - m) * m;
+    return base2 + (d - M) * (a + b - 1) - (M + d - 1) * (d - M) / 2;
+}
+__host__ __device__ inline int face_index(int u, int v, int a, int b) {
+    const int d = u + v;
+    const int vmin = d - (a - 1) > 0 ? d - (a - 1) : 0;
+    return diag_start(d, a, b) + (v - vmin);
+}
+// x faces hold (k, j), y faces (k, i), z faces (j, i)
+__host__ __device__ inline int fidx_x(int k, int j, int nj, int nk) { return face_index(k, j, nk, nj); }
+__host__ __device__ inline int fidx_y(int k, int i, int ni, int nk) { return face_index(k, i, nk, ni); }
+__host__ __device__ inline int fidx_z(int j, int i, int ni, int nj) { return face_index(j, i, nj, ni); }
 
 // Face buffer sub-offsets inside one block's face region.
 __host__ __device__ inline int face_base(int f, int ni, int nj, int nk) {

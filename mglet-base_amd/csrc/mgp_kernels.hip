@@ -90,13 +90,23 @@ __device__ __forceinline__ double bp_at(const Block &B, const int *slot_of,
     if (B.flags & 1) return 1.0;
     const int ni = B.ni, nj = B.nj, nk = B.nk;
     const uint8_t *f = m.bpf + B.fc;
-    if (i < 0) return f[face_base(0, ni, nj, nk) + k + nk * j];
-    if (i >= ni) return f[face_base(1, ni, nj, nk) + k + nk * j];
-    if (j < 0) return f[face_base(2, ni, nj, nk) + k + nk * i];
-    if (j >= nj) return f[face_base(3, ni, nj, nk) + k + nk * i];
-    if (k < 0) return f[face_base(4, ni, nj, nk) + j + nj * i];
-    if (k >= nk) return f[face_base(5, ni, nj, nk) + j + nj * i];
+    if (i < 0) return f[face_base(0, ni, nj, nk) + fidx_x(k, j, nj, nk)];
+    if (i >= ni) return f[face_base(1, ni, nj, nk) + fidx_x(k, j, nj, nk)];
+    if (j < 0) return f[face_base(2, ni, nj, nk) + fidx_y(k, i, ni, nk)];
+    if (j >= nj) return f[face_base(3, ni, nj, nk) + fidx_y(k, i, ni, nk)];
+    if (k < 0) return f[face_base(4, ni, nj, nk) + fidx_z(j, i, ni, nj)];
+    if (k >= nk) return f[face_base(5, ni, nj, nk) + fidx_z(j, i, ni, nj)];
     return m.bp[B.wo + slot_of[k + nk * (j + nj * i)]];
+}
+
+// Workgroup barrier that orders LDS only.  __syncthreads() also fences
+// global memory, which on CDNA waits for every outstanding global load
+// (s_waitcnt vmcnt(0)) and would drain the sweeps' prefetches each step;
+// all cross-lane traffic inside a sweep goes through LDS.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // ---------------------------------------------------------------------
@@ -106,8 +116,8 @@ __global__ void k_ap(Topo t, const double *dx_, const double *dy_,
                      const double *as_, const double *an_, const double *ab_,
                      const double *at_, Mask m, double *ap) {
     const int b = t.blist[blockIdx.y];
-    const Block B = t.blocks[b];
-    const Shape S = t.shapes[B.shape];
+    const Block &B = t.blocks[b];
+    const Shape &S = t.shapes[B.shape];
     const int *slot_of = t.slot_of + S.slot_off;
     const uint32_t *cells = t.cells + S.cell_off;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S.ncell;
@@ -174,8 +184,8 @@ __global__ __launch_bounds__(kThreads) void k_lu(
     T *un_o, T *ut_o) {
     extern __shared__ double lds_lu[];
     const int b = t.blist[blockIdx.x];
-    const Block B = t.blocks[b];
-    const Shape S = t.shapes[B.shape];
+    const Block &B = t.blocks[b];
+    const Shape &S = t.shapes[B.shape];
     const int NI = B.ni, NJ = B.nj;
     const int RW = NJ + 1, RS = (NJ + 1) * (NI + 1);
     double *U = lds_lu;  // [2][3][RS]
@@ -186,7 +196,7 @@ __global__ __launch_bounds__(kThreads) void k_lu(
     const double *as = as_ + B.o1y + 2, *an = an_ + B.o1y + 2;
     const double *ab = ab_ + B.o1z + 2, *at = at_ + B.o1z + 2;
     for (int e = threadIdx.x; e < 6 * RS; e += blockDim.x) U[e] = 0.0;
-    __syncthreads();
+    lds_barrier();
     for (int p = 0; p < S.np; ++p) {
         const double *pe = U + ((p - 1) & 1) * 3 * RS;
         const double *pn = pe + RS, *pt = pn + RS;
@@ -240,7 +250,7 @@ __global__ __launch_bounds__(kThreads) void k_lu(
             un_o[o] = (T)un;
             ut_o[o] = (T)ut;
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -275,8 +285,8 @@ __global__ void k_divcal(Topo t, const double *u, const double *v,
                          const double *bp_sp, double fak, double *rhs64,
                          T *rhs_t) {
     const int b = t.blist[blockIdx.y];
-    const Block B = t.blocks[b];
-    const Shape S = t.shapes[B.shape];
+    const Block &B = t.blocks[b];
+    const Shape &S = t.shapes[B.shape];
     const uint32_t *cells = t.cells + S.cell_off;
     const int64_t kk = B.nk + 4, jj = B.nj + 4;
     const double *rddx = rddx_ + B.o1x + 2, *rddy = rddy_ + B.o1y + 2,
@@ -342,12 +352,11 @@ __device__ void block_max_to_global(double m, unsigned long long *g) {
 // max |RHS| over interior cells with the level weight 2^(maxlevel-level):
 // src/flow/pressuresolver_mod.F90:623-689.
 __global__ void k_maxabs(Topo t, const double *rhs64, Outer o) {
-    const int b = t.blist[blockIdx.y];
-    const Block B = t.blocks[b];
-    const Shape S = t.shapes[B.shape];
+    const int b = t.blist[blockIdx.x];
+    const Block &B = t.blocks[b];
+    const Shape &S = t.shapes[B.shape];
     double m = 0.0;
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S.ncell;
-         s += gridDim.x * blockDim.x) {
+    for (int s = threadIdx.x; s < S.ncell; s += blockDim.x) {
         const double a = fabs(rhs64[B.wo + s]);
         m = (a > m || a != a) ? a : m;
     }
@@ -368,10 +377,8 @@ __global__ void k_maxabs_store(Outer o) {
 
 void launch_maxabs(const Topo &t, const double *rhs64, Outer o,
                    hipStream_t s) {
-    const int maxc = t.max_ni * t.max_nj * t.max_nk;
-    dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
     launch_fill<unsigned long long>(o.gmax, 1, 0ull, s);
-    k_maxabs<<<grid, kThreads, 0, s>>>(t, rhs64, o);
+    k_maxabs<<<t.nlist, kThreads, 0, s>>>(t, rhs64, o);
     k_maxabs_store<<<1, 1, 0, s>>>(o);
 }
 template void launch_fill<unsigned long long>(unsigned long long *, int64_t,
@@ -386,89 +393,6 @@ template <typename T>
 struct Window {
     const T *zlo, *zhi, *ylo, *yhi, *xlo, *xhi;  // neighbour faces or null
 };
-
-template <typename T>
-__device__ void fill_plane(T *Hq, int q, const Block &B, const Shape &S,
-                           const uint32_t *cells, const int *ps,
-                           const T *field, const Window<T> &w) {
-    const int NI = B.ni, NJ = B.nj, NK = B.nk, HW = NJ + 2;
-    if (q >= 0 && q < S.np) {
-        const int cnt = ps[q + 1] - ps[q];
-        for (int x = threadIdx.x; x < cnt; x += blockDim.x) {
-            const int s = ps[q] + x;
-            int i, j, k;
-            unpack(cells[s], i, j, k);
-            Hq[(j + 1) + HW * (i + 1)] = field[B.wo + s];
-        }
-    }
-    const int ng = 4 * NI + 2 * NJ;
-    for (int e = threadIdx.x; e < ng; e += blockDim.x) {
-        int i, j, k, idx = -1;
-        T v = T(0);
-        if (e < NI) {                      // k = -1
-            i = e; j = q + 1 - i;
-            if (j >= 0 && j < NJ) { idx = (j + 1) + HW * (i + 1); if (w.zlo) v = w.zlo[j + NJ * i]; }
-        } else if (e < 2 * NI) {           // k = NK
-            i = e - NI; j = q - NK - i;
-            if (j >= 0 && j < NJ) { idx = (j + 1) + HW * (i + 1); if (w.zhi) v = w.zhi[j + NJ * i]; }
-        } else if (e < 3 * NI) {           // j = -1
-            i = e - 2 * NI; k = q + 1 - i;
-            if (k >= 0 && k < NK) { idx = HW * (i + 1); if (w.ylo) v = w.ylo[k + NK * i]; }
-        } else if (e < 4 * NI) {           // j = NJ
-            i = e - 3 * NI; k = q - NJ - i;
-            if (k >= 0 && k < NK) { idx = (NJ + 1) + HW * (i + 1); if (w.yhi) v = w.yhi[k + NK * i]; }
-        } else if (e < 4 * NI + NJ) {      // i = -1
-            j = e - 4 * NI; k = q + 1 - j;
-            if (k >= 0 && k < NK) { idx = j + 1; if (w.xlo) v = w.xlo[k + NK * j]; }
-        } else {                           // i = NI
-            j = e - 4 * NI - NJ; k = q - NI - j;
-            if (k >= 0 && k < NK) { idx = (j + 1) + HW * (NI + 1); if (w.xhi) v = w.xhi[k + NK * j]; }
-        }
-        if (idx >= 0) Hq[idx] = v;
-    }
-}
-
-// Same for the BP mask (own ghost layer, face layout).
-__device__ void fill_plane_bp(uint8_t *Hq, int q, const Block &B,
-                              const Shape &S, const uint32_t *cells,
-                              const int *ps, Mask m) {
-    const int NI = B.ni, NJ = B.nj, NK = B.nk, HW = NJ + 2;
-    const uint8_t *f = m.bpf + B.fc;
-    if (q >= 0 && q < S.np) {
-        const int cnt = ps[q + 1] - ps[q];
-        for (int x = threadIdx.x; x < cnt; x += blockDim.x) {
-            const int s = ps[q] + x;
-            int i, j, k;
-            unpack(cells[s], i, j, k);
-            Hq[(j + 1) + HW * (i + 1)] = m.bp[B.wo + s];
-        }
-    }
-    const int ng = 4 * NI + 2 * NJ;
-    for (int e = threadIdx.x; e < ng; e += blockDim.x) {
-        int i, j, k, idx = -1;
-        uint8_t v = 0;
-        if (e < NI) {
-            i = e; j = q + 1 - i;
-            if (j >= 0 && j < NJ) { idx = (j + 1) + HW * (i + 1); v = f[face_base(4, NI, NJ, NK) + j + NJ * i]; }
-        } else if (e < 2 * NI) {
-            i = e - NI; j = q - NK - i;
-            if (j >= 0 && j < NJ) { idx = (j + 1) + HW * (i + 1); v = f[face_base(5, NI, NJ, NK) + j + NJ * i]; }
-        } else if (e < 3 * NI) {
-            i = e - 2 * NI; k = q + 1 - i;
-            if (k >= 0 && k < NK) { idx = HW * (i + 1); v = f[face_base(2, NI, NJ, NK) + k + NK * i]; }
-        } else if (e < 4 * NI) {
-            i = e - 3 * NI; k = q - NJ - i;
-            if (k >= 0 && k < NK) { idx = (NJ + 1) + HW * (i + 1); v = f[face_base(3, NI, NJ, NK) + k + NK * i]; }
-        } else if (e < 4 * NI + NJ) {
-            j = e - 4 * NI; k = q + 1 - j;
-            if (k >= 0 && k < NK) { idx = j + 1; v = f[face_base(0, NI, NJ, NK) + k + NK * j]; }
-        } else {
-            j = e - 4 * NI - NJ; k = q - NI - j;
-            if (k >= 0 && k < NK) { idx = (j + 1) + HW * (NI + 1); v = f[face_base(1, NI, NJ, NK) + k + NK * j]; }
-        }
-        if (idx >= 0) Hq[idx] = v;
-    }
-}
 
 template <typename T>
 __device__ __forceinline__ Window<T> hilf_window(const T *hface,
@@ -521,6 +445,134 @@ __device__ __forceinline__ A laplace_cell(const T *Hm, const T *H0,
     return r;
 }
 
+// Software pipeline shared by the plane-sweep kernels.  During step p a
+// workgroup (a) issues the global loads for step p+1 (per-cell data) and
+// for plane p+3 of the HILF window, (b) computes plane p from registers and
+// LDS, (c) moves the window plane loaded during step p-1 into LDS, then
+// (d) hits the step's single LDS-only barrier.  The step loop is unrolled
+// twice with ping-pong register buffers, so no register holding a load in
+// flight is ever copied (a copy would force the wave to wait for it), and
+// every load is unconditional (clamped index, selected afterwards), which
+// keeps the compiler's wait counts exact.  The 1-D coefficients of the
+// block sit in LDS.
+constexpr int kMaxPlanes = 512;
+constexpr int kMaxN1 = 64;      // interior cells per direction (4 ni <= 256)
+constexpr uint32_t kNone = 0xffffffffu;
+
+// 1-D coefficient rows of a block in LDS: [w e s n b t][kMaxN1]
+template <typename T>
+__device__ __forceinline__ void coef_to_lds(T *c, const T *aw, const T *ae,
+                                            const T *as, const T *an,
+                                            const T *ab, const T *at,
+                                            const Block &B) {
+    for (int e = threadIdx.x; e < 6 * kMaxN1; e += blockDim.x) {
+        const int r = e / kMaxN1, q = e % kMaxN1;
+        const int n = r < 2 ? B.ni : (r < 4 ? B.nj : B.nk);
+        if (q >= n) continue;
+        const T *src = r == 0 ? aw + B.o1x : r == 1 ? ae + B.o1x
+                     : r == 2 ? as + B.o1y : r == 3 ? an + B.o1y
+                     : r == 4 ? ab + B.o1z : at + B.o1z;
+        c[e] = src[q + 2];
+    }
+}
+
+// One hyperplane of a window: up to MS interior cells and one ghost entry
+// per lane (4 NI + 2 NJ <= 256 is enforced at init).
+template <typename T, int MS>
+struct WinReg {
+    uint32_t c[MS];
+    T v[MS];
+    uint8_t b[MS];
+    int gi;
+    T g;
+    uint8_t gb;
+};
+
+template <typename T, int MS, bool MASK>
+__device__ __forceinline__ void win_load(WinReg<T, MS> &w, int q, const Block &B,
+                                         int np, const int *sps,
+                                         const uint32_t *cells, const T *field,
+                                         const Window<T> &hw, Mask m) {
+    const int NI = B.ni, NJ = B.nj, NK = B.nk, HW = NJ + 2;
+    const bool inplane = q >= 0 && q < np;
+    const int base = inplane ? sps[q] : 0;
+    const int cnt = inplane ? sps[q + 1] - base : 0;
+#pragma unroll
+    for (int mm = 0; mm < MS; ++mm) {
+        const int x = threadIdx.x + mm * kThreads;
+        const bool ok = x < cnt;
+        const int s = base + (ok ? x : 0);
+        const uint32_t cv = cells[s];
+        const T fv = field[B.wo + s];
+        w.c[mm] = ok ? cv : kNone;
+        w.v[mm] = fv;
+        if (MASK) w.b[mm] = m.bp[B.wo + s];
+    }
+    // ghost entry (one index out of range by one)
+    const int e = threadIdx.x;
+    int idx = -1, fo = 0, ff = 0;
+    const T *src = nullptr;
+    int i, j, k;
+    if (e < NI) {
+        i = e; j = q + 1 - i;
+        if (j >= 0 && j < NJ) { idx = (j + 1) + HW * (i + 1); src = hw.zlo; fo = fidx_z(j, i, NI, NJ); ff = 4; }
+    } else if (e < 2 * NI) {
+        i = e - NI; j = q - NK - i;
+        if (j >= 0 && j < NJ) { idx = (j + 1) + HW * (i + 1); src = hw.zhi; fo = fidx_z(j, i, NI, NJ); ff = 5; }
+    } else if (e < 3 * NI) {
+        i = e - 2 * NI; k = q + 1 - i;
+        if (k >= 0 && k < NK) { idx = HW * (i + 1); src = hw.ylo; fo = fidx_y(k, i, NI, NK); ff = 2; }
+    } else if (e < 4 * NI) {
+        i = e - 3 * NI; k = q - NJ - i;
+        if (k >= 0 && k < NK) { idx = (NJ + 1) + HW * (i + 1); src = hw.yhi; fo = fidx_y(k, i, NI, NK); ff = 3; }
+    } else if (e < 4 * NI + NJ) {
+        j = e - 4 * NI; k = q + 1 - j;
+        if (k >= 0 && k < NK) { idx = j + 1; src = hw.xlo; fo = fidx_x(k, j, NJ, NK); ff = 0; }
+    } else if (e < 4 * NI + 2 * NJ) {
+        j = e - 4 * NI - NJ; k = q - NI - j;
+        if (k >= 0 && k < NK) { idx = (j + 1) + HW * (NI + 1); src = hw.xhi; fo = fidx_x(k, j, NJ, NK); ff = 1; }
+    }
+    const bool has = idx >= 0 && src != nullptr;
+    const T gv = *(has ? src + fo : field + B.wo);
+    w.gi = idx;
+    w.g = has ? gv : T(0);
+    if (MASK) {
+        const uint8_t *bf = m.bpf + B.fc;
+        const uint8_t bv = *(idx >= 0 ? bf + face_base(ff, NI, NJ, NK) + fo : bf);
+        w.gb = idx >= 0 ? bv : 0;
+    }
+}
+
+template <typename T, int MS, bool MASK>
+__device__ __forceinline__ void win_store(const WinReg<T, MS> &w, T *Hq,
+                                          uint8_t *Bq, int HW) {
+#pragma unroll
+    for (int mm = 0; mm < MS; ++mm)
+        if (w.c[mm] != kNone) {
+            int i, j, k;
+            unpack(w.c[mm], i, j, k);
+            Hq[(j + 1) + HW * (i + 1)] = w.v[mm];
+            if (MASK) Bq[(j + 1) + HW * (i + 1)] = w.b[mm];
+        }
+    if (w.gi >= 0) {
+        Hq[w.gi] = w.g;
+        if (MASK) Bq[w.gi] = w.gb;
+    }
+}
+
+// ghost value of a sweep's recurrence border, prefetched one step ahead
+template <typename T>
+struct Border {
+    T v;
+    int idx;
+};
+
+template <typename T>
+struct FwdSlot {
+    uint32_t c;
+    T rhs, lpr, lb, ls, lw, ap;
+};
+
 // ---------------------------------------------------------------------
 // Fused inner-iteration first half: RES = -(A HILF) (laplacephi_level),
 // then sipiter1 (src/flow/sip_hyperplane_mod.F90:450-505, classic order
@@ -529,25 +581,29 @@ __device__ __forceinline__ A laplace_cell(const T *Hm, const T *H0,
 // planes of this sweep go to rhi[1-rd] unless this is the last inner
 // iteration (the reference's conn(..., forward=-1) then skips faces
 // 1/3/5), the lower planes always go to rlo for the backward sweep.
-template <typename T, bool CLASSIC, bool MASKED>
+// LAP = false is the first inner iteration of an outer iteration, where
+// HILF = 0 and the Laplacian term vanishes.
+template <typename T, bool CLASSIC, bool MASKED, bool LAP, int MS>
 __global__ __launch_bounds__(kThreads) void k_forward(
-    Topo t, Fields<T> F, Mask m, int rd, int write_hi, int hilf_zero,
-    const int *done) {
+    Topo t, Fields<T> F, Mask m, int rd, int write_hi, const int *done) {
     if (*done) return;
     extern __shared__ __align__(16) unsigned char lds_fw[];
+    __shared__ int sps[kMaxPlanes + 1];
+    __shared__ T cf[6 * kMaxN1];
     const int b = t.blist[blockIdx.x];
-    const Block B = t.blocks[b];
-    const Shape S = t.shapes[B.shape];
+    const Block &B = t.blocks[b];
+    const Shape &S = t.shapes[B.shape];
     const int NI = B.ni, NJ = B.nj, NK = B.nk, NP = S.np;
     const int RW = NJ + 1, RS = (NJ + 1) * (NI + 1);
-    const int HW = NJ + 2, HS = (NJ + 2) * (NI + 2);
+    const int HW = NJ + 2;
     const int mRS = (t.max_nj + 1) * (t.max_ni + 1);
     const int mHS = (t.max_nj + 2) * (t.max_ni + 2);
     T *R = (T *)lds_fw;
     T *H = R + 2 * mRS;
     uint8_t *BH = (uint8_t *)(H + 4 * mHS);
     const uint32_t *cells = t.cells + S.cell_off;
-    const int *ps = t.pstart + S.pstart_off;
+    for (int e = threadIdx.x; e <= NP; e += blockDim.x)
+        sps[e] = t.pstart[S.pstart_off + e];
     const int64_t wo = B.wo;
     const T *rh = F.rhi[rd];
     T *rw = write_hi ? F.rhi[rd ^ 1] : nullptr;
@@ -555,103 +611,171 @@ __global__ __launch_bounds__(kThreads) void k_forward(
     const T *rylo = nbr_face(rh, t.blocks, B.nbr[2], 3, NI, NJ, NK);
     const T *rxlo = nbr_face(rh, t.blocks, B.nbr[0], 1, NI, NJ, NK);
     const Window<T> hw = hilf_window(F.hface, t.blocks, B);
-    const T *aw = F.aw + B.o1x + 2, *ae = F.ae + B.o1x + 2;
-    const T *as = F.as + B.o1y + 2, *an = F.an + B.o1y + 2;
-    const T *ab = F.ab + B.o1z + 2, *at = F.at + B.o1z + 2;
+    coef_to_lds(cf, F.aw, F.ae, F.as, F.an, F.ab, F.at, B);
+    const T *aw = cf, *ae = cf + kMaxN1, *as = cf + 2 * kMaxN1,
+            *an = cf + 3 * kMaxN1, *ab = cf + 4 * kMaxN1, *at = cf + 5 * kMaxN1;
     const T *lw = F.f[0] + wo, *ls = F.f[1] + wo, *lb = F.f[2] + wo,
             *lpr = F.f[3] + wo;
     const T *rhs = F.rhs + wo, *apv = F.ap + wo;
     T *res = F.res + wo;
+    const T *dummy = rhs;
     const int64_t fc = B.fc;
     const int fb0 = face_base(0, NI, NJ, NK), fb1 = face_base(1, NI, NJ, NK),
               fb2 = face_base(2, NI, NJ, NK), fb3 = face_base(3, NI, NJ, NK),
               fb4 = face_base(4, NI, NJ, NK), fb5 = face_base(5, NI, NJ, NK);
+    const bool masked = MASKED && !(B.flags & 1);
 
     // pencils start from the (lagged) ghost below k = 0
     for (int e = threadIdx.x; e < NJ * NI; e += blockDim.x) {
         const int j = e % NJ, i = e / NJ;
-        const T v = rzlo ? rzlo[j + NJ * i] : T(0);
+        const T v = rzlo ? rzlo[fidx_z(j, i, NI, NJ)] : T(0);
         R[(j + 1) + RW * (i + 1)] = v;
         R[RS + (j + 1) + RW * (i + 1)] = v;
     }
-    // borders j = -1 / i = -1 read at step pp from R[(pp-1)&1]
-    auto fill_border = [&](T *Rb, int pp) {
-        for (int e = threadIdx.x; e < NI + NJ; e += blockDim.x) {
-            if (e < NI) {
-                const int i = e, k = pp - i;
-                if (k >= 0 && k < NK)
-                    Rb[RW * (i + 1)] = rylo ? rylo[k + NK * i] : T(0);
-            } else {
-                const int j = e - NI, k = pp - j;
-                if (k >= 0 && k < NK) Rb[j + 1] = rxlo ? rxlo[k + NK * j] : T(0);
-            }
+    // border values j = -1 / i = -1 read at step pp from R[(pp-1)&1]
+    auto border_load = [&](int pp, Border<T> &bo) {
+        const int e = threadIdx.x;
+        int idx = -1;
+        const T *src = dummy;
+        if (e < NI) {
+            const int i = e, k = pp - i;
+            if (k >= 0 && k < NK) { idx = RW * (i + 1); if (rylo) src = rylo + fidx_y(k, i, NI, NK); }
+        } else if (e < NI + NJ) {
+            const int j = e - NI, k = pp - j;
+            if (k >= 0 && k < NK) { idx = j + 1; if (rxlo) src = rxlo + fidx_x(k, j, NJ, NK); }
+        }
+        const T v = *src;
+        bo.idx = idx;
+        bo.v = src != dummy ? v : T(0);
+    };
+    __syncthreads();  // sps, cf
+    auto load_slots = [&](int q, FwdSlot<T> (&d)[MS]) {
+        const bool ok = q < NP;
+        const int base = ok ? sps[q] : 0;
+        const int cnt = ok ? sps[q + 1] - base : 0;
+#pragma unroll
+        for (int mm = 0; mm < MS; ++mm) {
+            const int x = threadIdx.x + mm * kThreads;
+            const bool in = x < cnt;
+            const int s = base + (in ? x : 0);
+            const uint32_t cv = cells[s];
+            d[mm].rhs = rhs[s];
+            d[mm].lpr = lpr[s];
+            d[mm].lb = lb[s];
+            d[mm].ls = ls[s];
+            d[mm].lw = lw[s];
+            if (LAP) d[mm].ap = apv[s];
+            d[mm].c = in ? cv : kNone;
         }
     };
-    fill_border(R + RS, 0);
-    const bool lap = !hilf_zero;
-    const bool masked = MASKED && !(B.flags & 1);
-    if (lap) {
-        for (int q = -1; q <= 1; ++q) {
-            fill_plane(H + ((q + 1) & 3) * mHS, q, B, S, cells, ps, F.hilf, hw);
-            if (masked) fill_plane_bp(BH + ((q + 1) & 3) * mHS, q, B, S, cells, ps, m);
-        }
+    auto wload = [&](WinReg<T, MS> &w, int q) {
+        if (MASKED && masked) win_load<T, MS, true>(w, q, B, NP, sps, cells, F.hilf, hw, m);
+        else win_load<T, MS, false>(w, q, B, NP, sps, cells, F.hilf, hw, m);
+    };
+    auto wstore = [&](const WinReg<T, MS> &w, int q) {
+        const int sl = (q + 1) & 3;
+        if (MASKED && masked) win_store<T, MS, true>(w, H + sl * mHS, BH + sl * mHS, HW);
+        else win_store<T, MS, false>(w, H + sl * mHS, nullptr, HW);
+    };
+    {
+        Border<T> b0;
+        border_load(0, b0);
+        if (b0.idx >= 0) R[RS + b0.idx] = b0.v;
     }
-    __syncthreads();
-    (void)HS;
+    Border<T> ba;
+    border_load(1, ba);
+    WinReg<T, MS> wa;
+    if (LAP) {
+        for (int q = -1; q <= 1; ++q) {
+            wload(wa, q);
+            wstore(wa, q);
+        }
+        wload(wa, 2);
+    }
+    FwdSlot<T> sa[MS];
+    load_slots(0, sa);
+    lds_barrier();
 
+    // one step per iteration; the register rotation at its end waits for
+    // the prefetched loads, but keeps the kernel at 6-7 waves per SIMD
     for (int p = 0; p < NP; ++p) {
+        FwdSlot<T> sn[MS];
+        Border<T> bn;
+        WinReg<T, MS> wn;
+        load_slots(p + 1, sn);
+        border_load(p + 2, bn);
+        if (LAP) wload(wn, p + 3);
         const T *Rp = R + ((p - 1) & 1) * RS;
         T *Rc = R + (p & 1) * RS;
         const T *Hm = H + (p & 3) * mHS, *H0 = H + ((p + 1) & 3) * mHS,
                 *Hp = H + ((p + 2) & 3) * mHS;
         const uint8_t *Bm = BH + (p & 3) * mHS, *B0 = BH + ((p + 1) & 3) * mHS,
                       *Bp = BH + ((p + 2) & 3) * mHS;
-        const int cnt = ps[p + 1] - ps[p];
-        for (int x = threadIdx.x; x < cnt; x += blockDim.x) {
-            const int s = ps[p] + x;
+        const int base = sps[p];
+#pragma unroll
+        for (int mm = 0; mm < MS; ++mm) {
+            if (sa[mm].c == kNone) continue;
+            const int s = base + threadIdx.x + mm * kThreads;
             int i, j, k;
-            unpack(cells[s], i, j, k);
+            unpack(sa[mm].c, i, j, k);
             T r = T(0);
-            if (lap) {
+            if (LAP) {
                 const int c0 = (j + 1) + HW * (i + 1);
-                if (masked)
-                    r = laplace_cell<T, T, true>(Hm, H0, Hp, Bm, B0, Bp, c0, HW,
-                                                 aw[i], ae[i], as[j], an[j],
-                                                 ab[k], at[k], apv[s]);
+                if (MASKED && masked)
+                    r = laplace_cell<T, T, true>(Hm, H0, Hp, Bm, B0, Bp, c0, HW, aw[i], ae[i],
+                                                 as[j], an[j], ab[k], at[k], sa[mm].ap);
                 else
-                    r = laplace_cell<T, T, false>(Hm, H0, Hp, Bm, B0, Bp, c0,
-                                                  HW, aw[i], ae[i], as[j],
-                                                  an[j], ab[k], at[k], apv[s]);
+                    r = laplace_cell<T, T, false>(Hm, H0, Hp, Bm, B0, Bp, c0, HW, aw[i], ae[i],
+                                                  as[j], an[j], ab[k], at[k], sa[mm].ap);
             }
             const int c1 = (j + 1) + RW * (i + 1);
-            r = (rhs[s] + r) * lpr[s];
+            r = (sa[mm].rhs + r) * sa[mm].lpr;
             if (CLASSIC) {
-                r = r - lw[s] * Rp[c1 - RW];
-                r = r - ls[s] * Rp[c1 - 1];
-                r = r - lb[s] * Rp[c1];
+                r = r - sa[mm].lw * Rp[c1 - RW];
+                r = r - sa[mm].ls * Rp[c1 - 1];
+                r = r - sa[mm].lb * Rp[c1];
             } else {
-                r = r - lb[s] * Rp[c1];
-                r = r - ls[s] * Rp[c1 - 1];
-                r = r - lw[s] * Rp[c1 - RW];
+                r = r - sa[mm].lb * Rp[c1];
+                r = r - sa[mm].ls * Rp[c1 - 1];
+                r = r - sa[mm].lw * Rp[c1 - RW];
             }
             Rc[c1] = r;
             res[s] = r;
-            if (i == 0) F.rlo[fc + fb0 + k + NK * j] = r;
-            if (j == 0) F.rlo[fc + fb2 + k + NK * i] = r;
-            if (k == 0) F.rlo[fc + fb4 + j + NJ * i] = r;
+            if (i == 0) F.rlo[fc + fb0 + fidx_x(k, j, NJ, NK)] = r;
+            if (j == 0) F.rlo[fc + fb2 + fidx_y(k, i, NI, NK)] = r;
+            if (k == 0) F.rlo[fc + fb4 + fidx_z(j, i, NI, NJ)] = r;
             if (rw) {
-                if (i == NI - 1) rw[fc + fb1 + k + NK * j] = r;
-                if (j == NJ - 1) rw[fc + fb3 + k + NK * i] = r;
-                if (k == NK - 1) rw[fc + fb5 + j + NJ * i] = r;
+                if (i == NI - 1) rw[fc + fb1 + fidx_x(k, j, NJ, NK)] = r;
+                if (j == NJ - 1) rw[fc + fb3 + fidx_y(k, i, NI, NK)] = r;
+                if (k == NK - 1) rw[fc + fb5 + fidx_z(j, i, NI, NJ)] = r;
             }
         }
-        fill_border(Rc, p + 1);
-        if (lap && p + 2 <= NP) {
-            fill_plane(H + ((p + 3) & 3) * mHS, p + 2, B, S, cells, ps, F.hilf, hw);
-            if (masked) fill_plane_bp(BH + ((p + 3) & 3) * mHS, p + 2, B, S, cells, ps, m);
-        }
-        __syncthreads();
+        if (ba.idx >= 0) Rc[ba.idx] = ba.v;
+        if (LAP) wstore(wa, p + 2);
+        lds_barrier();
+#pragma unroll
+        for (int mm = 0; mm < MS; ++mm) sa[mm] = sn[mm];
+        wa = wn;
+        ba = bn;
     }
+}
+
+template <typename T, bool CLASSIC, bool MASKED, bool LAP>
+static void launch_fwd_ms(int ms, const Topo &t, const Fields<T> &f, Mask m,
+                          int rd, int write_hi, const int *done, size_t lds,
+                          hipStream_t s) {
+    switch (ms) {
+    case 1: k_forward<T, CLASSIC, MASKED, LAP, 1><<<t.nlist, kThreads, lds, s>>>(t, f, m, rd, write_hi, done); break;
+    case 2: k_forward<T, CLASSIC, MASKED, LAP, 2><<<t.nlist, kThreads, lds, s>>>(t, f, m, rd, write_hi, done); break;
+    case 3: k_forward<T, CLASSIC, MASKED, LAP, 3><<<t.nlist, kThreads, lds, s>>>(t, f, m, rd, write_hi, done); break;
+    default: k_forward<T, CLASSIC, MASKED, LAP, 4><<<t.nlist, kThreads, lds, s>>>(t, f, m, rd, write_hi, done); break;
+    }
+}
+
+// slots per lane: the largest hyperplane of any block shape / 256
+inline int slots_per_lane(const Topo &t) {
+    const int ms = (t.max_plane + kThreads - 1) / kThreads;
+    return ms < 1 ? 1 : ms;
 }
 
 template <typename T>
@@ -661,19 +785,19 @@ void launch_forward(const Topo &t, const Fields<T> &f, Mask m, int rd,
     const size_t mRS = (size_t)(t.max_nj + 1) * (t.max_ni + 1);
     const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
     const bool masked = m.bp != nullptr;
-    const size_t lds = 2 * mRS * sizeof(T) + 4 * mHS * sizeof(T) +
-                       (masked ? 4 * mHS : 0) + 16;
+    const size_t lds = 2 * mRS * sizeof(T) +
+                       (hilf_zero ? 0 : 4 * mHS * sizeof(T) + (masked ? 4 * mHS : 0)) + 16;
+    const int ms = slots_per_lane(t);
+    const bool lap = !hilf_zero;
+#define MGP_FWD(C, M, L) launch_fwd_ms<T, C, M, L>(ms, t, f, m, rd, write_hi, done, lds, s)
     if (classic) {
-        if (masked)
-            k_forward<T, true, true><<<t.nlist, kThreads, lds, s>>>(t, f, m, rd, write_hi, hilf_zero, done);
-        else
-            k_forward<T, true, false><<<t.nlist, kThreads, lds, s>>>(t, f, m, rd, write_hi, hilf_zero, done);
+        if (masked) { if (lap) MGP_FWD(true, true, true); else MGP_FWD(true, true, false); }
+        else { if (lap) MGP_FWD(true, false, true); else MGP_FWD(true, false, false); }
     } else {
-        if (masked)
-            k_forward<T, false, true><<<t.nlist, kThreads, lds, s>>>(t, f, m, rd, write_hi, hilf_zero, done);
-        else
-            k_forward<T, false, false><<<t.nlist, kThreads, lds, s>>>(t, f, m, rd, write_hi, hilf_zero, done);
+        if (masked) { if (lap) MGP_FWD(false, true, true); else MGP_FWD(false, true, false); }
+        else { if (lap) MGP_FWD(false, false, true); else MGP_FWD(false, false, false); }
     }
+#undef MGP_FWD
 }
 template void launch_forward<double>(const Topo &, const Fields<double> &,
                                      Mask, int, int, int, int, const int *,
@@ -683,25 +807,33 @@ template void launch_forward<float>(const Topo &, const Fields<float> &, Mask,
                                     hipStream_t);
 
 // ---------------------------------------------------------------------
+template <typename T>
+struct BwdSlot {
+    uint32_t c;
+    T res, ue, un, ut, h;
+};
+
 // Inner-iteration second half: sipiter2 (src/flow/sip_hyperplane_mod.F90:
 // 508-569; classic order src/flow/sip_classic_mod.F90:169-198), then
 // HILF += RES.  Upper ghosts of RES are the neighbours' lower planes of
 // this iteration's forward sweep.  Also writes HILF's boundary planes,
 // which stand in for the reference's conn(ilevel, 1, s1=dp) (:414).
-template <typename T, bool CLASSIC>
+// HZ: HILF is zero on entry (first inner iteration of an outer iteration).
+template <typename T, bool CLASSIC, bool HZ, int MS>
 __global__ __launch_bounds__(kThreads) void k_backward(Topo t, Fields<T> F,
-                                                       int hilf_zero,
                                                        const int *done) {
     if (*done) return;
     extern __shared__ __align__(16) unsigned char lds_bw[];
+    __shared__ int sps[kMaxPlanes + 1];
     const int b = t.blist[blockIdx.x];
-    const Block B = t.blocks[b];
-    const Shape S = t.shapes[B.shape];
+    const Block &B = t.blocks[b];
+    const Shape &S = t.shapes[B.shape];
     const int NI = B.ni, NJ = B.nj, NK = B.nk, NP = S.np;
     const int RW = NJ + 1, RS = (NJ + 1) * (NI + 1);
     T *R = (T *)lds_bw;
     const uint32_t *cells = t.cells + S.cell_off;
-    const int *ps = t.pstart + S.pstart_off;
+    for (int e = threadIdx.x; e <= NP; e += blockDim.x)
+        sps[e] = t.pstart[S.pstart_off + e];
     const int64_t wo = B.wo;
     const T *rzhi = nbr_face((const T *)F.rlo, t.blocks, B.nbr[5], 4, NI, NJ, NK);
     const T *ryhi = nbr_face((const T *)F.rlo, t.blocks, B.nbr[3], 2, NI, NJ, NK);
@@ -709,6 +841,7 @@ __global__ __launch_bounds__(kThreads) void k_backward(Topo t, Fields<T> F,
     const T *ue = F.f[4] + wo, *un = F.f[5] + wo, *ut = F.f[6] + wo;
     const T *res = F.res + wo;
     T *hilf = F.hilf + wo;
+    const T *dummy = res;
     const int64_t fc = B.fc;
     const int fb0 = face_base(0, NI, NJ, NK), fb1 = face_base(1, NI, NJ, NK),
               fb2 = face_base(2, NI, NJ, NK), fb3 = face_base(3, NI, NJ, NK),
@@ -716,58 +849,107 @@ __global__ __launch_bounds__(kThreads) void k_backward(Topo t, Fields<T> F,
 
     for (int e = threadIdx.x; e < NJ * NI; e += blockDim.x) {
         const int j = e % NJ, i = e / NJ;
-        const T v = rzhi ? rzhi[j + NJ * i] : T(0);
+        const T v = rzhi ? rzhi[fidx_z(j, i, NI, NJ)] : T(0);
         R[j + RW * i] = v;
         R[RS + j + RW * i] = v;
     }
-    // borders j = NJ / i = NI read at step pp from R[(pp+1)&1]
-    auto fill_border = [&](T *Rb, int pp) {
-        for (int e = threadIdx.x; e < NI + NJ; e += blockDim.x) {
-            if (e < NI) {
-                const int i = e, k = pp - (NJ - 1) - i;
-                if (k >= 0 && k < NK)
-                    Rb[NJ + RW * i] = ryhi ? ryhi[k + NK * i] : T(0);
-            } else {
-                const int j = e - NI, k = pp - j - (NI - 1);
-                if (k >= 0 && k < NK)
-                    Rb[j + RW * NI] = rxhi ? rxhi[k + NK * j] : T(0);
-            }
+    // border values j = NJ / i = NI read at step pp from R[(pp+1)&1]
+    auto border_load = [&](int pp, Border<T> &bo) {
+        const int e = threadIdx.x;
+        int idx = -1;
+        const T *src = dummy;
+        if (e < NI) {
+            const int i = e, k = pp - (NJ - 1) - i;
+            if (k >= 0 && k < NK) { idx = NJ + RW * i; if (ryhi) src = ryhi + fidx_y(k, i, NI, NK); }
+        } else if (e < NI + NJ) {
+            const int j = e - NI, k = pp - j - (NI - 1);
+            if (k >= 0 && k < NK) { idx = j + RW * NI; if (rxhi) src = rxhi + fidx_x(k, j, NJ, NK); }
+        }
+        const T v = *src;
+        bo.idx = idx;
+        bo.v = src != dummy ? v : T(0);
+    };
+    __syncthreads();  // sps
+    auto load_slots = [&](int q, BwdSlot<T> (&d)[MS]) {
+        const bool ok = q >= 0;
+        const int base = ok ? sps[q] : 0;
+        const int cnt = ok ? sps[q + 1] - base : 0;
+#pragma unroll
+        for (int mm = 0; mm < MS; ++mm) {
+            const int x = threadIdx.x + mm * kThreads;
+            const bool in = x < cnt;
+            const int s = base + (in ? x : 0);
+            const uint32_t cv = cells[s];
+            d[mm].res = res[s];
+            d[mm].ue = ue[s];
+            d[mm].un = un[s];
+            d[mm].ut = ut[s];
+            if (!HZ) d[mm].h = hilf[s];
+            d[mm].c = in ? cv : kNone;
         }
     };
-    fill_border(R + (NP & 1) * RS, NP - 1);
-    __syncthreads();
+    {
+        Border<T> b0;
+        border_load(NP - 1, b0);
+        if (b0.idx >= 0) R[(NP & 1) * RS + b0.idx] = b0.v;
+    }
+    Border<T> ba, bb;
+    border_load(NP - 2, ba);
+    BwdSlot<T> sa[MS], sb[MS];
+    load_slots(NP - 1, sa);
+    lds_barrier();
 
-    for (int p = NP - 1; p >= 0; --p) {
+    auto step = [&](int p, BwdSlot<T> (&cur)[MS], BwdSlot<T> (&nxt)[MS],
+                    Border<T> &bcur, Border<T> &bnxt) {
+        load_slots(p - 1, nxt);
+        border_load(p - 2, bnxt);
         const T *Rp = R + ((p + 1) & 1) * RS;
         T *Rc = R + (p & 1) * RS;
-        const int cnt = ps[p + 1] - ps[p];
-        for (int x = threadIdx.x; x < cnt; x += blockDim.x) {
-            const int s = ps[p] + x;
+        const int base = sps[p];
+#pragma unroll
+        for (int mm = 0; mm < MS; ++mm) {
+            if (cur[mm].c == kNone) continue;
+            const int s = base + threadIdx.x + mm * kThreads;
             int i, j, k;
-            unpack(cells[s], i, j, k);
+            unpack(cur[mm].c, i, j, k);
             const int c1 = j + RW * i;
-            T r = res[s];
+            T r = cur[mm].res;
             if (CLASSIC) {
-                r = r - un[s] * Rp[c1 + 1];
-                r = r - ue[s] * Rp[c1 + RW];
-                r = r - ut[s] * Rp[c1];
+                r = r - cur[mm].un * Rp[c1 + 1];
+                r = r - cur[mm].ue * Rp[c1 + RW];
+                r = r - cur[mm].ut * Rp[c1];
             } else {
-                r = r - ut[s] * Rp[c1];
-                r = r - un[s] * Rp[c1 + 1];
-                r = r - ue[s] * Rp[c1 + RW];
+                r = r - cur[mm].ut * Rp[c1];
+                r = r - cur[mm].un * Rp[c1 + 1];
+                r = r - cur[mm].ue * Rp[c1 + RW];
             }
             Rc[c1] = r;
-            const T h = (hilf_zero ? T(0) : hilf[s]) + r;
+            const T h = (HZ ? T(0) : cur[mm].h) + r;
             hilf[s] = h;
-            if (i == 0) F.hface[fc + fb0 + k + NK * j] = h;
-            if (i == NI - 1) F.hface[fc + fb1 + k + NK * j] = h;
-            if (j == 0) F.hface[fc + fb2 + k + NK * i] = h;
-            if (j == NJ - 1) F.hface[fc + fb3 + k + NK * i] = h;
-            if (k == 0) F.hface[fc + fb4 + j + NJ * i] = h;
-            if (k == NK - 1) F.hface[fc + fb5 + j + NJ * i] = h;
+            if (i == 0) F.hface[fc + fb0 + fidx_x(k, j, NJ, NK)] = h;
+            if (i == NI - 1) F.hface[fc + fb1 + fidx_x(k, j, NJ, NK)] = h;
+            if (j == 0) F.hface[fc + fb2 + fidx_y(k, i, NI, NK)] = h;
+            if (j == NJ - 1) F.hface[fc + fb3 + fidx_y(k, i, NI, NK)] = h;
+            if (k == 0) F.hface[fc + fb4 + fidx_z(j, i, NI, NJ)] = h;
+            if (k == NK - 1) F.hface[fc + fb5 + fidx_z(j, i, NI, NJ)] = h;
         }
-        if (p > 0) fill_border(Rc, p - 1);
-        __syncthreads();
+        if (bcur.idx >= 0) Rc[bcur.idx] = bcur.v;
+        lds_barrier();
+    };
+    for (int p = NP - 1; p >= 0; p -= 2) {
+        step(p, sa, sb, ba, bb);
+        if (p - 1 >= 0) step(p - 1, sb, sa, bb, ba);
+    }
+}
+
+template <typename T, bool CLASSIC, bool HZ>
+static void launch_bwd_ms(int ms, const Topo &t, const Fields<T> &f,
+                          const int *done, size_t lds, hipStream_t s) {
+    switch (ms) {
+    case 1: k_backward<T, CLASSIC, HZ, 1><<<t.nlist, kThreads, lds, s>>>(t, f, done); break;
+    case 2: k_backward<T, CLASSIC, HZ, 2><<<t.nlist, kThreads, lds, s>>>(t, f, done); break;
+    case 3: k_backward<T, CLASSIC, HZ, 3><<<t.nlist, kThreads, lds, s>>>(t, f, done); break;
+    default: k_backward<T, CLASSIC, HZ, 4><<<t.nlist, kThreads, lds, s>>>(t, f, done); break;
     }
 }
 
@@ -776,10 +958,14 @@ void launch_backward(const Topo &t, const Fields<T> &f, int hilf_zero,
                      int classic, const int *done, hipStream_t s) {
     const size_t mRS = (size_t)(t.max_nj + 1) * (t.max_ni + 1);
     const size_t lds = 2 * mRS * sizeof(T) + 16;
-    if (classic)
-        k_backward<T, true><<<t.nlist, kThreads, lds, s>>>(t, f, hilf_zero, done);
-    else
-        k_backward<T, false><<<t.nlist, kThreads, lds, s>>>(t, f, hilf_zero, done);
+    const int ms = slots_per_lane(t);
+    if (classic) {
+        if (hilf_zero) launch_bwd_ms<T, true, true>(ms, t, f, done, lds, s);
+        else launch_bwd_ms<T, true, false>(ms, t, f, done, lds, s);
+    } else {
+        if (hilf_zero) launch_bwd_ms<T, false, true>(ms, t, f, done, lds, s);
+        else launch_bwd_ms<T, false, false>(ms, t, f, done, lds, s);
+    }
 }
 template void launch_backward<double>(const Topo &, const Fields<double> &,
                                       int, int, const int *, hipStream_t);
@@ -787,71 +973,124 @@ template void launch_backward<float>(const Topo &, const Fields<float> &, int,
                                      int, const int *, hipStream_t);
 
 // ---------------------------------------------------------------------
+struct OutSlot {
+    uint32_t c;
+    double ap, rhs, dp;
+};
+
 // FP64 outer update (src/flow/pressuresolver_mod.F90:258-279): RES =
 // -(A HILF) with the FP64 operator, RHS += RES, max |RHS|, DP += HILF.
 // In the mixed-precision mode the sweeps' RHS copy is refreshed here.
-template <typename T, bool MASKED>
+template <typename T, bool MASKED, int MS>
 __global__ __launch_bounds__(kThreads) void k_outer(Topo t, Fields<T> F,
                                                     Mask m, Outer o,
                                                     T *rhs_t) {
     if (*o.done) return;
     extern __shared__ __align__(16) unsigned char lds_ou[];
+    __shared__ int sps[kMaxPlanes + 1];
+    __shared__ double cf[6 * kMaxN1];
     const int b = t.blist[blockIdx.x];
-    const Block B = t.blocks[b];
-    const Shape S = t.shapes[B.shape];
+    const Block &B = t.blocks[b];
+    const Shape &S = t.shapes[B.shape];
     const int NJ = B.nj, NP = S.np, HW = NJ + 2;
     const int mHS = (t.max_nj + 2) * (t.max_ni + 2);
     T *H = (T *)lds_ou;
     uint8_t *BH = (uint8_t *)(H + 4 * mHS);
     const uint32_t *cells = t.cells + S.cell_off;
-    const int *ps = t.pstart + S.pstart_off;
+    for (int e = threadIdx.x; e <= NP; e += blockDim.x)
+        sps[e] = t.pstart[S.pstart_off + e];
     const int64_t wo = B.wo;
     const Window<T> hw = hilf_window(F.hface, t.blocks, B);
-    const double *aw = o.aw + B.o1x + 2, *ae = o.ae + B.o1x + 2;
-    const double *as = o.as + B.o1y + 2, *an = o.an + B.o1y + 2;
-    const double *ab = o.ab + B.o1z + 2, *at = o.at + B.o1z + 2;
+    coef_to_lds(cf, o.aw, o.ae, o.as, o.an, o.ab, o.at, B);
+    const double *aw = cf, *ae = cf + kMaxN1, *as = cf + 2 * kMaxN1,
+                 *an = cf + 3 * kMaxN1, *ab = cf + 4 * kMaxN1, *at = cf + 5 * kMaxN1;
     const bool masked = MASKED && !(B.flags & 1);
     double mx = 0.0;
+    __syncthreads();  // sps, cf
+    auto load_slots = [&](int q, OutSlot (&d)[MS]) {
+        const bool ok = q < NP;
+        const int base = ok ? sps[q] : 0;
+        const int cnt = ok ? sps[q + 1] - base : 0;
+#pragma unroll
+        for (int mm = 0; mm < MS; ++mm) {
+            const int x = threadIdx.x + mm * kThreads;
+            const bool in = x < cnt;
+            const int64_t s = wo + base + (in ? x : 0);
+            const uint32_t cv = cells[base + (in ? x : 0)];
+            d[mm].ap = o.ap64[s];
+            d[mm].rhs = o.rhs64[s];
+            d[mm].dp = o.dp64[s];
+            d[mm].c = in ? cv : kNone;
+        }
+    };
+    auto wload = [&](WinReg<T, MS> &w, int q) {
+        if (MASKED && masked) win_load<T, MS, true>(w, q, B, NP, sps, cells, (const T *)F.hilf, hw, m);
+        else win_load<T, MS, false>(w, q, B, NP, sps, cells, (const T *)F.hilf, hw, m);
+    };
+    auto wstore = [&](const WinReg<T, MS> &w, int q) {
+        const int sl = (q + 1) & 3;
+        if (MASKED && masked) win_store<T, MS, true>(w, H + sl * mHS, BH + sl * mHS, HW);
+        else win_store<T, MS, false>(w, H + sl * mHS, nullptr, HW);
+    };
+    WinReg<T, MS> wa;
     for (int q = -1; q <= 1; ++q) {
-        fill_plane(H + ((q + 1) & 3) * mHS, q, B, S, cells, ps, (const T *)F.hilf, hw);
-        if (masked) fill_plane_bp(BH + ((q + 1) & 3) * mHS, q, B, S, cells, ps, m);
+        wload(wa, q);
+        wstore(wa, q);
     }
-    __syncthreads();
+    wload(wa, 2);
+    OutSlot sa[MS];
+    load_slots(0, sa);
+    lds_barrier();
     for (int p = 0; p < NP; ++p) {
+        OutSlot sn[MS];
+        WinReg<T, MS> wn;
+        load_slots(p + 1, sn);
+        wload(wn, p + 3);
         const T *Hm = H + (p & 3) * mHS, *H0 = H + ((p + 1) & 3) * mHS,
                 *Hp = H + ((p + 2) & 3) * mHS;
         const uint8_t *Bm = BH + (p & 3) * mHS, *B0 = BH + ((p + 1) & 3) * mHS,
                       *Bp = BH + ((p + 2) & 3) * mHS;
-        const int cnt = ps[p + 1] - ps[p];
-        for (int x = threadIdx.x; x < cnt; x += blockDim.x) {
-            const int s = ps[p] + x;
+        const int base = sps[p];
+#pragma unroll
+        for (int mm = 0; mm < MS; ++mm) {
+            if (sa[mm].c == kNone) continue;
+            const int64_t s = wo + base + threadIdx.x + mm * kThreads;
             int i, j, k;
-            unpack(cells[s], i, j, k);
+            unpack(sa[mm].c, i, j, k);
             const int c0 = (j + 1) + HW * (i + 1);
             double r;
-            if (masked)
-                r = laplace_cell<T, double, true>(Hm, H0, Hp, Bm, B0, Bp, c0, HW,
-                                                  aw[i], ae[i], as[j], an[j],
-                                                  ab[k], at[k], o.ap64[wo + s]);
+            if (MASKED && masked)
+                r = laplace_cell<T, double, true>(Hm, H0, Hp, Bm, B0, Bp, c0, HW, aw[i], ae[i],
+                                                  as[j], an[j], ab[k], at[k], sa[mm].ap);
             else
-                r = laplace_cell<T, double, false>(Hm, H0, Hp, Bm, B0, Bp, c0, HW,
-                                                   aw[i], ae[i], as[j], an[j],
-                                                   ab[k], at[k], o.ap64[wo + s]);
-            const double rn = o.rhs64[wo + s] + r;
-            o.rhs64[wo + s] = rn;
-            if (rhs_t) rhs_t[wo + s] = (T)rn;
+                r = laplace_cell<T, double, false>(Hm, H0, Hp, Bm, B0, Bp, c0, HW, aw[i], ae[i],
+                                                   as[j], an[j], ab[k], at[k], sa[mm].ap);
+            const double rn = sa[mm].rhs + r;
+            o.rhs64[s] = rn;
+            if (rhs_t) rhs_t[s] = (T)rn;
             const double a = fabs(rn);
             mx = (a > mx || a != a) ? a : mx;
-            o.dp64[wo + s] = o.dp64[wo + s] + (double)H0[c0];
+            o.dp64[s] = sa[mm].dp + (double)H0[c0];
         }
-        if (p + 2 <= NP) {
-            fill_plane(H + ((p + 3) & 3) * mHS, p + 2, B, S, cells, ps, (const T *)F.hilf, hw);
-            if (masked) fill_plane_bp(BH + ((p + 3) & 3) * mHS, p + 2, B, S, cells, ps, m);
-        }
-        __syncthreads();
+        wstore(wa, p + 2);
+        lds_barrier();
+#pragma unroll
+        for (int mm = 0; mm < MS; ++mm) sa[mm] = sn[mm];
+        wa = wn;
     }
     mx = mx * ldexp(1.0, t.maxlevel - B.level);
     block_max_to_global(mx, o.gmax);
+}
+
+template <typename T, bool MASKED>
+static void launch_out_ms(int ms, const Topo &t, const Fields<T> &f, Mask m,
+                          Outer o, T *rhs_t, size_t lds, hipStream_t s) {
+    switch (ms) {
+    case 1: k_outer<T, MASKED, 1><<<t.nlist, kThreads, lds, s>>>(t, f, m, o, rhs_t); break;
+    case 2: k_outer<T, MASKED, 2><<<t.nlist, kThreads, lds, s>>>(t, f, m, o, rhs_t); break;
+    case 3: k_outer<T, MASKED, 3><<<t.nlist, kThreads, lds, s>>>(t, f, m, o, rhs_t); break;
+    default: k_outer<T, MASKED, 4><<<t.nlist, kThreads, lds, s>>>(t, f, m, o, rhs_t); break;
+    }
 }
 
 template <typename T>
@@ -860,10 +1099,9 @@ void launch_outer(const Topo &t, const Fields<T> &f, Mask m, Outer o,
     const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
     const bool masked = m.bp != nullptr;
     const size_t lds = 4 * mHS * sizeof(T) + (masked ? 4 * mHS : 0) + 16;
-    if (masked)
-        k_outer<T, true><<<t.nlist, kThreads, lds, s>>>(t, f, m, o, rhs_t);
-    else
-        k_outer<T, false><<<t.nlist, kThreads, lds, s>>>(t, f, m, o, rhs_t);
+    const int ms = slots_per_lane(t);
+    if (masked) launch_out_ms<T, true>(ms, t, f, m, o, rhs_t, lds, s);
+    else launch_out_ms<T, false>(ms, t, f, m, o, rhs_t, lds, s);
 }
 template void launch_outer<double>(const Topo &, const Fields<double> &, Mask,
                                    Outer, double *, hipStream_t);
@@ -903,8 +1141,8 @@ void launch_check(Outer o, int nouter, int nouter_min, double epcorr,
 template <typename T>
 __global__ void k_wo_to_sp(Topo t, const T *src, double *dst) {
     const int b = t.blist[blockIdx.y];
-    const Block B = t.blocks[b];
-    const Shape S = t.shapes[B.shape];
+    const Block &B = t.blocks[b];
+    const Shape &S = t.shapes[B.shape];
     const uint32_t *cells = t.cells + S.cell_off;
     const int64_t kk = B.nk + 4, jj = B.nj + 4;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S.ncell;
@@ -936,8 +1174,8 @@ template <typename T>
 __global__ void k_to_ref_factors(Topo t, const T *wo, double *ref,
                                  int spatial) {
     const int b = t.blist[blockIdx.y];
-    const Block B = t.blocks[b];
-    const Shape S = t.shapes[B.shape];
+    const Block &B = t.blocks[b];
+    const Shape &S = t.shapes[B.shape];
     const uint32_t *cells = t.cells + S.cell_off;
     const int64_t kk = B.nk + 4, jj = B.nj + 4;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S.ncell;
@@ -969,7 +1207,7 @@ __global__ void k_mgpcorr(Topo t, double *u_, double *v_, double *w_,
                           const double *rdy_, const double *rdz_,
                           double rfak, const double *bp_) {
     const int b = t.blist[blockIdx.y];
-    const Block B = t.blocks[b];
+    const Block &B = t.blocks[b];
     const int kk = B.nk + 4, jj = B.nj + 4, ii = B.ni + 4;
     const int64_t n = (int64_t)kk * jj * ii;
     double *u = u_ + B.sp, *v = v_ + B.sp, *w = w_ + B.sp, *p = p_ + B.sp;

@@ -17,6 +17,7 @@ struct Topo {
     const int *pstart;     // plane starts per shape
     const int *slot_of;    // spatial (k + NK(j + NJ i)) -> WO slot
     int max_ni, max_nj, max_nk;
+    int max_plane;         // largest hyperplane over all shapes
     int maxlevel;
 };
 
