@@ -191,6 +191,14 @@ def main():
     fw_ms = r["kernels"]["forward_ms"]
     alg = ALG_BYTES[main_form]["forward"] * cells
     achieved = alg / (fw_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic_sip_32768x24.json"
+    if tfile.exists() and nb == (32, 32, 32) and n == (24, 24, 24):
+        key = (f"void mgp::k_forward<{'float' if main_form == 'fp32' else 'double'}, "
+               f"false, false, true, 2>")
+        per = json.loads(tfile.read_text())["per_launch"]
+        if key in per:
+            traffic = round(per[key]["total_bytes"] / 1e9, 3)
     line = {
         "metric": "GCell-updates/s + time-to-1e-10 residual, FP32-refined vs FP64, 1 MI355X",
         "value": round(value, 3),
@@ -212,7 +220,8 @@ def main():
                   for f in res},
         "roofline": {"bound": "hbm", "kernel": f"k_forward ({main_form})",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_unit": "GB per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
                      "alg_bytes_per_cell": ALG_BYTES[main_form]["forward"]},
     }
     if "fp32" in res and "fp64" in res:

@@ -509,6 +509,8 @@ int check_cfg(const mgp_config &c) {
     if (c.precision < 0 || c.precision > 1) return fail(MGP_ERR_ARG, "invalid precision %d", c.precision);
     if (c.ibtype < 0 || c.ibtype > 1) return fail(MGP_ERR_ARG, "invalid ibtype %d", c.ibtype);
     if (!(c.rho > 0.0)) return fail(MGP_ERR_ARG, "invalid rho %g", c.rho);
+    if (c.sor_all_levels)
+        return fail(MGP_ERR_ARG, "sor_all_levels is not supported by this build");
     return MGP_OK;
 }
 
