@@ -160,6 +160,12 @@ int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
  * ms[0] forward sweep, ms[1] backward sweep, ms[2] one outer update. */
 int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms);
 
+/* Level-weighted max |RHS| after each outer iteration of the last mgpoisl
+ * call (the quantity of the exit test, src/flow/pressuresolver_mod.F90:
+ * 274-301); writes min(count, cap) values and the count. */
+int mgp_get_history(mgp_handle *h, mgletreal *out, mgletint cap,
+                    mgletint *count);
+
 /* Number of interior cells and bytes of device memory held by h. */
 int mgp_info(mgp_handle *h, int64_t *ncells, int64_t *device_bytes);
 

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -136,6 +138,8 @@ struct mgp_handle {
     DBuf d_ap64, d_apT, d_f[7], d_hilf, d_rhs64, d_rhsT, d_res, d_dp64;
     DBuf d_bpw, d_bpf, d_bpsp;
     DBuf d_hface, d_rhi[2], d_rlo;
+    // SOR: SOR_RAP and the second HILF face buffer (frozen ghosts)
+    DBuf d_rapT, d_hface2;
     // reference-layout work arrays
     DBuf d_u, d_v, d_w, d_p, d_dpsp;
     // scalars and history
@@ -224,6 +228,7 @@ struct mgp_handle {
         const T **c[6] = {&F.aw, &F.ae, &F.as, &F.an, &F.ab, &F.at};
         for (int q = 0; q < 6; ++q)
             *c[q] = fp32() ? d_cT[q].as<T>() : d_c64[q].as<T>();
+        F.rap = d_rapT.as<T>();
         F.hface = d_hface.as<T>();
         F.rhi[0] = d_rhi[0].as<T>();
         F.rhi[1] = d_rhi[1].as<T>();
@@ -509,8 +514,6 @@ int check_cfg(const mgp_config &c) {
     if (c.precision < 0 || c.precision > 1) return fail(MGP_ERR_ARG, "invalid precision %d", c.precision);
     if (c.ibtype < 0 || c.ibtype > 1) return fail(MGP_ERR_ARG, "invalid ibtype %d", c.ibtype);
     if (!(c.rho > 0.0)) return fail(MGP_ERR_ARG, "invalid rho %g", c.rho);
-    if (c.sor_all_levels)
-        return fail(MGP_ERR_ARG, "sor_all_levels is not supported by this build");
     return MGP_OK;
 }
 
@@ -737,6 +740,10 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
         h->d_rhi[0].alloc(h->nface * tsz);
         h->d_rhi[1].alloc(h->nface * tsz);
         h->d_rlo.alloc(h->nface * tsz);
+        if (h->cfg.ityp == 1) {
+            h->d_rapT.alloc(h->ncell * tsz);
+            h->d_hface2.alloc(h->nface * tsz);
+        }
         h->d_scal.alloc(64);
         h->d_never.alloc(64);
         HC(hipMemsetAsync(h->d_never.p, 0, 64, h->stream));
@@ -756,6 +763,11 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
             launch_convert<float>(h->d_ap64.as<double>(), h->d_apT.as<float>(), h->ncell, h->stream);
         } else {
             setup_factors<double>(*h);
+        }
+        if (h->cfg.ityp == 1) {
+            if (h->fp32()) launch_rap<float>(h->d_ap64.as<double>(), h->d_rapT.as<float>(), h->ncell, h->stream);
+            else launch_rap<double>(h->d_ap64.as<double>(), h->d_rapT.as<double>(), h->ncell, h->stream);
+            HC(hipGetLastError());
         }
         HC(hipStreamSynchronize(h->stream));
     } catch (const HipError &e) {
@@ -782,6 +794,7 @@ int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
     launch_fill<T>(h.d_rhi[0].as<T>(), h.nface, T(0), s);
     launch_fill<T>(h.d_rhi[1].as<T>(), h.nface, T(0), s);
     launch_fill<T>(h.d_rlo.as<T>(), h.nface, T(0), s);
+    if (h.d_hface2.p) launch_fill<T>(h.d_hface2.as<T>(), h.nface, T(0), s);
     launch_divcal<T>(tall, u, v, w, h.d_rddx.as<double>(), h.d_rddy.as<double>(),
                      h.d_rddz.as<double>(), c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr,
                      prefak, h.d_rhs64.as<double>(), rhsT, s);
@@ -805,6 +818,18 @@ int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
             for (int it = 0; it < chunk; ++it) {
                 for (auto &kv : h.level_blocks) {
                     Topo t = h.topo(kv.first);
+                    // mgpoisit (src/flow/pressuresolver_mod.F90:403-414):
+                    // SOR on levels above the coarsest one
+                    if (c.ityp == 1 && (kv.first > h.minlevel || c.sor_all_levels)) {
+                        // the last relaxation writes the faces into
+                        // F.hface, where the SIP levels leave theirs too
+                        T *hf[2] = {F.hface, h.d_hface2.as<T>()};
+                        for (int il = 1; il <= c.ninner; ++il)
+                            launch_sor<T>(t, F, m, hf[(c.ninner - il + 1) & 1],
+                                          hf[(c.ninner - il) & 1], c.omega, il == 1,
+                                          o.done, s);
+                        continue;
+                    }
                     for (int il = 1; il <= c.ninner; ++il) {
                         const int last = il == c.ninner;
                         launch_forward<T>(t, F, m, rd, !last, il == 1, classic, o.done, s);
@@ -828,15 +853,30 @@ int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
         } else {
             (void)hipGetLastError();
         }
-        int hdone = 0;
+        // optional progress line on stderr (MGP_PROGRESS_SECONDS > 0)
+        const char *pe = getenv("MGP_PROGRESS_SECONDS");
+        const double every = pe ? atof(pe) : 0.0;
+        auto t_last = std::chrono::steady_clock::now();
+        int hstate[2] = {0, 0};   // outer iterations done, done flag
         while (launched < c.nouter) {
             if (exec) HC(hipGraphLaunch(exec, s));
             else enqueue_chunk();
             launched += chunk;
             if (launched >= c.nouter) break;
-            HC(hipMemcpyAsync(&hdone, o.done, sizeof(int), hipMemcpyDeviceToHost, s));
+            HC(hipMemcpyAsync(hstate, o.ctr, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
             HC(hipStreamSynchronize(s));
-            if (hdone) break;
+            if (every > 0.0) {
+                const auto now = std::chrono::steady_clock::now();
+                if (std::chrono::duration<double>(now - t_last).count() >= every) {
+                    double mr[2];
+                    HC(hipMemcpy(mr, o.maxrhs0, 2 * sizeof(double), hipMemcpyDeviceToHost));
+                    fprintf(stderr, "[mgp] outer %d (inner %d): max|RHS| %.3e, relative %.3e\n",
+                            hstate[0], hstate[0] * c.ninner, mr[1], mr[1] / mr[0]);
+                    fflush(stderr);
+                    t_last = now;
+                }
+            }
+            if (hstate[1]) break;
         }
         if (exec) HC(hipGraphExecDestroy(exec));
         if (graph) HC(hipGraphDestroy(graph));
@@ -1131,6 +1171,22 @@ int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms) {
     return MGP_OK;
 }
 
+int mgp_get_history(mgp_handle *h, mgletreal *out, mgletint cap,
+                    mgletint *count) {
+    if (!h || !out || !count || cap < 0) return fail(MGP_ERR_ARG, "bad argument");
+    if (!h->have_solution) return fail(MGP_ERR_STATE, "no mgpoisl call yet");
+    try {
+        int n = 0;
+        HC(hipMemcpy(&n, (char *)h->d_scal.p + 24, sizeof(int), hipMemcpyDeviceToHost));
+        n = std::min(n, std::min((int)cap, std::max(1, h->cfg.nouter)));
+        if (n > 0) HC(hipMemcpy(out, h->d_hist.p, n * sizeof(double), hipMemcpyDeviceToHost));
+        *count = n;
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    return MGP_OK;
+}
+
 int mgp_info(mgp_handle *h, int64_t *ncells, int64_t *device_bytes) {
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     if (ncells) *ncells = h->ncell;
@@ -1139,7 +1195,7 @@ int mgp_info(mgp_handle *h, int64_t *ncells, int64_t *device_bytes) {
         const DBuf *bs[] = {&h->d_blocks, &h->d_shapes, &h->d_cells, &h->d_pstart, &h->d_slot,
                             &h->d_ap64, &h->d_apT, &h->d_hilf, &h->d_rhs64, &h->d_rhsT, &h->d_res,
                             &h->d_dp64, &h->d_bpw, &h->d_bpf, &h->d_bpsp, &h->d_hface, &h->d_rhi[0],
-                            &h->d_rhi[1], &h->d_rlo, &h->d_u, &h->d_v, &h->d_w, &h->d_p, &h->d_dpsp};
+                            &h->d_rhi[1], &h->d_rlo, &h->d_rapT, &h->d_hface2, &h->d_u, &h->d_v, &h->d_w, &h->d_p, &h->d_dpsp};
         for (auto *b : bs) tot += b->n;
         for (auto &b : h->d_f) tot += b.n;
         *device_bytes = (int64_t)tot;

@@ -973,6 +973,218 @@ template void launch_backward<float>(const Topo &, const Fields<float> &, int,
                                      int, const int *, hipStream_t);
 
 // ---------------------------------------------------------------------
+// SOR_RAP = divide0(1, AP): src/flow/sor_mod.F90:18-49,
+// src/core/simdfunctions_mod.F90:70-82.
+template <typename T>
+__global__ void k_rap(const double *ap, T *rap, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const double a = ap[e];
+        rap[e] = (T)(a == 0.0 ? 0.0 : 1.0 / a);
+    }
+}
+
+template <typename T>
+void launch_rap(const double *ap, T *rap, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    k_rap<T><<<(unsigned)nb, 256, 0, s>>>(ap, rap, n);
+}
+template void launch_rap<double>(const double *, double *, int64_t, hipStream_t);
+template void launch_rap<float>(const double *, float *, int64_t, hipStream_t);
+
+template <typename T>
+struct SorSlot {
+    uint32_t c;
+    T rhs, rap;
+};
+
+constexpr int kSorRing = 8;  // window planes p-3 .. p+2 live in LDS
+
+// Red-black SOR of HILF (relax, src/flow/sor_mod.F90:102-203), both colours
+// in one pass over the block's hyperplanes.  The reference's first colour
+// (rb = 0: i+j+k even in its 1-based indices, :124-130) is exactly the set
+// of odd planes p = i+j+k of the 0-based interior, the second colour the
+// even planes, and every 7-point neighbour of a cell lies on plane p-1 or
+// p+1.  Step p relaxes plane p when p is odd (its neighbours are old even
+// planes) and plane p-2 when p is even (its neighbours p-3, p-1 were
+// relaxed at earlier steps), so each value is read and written once.
+// Ghost values stay those of the previous exchange (fin) for both colours,
+// as in the reference, where conn(DP) follows relax (:414); the new
+// boundary planes go to fout.  With an odd cell count along k the loop
+// bounds of :124-130 never reach the cells k = nk-1; they keep their value.
+// HZ: HILF is zero on entry (first inner iteration of an outer iteration).
+template <typename T, bool MASKED, bool HZ, int MS>
+__global__ __launch_bounds__(kThreads) void k_sor(Topo t, Fields<T> F, Mask m,
+                                                  const T *fin, T *fout,
+                                                  double omega,
+                                                  const int *done) {
+    if (*done) return;
+    extern __shared__ __align__(16) unsigned char lds_so[];
+    __shared__ int sps[kMaxPlanes + 1];
+    __shared__ T cf[6 * kMaxN1];
+    const int b = t.blist[blockIdx.x];
+    const Block &B = t.blocks[b];
+    const Shape &S = t.shapes[B.shape];
+    const int NI = B.ni, NJ = B.nj, NK = B.nk, NP = S.np, HW = NJ + 2;
+    const int mHS = (t.max_nj + 2) * (t.max_ni + 2);
+    T *H = (T *)lds_so;
+    uint8_t *BH = (uint8_t *)(H + kSorRing * mHS);
+    const uint32_t *cells = t.cells + S.cell_off;
+    for (int e = threadIdx.x; e <= NP; e += blockDim.x)
+        sps[e] = t.pstart[S.pstart_off + e];
+    const int64_t wo = B.wo;
+    const Window<T> hw = hilf_window(fin, t.blocks, B);
+    coef_to_lds(cf, F.aw, F.ae, F.as, F.an, F.ab, F.at, B);
+    const T *aw = cf, *ae = cf + kMaxN1, *as = cf + 2 * kMaxN1,
+            *an = cf + 3 * kMaxN1, *ab = cf + 4 * kMaxN1, *at = cf + 5 * kMaxN1;
+    const bool masked = MASKED && !(B.flags & 1);
+    const T om = (T)omega, om1 = (T)(1.0 - omega);
+    const bool keep_top = (NK & 1) != 0;
+    const int64_t fc = B.fc;
+    const int fb0 = face_base(0, NI, NJ, NK), fb1 = face_base(1, NI, NJ, NK),
+              fb2 = face_base(2, NI, NJ, NK), fb3 = face_base(3, NI, NJ, NK),
+              fb4 = face_base(4, NI, NJ, NK), fb5 = face_base(5, NI, NJ, NK);
+    __syncthreads();  // sps, cf
+
+    auto plane_of_step = [](int p) { return (p & 1) ? p : p - 2; };
+    auto load_slots = [&](int u, SorSlot<T> (&d)[MS]) {
+        const bool ok = u >= 0 && u < NP;
+        const int base = ok ? sps[u] : 0;
+        const int cnt = ok ? sps[u + 1] - base : 0;
+#pragma unroll
+        for (int mm = 0; mm < MS; ++mm) {
+            const int x = threadIdx.x + mm * kThreads;
+            const bool in = x < cnt;
+            const int s = base + (in ? x : 0);
+            const uint32_t cv = cells[s];
+            d[mm].rhs = F.rhs[wo + s];
+            d[mm].rap = F.rap[wo + s];
+            d[mm].c = in ? cv : kNone;
+        }
+    };
+    auto wload = [&](WinReg<T, MS> &w, int q) {
+        if (MASKED && masked) win_load<T, MS, true>(w, q, B, NP, sps, cells, (const T *)F.hilf, hw, m);
+        else win_load<T, MS, false>(w, q, B, NP, sps, cells, (const T *)F.hilf, hw, m);
+        if (HZ) {
+#pragma unroll
+            for (int mm = 0; mm < MS; ++mm) w.v[mm] = T(0);
+            w.g = T(0);
+        }
+    };
+    auto wstore = [&](const WinReg<T, MS> &w, int q) {
+        const int sl = (q + 1) & (kSorRing - 1);
+        if (MASKED && masked) win_store<T, MS, true>(w, H + sl * mHS, BH + sl * mHS, HW);
+        else win_store<T, MS, false>(w, H + sl * mHS, nullptr, HW);
+    };
+    WinReg<T, MS> wa;
+    for (int q = -1; q <= 1; ++q) {
+        wload(wa, q);
+        wstore(wa, q);
+    }
+    wload(wa, 2);
+    SorSlot<T> sa[MS];
+    load_slots(plane_of_step(0), sa);
+    lds_barrier();
+    for (int p = 0; p <= NP + 1; ++p) {
+        SorSlot<T> sn[MS];
+        WinReg<T, MS> wn;
+        load_slots(plane_of_step(p + 1), sn);
+        wload(wn, p + 3);
+        const int u = plane_of_step(p);
+        if (u >= 0 && u < NP) {
+            const int su = (u + 1) & (kSorRing - 1);
+            T *H0 = H + su * mHS;
+            const T *Hm = H + (u & (kSorRing - 1)) * mHS;
+            const T *Hp = H + ((u + 2) & (kSorRing - 1)) * mHS;
+            const uint8_t *B0 = BH + su * mHS;
+            const uint8_t *Bm = BH + (u & (kSorRing - 1)) * mHS;
+            const uint8_t *Bp = BH + ((u + 2) & (kSorRing - 1)) * mHS;
+            const int base = sps[u];
+#pragma unroll
+            for (int mm = 0; mm < MS; ++mm) {
+                if (sa[mm].c == kNone) continue;
+                int i, j, k;
+                unpack(sa[mm].c, i, j, k);
+                const int c0 = (j + 1) + HW * (i + 1);
+                const T old = H0[c0];
+                T nv = old;
+                if (!(keep_top && k == NK - 1)) {
+                    const T pw = Hm[c0 - HW], pe = Hp[c0 + HW];
+                    const T psv = Hm[c0 - 1], pn = Hp[c0 + 1];
+                    const T pb = Hm[c0], pt = Hp[c0];
+                    T r;
+                    if (MASKED && masked) {
+                        const T bc = (T)B0[c0];
+                        const T cw = (T)Bm[c0 - HW] * bc * aw[i];
+                        const T ce = bc * (T)Bp[c0 + HW] * ae[i];
+                        const T cs = (T)Bm[c0 - 1] * bc * as[j];
+                        const T cn = bc * (T)Bp[c0 + 1] * an[j];
+                        const T cb = (T)Bm[c0] * bc * ab[k];
+                        const T ct = bc * (T)Bp[c0] * at[k];
+                        r = cw * pw + ce * pe + cs * psv + cn * pn + cb * pb + ct * pt;
+                    } else {
+                        r = aw[i] * pw + ae[i] * pe + as[j] * psv + an[j] * pn +
+                            ab[k] * pb + at[k] * pt;
+                    }
+                    r = (r - sa[mm].rhs) * sa[mm].rap;
+                    nv = om1 * old - om * r;
+                }
+                H0[c0] = nv;
+                F.hilf[wo + base + threadIdx.x + mm * kThreads] = nv;
+                if (i == 0) fout[fc + fb0 + fidx_x(k, j, NJ, NK)] = nv;
+                if (i == NI - 1) fout[fc + fb1 + fidx_x(k, j, NJ, NK)] = nv;
+                if (j == 0) fout[fc + fb2 + fidx_y(k, i, NI, NK)] = nv;
+                if (j == NJ - 1) fout[fc + fb3 + fidx_y(k, i, NI, NK)] = nv;
+                if (k == 0) fout[fc + fb4 + fidx_z(j, i, NI, NJ)] = nv;
+                if (k == NK - 1) fout[fc + fb5 + fidx_z(j, i, NI, NJ)] = nv;
+            }
+        }
+        wstore(wa, p + 2);
+        lds_barrier();
+#pragma unroll
+        for (int mm = 0; mm < MS; ++mm) sa[mm] = sn[mm];
+        wa = wn;
+    }
+}
+
+template <typename T, bool MASKED, bool HZ>
+static void launch_sor_ms(int ms, const Topo &t, const Fields<T> &f, Mask m,
+                          const T *fin, T *fout, double omega,
+                          const int *done, size_t lds, hipStream_t s) {
+    switch (ms) {
+    case 1: k_sor<T, MASKED, HZ, 1><<<t.nlist, kThreads, lds, s>>>(t, f, m, fin, fout, omega, done); break;
+    case 2: k_sor<T, MASKED, HZ, 2><<<t.nlist, kThreads, lds, s>>>(t, f, m, fin, fout, omega, done); break;
+    case 3: k_sor<T, MASKED, HZ, 3><<<t.nlist, kThreads, lds, s>>>(t, f, m, fin, fout, omega, done); break;
+    default: k_sor<T, MASKED, HZ, 4><<<t.nlist, kThreads, lds, s>>>(t, f, m, fin, fout, omega, done); break;
+    }
+}
+
+template <typename T>
+void launch_sor(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
+                T *fout, double omega, int hilf_zero, const int *done,
+                hipStream_t s) {
+    const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
+    const bool masked = m.bp != nullptr;
+    const size_t lds = kSorRing * mHS * sizeof(T) + (masked ? kSorRing * mHS : 0) + 16;
+    const int ms = slots_per_lane(t);
+    if (masked) {
+        if (hilf_zero) launch_sor_ms<T, true, true>(ms, t, f, m, fin, fout, omega, done, lds, s);
+        else launch_sor_ms<T, true, false>(ms, t, f, m, fin, fout, omega, done, lds, s);
+    } else {
+        if (hilf_zero) launch_sor_ms<T, false, true>(ms, t, f, m, fin, fout, omega, done, lds, s);
+        else launch_sor_ms<T, false, false>(ms, t, f, m, fin, fout, omega, done, lds, s);
+    }
+}
+template void launch_sor<double>(const Topo &, const Fields<double> &, Mask,
+                                 const double *, double *, double, int,
+                                 const int *, hipStream_t);
+template void launch_sor<float>(const Topo &, const Fields<float> &, Mask,
+                                const float *, float *, double, int,
+                                const int *, hipStream_t);
+
+// ---------------------------------------------------------------------
 struct OutSlot {
     uint32_t c;
     double ap, rhs, dp;

@@ -28,6 +28,7 @@ struct Fields {
     T *rhs;                     // right-hand side read by the sweeps
     T *res;                     // forward-sweep result RES
     const T *ap;                // GSAP
+    const T *rap;               // SOR_RAP = divide0(1, GSAP) (SOR only)
     const T *f[7];              // LW LS LB LPR UE UN UT
     const T *aw, *ae, *as, *an, *ab, *at;  // 1-D coefficients (ref. layout)
     T *hface;                   // HILF boundary planes (own faces)
@@ -90,6 +91,14 @@ void launch_forward(const Topo &t, const Fields<T> &f, Mask m, int rd,
 template <typename T>
 void launch_backward(const Topo &t, const Fields<T> &f, int hilf_zero,
                      int classic, const int *done, hipStream_t s);
+
+template <typename T>
+void launch_rap(const double *ap, T *rap, int64_t n, hipStream_t s);
+
+template <typename T>
+void launch_sor(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
+                T *fout, double omega, int hilf_zero, const int *done,
+                hipStream_t s);
 
 template <typename T>
 void launch_outer(const Topo &t, const Fields<T> &f, Mask m, Outer o,

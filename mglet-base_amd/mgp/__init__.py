@@ -32,7 +32,7 @@ def _ptr(a, ct=C.c_double):
 class PressureSolver:
     def __init__(self, grids: Grids, type="sip", ninner=5, nouter=9, nouter_min=0,
                  epcorr=0.0, omega=1.1, rho=1.0, ib="noib", precision="fp64",
-                 eps_rel=0.0, check_every=4, bp=None, blocked=None):
+                 eps_rel=0.0, check_every=4, bp=None, blocked=None, sor_all_levels=False):
         self.lib = _lib.load()
         self.grids = g = grids
         t = type.lower()
@@ -48,7 +48,8 @@ class PressureSolver:
                                _ptr(k[6], C.c_int64), _ptr(k[7], C.c_int64),
                                _ptr(k[8], C.c_int64), _ptr(k[9], C.c_int64))
         self.cfg = MgpConfig(_TYPES[t], ninner, nouter, nouter_min, epcorr, omega, rho,
-                             _IB[ib], _PREC[precision], eps_rel, 0, check_every)
+                             _IB[ib], _PREC[precision], eps_rel, int(bool(sor_all_levels)),
+                             check_every)
         h = C.c_void_p()
         if blocked is not None:
             blocked = np.ascontiguousarray(blocked, dtype=np.int8)
@@ -134,6 +135,15 @@ class PressureSolver:
         ms = np.zeros(3)
         _lib.check(self.lib.mgp_time_kernels(self.h, niter, _ptr(ms)), "mgp_time_kernels")
         return dict(forward_ms=ms[0], backward_ms=ms[1], outer_ms=ms[2])
+
+    def history(self):
+        """max |RHS| after each outer iteration of the last mgpoisl call."""
+        cap = max(1, self.cfg.nouter)
+        out = np.zeros(cap)
+        n = C.c_int32()
+        _lib.check(self.lib.mgp_get_history(self.h, _ptr(out), cap, C.byref(n)),
+                   "mgp_get_history")
+        return out[:n.value].copy()
 
     def info(self):
         a, b = C.c_int64(), C.c_int64()

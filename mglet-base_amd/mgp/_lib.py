@@ -87,6 +87,7 @@ def load():
         "mgp_divcal_device": [vp, vp, vp, vp, vp, C.c_double],
         "mgp_time_kernels": [vp, C.c_int32, _dp],
         "mgp_info": [vp, _i64p, _i64p],
+        "mgp_get_history": [vp, _dp, C.c_int32, _i32p],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -15,6 +15,8 @@ def topology(name):
                            periodic=(True, False, True), wall=NOS)
     if name == "walls":         # closed box
         return uniform_box((2, 2, 2), (6, 8, 6), periodic=(False, False, False))
+    if name == "odd":           # odd interior counts, periodic
+        return uniform_box((2, 1, 2), (7, 6, 5))
     if name == "c2":            # 8x8x8 periodic blocks of 24^3 (192^3)
         return uniform_box((8, 8, 8), (24, 24, 24))
     raise KeyError(name)

@@ -11,7 +11,7 @@ from cases import blocked_mask, topology, velocity
 
 pytestmark = pytest.mark.gpu
 
-TOPOS = ["self12", "box2", "aniso", "walls"]
+TOPOS = ["self12", "box2", "aniso", "walls", "odd"]
 
 
 def _solver(g, **kw):
@@ -20,7 +20,8 @@ def _solver(g, **kw):
 
 
 def _run_both(oracle_mod, name, seed, ityp=0, ninner=5, nouter=3, ibtype=0,
-              frac=0.0, eps_rel=0.0, precision="fp64", nouter_min=0):
+              frac=0.0, eps_rel=0.0, precision="fp64", nouter_min=0, omega=1.1,
+              sor_all_levels=0):
     g = topology(name)
     o = oracle_mod.Oracle(g)
     bp = None
@@ -33,10 +34,12 @@ def _run_both(oracle_mod, name, seed, ityp=0, ninner=5, nouter=3, ibtype=0,
     p = np.zeros(g.n3d)
     ins = [a.copy() for a in (u, v, w, p)]
     so = o.mgpoisl(st, u, v, w, p, 1e-3, ityp=ityp, ninner=ninner, nouter=nouter,
-                   nouter_min=nouter_min, eps_rel=eps_rel, ibtype=ibtype)
+                   nouter_min=nouter_min, eps_rel=eps_rel, ibtype=ibtype, omega=omega,
+                   sor_all_levels=sor_all_levels)
     s = _solver(g, type={0: "sip", 1: "sor", 2: "classic"}[ityp], ninner=ninner,
                 nouter=nouter, nouter_min=nouter_min, eps_rel=eps_rel,
-                ib={0: "noib", 1: "ghostcell"}[ibtype], precision=precision, bp=bp)
+                ib={0: "noib", 1: "ghostcell"}[ibtype], precision=precision, bp=bp,
+                omega=omega, sor_all_levels=sor_all_levels)
     gu, gv, gw, gp = ins
     sg = s.mgpoisl(gu, gv, gw, gp, 1e-3)
     return g, o, st, so, s, sg, (u, v, w, p), (gu, gv, gw, gp)
@@ -63,6 +66,7 @@ def test_mgpoisl_fixed_iterations_bitwise(oracle_mod, name, ityp):
     assert sg["ipc"] == so["ipc"] and sg["nouter_done"] == so["nouter_done"]
     assert sg["maxrhs0"] == so["maxrhs0"]
     assert sg["maxrhs"] == so["maxrhs"]
+    assert np.array_equal(s.history(), so["history"])
     for a, b, nm in zip(ref, got, "uvwp"):
         assert np.array_equal(a, b), nm
     assert np.array_equal(s.get_field("DP"), so["dp"] * g.interior_mask())
@@ -101,6 +105,66 @@ def test_converged_fp64_and_fp32(oracle_mod):
     pr = ref[3][inner] - ref[3][inner].mean()
     pq = p[inner] - p[inner].mean()
     assert np.abs(pq - pr).max() <= 1e-8 * np.abs(pr).max()
+
+
+@pytest.mark.parametrize("name", TOPOS)
+@pytest.mark.parametrize("ninner", [1, 4, 5])
+def test_sor_bitwise(oracle_mod, name, ninner):
+    """SOR (relax, src/flow/sor_mod.F90:102-203) on every level, bitwise."""
+    g, o, st, so, s, sg, ref, got = _run_both(oracle_mod, name, seed=8, ityp=1,
+                                              ninner=ninner, nouter=3, omega=1.2,
+                                              sor_all_levels=1)
+    assert sg["ipc"] == so["ipc"] and sg["nouter_done"] == so["nouter_done"]
+    assert sg["maxrhs"] == so["maxrhs"]
+    assert np.array_equal(s.history(), so["history"])
+    for a, b, nm in zip(ref, got, "uvwp"):
+        assert np.array_equal(a, b), nm
+
+
+def test_sor_masked_bitwise(oracle_mod):
+    g, o, st, so, s, sg, ref, got = _run_both(oracle_mod, "box2", seed=9, ityp=1,
+                                              ninner=3, nouter=4, omega=1.3,
+                                              sor_all_levels=1, ibtype=1, frac=0.15)
+    assert sg["maxrhs"] == so["maxrhs"]
+    for a, b, nm in zip(ref, got, "uvwp"):
+        assert np.array_equal(a, b), nm
+
+
+def test_sor_single_level_runs_sip(oracle_mod):
+    """type "sor" relaxes only levels above the coarsest (mgpoisit,
+    src/flow/pressuresolver_mod.F90:403-412): on one level it is SIP."""
+    g, o, st, so, s, sg, ref, got = _run_both(oracle_mod, "box2", seed=10, ityp=1,
+                                              ninner=2, nouter=2)
+    _, _, _, so0, _, _, ref0, _ = _run_both(oracle_mod, "box2", seed=10, ityp=0,
+                                            ninner=2, nouter=2)
+    assert so["maxrhs"] == so0["maxrhs"] and sg["maxrhs"] == so["maxrhs"]
+    for a, b, nm in zip(ref, got, "uvwp"):
+        assert np.array_equal(a, b), nm
+
+
+def test_sor_converges_fp64_and_fp32(oracle_mod):
+    g = topology("box2")
+    o = oracle_mod.Oracle(g)
+    st = o.setup()
+    u, v, w = velocity(g, o, 12)
+    p = np.zeros(g.n3d)
+    so = o.mgpoisl(st, u.copy(), v.copy(), w.copy(), p, 1e-3, ityp=1, omega=1.2,
+                   sor_all_levels=1, ninner=4, nouter=3000, eps_rel=1e-10)
+    assert so["maxrhs"] <= 1e-10 * so["maxrhs0"]
+    for prec in ("fp64", "fp32"):
+        s = _solver(g, type="sor", omega=1.2, sor_all_levels=True, ninner=4,
+                    nouter=3000, eps_rel=1e-10, precision=prec)
+        gp = np.zeros(g.n3d)
+        r = s.mgpoisl(u.copy(), v.copy(), w.copy(), gp, 1e-3)
+        assert r["converged"] and r["maxrhs"] <= 1e-10 * r["maxrhs0"], prec
+        if prec == "fp64":
+            assert r["nouter_done"] == so["nouter_done"]
+            assert np.array_equal(gp, p)
+        else:
+            # periodic: the pressure is fixed up to a constant
+            inner = g.interior_mask()
+            a, b = gp[inner] - gp[inner].mean(), p[inner] - p[inner].mean()
+            assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
 
 
 def test_connect_identity(oracle_mod):

@@ -352,6 +352,61 @@ def test_projection_idempotent(oracle_mod):
         assert np.abs(a[inner] - b[inner]).max() < 1e-9 * scale
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_sor_red_black_dense(oracle_mod, periodic):
+    """relax against red-black SOR written with the assembled operator A
+    (diagonal AP, off-diagonals GSA*): x_c <- (1-w) x_c + w (b_c - sum_nb
+    A_c,nb x_nb) / A_cc, visiting cells in the loop order of
+    src/flow/sor_mod.F90:122-131 (Fortran indices, rb = 0 then 1).  Ghost
+    values are frozen at their values before the call, which matters for a
+    self-periodic block; with an odd interior count along k the loop bounds
+    never visit k = kk-2, which must keep its value."""
+    g = uniform_box((1, 1, 1), (4, 3, 5), length=(1.0, 0.75, 1.25),
+                    periodic=(periodic,) * 3)
+    o = oracle_mod.Oracle(g)
+    st = o.setup()
+    c = st["coef"]
+    ii, jj, kk = g.ii[0], g.jj[0], g.kk[0]
+    ni, nj, nk = ii - 4, jj - 4, kk - 4
+    rng = np.random.default_rng(17)
+    inner = g.interior_mask()
+    dp = np.where(inner, rng.standard_normal(g.n3d), 0.0)
+    rhs = np.where(inner, rng.standard_normal(g.n3d), 0.0)
+    o.exchange([dp])                       # ghosts of the previous conn
+    omega = 1.3
+    x = g.block_view(dp.copy(), 1)
+    x0 = x.copy()
+    b = g.block_view(rhs, 1)
+    ap = g.block_view(c["ap"], 1)
+    visited = np.zeros((ii, jj, kk), bool)
+    for rb in (0, 1):
+        for i in range(3, ii - 1):
+            for j in range(3, jj - 1):
+                ks = (4, kk - 2) if (i + j) % 2 == rb else (3, kk - 3)
+                for k in range(ks[0], ks[1] + 1, 2):
+                    I, J, K = i - 1, j - 1, k - 1           # 0-based array index
+                    s = 0.0
+                    for (a, bb, d, coef) in ((I - 1, J, K, c["aw"][i - 1]),
+                                             (I + 1, J, K, c["ae"][i - 1]),
+                                             (I, J - 1, K, c["as"][j - 1]),
+                                             (I, J + 1, K, c["an"][j - 1]),
+                                             (I, J, K - 1, c["ab"][k - 1]),
+                                             (I, J, K + 1, c["at"][k - 1])):
+                        ghost = not (2 <= a < ii - 2 and 2 <= bb < jj - 2 and 2 <= d < kk - 2)
+                        s += coef * (x0[a, bb, d] if ghost else x[a, bb, d])
+                    x[I, J, K] = (1 - omega) * x[I, J, K] + omega * (b[I, J, K] - s) / ap[I, J, K]
+                    visited[I, J, K] = True
+    got = dp.copy()
+    o.relax(got, rhs, c, st["rap"], omega)
+    gv = g.block_view(got, 1)
+    sl = (slice(2, ii - 2), slice(2, jj - 2), slice(2, kk - 2))
+    assert np.allclose(gv[sl], x[sl], rtol=1e-13, atol=1e-13 * np.abs(x[sl]).max())
+    assert nk % 2 == 1
+    assert not visited[sl][:, :, nk - 1].any() and visited[sl][:, :, :nk - 1].all()
+    assert np.array_equal(gv[sl][:, :, nk - 1], x0[sl][:, :, nk - 1])
+    assert np.array_equal(got[~inner], dp[~inner])      # ghosts untouched
+
+
 def test_sor_fixed_point_and_solution(oracle_mod):
     """SOR leaves the exact solution unchanged, and SOR and SIP converge to
     the same pressure (sor_mod.F90:102-203 against sip)."""
