@@ -117,6 +117,24 @@ int mgp_mgpoisl_device(mgp_handle *h, mgletreal *u, mgletreal *v,
                        mgletreal *w, mgletreal *p, mgletreal dt,
                        mgletint ittot, mgletint irk, mgp_stats *stats);
 
+/* Adjoint of the projection.  No counterpart in the reference: for
+ * differentiable callers (gradient-based optimisation and data assimilation
+ * through the time loop).  mgpoisl maps the interior face velocities u to
+ * u' = u - G A^-1 D u (D: ib%divcal, src/ib/noib_mod.F90:304-354; A: the
+ * GSA* operator; G: mgpcorr, src/flow/pressuresolver_mod.F90:787-879).  On
+ * entry ubar, vbar, wbar hold dJ/du' on interior faces, on exit dJ/du =
+ * ubar - D^T A^-T G^T ubar.  The adjoint Poisson problem A^T y = G^T ubar is
+ * solved with the handle's solver settings and the transposed SIP factors
+ * (U^T, then L^T).  Ghost layers and faces on block boundaries without a
+ * connected neighbour (the caller's boundary conditions) are zero on exit.
+ * One grid level only; overwrites the state read by mgp_get_field("DP"). */
+int mgp_adjoint(mgp_handle *h, mgletreal *ubar, mgletreal *vbar,
+                mgletreal *wbar, mgletreal dt, mgp_stats *stats);
+
+/* The same with ubar, vbar, wbar resident in device memory. */
+int mgp_adjoint_device(mgp_handle *h, mgletreal *ubar, mgletreal *vbar,
+                       mgletreal *wbar, mgletreal dt, mgp_stats *stats);
+
 /* finish_pressuresolver (src/flow/pressuresolver_mod.F90:148-157). */
 int mgp_finish(mgp_handle *h);
 

@@ -140,6 +140,10 @@ struct mgp_handle {
     DBuf d_hface, d_rhi[2], d_rlo;
     // SOR: SOR_RAP and the second HILF face buffer (frozen ghosts)
     DBuf d_rapT, d_hface2;
+    // adjoint: transposed factors and 1-D coefficients (built on first use)
+    DBuf d_ft[6], d_c64t[6], d_cTt[6];
+    std::vector<double> fold_host[6];
+    bool adj_ready = false;
     // reference-layout work arrays
     DBuf d_u, d_v, d_w, d_p, d_dpsp;
     // scalars and history
@@ -234,6 +238,34 @@ struct mgp_handle {
         F.rhi[1] = d_rhi[1].as<T>();
         F.rlo = d_rlo.as<T>();
         return F;
+    }
+
+    // operands of A^T: transposed factors in the slots the sweeps read
+    // (LWt LSt LBt LPR UEt UNt UTt) and the transposed 1-D coefficients
+    template <typename T>
+    Fields<T> fields_t() const {
+        Fields<T> F = fields<T>();
+        F.f[0] = d_ft[0].as<T>();
+        F.f[1] = d_ft[1].as<T>();
+        F.f[2] = d_ft[2].as<T>();
+        F.f[4] = d_ft[3].as<T>();
+        F.f[5] = d_ft[4].as<T>();
+        F.f[6] = d_ft[5].as<T>();
+        const T **c[6] = {&F.aw, &F.ae, &F.as, &F.an, &F.ab, &F.at};
+        for (int q = 0; q < 6; ++q)
+            *c[q] = fp32() ? d_cTt[q].as<T>() : d_c64t[q].as<T>();
+        return F;
+    }
+
+    Outer outer_t() const {
+        Outer o = outer();
+        o.aw = d_c64t[0].as<double>();
+        o.ae = d_c64t[1].as<double>();
+        o.as = d_c64t[2].as<double>();
+        o.an = d_c64t[3].as<double>();
+        o.ab = d_c64t[4].as<double>();
+        o.at = d_c64t[5].as<double>();
+        return o;
     }
 };
 
@@ -715,6 +747,7 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
             if (is_wall_lo(bc[4])) fold[4][h->ip1z[g] + 2] = 0.0;
             if (is_wall_hi(bc[5])) fold[5][h->ip1z[g] + h->kk[g] - 3] = 0.0;
         }
+        for (int q = 0; q < 6; ++q) h->fold_host[q] = fold[q];
         for (int q = 0; q < 6; ++q) {
             upload(h->d_raw[q], raw[q]);
             upload(h->d_c64[q], fold[q]);
@@ -778,16 +811,10 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
     return MGP_OK;
 }
 
+// Pool fields start at zero (src/core/fieldpool_mod.F90:113-122).
 template <typename T>
-int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
-          double dt, mgp_stats *st) {
-    const mgp_config &c = h.cfg;
-    Outer o = h.outer();
-    const double prefak = c.rho / dt;
+void reset_state(mgp_handle &h) {
     const hipStream_t s = h.stream;
-    Topo tall = h.topo(-1);
-    T *rhsT = h.fp32() ? h.d_rhsT.as<T>() : nullptr;
-    // pool fields start at zero (src/core/fieldpool_mod.F90:113-122)
     launch_fill<double>(h.d_dp64.as<double>(), h.ncell, 0.0, s);
     launch_fill<T>(h.d_hilf.as<T>(), h.ncell, T(0), s);
     launch_fill<T>(h.d_hface.as<T>(), h.nface, T(0), s);
@@ -795,12 +822,19 @@ int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
     launch_fill<T>(h.d_rhi[1].as<T>(), h.nface, T(0), s);
     launch_fill<T>(h.d_rlo.as<T>(), h.nface, T(0), s);
     if (h.d_hface2.p) launch_fill<T>(h.d_hface2.as<T>(), h.nface, T(0), s);
-    launch_divcal<T>(tall, u, v, w, h.d_rddx.as<double>(), h.d_rddy.as<double>(),
-                     h.d_rddz.as<double>(), c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr,
-                     prefak, h.d_rhs64.as<double>(), rhsT, s);
-    launch_maxabs(tall, h.d_rhs64.as<double>(), o, s);
-    HC(hipGetLastError());
+}
 
+// The outer defect-correction loop (src/flow/pressuresolver_mod.F90:
+// 253-303) on the RHS already in place, with the operator set (F, o):
+// the forward operators, or the transposed ones for the adjoint.  Returns
+// the device time of the loop in ms.
+template <typename T>
+float iterate(mgp_handle &h, const Fields<T> &F, const Outer &o,
+              double prefak) {
+    const mgp_config &c = h.cfg;
+    const hipStream_t s = h.stream;
+    Topo tall = h.topo(-1);
+    T *rhsT = h.fp32() ? h.d_rhsT.as<T>() : nullptr;
     hipEvent_t e0, e1;
     HC(hipEventCreate(&e0));
     HC(hipEventCreate(&e1));
@@ -813,7 +847,6 @@ int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
         const bool classic = c.ityp == 2;
         auto enqueue_chunk = [&]() {
             int rd = 0;
-            Fields<T> F = h.fields<T>();
             Mask m = h.mask();
             for (int it = 0; it < chunk; ++it) {
                 for (auto &kv : h.level_blocks) {
@@ -882,33 +915,34 @@ int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
         if (graph) HC(hipGraphDestroy(graph));
     }
     HC(hipEventRecord(e1, s));
+    HC(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HC(hipEventElapsedTime(&ms, e0, e1));
+    HC(hipEventDestroy(e0));
+    HC(hipEventDestroy(e1));
+    return ms;
+}
 
-    // DP -> reference layout with the face ghosts of the exchange
-    // (src/flow/pressuresolver_mod.F90:278 accumulates HILF including the
-    // ghosts that conn(...) copied from the neighbours), then mgpcorr and
-    // the post-projection exchange (:333-348).
+// DP (or the adjoint's y) to the reference layout in h.d_dpsp, with the
+// face ghosts of a 1-layer exchange (src/flow/pressuresolver_mod.F90:278
+// accumulates HILF including the ghosts that conn(...) copied).
+void dp_to_reference(mgp_handle &h) {
+    const hipStream_t s = h.stream;
+    Topo tall = h.topo(-1);
     if (h.d_dpsp.n < (size_t)h.n3d * 8) h.d_dpsp.alloc((size_t)h.n3d * 8);
     launch_fill<double>(h.d_dpsp.as<double>(), h.n3d, 0.0, s);
     launch_wo_to_sp<double>(tall, h.d_dp64.as<double>(), h.d_dpsp.as<double>(), s);
     double *fd[1] = {h.d_dpsp.as<double>()};
     exchange(h, -1, 1, 0, 0, fd, 1);
-    launch_mgpcorr(tall, u, v, w, p, h.d_dpsp.as<double>(), h.d_rdx.as<double>(),
-                   h.d_rdy.as<double>(), h.d_rdz.as<double>(), dt / c.rho,
-                   c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr, s);
-    double *fs[4] = {u, v, w, p};
-    for (int l = h.minlevel; l <= h.maxlevel; ++l)
-        if (h.level_blocks.count(l)) exchange(h, l, 2, 1, 0, fs, 4);
-    HC(hipGetLastError());
+}
 
+int finish_stats(mgp_handle &h, float ms, mgp_stats *st) {
+    const mgp_config &c = h.cfg;
     double sc[3];
     int iv[4];
-    HC(hipMemcpyAsync(sc, h.d_scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
-    HC(hipMemcpyAsync(iv, (char *)h.d_scal.p + 24, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
-    HC(hipStreamSynchronize(s));
-    float ms = 0.f;
-    HC(hipEventElapsedTime(&ms, e0, e1));
-    HC(hipEventDestroy(e0));
-    HC(hipEventDestroy(e1));
+    HC(hipMemcpyAsync(sc, h.d_scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, h.stream));
+    HC(hipMemcpyAsync(iv, (char *)h.d_scal.p + 24, 4 * sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    HC(hipStreamSynchronize(h.stream));
     h.have_solution = true;
     if (st) {
         st->nouter_done = iv[0];
@@ -920,6 +954,138 @@ int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
     }
     if (iv[3]) return fail(MGP_ERR_NONFINITE, "residual became NaN or Inf");
     return MGP_OK;
+}
+
+template <typename T>
+int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
+          double dt, mgp_stats *st) {
+    const mgp_config &c = h.cfg;
+    const double prefak = c.rho / dt;
+    const hipStream_t s = h.stream;
+    Topo tall = h.topo(-1);
+    T *rhsT = h.fp32() ? h.d_rhsT.as<T>() : nullptr;
+    reset_state<T>(h);
+    launch_divcal<T>(tall, u, v, w, h.d_rddx.as<double>(), h.d_rddy.as<double>(),
+                     h.d_rddz.as<double>(), c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr,
+                     prefak, h.d_rhs64.as<double>(), rhsT, s);
+    launch_maxabs(tall, h.d_rhs64.as<double>(), h.outer(), s);
+    HC(hipGetLastError());
+    const float ms = iterate<T>(h, h.fields<T>(), h.outer(), prefak);
+
+    // DP -> reference layout, mgpcorr and the post-projection exchange
+    // (src/flow/pressuresolver_mod.F90:323-348)
+    dp_to_reference(h);
+    launch_mgpcorr(tall, u, v, w, p, h.d_dpsp.as<double>(), h.d_rdx.as<double>(),
+                   h.d_rdy.as<double>(), h.d_rdz.as<double>(), dt / c.rho,
+                   c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr, s);
+    double *fs[4] = {u, v, w, p};
+    for (int l = h.minlevel; l <= h.maxlevel; ++l)
+        if (h.level_blocks.count(l)) exchange(h, l, 2, 1, 0, fs, 4);
+    HC(hipGetLastError());
+    return finish_stats(h, ms, st);
+}
+
+// Operands of the transposed problem A^T y = b, built on first use: the
+// 1-D coefficients of A^T (the coefficient of cell c towards neighbour n
+// is A's coefficient of n towards c: awT(i) = GSAE(i-1), taken from the
+// neighbouring block at the block's first cell, zero without a connected
+// neighbour) and the transposed SIP factors (launch_tfactors).
+template <typename T>
+void ensure_adjoint(mgp_handle &h) {
+    if (h.adj_ready) return;
+    const int n = h.ngrid;
+    std::vector<double> tr[6];
+    for (int q = 0; q < 6; ++q) tr[q].assign(h.fold_host[q].size(), 0.0);
+    const int64_t *o1[3] = {h.ip1x.data(), h.ip1y.data(), h.ip1z.data()};
+    for (int g = 0; g < n; ++g) {
+        const Block &B = h.blocks[g];
+        const int nn[3] = {h.ii[g], h.jj[g], h.kk[g]};
+        for (int a = 0; a < 3; ++a) {
+            const std::vector<double> &lo = h.fold_host[2 * a], &hi = h.fold_host[2 * a + 1];
+            double *tlo = tr[2 * a].data() + o1[a][g] - 1;   // 1-based index q
+            double *thi = tr[2 * a + 1].data() + o1[a][g] - 1;
+            const double *flo = lo.data() + o1[a][g] - 1, *fhi = hi.data() + o1[a][g] - 1;
+            const int last = nn[a] - 2;
+            for (int q = 3; q <= last; ++q) {
+                if (q > 3) tlo[q] = fhi[q - 1];
+                else if (B.nbr[2 * a] >= 0) {
+                    const int nb = B.nbr[2 * a];
+                    const int nlast = (a == 0 ? h.ii[nb] : a == 1 ? h.jj[nb] : h.kk[nb]) - 2;
+                    tlo[q] = hi[o1[a][nb] - 1 + nlast];
+                }
+                if (q < last) thi[q] = flo[q + 1];
+                else if (B.nbr[2 * a + 1] >= 0) {
+                    const int nb = B.nbr[2 * a + 1];
+                    thi[q] = lo[o1[a][nb] - 1 + 3];
+                }
+            }
+        }
+    }
+    for (int q = 0; q < 6; ++q) {
+        upload(h.d_c64t[q], tr[q]);
+        if (h.fp32()) {
+            std::vector<float> fq(tr[q].begin(), tr[q].end());
+            upload(h.d_cTt[q], fq);
+        }
+    }
+    const size_t tsz = h.tsz();
+    for (int q = 0; q < 6; ++q) h.d_ft[q].alloc(h.ncell * tsz);
+    const T *f[7];
+    T *g[6];
+    for (int q = 0; q < 7; ++q) f[q] = h.d_f[q].as<T>();
+    for (int q = 0; q < 6; ++q) g[q] = h.d_ft[q].as<T>();
+    launch_tfactors<T>(h.topo(-1), h.mask(), f, g, h.stream);
+    HC(hipGetLastError());
+    HC(hipStreamSynchronize(h.stream));
+    h.adj_ready = true;
+}
+
+// Adjoint of the projection (no counterpart in the reference): on entry
+// u, v, w hold dJ/du' on the interior faces, on exit dJ/du.
+template <typename T>
+int solve_adjoint(mgp_handle &h, double *u, double *v, double *w, double dt,
+                  mgp_stats *st) {
+    ensure_adjoint<T>(h);
+    const mgp_config &c = h.cfg;
+    const double prefak = c.rho / dt;
+    const hipStream_t s = h.stream;
+    Topo tall = h.topo(-1);
+    T *rhsT = h.fp32() ? h.d_rhsT.as<T>() : nullptr;
+    const double *bpsp = c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr;
+    reset_state<T>(h);
+    launch_adj_clean(tall, u, v, w, bpsp, s);
+    double *fs[3] = {u, v, w};
+    exchange(h, -1, 1, 0, 0, fs, 3);
+    launch_gradT<T>(tall, u, v, w, h.d_rdx.as<double>(), h.d_rdy.as<double>(),
+                    h.d_rdz.as<double>(), bpsp, h.d_rhs64.as<double>(), rhsT, s);
+    Outer o = h.outer_t();
+    launch_maxabs(tall, h.d_rhs64.as<double>(), o, s);
+    HC(hipGetLastError());
+    const float ms = iterate<T>(h, h.fields_t<T>(), o, prefak);
+    dp_to_reference(h);
+    launch_divT(tall, u, v, w, h.d_dpsp.as<double>(), h.d_rddx.as<double>(),
+                h.d_rddy.as<double>(), h.d_rddz.as<double>(), bpsp, s);
+    launch_adj_clean(tall, u, v, w, bpsp, s);
+    HC(hipGetLastError());
+    return finish_stats(h, ms, st);
+}
+
+int adjoint_any(mgp_handle *h, double *u, double *v, double *w, double dt,
+                mgp_stats *st) {
+    if (!(dt > 0.0) && !(dt < 0.0)) return fail(MGP_ERR_ARG, "dt must be non-zero");
+    if (h->masked && h->cfg.ibtype != 1)
+        return fail(MGP_ERR_ARG, "the adjoint of a masked operator needs ibtype 1 "
+                    "(ghostcell divcal and mgpcorr)");
+    if (h->level_blocks.size() > 1)
+        return fail(MGP_ERR_TOPOLOGY, "the adjoint supports one grid level");
+    try {
+        int rc = h->fp32() ? solve_adjoint<float>(*h, u, v, w, dt, st)
+                           : solve_adjoint<double>(*h, u, v, w, dt, st);
+        if (rc == MGP_OK) g_err.clear();
+        return rc;
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
 }
 
 int solve_any(mgp_handle *h, double *u, double *v, double *w, double *p,
@@ -982,6 +1148,36 @@ int mgp_mgpoisl(mgp_handle *h, mgletreal *u, mgletreal *v, mgletreal *w,
         int rc = solve_any(h, h->d_u.as<double>(), h->d_v.as<double>(),
                            h->d_w.as<double>(), h->d_p.as<double>(), dt, stats);
         for (int q = 0; q < 4; ++q)
+            HC(hipMemcpyAsync(hp[q], d[q]->p, bytes, hipMemcpyDeviceToHost, h->stream));
+        HC(hipStreamSynchronize(h->stream));
+        return rc;
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+}
+
+int mgp_adjoint_device(mgp_handle *h, mgletreal *ubar, mgletreal *vbar,
+                       mgletreal *wbar, mgletreal dt, mgp_stats *stats) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (!ubar || !vbar || !wbar) return fail(MGP_ERR_ARG, "null field");
+    return adjoint_any(h, ubar, vbar, wbar, dt, stats);
+}
+
+int mgp_adjoint(mgp_handle *h, mgletreal *ubar, mgletreal *vbar,
+                mgletreal *wbar, mgletreal dt, mgp_stats *stats) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (!ubar || !vbar || !wbar) return fail(MGP_ERR_ARG, "null field");
+    try {
+        const size_t bytes = (size_t)h->n3d * 8;
+        DBuf *d[3] = {&h->d_u, &h->d_v, &h->d_w};
+        double *hp[3] = {ubar, vbar, wbar};
+        for (int q = 0; q < 3; ++q) {
+            if (d[q]->n < bytes) d[q]->alloc(bytes);
+            HC(hipMemcpyAsync(d[q]->p, hp[q], bytes, hipMemcpyHostToDevice, h->stream));
+        }
+        int rc = adjoint_any(h, h->d_u.as<double>(), h->d_v.as<double>(),
+                             h->d_w.as<double>(), dt, stats);
+        for (int q = 0; q < 3; ++q)
             HC(hipMemcpyAsync(hp[q], d[q]->p, bytes, hipMemcpyDeviceToHost, h->stream));
         HC(hipStreamSynchronize(h->stream));
         return rc;
@@ -1195,7 +1391,8 @@ int mgp_info(mgp_handle *h, int64_t *ncells, int64_t *device_bytes) {
         const DBuf *bs[] = {&h->d_blocks, &h->d_shapes, &h->d_cells, &h->d_pstart, &h->d_slot,
                             &h->d_ap64, &h->d_apT, &h->d_hilf, &h->d_rhs64, &h->d_rhsT, &h->d_res,
                             &h->d_dp64, &h->d_bpw, &h->d_bpf, &h->d_bpsp, &h->d_hface, &h->d_rhi[0],
-                            &h->d_rhi[1], &h->d_rlo, &h->d_rapT, &h->d_hface2, &h->d_u, &h->d_v, &h->d_w, &h->d_p, &h->d_dpsp};
+                            &h->d_rhi[1], &h->d_rlo, &h->d_rapT, &h->d_hface2, &h->d_ft[0], &h->d_ft[1], &h->d_ft[2],
+                            &h->d_ft[3], &h->d_ft[4], &h->d_ft[5], &h->d_u, &h->d_v, &h->d_w, &h->d_p, &h->d_dpsp};
         for (auto *b : bs) tot += b->n;
         for (auto &b : h->d_f) tot += b.n;
         *device_bytes = (int64_t)tot;

@@ -124,4 +124,19 @@ template <typename T>
 void launch_to_ref_factors(const Topo &t, const T *wo, double *ref,
                            int spatial, hipStream_t s);
 
+// adjoint projection (mgp_adjoint.hip)
+template <typename T>
+void launch_tfactors(const Topo &t, Mask m, const T *const f[7],
+                     T *const g[6], hipStream_t s);
+void launch_adj_clean(const Topo &t, double *u, double *v, double *w,
+                      const double *bp, hipStream_t s);
+template <typename T>
+void launch_gradT(const Topo &t, const double *u, const double *v,
+                  const double *w, const double *rdx, const double *rdy,
+                  const double *rdz, const double *bp, double *rhs64,
+                  T *rhs_t, hipStream_t s);
+void launch_divT(const Topo &t, double *u, double *v, double *w,
+                 const double *y, const double *rddx, const double *rddy,
+                 const double *rddz, const double *bp, hipStream_t s);
+
 }  // namespace mgp

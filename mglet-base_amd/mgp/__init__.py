@@ -101,6 +101,30 @@ class PressureSolver:
         return dict(ipc=st.ipc, nouter_done=st.nouter_done, converged=bool(st.converged),
                     maxrhs0=st.maxrhs0, maxrhs=st.maxrhs, solve_ms=st.solve_ms)
 
+    def adjoint(self, ubar, vbar, wbar, dt):
+        """Adjoint of `mgpoisl` with respect to the interior face velocities:
+        (ubar, vbar, wbar) = dJ/du' on entry, dJ/du on exit (in place).  The
+        adjoint Poisson problem uses this solver's iteration settings."""
+        st = MgpStats()
+        if isinstance(ubar, np.ndarray):
+            for a in (ubar, vbar, wbar):
+                if a.dtype != np.float64 or not a.flags.c_contiguous or a.size != self.grids.n3d:
+                    raise MgpError("fields must be contiguous float64 of the 3-D length")
+            rc = self.lib.mgp_adjoint(self.h, _ptr(ubar), _ptr(vbar), _ptr(wbar), dt,
+                                      C.byref(st))
+        else:
+            import torch
+            for a in (ubar, vbar, wbar):
+                if (a.dtype != torch.float64 or not a.is_cuda or not a.is_contiguous()
+                        or a.numel() != self.grids.n3d):
+                    raise MgpError("fields must be contiguous CUDA float64 tensors")
+            self._bind_stream()
+            rc = self.lib.mgp_adjoint_device(self.h, ubar.data_ptr(), vbar.data_ptr(),
+                                             wbar.data_ptr(), dt, C.byref(st))
+        _lib.check(rc, "mgp_adjoint")
+        return dict(ipc=st.ipc, nouter_done=st.nouter_done, converged=bool(st.converged),
+                    maxrhs0=st.maxrhs0, maxrhs=st.maxrhs, solve_ms=st.solve_ms)
+
     def _bind_stream(self):
         # the library works on its own stream and synchronises it before
         # returning; torch work queued before the call must be complete

@@ -88,6 +88,8 @@ def load():
         "mgp_time_kernels": [vp, C.c_int32, _dp],
         "mgp_info": [vp, _i64p, _i64p],
         "mgp_get_history": [vp, _dp, C.c_int32, _i32p],
+        "mgp_adjoint": [vp, _dp, _dp, _dp, C.c_double, C.POINTER(MgpStats)],
+        "mgp_adjoint_device": [vp, vp, vp, vp, C.c_double, C.POINTER(MgpStats)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

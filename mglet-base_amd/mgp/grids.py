@@ -157,3 +157,44 @@ def uniform_box(nb, n, length=(1.0, 1.0, 1.0), periodic=(True, True, True),
                     else:
                         bc[g, f] = CON
     return Grids(dims, np.full(ngrid, level), nbr, bc, np.tile(h, (ngrid, 1)), org)
+
+
+def stretched_box(nb, n, length=(1.0, 1.0, 1.0), ratio=1.15, axis=1, wall=NOS):
+    """uniform_box with the cell-centre spacing along `axis` growing by
+    `ratio` from cell to cell (walls on that axis, periodic elsewhere).
+    The metrics keep the reference's relations: DX(i) = X(i+1) - X(i),
+    faces midway between centres, DDX(i) = (DX(i-1) + DX(i)) / 2; ghost
+    metrics continue the global progression, so neighbouring blocks agree
+    on their shared faces.  The Poisson operator is then not symmetric."""
+    periodic = [True, True, True]
+    periodic[axis] = False
+    g = uniform_box(nb, n, length, tuple(periodic), wall)
+    nn = nb[axis] * n[axis]
+    # global centre spacings DX for global cells c = -3 .. nn + 2
+    cg = np.arange(-3, nn + 3)
+    d = ratio ** cg.astype(np.float64)
+    d *= length[axis] / d[3:3 + nn].sum()
+    xc = np.concatenate([[0.0], np.cumsum(d)])       # centres of cells -3 .. nn + 3
+    ip1 = (g.ip1x, g.ip1y, g.ip1z)[axis]
+    nloc = (g.ii, g.jj, g.kk)[axis]
+    x = np.zeros_like((g.x, g.y, g.z)[axis])
+    dx = np.zeros_like(x)
+    ddx = np.zeros_like(x)
+    nbx, nby = nb[0], nb[1]
+    for b in range(g.ngrid):
+        pos = (b % nbx, (b // nbx) % nby, b // (nbx * nby))[axis]
+        m = int(nloc[b])
+        q = np.arange(1, m + 1)                        # local 1-based index
+        c = pos * n[axis] + (q - 3)                    # global cell, interior from 0
+        sl = slice(ip1[b], ip1[b] + m)
+        x[sl] = xc[c + 3]
+        dx[sl] = xc[c + 4] - xc[c + 3]
+        ddx[sl] = 0.5 * ((xc[c + 3] - xc[c + 2]) + (xc[c + 4] - xc[c + 3]))
+    names = ("x", "y", "z")[axis]
+    setattr(g, names, x)
+    setattr(g, "d" + names, dx)
+    setattr(g, "dd" + names, ddx)
+    setattr(g, "rd" + names, 1.0 / dx)
+    setattr(g, "rdd" + names, 1.0 / ddx)
+    setattr(g, names + "stag", x + 0.5 * dx)
+    return g

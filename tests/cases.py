@@ -2,7 +2,7 @@
 fields in the reference layout (see mgp.grids)."""
 import numpy as np
 
-from mgp.grids import NOS, uniform_box
+from mgp.grids import NOS, stretched_box, uniform_box
 
 
 def topology(name):
@@ -17,6 +17,8 @@ def topology(name):
         return uniform_box((2, 2, 2), (6, 8, 6), periodic=(False, False, False))
     if name == "odd":           # odd interior counts, periodic
         return uniform_box((2, 1, 2), (7, 6, 5))
+    if name == "stretched":     # geometric spacing along y (walls), periodic x/z
+        return stretched_box((2, 2, 2), (6, 8, 6), length=(1.0, 1.0, 1.0), ratio=1.15)
     if name == "c2":            # 8x8x8 periodic blocks of 24^3 (192^3)
         return uniform_box((8, 8, 8), (24, 24, 24))
     raise KeyError(name)
@@ -57,3 +59,17 @@ def blocked_mask(g, seed, frac=0.1):
     inner = g.interior_mask()
     m = np.where(inner, (rng.random(g.n3d) < frac).astype(np.float64), 0.0)
     return m
+
+
+def adjoint_case(oracle_mod, name, ibtype, seed):
+    """Topology, oracle, oracle state and the sparse operators of the
+    projection (oracle.adjoint) for the adjoint checks."""
+    from oracle.adjoint import Operators
+    g = topology(name)
+    o = oracle_mod.Oracle(g)
+    bp = None
+    if ibtype:
+        bp = 1.0 - blocked_mask(g, seed + 100, 0.12)
+        o.exchange([bp], layers=2, corners=True)
+    st = o.setup(bp=bp)
+    return g, o, st, Operators(o, st, ibtype), bp

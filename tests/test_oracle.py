@@ -431,3 +431,68 @@ def test_sor_fixed_point_and_solution(oracle_mod):
     before = dp.copy()
     o.relax(dp, rhs, st["coef"], st["rap"], 1.3)
     assert np.abs(dp[inner] - before[inner]).max() < 1e-7 * np.abs(before[inner]).max()
+
+
+# ---------------------------------------------------------------- adjoint
+def _adjoint_case(oracle_mod, name, ibtype, seed):
+    from cases import adjoint_case
+    return adjoint_case(oracle_mod, name, ibtype, seed)[:4]
+
+
+@pytest.mark.parametrize("name,ibtype", [("box2", 0), ("walls", 0), ("stretched", 0),
+                                         ("aniso", 0), ("box2", 1)])
+def test_adjoint_operators_equal_oracle_kernels(oracle_mod, name, ibtype):
+    """The assembled D, A, G reproduce divcal, laplacephi and mgpcorr."""
+    g, o, st, ops = _adjoint_case(oracle_mod, name, ibtype, 31)
+    rng = np.random.default_rng(32)
+    inner = g.interior_mask()
+    cells = ops.cells
+    # A phi = -laplacephi(phi)
+    phi = np.where(inner, rng.standard_normal(g.n3d), 0.0)
+    o.exchange([phi])
+    res = g.zeros()
+    o.laplacephi(res, phi, st["coef"], st["bp"])
+    a = ops.A @ phi[cells]
+    assert np.allclose(a, -res[cells], rtol=1e-13, atol=1e-13 * np.abs(res).max())
+    # D u = divcal(u) on unknown faces (boundary-condition faces zero)
+    x = rng.standard_normal(ops.nface)
+    u, v, w = ops.scatter(x)
+    o.exchange([u, v, w])
+    div = g.zeros()
+    o.divcal(div, u, v, w, 1.0, bp=st["bp"] if ibtype else None,
+             sdiv=g.zeros() if ibtype else None)
+    d = ops.D @ x
+    assert np.allclose(d, div[cells], rtol=1e-13, atol=1e-13 * np.abs(div).max())
+    # u' - u = -G p (mgpcorr with rfak = 1)
+    p = np.where(inner, rng.standard_normal(g.n3d), 0.0)
+    o.exchange([p])
+    uu, vv, ww, pp = g.zeros(), g.zeros(), g.zeros(), g.zeros()
+    o.mgpcorr(uu, vv, ww, pp, p, 1.0, bp=st["bp"] if ibtype else None)
+    gp = ops.G @ p[cells]
+    assert np.allclose(-gp, ops.gather(uu, vv, ww), rtol=1e-13, atol=1e-13 * np.abs(gp).max())
+
+
+def test_stretched_operator_is_not_symmetric(oracle_mod):
+    g, o, st, ops = _adjoint_case(oracle_mod, "stretched", 0, 33)
+    A = ops.A
+    assert abs(A - A.T).max() > 1e-3 * abs(A).max()
+
+
+@pytest.mark.parametrize("name,ibtype", [("walls", 0), ("stretched", 0), ("box2", 1)])
+def test_adjoint_dot_product_with_oracle_projection(oracle_mod, name, ibtype):
+    """<w, P u> = <P^T w, u> with P the oracle's converged mgpoisl and P^T
+    the sparse adjoint."""
+    g, o, st, ops = _adjoint_case(oracle_mod, name, ibtype, 34)
+    rng = np.random.default_rng(35)
+    x = rng.uniform(-1, 1, ops.nface)
+    wx = rng.standard_normal(ops.nface)
+    u, v, w = ops.scatter(x)
+    o.exchange([u, v, w], layers=2, corners=True)
+    r = o.mgpoisl(st, u, v, w, g.zeros(), 1e-3, nouter=4000, eps_rel=1e-13, ibtype=ibtype)
+    assert r["maxrhs"] <= 1e-13 * r["maxrhs0"]
+    lhs = wx @ ops.gather(u, v, w)
+    rhs = ops.adjoint(wx) @ x
+    assert abs(lhs - rhs) <= 1e-10 * np.abs(wx).sum() * np.abs(x).max()
+    # the adjoint is a projection too
+    y = ops.adjoint(wx)
+    assert np.allclose(ops.adjoint(y), y, rtol=0, atol=1e-10 * np.abs(y).max())
