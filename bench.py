@@ -132,6 +132,125 @@ def cpu_baseline(target_s=12.0):
                        f"OpenMP over blocks, {t:.1f} s")
 
 
+def adjoint_config():
+    """Adjoint projection at 256 blocks of 24^3 (8x8x4, periodic): U(-1,1)
+    velocity (numpy default_rng(5)), weights N(0,1) (rng 6), J = sum w u',
+    dJ/du through torch.autograd (the library's adjoint) against central
+    differences (h = 1e-6) in 64 random directions (rng 7) at a fixed
+    iteration count, for FP64 and FP32-refined; then a 20-step rollout
+    u <- P(u + 0.1 tanh u) differentiated with checkpoints every 5 steps and
+    without, and checked by central differences in 2 directions."""
+    import numpy as np
+    import torch
+    import mgp
+    import mgp.autograd as ag
+    from mgp.grids import uniform_box
+    g = uniform_box((8, 8, 4), (24, 24, 24), length=(1.0, 1.0, 0.5))
+    inner = torch.from_numpy(g.interior_mask()).cuda()
+
+    def field(rng, kind):
+        a = rng.uniform(-1, 1, g.n3d) if kind == "u" else rng.standard_normal(g.n3d)
+        return torch.where(inner, torch.from_numpy(a).cuda(), 0.0)
+
+    r5, r6, r7 = (np.random.default_rng(k) for k in (5, 6, 7))
+    u0 = [field(r5, "u") for _ in range(3)]
+    wts = [field(r6, "w") for _ in range(3)]
+    dirs = [[field(r7, "w") for _ in range(3)] for _ in range(64)]
+    out = {"config": "256 blocks of 24^3 (8x8x4 periodic), 3538944 cells", "h": 1e-6}
+    g64 = None
+    for form in ("fp64", "fp32"):
+        s = mgp.PressureSolver(g, ninner=5, nouter=4000, eps_rel=1e-12, precision=form,
+                               check_every=8)
+        x = [t.clone().requires_grad_(True) for t in u0]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = ag.project(s, *x, 1e-3)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sum((a * b).sum() for a, b in zip(wts, res[:3])).backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd, adj = s.last_stats, s.last_adjoint_stats
+        nfix = fwd["nouter_done"]
+        s.set_iterations(nouter=nfix, eps_rel=0.0)
+
+        def J(f):
+            with torch.no_grad():
+                o = ag.project(s, *f, 1e-3)
+            return float(sum((a * b).sum() for a, b in zip(wts, o[:3])))
+
+        errs = []
+        for d in dirs:
+            fd = (J([a + 1e-6 * b for a, b in zip(u0, d)]) -
+                  J([a - 1e-6 * b for a, b in zip(u0, d)])) / 2e-6
+            ad = float(sum((t.grad * b).sum() for t, b in zip(x, d)))
+            errs.append(abs(fd - ad) / max(abs(ad), 1e-300))
+            if len(errs) % 8 == 0:
+                log(f"[c5] {form}: {len(errs)} directions, max rel err {max(errs):.2e}")
+        log(f"[c5] {form}: fwd {t1 - t0:.2f} s ({nfix} outer), adjoint {t2 - t1:.2f} s "
+            f"({adj['nouter_done']} outer), FD rel err max {max(errs):.2e}")
+        out[form] = {"forward_s": round(t1 - t0, 3), "adjoint_s": round(t2 - t1, 3),
+                     "outer_iterations": nfix, "adjoint_outer_iterations": adj["nouter_done"],
+                     "fd_rel_err_max": float(max(errs)),
+                     "fd_rel_err_median": float(np.median(errs)), "directions": len(dirs)}
+        grads = [t.grad.clone() for t in x]
+        if g64 is None:
+            g64 = grads
+        else:
+            out["fp32_grad_rel_err_vs_fp64"] = (
+                max(float((a - b).abs().max()) for a, b in zip(grads, g64))
+                / max(float(b.abs().max()) for b in g64))
+        s.close()
+
+    # rollout with checkpointing
+    s = mgp.PressureSolver(g, ninner=5, nouter=4000, eps_rel=1e-12, check_every=8)
+
+    def step(a, b, c):
+        p = ag.project(s, a + 0.1 * torch.tanh(a), b + 0.1 * torch.tanh(b),
+                       c + 0.1 * torch.tanh(c), 1e-3)
+        return p[0], p[1], p[2]
+
+    grads, mem, tim = [], [], []
+    for every in (0, 5):
+        torch.cuda.reset_peak_memory_stats()
+        base = torch.cuda.memory_allocated()
+        x = [t.clone().requires_grad_(True) for t in u0]
+        t0 = time.perf_counter()
+        o = ag.rollout(step, x, 20, checkpoint_every=every)
+        sum((a * b).sum() for a, b in zip(wts, o)).backward()
+        torch.cuda.synchronize()
+        tim.append(time.perf_counter() - t0)
+        mem.append((torch.cuda.max_memory_allocated() - base) / 2**20)
+        grads.append([t.grad.clone() for t in x])
+        log(f"[c5] rollout checkpoint_every={every}: {tim[-1]:.1f} s, peak +{mem[-1]:.0f} MiB")
+    diff = max(float((a - b).abs().max()) for a, b in zip(*grads))
+    scale = max(float(a.abs().max()) for a in grads[0])
+    nfix = s.last_stats["nouter_done"]
+    s.set_iterations(nouter=nfix, eps_rel=0.0)
+
+    def Jr(f):
+        with torch.no_grad():
+            o = ag.rollout(step, f, 20)
+        return float(sum((a * b).sum() for a, b in zip(wts, o)))
+
+    fd_err = []
+    for d in dirs[:2]:
+        fd = (Jr([a + 1e-6 * b for a, b in zip(u0, d)]) -
+              Jr([a - 1e-6 * b for a, b in zip(u0, d)])) / 2e-6
+        ad = float(sum((t * b).sum() for t, b in zip(grads[1], d)))
+        fd_err.append(abs(fd - ad) / max(abs(ad), 1e-300))
+        log(f"[c5] rollout FD direction: rel err {fd_err[-1]:.2e}")
+    out["rollout"] = {"steps": 20, "checkpoint_every": 5,
+                      "grad_rel_diff_checkpointed_vs_plain": diff / scale,
+                      "seconds": {"plain": round(tim[0], 2), "checkpointed": round(tim[1], 2)},
+                      "peak_extra_mib": {"plain": round(mem[0], 1),
+                                         "checkpointed": round(mem[1], 1)},
+                      "fd_outer_iterations_fixed": nfix,
+                      "fd_rel_err_max": float(max(fd_err)), "fd_directions": len(fd_err)}
+    s.close()
+    print(json.dumps({"adjoint_projection": out}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,7 +264,12 @@ def main():
     ap.add_argument("--tts-cap", type=int, default=4000, help="max outer iterations")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp32,fp64")
+    ap.add_argument("--config", choices=["main", "c5"], default="main",
+                    help="c5: the adjoint-projection check (not the headline line)")
     args = ap.parse_args()
+    if args.config == "c5":
+        adjoint_config()
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (every block of the projection "

@@ -42,7 +42,7 @@ class Grids:
     """Topology + metrics of a set of blocks (one rank owns them all)."""
 
     def __init__(self, dims, level, nbr, bc, spacing, origin, minlevel=None,
-                 maxlevel=None):
+                 maxlevel=None, iparent=None, position=None):
         dims = np.asarray(dims, dtype=np.int32).reshape(-1, 3)  # (nx, ny, nz) interior
         self.ngrid = dims.shape[0]
         self.ii = (dims[:, 0] + 4).astype(np.int32)
@@ -51,6 +51,14 @@ class Grids:
         self.level = np.asarray(level, dtype=np.int32)
         self.nbr = np.ascontiguousarray(np.asarray(nbr, dtype=np.int32).reshape(self.ngrid, 26))
         self.bc = np.ascontiguousarray(np.asarray(bc, dtype=np.int32).reshape(self.ngrid, 6))
+        # parent grid id (0: none) and iposition/jposition/kposition: the
+        # parent's 1-based index (ghosts included) of the first parent cell
+        # the grid covers (GRIDINFO, src/core/gridio_mod.F90:15-37)
+        self.iparent = (np.zeros(self.ngrid, np.int32) if iparent is None
+                        else np.asarray(iparent, dtype=np.int32))
+        self.position = (np.zeros((self.ngrid, 3), np.int32) if position is None
+                         else np.ascontiguousarray(np.asarray(position, dtype=np.int32)
+                                                   .reshape(self.ngrid, 3)))
         self.minlevel = int(self.level.min()) if minlevel is None else minlevel
         self.maxlevel = int(self.level.max()) if maxlevel is None else maxlevel
         if np.any(np.diff(self.level) < 0):
@@ -197,4 +205,98 @@ def stretched_box(nb, n, length=(1.0, 1.0, 1.0), ratio=1.15, axis=1, wall=NOS):
     setattr(g, "rd" + names, 1.0 / dx)
     setattr(g, "rdd" + names, 1.0 / ddx)
     setattr(g, names + "stag", x + 0.5 * dx)
+    return g
+
+
+def refined_box(base_nb, base_n, levels, length=(1.0, 1.0, 1.0),
+                periodic=(True, True, True), wall=NOS, base_level=1):
+    """Nested grid hierarchy.  The base level is a box of `base_nb` blocks of
+    `base_n` cells (an int or a 3-tuple).  Each entry of `levels` is
+    (n, refine): every block of the previous level for which
+    refine(block) is true is covered by children of n cells per direction
+    (cell size halved, so a child covers n/2 parent cells per direction and
+    the children tile the parent).  A block is a dict with keys level,
+    origin (first interior cell in cells of its own level), n, centre.
+    Faces with a same-level neighbour are CON, faces on a non-periodic
+    domain boundary are walls, the remaining faces of refined blocks are
+    PAR (parent-coupled)."""
+    n0 = np.array(base_n if np.ndim(base_n) else (base_n,) * 3, dtype=np.int64)
+    nb0 = np.array(base_nb, dtype=np.int64)
+    L = np.asarray(length, dtype=np.float64)
+    cells0 = nb0 * n0
+    h0 = L / cells0
+    blocks = []
+    for bz in range(nb0[2]):
+        for by in range(nb0[1]):
+            for bx in range(nb0[0]):
+                o = np.array((bx, by, bz)) * n0
+                blocks.append(dict(level=base_level, origin=o, n=n0.copy(), parent=-1))
+    prev = list(range(len(blocks)))
+    for li, (n, refine) in enumerate(levels, start=1):
+        nn = np.array(n if np.ndim(n) else (n,) * 3, dtype=np.int64)
+        lvl = base_level + li
+        h = h0 / 2 ** li
+        cur = []
+        for pidx in prev:
+            P = blocks[pidx]
+            hp = h0 / 2 ** (li - 1)
+            P["centre"] = (P["origin"] + P["n"] / 2) * hp
+            if not refine(P):
+                continue
+            if np.any((2 * P["n"]) % nn):
+                raise ValueError("children must tile their parent")
+            m = 2 * P["n"] // nn
+            for cz in range(m[2]):
+                for cy in range(m[1]):
+                    for cx in range(m[0]):
+                        o = 2 * P["origin"] + np.array((cx, cy, cz)) * nn
+                        blocks.append(dict(level=lvl, origin=o, n=nn.copy(), parent=pidx))
+                        cur.append(len(blocks) - 1)
+        prev = cur
+    for B in blocks:
+        hl = h0 / 2 ** (B["level"] - base_level)
+        B["centre"] = (B["origin"] + B["n"] / 2) * hl
+    ngrid = len(blocks)
+    # lattice lookup per level
+    index = {}
+    for g, B in enumerate(blocks):
+        index[(B["level"], tuple(B["origin"] // B["n"]))] = g
+    nbr = np.zeros((ngrid, 26), dtype=np.int32)
+    bc = np.zeros((ngrid, 6), dtype=np.int32)
+    for g, B in enumerate(blocks):
+        li = B["level"] - base_level
+        span = cells0 * 2 ** li // B["n"]           # lattice size of the level
+        pos = B["origin"] // B["n"]
+        for d, off in enumerate(DIR_OFFSET):
+            q = pos + np.array(off)
+            ok = True
+            for a in range(3):
+                if q[a] < 0 or q[a] >= span[a]:
+                    if periodic[a]:
+                        q[a] %= span[a]
+                    else:
+                        ok = False
+            j = index.get((B["level"], tuple(q))) if ok else None
+            nbr[g, d] = 0 if j is None else j + 1
+        for f in range(6):
+            a = f // 2
+            edge = (pos[a] == span[a] - 1) if f % 2 else (pos[a] == 0)
+            if nbr[g, f]:
+                bc[g, f] = CON
+            elif edge and not periodic[a]:
+                bc[g, f] = wall
+            else:
+                bc[g, f] = PAR
+    dims = np.array([B["n"] for B in blocks])
+    level = np.array([B["level"] for B in blocks])
+    spacing = np.array([h0 / 2 ** (B["level"] - base_level) for B in blocks])
+    origin = np.array([B["origin"] * s for B, s in zip(blocks, spacing)])
+    iparent = np.array([B["parent"] + 1 for B in blocks])
+    position = np.zeros((ngrid, 3), dtype=np.int32)
+    for g, B in enumerate(blocks):
+        if B["parent"] >= 0:
+            P = blocks[B["parent"]]
+            position[g] = B["origin"] // 2 - P["origin"] + 3
+    g = Grids(dims, level, nbr, bc, spacing, origin, iparent=iparent, position=position)
+    g.blocks = blocks
     return g

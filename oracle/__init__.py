@@ -28,7 +28,7 @@ class OgTopo(C.Structure):
     _fields_ = [("ngrid", C.c_int32), ("minlevel", C.c_int32), ("maxlevel", C.c_int32),
                 ("kk", _i32p), ("jj", _i32p), ("ii", _i32p), ("level", _i32p),
                 ("nbr", _i32p), ("bc", _i32p), ("ip3", _i64p), ("ip1x", _i64p),
-                ("ip1y", _i64p), ("ip1z", _i64p)]
+                ("ip1y", _i64p), ("ip1z", _i64p), ("iparent", _i32p), ("pos", _i32p)]
 
 
 class OgConn(C.Structure):
@@ -61,6 +61,7 @@ def lib():
         _lib = C.CDLL(str(_LIB_PATH))
         _lib.og_maxabscal.restype = C.c_double
         _lib.og_time_sip.restype = C.c_double
+        _lib.og_par_buffers.restype = C.c_int64
     return _lib
 
 
@@ -75,11 +76,15 @@ class Oracle:
         self.g = grids
         self.lib = lib()
         g = grids
-        self._keep = [g.kk, g.jj, g.ii, g.level, g.nbr, g.bc, g.ip3, g.ip1x, g.ip1y, g.ip1z]
+        self._keep = [g.kk, g.jj, g.ii, g.level, g.nbr, g.bc, g.ip3, g.ip1x, g.ip1y, g.ip1z,
+                      g.iparent, g.position]
         self.topo = OgTopo(g.ngrid, g.minlevel, g.maxlevel, _p(g.kk, _i32p), _p(g.jj, _i32p),
                            _p(g.ii, _i32p), _p(g.level, _i32p), _p(g.nbr, _i32p),
                            _p(g.bc, _i32p), _p(g.ip3, _i64p), _p(g.ip1x, _i64p),
-                           _p(g.ip1y, _i64p), _p(g.ip1z, _i64p))
+                           _p(g.ip1y, _i64p), _p(g.ip1z, _i64p), _p(g.iparent, _i32p),
+                           _p(g.position, _i32p))
+        self.ipbb = np.zeros(6 * g.ngrid, dtype=np.int64)
+        self.nbuf = int(self.lib.og_par_buffers(C.byref(self.topo), _p(self.ipbb, _i64p)))
         cap = 26 * g.ngrid + 26
         self.conns = (OgConn * cap)()
         n = self.lib.og_connect_build(C.byref(self.topo), self.conns, cap)
@@ -210,13 +215,41 @@ class Oracle:
             _p(c["an"]), _p(c["ab"]), _p(c["at"]), _p(c["ap"]),
             *[_p(f[k]) for k in ("lw", "ls", "lb", "lpr", "ue", "un", "ut")],
             _p(st["mip"], _i64p), _p(st["idx"], _i64p), _p(st["rap"]), _p(g.rdx), _p(g.rdy),
-            _p(g.rdz), _p(g.rddx), _p(g.rddy), _p(g.rddz), *[_p(a) for a in work],
+            _p(g.rdz), _p(g.rddx), _p(g.rddy), _p(g.rddz), _p(g.dx), _p(g.dy), _p(g.dz),
+            _p(g.ddx), _p(g.ddy), _p(g.ddz), *[_p(a) for a in work],
             C.byref(stats))
         if rc != 0:
             raise RuntimeError(f"og_mgpoisl failed ({rc})")
         return dict(ipc=stats.ipc, nouter_done=stats.nouter_done, maxrhs0=stats.maxrhs0,
                     maxrhs=stats.maxrhs, history=hist[:stats.nouter_done].copy(),
                     dp=work[0], rhs=work[2])
+
+    # -- multi-level coupling ------------------------------------------
+    def par_buffer(self):
+        return np.zeros(max(self.nbuf, 1))
+
+    def ctof(self, level, f):
+        self.lib.og_ctof(C.byref(self.topo), level, _p(f), _p(f))
+
+    def parent(self, level, f, pbuf):
+        self.lib.og_parent(C.byref(self.topo), level, _p(f), _p(pbuf), _p(self.ipbb, _i64p))
+
+    def bound_pressure(self, level, p, pbuf, bp=None):
+        g = self.g
+        self.lib.og_bound_pressure(C.byref(self.topo), level, _p(p), _p(pbuf),
+                                   _p(self.ipbb, _i64p), _p(bp) if bp is not None else None,
+                                   _p(g.dx), _p(g.dy), _p(g.dz), _p(g.ddx), _p(g.ddy),
+                                   _p(g.ddz))
+
+    def ftoc(self, level, f):
+        g = self.g
+        self.lib.og_ftoc_s(C.byref(self.topo), level, _p(f), _p(f), _p(g.ddx), _p(g.ddy),
+                           _p(g.ddz))
+
+    def ftoc_uvwp(self, level, u, v, w, p):
+        g = self.g
+        self.lib.og_ftoc_uvwp(C.byref(self.topo), level, _p(u), _p(v), _p(w), _p(p),
+                              _p(g.ddx), _p(g.ddy), _p(g.ddz))
 
     def time_sip(self, st, rhs, niter, nthreads=0):
         """Seconds for `niter` hyperplane-SIP inner iterations (CPU baseline)."""

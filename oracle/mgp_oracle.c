@@ -919,11 +919,373 @@ int og_exchange(const og_topo *t, const og_conn *conns, int nconn, int level,
 }
 
 /* ------------------------------------------------------------------ */
+/* Multi-level coupling (levels are nested: a grid's parent is one level
+ * coarser; iposition.. is the parent's 1-based index of the first parent
+ * cell the grid covers, src/core/gridio_mod.F90:15-37). */
+#define PAR_BC 8
+
+/* PAR face buffers: offsets (-1 = none) and total length; one buffer per
+ * PAR face of kk*jj / kk*ii / jj*ii values (src/core/pointers_mod.F90:
+ * 64-92). */
+int64_t og_par_buffers(const og_topo *t, int64_t *ipbb)
+{
+    int64_t n = 0;
+    for (int g = 1; g <= t->ngrid; ++g) {
+        GDIMS(t, g);
+        for (int f = 1; f <= 6; ++f) {
+            int64_t *o = &ipbb[6 * (g - 1) + f - 1];
+            if (t->bc[6 * (g - 1) + f - 1] != PAR_BC) { *o = -1; continue; }
+            *o = n;
+            n += f <= 2 ? (int64_t)kk * jj : f <= 4 ? (int64_t)kk * ii : (int64_t)jj * ii;
+        }
+    }
+    return n;
+}
+
+#define POS(g, a) (t->pos[3 * ((g)-1) + (a)])
+
+/* ctof1: write_buffer box (src/ib/ctof/ctof1_mod.F90:232-262) and
+ * unpack_single (:308-331): piecewise-constant injection of the parent's
+ * values into the whole fine array, ghost layers included. */
+void og_ctof(const og_topo *t, int level, double *ff_, const double *fc_)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        if (t->level[g - 1] != level) continue;
+        const int c = t->iparent[g - 1];
+        if (c == 0) continue;
+        GDIMS(t, g);
+        const int ipos = POS(g, 0), jpos = POS(g, 1), kpos = POS(g, 2);
+        const int64_t kkc = t->kk[c - 1], jjc = t->jj[c - 1];
+        double *ff = ff_ + t->ip3[g - 1];
+        const double *fc = fc_ + t->ip3[c - 1];
+        for (int i = 1; i <= ii; ++i)
+            for (int j = 1; j <= jj; ++j)
+                for (int k = 1; k <= kk; ++k) {
+                    const int ic = ipos - 1 + (i - 1) / 2;
+                    const int jc = jpos - 1 + (j - 1) / 2;
+                    const int kc = kpos - 1 + (k - 1) / 2;
+                    ff[IX(k, j, i)] = fc[(kc - 1) + (jc - 1) * kkc + (ic - 1) * kkc * jjc];
+                }
+    }
+}
+
+/* parent for one scalar field: the coarse slab of start_and_stop
+ * (src/ib/parent/parent_core_mod.F90:214-262, icomp = 0), packed as in
+ * parent1 pack_single (src/ib/parent/parent1_mod.F90:296-338) and prolonged
+ * to the fine face with prolong1/prolong2, istag = 0 (:523-583): fine face
+ * cell (a, b) takes slab cell (2 + (a-1)/2, 2 + (b-1)/2). */
+void og_parent(const og_topo *t, int level, const double *f_, double *pbuf,
+               const int64_t *ipbb)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        if (t->level[g - 1] != level) continue;
+        const int c = t->iparent[g - 1];
+        if (c == 0) continue;
+        GDIMS(t, g);
+        const int ipos = POS(g, 0), jpos = POS(g, 1), kpos = POS(g, 2);
+        const int64_t kkc = t->kk[c - 1], jjc = t->jj[c - 1];
+        const double *fc = f_ + t->ip3[c - 1];
+#define FC(k, j, i) fc[((k)-1) + ((j)-1) * kkc + ((i)-1) * kkc * jjc]
+        for (int f = 1; f <= 6; ++f) {
+            const int64_t o = ipbb[6 * (g - 1) + f - 1];
+            if (o < 0) continue;
+            double *pb = pbuf + o;
+            int ks = kpos - 2, js = jpos - 2, is = ipos - 2;
+            switch (f) {
+            case 1: is = ipos - 1; break;
+            case 2: is = ipos + (ii - 4) / 2; break;
+            case 3: js = jpos - 1; break;
+            case 4: js = jpos + (jj - 4) / 2; break;
+            case 5: ks = kpos - 1; break;
+            case 6: ks = kpos + (kk - 4) / 2; break;
+            }
+            if (f <= 2) {
+                for (int j = 1; j <= jj; ++j)
+                    for (int k = 1; k <= kk; ++k)
+                        pb[(k - 1) + (int64_t)(j - 1) * kk] =
+                            FC(ks + 1 + (k - 1) / 2, js + 1 + (j - 1) / 2, is);
+            } else if (f <= 4) {
+                for (int i = 1; i <= ii; ++i)
+                    for (int k = 1; k <= kk; ++k)
+                        pb[(k - 1) + (int64_t)(i - 1) * kk] =
+                            FC(ks + 1 + (k - 1) / 2, js, is + 1 + (i - 1) / 2);
+            } else {
+                for (int i = 1; i <= ii; ++i)
+                    for (int j = 1; j <= jj; ++j)
+                        pb[(j - 1) + (int64_t)(i - 1) * jj] =
+                            FC(ks, js + 1 + (j - 1) / 2, is + 1 + (i - 1) / 2);
+            }
+        }
+#undef FC
+    }
+}
+
+/* pressureftocone_A / _B: src/flow/bound_pressure_mod.F90:536-608 */
+static void pftocone(int k, int j, int i, int kk, int jj, const double *p,
+                     const double *bp, const double *ddx, const double *ddy,
+                     const double *ddz, double *pc, double *bpc)
+{
+    if (!bp) {
+        *bpc = 1.0;
+        double sump = 0.0, sumvol = 0.0;
+        for (int l = 0; l <= 1; ++l)
+            for (int m = 0; m <= 1; ++m)
+                for (int n = 0; n <= 1; ++n) {
+                    const double vol = ddz[k + n] * ddy[j + m] * ddx[i + l];
+                    sump = sump + p[IX(k + n, j + m, i + l)] * vol;
+                    sumvol = sumvol + vol;
+                }
+        *pc = sump / sumvol;
+        return;
+    }
+    double sumbp = 0.0;
+    for (int l = 0; l <= 1; ++l)
+        for (int m = 0; m <= 1; ++m)
+            for (int n = 0; n <= 1; ++n)
+                sumbp = sumbp + bp[IX(k + n, j + m, i + l)];
+    *bpc = sumbp < 1.0 ? sumbp : 1.0;
+    if (*bpc < 0.5) {
+        *pc = 0.0;
+        return;
+    }
+    double sump = 0.0, sumvol = 0.0;
+    for (int l = 0; l <= 1; ++l)
+        for (int m = 0; m <= 1; ++m)
+            for (int n = 0; n <= 1; ++n) {
+                const double vol = bp[IX(k + n, j + m, i + l)] * ddz[k + n] *
+                                   ddy[j + m] * ddx[i + l];
+                sump = sump + p[IX(k + n, j + m, i + l)] * vol;
+                sumvol = sumvol + vol;
+            }
+    *pc = sump / sumvol;
+}
+
+/* bound_pressure: src/flow/bound_pressure_mod.F90:106-534 (bfront,
+ * bright, bbottom and their _bp forms); bp NULL selects the A forms.
+ * Metric pointers are 1-based. */
+void og_bound_pressure(const og_topo *t, int level, double *p_,
+                       const double *pbuf, const int64_t *ipbb,
+                       const double *bp_, const double *dx_,
+                       const double *dy_, const double *dz_,
+                       const double *ddx_, const double *ddy_,
+                       const double *ddz_)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        if (t->level[g - 1] != level) continue;
+        GDIMS(t, g);
+        double *p = p_ + t->ip3[g - 1];
+        const double *bp = bp_ ? bp_ + t->ip3[g - 1] : NULL;
+        const double *dx = dx_ + t->ip1x[g - 1] - 1, *ddx = ddx_ + t->ip1x[g - 1] - 1;
+        const double *dy = dy_ + t->ip1y[g - 1] - 1, *ddy = ddy_ + t->ip1y[g - 1] - 1;
+        const double *dz = dz_ + t->ip1z[g - 1] - 1, *ddz = ddz_ + t->ip1z[g - 1] - 1;
+        for (int f = 1; f <= 6; ++f) {
+            const int64_t o = ipbb[6 * (g - 1) + f - 1];
+            if (o < 0) continue;
+            const double *pb = pbuf + o;
+            if (f <= 2) {
+                const int i2 = f == 1 ? 2 : ii - 1, i3 = f == 1 ? 3 : ii - 2;
+                const int i4 = f == 1 ? 4 : ii - 3, is2 = f == 1 ? 2 : ii - 2;
+                const int i = i3 < i4 ? i3 : i4;
+                for (int j = 3; j <= jj - 2; j += 2)
+                    for (int k = 3; k <= kk - 2; k += 2) {
+                        double pcnew, bpc, fak = 1.0;
+                        pftocone(k, j, i, kk, jj, p, bp, ddx, ddy, ddz, &pcnew, &bpc);
+                        const double pbv = pb[(k - 1) + (int64_t)(j - 1) * kk];
+                        if (bp) {
+                            if (bpc < 0.5) pcnew = pbv;
+                            const double sb11 = bp[IX(k, j, i2)] * bp[IX(k, j, i3)];
+                            const double sb12 = bp[IX(k, j + 1, i2)] * bp[IX(k, j + 1, i3)];
+                            const double sb13 = bp[IX(k + 1, j, i2)] * bp[IX(k + 1, j, i3)];
+                            const double sb14 = bp[IX(k + 1, j + 1, i2)] * bp[IX(k + 1, j + 1, i3)];
+                            fak = (sb11 * ddy[j] * ddz[k] + sb12 * ddy[j + 1] * ddz[k] +
+                                   sb13 * ddy[j] * ddz[k + 1] + sb14 * ddy[j + 1] * ddz[k + 1]) /
+                                  ((ddy[j] + ddy[j + 1]) * (ddz[k] + ddz[k + 1]));
+                            if (fak < 0.1) fak = 1.0;
+                            fak = 1.0 / fak;
+                        }
+                        for (int m = 0; m <= 1; ++m)
+                            for (int n = 0; n <= 1; ++n)
+                                p[IX(k + n, j + m, i2)] = bp
+                                    ? p[IX(k + n, j + m, i3)] + fak * dx[is2] / (ddx[i3] + ddx[i2]) * (pbv - pcnew)
+                                    : p[IX(k + n, j + m, i3)] + dx[is2] / (ddx[i3] + ddx[i2]) * (pbv - pcnew);
+                    }
+            } else if (f <= 4) {
+                const int j2 = f == 3 ? 2 : jj - 1, j3 = f == 3 ? 3 : jj - 2;
+                const int j4 = f == 3 ? 4 : jj - 3, js2 = f == 3 ? 2 : jj - 2;
+                const int j = j3 < j4 ? j3 : j4;
+                for (int i = 3; i <= ii - 2; i += 2)
+                    for (int k = 3; k <= kk - 2; k += 2) {
+                        double pcnew, bpc, fak = 1.0;
+                        pftocone(k, j, i, kk, jj, p, bp, ddx, ddy, ddz, &pcnew, &bpc);
+                        const double pbv = pb[(k - 1) + (int64_t)(i - 1) * kk];
+                        if (bp) {
+                            if (bpc < 0.5) pcnew = pbv;
+                            const double sb11 = bp[IX(k, j2, i)] * bp[IX(k, j3, i)];
+                            const double sb12 = bp[IX(k, j2, i + 1)] * bp[IX(k, j3, i + 1)];
+                            const double sb13 = bp[IX(k + 1, j2, i)] * bp[IX(k + 1, j3, i)];
+                            const double sb14 = bp[IX(k + 1, j2, i + 1)] * bp[IX(k + 1, j3, i + 1)];
+                            fak = (sb11 * ddx[i] * ddz[k] + sb12 * ddx[i + 1] * ddz[k] +
+                                   sb13 * ddx[i] * ddz[k + 1] + sb14 * ddx[i + 1] * ddz[k + 1]) /
+                                  ((ddx[i] + ddx[i + 1]) * (ddz[k] + ddz[k + 1]));
+                            if (fak < 0.1) fak = 1.0;
+                            fak = 1.0 / fak;
+                        }
+                        for (int l = 0; l <= 1; ++l)
+                            for (int n = 0; n <= 1; ++n)
+                                p[IX(k + n, j2, i + l)] = bp
+                                    ? p[IX(k + n, j3, i + l)] + fak * dy[js2] / (ddy[j3] + ddy[j2]) * (pbv - pcnew)
+                                    : p[IX(k + n, j3, i + l)] + dy[js2] / (ddy[j3] + ddy[j2]) * (pbv - pcnew);
+                    }
+            } else {
+                const int k2 = f == 5 ? 2 : kk - 1, k3 = f == 5 ? 3 : kk - 2;
+                const int k4 = f == 5 ? 4 : kk - 3, ks2 = f == 5 ? 2 : kk - 2;
+                const int k = k3 < k4 ? k3 : k4;
+                for (int i = 3; i <= ii - 2; i += 2)
+                    for (int j = 3; j <= jj - 2; j += 2) {
+                        double pcnew, bpc, fak = 1.0;
+                        pftocone(k, j, i, kk, jj, p, bp, ddx, ddy, ddz, &pcnew, &bpc);
+                        const double pbv = pb[(j - 1) + (int64_t)(i - 1) * jj];
+                        if (bp) {
+                            if (bpc < 0.5) pcnew = pbv;
+                            const double sb11 = bp[IX(k2, j, i)] * bp[IX(k3, j, i)];
+                            const double sb12 = bp[IX(k2, j, i + 1)] * bp[IX(k3, j, i + 1)];
+                            const double sb13 = bp[IX(k2, j + 1, i)] * bp[IX(k3, j + 1, i)];
+                            const double sb14 = bp[IX(k2, j + 1, i + 1)] * bp[IX(k3, j + 1, i + 1)];
+                            fak = (sb11 * ddx[i] * ddy[j] + sb12 * ddx[i + 1] * ddy[j] +
+                                   sb13 * ddx[i] * ddy[j + 1] + sb14 * ddx[i + 1] * ddy[j + 1]) /
+                                  ((ddx[i] + ddx[i + 1]) * (ddy[j] + ddy[j + 1]));
+                            if (fak < 0.1) fak = 1.0;
+                            fak = 1.0 / fak;
+                        }
+                        for (int l = 0; l <= 1; ++l)
+                            for (int m = 0; m <= 1; ++m)
+                                p[IX(k2, j + m, i + l)] = bp
+                                    ? p[IX(k3, j + m, i + l)] + fak * dz[ks2] / (ddz[k3] + ddz[k2]) * (pbv - pcnew)
+                                    : p[IX(k3, j + m, i + l)] + dz[ks2] / (ddz[k3] + ddz[k2]) * (pbv - pcnew);
+                    }
+            }
+        }
+    }
+}
+
+/* ftoc of a cell-centred scalar ('P', 'R'): restrict_s
+ * (src/ib/noib_restrict_mod.F90:255-298) into restrict_recieve_open
+ * (src/ib/ftoc_mod.F90:581-626): volume-weighted mean of each 2x2x2 group
+ * of fine interior cells replaces the parent cell under it. */
+void og_ftoc_s(const og_topo *t, int level, const double *ff_, double *fc_,
+               const double *ddx_, const double *ddy_, const double *ddz_)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        if (t->level[g - 1] != level) continue;
+        const int c = t->iparent[g - 1];
+        if (c == 0) continue;
+        GDIMS(t, g);
+        const int ipos = POS(g, 0), jpos = POS(g, 1), kpos = POS(g, 2);
+        const int64_t kkc = t->kk[c - 1], jjc = t->jj[c - 1];
+        const double *ff = ff_ + t->ip3[g - 1];
+        double *fc = fc_ + t->ip3[c - 1];
+        const double *ddx = ddx_ + t->ip1x[g - 1] - 1, *ddy = ddy_ + t->ip1y[g - 1] - 1,
+                     *ddz = ddz_ + t->ip1z[g - 1] - 1;
+        for (int i = 3; i <= ii - 3; i += 2)
+            for (int j = 3; j <= jj - 3; j += 2)
+                for (int k = 3; k <= kk - 3; k += 2) {
+                    const double spv =
+                        ff[IX(k, j, i)] * ddz[k] * ddy[j] * ddx[i] +
+                        ff[IX(k, j, i + 1)] * ddz[k] * ddy[j] * ddx[i + 1] +
+                        ff[IX(k, j + 1, i)] * ddz[k] * ddy[j + 1] * ddx[i] +
+                        ff[IX(k, j + 1, i + 1)] * ddz[k] * ddy[j + 1] * ddx[i + 1] +
+                        ff[IX(k + 1, j, i)] * ddz[k + 1] * ddy[j] * ddx[i] +
+                        ff[IX(k + 1, j, i + 1)] * ddz[k + 1] * ddy[j] * ddx[i + 1] +
+                        ff[IX(k + 1, j + 1, i)] * ddz[k + 1] * ddy[j + 1] * ddx[i] +
+                        ff[IX(k + 1, j + 1, i + 1)] * ddz[k + 1] * ddy[j + 1] * ddx[i + 1];
+                    const double sv = (ddx[i] + ddx[i + 1]) * (ddy[j] + ddy[j + 1]) *
+                                      (ddz[k] + ddz[k + 1]);
+                    const int ic = ipos + (i - 3) / 2, jc = jpos + (j - 3) / 2,
+                              kc = kpos + (k - 3) / 2;
+                    fc[(kc - 1) + (jc - 1) * kkc + (ic - 1) * kkc * jjc] = spv / sv;
+                }
+    }
+}
+
+/* ftoc(ilevel, u, v, w, p): restrict_u/v/w/s (src/ib/noib_restrict_mod.F90:
+ * 135-298) with the staggered offsets of restrict_recieve_open
+ * (src/ib/ftoc_mod.F90:581-626). */
+void og_ftoc_uvwp(const og_topo *t, int level, double *u_, double *v_,
+                  double *w_, double *p_, const double *ddx_,
+                  const double *ddy_, const double *ddz_)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        if (t->level[g - 1] != level) continue;
+        const int c = t->iparent[g - 1];
+        if (c == 0) continue;
+        GDIMS(t, g);
+        const int ipos = POS(g, 0), jpos = POS(g, 1), kpos = POS(g, 2);
+        const int64_t kkc = t->kk[c - 1], jjc = t->jj[c - 1];
+        const int64_t of = t->ip3[g - 1], oc = t->ip3[c - 1];
+        const double *ddx = ddx_ + t->ip1x[g - 1] - 1, *ddy = ddy_ + t->ip1y[g - 1] - 1,
+                     *ddz = ddz_ + t->ip1z[g - 1] - 1;
+#define CI(kc, jc, ic) (oc + ((kc)-1) + ((jc)-1) * kkc + ((ic)-1) * kkc * jjc)
+        {   /* U */
+            const double *ff = u_ + of;
+            for (int i = 2; i <= ii - 2; i += 2)
+                for (int j = 3; j <= jj - 3; j += 2)
+                    for (int k = 3; k <= kk - 3; k += 2) {
+                        const double sa = ff[IX(k, j, i)] * ddy[j] * ddz[k] +
+                                          ff[IX(k, j + 1, i)] * ddy[j + 1] * ddz[k] +
+                                          ff[IX(k + 1, j, i)] * ddy[j] * ddz[k + 1] +
+                                          ff[IX(k + 1, j + 1, i)] * ddy[j + 1] * ddz[k + 1];
+                        const double a = (ddy[j] + ddy[j + 1]) * (ddz[k] + ddz[k + 1]);
+                        u_[CI(kpos + (k - 3) / 2, jpos + (j - 3) / 2,
+                              ipos + (i - 2) / 2 - 1)] = sa / a;
+                    }
+        }
+        {   /* V */
+            const double *ff = v_ + of;
+            for (int i = 3; i <= ii - 3; i += 2)
+                for (int j = 2; j <= jj - 2; j += 2)
+                    for (int k = 3; k <= kk - 3; k += 2) {
+                        const double sa = ff[IX(k, j, i)] * ddx[i] * ddz[k] +
+                                          ff[IX(k, j, i + 1)] * ddx[i + 1] * ddz[k] +
+                                          ff[IX(k + 1, j, i)] * ddx[i] * ddz[k + 1] +
+                                          ff[IX(k + 1, j, i + 1)] * ddx[i + 1] * ddz[k + 1];
+                        const double a = (ddx[i] + ddx[i + 1]) * (ddz[k] + ddz[k + 1]);
+                        v_[CI(kpos + (k - 3) / 2, jpos + (j - 2) / 2 - 1,
+                              ipos + (i - 3) / 2)] = sa / a;
+                    }
+        }
+        {   /* W */
+            const double *ff = w_ + of;
+            for (int i = 3; i <= ii - 3; i += 2)
+                for (int j = 3; j <= jj - 3; j += 2)
+                    for (int k = 2; k <= kk - 2; k += 2) {
+                        const double sa = ff[IX(k, j, i)] * ddx[i] * ddy[j] +
+                                          ff[IX(k, j, i + 1)] * ddx[i + 1] * ddy[j] +
+                                          ff[IX(k, j + 1, i)] * ddx[i] * ddy[j + 1] +
+                                          ff[IX(k, j + 1, i + 1)] * ddx[i + 1] * ddy[j + 1];
+                        const double a = (ddx[i] + ddx[i + 1]) * (ddy[j] + ddy[j + 1]);
+                        w_[CI(kpos + (k - 2) / 2 - 1, jpos + (j - 3) / 2,
+                              ipos + (i - 3) / 2)] = sa / a;
+                    }
+        }
+#undef CI
+    }
+    og_ftoc_s(t, level, p_, p_, ddx_, ddy_, ddz_);
+}
+
+/* ------------------------------------------------------------------ */
 static int64_t field_len(const og_topo *t)
 {
     const int g = t->ngrid;
     return t->ip3[g - 1] + (int64_t)t->kk[g - 1] * t->jj[g - 1] * t->ii[g - 1];
 }
+
+/* Everything the multi-level steps need besides the SIP operands. */
+typedef struct ml_ctx {
+    const int64_t *ipbb;
+    double *pbuf;
+    const double *bp;   /* NULL: bound_pressure A forms (no IB)          */
+    const double *dx, *dy, *dz, *ddx, *ddy, *ddz;
+} ml_ctx;
 
 /* src/flow/pressuresolver_mod.F90:423-465 (sip) and :361-420 (mgpoisit) */
 static void mgpoisit(const og_topo *t, const og_conn *c, int nconn,
@@ -934,10 +1296,11 @@ static void mgpoisit(const og_topo *t, const og_conn *c, int nconn,
                      const double *ap, const double *lw, const double *ls,
                      const double *lb, const double *lpr, const double *ue,
                      const double *un, const double *ut, const int64_t *mip,
-                     const int64_t *idx, const double *rap)
+                     const int64_t *idx, const double *rap, const ml_ctx *ml)
 {
     for (int iloop = 1; iloop <= prm->ninner; ++iloop) {
-        /* bound_pressure: no PAR faces on the supported topologies */
+        og_bound_pressure(t, ilevel, dp, ml->pbuf, ml->ipbb, ml->bp, ml->dx,
+                          ml->dy, ml->dz, ml->ddx, ml->ddy, ml->ddz);
         const int use_sor = prm->ityp == 1 &&
                             (ilevel > t->minlevel || prm->sor_all_levels);
         if (use_sor) {
@@ -960,6 +1323,8 @@ static void mgpoisit(const og_topo *t, const og_conn *c, int nconn,
         double *fd[1] = {dp};
         og_exchange(t, c, nconn, ilevel, 1, 0, 0, fd, 1);
     }
+    og_bound_pressure(t, ilevel, dp, ml->pbuf, ml->ipbb, ml->bp, ml->dx,
+                      ml->dy, ml->dz, ml->ddx, ml->ddy, ml->ddz);
 }
 
 /* src/flow/pressuresolver_mod.F90:160-357 */
@@ -974,6 +1339,8 @@ int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
                const int64_t *idx, const double *rap,
                const double *rdx, const double *rdy, const double *rdz,
                const double *rddx, const double *rddy, const double *rddz,
+               const double *dx, const double *dy, const double *dz,
+               const double *ddx, const double *ddy, const double *ddz,
                double *dp, double *hilf, double *rhs, double *res,
                og_stats *st)
 {
@@ -982,6 +1349,15 @@ int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
     memset(hilf, 0, (size_t)n * sizeof(double));
     memset(rhs, 0, (size_t)n * sizeof(double));
     memset(res, 0, (size_t)n * sizeof(double));
+
+    int64_t *ipbb = malloc(sizeof(int64_t) * 6 * (size_t)t->ngrid);
+    const int64_t nbuf = og_par_buffers(t, ipbb);
+    /* face buffers of HILF and of DP (fields carry their own buffers) */
+    double *pbh = calloc((size_t)(nbuf > 0 ? nbuf : 1), sizeof(double));
+    double *pbd = calloc((size_t)(nbuf > 0 ? nbuf : 1), sizeof(double));
+    ml_ctx mh = {ipbb, pbh, prm->ibtype ? bp : NULL, dx, dy, dz, ddx, ddy, ddz};
+    ml_ctx md = mh;
+    md.pbuf = pbd;
 
     const double prefak = prm->rho / dt;
     if (prm->ibtype == 0) {
@@ -993,18 +1369,27 @@ int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
         og_divcal(t, rhs, u, v, w, rddx, rddy, rddz, prefak, bp, sdiv);
         free(sdiv);
     }
+    for (int l = t->maxlevel; l >= t->minlevel; --l)
+        og_ftoc_s(t, l, rhs, rhs, ddx, ddy, ddz);
     st->maxrhs0 = og_maxabscal(t, rhs, NULL);
 
     int ipc = 0, ipcount;
     double maxrhs = st->maxrhs0;
     for (ipcount = 1; ipcount <= prm->nouter; ++ipcount) {
-        for (int l = t->minlevel; l <= t->maxlevel; ++l)
+        for (int l = t->minlevel; l <= t->maxlevel; ++l) {
+            og_ctof(t, l, hilf, hilf);
+            og_parent(t, l, hilf, pbh, ipbb);
             mgpoisit(t, c, nconn, prm, l, hilf, rhs, res, bp, aw, ae, as, an,
-                     ab, at, ap, lw, ls, lb, lpr, ue, un, ut, mip, idx, rap);
+                     ab, at, ap, lw, ls, lb, lpr, ue, un, ut, mip, idx, rap, &mh);
+        }
+        for (int l = t->maxlevel; l >= t->minlevel; --l)
+            og_ftoc_s(t, l, hilf, hilf, ddx, ddy, ddz);
         double *fh[1] = {hilf};
         og_exchange(t, c, nconn, -1, 1, 0, 0, fh, 1);
         og_laplacephi(t, -1, res, hilf, aw, ae, as, an, ab, at, ap, bp);
         og_rescal(t, rhs, res);
+        for (int l = t->maxlevel; l >= t->minlevel + 1; --l)
+            og_ftoc_s(t, l, rhs, rhs, ddx, ddy, ddz);
         maxrhs = og_maxabscal(t, rhs, NULL);
         og_accumulate(n, dp, hilf);
         memset(hilf, 0, (size_t)n * sizeof(double));
@@ -1023,11 +1408,20 @@ int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
     st->nouter_done = ipcount - 1;
     st->maxrhs = maxrhs;
 
+    for (int l = t->minlevel; l <= t->maxlevel; ++l) {
+        og_parent(t, l, dp, pbd, ipbb);
+        og_bound_pressure(t, l, dp, pbd, ipbb, md.bp, dx, dy, dz, ddx, ddy, ddz);
+    }
     og_mgpcorr(t, u, v, w, p, dp, rdx, rdy, rdz, dt / prm->rho,
                prm->ibtype ? bp : NULL);
+    for (int l = t->maxlevel; l >= t->minlevel; --l)
+        og_ftoc_uvwp(t, l, u, v, w, p, ddx, ddy, ddz);
     double *fs[4] = {u, v, w, p};
     for (int l = t->minlevel; l <= t->maxlevel; ++l)
         og_exchange(t, c, nconn, l, 2, 1, 0, fs, 4);
+    free(pbh);
+    free(pbd);
+    free(ipbb);
     return 0;
 }
 

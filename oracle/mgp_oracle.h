@@ -36,6 +36,9 @@ typedef struct og_topo {
     const int32_t *bc;                 /* [ngrid*6]  itypboconds(1,f,g)    */
     const int64_t *ip3;                /* [ngrid] 0-based 3-D offsets      */
     const int64_t *ip1x, *ip1y, *ip1z; /* [ngrid] 0-based 1-D offsets      */
+    const int32_t *iparent;            /* [ngrid] parent grid id, 0 = none */
+    const int32_t *pos;                /* [ngrid*3] iposition, jposition,  */
+                                       /*   kposition (1-based, in parent) */
 } og_topo;
 
 /* One connection record (src/core/connect/connect_core_mod.F90:19-31).   */
@@ -143,8 +146,33 @@ int og_exchange(const og_topo *t, const og_conn *c, int nconn, int level,
             int layers, int corners, int forward, double **fields,
             int nfield);
 
-/* src/flow/pressuresolver_mod.F90:160-357, single rank, levels without
- * PAR faces.  Work arrays dp, hilf, rhs, res are full 3-D fields. */
+/* Multi-level coupling (grids need iparent/pos).  PAR face buffers:
+ * og_par_buffers fills ipbb[6*ngrid] (-1 = no buffer) and returns the
+ * total length.  Metric pointers are the flat 1-D fields. */
+int64_t og_par_buffers(const og_topo *t, int64_t *ipbb);
+/* src/ib/ctof/ctof1_mod.F90:232-331 */
+void og_ctof(const og_topo *t, int level, double *ff, const double *fc);
+/* src/ib/parent/parent1_mod.F90:296-583 (one scalar field) */
+void og_parent(const og_topo *t, int level, const double *f, double *pbuf,
+               const int64_t *ipbb);
+/* src/flow/bound_pressure_mod.F90:106-608 (bp NULL: A forms) */
+void og_bound_pressure(const og_topo *t, int level, double *p,
+                       const double *pbuf, const int64_t *ipbb,
+                       const double *bp, const double *dx, const double *dy,
+                       const double *dz, const double *ddx, const double *ddy,
+                       const double *ddz);
+/* src/ib/noib_restrict_mod.F90:255-298 + src/ib/ftoc_mod.F90:581-626 */
+void og_ftoc_s(const og_topo *t, int level, const double *ff, double *fc,
+               const double *ddx, const double *ddy, const double *ddz);
+/* ftoc(ilevel, u, v, w, p): src/ib/noib_restrict_mod.F90:135-298 */
+void og_ftoc_uvwp(const og_topo *t, int level, double *u, double *v,
+                  double *w, double *p, const double *ddx, const double *ddy,
+                  const double *ddz);
+
+/* src/flow/pressuresolver_mod.F90:160-357, single rank, all levels (the
+ * final per-level parent/bound_flow of U, V, W, P belong to the caller;
+ * the per-level 2-layer connect with corners is done here).  Work arrays
+ * dp, hilf, rhs, res are full 3-D fields. */
 int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
                const og_params *prm, double *u, double *v, double *w,
                double *p, double dt, const double *bp,
@@ -156,6 +184,8 @@ int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
                const int64_t *idx, const double *rap,
                const double *rdx, const double *rdy, const double *rdz,
                const double *rddx, const double *rddy, const double *rddz,
+               const double *dx, const double *dy, const double *dz,
+               const double *ddx, const double *ddy, const double *ddz,
                double *dp, double *hilf, double *rhs, double *res,
                og_stats *st);
 

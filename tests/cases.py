@@ -73,3 +73,27 @@ def adjoint_case(oracle_mod, name, ibtype, seed):
         o.exchange([bp], layers=2, corners=True)
     st = o.setup(bp=bp)
     return g, o, st, Operators(o, st, ibtype), bp
+
+
+def two_level(periodic=False, n=8):
+    """2x2x2 blocks of n^3 on level 1; the first block refined into 8
+    children of n^3 (level 2), whose outer faces inside the domain are PAR."""
+    from mgp.grids import refined_box
+    return refined_box((2, 2, 2), n, [(n, lambda B: tuple(B["origin"]) == (0, 0, 0))],
+                       periodic=(periodic,) * 3)
+
+
+def centres(g):
+    """Cell-centre coordinates (x, y, z) of every stored position."""
+    X = np.zeros(g.n3d)
+    Y = np.zeros(g.n3d)
+    Z = np.zeros(g.n3d)
+    for b in range(1, g.ngrid + 1):
+        ii, jj, kk = g.ii[b - 1], g.jj[b - 1], g.kk[b - 1]
+        x = g.x[g.ip1x[b - 1]:g.ip1x[b - 1] + ii]
+        y = g.y[g.ip1y[b - 1]:g.ip1y[b - 1] + jj]
+        z = g.z[g.ip1z[b - 1]:g.ip1z[b - 1] + kk]
+        g.block_view(X, b)[...] = x[:, None, None]
+        g.block_view(Y, b)[...] = y[None, :, None]
+        g.block_view(Z, b)[...] = z[None, None, :]
+    return X, Y, Z

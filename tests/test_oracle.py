@@ -496,3 +496,147 @@ def test_adjoint_dot_product_with_oracle_projection(oracle_mod, name, ibtype):
     # the adjoint is a projection too
     y = ops.adjoint(wx)
     assert np.allclose(ops.adjoint(y), y, rtol=0, atol=1e-10 * np.abs(y).max())
+
+
+# ---------------------------------------------------------------- levels
+def test_ctof_injects_the_parent_cell(oracle_mod):
+    from cases import centres, two_level
+    g = two_level()
+    o = oracle_mod.Oracle(g)
+    X, Y, Z = centres(g)
+    coarse = 0.3 + 1.7 * X - 0.9 * Y + 2.3 * Z
+    coarse[g.ip3[8]:] = 0.0                      # fine grids: unknown
+    o.exchange([coarse], layers=2, corners=True)
+    o.ctof(2, coarse)
+    # every fine cell (ghosts included) holds the value of the coarse cell
+    # whose volume contains the fine cell centre
+    h1 = g.spacing[0, 0]
+    for b in range(9, 17):
+        fv = g.block_view(coarse, b)
+        xf, yf, zf = (a[g.ip1x[b - 1]:g.ip1x[b - 1] + g.ii[b - 1]] if k == 0 else
+                      a[g.ip1y[b - 1]:g.ip1y[b - 1] + g.jj[b - 1]] if k == 1 else
+                      a[g.ip1z[b - 1]:g.ip1z[b - 1] + g.kk[b - 1]]
+                      for k, a in enumerate((g.x, g.y, g.z)))
+        cx = (np.floor(xf / h1) + 0.5) * h1
+        cy = (np.floor(yf / h1) + 0.5) * h1
+        cz = (np.floor(zf / h1) + 0.5) * h1
+        want = 0.3 + 1.7 * cx[:, None, None] - 0.9 * cy[None, :, None] + 2.3 * cz[None, None, :]
+        assert np.allclose(fv, want, rtol=0, atol=1e-12), b
+
+
+def test_ftoc_restriction_exact_for_linear_fields(oracle_mod):
+    """Volume-weighted means of linear cell and face fields over 2x2x2 /
+    2x2 fine groups equal the field at the coarse cell / face centre."""
+    from cases import centres, two_level
+    g = two_level()
+    o = oracle_mod.Oracle(g)
+    X, Y, Z = centres(g)
+    lin = lambda x, y, z: 0.3 + 1.7 * x - 0.9 * y + 2.3 * z
+    f = lin(X, Y, Z)
+    o.ftoc(2, f)
+    ref = lin(X, Y, Z)
+    assert np.allclose(f, ref, rtol=0, atol=1e-12)
+    # staggered: u at x-faces (X + dx/2), v at y-faces, w at z-faces
+    hx = np.repeat([g.spacing[b, 0] for b in range(g.ngrid)],
+                   [int(g.ii[b] * g.jj[b] * g.kk[b]) for b in range(g.ngrid)])
+    u = lin(X + 0.5 * hx, Y, Z)
+    v = lin(X, Y + 0.5 * hx, Z)
+    w = lin(X, Y, Z + 0.5 * hx)
+    u0, v0, w0 = u.copy(), v.copy(), w.copy()
+    p = f.copy()
+    o.ftoc_uvwp(2, u, v, w, p)
+    for a, b in ((u, u0), (v, v0), (w, w0)):
+        assert np.allclose(a, b, rtol=0, atol=1e-12)
+    # the parent cells under the children are overwritten: a field that is
+    # 1 on the fine level and 0 elsewhere fills the refined parent with 1
+    z = g.zeros()
+    z[g.ip3[8]:] = 1.0
+    o.ftoc(2, z)
+    assert np.all(g.block_view(z, 1)[2:-2, 2:-2, 2:-2] == 1.0)
+    assert not g.block_view(z, 2).any()
+
+
+def test_parent_and_bound_pressure_exact_for_linear_pressure(oracle_mod):
+    """With a linear pressure on both levels, parent's face buffers and
+    bound_pressure reproduce the linear field at the PAR-face ghost centres
+    (A form, and the B form with BP = 1)."""
+    from cases import centres, two_level
+    g = two_level()
+    o = oracle_mod.Oracle(g)
+    X, Y, Z = centres(g)
+    lin = 0.3 + 1.7 * X - 0.9 * Y + 2.3 * Z
+    for bp in (None, g.ones()):
+        p = np.where(g.interior_mask(), lin, 0.0)
+        o.exchange([p], layers=2, corners=True)
+        pb = o.par_buffer()
+        o.parent(2, p, pb)
+        o.bound_pressure(2, p, pb, bp=bp)
+        nchecked = 0
+        for b in range(9, 17):
+            for f in range(6):
+                if g.bc[b - 1, f] != 8:
+                    continue
+                pv, lv = g.block_view(p, b), g.block_view(lin, b)
+                sl = [slice(2, -2)] * 3
+                sl[f // 2] = 1 if f % 2 == 0 else -2
+                assert np.allclose(pv[tuple(sl)], lv[tuple(sl)], rtol=0, atol=1e-12), (b, f)
+                nchecked += 1
+        assert nchecked == 12               # 4 children per refined-region face
+
+
+def test_bound_pressure_blocked_group_keeps_interior(oracle_mod):
+    """B form: a 2x2x2 group with BP = 0 takes pc from the buffer, so the
+    ghost equals the interior value (bound_pressure_mod.F90:320-330)."""
+    from cases import two_level
+    g = two_level()
+    o = oracle_mod.Oracle(g)
+    rng = np.random.default_rng(61)
+    p = np.where(g.interior_mask(), rng.standard_normal(g.n3d), 0.0)
+    o.exchange([p], layers=2, corners=True)
+    bp = g.ones()
+    b = 9 + 1                                 # child at (1, 0, 0): PAR on its +x face
+    assert g.bc[b - 1, 1] == 8
+    v = g.block_view(bp, b)
+    v[-4:-2, 2:4, 2:4] = 0.0                  # group at the +x face, j, k = 3..4
+    pb = o.par_buffer()
+    o.parent(2, p, pb)
+    o.bound_pressure(2, p, pb, bp=bp)
+    pv = g.block_view(p, b)
+    assert np.array_equal(pv[-2, 2:4, 2:4], pv[-3, 2:4, 2:4])
+
+
+def test_multilevel_full_refinement_matches_single_level(oracle_mod):
+    """A fine level that covers the whole periodic domain has no PAR faces;
+    converged, its pressure and velocity equal those of a single-level
+    solve on the fine blocks alone (the coarse level only accelerates)."""
+    from mgp.grids import refined_box, uniform_box
+    from cases import velocity
+    gm = refined_box((2, 2, 2), 8, [(8, lambda B: True)])
+    gs = uniform_box((4, 4, 4), (8, 8, 8))
+    om, os_ = oracle_mod.Oracle(gm), oracle_mod.Oracle(gs)
+    u, v, w = velocity(gs, os_, 71)
+    # copy the single-level fields into the fine level of the hierarchy
+    off = gm.ip3[8]
+    fine = [gm.zeros() for _ in range(3)]
+    order = []
+    for b in range(9, gm.ngrid + 1):
+        o_ = gm.blocks[b - 1]["origin"] // 8
+        order.append(1 + o_[0] + 4 * (o_[1] + 4 * o_[2]))
+    for arr_m, arr_s in zip(fine, (u, v, w)):
+        for bm, bs in zip(range(9, gm.ngrid + 1), order):
+            gm.block_view(arr_m, bm)[...] = gs.block_view(arr_s, bs)
+    stm, sts = om.setup(), os_.setup()
+    pm, ps = gm.zeros(), gs.zeros()
+    rm = om.mgpoisl(stm, *fine, pm, 1e-3, nouter=3000, eps_rel=1e-12)
+    rs = os_.mgpoisl(sts, u, v, w, ps, 1e-3, nouter=3000, eps_rel=1e-12)
+    assert rm["maxrhs"] <= 1e-12 * rm["maxrhs0"] and rs["maxrhs"] <= 1e-12 * rs["maxrhs0"]
+    assert rm["nouter_done"] < rs["nouter_done"]          # the coarse level helps
+    inner = gs.interior_mask()
+    for arr_m, arr_s in zip(fine + [pm], (u, v, w, ps)):
+        got = np.concatenate([gm.block_view(arr_m, bm)[2:-2, 2:-2, 2:-2].ravel()
+                              for bm in range(9, gm.ngrid + 1)])
+        ref = np.concatenate([gs.block_view(arr_s, bs)[2:-2, 2:-2, 2:-2].ravel()
+                              for bs in order])
+        if arr_s is ps:
+            got, ref = got - got.mean(), ref - ref.mean()
+        assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
