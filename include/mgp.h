@@ -55,6 +55,10 @@ typedef struct mgp_grids {
     const mgletint *itypbc;         /* [ngrid*6] first BC code per face     */
     const int64_t *ip3;             /* [ngrid] 1-based 3-D offsets          */
     const int64_t *ip1x, *ip1y, *ip1z; /* [ngrid] 1-based 1-D offsets       */
+    const mgletint *iparent;        /* [ngrid] parent grid id, 0 = none, or */
+                                    /*   NULL for a single level            */
+    const mgletint *position;       /* [ngrid*3] iposition, jposition,      */
+                                    /*   kposition in the parent (GRIDINFO) */
 } mgp_grids;
 
 /* /flow/pressuresolver keys (src/flow/pressuresolver_mod.F90:57-142) plus
@@ -155,7 +159,9 @@ int mgp_set_stream(mgp_handle *h, void *stream);
  * "GSAP", "BP", "SIPLW", "SIPLS", "SIPLB", "SIPLPR", "SIPUE", "SIPUN",
  * "SIPUT" (the factors in hyperplane access order, as
  * src/flow/sip_hyperplane_mod.F90:224-230 stores them, or spatial order
- * for ityp 2), "DP", "RHS" (state after the last mgpoisl).  1-D fields
+ * for ityp 2), "DP", "RHS" (state after the last mgpoisl), "DPG" (DP with
+ * the ghost layer mgpcorr reads: exchanged, bound_pressure at PAR faces,
+ * accumulated injected values elsewhere).  1-D fields
  * "GSAW", "GSAE", "GSAS", "GSAN", "GSAB", "GSAT" use the 1-D layout. */
 int mgp_get_field(mgp_handle *h, const char *name, mgletreal *out);
 

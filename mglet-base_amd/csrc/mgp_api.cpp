@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <set>
 #include <memory>
 #include <string>
 #include <tuple>
@@ -98,6 +99,7 @@ const int RESCUE_NBR[26][7] = {
     {20, 19, 22, 24, 21, 23, 26}, {19, 20, 21, 23, 22, 24, 25}};
 
 bool is_con(int c) { return c == 7 || c == 19; }
+constexpr int PAR_BC = 8;
 bool is_wall_lo(int c) { return c == 2 || c == 5 || c == 6 || c == 19; }
 bool is_wall_hi(int c) { return c == 2 || c == 5 || c == 6; }
 
@@ -118,7 +120,8 @@ struct mgp_handle {
     int ngrid = 0, minlevel = 0, maxlevel = 0;
     std::vector<int> kk, jj, ii, level, nbr, bc;
     std::vector<int64_t> ip3, ip1x, ip1y, ip1z;  // 0-based
-    int64_t n3d = 0, n1x = 0, n1y = 0, n1z = 0, ncell = 0, nface = 0;
+    std::vector<int> iparent, position;         // 1-based parent ids, positions
+    int64_t n3d = 0, n1x = 0, n1y = 0, n1z = 0, ncell = 0, nface = 0, nshell = 0;
     std::vector<Shape> shapes;
     std::vector<Block> blocks;
     std::vector<Conn> conns;
@@ -133,6 +136,7 @@ struct mgp_handle {
     std::map<int, DBuf> d_blist_lvl;
     // 1-D fields (FP64 and working precision)
     DBuf d_dx, d_dy, d_dz, d_rdx, d_rdy, d_rdz, d_rddx, d_rddy, d_rddz;
+    DBuf d_ddx, d_ddy, d_ddz;
     DBuf d_raw[6], d_c64[6], d_cT[6];
     // per-cell WO arrays
     DBuf d_ap64, d_apT, d_f[7], d_hilf, d_rhs64, d_rhsT, d_res, d_dp64;
@@ -140,6 +144,14 @@ struct mgp_handle {
     DBuf d_hface, d_rhi[2], d_rlo;
     // SOR: SOR_RAP and the second HILF face buffer (frozen ghosts)
     DBuf d_rapT, d_hface2;
+    // levels: PAR ghosts of HILF, parent values per PAR face group (HILF in
+    // the working precision, DP in FP64)
+    DBuf d_hpar, d_pbuf, d_pbuf64;
+    // first ghost layer ("shell") of refined grids: HILF as ctof injects it
+    // (read by the next level's ctof), and its sum over outer iterations,
+    // which DP keeps in the ghost cells no exchange or bound_pressure sets
+    DBuf d_hsh, d_dsh;
+    std::set<int> level_par;   // levels with PAR faces
     // adjoint: transposed factors and 1-D coefficients (built on first use)
     DBuf d_ft[6], d_c64t[6], d_cTt[6];
     std::vector<double> fold_host[6];
@@ -234,6 +246,7 @@ struct mgp_handle {
             *c[q] = fp32() ? d_cT[q].as<T>() : d_c64[q].as<T>();
         F.rap = d_rapT.as<T>();
         F.hface = d_hface.as<T>();
+        F.hpar = d_hpar.as<T>();
         F.rhi[0] = d_rhi[0].as<T>();
         F.rhi[1] = d_rhi[1].as<T>();
         F.rlo = d_rlo.as<T>();
@@ -450,7 +463,7 @@ void build_shapes(mgp_handle &h, std::vector<uint32_t> &cells,
                   std::vector<int> &pstart, std::vector<int> &slot) {
     std::map<std::tuple<int, int, int>, int> idx;
     h.blocks.resize(h.ngrid);
-    int64_t wo = 0, fc = 0;
+    int64_t wo = 0, fc = 0, sh = 0;
     for (int g = 0; g < h.ngrid; ++g) {
         const int ni = h.ii[g] - 4, nj = h.jj[g] - 4, nk = h.kk[g] - 4;
         auto key = std::make_tuple(ni, nj, nk);
@@ -504,23 +517,30 @@ void build_shapes(mgp_handle &h, std::vector<uint32_t> &cells,
         B.ni = ni; B.nj = nj; B.nk = nk;
         B.level = h.level[g];
         B.flags = 0;
+        B.par = 0;
+        B.parent = h.iparent.empty() ? -1 : h.iparent[g] - 1;
+        for (int a = 0; a < 3; ++a) B.pos[a] = h.position.empty() ? 0 : h.position[3 * g + a];
         for (int f = 0; f < 6; ++f) {
             B.bc[f] = h.bc[6 * g + f];
             const int ng = h.nbr[26 * g + f];
             B.nbr[f] = (is_con(B.bc[f]) && ng > 0) ? ng - 1 : -1;
+            if (B.bc[f] == PAR_BC) B.par |= 1 << f;
         }
         B.wo = wo;
         B.sp = h.ip3[g];
         B.fc = fc;
+        B.sh = sh;
         B.o1x = h.ip1x[g]; B.o1y = h.ip1y[g]; B.o1z = h.ip1z[g];
         wo += (int64_t)ni * nj * nk;
         fc += h.shapes[sid].fsize;
+        sh += shell_size(ni, nj, nk);
         h.max_ni = std::max(h.max_ni, ni);
         h.max_nj = std::max(h.max_nj, nj);
         h.max_nk = std::max(h.max_nk, nk);
     }
     h.ncell = wo;
     h.nface = fc;
+    h.nshell = sh;
 }
 
 template <typename T>
@@ -585,6 +605,11 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
     h->level.assign(G->level, G->level + n);
     h->nbr.assign(G->nbrgrid, G->nbrgrid + 26 * n);
     h->bc.assign(G->itypbc, G->itypbc + 6 * n);
+    if (G->iparent) {
+        if (!G->position) return fail(MGP_ERR_ARG, "iparent given without position");
+        h->iparent.assign(G->iparent, G->iparent + n);
+        h->position.assign(G->position, G->position + 3 * n);
+    }
     for (int g = 0; g < n; ++g) {
         h->ip3.push_back(G->ip3[g] - 1);
         h->ip1x.push_back(G->ip1x[g] - 1);
@@ -597,16 +622,29 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
                         "block sizes (4*ni + 2*nj <= 256, ni*nj <= 1024)", g + 1, ni, nj, nk);
         if (h->level[g] < h->minlevel || h->level[g] > h->maxlevel)
             return fail(MGP_ERR_TOPOLOGY, "grid %d: level out of range", g + 1);
-        for (int f = 0; f < 6; ++f)
-            if (h->bc[6 * g + f] == 8)
-                return fail(MGP_ERR_TOPOLOGY,
-                            "grid %d face %d: PAR (parent-coupled) faces are "
-                            "not supported by this build", g + 1, f + 1);
+        if (g > 0 && h->level[g] < h->level[g - 1])
+            return fail(MGP_ERR_TOPOLOGY, "grids must be numbered level by level");
+        // nesting (src/ib/ctof/ctof1_mod.F90:240-255, parent_core_mod.F90:
+        // 214-262): a grid above the coarsest level has a parent one level
+        // coarser, even cell counts, and covers parent cells only
+        const int par = h->iparent.empty() ? 0 : h->iparent[g];
+        bool has_par_face = false;
+        for (int f = 0; f < 6; ++f) has_par_face = has_par_face || h->bc[6 * g + f] == PAR_BC;
+        if (h->level[g] > h->minlevel || par != 0 || has_par_face) {
+            if (par < 1 || par > n || h->level[par - 1] != h->level[g] - 1)
+                return fail(MGP_ERR_TOPOLOGY, "grid %d: level %d needs a parent on level %d",
+                            g + 1, h->level[g], h->level[g] - 1);
+            if ((ni | nj | nk) & 1)
+                return fail(MGP_ERR_TOPOLOGY, "grid %d: refined grids need even cell counts", g + 1);
+            const int pn[3] = {h->ii[par - 1] - 4, h->jj[par - 1] - 4, h->kk[par - 1] - 4};
+            const int cn[3] = {ni, nj, nk};
+            for (int a = 0; a < 3; ++a) {
+                const int p0 = h->position[3 * g + a];
+                if (p0 < 3 || p0 - 3 + cn[a] / 2 > pn[a])
+                    return fail(MGP_ERR_TOPOLOGY, "grid %d: position outside its parent", g + 1);
+            }
+        }
         h->level_blocks[h->level[g]].push_back(g);
-        if (h->level[g] != h->level[0])
-            return fail(MGP_ERR_TOPOLOGY,
-                        "grids on more than one level (ctof/ftoc/parent "
-                        "coupling) are not supported by this build");
         h->n3d = std::max<int64_t>(h->n3d, h->ip3[g] + (int64_t)h->ii[g] * h->jj[g] * h->kk[g]);
         h->n1x = std::max<int64_t>(h->n1x, h->ip1x[g] + h->ii[g]);
         h->n1y = std::max<int64_t>(h->n1y, h->ip1y[g] + h->jj[g]);
@@ -714,6 +752,9 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
             return r;
         };
         upload_raw(h->d_dx, dx, h->n1x * 8);
+        upload_raw(h->d_ddx, ddx, h->n1x * 8);
+        upload_raw(h->d_ddy, ddy, h->n1y * 8);
+        upload_raw(h->d_ddz, ddz, h->n1z * 8);
         upload_raw(h->d_dy, dy, h->n1y * 8);
         upload_raw(h->d_dz, dz, h->n1z * 8);
         upload(h->d_rdx, recip(dx, h->n1x));
@@ -777,6 +818,17 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
             h->d_rapT.alloc(h->ncell * tsz);
             h->d_hface2.alloc(h->nface * tsz);
         }
+        for (const auto &B : h->blocks)
+            if (B.par) h->level_par.insert(B.level);
+        if (!h->level_par.empty()) {
+            h->d_hpar.alloc(h->nface * tsz);
+            h->d_pbuf.alloc(h->nface * tsz);
+            h->d_pbuf64.alloc(h->nface * 8);
+        }
+        if (h->level_blocks.size() > 1) {
+            h->d_hsh.alloc(h->nshell * tsz);
+            h->d_dsh.alloc(h->nshell * 8);
+        }
         h->d_scal.alloc(64);
         h->d_never.alloc(64);
         HC(hipMemsetAsync(h->d_never.p, 0, 64, h->stream));
@@ -822,6 +874,11 @@ void reset_state(mgp_handle &h) {
     launch_fill<T>(h.d_rhi[1].as<T>(), h.nface, T(0), s);
     launch_fill<T>(h.d_rlo.as<T>(), h.nface, T(0), s);
     if (h.d_hface2.p) launch_fill<T>(h.d_hface2.as<T>(), h.nface, T(0), s);
+    if (h.d_hpar.p) launch_fill<T>(h.d_hpar.as<T>(), h.nface, T(0), s);
+    if (h.d_hsh.p) {
+        launch_fill<T>(h.d_hsh.as<T>(), h.nshell, T(0), s);
+        launch_fill<double>(h.d_dsh.as<double>(), h.nshell, 0.0, s);
+    }
 }
 
 // The outer defect-correction loop (src/flow/pressuresolver_mod.F90:
@@ -831,6 +888,7 @@ void reset_state(mgp_handle &h) {
 template <typename T>
 float iterate(mgp_handle &h, const Fields<T> &F, const Outer &o,
               double prefak) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "T");
     const mgp_config &c = h.cfg;
     const hipStream_t s = h.stream;
     Topo tall = h.topo(-1);
@@ -845,32 +903,74 @@ float iterate(mgp_handle &h, const Fields<T> &F, const Outer &o,
         if (((c.ninner - 1) & 1) && (chunk & 1)) ++chunk;
         int launched = 0;
         const bool classic = c.ityp == 2;
+        const bool multi = h.level_blocks.size() > 1;
+        const int coarsest = h.level_blocks.begin()->first;
+        Outer o_inner = o;
+        if (multi) o_inner.gmax = (unsigned long long *)(h.scal() + 5);  // unused slot
+        const double *mt[6] = {h.d_dx.as<double>(), h.d_dy.as<double>(), h.d_dz.as<double>(),
+                               h.d_ddx.as<double>(), h.d_ddy.as<double>(), h.d_ddz.as<double>()};
+        const double *dd[3] = {mt[3], mt[4], mt[5]};
+        T *pbuf = h.d_pbuf.as<T>();
+        const int bpv = c.ibtype == 1;
         auto enqueue_chunk = [&]() {
-            int rd = 0;
+            // RES buffer parity per level (each level exchanges its own RES)
+            std::map<int, int> rd;
             Mask m = h.mask();
             for (int it = 0; it < chunk; ++it) {
+                // coarse to fine: ctof, parent, mgpoisit
+                // (src/flow/pressuresolver_mod.F90:224-231, 361-420)
                 for (auto &kv : h.level_blocks) {
-                    Topo t = h.topo(kv.first);
-                    // mgpoisit (src/flow/pressuresolver_mod.F90:403-414):
-                    // SOR on levels above the coarsest one
-                    if (c.ityp == 1 && (kv.first > h.minlevel || c.sor_all_levels)) {
-                        // the last relaxation writes the faces into
-                        // F.hface, where the SIP levels leave theirs too
-                        T *hf[2] = {F.hface, h.d_hface2.as<T>()};
-                        for (int il = 1; il <= c.ninner; ++il)
+                    const int L = kv.first;
+                    Topo t = h.topo(L);
+                    const bool first = L == coarsest;
+                    const bool par = h.level_par.count(L) > 0;
+                    const bool sor = c.ityp == 1 && (L > h.minlevel || c.sor_all_levels);
+                    T *hf[2] = {F.hface, h.d_hface2.as<T>()};
+                    if (!first) {
+                        // the first relaxation or sweep reads its ghosts from
+                        // the buffer ctof fills
+                        launch_ctof<T>(t, F, sor ? hf[c.ninner & 1] : F.hface,
+                                       h.d_hsh.as<T>(), h.d_dsh.as<double>(), o.done, s);
+                        if (par) launch_parent<T>(t, F, pbuf, o.done, s);
+                    }
+                    if (sor) {
+                        // the last relaxation writes the faces into F.hface,
+                        // where the SIP levels leave theirs too
+                        for (int il = 1; il <= c.ninner; ++il) {
+                            if (par) launch_bound<T>(t, F, pbuf, m, bpv, mt, o.done, s);
                             launch_sor<T>(t, F, m, hf[(c.ninner - il + 1) & 1],
-                                          hf[(c.ninner - il) & 1], c.omega, il == 1,
-                                          o.done, s);
-                        continue;
+                                          hf[(c.ninner - il) & 1], c.omega,
+                                          il == 1 && first, o.done, s);
+                        }
+                    } else {
+                        int &r = rd[L];
+                        for (int il = 1; il <= c.ninner; ++il) {
+                            const int last = il == c.ninner;
+                            if (par) launch_bound<T>(t, F, pbuf, m, bpv, mt, o.done, s);
+                            launch_forward<T>(t, F, m, r, !last, il == 1 && first, classic,
+                                              o.done, s);
+                            launch_backward<T>(t, F, il == 1 && first, classic, o.done, s);
+                            if (!last) r ^= 1;
+                        }
                     }
-                    for (int il = 1; il <= c.ninner; ++il) {
-                        const int last = il == c.ninner;
-                        launch_forward<T>(t, F, m, rd, !last, il == 1, classic, o.done, s);
-                        launch_backward<T>(t, F, il == 1, classic, o.done, s);
-                        if (!last) rd ^= 1;
-                    }
+                    if (par) launch_bound<T>(t, F, pbuf, m, bpv, mt, o.done, s);
                 }
-                launch_outer<T>(tall, F, m, o, rhsT, s);
+                // fine to coarse: HILF into the parents, the residual, the
+                // RHS into the parents, max |RHS| (:233-298)
+                for (auto lv = h.level_blocks.rbegin(); lv != h.level_blocks.rend(); ++lv)
+                    if (lv->first != coarsest)
+                        launch_restrict<T, T>(h.topo(lv->first), F.hilf, F.hilf, F.hface,
+                                              nullptr, m, bpv ? 1 : 0, dd, o.done, s);
+                launch_outer<T>(tall, F, m, o_inner, rhsT, s);
+                if (multi) {
+                    for (auto lv = h.level_blocks.rbegin(); lv != h.level_blocks.rend(); ++lv)
+                        if (lv->first != coarsest)
+                            launch_restrict<double, double>(h.topo(lv->first), o.rhs64, o.rhs64,
+                                                            nullptr, h.fp32() ? h.d_rhsT.as<float>()
+                                                                              : nullptr,
+                                                            m, bpv ? 2 : 0, dd, o.done, s);
+                    launch_maxabs_acc(tall, o.rhs64, o, s);
+                }
                 launch_check(o, c.nouter, c.nouter_min, c.epcorr, c.eps_rel,
                              prefak, h.d_hist.as<double>(), c.nouter, s);
             }
@@ -932,6 +1032,7 @@ void dp_to_reference(mgp_handle &h) {
     if (h.d_dpsp.n < (size_t)h.n3d * 8) h.d_dpsp.alloc((size_t)h.n3d * 8);
     launch_fill<double>(h.d_dpsp.as<double>(), h.n3d, 0.0, s);
     launch_wo_to_sp<double>(tall, h.d_dp64.as<double>(), h.d_dpsp.as<double>(), s);
+    if (h.d_dsh.p) launch_shell_to_sp(tall, h.d_dsh.as<double>(), h.d_dpsp.as<double>(), s);
     double *fd[1] = {h.d_dpsp.as<double>()};
     exchange(h, -1, 1, 0, 0, fd, 1);
 }
@@ -968,16 +1069,36 @@ int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
     launch_divcal<T>(tall, u, v, w, h.d_rddx.as<double>(), h.d_rddy.as<double>(),
                      h.d_rddz.as<double>(), c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr,
                      prefak, h.d_rhs64.as<double>(), rhsT, s);
+    // RHS of the refined grids into their parents (:197-199)
+    const double *dd[3] = {h.d_ddx.as<double>(), h.d_ddy.as<double>(), h.d_ddz.as<double>()};
+    for (auto lv = h.level_blocks.rbegin(); lv != h.level_blocks.rend(); ++lv)
+        if (lv->first != h.level_blocks.begin()->first)
+            launch_restrict<double, double>(h.topo(lv->first), h.d_rhs64.as<double>(),
+                                            h.d_rhs64.as<double>(), nullptr,
+                                            h.fp32() ? h.d_rhsT.as<float>() : nullptr, h.mask(),
+                                            c.ibtype == 1 ? 2 : 0, dd, h.d_never.as<int>(), s);
     launch_maxabs(tall, h.d_rhs64.as<double>(), h.outer(), s);
     HC(hipGetLastError());
     const float ms = iterate<T>(h, h.fields<T>(), h.outer(), prefak);
 
-    // DP -> reference layout, mgpcorr and the post-projection exchange
-    // (src/flow/pressuresolver_mod.F90:323-348)
+    // DP -> reference layout; parent and bound_pressure of DP per level,
+    // mgpcorr, U, V, W, P into the parents and the post-projection
+    // exchange (src/flow/pressuresolver_mod.F90:323-348)
     dp_to_reference(h);
+    const double *mt[6] = {h.d_dx.as<double>(), h.d_dy.as<double>(), h.d_dz.as<double>(),
+                           h.d_ddx.as<double>(), h.d_ddy.as<double>(), h.d_ddz.as<double>()};
+    for (int L : h.level_par) {
+        launch_parent_sp(h.topo(L), h.d_dpsp.as<double>(), h.d_pbuf64.as<double>(), s);
+        launch_bound_sp(h.topo(L), h.d_dpsp.as<double>(), h.d_pbuf64.as<double>(),
+                        c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr, c.ibtype == 1, mt, s);
+    }
     launch_mgpcorr(tall, u, v, w, p, h.d_dpsp.as<double>(), h.d_rdx.as<double>(),
                    h.d_rdy.as<double>(), h.d_rdz.as<double>(), dt / c.rho,
                    c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr, s);
+    for (auto lv = h.level_blocks.rbegin(); lv != h.level_blocks.rend(); ++lv)
+        if (lv->first != h.level_blocks.begin()->first)
+            launch_ftoc_uvwp(h.topo(lv->first), u, v, w, p, c.ibtype == 1,
+                             c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr, dd, s);
     double *fs[4] = {u, v, w, p};
     for (int l = h.minlevel; l <= h.maxlevel; ++l)
         if (h.level_blocks.count(l)) exchange(h, l, 2, 1, 0, fs, 4);
@@ -1253,6 +1374,10 @@ int mgp_get_field(mgp_handle *h, const char *name, mgletreal *out) {
             }
         if (nm == "GSAP") {
             launch_wo_to_sp<double>(t, h->d_ap64.as<double>(), tmp.as<double>(), s);
+            found = true;
+        } else if (nm == "DPG") {
+            if (!h->have_solution) return fail(MGP_ERR_STATE, "no mgpoisl call yet");
+            HC(hipMemcpyAsync(tmp.p, h->d_dpsp.p, (size_t)h->n3d * 8, hipMemcpyDeviceToDevice, s));
             found = true;
         } else if (nm == "DP" || nm == "RHS") {
             if (!h->have_solution) return fail(MGP_ERR_STATE, "no mgpoisl call yet");

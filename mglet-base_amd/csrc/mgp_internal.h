@@ -37,10 +37,14 @@ struct Block {
     int level;
     int flags;         // bit 0: BP == 1 on the block and its ghost layer
     int nbr[6];        // 0-based block index feeding each face ghost, -1 none
+    int par;           // bit f: face f is PAR (ghost from bound_pressure)
+    int parent;        // 0-based parent block, -1 none
+    int pos[3];        // iposition, jposition, kposition in the parent
     int bc[6];         // first BC code per face
     int64_t wo;        // offset of the block's interior cells in WO arrays
     int64_t sp;        // offset of the block in reference-layout 3-D fields
     int64_t fc;        // offset of the block's face buffers
+    int64_t sh;        // offset of the block's ghost-shell buffers
     int64_t o1x, o1y, o1z; // offsets of the block in 1-D fields
 };
 
@@ -77,6 +81,22 @@ __host__ __device__ inline int face_base(int f, int ni, int nj, int nk) {
     case 4: return 2 * ax + 2 * ay;
     default: return 2 * ax + 2 * ay + az;
     }
+}
+
+// The first ghost layer of a block ("shell": cells (i, j, k) in [-1, n]^3
+// with at least one index outside [0, n)), enumerated slab by slab in i.
+__host__ __device__ inline int shell_size(int ni, int nj, int nk) {
+    return 2 * (nj + 2) * (nk + 2) + ni * (2 * (nk + 2) + 2 * nj);
+}
+__host__ __device__ inline int shell_index(int i, int j, int k, int ni, int nj,
+                                           int nk) {
+    const int a = (nj + 2) * (nk + 2);
+    if (i < 0) return (j + 1) * (nk + 2) + (k + 1);
+    if (i >= ni) return a + (j + 1) * (nk + 2) + (k + 1);
+    const int base = 2 * a + i * (2 * (nk + 2) + 2 * nj);
+    if (j < 0) return base + (k + 1);
+    if (j >= nj) return base + (nk + 2) + (k + 1);
+    return base + 2 * (nk + 2) + 2 * j + (k < 0 ? 0 : 1);
 }
 
 // One box copy of the ghost-layer exchange (reference layout).

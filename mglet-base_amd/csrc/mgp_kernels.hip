@@ -37,12 +37,15 @@ __device__ __forceinline__ void unpack(uint32_t c, int &i, int &j, int &k) {
     k = c >> 20;
 }
 
+// Face f of block nb (the neighbour's own dimensions place its faces;
+// only the tangential dimensions match across a CON face).
 template <typename T>
 __device__ __forceinline__ const T *nbr_face(const T *base,
                                              const Block *blocks, int nb,
-                                             int f, int ni, int nj, int nk) {
-    return nb >= 0 ? base + blocks[nb].fc + face_base(f, ni, nj, nk)
-                   : nullptr;
+                                             int f) {
+    if (nb < 0) return nullptr;
+    const Block &N = blocks[nb];
+    return base + N.fc + face_base(f, N.ni, N.nj, N.nk);
 }
 
 // ---------------------------------------------------------------------
@@ -384,6 +387,15 @@ void launch_maxabs(const Topo &t, const double *rhs64, Outer o,
 template void launch_fill<unsigned long long>(unsigned long long *, int64_t,
                                               unsigned long long, hipStream_t);
 
+// max |RHS| of the outer iteration on a multi-level grid, taken after the
+// restriction of the RHS to the parents (src/flow/pressuresolver_mod.F90:
+// 288-298)
+void launch_maxabs_acc(const Topo &t, const double *rhs64, Outer o,
+                       hipStream_t s) {
+    launch_fill<unsigned long long>(o.gmax, 1, 0ull, s);
+    k_maxabs<<<t.nlist, kThreads, 0, s>>>(t, rhs64, o);
+}
+
 // ---------------------------------------------------------------------
 // Plane-window helpers.  H holds 4 hyperplanes of a field, each stored per
 // pencil [j][i] with a one-cell border: index (j+1) + (NJ+2)(i+1).  Plane q
@@ -394,18 +406,26 @@ struct Window {
     const T *zlo, *zhi, *ylo, *yhi, *xlo, *xhi;  // neighbour faces or null
 };
 
+// Ghost sources of HILF per face: the neighbour's boundary plane (CON),
+// the block's own PAR ghosts from bound_pressure, or none (zero).
 template <typename T>
 __device__ __forceinline__ Window<T> hilf_window(const T *hface,
+                                                 const T *hpar,
                                                  const Block *blocks,
                                                  const Block &B) {
     Window<T> w;
-    const int ni = B.ni, nj = B.nj, nk = B.nk;
-    w.xlo = nbr_face(hface, blocks, B.nbr[0], 1, ni, nj, nk);
-    w.xhi = nbr_face(hface, blocks, B.nbr[1], 0, ni, nj, nk);
-    w.ylo = nbr_face(hface, blocks, B.nbr[2], 3, ni, nj, nk);
-    w.yhi = nbr_face(hface, blocks, B.nbr[3], 2, ni, nj, nk);
-    w.zlo = nbr_face(hface, blocks, B.nbr[4], 5, ni, nj, nk);
-    w.zhi = nbr_face(hface, blocks, B.nbr[5], 4, ni, nj, nk);
+    const T *g[6];
+    for (int f = 0; f < 6; ++f) {
+        g[f] = nbr_face(hface, blocks, B.nbr[f], f ^ 1);
+        if (!g[f] && ((B.par >> f) & 1))
+            g[f] = hpar + B.fc + face_base(f, B.ni, B.nj, B.nk);
+    }
+    w.xlo = g[0];
+    w.xhi = g[1];
+    w.ylo = g[2];
+    w.yhi = g[3];
+    w.zlo = g[4];
+    w.zhi = g[5];
     return w;
 }
 
@@ -607,10 +627,10 @@ __global__ __launch_bounds__(kThreads) void k_forward(
     const int64_t wo = B.wo;
     const T *rh = F.rhi[rd];
     T *rw = write_hi ? F.rhi[rd ^ 1] : nullptr;
-    const T *rzlo = nbr_face(rh, t.blocks, B.nbr[4], 5, NI, NJ, NK);
-    const T *rylo = nbr_face(rh, t.blocks, B.nbr[2], 3, NI, NJ, NK);
-    const T *rxlo = nbr_face(rh, t.blocks, B.nbr[0], 1, NI, NJ, NK);
-    const Window<T> hw = hilf_window(F.hface, t.blocks, B);
+    const T *rzlo = nbr_face(rh, t.blocks, B.nbr[4], 5);
+    const T *rylo = nbr_face(rh, t.blocks, B.nbr[2], 3);
+    const T *rxlo = nbr_face(rh, t.blocks, B.nbr[0], 1);
+    const Window<T> hw = hilf_window(F.hface, F.hpar, t.blocks, B);
     coef_to_lds(cf, F.aw, F.ae, F.as, F.an, F.ab, F.at, B);
     const T *aw = cf, *ae = cf + kMaxN1, *as = cf + 2 * kMaxN1,
             *an = cf + 3 * kMaxN1, *ab = cf + 4 * kMaxN1, *at = cf + 5 * kMaxN1;
@@ -835,9 +855,9 @@ __global__ __launch_bounds__(kThreads) void k_backward(Topo t, Fields<T> F,
     for (int e = threadIdx.x; e <= NP; e += blockDim.x)
         sps[e] = t.pstart[S.pstart_off + e];
     const int64_t wo = B.wo;
-    const T *rzhi = nbr_face((const T *)F.rlo, t.blocks, B.nbr[5], 4, NI, NJ, NK);
-    const T *ryhi = nbr_face((const T *)F.rlo, t.blocks, B.nbr[3], 2, NI, NJ, NK);
-    const T *rxhi = nbr_face((const T *)F.rlo, t.blocks, B.nbr[1], 0, NI, NJ, NK);
+    const T *rzhi = nbr_face((const T *)F.rlo, t.blocks, B.nbr[5], 4);
+    const T *ryhi = nbr_face((const T *)F.rlo, t.blocks, B.nbr[3], 2);
+    const T *rxhi = nbr_face((const T *)F.rlo, t.blocks, B.nbr[1], 0);
     const T *ue = F.f[4] + wo, *un = F.f[5] + wo, *ut = F.f[6] + wo;
     const T *res = F.res + wo;
     T *hilf = F.hilf + wo;
@@ -1035,7 +1055,7 @@ __global__ __launch_bounds__(kThreads) void k_sor(Topo t, Fields<T> F, Mask m,
     for (int e = threadIdx.x; e <= NP; e += blockDim.x)
         sps[e] = t.pstart[S.pstart_off + e];
     const int64_t wo = B.wo;
-    const Window<T> hw = hilf_window(fin, t.blocks, B);
+    const Window<T> hw = hilf_window(fin, F.hpar, t.blocks, B);
     coef_to_lds(cf, F.aw, F.ae, F.as, F.an, F.ab, F.at, B);
     const T *aw = cf, *ae = cf + kMaxN1, *as = cf + 2 * kMaxN1,
             *an = cf + 3 * kMaxN1, *ab = cf + 4 * kMaxN1, *at = cf + 5 * kMaxN1;
@@ -1212,7 +1232,7 @@ __global__ __launch_bounds__(kThreads) void k_outer(Topo t, Fields<T> F,
     for (int e = threadIdx.x; e <= NP; e += blockDim.x)
         sps[e] = t.pstart[S.pstart_off + e];
     const int64_t wo = B.wo;
-    const Window<T> hw = hilf_window(F.hface, t.blocks, B);
+    const Window<T> hw = hilf_window(F.hface, F.hpar, t.blocks, B);
     coef_to_lds(cf, o.aw, o.ae, o.as, o.an, o.ab, o.at, B);
     const double *aw = cf, *ae = cf + kMaxN1, *as = cf + 2 * kMaxN1,
                  *an = cf + 3 * kMaxN1, *ab = cf + 4 * kMaxN1, *at = cf + 5 * kMaxN1;

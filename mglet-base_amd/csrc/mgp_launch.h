@@ -32,6 +32,7 @@ struct Fields {
     const T *f[7];              // LW LS LB LPR UE UN UT
     const T *aw, *ae, *as, *an, *ab, *at;  // 1-D coefficients (ref. layout)
     T *hface;                   // HILF boundary planes (own faces)
+    T *hpar;                    // HILF ghosts of the block's PAR faces
     T *rhi[2];                  // RES upper planes, double-buffered
     T *rlo;                     // RES lower planes
 };
@@ -123,6 +124,37 @@ void launch_copy(const CopyTask *tasks, int ntask, double *const *fields,
 template <typename T>
 void launch_to_ref_factors(const Topo &t, const T *wo, double *ref,
                            int spatial, hipStream_t s);
+
+// coupling of grid levels (mgp_levels.hip)
+// `done`: the outer loop's stop flag (the kernels return when it is set)
+template <typename T>
+void launch_ctof(const Topo &t, const Fields<T> &f, T *fout, T *hsh,
+                 double *dsh, const int *done, hipStream_t s);
+void launch_shell_to_sp(const Topo &t, const double *dsh, double *sp,
+                        hipStream_t s);
+template <typename T>
+void launch_parent(const Topo &t, const Fields<T> &f, T *pbuf, const int *done,
+                   hipStream_t s);
+template <typename T>
+void launch_bound(const Topo &t, const Fields<T> &f, const T *pbuf, Mask m,
+                  int bpv, const double *const metric[6], const int *done,
+                  hipStream_t s);
+// mode 0 plain restriction, 1 / 2 the ghostcell 'P' / 'R' restrictions
+template <typename S, typename D>
+void launch_restrict(const Topo &t, const S *src, D *dst, D *face,
+                     float *dst2, Mask m, int mode, const double *const dd[3],
+                     const int *done, hipStream_t s);
+void launch_parent_sp(const Topo &t, const double *f, double *pbuf,
+                      hipStream_t s);
+void launch_bound_sp(const Topo &t, double *p, const double *pbuf,
+                     const double *bp, int bpv, const double *const metric[6],
+                     hipStream_t s);
+// gc: ghostcell restriction of P (bp may be null: all fluid)
+void launch_ftoc_uvwp(const Topo &t, double *u, double *v, double *w,
+                      double *p, int gc, const double *bp,
+                      const double *const dd[3], hipStream_t s);
+void launch_maxabs_acc(const Topo &t, const double *rhs64, Outer o,
+                       hipStream_t s);  // harmless after the stop: k_check skips
 
 // adjoint projection (mgp_adjoint.hip)
 template <typename T>

@@ -19,6 +19,8 @@ MODULE mgp_bindc_mod
         TYPE(C_PTR) :: itypbc = C_NULL_PTR
         TYPE(C_PTR) :: ip3 = C_NULL_PTR, ip1x = C_NULL_PTR
         TYPE(C_PTR) :: ip1y = C_NULL_PTR, ip1z = C_NULL_PTR
+        ! GRIDINFO iparent and i/j/kposition; C_NULL_PTR for one level
+        TYPE(C_PTR) :: iparent = C_NULL_PTR, position = C_NULL_PTR
     END TYPE mgp_grids_t
 
     ! struct mgp_config (/flow/pressuresolver keys,

@@ -39,14 +39,19 @@ class PressureSolver:
         if t not in _TYPES:
             raise MgpError(f"Invalid pressure solver type: {type}")
         self._keep = [g.kk, g.jj, g.ii, g.level, g.nbr, g.bc,
-                      g.ip3 + 1, g.ip1x + 1, g.ip1y + 1, g.ip1z + 1]
+                      g.ip3 + 1, g.ip1x + 1, g.ip1y + 1, g.ip1z + 1,
+                      np.ascontiguousarray(g.iparent, dtype=np.int32),
+                      np.ascontiguousarray(g.position, dtype=np.int32)]
         k = self._keep
+        nested = bool(np.any(g.iparent))
         self._grids = MgpGrids(g.ngrid, g.minlevel, g.maxlevel,
                                _ptr(k[0], C.c_int32), _ptr(k[1], C.c_int32),
                                _ptr(k[2], C.c_int32), _ptr(k[3], C.c_int32),
                                _ptr(k[4], C.c_int32), _ptr(k[5], C.c_int32),
                                _ptr(k[6], C.c_int64), _ptr(k[7], C.c_int64),
-                               _ptr(k[8], C.c_int64), _ptr(k[9], C.c_int64))
+                               _ptr(k[8], C.c_int64), _ptr(k[9], C.c_int64),
+                               _ptr(k[10], C.c_int32) if nested else None,
+                               _ptr(k[11], C.c_int32) if nested else None)
         self.cfg = MgpConfig(_TYPES[t], ninner, nouter, nouter_min, epcorr, omega, rho,
                              _IB[ib], _PREC[precision], eps_rel, int(bool(sor_all_levels)),
                              check_every)

@@ -23,7 +23,7 @@ class MgpGrids(C.Structure):
     _fields_ = [("ngrid", C.c_int32), ("minlevel", C.c_int32), ("maxlevel", C.c_int32),
                 ("kk", _i32p), ("jj", _i32p), ("ii", _i32p), ("level", _i32p),
                 ("nbrgrid", _i32p), ("itypbc", _i32p), ("ip3", _i64p), ("ip1x", _i64p),
-                ("ip1y", _i64p), ("ip1z", _i64p)]
+                ("ip1y", _i64p), ("ip1z", _i64p), ("iparent", _i32p), ("position", _i32p)]
 
 
 class MgpConfig(C.Structure):

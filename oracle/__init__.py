@@ -241,15 +241,17 @@ class Oracle:
                                    _p(g.dx), _p(g.dy), _p(g.dz), _p(g.ddx), _p(g.ddy),
                                    _p(g.ddz))
 
-    def ftoc(self, level, f):
+    def ftoc(self, level, f, bp=None, flag="P"):
         g = self.g
         self.lib.og_ftoc_s(C.byref(self.topo), level, _p(f), _p(f), _p(g.ddx), _p(g.ddy),
-                           _p(g.ddz))
+                           _p(g.ddz), _p(bp) if bp is not None else None,
+                           C.c_char(flag.encode()))
 
-    def ftoc_uvwp(self, level, u, v, w, p):
+    def ftoc_uvwp(self, level, u, v, w, p, bp=None):
         g = self.g
         self.lib.og_ftoc_uvwp(C.byref(self.topo), level, _p(u), _p(v), _p(w), _p(p),
-                              _p(g.ddx), _p(g.ddy), _p(g.ddz))
+                              _p(g.ddx), _p(g.ddy), _p(g.ddz),
+                              _p(bp) if bp is not None else None)
 
     def time_sip(self, st, rhs, niter, nthreads=0):
         """Seconds for `niter` hyperplane-SIP inner iterations (CPU baseline)."""

@@ -1168,12 +1168,16 @@ void og_bound_pressure(const og_topo *t, int level, double *p_,
     }
 }
 
-/* ftoc of a cell-centred scalar ('P', 'R'): restrict_s
- * (src/ib/noib_restrict_mod.F90:255-298) into restrict_recieve_open
- * (src/ib/ftoc_mod.F90:581-626): volume-weighted mean of each 2x2x2 group
- * of fine interior cells replaces the parent cell under it. */
+/* ftoc of a cell-centred scalar into the parent cells under the grid
+ * (restrict_recieve_open, src/ib/ftoc_mod.F90:581-626).  bp NULL: noib
+ * restrict_s, the volume-weighted mean of each 2x2x2 fine group
+ * (src/ib/noib_restrict_mod.F90:255-298).  With bp (ghostcell,
+ * src/ib/gc_restrict_mod.F90): flag 'P' restrict_p, the BP-weighted mean
+ * (0 for a fully blocked group, :59-164); flag 'R' restrict_r, the sum of
+ * BP-weighted values over the full group volume (:167-259). */
 void og_ftoc_s(const og_topo *t, int level, const double *ff_, double *fc_,
-               const double *ddx_, const double *ddy_, const double *ddz_)
+               const double *ddx_, const double *ddy_, const double *ddz_,
+               const double *bp_, char flag)
 {
     for (int g = 1; g <= t->ngrid; ++g) {
         if (t->level[g - 1] != level) continue;
@@ -1183,26 +1187,70 @@ void og_ftoc_s(const og_topo *t, int level, const double *ff_, double *fc_,
         const int ipos = POS(g, 0), jpos = POS(g, 1), kpos = POS(g, 2);
         const int64_t kkc = t->kk[c - 1], jjc = t->jj[c - 1];
         const double *ff = ff_ + t->ip3[g - 1];
+        const double *bp = bp_ ? bp_ + t->ip3[g - 1] : NULL;
         double *fc = fc_ + t->ip3[c - 1];
         const double *ddx = ddx_ + t->ip1x[g - 1] - 1, *ddy = ddy_ + t->ip1y[g - 1] - 1,
                      *ddz = ddz_ + t->ip1z[g - 1] - 1;
         for (int i = 3; i <= ii - 3; i += 2)
             for (int j = 3; j <= jj - 3; j += 2)
                 for (int k = 3; k <= kk - 3; k += 2) {
-                    const double spv =
-                        ff[IX(k, j, i)] * ddz[k] * ddy[j] * ddx[i] +
-                        ff[IX(k, j, i + 1)] * ddz[k] * ddy[j] * ddx[i + 1] +
-                        ff[IX(k, j + 1, i)] * ddz[k] * ddy[j + 1] * ddx[i] +
-                        ff[IX(k, j + 1, i + 1)] * ddz[k] * ddy[j + 1] * ddx[i + 1] +
-                        ff[IX(k + 1, j, i)] * ddz[k + 1] * ddy[j] * ddx[i] +
-                        ff[IX(k + 1, j, i + 1)] * ddz[k + 1] * ddy[j] * ddx[i + 1] +
-                        ff[IX(k + 1, j + 1, i)] * ddz[k + 1] * ddy[j + 1] * ddx[i] +
-                        ff[IX(k + 1, j + 1, i + 1)] * ddz[k + 1] * ddy[j + 1] * ddx[i + 1];
-                    const double sv = (ddx[i] + ddx[i + 1]) * (ddy[j] + ddy[j + 1]) *
-                                      (ddz[k] + ddz[k + 1]);
+                    double v;
+                    if (!bp) {
+                        const double spv =
+                            ff[IX(k, j, i)] * ddz[k] * ddy[j] * ddx[i] +
+                            ff[IX(k, j, i + 1)] * ddz[k] * ddy[j] * ddx[i + 1] +
+                            ff[IX(k, j + 1, i)] * ddz[k] * ddy[j + 1] * ddx[i] +
+                            ff[IX(k, j + 1, i + 1)] * ddz[k] * ddy[j + 1] * ddx[i + 1] +
+                            ff[IX(k + 1, j, i)] * ddz[k + 1] * ddy[j] * ddx[i] +
+                            ff[IX(k + 1, j, i + 1)] * ddz[k + 1] * ddy[j] * ddx[i + 1] +
+                            ff[IX(k + 1, j + 1, i)] * ddz[k + 1] * ddy[j + 1] * ddx[i] +
+                            ff[IX(k + 1, j + 1, i + 1)] * ddz[k + 1] * ddy[j + 1] * ddx[i + 1];
+                        const double sv = (ddx[i] + ddx[i + 1]) * (ddy[j] + ddy[j + 1]) *
+                                          (ddz[k] + ddz[k + 1]);
+                        v = spv / sv;
+                    } else if (flag == 'P') {
+                        double vol[8], pv[8];
+                        int q = 0;
+                        for (int n = 0; n <= 1; ++n) {      /* order of :96-125 */
+                            const int kq = k + n;
+                            vol[q] = bp[IX(kq, j, i)] * ddz[kq] * ddy[j] * ddx[i];
+                            vol[q + 1] = bp[IX(kq, j, i + 1)] * ddz[kq] * ddy[j] * ddx[i + 1];
+                            vol[q + 2] = bp[IX(kq, j + 1, i)] * ddz[kq] * ddy[j + 1] * ddx[i];
+                            vol[q + 3] = bp[IX(kq, j + 1, i + 1)] * ddz[kq] * ddy[j + 1] * ddx[i + 1];
+                            pv[q] = ff[IX(kq, j, i)] * vol[q];
+                            pv[q + 1] = ff[IX(kq, j, i + 1)] * vol[q + 1];
+                            pv[q + 2] = ff[IX(kq, j + 1, i)] * vol[q + 2];
+                            pv[q + 3] = ff[IX(kq, j + 1, i + 1)] * vol[q + 3];
+                            q += 4;
+                        }
+                        const double spv = pv[0] + pv[1] + pv[2] + pv[3] + pv[4] + pv[5] +
+                                           pv[6] + pv[7];
+                        const double sv = vol[0] + vol[1] + vol[2] + vol[3] + vol[4] +
+                                          vol[5] + vol[6] + vol[7];
+                        v = sv > 0.0 ? spv / sv : 0.0;
+                    } else {
+                        double pv[8];
+                        int q = 0;
+                        for (int n = 0; n <= 1; ++n) {
+                            const int kq = k + n;
+                            pv[q] = ff[IX(kq, j, i)] * bp[IX(kq, j, i)] * ddz[kq] * ddy[j] * ddx[i];
+                            pv[q + 1] = ff[IX(kq, j, i + 1)] * bp[IX(kq, j, i + 1)] * ddz[kq] *
+                                        ddy[j] * ddx[i + 1];
+                            pv[q + 2] = ff[IX(kq, j + 1, i)] * bp[IX(kq, j + 1, i)] * ddz[kq] *
+                                        ddy[j + 1] * ddx[i];
+                            pv[q + 3] = ff[IX(kq, j + 1, i + 1)] * bp[IX(kq, j + 1, i + 1)] *
+                                        ddz[kq] * ddy[j + 1] * ddx[i + 1];
+                            q += 4;
+                        }
+                        const double spv = pv[0] + pv[1] + pv[2] + pv[3] + pv[4] + pv[5] +
+                                           pv[6] + pv[7];
+                        const double sv = (ddx[i] + ddx[i + 1]) * (ddy[j] + ddy[j + 1]) *
+                                          (ddz[k] + ddz[k + 1]);
+                        v = spv / sv;
+                    }
                     const int ic = ipos + (i - 3) / 2, jc = jpos + (j - 3) / 2,
                               kc = kpos + (k - 3) / 2;
-                    fc[(kc - 1) + (jc - 1) * kkc + (ic - 1) * kkc * jjc] = spv / sv;
+                    fc[(kc - 1) + (jc - 1) * kkc + (ic - 1) * kkc * jjc] = v;
                 }
     }
 }
@@ -1212,7 +1260,7 @@ void og_ftoc_s(const og_topo *t, int level, const double *ff_, double *fc_,
  * (src/ib/ftoc_mod.F90:581-626). */
 void og_ftoc_uvwp(const og_topo *t, int level, double *u_, double *v_,
                   double *w_, double *p_, const double *ddx_,
-                  const double *ddy_, const double *ddz_)
+                  const double *ddy_, const double *ddz_, const double *bp_)
 {
     for (int g = 1; g <= t->ngrid; ++g) {
         if (t->level[g - 1] != level) continue;
@@ -1269,7 +1317,7 @@ void og_ftoc_uvwp(const og_topo *t, int level, double *u_, double *v_,
         }
 #undef CI
     }
-    og_ftoc_s(t, level, p_, p_, ddx_, ddy_, ddz_);
+    og_ftoc_s(t, level, p_, p_, ddx_, ddy_, ddz_, bp_, 'P');
 }
 
 /* ------------------------------------------------------------------ */
@@ -1369,8 +1417,9 @@ int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
         og_divcal(t, rhs, u, v, w, rddx, rddy, rddz, prefak, bp, sdiv);
         free(sdiv);
     }
+    const double *bpr = prm->ibtype ? bp : NULL;   /* gc_restrict with the IB */
     for (int l = t->maxlevel; l >= t->minlevel; --l)
-        og_ftoc_s(t, l, rhs, rhs, ddx, ddy, ddz);
+        og_ftoc_s(t, l, rhs, rhs, ddx, ddy, ddz, bpr, 'R');
     st->maxrhs0 = og_maxabscal(t, rhs, NULL);
 
     int ipc = 0, ipcount;
@@ -1383,13 +1432,13 @@ int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
                      ab, at, ap, lw, ls, lb, lpr, ue, un, ut, mip, idx, rap, &mh);
         }
         for (int l = t->maxlevel; l >= t->minlevel; --l)
-            og_ftoc_s(t, l, hilf, hilf, ddx, ddy, ddz);
+            og_ftoc_s(t, l, hilf, hilf, ddx, ddy, ddz, bpr, 'P');
         double *fh[1] = {hilf};
         og_exchange(t, c, nconn, -1, 1, 0, 0, fh, 1);
         og_laplacephi(t, -1, res, hilf, aw, ae, as, an, ab, at, ap, bp);
         og_rescal(t, rhs, res);
         for (int l = t->maxlevel; l >= t->minlevel + 1; --l)
-            og_ftoc_s(t, l, rhs, rhs, ddx, ddy, ddz);
+            og_ftoc_s(t, l, rhs, rhs, ddx, ddy, ddz, bpr, 'R');
         maxrhs = og_maxabscal(t, rhs, NULL);
         og_accumulate(n, dp, hilf);
         memset(hilf, 0, (size_t)n * sizeof(double));
@@ -1415,7 +1464,7 @@ int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
     og_mgpcorr(t, u, v, w, p, dp, rdx, rdy, rdz, dt / prm->rho,
                prm->ibtype ? bp : NULL);
     for (int l = t->maxlevel; l >= t->minlevel; --l)
-        og_ftoc_uvwp(t, l, u, v, w, p, ddx, ddy, ddz);
+        og_ftoc_uvwp(t, l, u, v, w, p, ddx, ddy, ddz, bpr);
     double *fs[4] = {u, v, w, p};
     for (int l = t->minlevel; l <= t->maxlevel; ++l)
         og_exchange(t, c, nconn, l, 2, 1, 0, fs, 4);

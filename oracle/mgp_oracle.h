@@ -161,13 +161,17 @@ void og_bound_pressure(const og_topo *t, int level, double *p,
                        const double *bp, const double *dx, const double *dy,
                        const double *dz, const double *ddx, const double *ddy,
                        const double *ddz);
-/* src/ib/noib_restrict_mod.F90:255-298 + src/ib/ftoc_mod.F90:581-626 */
+/* src/ib/noib_restrict_mod.F90:255-298 (bp NULL) or the ghostcell
+ * src/ib/gc_restrict_mod.F90 restrict_p / restrict_r (flag 'P' / 'R'),
+ * into src/ib/ftoc_mod.F90:581-626 */
 void og_ftoc_s(const og_topo *t, int level, const double *ff, double *fc,
-               const double *ddx, const double *ddy, const double *ddz);
-/* ftoc(ilevel, u, v, w, p): src/ib/noib_restrict_mod.F90:135-298 */
+               const double *ddx, const double *ddy, const double *ddz,
+               const double *bp, char flag);
+/* ftoc(ilevel, u, v, w, p): src/ib/noib_restrict_mod.F90:135-298; with bp
+ * the pressure uses the ghostcell restrict_p */
 void og_ftoc_uvwp(const og_topo *t, int level, double *u, double *v,
                   double *w, double *p, const double *ddx, const double *ddy,
-                  const double *ddz);
+                  const double *ddz, const double *bp);
 
 /* src/flow/pressuresolver_mod.F90:160-357, single rank, all levels (the
  * final per-level parent/bound_flow of U, V, W, P belong to the caller;

@@ -97,3 +97,134 @@ def centres(g):
         g.block_view(Y, b)[...] = y[None, :, None]
         g.block_view(Z, b)[...] = z[None, None, :]
     return X, Y, Z
+
+
+def three_level():
+    """Three levels with mixed block sizes: 2x2x2 blocks of 8^3; the first
+    refined into 4^3 children (64, each covering 2^3 parent cells); two
+    neighbouring ones refined into one 8^3 child each (covering the whole
+    4^3 parent), so the finest level has CON and PAR faces."""
+    from mgp.grids import refined_box
+    pick = {(4, 4, 4), (8, 4, 4)}
+    return refined_box((2, 2, 2), 8,
+                       [(4, lambda B: tuple(B["origin"]) == (0, 0, 0)),
+                        (8, lambda B: tuple(B["origin"]) in pick)],
+                       periodic=(True, True, True))
+
+
+def sphere_mask(g, radius=0.2, centre=(0.5, 0.5, 0.5)):
+    """int8 mask (1 = blocked) from the signed distance to a sphere at the
+    cell centres, ghosts included (every block owns its ghost cells' centres)."""
+    X, Y, Z = centres(g)
+    d = np.sqrt((X - centre[0]) ** 2 + (Y - centre[1]) ** 2 + (Z - centre[2]) ** 2) - radius
+    return (d < 0).astype(np.int8)
+
+
+def velocity_levels(g, o, seed, amp=1.0, blocked=None):
+    """U(-amp, amp) face velocity on a nested hierarchy whose pressure
+    problem is consistent on every level: faces next to blocked cells, faces
+    on the planes of PAR faces and parent faces between covered and
+    uncovered parent cells carry no flux, the parent faces under a refined
+    grid hold the restriction of the fine faces (ftoc), and the ghosts of
+    every level are exchanged."""
+    rng = np.random.default_rng(seed)
+    inner = g.interior_mask()
+    u, v, w = (np.where(inner, rng.uniform(-amp, amp, g.n3d), 0.0) for _ in range(3))
+    apply_walls(g, u, v, w)
+    if blocked is not None:
+        close_blocked_faces(g, blocked, u, v, w)
+    covered = g.zeros()
+    for b in range(1, g.ngrid + 1):
+        if g.iparent[b - 1]:
+            n = (g.ii[b - 1] - 4, g.jj[b - 1] - 4, g.kk[b - 1] - 4)
+            p0 = g.position[b - 1] - 1
+            g.block_view(covered, g.iparent[b - 1])[
+                p0[0]:p0[0] + n[0] // 2, p0[1]:p0[1] + n[1] // 2, p0[2]:p0[2] + n[2] // 2] = 1.0
+    for lvl in range(g.minlevel, g.maxlevel + 1):
+        o.exchange([covered], level=lvl, layers=2, corners=True)
+    for b in range(1, g.ngrid + 1):
+        c = g.block_view(covered, b)
+        for f, (arr, ax) in enumerate(((u, 0), (u, 0), (v, 1), (v, 1), (w, 2), (w, 2))):
+            a = g.block_view(arr, b)
+            if f % 2 == 0:
+                sl0, sl1 = [slice(None)] * 3, [slice(None)] * 3
+                sl0[ax], sl1[ax] = slice(0, -1), slice(1, None)
+                a[tuple(sl0)] *= (c[tuple(sl0)] == c[tuple(sl1)])
+            if g.bc[b - 1, f] == 8:
+                sl = [slice(None)] * 3
+                sl[ax] = 1 if f % 2 == 0 else a.shape[ax] - 3
+                a[tuple(sl)] = 0.0
+    p = g.zeros()
+    for lvl in range(g.maxlevel, g.minlevel, -1):
+        o.ftoc_uvwp(lvl, u, v, w, p)
+    for lvl in range(g.minlevel, g.maxlevel + 1):
+        o.exchange([u, v, w], level=lvl, layers=2, corners=True)
+    return u, v, w
+
+
+def config3(width=0.1):
+    """Multi-level batch with an immersed sphere: a level-1 box of 4x4x4
+    blocks of 24^3 on the unit cube (periodic), refined everywhere into
+    16^3 blocks (level 2), and around the sphere of radius 0.2 at the centre
+    into 32^3 blocks (level 3, one per level-2 block whose centre lies
+    within `width` of the sphere surface)."""
+    from mgp.grids import refined_box
+    shell = lambda B: abs(np.linalg.norm(B["centre"] - 0.5) - 0.2) < width
+    return refined_box((4, 4, 4), 24, [(16, lambda B: True), (32, shell)])
+
+
+def close_blocked_faces(g, blocked, u, v, w):
+    """Zero the face velocities next to blocked cells (the immersed
+    boundary's no-through-flow condition)."""
+    fluid = 1.0 - blocked.astype(np.float64)
+    for b in range(1, g.ngrid + 1):
+        f = g.block_view(fluid, b)
+        for arr, ax in ((u, 0), (v, 1), (w, 2)):
+            a = g.block_view(arr, b)
+            sl0 = [slice(None)] * 3
+            sl1 = [slice(None)] * 3
+            sl0[ax] = slice(0, -1)
+            sl1[ax] = slice(1, None)
+            open_ = f[tuple(sl0)] * f[tuple(sl1)]
+            a[tuple(sl0)] *= open_
+
+
+def mask_levels(g, o, blocked):
+    """BP of a nested hierarchy as the immersed-boundary setup leaves it
+    (src/ib/gc_blockbp_mod.F90:299): the finest cells from the geometry,
+    parent cells under a refined grid fluid if any of their 8 children is
+    (ftoc 'E', the max over each 2x2x2 group, src/ib/gc_restrict_mod.F90:
+    289-320), ghosts exchanged per level.  Returns (bp, blocked int8)."""
+    bp = 1.0 - blocked.astype(np.float64)
+    for lvl in range(g.maxlevel, g.minlevel, -1):
+        for b in range(1, g.ngrid + 1):
+            if g.level[b - 1] != lvl:
+                continue
+            f = g.block_view(bp, b)[2:-2, 2:-2, 2:-2]
+            n = f.shape
+            m = f.reshape(n[0] // 2, 2, n[1] // 2, 2, n[2] // 2, 2).max(axis=(1, 3, 5))
+            pv = g.block_view(bp, g.iparent[b - 1])
+            p0 = g.position[b - 1] - 1
+            pv[p0[0]:p0[0] + n[0] // 2, p0[1]:p0[1] + n[1] // 2, p0[2]:p0[2] + n[2] // 2] = m
+    for lvl in range(g.minlevel, g.maxlevel + 1):
+        o.exchange([bp], level=lvl, layers=2, corners=True)
+        # ghost layers of PAR faces one level finer: the parent's cells
+        # across the face (piecewise constant, as parent prolongs them)
+        for b in range(1, g.ngrid + 1):
+            if g.level[b - 1] != lvl + 1:
+                continue
+            fv, pv = g.block_view(bp, b), g.block_view(bp, g.iparent[b - 1])
+            p0 = g.position[b - 1] - 1
+            n = fv.shape
+            for f in range(6):
+                if g.bc[b - 1, f] != 8:
+                    continue
+                ax = f // 2
+                idx = [np.arange(2, n[a] - 2) for a in range(3)]
+                cidx = [p0[a] + (idx[a] - 2) // 2 for a in range(3)]
+                for layer in (0, 1):
+                    idx[ax] = np.array([1 - layer if f % 2 == 0 else n[ax] - 2 + layer])
+                    cidx[ax] = np.array([p0[ax] - 1 if f % 2 == 0
+                                         else p0[ax] + (n[ax] - 4) // 2])
+                    fv[np.ix_(*idx)] = pv[np.ix_(*cidx)]
+    return bp, (bp == 0.0).astype(np.int8)

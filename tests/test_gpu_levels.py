@@ -1,0 +1,95 @@
+"""GPU parity on nested grid levels (ctof, parent, bound_pressure, ftoc;
+src/flow/pressuresolver_mod.F90:160-420): bitwise FP64 equality with the
+oracle at a fixed iteration count, and the FP32-refined form converged."""
+import numpy as np
+import pytest
+
+from cases import blocked_mask, mask_levels, sphere_mask, three_level, two_level, velocity_levels
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(oracle_mod, g, seed, ityp=0, ninner=3, nouter=3, ibtype=0, blocked=None,
+         eps_rel=0.0, precision="fp64", omega=1.1):
+    import mgp
+    o = oracle_mod.Oracle(g)
+    bp = None
+    if blocked is not None:
+        bp, blocked = mask_levels(g, o, blocked)
+    st = o.setup(bp=bp, ityp=ityp)
+    u, v, w = velocity_levels(g, o, seed, blocked=blocked)
+    p = np.zeros(g.n3d)
+    ins = [a.copy() for a in (u, v, w, p)]
+    so = o.mgpoisl(st, u, v, w, p, 1e-3, ityp=ityp, ninner=ninner, nouter=nouter,
+                   eps_rel=eps_rel, ibtype=ibtype, omega=omega)
+    s = mgp.PressureSolver(g, type={0: "sip", 1: "sor", 2: "classic"}[ityp], ninner=ninner,
+                           nouter=nouter, eps_rel=eps_rel, omega=omega,
+                           ib={0: "noib", 1: "ghostcell"}[ibtype], precision=precision,
+                           bp=bp)
+    sg = s.mgpoisl(*ins, 1e-3)
+    return so, sg, s, (u, v, w, p), ins
+
+
+def _equal(so, sg, s, ref, got):
+    assert sg["maxrhs0"] == so["maxrhs0"]
+    assert sg["maxrhs"] == so["maxrhs"]
+    assert np.array_equal(s.history(), so["history"])
+    for a, b, nm in zip(ref, got, "uvwp"):
+        assert np.array_equal(a, b), nm
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("ityp", [0, 2])
+def test_two_level_bitwise(oracle_mod, periodic, ityp):
+    g = two_level(periodic=periodic)
+    so, sg, s, ref, got = _run(oracle_mod, g, 81, ityp=ityp)
+    _equal(so, sg, s, ref, got)
+
+
+def test_two_level_sor_on_the_fine_level(oracle_mod):
+    """type "sor": SOR on the refined level, SIP on the coarsest
+    (src/flow/pressuresolver_mod.F90:403-412)."""
+    g = two_level()
+    so, sg, s, ref, got = _run(oracle_mod, g, 82, ityp=1, ninner=4, omega=1.2)
+    _equal(so, sg, s, ref, got)
+
+
+def test_two_level_masked_ghostcell_bitwise(oracle_mod):
+    """Random blocked cells with BP consistent across levels (a parent cell
+    is fluid when any child is); ghostcell restriction gc_restrict."""
+    g = two_level(periodic=True)
+    so, sg, s, ref, got = _run(oracle_mod, g, 84, ibtype=1,
+                               blocked=blocked_mask(g, 83, 0.12).astype(np.int8))
+    _equal(so, sg, s, ref, got)
+
+
+@pytest.mark.parametrize("ninner", [1, 2, 5])
+def test_three_level_mixed_sizes_bitwise(oracle_mod, ninner):
+    g = three_level()
+    so, sg, s, ref, got = _run(oracle_mod, g, 85, ninner=ninner, nouter=3)
+    _equal(so, sg, s, ref, got)
+
+
+def test_sphere_three_levels_masked_bitwise(oracle_mod):
+    """Blocked cells inside a sphere (int8 mask from the signed distance)
+    across three levels, ghostcell operator."""
+    g = three_level()
+    blocked = sphere_mask(g, radius=0.3, centre=(0.25, 0.25, 0.25))
+    so, sg, s, ref, got = _run(oracle_mod, g, 86, ibtype=1, blocked=blocked, nouter=2)
+    _equal(so, sg, s, ref, got)
+
+
+@pytest.mark.parametrize("masked", [False, True])
+def test_multilevel_converged_fp32_refined(oracle_mod, masked):
+    g = three_level()
+    blocked = sphere_mask(g, radius=0.3, centre=(0.25, 0.25, 0.25)) if masked else None
+    so, sg, s, ref, got = _run(oracle_mod, g, 87, nouter=2000, eps_rel=1e-10,
+                               precision="fp32", ibtype=int(masked), blocked=blocked)
+    assert so["maxrhs"] <= 1e-10 * so["maxrhs0"]
+    assert sg["converged"] and sg["maxrhs"] <= 1e-10 * sg["maxrhs0"]
+    inner = g.interior_mask()
+    a, b = got[3][inner], ref[3][inner]
+    a, b = a - a.mean(), b - b.mean()
+    assert np.abs(a - b).max() <= 1e-7 * np.abs(b).max()
+    for x, y in zip(got[:3], ref[:3]):
+        assert np.abs(x - y).max() <= 1e-7 * np.abs(y).max()

@@ -640,3 +640,37 @@ def test_multilevel_full_refinement_matches_single_level(oracle_mod):
         if arr_s is ps:
             got, ref = got - got.mean(), ref - ref.mean()
         assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def test_ghostcell_restriction_forms(oracle_mod):
+    """gc_restrict restrict_p / restrict_r (src/ib/gc_restrict_mod.F90):
+    with BP = 1 both equal the plain volume-weighted mean (to rounding); a
+    fully blocked group restricts to 0 under 'P'; 'R' sums only fluid
+    cells over the full group volume."""
+    from cases import two_level
+    g = two_level()
+    o = oracle_mod.Oracle(g)
+    rng = np.random.default_rng(91)
+    f = rng.standard_normal(g.n3d)
+    ref = f.copy()
+    o.ftoc(2, ref)
+    for flag in ("P", "R"):
+        a = f.copy()
+        o.ftoc(2, a, bp=g.ones(), flag=flag)
+        assert np.allclose(a, ref, rtol=1e-14, atol=1e-14)
+    bp = g.ones()
+    child = g.block_view(bp, 9)
+    child[2:4, 2:4, 2:4] = 0.0                 # first 2x2x2 group blocked
+    child[4:6, 2:4, 2:4] = 0.0
+    child[4, 2, 2] = 1.0                       # second group: one fluid cell
+    fv = g.block_view(f, 9)
+    a = f.copy()
+    o.ftoc(2, a, bp=bp, flag="P")
+    pv = g.block_view(a, 1)
+    assert pv[2, 2, 2] == 0.0
+    assert np.isclose(pv[3, 2, 2], fv[4, 2, 2], rtol=1e-14)
+    a = f.copy()
+    o.ftoc(2, a, bp=bp, flag="R")
+    pv = g.block_view(a, 1)
+    assert pv[2, 2, 2] == 0.0
+    assert np.isclose(pv[3, 2, 2], fv[4, 2, 2] / 8.0, rtol=1e-14)
