@@ -93,3 +93,41 @@ def test_multilevel_converged_fp32_refined(oracle_mod, masked):
     assert np.abs(a - b).max() <= 1e-7 * np.abs(b).max()
     for x, y in zip(got[:3], ref[:3]):
         assert np.abs(x - y).max() <= 1e-7 * np.abs(y).max()
+
+
+def test_config3_sphere_full_size(oracle_mod):
+    """Multi-level batch with the immersed sphere at full size (level-1 box of
+    4x4x4 blocks of 24^3, 1728 blocks of 16^3, 200 of 32^3 around the sphere
+    of radius 0.2): FP64 bitwise equal to the oracle over two outer
+    iterations; the FP32-refined solve converged to 1e-10 against the FP64
+    one (error reported in the assertion message)."""
+    import mgp
+    from cases import config3
+    g = config3()
+    assert g.ngrid == 1992
+    o = oracle_mod.Oracle(g)
+    bp, blocked = mask_levels(g, o, sphere_mask(g))
+    st = o.setup(bp=bp)
+    u, v, w = velocity_levels(g, o, 3, blocked=blocked)
+    ins = [a.copy() for a in (u, v, w, np.zeros(g.n3d))]
+    ref = [a.copy() for a in ins]
+    so = o.mgpoisl(st, *ref, 1e-3, ninner=5, nouter=2, ibtype=1)
+    s64 = mgp.PressureSolver(g, ninner=5, nouter=2, ib="ghostcell", bp=bp)
+    got = [a.copy() for a in ins]
+    sg = s64.mgpoisl(*got, 1e-3)
+    _equal(so, sg, s64, ref, got)
+    s64.set_iterations(nouter=3000, eps_rel=1e-10)
+    r64 = [a.copy() for a in ins]
+    st64 = s64.mgpoisl(*r64, 1e-3)
+    s32 = mgp.PressureSolver(g, ninner=5, nouter=3000, eps_rel=1e-10, ib="ghostcell", bp=bp,
+                             precision="fp32")
+    r32 = [a.copy() for a in ins]
+    st32 = s32.mgpoisl(*r32, 1e-3)
+    assert st64["converged"] and st32["converged"]
+    inner = g.interior_mask()
+    a, b = r32[3][inner], r64[3][inner]
+    err_p = np.abs((a - a.mean()) - (b - b.mean())).max() / np.abs(b - b.mean()).max()
+    err_u = max(np.abs(x - y).max() for x, y in zip(r32[:3], r64[:3])) / \
+        max(np.abs(y).max() for y in r64[:3])
+    assert err_p < 1e-7 and err_u < 1e-7, (err_p, err_u, st64["nouter_done"],
+                                           st32["nouter_done"])
