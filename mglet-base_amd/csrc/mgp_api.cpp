@@ -133,6 +133,7 @@ struct mgp_handle {
 
     // device topology
     DBuf d_blocks, d_shapes, d_cells, d_pstart, d_slot, d_blist_all;
+    DBuf d_items, d_ents, d_zero, d_cellx;   // plans of the plane kernels
     std::map<int, DBuf> d_blist_lvl;
     // 1-D fields (FP64 and working precision)
     DBuf d_dx, d_dy, d_dz, d_rdx, d_rdy, d_rdz, d_rddx, d_rddy, d_rddz;
@@ -191,6 +192,10 @@ struct mgp_handle {
         t.max_nk = max_nk;
         t.max_plane = max_plane;
         t.maxlevel = maxlevel;
+        t.items = d_items.as<uint2>();
+        t.ents = d_ents.as<uint32_t>();
+        t.cellx = d_cellx.as<uint2>();
+        t.zero = d_zero.p;
         if (lvl < 0) {
             t.blist = d_blist_all.as<int>();
             t.nlist = ngrid;
@@ -456,6 +461,176 @@ void exchange(mgp_handle &h, int level, int layers, int corners, int forward,
 }
 
 // ------------------------------------------------------------------
+// Plans of the plane kernels (mgp_sweeps.hip), per shape.  Step p of a
+// plan computes hyperplane p (SOR: the plane its colour schedule relaxes)
+// and moves the entries that later steps read into LDS: the HILF window
+// plane p+2 into ring slot (p+3) mod 4 (mod 8 for SOR), and the ghosts of
+// the recurrence that step p+1 reads (the backward sweep: p-1) into pencil
+// buffer p mod 2.  Steps before the first plane only fill LDS.
+void build_plans(mgp_handle &h, const std::vector<uint32_t> &cells,
+                 const std::vector<int> &pstart, std::vector<uint2> &items,
+                 std::vector<uint32_t> &ents, std::vector<uint2> &cellx) {
+    cellx.resize(cells.size());
+    for (Shape &S : h.shapes) {
+        const int NI = S.ni, NJ = S.nj, NK = S.nk, NP = S.np;
+        const int HW = NJ + 2, RW = NJ + 1;
+        const uint32_t *cl = cells.data() + S.cell_off;
+        const int *ps = pstart.data() + S.pstart_off;
+        // cell words: indices, face positions and boundary flags
+        for (int s = 0; s < S.ncell; ++s) {
+            const int i = cl[s] & 1023, j = (cl[s] >> 10) & 1023, k = cl[s] >> 20;
+            const uint32_t fl = (i == 0) | (i == NI - 1) << 1 | (j == 0) << 2 |
+                                (j == NJ - 1) << 3 | (k == 0) << 4 | (k == NK - 1) << 5;
+            uint2 w;
+            w.x = (uint32_t)i | (uint32_t)j << 6 | (uint32_t)k << 12 |
+                  (uint32_t)fidx_z(j, i, NI, NJ) << 18;
+            w.y = (uint32_t)fidx_x(k, j, NJ, NK) | (uint32_t)fidx_y(k, i, NI, NK) << 12 | fl << 24;
+            cellx[S.cell_off + s] = w;
+        }
+        using Ents = std::vector<uint32_t>;
+        // HILF window plane q: its interior cells, then the face ghosts
+        // (i, j, -1), (i, j, NK), (i, -1, k), (i, NJ, k), (-1, j, k),
+        // (NI, j, k) that lie on plane q, each at its pencil position
+        auto window = [&](int q, Ents &E) {
+            if (q >= 0 && q < NP)
+                for (int s = ps[q]; s < ps[q + 1]; ++s) {
+                    const int i = cl[s] & 1023, j = (cl[s] >> 10) & 1023;
+                    E.push_back(plan_entry((j + 1) + HW * (i + 1), 6, 0, s));
+                }
+            for (int i = 0; i < NI; ++i) {
+                const int j = q + 1 - i;
+                if (j >= 0 && j < NJ) E.push_back(plan_entry((j + 1) + HW * (i + 1), 4, 0, fidx_z(j, i, NI, NJ)));
+            }
+            for (int i = 0; i < NI; ++i) {
+                const int j = q - NK - i;
+                if (j >= 0 && j < NJ) E.push_back(plan_entry((j + 1) + HW * (i + 1), 5, 0, fidx_z(j, i, NI, NJ)));
+            }
+            for (int i = 0; i < NI; ++i) {
+                const int k = q + 1 - i;
+                if (k >= 0 && k < NK) E.push_back(plan_entry(HW * (i + 1), 2, 0, fidx_y(k, i, NI, NK)));
+            }
+            for (int i = 0; i < NI; ++i) {
+                const int k = q - NJ - i;
+                if (k >= 0 && k < NK) E.push_back(plan_entry((NJ + 1) + HW * (i + 1), 3, 0, fidx_y(k, i, NI, NK)));
+            }
+            for (int j = 0; j < NJ; ++j) {
+                const int k = q + 1 - j;
+                if (k >= 0 && k < NK) E.push_back(plan_entry(j + 1, 0, 0, fidx_x(k, j, NJ, NK)));
+            }
+            for (int j = 0; j < NJ; ++j) {
+                const int k = q - NI - j;
+                if (k >= 0 && k < NK) E.push_back(plan_entry((j + 1) + HW * (NI + 1), 1, 0, fidx_x(k, j, NJ, NK)));
+            }
+        };
+        // forward recurrence: pencil starts (i, j, -1), and the ghosts
+        // (i, -1, k), (-1, j, k) read by the cells of plane pp
+        auto fwd_start = [&](Ents &E) {
+            for (int i = 0; i < NI; ++i)
+                for (int j = 0; j < NJ; ++j)
+                    E.push_back(plan_entry((j + 1) + RW * (i + 1), 4, 1, fidx_z(j, i, NI, NJ)));
+        };
+        auto fwd_border = [&](int pp, Ents &E) {
+            for (int i = 0; i < NI; ++i) {
+                const int k = pp - i;
+                if (k >= 0 && k < NK) E.push_back(plan_entry(RW * (i + 1), 2, 1, fidx_y(k, i, NI, NK)));
+            }
+            for (int j = 0; j < NJ; ++j) {
+                const int k = pp - j;
+                if (k >= 0 && k < NK) E.push_back(plan_entry(j + 1, 0, 1, fidx_x(k, j, NJ, NK)));
+            }
+        };
+        // backward recurrence: pencil starts (i, j, NK), ghosts (i, NJ, k),
+        // (NI, j, k) read by the cells of plane pp
+        auto bwd_start = [&](Ents &E) {
+            for (int i = 0; i < NI; ++i)
+                for (int j = 0; j < NJ; ++j)
+                    E.push_back(plan_entry(j + RW * i, 5, 1, fidx_z(j, i, NI, NJ)));
+        };
+        auto bwd_border = [&](int pp, Ents &E) {
+            for (int i = 0; i < NI; ++i) {
+                const int k = pp - (NJ - 1) - i;
+                if (k >= 0 && k < NK) E.push_back(plan_entry(NJ + RW * i, 3, 1, fidx_y(k, i, NI, NK)));
+            }
+            for (int j = 0; j < NJ; ++j) {
+                const int k = pp - j - (NI - 1);
+                if (k >= 0 && k < NK) E.push_back(plan_entry(j + RW * NI, 1, 1, fidx_x(k, j, NJ, NK)));
+            }
+        };
+        auto plane = [&](int q, int &cpos, int &ccnt) {
+            if (q >= 0 && q < NP) { cpos = ps[q]; ccnt = ps[q + 1] - ps[q]; }
+            else { cpos = 0; ccnt = 0; }
+        };
+        for (int kind = 0; kind < kPlanKinds; ++kind) {
+            S.plan_ent[kind] = (int)ents.size();
+            const int ebase0 = (int)ents.size();
+            struct Step { int p, cpos, ccnt, epos, ecnt; };
+            std::vector<Step> steps;
+            std::vector<int> order;
+            if (kind == kPlanBwd)
+                for (int p = NP + 1; p >= 0; --p) order.push_back(p);
+            else
+                for (int p = -3; p < (kind == kPlanSor ? NP + 2 : NP); ++p) order.push_back(p);
+            for (int p : order) {
+                Ents E;
+                int cpos, ccnt;
+                switch (kind) {
+                case kPlanFwdLap:
+                case kPlanFwdZero:
+                    if (kind == kPlanFwdLap && p + 2 <= NP) window(p + 2, E);
+                    if (p == -2 || p == -1) fwd_start(E);
+                    if (p >= -1 && p + 1 < NP) fwd_border(p + 1, E);
+                    plane(p, cpos, ccnt);
+                    break;
+                case kPlanBwd:
+                    if (p >= NP) bwd_start(E);
+                    if (p <= NP && p >= 1) bwd_border(p - 1, E);
+                    plane(p, cpos, ccnt);
+                    break;
+                case kPlanOut:
+                    if (p + 2 <= NP) window(p + 2, E);
+                    plane(p, cpos, ccnt);
+                    break;
+                default:  // kPlanSor
+                    if (p + 2 <= NP) window(p + 2, E);
+                    plane(p < 0 ? -1 : ((p & 1) ? p : p - 2), cpos, ccnt);
+                    break;
+                }
+                steps.push_back({p, cpos, ccnt, (int)ents.size() - ebase0, (int)E.size()});
+                ents.insert(ents.end(), E.begin(), E.end());
+            }
+            // one valid entry for the clamped loads of empty items
+            ents.push_back(plan_entry(0, 7, 0, 0));
+            if ((int)ents.size() - ebase0 >= (1 << 21))
+                throw HipError{"plan entry table too large"};
+            auto pack = [](int cpos, int cc, int ec, int epos, int p) {
+                uint2 v;
+                v.x = (uint32_t)cpos | (uint32_t)cc << 16 | (uint32_t)ec << 24;
+                v.y = (uint32_t)epos | (uint32_t)(p + 8) << 21;
+                return v;
+            };
+            for (int wi = 0; wi < 2; ++wi) {
+                const int W = 64 << wi;
+                S.plan_item[wi][kind] = (int)items.size();
+                for (const Step &st : steps) {
+                    const int n = std::max((st.ccnt + W - 1) / W, (st.ecnt + W - 1) / W);
+                    for (int c = 0; c < n; ++c) {
+                        const int cc = std::min(std::max(st.ccnt - W * c, 0), W);
+                        const int ec = std::min(std::max(st.ecnt - W * c, 0), W);
+                        items.push_back(pack(cc ? st.cpos + W * c : 0, cc, ec,
+                                             ec ? st.epos + W * c : 0, st.p));
+                    }
+                }
+                // a closing no-op item (the engine clamps item indices to it)
+                do
+                    items.push_back(pack(0, 0, 0, 0, 0));
+                while ((items.size() - S.plan_item[wi][kind]) % kPlanPad);
+                S.plan_nitem[wi][kind] = (int)items.size() - S.plan_item[wi][kind];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------
 // Shape tables: hyperplane order of src/flow/sip_hyperplane_mod.F90:
 // 269-361 (planes m = i+j+k, ascending linear index inside a plane, i.e.
 // i ascending then j ascending).
@@ -589,6 +764,8 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
     if (!G || !cfg || !out || !dx || !dy || !dz || !ddx || !ddy || !ddz)
         return fail(MGP_ERR_ARG, "null argument");
     *out = nullptr;
+    // pipeline shape of the plane kernels (1, 2 or 3)
+    if (const char *d = getenv("MGP_PLAN_SHAPE")) set_plan_shape(atoi(d));
     int rc = check_cfg(*cfg);
     if (rc) return rc;
     if (G->ngrid <= 0) return fail(MGP_ERR_TOPOLOGY, "no grids");
@@ -616,10 +793,12 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
         h->ip1y.push_back(G->ip1y[g] - 1);
         h->ip1z.push_back(G->ip1z[g] - 1);
         const int ni = h->ii[g] - 4, nj = h->jj[g] - 4, nk = h->kk[g] - 4;
-        if (ni < 2 || nj < 2 || nk < 2 || 4 * ni + 2 * nj > 256 || ni + nj + nk - 2 > 512 ||
+        // plane kernels: 1-D coefficient rows of kMaxN1 entries, pencil
+        // arrays of (ni+2)(nj+2) cells in LDS, 16-bit plan offsets
+        if (ni < 2 || nj < 2 || nk < 2 || ni > kMaxN1 || nj > kMaxN1 || nk > kMaxN1 ||
             ni * nj > 1024)
             return fail(MGP_ERR_TOPOLOGY, "grid %d: interior %dx%dx%d outside the supported "
-                        "block sizes (4*ni + 2*nj <= 256, ni*nj <= 1024)", g + 1, ni, nj, nk);
+                        "block sizes (each <= %d, ni*nj <= 1024)", g + 1, ni, nj, nk, kMaxN1);
         if (h->level[g] < h->minlevel || h->level[g] > h->maxlevel)
             return fail(MGP_ERR_TOPOLOGY, "grid %d: level out of range", g + 1);
         if (g > 0 && h->level[g] < h->level[g - 1])
@@ -669,6 +848,10 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
         std::vector<uint32_t> cells;
         std::vector<int> pstart, slot;
         build_shapes(*h, cells, pstart, slot);
+        std::vector<uint2> items;
+        std::vector<uint32_t> ents;
+        std::vector<uint2> cellx;
+        build_plans(*h, cells, pstart, items, ents, cellx);
         build_conns(*h);
 
         // BP: per WO cell, own ghost layer per face, reference layout
@@ -733,6 +916,11 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
         upload(h->d_cells, cells);
         upload(h->d_pstart, pstart);
         upload(h->d_slot, slot);
+        upload(h->d_items, items);
+        upload(h->d_ents, ents);
+        upload(h->d_cellx, cellx);
+        h->d_zero.alloc(kZeroBytes);
+        HC(hipMemsetAsync(h->d_zero.p, 0, kZeroBytes, h->stream));
         std::vector<int> all(n);
         for (int g = 0; g < n; ++g) all[g] = g;
         upload(h->d_blist_all, all);
@@ -1514,6 +1702,7 @@ int mgp_info(mgp_handle *h, int64_t *ncells, int64_t *device_bytes) {
     if (device_bytes) {
         size_t tot = 0;
         const DBuf *bs[] = {&h->d_blocks, &h->d_shapes, &h->d_cells, &h->d_pstart, &h->d_slot,
+                            &h->d_items, &h->d_ents, &h->d_zero, &h->d_cellx,
                             &h->d_ap64, &h->d_apT, &h->d_hilf, &h->d_rhs64, &h->d_rhsT, &h->d_res,
                             &h->d_dp64, &h->d_bpw, &h->d_bpf, &h->d_bpsp, &h->d_hface, &h->d_rhi[0],
                             &h->d_rhi[1], &h->d_rlo, &h->d_rapT, &h->d_hface2, &h->d_ft[0], &h->d_ft[1], &h->d_ft[2],

@@ -19,7 +19,12 @@ struct Topo {
     int max_ni, max_nj, max_nk;
     int max_plane;         // largest hyperplane over all shapes
     int maxlevel;
+    const uint2 *items;    // plan items of all shapes, widths and kinds
+    const uint32_t *ents;  // plan entries
+    const uint2 *cellx;    // cell words of the plane kernels (per WO slot)
+    const void *zero;      // kZeroBytes of zeros: source of absent ghosts
 };
+constexpr size_t kZeroBytes = 65536 * sizeof(double);
 
 // Field set of one precision T (double or float) in WO layout.
 template <typename T>
@@ -100,6 +105,11 @@ template <typename T>
 void launch_sor(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
                 T *fout, double omega, int hilf_zero, const int *done,
                 hipStream_t s);
+
+// pipeline shape of the plane kernels (1: 4 items of 64 cells in flight,
+// 2: 2 items of 128 cells, 3: 3 items of 128 cells)
+void set_plan_shape(int v);
+int plan_shape();
 
 template <typename T>
 void launch_outer(const Topo &t, const Fields<T> &f, Mask m, Outer o,

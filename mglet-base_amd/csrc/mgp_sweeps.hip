@@ -1,0 +1,908 @@
+// mgp_sweeps.hip -- the plane kernels of the pressure iteration on gfx950:
+// the SIP forward sweep with the fused Laplacian, the SIP backward sweep,
+// red-black SOR and the FP64 defect update.
+//
+// One wave64 owns one block and walks its i+j+k hyperplanes (the
+// reference's dependency order, src/flow/sip_hyperplane_mod.F90:450-569).
+// A single wave needs no workgroup barrier between planes: its LDS
+// accesses are processed in program order.
+//
+// The work of a block is a host-built plan (mgp_api.cpp, build_plans): a
+// sequence of items, each one 64-lane chunk of a hyperplane to compute and
+// one 64-lane chunk of entries to move into LDS (window values of HILF
+// with their ghosts, recurrence ghosts from the neighbours' face buffers).
+// The engine keeps D items of global loads in flight: while item n is
+// computed, the loads of items n+1 .. n+D-1 are outstanding, and the entry
+// words of items n+D .. n+2D-1 as well.  Every item issues the same
+// number of loads (lanes beyond a chunk read a clamped, valid address), so
+// the compiler's wait counts stay exact across the unrolled loop.
+//
+// LDS per wave: source table, 1-D coefficient rows, the pencil arrays R
+// (two planes of the recurrence, indexed [i][j]) and the HILF window ring
+// H (4 planes, 8 for SOR, indexed [i+1][j+1], with BP bytes when masked).
+// All arithmetic follows the reference's association order (the build
+// uses -ffp-contract=off), so the FP64 path is bitwise equal to the oracle.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "mgp_device.h"
+#include "mgp_launch.h"
+
+namespace mgp {
+
+constexpr int kWave = 64;
+
+// Loads at a 32-bit element offset from a uniform base: the byte offset
+// stays an unsigned 32-bit value, so the load takes the base from SGPRs.
+template <typename T>
+__device__ __forceinline__ T ld(const T *base, int s) {
+    return *(const T *)((const char *)base + (uint32_t)s * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st(T *base, int s, T v) {
+    *(T *)((char *)base + (uint32_t)s * (uint32_t)sizeof(T)) = v;
+}
+
+// Cell word of the plane kernels (per shape and WO cell, built on the
+// host): x = i | j << 6 | k << 12 | fz << 18, y = fx | fy << 12 |
+// flags << 24, with fx, fy, fz the cell's position in the x, y and z face
+// buffers and flags bit 0..5 = (i == 0, i == NI-1, j == 0, j == NJ-1,
+// k == 0, k == NK-1).
+__device__ __forceinline__ void cell_ijk(uint32_t w, int &i, int &j, int &k) {
+    i = w & 63;
+    j = (w >> 6) & 63;
+    k = (w >> 12) & 63;
+}
+
+// Load through a pointer read from LDS: marks it as global memory (a
+// generic pointer would become a flat load, which the compiler can only
+// wait for by draining every outstanding load).
+template <typename T>
+__device__ __forceinline__ T gld(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+
+// Per-wave table of entry sources, by entry kind.  Absent sources point at
+// the zero buffer, so every entry load is valid and yields 0 there.
+template <typename T>
+struct SrcTab {
+    const T *h[8];        // window (target 0) values
+    const T *r[8];        // recurrence buffer (target 1) values
+    const uint8_t *b[8];  // window BP
+};
+
+// res = -(A phi) at one cell, src/flow/laplacephi_mod.F90:149-162, with
+// phi and BP read from the window planes p-1, p, p+1.
+template <typename T, typename A, bool MASKED>
+__device__ __forceinline__ A laplace_cell(const T *Hm, const T *H0,
+                                          const T *Hp, const uint8_t *Bm,
+                                          const uint8_t *B0,
+                                          const uint8_t *Bp, int c0, int HW,
+                                          A aw, A ae, A as, A an, A ab, A at,
+                                          A ap) {
+    const A pw = (A)Hm[c0 - HW], pe = (A)Hp[c0 + HW];
+    const A psv = (A)Hm[c0 - 1], pn = (A)Hp[c0 + 1];
+    const A pb = (A)Hm[c0], pt = (A)Hp[c0], pc = (A)H0[c0];
+    A r;
+    if (MASKED) {
+        const A bc = (A)B0[c0];
+        const A bw = (A)Bm[c0 - HW], be = (A)Bp[c0 + HW];
+        const A bs = (A)Bm[c0 - 1], bn = (A)Bp[c0 + 1];
+        const A bb = (A)Bm[c0], bt = (A)Bp[c0];
+        r = -(aw * pw * bw * bc);
+        r = r - ae * pe * bc * be;
+        r = r - as * psv * bs * bc;
+        r = r - an * pn * bc * bn;
+        r = r - ab * pb * bb * bc;
+        r = r - at * pt * bc * bt;
+    } else {
+        r = -(aw * pw);
+        r = r - ae * pe;
+        r = r - as * psv;
+        r = r - an * pn;
+        r = r - ab * pb;
+        r = r - at * pt;
+    }
+    r = r - ap * pc;
+    return r;
+}
+
+// 1-D coefficient rows of a block in LDS: [w e s n b t][kMaxN1]
+template <typename C>
+__device__ __forceinline__ void coef_rows(C *c, const C *aw, const C *ae,
+                                          const C *as, const C *an,
+                                          const C *ab, const C *at,
+                                          const Block &B) {
+    for (int e = threadIdx.x; e < 6 * kMaxN1; e += kWave) {
+        const int r = e / kMaxN1, q = e % kMaxN1;
+        const int n = r < 2 ? B.ni : (r < 4 ? B.nj : B.nk);
+        if (q >= n) continue;
+        const C *src = r == 0 ? aw + B.o1x : r == 1 ? ae + B.o1x
+                     : r == 2 ? as + B.o1y : r == 3 ? an + B.o1y
+                     : r == 4 ? ab + B.o1z : at + B.o1z;
+        c[e] = src[q + 2];
+    }
+}
+
+// Window sources of HILF for the 6 faces: the neighbour's boundary plane
+// (CON), the block's own PAR ghosts from bound_pressure, or none.
+template <typename T>
+__device__ __forceinline__ const T *window_face(const T *hface, const T *hpar,
+                                                const Block *blocks,
+                                                const Block &B, int f) {
+    const T *g = nbr_face(hface, blocks, B.nbr[f], f ^ 1);
+    if (!g && ((B.par >> f) & 1))
+        g = hpar + B.fc + face_base(f, B.ni, B.nj, B.nk);
+    return g;
+}
+
+// Fills the source table: lanes 0-7 window values, 8-15 recurrence
+// values, 16-23 window BP.
+template <typename T>
+__device__ __forceinline__ void fill_tab(SrcTab<T> *tab, const Topo &t,
+                                         const Block &B, const T *hface,
+                                         const T *hpar, const T *hilf,
+                                         const T *rsrc, int rmask, Mask m) {
+    const T *zero = (const T *)t.zero;
+    const int l = threadIdx.x;
+    if (l < 8) {
+        const T *g = nullptr;
+        if (l < 6 && hface) g = window_face(hface, hpar, t.blocks, B, l);
+        if (l == 6) g = hilf + B.wo;
+        tab->h[l] = g ? g : zero;
+    } else if (l < 16) {
+        const int f = l - 8;
+        const T *g = nullptr;
+        if (f < 6 && rsrc && ((rmask >> f) & 1))
+            g = nbr_face(rsrc, t.blocks, B.nbr[f], f ^ 1);
+        tab->r[f] = g ? g : zero;
+    } else if (l < 24) {
+        const int f = l - 16;
+        const uint8_t *g = nullptr;
+        if (f < 6 && m.bpf) g = m.bpf + B.fc + face_base(f, B.ni, B.nj, B.nk);
+        if (f == 6 && m.bp) g = m.bp + B.wo;
+        tab->b[f] = g ? g : (const uint8_t *)t.zero;
+    }
+}
+
+// The plan engine: runs the items of one block through functor K.  An
+// item covers up to 64 M cells (M per lane) and 64 M entries.  K provides
+// Data, load(Data&, s), compute(const Data&, s, p) and
+// store(entry, value, bp, p).
+//
+// Pipeline (W = 2D items per loop trip, every ring slot named statically,
+// so no register holding a load in flight is ever copied -- a copy would
+// make the wave wait for that load):
+//   * descriptor of item q: vector load at issue(q - 2D), ring iv[q mod W];
+//   * entry words of item q: loaded at issue(q - D), ring ew[q mod W];
+//   * data and entry values of item q: loaded at issue(q), ring q mod D,
+//     consumed by compute(q), which runs D items later;
+//   * the LDS source pointers of item q's entries are looked up at the end
+//     of issue(q - 1).
+// All loads go through the in-order vector memory counter, so each item
+// waits only for its own loads.  The loop starts W items early with no-op
+// items instead of a prologue, so it is entered with no load outstanding
+// and the compiler's wait counts at the loop head are those of the steady
+// state.  Indices outside the plan are clamped to its closing no-op item.
+template <int D, int M, bool EBP, typename T, class K>
+__device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
+                                         int nitem,
+                                         const uint32_t *__restrict__ ents,
+                                         const SrcTab<T> *tab) {
+    constexpr int W = 2 * D;
+    const int lane = threadIdx.x;
+    const int last = nitem - 1;
+    typename K::Data cd[D][M];
+    uint32_t ew[W][M];
+    T ev[D][M];
+    uint8_t eb[D][M];
+    uint2 iv[W];
+    uint32_t ix[D], iy[D];
+    const T *pn[M];
+    const uint8_t *bn[M];
+    auto ldi = [&](int q) -> uint2 { return ld(items, (q < 0 || q > last) ? last : q); };
+    auto lookup = [&](int m, uint32_t e) {
+        const int kind = (e >> 12) & 7;
+        pn[m] = (e & 0x8000u) ? tab->r[kind] : tab->h[kind];
+        if (EBP) bn[m] = tab->b[kind];
+    };
+    // issue(q), q = q0 + w + D: r = q mod W, u = q mod D
+    auto issue = [&](int u, int r, int q) {
+        const uint32_t x = __builtin_amdgcn_readfirstlane(iv[r].x);
+        const uint32_t y = __builtin_amdgcn_readfirstlane(iv[r].y);
+        ix[u] = x;
+        iy[u] = y;
+        const int cpos = x & 0xffff, cc = (x >> 16) & 0xff;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int l = lane + 64 * m;
+            k.load(cd[u][m], cpos + (l < cc ? l : (cc > 0 ? cc - 1 : 0)));
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int off = (int)(ew[r][m] >> 16);
+            ev[u][m] = gld(pn[m] + off);
+            if (EBP) eb[u][m] = gld(bn[m] + off);
+        }
+        const int r2 = (r + D) % W;
+        const uint32_t x2 = __builtin_amdgcn_readfirstlane(iv[r2].x);
+        const uint32_t y2 = __builtin_amdgcn_readfirstlane(iv[r2].y);
+        const int ec2 = x2 >> 24, ep2 = y2 & 0x1fffff;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int l = lane + 64 * m;
+            ew[r2][m] = ld(ents, ep2 + (l < ec2 ? l : (ec2 > 0 ? ec2 - 1 : 0)));
+        }
+        iv[r] = ldi(q + W);
+        const int r1 = (r + 1) % W;
+#pragma unroll
+        for (int m = 0; m < M; ++m) lookup(m, ew[r1][m]);
+    };
+#pragma unroll
+    for (int r = 0; r < W; ++r) {
+        uint2 v = ldi(r < D ? r : -1);
+        v.x = __builtin_amdgcn_readfirstlane(v.x);  // completes the load here
+        v.y = __builtin_amdgcn_readfirstlane(v.y);
+        iv[r] = v;
+#pragma unroll
+        for (int m = 0; m < M; ++m) ew[r][m] = 0u;  // entry word 0: a valid address
+    }
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+        ix[u] = 0u;
+        iy[u] = 8u << 21;
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m) lookup(m, 0u);
+    for (int q0 = -W; q0 < nitem; q0 += W) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const int u = w % D;
+            const uint32_t x = ix[u], y = iy[u];
+            const int cpos = x & 0xffff, cc = (x >> 16) & 0xff, ec = x >> 24;
+            const int p = (int)(y >> 21) - 8;
+#pragma unroll
+            for (int m = 0; m < M; ++m)
+                if (lane + 64 * m < cc) k.compute(cd[u][m], cpos + lane + 64 * m, p);
+#pragma unroll
+            for (int m = 0; m < M; ++m)
+                if (lane + 64 * m < ec)
+                    k.store(ew[w][m], ev[u][m], EBP ? eb[u][m] : (uint8_t)0, p);
+            wave_sync();
+            issue(u, (w + D) % W, q0 + w + D);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// Fused inner-iteration first half: RES = -(A HILF) (laplacephi_level),
+// then sipiter1 (src/flow/sip_hyperplane_mod.F90:450-505, classic order
+// src/flow/sip_classic_mod.F90:143-166).  Low-side ghosts of RES are the
+// neighbours' upper planes of the previous exchange (rhi[rd]); the upper
+// planes of this sweep go to rhi[1-rd] unless this is the last inner
+// iteration (the reference's conn(..., forward=-1) then skips faces
+// 1/3/5), the lower planes always go to rlo for the backward sweep.
+// LAP = false is the first inner iteration of an outer iteration, where
+// HILF = 0 and the Laplacian term vanishes.
+template <typename T, bool CLASSIC, bool MASKED, bool LAP>
+struct FwdK {
+    struct Data {
+        uint2 c;
+        T rhs, lpr, lb, ls, lw, ap;
+    };
+    const uint2 *cells;
+    const T *rhs, *lpr, *lb, *ls, *lw, *ap;
+    T *res, *rlo, *rw;
+    const T *cf;
+    T *R, *H;
+    uint8_t *BH;
+    int NI, NJ, NK, HW, RW, RS, mHS;
+    int fb0, fb1, fb2, fb3, fb4, fb5;
+    bool masked;
+
+    __device__ __forceinline__ void load(Data &d, int s) const {
+        d.c = ld(cells, s);
+        d.rhs = ld(rhs, s);
+        d.lpr = ld(lpr, s);
+        d.lb = ld(lb, s);
+        d.ls = ld(ls, s);
+        d.lw = ld(lw, s);
+        if (LAP) d.ap = ld(ap, s);
+    }
+    __device__ __forceinline__ void compute(const Data &d, int s, int p) const {
+        int i, j, k;
+        cell_ijk(d.c.x, i, j, k);
+        T r = T(0);
+        if (LAP) {
+            const int c0 = (j + 1) + HW * (i + 1);
+            const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS,
+                      sp = ((p + 2) & 3) * mHS;
+            const T *aw = cf, *ae = cf + kMaxN1, *as = cf + 2 * kMaxN1,
+                    *an = cf + 3 * kMaxN1, *ab = cf + 4 * kMaxN1,
+                    *at = cf + 5 * kMaxN1;
+            if (MASKED && masked)
+                r = laplace_cell<T, T, true>(H + sm, H + s0, H + sp, BH + sm, BH + s0,
+                                             BH + sp, c0, HW, aw[i], ae[i], as[j], an[j],
+                                             ab[k], at[k], d.ap);
+            else
+                r = laplace_cell<T, T, false>(H + sm, H + s0, H + sp, nullptr, nullptr,
+                                              nullptr, c0, HW, aw[i], ae[i], as[j], an[j],
+                                              ab[k], at[k], d.ap);
+        }
+        const int c1 = (j + 1) + RW * (i + 1);
+        const T *Rp = R + ((p - 1) & 1) * RS;
+        T *Rc = R + (p & 1) * RS;
+        r = (d.rhs + r) * d.lpr;
+        if (CLASSIC) {
+            r = r - d.lw * Rp[c1 - RW];
+            r = r - d.ls * Rp[c1 - 1];
+            r = r - d.lb * Rp[c1];
+        } else {
+            r = r - d.lb * Rp[c1];
+            r = r - d.ls * Rp[c1 - 1];
+            r = r - d.lw * Rp[c1 - RW];
+        }
+        Rc[c1] = r;
+        st(res, s, r);
+        const uint32_t fl = d.c.y >> 24;
+        if (fl) {
+            const int fx = d.c.y & 4095, fy = (d.c.y >> 12) & 4095, fz = d.c.x >> 18;
+            if (fl & 1) st(rlo, fb0 + fx, r);
+            if (fl & 4) st(rlo, fb2 + fy, r);
+            if (fl & 16) st(rlo, fb4 + fz, r);
+            if (rw) {
+                if (fl & 2) st(rw, fb1 + fx, r);
+                if (fl & 8) st(rw, fb3 + fy, r);
+                if (fl & 32) st(rw, fb5 + fz, r);
+            }
+        }
+    }
+    __device__ __forceinline__ void store(uint32_t e, T v, uint8_t b, int p) const {
+        const int x = e & 0xfff;
+        if (e & 0x8000u) {
+            R[(p & 1) * RS + x] = v;
+        } else {
+            const int sl = ((p + 3) & 3) * mHS;
+            H[sl + x] = v;
+            if (MASKED && masked) BH[sl + x] = b;
+        }
+    }
+};
+
+template <typename T, bool CLASSIC, bool MASKED, bool LAP, int D, int M>
+__global__ __launch_bounds__(kWave) void k_forward(
+    const uint2 *__restrict__ items, Topo t, Fields<T> F, Mask m, int rd,
+    int write_hi, const int *done) {
+    if (*done) return;
+    extern __shared__ __align__(16) unsigned char lds_fw[];
+    const int b = t.blist[blockIdx.x];
+    const Block &B = t.blocks[b];
+    const Shape &S = t.shapes[B.shape];
+    const int mRS = (t.max_nj + 1) * (t.max_ni + 1);
+    const int mHS = (t.max_nj + 2) * (t.max_ni + 2);
+    SrcTab<T> *tab = (SrcTab<T> *)lds_fw;
+    T *cf = (T *)(lds_fw + sizeof(SrcTab<T>));
+    T *R = cf + 6 * kMaxN1;
+    T *H = R + 2 * mRS;
+    uint8_t *BH = (uint8_t *)(H + 4 * mHS);
+    // R sources: the neighbours' upper planes feed faces 0, 2, 4
+    fill_tab<T>(tab, t, B, LAP ? F.hface : nullptr, F.hpar, F.hilf, F.rhi[rd],
+                0x15, m);
+    if (LAP) coef_rows<T>(cf, F.aw, F.ae, F.as, F.an, F.ab, F.at, B);
+    wave_sync();
+    const int NI = B.ni, NJ = B.nj, NK = B.nk;
+    const int64_t wo = B.wo, fc = B.fc;
+    FwdK<T, CLASSIC, MASKED, LAP> k;
+    k.cells = t.cellx + S.cell_off;
+    k.rhs = F.rhs + wo;
+    k.lw = F.f[0] + wo;
+    k.ls = F.f[1] + wo;
+    k.lb = F.f[2] + wo;
+    k.lpr = F.f[3] + wo;
+    k.ap = F.ap + wo;
+    k.res = F.res + wo;
+    k.rlo = F.rlo + fc;
+    k.rw = write_hi ? F.rhi[rd ^ 1] + fc : nullptr;
+    k.cf = cf;
+    k.R = R;
+    k.H = H;
+    k.BH = BH;
+    k.NI = NI;
+    k.NJ = NJ;
+    k.NK = NK;
+    k.HW = NJ + 2;
+    k.RW = NJ + 1;
+    k.RS = (NJ + 1) * (NI + 1);
+    k.mHS = mHS;
+    k.fb0 = face_base(0, NI, NJ, NK);
+    k.fb1 = face_base(1, NI, NJ, NK);
+    k.fb2 = face_base(2, NI, NJ, NK);
+    k.fb3 = face_base(3, NI, NJ, NK);
+    k.fb4 = face_base(4, NI, NJ, NK);
+    k.fb5 = face_base(5, NI, NJ, NK);
+    k.masked = MASKED && !(B.flags & 1);
+    const int kind = LAP ? kPlanFwdLap : kPlanFwdZero;
+    run_plan<D, M, MASKED && LAP, T>(k, items + S.plan_item[M - 1][kind],
+                                     S.plan_nitem[M - 1][kind], t.ents + S.plan_ent[kind], tab);
+}
+
+// ---------------------------------------------------------------------
+// Inner-iteration second half: sipiter2 (src/flow/sip_hyperplane_mod.F90:
+// 508-569; classic order src/flow/sip_classic_mod.F90:169-198), then
+// HILF += RES.  Upper ghosts of RES are the neighbours' lower planes of
+// this iteration's forward sweep.  Also writes HILF's boundary planes,
+// which stand in for the reference's conn(ilevel, 1, s1=dp) (:414).
+// HZ: HILF is zero on entry (first inner iteration of an outer iteration).
+template <typename T, bool CLASSIC, bool HZ>
+struct BwdK {
+    struct Data {
+        uint2 c;
+        T res, ue, un, ut, h;
+    };
+    const uint2 *cells;
+    const T *res, *ue, *un, *ut;
+    T *hilf, *hface;
+    T *R;
+    int NI, NJ, NK, RW, RS;
+    int fb0, fb1, fb2, fb3, fb4, fb5;
+
+    __device__ __forceinline__ void load(Data &d, int s) const {
+        d.c = ld(cells, s);
+        d.res = ld(res, s);
+        d.ue = ld(ue, s);
+        d.un = ld(un, s);
+        d.ut = ld(ut, s);
+        if (!HZ) d.h = ld((const T *)hilf, s);
+    }
+    __device__ __forceinline__ void compute(const Data &d, int s, int p) const {
+        int i, j, k;
+        cell_ijk(d.c.x, i, j, k);
+        const int c1 = j + RW * i;
+        const T *Rp = R + ((p + 1) & 1) * RS;
+        T *Rc = R + (p & 1) * RS;
+        T r = d.res;
+        if (CLASSIC) {
+            r = r - d.un * Rp[c1 + 1];
+            r = r - d.ue * Rp[c1 + RW];
+            r = r - d.ut * Rp[c1];
+        } else {
+            r = r - d.ut * Rp[c1];
+            r = r - d.un * Rp[c1 + 1];
+            r = r - d.ue * Rp[c1 + RW];
+        }
+        Rc[c1] = r;
+        const T h = (HZ ? T(0) : d.h) + r;
+        st(hilf, s, h);
+        const uint32_t fl = d.c.y >> 24;
+        if (fl) {
+            const int fx = d.c.y & 4095, fy = (d.c.y >> 12) & 4095, fz = d.c.x >> 18;
+            if (fl & 1) st(hface, fb0 + fx, h);
+            if (fl & 2) st(hface, fb1 + fx, h);
+            if (fl & 4) st(hface, fb2 + fy, h);
+            if (fl & 8) st(hface, fb3 + fy, h);
+            if (fl & 16) st(hface, fb4 + fz, h);
+            if (fl & 32) st(hface, fb5 + fz, h);
+        }
+    }
+    __device__ __forceinline__ void store(uint32_t e, T v, uint8_t, int p) const {
+        R[(p & 1) * RS + (e & 0xfff)] = v;
+    }
+};
+
+template <typename T, bool CLASSIC, bool HZ, int D, int M>
+__global__ __launch_bounds__(kWave) void k_backward(
+    const uint2 *__restrict__ items, Topo t, Fields<T> F, const int *done) {
+    if (*done) return;
+    extern __shared__ __align__(16) unsigned char lds_bw[];
+    const int b = t.blist[blockIdx.x];
+    const Block &B = t.blocks[b];
+    const Shape &S = t.shapes[B.shape];
+    SrcTab<T> *tab = (SrcTab<T> *)lds_bw;
+    T *R = (T *)(lds_bw + sizeof(SrcTab<T>));
+    // R sources: the neighbours' lower planes feed faces 1, 3, 5
+    fill_tab<T>(tab, t, B, nullptr, nullptr, F.hilf, F.rlo, 0x2a, Mask{});
+    wave_sync();
+    const int NI = B.ni, NJ = B.nj, NK = B.nk;
+    const int64_t wo = B.wo;
+    BwdK<T, CLASSIC, HZ> k;
+    k.cells = t.cellx + S.cell_off;
+    k.res = F.res + wo;
+    k.ue = F.f[4] + wo;
+    k.un = F.f[5] + wo;
+    k.ut = F.f[6] + wo;
+    k.hilf = F.hilf + wo;
+    k.hface = F.hface + B.fc;
+    k.R = R;
+    k.NI = NI;
+    k.NJ = NJ;
+    k.NK = NK;
+    k.RW = NJ + 1;
+    k.RS = (NJ + 1) * (NI + 1);
+    k.fb0 = face_base(0, NI, NJ, NK);
+    k.fb1 = face_base(1, NI, NJ, NK);
+    k.fb2 = face_base(2, NI, NJ, NK);
+    k.fb3 = face_base(3, NI, NJ, NK);
+    k.fb4 = face_base(4, NI, NJ, NK);
+    k.fb5 = face_base(5, NI, NJ, NK);
+    run_plan<D, M, false, T>(k, items + S.plan_item[M - 1][kPlanBwd],
+                             S.plan_nitem[M - 1][kPlanBwd], t.ents + S.plan_ent[kPlanBwd], tab);
+}
+
+// ---------------------------------------------------------------------
+// Red-black SOR of HILF (relax, src/flow/sor_mod.F90:102-203), both colours
+// in one pass over the block's hyperplanes.  The reference's first colour
+// (rb = 0: i+j+k even in its 1-based indices, :124-130) is exactly the set
+// of odd planes p = i+j+k of the 0-based interior, the second colour the
+// even planes, and every 7-point neighbour of a cell lies on plane p-1 or
+// p+1.  Step p relaxes plane p when p is odd (its neighbours are old even
+// planes) and plane p-2 when p is even (its neighbours p-3, p-1 were
+// relaxed at earlier steps), so each value is read and written once; the
+// window ring holds 8 planes for that.  Ghost values stay those of the
+// previous exchange (fin) for both colours, as in the reference, where
+// conn(DP) follows relax (:414); the new boundary planes go to fout.  With
+// an odd cell count along k the loop bounds of :124-130 never reach the
+// cells k = nk-1; they keep their value.
+// HZ: HILF is zero on entry (first inner iteration of an outer iteration).
+template <typename T, bool MASKED, bool HZ>
+struct SorK {
+    struct Data {
+        uint2 c;
+        T rhs, rap;
+    };
+    const uint2 *cells;
+    const T *rhs, *rap;
+    T *hilf, *fout;
+    const T *cf;
+    T *H;
+    uint8_t *BH;
+    int NI, NJ, NK, HW, mHS;
+    int fb0, fb1, fb2, fb3, fb4, fb5;
+    T om, om1;
+    bool masked, keep_top;
+
+    __device__ __forceinline__ void load(Data &d, int s) const {
+        d.c = ld(cells, s);
+        d.rhs = ld(rhs, s);
+        d.rap = ld(rap, s);
+    }
+    __device__ __forceinline__ void compute(const Data &d, int s, int p) const {
+        const int u = (p & 1) ? p : p - 2;
+        int i, j, k;
+        cell_ijk(d.c.x, i, j, k);
+        const int c0 = (j + 1) + HW * (i + 1);
+        T *H0 = H + ((u + 1) & 7) * mHS;
+        const T *Hm = H + (u & 7) * mHS, *Hp = H + ((u + 2) & 7) * mHS;
+        const T old = H0[c0];
+        T nv = old;
+        if (!(keep_top && k == NK - 1)) {
+            const T *aw = cf, *ae = cf + kMaxN1, *as = cf + 2 * kMaxN1,
+                    *an = cf + 3 * kMaxN1, *ab = cf + 4 * kMaxN1,
+                    *at = cf + 5 * kMaxN1;
+            const T pw = Hm[c0 - HW], pe = Hp[c0 + HW];
+            const T psv = Hm[c0 - 1], pn = Hp[c0 + 1];
+            const T pb = Hm[c0], pt = Hp[c0];
+            T r;
+            if (MASKED && masked) {
+                const uint8_t *B0 = BH + ((u + 1) & 7) * mHS;
+                const uint8_t *Bm = BH + (u & 7) * mHS, *Bp = BH + ((u + 2) & 7) * mHS;
+                const T bc = (T)B0[c0];
+                const T cw = (T)Bm[c0 - HW] * bc * aw[i];
+                const T ce = bc * (T)Bp[c0 + HW] * ae[i];
+                const T cs = (T)Bm[c0 - 1] * bc * as[j];
+                const T cn = bc * (T)Bp[c0 + 1] * an[j];
+                const T cb = (T)Bm[c0] * bc * ab[k];
+                const T ct = bc * (T)Bp[c0] * at[k];
+                r = cw * pw + ce * pe + cs * psv + cn * pn + cb * pb + ct * pt;
+            } else {
+                r = aw[i] * pw + ae[i] * pe + as[j] * psv + an[j] * pn +
+                    ab[k] * pb + at[k] * pt;
+            }
+            r = (r - d.rhs) * d.rap;
+            nv = om1 * old - om * r;
+        }
+        H0[c0] = nv;
+        st(hilf, s, nv);
+        const uint32_t fl = d.c.y >> 24;
+        if (fl) {
+            const int fx = d.c.y & 4095, fy = (d.c.y >> 12) & 4095, fz = d.c.x >> 18;
+            if (fl & 1) st(fout, fb0 + fx, nv);
+            if (fl & 2) st(fout, fb1 + fx, nv);
+            if (fl & 4) st(fout, fb2 + fy, nv);
+            if (fl & 8) st(fout, fb3 + fy, nv);
+            if (fl & 16) st(fout, fb4 + fz, nv);
+            if (fl & 32) st(fout, fb5 + fz, nv);
+        }
+    }
+    __device__ __forceinline__ void store(uint32_t e, T v, uint8_t b, int p) const {
+        const int x = e & 0xfff;
+        const int sl = ((p + 3) & 7) * mHS;
+        H[sl + x] = HZ ? T(0) : v;
+        if (MASKED && masked) BH[sl + x] = b;
+    }
+};
+
+template <typename T, bool MASKED, bool HZ, int D, int M>
+__global__ __launch_bounds__(kWave) void k_sor(
+    const uint2 *__restrict__ items, Topo t, Fields<T> F, Mask m, const T *fin,
+    T *fout, double omega, const int *done) {
+    if (*done) return;
+    extern __shared__ __align__(16) unsigned char lds_so[];
+    const int b = t.blist[blockIdx.x];
+    const Block &B = t.blocks[b];
+    const Shape &S = t.shapes[B.shape];
+    const int mHS = (t.max_nj + 2) * (t.max_ni + 2);
+    SrcTab<T> *tab = (SrcTab<T> *)lds_so;
+    T *cf = (T *)(lds_so + sizeof(SrcTab<T>));
+    T *H = cf + 6 * kMaxN1;
+    uint8_t *BH = (uint8_t *)(H + 8 * mHS);
+    fill_tab<T>(tab, t, B, fin, F.hpar, F.hilf, nullptr, 0, m);
+    coef_rows<T>(cf, F.aw, F.ae, F.as, F.an, F.ab, F.at, B);
+    wave_sync();
+    const int NI = B.ni, NJ = B.nj, NK = B.nk;
+    const int64_t wo = B.wo;
+    SorK<T, MASKED, HZ> k;
+    k.cells = t.cellx + S.cell_off;
+    k.rhs = F.rhs + wo;
+    k.rap = F.rap + wo;
+    k.hilf = F.hilf + wo;
+    k.fout = fout + B.fc;
+    k.cf = cf;
+    k.H = H;
+    k.BH = BH;
+    k.NI = NI;
+    k.NJ = NJ;
+    k.NK = NK;
+    k.HW = NJ + 2;
+    k.mHS = mHS;
+    k.fb0 = face_base(0, NI, NJ, NK);
+    k.fb1 = face_base(1, NI, NJ, NK);
+    k.fb2 = face_base(2, NI, NJ, NK);
+    k.fb3 = face_base(3, NI, NJ, NK);
+    k.fb4 = face_base(4, NI, NJ, NK);
+    k.fb5 = face_base(5, NI, NJ, NK);
+    k.om = (T)omega;
+    k.om1 = (T)(1.0 - omega);
+    k.masked = MASKED && !(B.flags & 1);
+    k.keep_top = (NK & 1) != 0;
+    run_plan<D, M, MASKED, T>(k, items + S.plan_item[M - 1][kPlanSor],
+                              S.plan_nitem[M - 1][kPlanSor], t.ents + S.plan_ent[kPlanSor], tab);
+}
+
+// ---------------------------------------------------------------------
+// FP64 outer update (src/flow/pressuresolver_mod.F90:258-279): RES =
+// -(A HILF) with the FP64 operator, RHS += RES, max |RHS|, DP += HILF.
+// In the mixed-precision mode the sweeps' RHS copy is refreshed here.
+template <typename T, bool MASKED>
+struct OutK {
+    struct Data {
+        uint32_t c;
+        double ap, rhs, dp;
+    };
+    const uint32_t *cells;   // word 0 of the cell words (stride 2)
+    const double *ap;
+    double *rhs, *dp;
+    T *rhs_t;
+    const double *cf;
+    T *H;
+    uint8_t *BH;
+    int HW, mHS;
+    bool masked;
+    double mx;
+
+    __device__ __forceinline__ void load(Data &d, int s) const {
+        d.c = ld(cells, 2 * s);
+        d.ap = ld(ap, s);
+        d.rhs = ld((const double *)rhs, s);
+        d.dp = ld((const double *)dp, s);
+    }
+    __device__ __forceinline__ void compute(const Data &d, int s, int p) {
+        int i, j, k;
+        cell_ijk(d.c, i, j, k);
+        const int c0 = (j + 1) + HW * (i + 1);
+        const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS, sp = ((p + 2) & 3) * mHS;
+        const double *aw = cf, *ae = cf + kMaxN1, *as = cf + 2 * kMaxN1,
+                     *an = cf + 3 * kMaxN1, *ab = cf + 4 * kMaxN1,
+                     *at = cf + 5 * kMaxN1;
+        double r;
+        if (MASKED && masked)
+            r = laplace_cell<T, double, true>(H + sm, H + s0, H + sp, BH + sm, BH + s0,
+                                              BH + sp, c0, HW, aw[i], ae[i], as[j], an[j],
+                                              ab[k], at[k], d.ap);
+        else
+            r = laplace_cell<T, double, false>(H + sm, H + s0, H + sp, nullptr, nullptr,
+                                               nullptr, c0, HW, aw[i], ae[i], as[j], an[j],
+                                               ab[k], at[k], d.ap);
+        const double rn = d.rhs + r;
+        st(rhs, s, rn);
+        if (rhs_t) st(rhs_t, s, (T)rn);
+        const double a = fabs(rn);
+        mx = (a > mx || a != a) ? a : mx;
+        st(dp, s, d.dp + (double)H[s0 + c0]);
+    }
+    __device__ __forceinline__ void store(uint32_t e, T v, uint8_t b, int p) const {
+        const int x = e & 0xfff;
+        const int sl = ((p + 3) & 3) * mHS;
+        H[sl + x] = v;
+        if (MASKED && masked) BH[sl + x] = b;
+    }
+};
+
+template <typename T, bool MASKED, int D, int M>
+__global__ __launch_bounds__(kWave) void k_outer(
+    const uint2 *__restrict__ items, Topo t, Fields<T> F, Mask m, Outer o,
+    T *rhs_t) {
+    if (*o.done) return;
+    extern __shared__ __align__(16) unsigned char lds_ou[];
+    const int b = t.blist[blockIdx.x];
+    const Block &B = t.blocks[b];
+    const Shape &S = t.shapes[B.shape];
+    const int mHS = (t.max_nj + 2) * (t.max_ni + 2);
+    SrcTab<T> *tab = (SrcTab<T> *)lds_ou;
+    double *cf = (double *)(lds_ou + sizeof(SrcTab<T>));
+    T *H = (T *)(cf + 6 * kMaxN1);
+    uint8_t *BH = (uint8_t *)(H + 4 * mHS);
+    fill_tab<T>(tab, t, B, F.hface, F.hpar, F.hilf, nullptr, 0, m);
+    coef_rows<double>(cf, o.aw, o.ae, o.as, o.an, o.ab, o.at, B);
+    wave_sync();
+    const int64_t wo = B.wo;
+    OutK<T, MASKED> k;
+    k.cells = (const uint32_t *)(t.cellx + S.cell_off);
+    k.ap = o.ap64 + wo;
+    k.rhs = o.rhs64 + wo;
+    k.dp = o.dp64 + wo;
+    k.rhs_t = rhs_t ? rhs_t + wo : nullptr;
+    k.cf = cf;
+    k.H = H;
+    k.BH = BH;
+    k.HW = B.nj + 2;
+    k.mHS = mHS;
+    k.masked = MASKED && !(B.flags & 1);
+    k.mx = 0.0;
+    run_plan<D, M, MASKED, T>(k, items + S.plan_item[M - 1][kPlanOut],
+                              S.plan_nitem[M - 1][kPlanOut], t.ents + S.plan_ent[kPlanOut], tab);
+    double mx = k.mx * ldexp(1.0, t.maxlevel - B.level);
+    for (int off = 32; off > 0; off >>= 1) {
+        const double x = __shfl_down(mx, off, kWave);
+        mx = (x > mx || mx != mx) ? x : mx;
+    }
+    if (threadIdx.x == 0) atomicMax(o.gmax, (unsigned long long)__double_as_longlong(mx));
+}
+
+// ---------------------------------------------------------------------
+// Launchers.  The pipeline shape is a tuning knob, set_plan_shape(): 1 =
+// 4 items of 64 cells in flight, 2 = 2 items of 128 cells (two cells per
+// lane, independent, so their latencies overlap), 3 = 3 items of 128.
+static int g_shape = 2;
+
+void set_plan_shape(int v) { g_shape = (v >= 1 && v <= 3) ? v : 2; }
+int plan_shape() { return g_shape; }
+
+template <typename T>
+static size_t tab_bytes() { return sizeof(SrcTab<T>); }
+
+template <typename T, bool CLASSIC, bool MASKED, bool LAP>
+static void fwd_launch(const Topo &t, const Fields<T> &f, Mask m, int rd,
+                       int write_hi, const int *done, size_t lds, hipStream_t s) {
+    switch (g_shape) {
+    case 1: k_forward<T, CLASSIC, MASKED, LAP, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    case 3: k_forward<T, CLASSIC, MASKED, LAP, 3, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    default: k_forward<T, CLASSIC, MASKED, LAP, 2, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    }
+}
+
+template <typename T>
+void launch_forward(const Topo &t, const Fields<T> &f, Mask m, int rd,
+                    int write_hi, int hilf_zero, int classic, const int *done,
+                    hipStream_t s) {
+    const size_t mRS = (size_t)(t.max_nj + 1) * (t.max_ni + 1);
+    const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
+    const bool masked = m.bp != nullptr;
+    const bool lap = !hilf_zero;
+    const size_t lds = tab_bytes<T>() + 6 * kMaxN1 * sizeof(T) + 2 * mRS * sizeof(T) +
+                       (lap ? 4 * mHS * sizeof(T) + (masked ? 4 * mHS : 0) : 0);
+#define MGP_FWD(C, M, L) fwd_launch<T, C, M, L>(t, f, m, rd, write_hi, done, lds, s)
+    if (classic) {
+        if (masked) { if (lap) MGP_FWD(true, true, true); else MGP_FWD(true, true, false); }
+        else { if (lap) MGP_FWD(true, false, true); else MGP_FWD(true, false, false); }
+    } else {
+        if (masked) { if (lap) MGP_FWD(false, true, true); else MGP_FWD(false, true, false); }
+        else { if (lap) MGP_FWD(false, false, true); else MGP_FWD(false, false, false); }
+    }
+#undef MGP_FWD
+}
+template void launch_forward<double>(const Topo &, const Fields<double> &,
+                                     Mask, int, int, int, int, const int *,
+                                     hipStream_t);
+template void launch_forward<float>(const Topo &, const Fields<float> &, Mask,
+                                    int, int, int, int, const int *,
+                                    hipStream_t);
+
+template <typename T, bool CLASSIC, bool HZ>
+static void bwd_launch(const Topo &t, const Fields<T> &f, const int *done,
+                       size_t lds, hipStream_t s) {
+    switch (g_shape) {
+    case 1: k_backward<T, CLASSIC, HZ, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
+    case 3: k_backward<T, CLASSIC, HZ, 3, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
+    default: k_backward<T, CLASSIC, HZ, 2, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
+    }
+}
+
+template <typename T>
+void launch_backward(const Topo &t, const Fields<T> &f, int hilf_zero,
+                     int classic, const int *done, hipStream_t s) {
+    const size_t mRS = (size_t)(t.max_nj + 1) * (t.max_ni + 1);
+    const size_t lds = tab_bytes<T>() + 2 * mRS * sizeof(T);
+    if (classic) {
+        if (hilf_zero) bwd_launch<T, true, true>(t, f, done, lds, s);
+        else bwd_launch<T, true, false>(t, f, done, lds, s);
+    } else {
+        if (hilf_zero) bwd_launch<T, false, true>(t, f, done, lds, s);
+        else bwd_launch<T, false, false>(t, f, done, lds, s);
+    }
+}
+template void launch_backward<double>(const Topo &, const Fields<double> &,
+                                      int, int, const int *, hipStream_t);
+template void launch_backward<float>(const Topo &, const Fields<float> &, int,
+                                     int, const int *, hipStream_t);
+
+template <typename T, bool MASKED, bool HZ>
+static void sor_launch(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
+                       T *fout, double omega, const int *done, size_t lds,
+                       hipStream_t s) {
+    switch (g_shape) {
+    case 1: k_sor<T, MASKED, HZ, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
+    case 3: k_sor<T, MASKED, HZ, 3, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
+    default: k_sor<T, MASKED, HZ, 2, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
+    }
+}
+
+template <typename T>
+void launch_sor(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
+                T *fout, double omega, int hilf_zero, const int *done,
+                hipStream_t s) {
+    const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
+    const bool masked = m.bp != nullptr;
+    const size_t lds = tab_bytes<T>() + 6 * kMaxN1 * sizeof(T) + 8 * mHS * sizeof(T) +
+                       (masked ? 8 * mHS : 0);
+    if (masked) {
+        if (hilf_zero) sor_launch<T, true, true>(t, f, m, fin, fout, omega, done, lds, s);
+        else sor_launch<T, true, false>(t, f, m, fin, fout, omega, done, lds, s);
+    } else {
+        if (hilf_zero) sor_launch<T, false, true>(t, f, m, fin, fout, omega, done, lds, s);
+        else sor_launch<T, false, false>(t, f, m, fin, fout, omega, done, lds, s);
+    }
+}
+template void launch_sor<double>(const Topo &, const Fields<double> &, Mask,
+                                 const double *, double *, double, int,
+                                 const int *, hipStream_t);
+template void launch_sor<float>(const Topo &, const Fields<float> &, Mask,
+                                const float *, float *, double, int,
+                                const int *, hipStream_t);
+
+template <typename T, bool MASKED>
+static void out_launch(const Topo &t, const Fields<T> &f, Mask m, Outer o,
+                       T *rhs_t, size_t lds, hipStream_t s) {
+    switch (g_shape) {
+    case 1: k_outer<T, MASKED, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    case 3: k_outer<T, MASKED, 3, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    default: k_outer<T, MASKED, 2, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    }
+}
+
+template <typename T>
+void launch_outer(const Topo &t, const Fields<T> &f, Mask m, Outer o,
+                  T *rhs_t, hipStream_t s) {
+    const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
+    const bool masked = m.bp != nullptr;
+    const size_t lds = tab_bytes<T>() + 6 * kMaxN1 * sizeof(double) +
+                       4 * mHS * sizeof(T) + (masked ? 4 * mHS : 0);
+    if (masked) out_launch<T, true>(t, f, m, o, rhs_t, lds, s);
+    else out_launch<T, false>(t, f, m, o, rhs_t, lds, s);
+}
+template void launch_outer<double>(const Topo &, const Fields<double> &, Mask,
+                                   Outer, double *, hipStream_t);
+template void launch_outer<float>(const Topo &, const Fields<float> &, Mask,
+                                  Outer, float *, hipStream_t);
+
+}  // namespace mgp
