@@ -12,7 +12,11 @@ the all-FP64 form is timed the same way and reported beside it.
 The line also carries:
   roofline     -- dominant kernel (forward sweep) timed with HIP events on
                   the library's stream; algorithmic bytes per launch from the
-                  per-cell figures in `ALG_BYTES` below;
+                  per-cell figures in `ALG_BYTES` below, against the 8 TB/s
+                  peak and against a measured device copy (torch copy_ of
+                  2 x 1.8 GB) as the practical streaming rate;
+  autotune     -- the pipeline shape the library's autotuner picked for the
+                  plane kernels, with its per-shape timings;
   cpu_baseline -- the oracle (C restatement of the reference CPU path) timed
                   on this host on a bounded sample of the same workload;
   tts          -- time to a 1e-10 relative residual (FP32-refined and FP64)
@@ -31,10 +35,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path[:0] = [str(ROOT), str(ROOT / "mglet-base_amd")]
 
 # algorithmic bytes per cell per launch (all-fluid blocks read no mask)
-#   forward sweep: HILF, RHS, AP, LW, LS, LB, LPR            (7 reads)
-#   backward sweep: UE, UN, UT reads, HILF write             (4 accesses)
-ALG_BYTES = {"fp32": {"forward": 28, "backward": 16},
-             "fp64": {"forward": 56, "backward": 32}}
+#   forward sweep: HILF, RHS, AP, LW, LS, LB, LPR reads, RES write   (8)
+#   backward sweep: RES, UE, UN, UT, HILF reads, HILF write          (6)
+ALG_BYTES = {"fp32": {"forward": 32, "backward": 24},
+             "fp64": {"forward": 64, "backward": 48}}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -251,7 +255,74 @@ def adjoint_config():
     print(json.dumps({"adjoint_projection": out}))
 
 
+def copy_bandwidth(nbytes=1 << 31, reps=5):
+    """Device copy rate in GB/s (bytes read + written) of torch's copy_ on
+    two float32 tensors of `nbytes` each: the practical streaming rate the
+    plane kernels are compared with."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    best = 1e30
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * n * 4 / (best * 1e-3) / 1e9
+
+
+def pmc_summary(fetch_dir, write_dir, out, cells=452984832):
+    """Bytes per launch from two rocprofv3 --pmc runs (FETCH_SIZE, then
+    WRITE_SIZE, both in KiB per dispatch), scaled by what the counters read
+    for the library's plain convert kernel k_convert<float> (reads `cells`
+    doubles, writes `cells` floats).  Writes the JSON the roofline line
+    reads; prints the table."""
+    import csv
+    import re
+
+    def load(d, counter):
+        rows = []
+        for f in Path(d).glob("*counter_collection.csv"):
+            rows += list(csv.DictReader(open(f)))
+        per = {}
+        for r in rows:
+            if r["Counter_Name"] != counter:
+                continue
+            name = re.sub(r"\(.*", "", r["Kernel_Name"]).strip()
+            per.setdefault(name, []).append(float(r["Counter_Value"]) * 1024.0)
+        return {k: sum(v) / len(v) for k, v in per.items()}
+
+    fe, wr = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    conv = "void mgp::k_convert<float>"
+    fs = 8.0 * cells / fe[conv]
+    ws = 4.0 * cells / wr[conv]
+    per = {}
+    for k in sorted(fe):
+        if "mgp::" in k and k in wr:
+            per[k] = {"fetch_bytes": fe[k] * fs, "write_bytes": wr[k] * ws,
+                      "total_bytes": fe[k] * fs + wr[k] * ws}
+    res = {"workload": f"32768 blocks of 24^3, {cells} cells", "fetch_scale": fs,
+           "write_scale": ws, "per_launch": per}
+    Path(out).write_text(json.dumps(res, indent=1))
+    print(f"fetch scale x{fs:.3f}, write scale x{ws:.3f}")
+    print("| kernel | fetch | write | total |\n|---|---|---|---|")
+    for k, v in per.items():
+        print(f"| `{k}` | {v['fetch_bytes'] / 1e9:.2f} | {v['write_bytes'] / 1e9:.2f} "
+              f"| {v['total_bytes'] / 1e9:.2f} |")
+
+
 def main():
+    if len(sys.argv) == 5 and sys.argv[1] == "--pmc-summary":
+        pmc_summary(*sys.argv[2:5])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8, help="timed outer iterations")
@@ -301,12 +372,16 @@ def main():
         log(f"[bench] timing {form}: {args.steps} outer x {args.ninner} inner")
         wall, st = run_steps(s, fields0, args.steps, args.ninner)
         kt = s.time_kernels(niter=3)
-        res[form] = dict(wall=wall, stats=st, kernels=kt, info=s.info())
+        shape, shape_ms = s.plan_shape()
+        res[form] = dict(wall=wall, stats=st, kernels=kt, info=s.info(),
+                         autotune={"shape": shape,
+                                   "ms_per_outer_by_shape": [round(x, 3) for x in shape_ms]})
         log(f"[bench] {form}: {wall:.3f} s, device loop {st['solve_ms']:.1f} ms, kernels {kt}")
         s.close()
         del s
         torch.cuda.empty_cache()
 
+    copy_gbs = copy_bandwidth()
     cells = g.ncells
     main_form = "fp32" if "fp32" in res else forms[0]
     r = res[main_form]
@@ -319,10 +394,11 @@ def main():
     tfile = ROOT / "profiles" / "traffic_sip_32768x24.json"
     if tfile.exists() and nb == (32, 32, 32) and n == (24, 24, 24):
         key = (f"void mgp::k_forward<{'float' if main_form == 'fp32' else 'double'}, "
-               f"false, false, true, 2>")
+               f"false, false, true, ")
         per = json.loads(tfile.read_text())["per_launch"]
-        if key in per:
-            traffic = round(per[key]["total_bytes"] / 1e9, 3)
+        hit = [k for k in per if k.startswith(key)]
+        if hit:
+            traffic = round(per[hit[0]]["total_bytes"] / 1e9, 3)
     line = {
         "metric": "GCell-updates/s + time-to-1e-10 residual, FP32-refined vs FP64, 1 MI355X",
         "value": round(value, 3),
@@ -340,13 +416,19 @@ def main():
                                                    / res[f]["wall"] / 1e9, 3),
                       "ms_per_step": round(res[f]["wall"] / args.steps * 1e3, 3),
                       "kernel_ms": {k: round(v, 4) for k, v in res[f]["kernels"].items()},
-                      "device_gb": round(res[f]["info"]["device_bytes"] / 1e9, 2)}
+                      "device_gb": round(res[f]["info"]["device_bytes"] / 1e9, 2),
+                      "autotune": res[f]["autotune"]}
                   for f in res},
         "roofline": {"bound": "hbm", "kernel": f"k_forward ({main_form})",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_unit": "GB per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
-                     "alg_bytes_per_cell": ALG_BYTES[main_form]["forward"]},
+                     "alg_bytes_per_cell": ALG_BYTES[main_form]["forward"],
+                     "copy_gbs": round(copy_gbs, 1),
+                     "frac_of_copy": round(achieved / copy_gbs, 4),
+                     "inner_iteration_gbs": round(
+                         (ALG_BYTES[main_form]["forward"] + ALG_BYTES[main_form]["backward"])
+                         * cells / ((fw_ms + r["kernels"]["backward_ms"]) * 1e-3) / 1e9, 1)},
     }
     if "fp32" in res and "fp64" in res:
         line["fp32_over_fp64_throughput"] = round(res["fp64"]["wall"] / res["fp32"]["wall"], 3)

@@ -184,6 +184,18 @@ int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
  * ms[0] forward sweep, ms[1] backward sweep, ms[2] one outer update. */
 int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms);
 
+/* Pipeline shape of the plane kernels (the sweeps, SOR and the outer
+ * update): 1 = four 64-cell items of loads in flight per wave, 2 = two
+ * 128-cell items, 3 = three 128-cell items.  request -1 only reports;
+ * 0 runs the autotuner:
+ * it times one inner iteration per shape (ninner x forward + backward, or
+ * SOR, plus one outer update) on h's own blocks and keeps the fastest;
+ * 1..3 sets the shape.  Results do not depend on the shape.  chosen gets
+ * the shape in use, ms[3] (may be NULL) the timed ms per shape (0 when not
+ * timed).  mgp_init autotunes unless MGP_PLAN_SHAPE=1..3 is set. */
+int mgp_plan_shape(mgp_handle *h, mgletint request, mgletint *chosen,
+                   double *ms);
+
 /* Level-weighted max |RHS| after each outer iteration of the last mgpoisl
  * call (the quantity of the exit test, src/flow/pressuresolver_mod.F90:
  * 274-301); writes min(count, cap) values and the count. */

@@ -138,7 +138,7 @@ struct mgp_handle {
     // 1-D fields (FP64 and working precision)
     DBuf d_dx, d_dy, d_dz, d_rdx, d_rdy, d_rdz, d_rddx, d_rddy, d_rddz;
     DBuf d_ddx, d_ddy, d_ddz;
-    DBuf d_raw[6], d_c64[6], d_cT[6];
+    DBuf d_raw[6], d_c64[6], d_cT[6], d_apd[3];
     // per-cell WO arrays
     DBuf d_ap64, d_apT, d_f[7], d_hilf, d_rhs64, d_rhsT, d_res, d_dp64;
     DBuf d_bpw, d_bpf, d_bpsp;
@@ -171,6 +171,8 @@ struct mgp_handle {
     hipGraphExec_t graph = nullptr;
     int graph_key[4] = {-1, -1, -1, -1};
     bool have_solution = false;
+    int plan_shape = 2;          // pipeline shape of the plane kernels
+    double shape_ms[3] = {0, 0, 0};
 
     ~mgp_handle() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -250,6 +252,9 @@ struct mgp_handle {
         for (int q = 0; q < 6; ++q)
             *c[q] = fp32() ? d_cT[q].as<T>() : d_c64[q].as<T>();
         F.rap = d_rapT.as<T>();
+        F.apx = d_apd[0].as<double>();
+        F.apy = d_apd[1].as<double>();
+        F.apz = d_apd[2].as<double>();
         F.hface = d_hface.as<T>();
         F.hpar = d_hpar.as<T>();
         F.rhi[0] = d_rhi[0].as<T>();
@@ -757,6 +762,9 @@ void setup_factors(mgp_handle &h) {
     HC(hipGetLastError());
 }
 
+template <typename T>
+void tune_shape(mgp_handle &h);
+
 int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
                 const double *dy, const double *dz, const double *ddx,
                 const double *ddy, const double *ddz, const double *bp,
@@ -764,8 +772,6 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
     if (!G || !cfg || !out || !dx || !dy || !dz || !ddx || !ddy || !ddz)
         return fail(MGP_ERR_ARG, "null argument");
     *out = nullptr;
-    // pipeline shape of the plane kernels (1, 2 or 3)
-    if (const char *d = getenv("MGP_PLAN_SHAPE")) set_plan_shape(atoi(d));
     int rc = check_cfg(*cfg);
     if (rc) return rc;
     if (G->ngrid <= 0) return fail(MGP_ERR_TOPOLOGY, "no grids");
@@ -966,6 +972,33 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
                     lo[a][q] = 2.0 / ((d[a][q - 1] + d[a][q]) * d[a][q - 1]);
                 }
         }
+        // separable part of GSAP per direction (src/ib/noib_mod.F90:154-157)
+        // and the wall fold-in terms per block (:178-244)
+        {
+            std::vector<double> apd[3];
+            const double *dd[3] = {dx, dy, dz};
+            const int64_t n1[3] = {h->n1x, h->n1y, h->n1z};
+            for (int a = 0; a < 3; ++a) apd[a].assign(n1[a], 0.0);
+            for (int g = 0; g < n; ++g) {
+                const int64_t o[3] = {h->ip1x[g], h->ip1y[g], h->ip1z[g]};
+                const int nn[3] = {h->ii[g], h->jj[g], h->kk[g]};
+                for (int a = 0; a < 3; ++a)
+                    for (int q = 3; q <= nn[a] - 2; ++q)
+                        apd[a][o[a] - 1 + q] = 2.0 / (dd[a][o[a] - 2 + q] * dd[a][o[a] - 1 + q]);
+                Block &B = h->blocks[g];
+                const int *bc = &h->bc[6 * g];
+                B.wallmask = is_wall_lo(bc[0]) | is_wall_hi(bc[1]) << 1 | is_wall_lo(bc[2]) << 2 |
+                             is_wall_hi(bc[3]) << 3 | is_wall_lo(bc[4]) << 4 | is_wall_hi(bc[5]) << 5;
+                B.wall[0] = raw[0][o[0] + 2];
+                B.wall[1] = raw[1][o[0] + 2 + B.ni - 1];
+                B.wall[2] = raw[2][o[1] + 2];
+                B.wall[3] = raw[3][o[1] + 2 + B.nj - 1];
+                B.wall[4] = raw[4][o[2] + 2];
+                B.wall[5] = raw[5][o[2] + 2 + B.nk - 1];
+            }
+            for (int a = 0; a < 3; ++a) upload(h->d_apd[a], apd[a]);
+            upload(h->d_blocks, h->blocks);
+        }
         for (int q = 0; q < 6; ++q) fold[q] = raw[q];
         for (int g = 0; g < n; ++g) {
             const int *bc = &h->bc[6 * g];
@@ -1043,12 +1076,72 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
             HC(hipGetLastError());
         }
         HC(hipStreamSynchronize(h->stream));
+        const char *ps = getenv("MGP_PLAN_SHAPE");
+        const int forced = ps ? atoi(ps) : 0;
+        if (forced >= 1 && forced <= 3) {
+            h->plan_shape = forced;
+        } else {
+            if (h->fp32()) tune_shape<float>(*h);
+            else tune_shape<double>(*h);
+        }
     } catch (const HipError &e) {
         return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
     }
     *out = h.release();
     g_err.clear();
     return MGP_OK;
+}
+
+// Autotuner of the plane kernels' pipeline shape: times, per shape, the
+// kernels of one outer iteration (ninner sweeps or relaxations and one
+// outer update) on the handle's blocks and keeps the fastest.  Runs on
+// whatever the work arrays hold (before the first solve: scratch); the
+// solver state is reset at the start of every solve.
+template <typename T>
+void tune_shape(mgp_handle &h) {
+    const hipStream_t s = h.stream;
+    Topo tall = h.topo(-1);
+    Fields<T> F = h.fields<T>();
+    Mask m = h.mask();
+    Outer o = h.outer();
+    o.done = (int *)h.d_never.p;
+    const int *never = (const int *)h.d_never.p;
+    const bool classic = h.cfg.ityp == 2;
+    const bool sor = h.cfg.ityp == 1;
+    T *rhsT = h.fp32() ? h.d_rhsT.as<T>() : nullptr;
+    hipEvent_t e0, e1;
+    HC(hipEventCreate(&e0));
+    HC(hipEventCreate(&e1));
+    auto pass = [&]() {
+        if (sor) launch_sor<T>(tall, F, m, F.hface, h.d_hface2.as<T>(), h.cfg.omega, 0, never, s);
+        else {
+            launch_forward<T>(tall, F, m, 0, 1, 0, classic, never, s);
+            launch_backward<T>(tall, F, 0, classic, never, s);
+        }
+    };
+    int best = 0;
+    for (int v = 1; v <= 3; ++v) {
+        set_plan_shape(v);
+        pass();  // warm-up
+        float a = 0.f, b = 0.f;
+        HC(hipEventRecord(e0, s));
+        for (int r = 0; r < 2; ++r) pass();
+        HC(hipEventRecord(e1, s));
+        HC(hipEventSynchronize(e1));
+        HC(hipEventElapsedTime(&a, e0, e1));
+        HC(hipEventRecord(e0, s));
+        launch_outer<T>(tall, F, m, o, rhsT, s);
+        HC(hipEventRecord(e1, s));
+        HC(hipEventSynchronize(e1));
+        HC(hipEventElapsedTime(&b, e0, e1));
+        h.shape_ms[v - 1] = 0.5 * a * std::max(1, h.cfg.ninner) + b;
+        if (!best || h.shape_ms[v - 1] < h.shape_ms[best - 1]) best = v;
+    }
+    HC(hipGetLastError());
+    HC(hipEventDestroy(e0));
+    HC(hipEventDestroy(e1));
+    h.plan_shape = best;
+    set_plan_shape(best);
 }
 
 // Pool fields start at zero (src/core/fieldpool_mod.F90:113-122).
@@ -1079,6 +1172,7 @@ float iterate(mgp_handle &h, const Fields<T> &F, const Outer &o,
     static_assert(sizeof(T) == 4 || sizeof(T) == 8, "T");
     const mgp_config &c = h.cfg;
     const hipStream_t s = h.stream;
+    set_plan_shape(h.plan_shape);
     Topo tall = h.topo(-1);
     T *rhsT = h.fp32() ? h.d_rhsT.as<T>() : nullptr;
     hipEvent_t e0, e1;
@@ -1630,6 +1724,7 @@ int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
 int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms) {
     if (!h || !ms || niter < 1) return fail(MGP_ERR_ARG, "bad argument");
     try {
+        set_plan_shape(h->plan_shape);
         const hipStream_t s = h->stream;
         hipEvent_t ev[4];
         for (auto &e : ev) HC(hipEventCreate(&e));
@@ -1677,6 +1772,27 @@ int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms) {
     } catch (const HipError &e) {
         return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
     }
+    return MGP_OK;
+}
+
+int mgp_plan_shape(mgp_handle *h, mgletint request, mgletint *chosen,
+                   double *ms) {
+    if (!h || request < -1 || request > 3) return fail(MGP_ERR_ARG, "bad argument");
+    try {
+        if (request < 0) {
+            // report only
+        } else if (request == 0) {
+            if (h->fp32()) tune_shape<float>(*h);
+            else tune_shape<double>(*h);
+        } else {
+            h->plan_shape = request;
+        }
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    if (chosen) *chosen = h->plan_shape;
+    if (ms)
+        for (int v = 0; v < 3; ++v) ms[v] = h->shape_ms[v];
     return MGP_OK;
 }
 

@@ -67,6 +67,8 @@ struct Block {
     int64_t fc;        // offset of the block's face buffers
     int64_t sh;        // offset of the block's ghost-shell buffers
     int64_t o1x, o1y, o1z; // offsets of the block in 1-D fields
+    int wallmask;      // bit f: face f is a wall (GSAP fold-in)
+    double wall[6];    // raw boundary coefficient of face f (aw(0), ae(ni-1), ...)
 };
 
 // Position of face cell (u, v), u in [0, a), v in [0, b), inside one face

@@ -36,6 +36,10 @@ struct Fields {
     const T *rap;               // SOR_RAP = divide0(1, GSAP) (SOR only)
     const T *f[7];              // LW LS LB LPR UE UN UT
     const T *aw, *ae, *as, *an, *ab, *at;  // 1-D coefficients (ref. layout)
+    // 2 / (d(i-1) d(i)) per direction (ref. 1-D layout, FP64): the separable
+    // part of GSAP, from which the plane kernels rebuild GSAP of blocks
+    // without blocked cells
+    const double *apx, *apy, *apz;
     T *hface;                   // HILF boundary planes (own faces)
     T *hpar;                    // HILF ghosts of the block's PAR faces
     T *rhi[2];                  // RES upper planes, double-buffered

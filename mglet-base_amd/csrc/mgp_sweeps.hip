@@ -125,6 +125,41 @@ __device__ __forceinline__ void coef_rows(C *c, const C *aw, const C *ae,
     }
 }
 
+// Separable part of GSAP in LDS: [x y z][kMaxN1] of 2 / (d(i-1) d(i)).
+__device__ __forceinline__ void ap_rows(double *c, const double *ax,
+                                        const double *ay, const double *az,
+                                        const Block &B) {
+    for (int e = threadIdx.x; e < 3 * kMaxN1; e += kWave) {
+        const int r = e / kMaxN1, q = e % kMaxN1;
+        const int n = r == 0 ? B.ni : (r == 1 ? B.nj : B.nk);
+        if (q >= n) continue;
+        const double *src = r == 0 ? ax + B.o1x : r == 1 ? ay + B.o1y : az + B.o1z;
+        c[e] = src[q + 2];
+    }
+}
+
+// GSAP of a cell of a block without blocked cells (BP = 1 on the block
+// and its ghost layer), in the order of src/ib/noib_mod.F90:154-244: the
+// separable part, the six a_nb (1 - BP BP_nb) terms (all +0 here, exact),
+// then the wall fold-ins of the boundary cells (a_nb BP BP_nb = a_nb).
+// Equal bit for bit to the stored GSAP.
+__device__ __forceinline__ double ap_cell(const double *ar, int i, int j, int k,
+                                          uint32_t fl, const Block &B) {
+    double a = -ar[i];
+    a = a - ar[kMaxN1 + j];
+    a = a - ar[2 * kMaxN1 + k];
+    const uint32_t w = fl & (uint32_t)B.wallmask;
+    if (w) {
+        if (w & 1) a = a + B.wall[0];
+        if (w & 2) a = a + B.wall[1];
+        if (w & 4) a = a + B.wall[2];
+        if (w & 8) a = a + B.wall[3];
+        if (w & 16) a = a + B.wall[4];
+        if (w & 32) a = a + B.wall[5];
+    }
+    return a;
+}
+
 // Window sources of HILF for the 6 faces: the neighbour's boundary plane
 // (CON), the block's own PAR ghosts from bound_pressure, or none.
 template <typename T>
@@ -287,6 +322,9 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
 // HILF = 0 and the Laplacian term vanishes.
 template <typename T, bool CLASSIC, bool MASKED, bool LAP>
 struct FwdK {
+    // GSAP rebuilt from the separable part in launches without blocked
+    // cells (saves its load)
+    static constexpr bool APC = LAP && !MASKED;
     struct Data {
         uint2 c;
         T rhs, lpr, lb, ls, lw, ap;
@@ -295,6 +333,8 @@ struct FwdK {
     const T *rhs, *lpr, *lb, *ls, *lw, *ap;
     T *res, *rlo, *rw;
     const T *cf;
+    const double *ar;
+    const Block *blk;
     T *R, *H;
     uint8_t *BH;
     int NI, NJ, NK, HW, RW, RS, mHS;
@@ -308,7 +348,7 @@ struct FwdK {
         d.lb = ld(lb, s);
         d.ls = ld(ls, s);
         d.lw = ld(lw, s);
-        if (LAP) d.ap = ld(ap, s);
+        if (LAP && !APC) d.ap = ld(ap, s);
     }
     __device__ __forceinline__ void compute(const Data &d, int s, int p) const {
         int i, j, k;
@@ -321,14 +361,15 @@ struct FwdK {
             const T *aw = cf, *ae = cf + kMaxN1, *as = cf + 2 * kMaxN1,
                     *an = cf + 3 * kMaxN1, *ab = cf + 4 * kMaxN1,
                     *at = cf + 5 * kMaxN1;
+            const T apv = APC ? (T)ap_cell(ar, i, j, k, d.c.y >> 24, *blk) : d.ap;
             if (MASKED && masked)
                 r = laplace_cell<T, T, true>(H + sm, H + s0, H + sp, BH + sm, BH + s0,
                                              BH + sp, c0, HW, aw[i], ae[i], as[j], an[j],
-                                             ab[k], at[k], d.ap);
+                                             ab[k], at[k], apv);
             else
                 r = laplace_cell<T, T, false>(H + sm, H + s0, H + sp, nullptr, nullptr,
                                               nullptr, c0, HW, aw[i], ae[i], as[j], an[j],
-                                              ab[k], at[k], d.ap);
+                                              ab[k], at[k], apv);
         }
         const int c1 = (j + 1) + RW * (i + 1);
         const T *Rp = R + ((p - 1) & 1) * RS;
@@ -381,15 +422,18 @@ __global__ __launch_bounds__(kWave) void k_forward(
     const Shape &S = t.shapes[B.shape];
     const int mRS = (t.max_nj + 1) * (t.max_ni + 1);
     const int mHS = (t.max_nj + 2) * (t.max_ni + 2);
+    constexpr bool APC = FwdK<T, CLASSIC, MASKED, LAP>::APC;
     SrcTab<T> *tab = (SrcTab<T> *)lds_fw;
     T *cf = (T *)(lds_fw + sizeof(SrcTab<T>));
-    T *R = cf + 6 * kMaxN1;
+    double *ar = (double *)(cf + 6 * kMaxN1);
+    T *R = (T *)(ar + (APC ? 3 * kMaxN1 : 0));
     T *H = R + 2 * mRS;
     uint8_t *BH = (uint8_t *)(H + 4 * mHS);
     // R sources: the neighbours' upper planes feed faces 0, 2, 4
     fill_tab<T>(tab, t, B, LAP ? F.hface : nullptr, F.hpar, F.hilf, F.rhi[rd],
                 0x15, m);
     if (LAP) coef_rows<T>(cf, F.aw, F.ae, F.as, F.an, F.ab, F.at, B);
+    if (APC) ap_rows(ar, F.apx, F.apy, F.apz, B);
     wave_sync();
     const int NI = B.ni, NJ = B.nj, NK = B.nk;
     const int64_t wo = B.wo, fc = B.fc;
@@ -405,6 +449,8 @@ __global__ __launch_bounds__(kWave) void k_forward(
     k.rlo = F.rlo + fc;
     k.rw = write_hi ? F.rhi[rd ^ 1] + fc : nullptr;
     k.cf = cf;
+    k.ar = ar;
+    k.blk = &B;
     k.R = R;
     k.H = H;
     k.BH = BH;
@@ -554,6 +600,8 @@ struct SorK {
     const T *rhs, *rap;
     T *hilf, *fout;
     const T *cf;
+    const double *ar;
+    const Block *blk;
     T *H;
     uint8_t *BH;
     int NI, NJ, NK, HW, mHS;
@@ -564,7 +612,7 @@ struct SorK {
     __device__ __forceinline__ void load(Data &d, int s) const {
         d.c = ld(cells, s);
         d.rhs = ld(rhs, s);
-        d.rap = ld(rap, s);
+        if (MASKED) d.rap = ld(rap, s);
     }
     __device__ __forceinline__ void compute(const Data &d, int s, int p) const {
         const int u = (p & 1) ? p : p - 2;
@@ -598,7 +646,15 @@ struct SorK {
                 r = aw[i] * pw + ae[i] * pe + as[j] * psv + an[j] * pn +
                     ab[k] * pb + at[k] * pt;
             }
-            r = (r - d.rhs) * d.rap;
+            T rap;
+            if (MASKED) {
+                rap = d.rap;
+            } else {
+                // SOR_RAP = divide0(1, GSAP), src/flow/sor_mod.F90:18-49
+                const double a = ap_cell(ar, i, j, k, d.c.y >> 24, *blk);
+                rap = (T)(a == 0.0 ? 0.0 : 1.0 / a);
+            }
+            r = (r - d.rhs) * rap;
             nv = om1 * old - om * r;
         }
         H0[c0] = nv;
@@ -634,10 +690,12 @@ __global__ __launch_bounds__(kWave) void k_sor(
     const int mHS = (t.max_nj + 2) * (t.max_ni + 2);
     SrcTab<T> *tab = (SrcTab<T> *)lds_so;
     T *cf = (T *)(lds_so + sizeof(SrcTab<T>));
-    T *H = cf + 6 * kMaxN1;
+    double *ar = (double *)(cf + 6 * kMaxN1);
+    T *H = (T *)(ar + (MASKED ? 0 : 3 * kMaxN1));
     uint8_t *BH = (uint8_t *)(H + 8 * mHS);
     fill_tab<T>(tab, t, B, fin, F.hpar, F.hilf, nullptr, 0, m);
     coef_rows<T>(cf, F.aw, F.ae, F.as, F.an, F.ab, F.at, B);
+    if (!MASKED) ap_rows(ar, F.apx, F.apy, F.apz, B);
     wave_sync();
     const int NI = B.ni, NJ = B.nj, NK = B.nk;
     const int64_t wo = B.wo;
@@ -648,6 +706,8 @@ __global__ __launch_bounds__(kWave) void k_sor(
     k.hilf = F.hilf + wo;
     k.fout = fout + B.fc;
     k.cf = cf;
+    k.ar = ar;
+    k.blk = &B;
     k.H = H;
     k.BH = BH;
     k.NI = NI;
@@ -676,10 +736,12 @@ __global__ __launch_bounds__(kWave) void k_sor(
 template <typename T, bool MASKED>
 struct OutK {
     struct Data {
-        uint32_t c;
+        uint2 c;
         double ap, rhs, dp;
     };
-    const uint32_t *cells;   // word 0 of the cell words (stride 2)
+    const uint2 *cells;
+    const double *ar;
+    const Block *blk;
     const double *ap;
     double *rhs, *dp;
     T *rhs_t;
@@ -691,16 +753,17 @@ struct OutK {
     double mx;
 
     __device__ __forceinline__ void load(Data &d, int s) const {
-        d.c = ld(cells, 2 * s);
-        d.ap = ld(ap, s);
+        d.c = ld(cells, s);
+        if (MASKED) d.ap = ld(ap, s);
         d.rhs = ld((const double *)rhs, s);
         d.dp = ld((const double *)dp, s);
     }
     __device__ __forceinline__ void compute(const Data &d, int s, int p) {
         int i, j, k;
-        cell_ijk(d.c, i, j, k);
+        cell_ijk(d.c.x, i, j, k);
         const int c0 = (j + 1) + HW * (i + 1);
         const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS, sp = ((p + 2) & 3) * mHS;
+        const double apv = MASKED ? d.ap : ap_cell(ar, i, j, k, d.c.y >> 24, *blk);
         const double *aw = cf, *ae = cf + kMaxN1, *as = cf + 2 * kMaxN1,
                      *an = cf + 3 * kMaxN1, *ab = cf + 4 * kMaxN1,
                      *at = cf + 5 * kMaxN1;
@@ -708,11 +771,11 @@ struct OutK {
         if (MASKED && masked)
             r = laplace_cell<T, double, true>(H + sm, H + s0, H + sp, BH + sm, BH + s0,
                                               BH + sp, c0, HW, aw[i], ae[i], as[j], an[j],
-                                              ab[k], at[k], d.ap);
+                                              ab[k], at[k], apv);
         else
             r = laplace_cell<T, double, false>(H + sm, H + s0, H + sp, nullptr, nullptr,
                                                nullptr, c0, HW, aw[i], ae[i], as[j], an[j],
-                                               ab[k], at[k], d.ap);
+                                               ab[k], at[k], apv);
         const double rn = d.rhs + r;
         st(rhs, s, rn);
         if (rhs_t) st(rhs_t, s, (T)rn);
@@ -740,14 +803,18 @@ __global__ __launch_bounds__(kWave) void k_outer(
     const int mHS = (t.max_nj + 2) * (t.max_ni + 2);
     SrcTab<T> *tab = (SrcTab<T> *)lds_ou;
     double *cf = (double *)(lds_ou + sizeof(SrcTab<T>));
-    T *H = (T *)(cf + 6 * kMaxN1);
+    double *ar = cf + 6 * kMaxN1;
+    T *H = (T *)(ar + (MASKED ? 0 : 3 * kMaxN1));
     uint8_t *BH = (uint8_t *)(H + 4 * mHS);
     fill_tab<T>(tab, t, B, F.hface, F.hpar, F.hilf, nullptr, 0, m);
     coef_rows<double>(cf, o.aw, o.ae, o.as, o.an, o.ab, o.at, B);
+    if (!MASKED) ap_rows(ar, F.apx, F.apy, F.apz, B);
     wave_sync();
     const int64_t wo = B.wo;
     OutK<T, MASKED> k;
-    k.cells = (const uint32_t *)(t.cellx + S.cell_off);
+    k.cells = t.cellx + S.cell_off;
+    k.ar = ar;
+    k.blk = &B;
     k.ap = o.ap64 + wo;
     k.rhs = o.rhs64 + wo;
     k.dp = o.dp64 + wo;
@@ -800,7 +867,8 @@ void launch_forward(const Topo &t, const Fields<T> &f, Mask m, int rd,
     const bool masked = m.bp != nullptr;
     const bool lap = !hilf_zero;
     const size_t lds = tab_bytes<T>() + 6 * kMaxN1 * sizeof(T) + 2 * mRS * sizeof(T) +
-                       (lap ? 4 * mHS * sizeof(T) + (masked ? 4 * mHS : 0) : 0);
+                       (lap ? 4 * mHS * sizeof(T) + (masked ? 4 * mHS : 0) : 0) +
+                       (lap && !masked ? 3 * kMaxN1 * sizeof(double) : 0);
 #define MGP_FWD(C, M, L) fwd_launch<T, C, M, L>(t, f, m, rd, write_hi, done, lds, s)
     if (classic) {
         if (masked) { if (lap) MGP_FWD(true, true, true); else MGP_FWD(true, true, false); }
@@ -864,7 +932,7 @@ void launch_sor(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
     const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
     const bool masked = m.bp != nullptr;
     const size_t lds = tab_bytes<T>() + 6 * kMaxN1 * sizeof(T) + 8 * mHS * sizeof(T) +
-                       (masked ? 8 * mHS : 0);
+                       (masked ? 8 * mHS : 3 * kMaxN1 * sizeof(double));
     if (masked) {
         if (hilf_zero) sor_launch<T, true, true>(t, f, m, fin, fout, omega, done, lds, s);
         else sor_launch<T, true, false>(t, f, m, fin, fout, omega, done, lds, s);
@@ -896,7 +964,7 @@ void launch_outer(const Topo &t, const Fields<T> &f, Mask m, Outer o,
     const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
     const bool masked = m.bp != nullptr;
     const size_t lds = tab_bytes<T>() + 6 * kMaxN1 * sizeof(double) +
-                       4 * mHS * sizeof(T) + (masked ? 4 * mHS : 0);
+                       4 * mHS * sizeof(T) + (masked ? 4 * mHS : 3 * kMaxN1 * sizeof(double));
     if (masked) out_launch<T, true>(t, f, m, o, rhs_t, lds, s);
     else out_launch<T, false>(t, f, m, o, rhs_t, lds, s);
 }

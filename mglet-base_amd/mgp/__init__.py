@@ -165,6 +165,16 @@ class PressureSolver:
         _lib.check(self.lib.mgp_time_kernels(self.h, niter, _ptr(ms)), "mgp_time_kernels")
         return dict(forward_ms=ms[0], backward_ms=ms[1], outer_ms=ms[2])
 
+    def plan_shape(self, request=-1):
+        """Pipeline shape of the plane kernels.  request -1: report; 0: run
+        the autotuner again; 1..3: set.  Returns (shape, ms per shape as
+        last timed by the autotuner, 0 where not timed)."""
+        chosen = C.c_int32()
+        ms = np.zeros(3)
+        _lib.check(self.lib.mgp_plan_shape(self.h, request, C.byref(chosen), _ptr(ms)),
+                   "mgp_plan_shape")
+        return chosen.value, ms
+
     def history(self):
         """max |RHS| after each outer iteration of the last mgpoisl call."""
         cap = max(1, self.cfg.nouter)

@@ -86,6 +86,7 @@ def load():
                                C.POINTER(vp), C.c_int32],
         "mgp_divcal_device": [vp, vp, vp, vp, vp, C.c_double],
         "mgp_time_kernels": [vp, C.c_int32, _dp],
+        "mgp_plan_shape": [vp, C.c_int32, _i32p, _dp],
         "mgp_info": [vp, _i64p, _i64p],
         "mgp_get_history": [vp, _dp, C.c_int32, _i32p],
         "mgp_adjoint": [vp, _dp, _dp, _dp, C.c_double, C.POINTER(MgpStats)],

@@ -194,3 +194,45 @@ def test_single_hip_runtime():
     libs = {ln.split()[-1] for ln in maps.splitlines() if "libamdhip64" in ln}
     assert len(libs) == 1, libs
     s.close()
+
+
+@pytest.mark.parametrize("shape", [1, 2, 3])
+@pytest.mark.parametrize("case", ["sip-walls", "classic-masked", "sor-odd", "sor-masked"])
+def test_plan_shapes_bitwise(oracle_mod, shape, case):
+    """Every pipeline shape of the plane kernels (items of 64 or 128 cells,
+    2 to 4 in flight) gives the oracle's FP64 result bit for bit."""
+    kind, name = case.split("-")
+    ityp = {"sip": 0, "classic": 2, "sor": 1}[kind]
+    topo = {"walls": "walls", "masked": "box2", "odd": "odd"}[name]
+    frac = 0.15 if name == "masked" else 0.0
+    g = topology(topo)
+    o = oracle_mod.Oracle(g)
+    bp = None
+    if frac > 0:
+        bp = 1.0 - blocked_mask(g, 107, frac)
+        o.exchange([bp], layers=2, corners=True)
+    ibtype = 1 if frac > 0 else 0
+    st = o.setup(bp=bp, ityp=ityp)
+    u, v, w = velocity(g, o, 7)
+    p = np.zeros(g.n3d)
+    ins = [a.copy() for a in (u, v, w, p)]
+    kw = dict(sor_all_levels=1) if ityp == 1 else {}
+    so = o.mgpoisl(st, u, v, w, p, 1e-3, ityp=ityp, ninner=3, nouter=3, ibtype=ibtype, **kw)
+    s = _solver(g, type={0: "sip", 1: "sor", 2: "classic"}[ityp], ninner=3, nouter=3,
+                ib={0: "noib", 1: "ghostcell"}[ibtype], bp=bp, **kw)
+    assert s.plan_shape(shape)[0] == shape
+    sg = s.mgpoisl(*ins, 1e-3)
+    assert sg["maxrhs"] == so["maxrhs"]
+    for a, b, nm in zip((u, v, w, p), ins, "uvwp"):
+        assert np.array_equal(a, b), nm
+
+
+def test_plan_shape_autotuner():
+    """The autotuner times every shape on the handle's blocks and keeps the
+    fastest."""
+    g = topology("box2")
+    s = _solver(g)
+    shape, ms = s.plan_shape(0)
+    assert shape in (1, 2, 3)
+    assert np.all(ms > 0) and ms[shape - 1] == ms.min()
+    assert s.plan_shape()[0] == shape
