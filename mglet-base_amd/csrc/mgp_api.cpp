@@ -171,7 +171,7 @@ struct mgp_handle {
     hipGraphExec_t graph = nullptr;
     int graph_key[4] = {-1, -1, -1, -1};
     bool have_solution = false;
-    int plan_shape = 2;          // pipeline shape of the plane kernels
+    int plan_shape = 1;          // pipeline shape of the plane kernels
     double shape_ms[3] = {0, 0, 0};
 
     ~mgp_handle() {

@@ -108,14 +108,21 @@ __device__ __forceinline__ A laplace_cell(const T *Hm, const T *H0,
     return r;
 }
 
-// 1-D coefficient rows of a block in LDS: [w e s n b t][kMaxN1]
+// Length of the LDS rows of 1-D data: the largest block dimension, even
+// (so that FP32 rows keep the FP64 rows behind them 8-byte aligned).
+__host__ __device__ inline int row_len(const Topo &t) {
+    const int m = t.max_ni > t.max_nj ? t.max_ni : t.max_nj;
+    return ((m > t.max_nk ? m : t.max_nk) + 1) & ~1;
+}
+
+// 1-D coefficient rows of a block in LDS: [w e s n b t][nr]
 template <typename C>
 __device__ __forceinline__ void coef_rows(C *c, const C *aw, const C *ae,
                                           const C *as, const C *an,
                                           const C *ab, const C *at,
-                                          const Block &B) {
-    for (int e = threadIdx.x; e < 6 * kMaxN1; e += kWave) {
-        const int r = e / kMaxN1, q = e % kMaxN1;
+                                          const Block &B, int nr) {
+    for (int e = threadIdx.x; e < 6 * nr; e += kWave) {
+        const int r = e / nr, q = e % nr;
         const int n = r < 2 ? B.ni : (r < 4 ? B.nj : B.nk);
         if (q >= n) continue;
         const C *src = r == 0 ? aw + B.o1x : r == 1 ? ae + B.o1x
@@ -125,12 +132,12 @@ __device__ __forceinline__ void coef_rows(C *c, const C *aw, const C *ae,
     }
 }
 
-// Separable part of GSAP in LDS: [x y z][kMaxN1] of 2 / (d(i-1) d(i)).
+// Separable part of GSAP in LDS: [x y z][nr] of 2 / (d(i-1) d(i)).
 __device__ __forceinline__ void ap_rows(double *c, const double *ax,
                                         const double *ay, const double *az,
-                                        const Block &B) {
-    for (int e = threadIdx.x; e < 3 * kMaxN1; e += kWave) {
-        const int r = e / kMaxN1, q = e % kMaxN1;
+                                        const Block &B, int nr) {
+    for (int e = threadIdx.x; e < 3 * nr; e += kWave) {
+        const int r = e / nr, q = e % nr;
         const int n = r == 0 ? B.ni : (r == 1 ? B.nj : B.nk);
         if (q >= n) continue;
         const double *src = r == 0 ? ax + B.o1x : r == 1 ? ay + B.o1y : az + B.o1z;
@@ -143,11 +150,11 @@ __device__ __forceinline__ void ap_rows(double *c, const double *ax,
 // separable part, the six a_nb (1 - BP BP_nb) terms (all +0 here, exact),
 // then the wall fold-ins of the boundary cells (a_nb BP BP_nb = a_nb).
 // Equal bit for bit to the stored GSAP.
-__device__ __forceinline__ double ap_cell(const double *ar, int i, int j, int k,
-                                          uint32_t fl, const Block &B) {
+__device__ __forceinline__ double ap_cell(const double *ar, int nr, int i, int j,
+                                          int k, uint32_t fl, const Block &B) {
     double a = -ar[i];
-    a = a - ar[kMaxN1 + j];
-    a = a - ar[2 * kMaxN1 + k];
+    a = a - ar[nr + j];
+    a = a - ar[2 * nr + k];
     const uint32_t w = fl & (uint32_t)B.wallmask;
     if (w) {
         if (w & 1) a = a + B.wall[0];
@@ -337,7 +344,7 @@ struct FwdK {
     const Block *blk;
     T *R, *H;
     uint8_t *BH;
-    int NI, NJ, NK, HW, RW, RS, mHS;
+    int NI, NJ, NK, HW, RW, RS, mHS, nr;
     int fb0, fb1, fb2, fb3, fb4, fb5;
     bool masked;
 
@@ -358,10 +365,9 @@ struct FwdK {
             const int c0 = (j + 1) + HW * (i + 1);
             const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS,
                       sp = ((p + 2) & 3) * mHS;
-            const T *aw = cf, *ae = cf + kMaxN1, *as = cf + 2 * kMaxN1,
-                    *an = cf + 3 * kMaxN1, *ab = cf + 4 * kMaxN1,
-                    *at = cf + 5 * kMaxN1;
-            const T apv = APC ? (T)ap_cell(ar, i, j, k, d.c.y >> 24, *blk) : d.ap;
+            const T *aw = cf, *ae = cf + nr, *as = cf + 2 * nr,
+                    *an = cf + 3 * nr, *ab = cf + 4 * nr, *at = cf + 5 * nr;
+            const T apv = APC ? (T)ap_cell(ar, nr, i, j, k, d.c.y >> 24, *blk) : d.ap;
             if (MASKED && masked)
                 r = laplace_cell<T, T, true>(H + sm, H + s0, H + sp, BH + sm, BH + s0,
                                              BH + sp, c0, HW, aw[i], ae[i], as[j], an[j],
@@ -425,15 +431,16 @@ __global__ __launch_bounds__(kWave) void k_forward(
     constexpr bool APC = FwdK<T, CLASSIC, MASKED, LAP>::APC;
     SrcTab<T> *tab = (SrcTab<T> *)lds_fw;
     T *cf = (T *)(lds_fw + sizeof(SrcTab<T>));
-    double *ar = (double *)(cf + 6 * kMaxN1);
-    T *R = (T *)(ar + (APC ? 3 * kMaxN1 : 0));
+    const int nr = row_len(t);
+    double *ar = (double *)(cf + 6 * nr);
+    T *R = (T *)(ar + (APC ? 3 * nr : 0));
     T *H = R + 2 * mRS;
     uint8_t *BH = (uint8_t *)(H + 4 * mHS);
     // R sources: the neighbours' upper planes feed faces 0, 2, 4
     fill_tab<T>(tab, t, B, LAP ? F.hface : nullptr, F.hpar, F.hilf, F.rhi[rd],
                 0x15, m);
-    if (LAP) coef_rows<T>(cf, F.aw, F.ae, F.as, F.an, F.ab, F.at, B);
-    if (APC) ap_rows(ar, F.apx, F.apy, F.apz, B);
+    if (LAP) coef_rows<T>(cf, F.aw, F.ae, F.as, F.an, F.ab, F.at, B, nr);
+    if (APC) ap_rows(ar, F.apx, F.apy, F.apz, B, nr);
     wave_sync();
     const int NI = B.ni, NJ = B.nj, NK = B.nk;
     const int64_t wo = B.wo, fc = B.fc;
@@ -461,6 +468,7 @@ __global__ __launch_bounds__(kWave) void k_forward(
     k.RW = NJ + 1;
     k.RS = (NJ + 1) * (NI + 1);
     k.mHS = mHS;
+    k.nr = nr;
     k.fb0 = face_base(0, NI, NJ, NK);
     k.fb1 = face_base(1, NI, NJ, NK);
     k.fb2 = face_base(2, NI, NJ, NK);
@@ -604,7 +612,7 @@ struct SorK {
     const Block *blk;
     T *H;
     uint8_t *BH;
-    int NI, NJ, NK, HW, mHS;
+    int NI, NJ, NK, HW, mHS, nr;
     int fb0, fb1, fb2, fb3, fb4, fb5;
     T om, om1;
     bool masked, keep_top;
@@ -624,9 +632,8 @@ struct SorK {
         const T old = H0[c0];
         T nv = old;
         if (!(keep_top && k == NK - 1)) {
-            const T *aw = cf, *ae = cf + kMaxN1, *as = cf + 2 * kMaxN1,
-                    *an = cf + 3 * kMaxN1, *ab = cf + 4 * kMaxN1,
-                    *at = cf + 5 * kMaxN1;
+            const T *aw = cf, *ae = cf + nr, *as = cf + 2 * nr,
+                    *an = cf + 3 * nr, *ab = cf + 4 * nr, *at = cf + 5 * nr;
             const T pw = Hm[c0 - HW], pe = Hp[c0 + HW];
             const T psv = Hm[c0 - 1], pn = Hp[c0 + 1];
             const T pb = Hm[c0], pt = Hp[c0];
@@ -651,7 +658,7 @@ struct SorK {
                 rap = d.rap;
             } else {
                 // SOR_RAP = divide0(1, GSAP), src/flow/sor_mod.F90:18-49
-                const double a = ap_cell(ar, i, j, k, d.c.y >> 24, *blk);
+                const double a = ap_cell(ar, nr, i, j, k, d.c.y >> 24, *blk);
                 rap = (T)(a == 0.0 ? 0.0 : 1.0 / a);
             }
             r = (r - d.rhs) * rap;
@@ -690,12 +697,13 @@ __global__ __launch_bounds__(kWave) void k_sor(
     const int mHS = (t.max_nj + 2) * (t.max_ni + 2);
     SrcTab<T> *tab = (SrcTab<T> *)lds_so;
     T *cf = (T *)(lds_so + sizeof(SrcTab<T>));
-    double *ar = (double *)(cf + 6 * kMaxN1);
-    T *H = (T *)(ar + (MASKED ? 0 : 3 * kMaxN1));
+    const int nr = row_len(t);
+    double *ar = (double *)(cf + 6 * nr);
+    T *H = (T *)(ar + (MASKED ? 0 : 3 * nr));
     uint8_t *BH = (uint8_t *)(H + 8 * mHS);
     fill_tab<T>(tab, t, B, fin, F.hpar, F.hilf, nullptr, 0, m);
-    coef_rows<T>(cf, F.aw, F.ae, F.as, F.an, F.ab, F.at, B);
-    if (!MASKED) ap_rows(ar, F.apx, F.apy, F.apz, B);
+    coef_rows<T>(cf, F.aw, F.ae, F.as, F.an, F.ab, F.at, B, nr);
+    if (!MASKED) ap_rows(ar, F.apx, F.apy, F.apz, B, nr);
     wave_sync();
     const int NI = B.ni, NJ = B.nj, NK = B.nk;
     const int64_t wo = B.wo;
@@ -715,6 +723,7 @@ __global__ __launch_bounds__(kWave) void k_sor(
     k.NK = NK;
     k.HW = NJ + 2;
     k.mHS = mHS;
+    k.nr = nr;
     k.fb0 = face_base(0, NI, NJ, NK);
     k.fb1 = face_base(1, NI, NJ, NK);
     k.fb2 = face_base(2, NI, NJ, NK);
@@ -748,7 +757,7 @@ struct OutK {
     const double *cf;
     T *H;
     uint8_t *BH;
-    int HW, mHS;
+    int HW, mHS, nr;
     bool masked;
     double mx;
 
@@ -763,10 +772,9 @@ struct OutK {
         cell_ijk(d.c.x, i, j, k);
         const int c0 = (j + 1) + HW * (i + 1);
         const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS, sp = ((p + 2) & 3) * mHS;
-        const double apv = MASKED ? d.ap : ap_cell(ar, i, j, k, d.c.y >> 24, *blk);
-        const double *aw = cf, *ae = cf + kMaxN1, *as = cf + 2 * kMaxN1,
-                     *an = cf + 3 * kMaxN1, *ab = cf + 4 * kMaxN1,
-                     *at = cf + 5 * kMaxN1;
+        const double apv = MASKED ? d.ap : ap_cell(ar, nr, i, j, k, d.c.y >> 24, *blk);
+        const double *aw = cf, *ae = cf + nr, *as = cf + 2 * nr,
+                     *an = cf + 3 * nr, *ab = cf + 4 * nr, *at = cf + 5 * nr;
         double r;
         if (MASKED && masked)
             r = laplace_cell<T, double, true>(H + sm, H + s0, H + sp, BH + sm, BH + s0,
@@ -803,12 +811,13 @@ __global__ __launch_bounds__(kWave) void k_outer(
     const int mHS = (t.max_nj + 2) * (t.max_ni + 2);
     SrcTab<T> *tab = (SrcTab<T> *)lds_ou;
     double *cf = (double *)(lds_ou + sizeof(SrcTab<T>));
-    double *ar = cf + 6 * kMaxN1;
-    T *H = (T *)(ar + (MASKED ? 0 : 3 * kMaxN1));
+    const int nr = row_len(t);
+    double *ar = cf + 6 * nr;
+    T *H = (T *)(ar + (MASKED ? 0 : 3 * nr));
     uint8_t *BH = (uint8_t *)(H + 4 * mHS);
     fill_tab<T>(tab, t, B, F.hface, F.hpar, F.hilf, nullptr, 0, m);
-    coef_rows<double>(cf, o.aw, o.ae, o.as, o.an, o.ab, o.at, B);
-    if (!MASKED) ap_rows(ar, F.apx, F.apy, F.apz, B);
+    coef_rows<double>(cf, o.aw, o.ae, o.as, o.an, o.ab, o.at, B, nr);
+    if (!MASKED) ap_rows(ar, F.apx, F.apy, F.apz, B, nr);
     wave_sync();
     const int64_t wo = B.wo;
     OutK<T, MASKED> k;
@@ -824,6 +833,7 @@ __global__ __launch_bounds__(kWave) void k_outer(
     k.BH = BH;
     k.HW = B.nj + 2;
     k.mHS = mHS;
+    k.nr = nr;
     k.masked = MASKED && !(B.flags & 1);
     k.mx = 0.0;
     run_plan<D, M, MASKED, T>(k, items + S.plan_item[M - 1][kPlanOut],
@@ -840,9 +850,9 @@ __global__ __launch_bounds__(kWave) void k_outer(
 // Launchers.  The pipeline shape is a tuning knob, set_plan_shape(): 1 =
 // 4 items of 64 cells in flight, 2 = 2 items of 128 cells (two cells per
 // lane, independent, so their latencies overlap), 3 = 3 items of 128.
-static int g_shape = 2;
+static int g_shape = 1;
 
-void set_plan_shape(int v) { g_shape = (v >= 1 && v <= 3) ? v : 2; }
+void set_plan_shape(int v) { g_shape = (v >= 1 && v <= 3) ? v : 1; }
 int plan_shape() { return g_shape; }
 
 template <typename T>
@@ -866,9 +876,10 @@ void launch_forward(const Topo &t, const Fields<T> &f, Mask m, int rd,
     const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
     const bool masked = m.bp != nullptr;
     const bool lap = !hilf_zero;
-    const size_t lds = tab_bytes<T>() + 6 * kMaxN1 * sizeof(T) + 2 * mRS * sizeof(T) +
+    const size_t nr = row_len(t);
+    const size_t lds = tab_bytes<T>() + 6 * nr * sizeof(T) + 2 * mRS * sizeof(T) +
                        (lap ? 4 * mHS * sizeof(T) + (masked ? 4 * mHS : 0) : 0) +
-                       (lap && !masked ? 3 * kMaxN1 * sizeof(double) : 0);
+                       (lap && !masked ? 3 * nr * sizeof(double) : 0);
 #define MGP_FWD(C, M, L) fwd_launch<T, C, M, L>(t, f, m, rd, write_hi, done, lds, s)
     if (classic) {
         if (masked) { if (lap) MGP_FWD(true, true, true); else MGP_FWD(true, true, false); }
@@ -931,8 +942,9 @@ void launch_sor(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
                 hipStream_t s) {
     const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
     const bool masked = m.bp != nullptr;
-    const size_t lds = tab_bytes<T>() + 6 * kMaxN1 * sizeof(T) + 8 * mHS * sizeof(T) +
-                       (masked ? 8 * mHS : 3 * kMaxN1 * sizeof(double));
+    const size_t nr = row_len(t);
+    const size_t lds = tab_bytes<T>() + 6 * nr * sizeof(T) + 8 * mHS * sizeof(T) +
+                       (masked ? 8 * mHS : 3 * nr * sizeof(double));
     if (masked) {
         if (hilf_zero) sor_launch<T, true, true>(t, f, m, fin, fout, omega, done, lds, s);
         else sor_launch<T, true, false>(t, f, m, fin, fout, omega, done, lds, s);
@@ -963,8 +975,9 @@ void launch_outer(const Topo &t, const Fields<T> &f, Mask m, Outer o,
                   T *rhs_t, hipStream_t s) {
     const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
     const bool masked = m.bp != nullptr;
-    const size_t lds = tab_bytes<T>() + 6 * kMaxN1 * sizeof(double) +
-                       4 * mHS * sizeof(T) + (masked ? 4 * mHS : 3 * kMaxN1 * sizeof(double));
+    const size_t nr = row_len(t);
+    const size_t lds = tab_bytes<T>() + 6 * nr * sizeof(double) +
+                       4 * mHS * sizeof(T) + (masked ? 4 * mHS : 3 * nr * sizeof(double));
     if (masked) out_launch<T, true>(t, f, m, o, rhs_t, lds, s);
     else out_launch<T, false>(t, f, m, o, rhs_t, lds, s);
 }
