@@ -252,37 +252,60 @@ template void launch_lu<float>(const Topo &, const double *, const double *,
                                float *const[7], hipStream_t);
 
 // ---------------------------------------------------------------------
+// Layout changes between the reference layout and WO go through LDS, one
+// workgroup per block: the reference-layout side is read or written in its
+// own order (k fastest, coalesced), the WO side in WO order, and LDS holds
+// up to kStage WO slots of the block at a time (blocks with more cells take
+// several passes).
+constexpr int kStageThreads = 1024;
+constexpr int kStage = 8192;   // 64 KB of doubles
+
 // RHS = fak * div(u, v, w): src/ib/noib_mod.F90:304-354 (noib) and the
 // ghostcell variant src/ib/gc_mod.F90:278-347 (+ fak*SDIV with SDIV = 0,
-// then x BP).  One thread per WO cell; u, v, w in the reference layout.
+// then x BP); u, v, w in the reference layout, RHS in WO.
 template <typename T>
-__global__ void k_divcal(Topo t, const double *u, const double *v,
-                         const double *w, const double *rddx_,
-                         const double *rddy_, const double *rddz_,
-                         const double *bp_sp, double fak, double *rhs64,
-                         T *rhs_t) {
-    const int b = t.blist[blockIdx.y];
+__global__ __launch_bounds__(kStageThreads) void k_divcal(
+    Topo t, const double *u, const double *v, const double *w,
+    const double *rddx_, const double *rddy_, const double *rddz_,
+    const double *bp_sp, double fak, double *rhs64, T *rhs_t) {
+    extern __shared__ double st_dv[];
+    const int b = t.blist[blockIdx.x];
     const Block &B = t.blocks[b];
     const Shape &S = t.shapes[B.shape];
-    const uint32_t *cells = t.cells + S.cell_off;
+    const int *slot_of = t.slot_of + S.slot_off;
+    const int NJ = B.nj, NK = B.nk, n = S.ncell;
     const int64_t kk = B.nk + 4, jj = B.nj + 4;
     const double *rddx = rddx_ + B.o1x + 2, *rddy = rddy_ + B.o1y + 2,
                  *rddz = rddz_ + B.o1z + 2;
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S.ncell;
-         s += gridDim.x * blockDim.x) {
-        int i, j, k;
-        unpack(cells[s], i, j, k);
-        const int64_t c = B.sp + (k + 2) + (j + 2) * kk + (i + 2) * kk * jj;
-        double d = fak * ((u[c] - u[c - kk * jj]) * rddx[i] +
-                          (v[c] - v[c - kk]) * rddy[j] +
-                          (w[c] - w[c - 1]) * rddz[k]);
-        if (bp_sp) {
-            d = d + fak * 0.0;
-            d = bp_sp[c] * d;
+    for (int base = 0; base < n; base += kStage) {
+        const int hi = n < base + kStage ? n : base + kStage;
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const int sl = slot_of[e];   // e = k + NK (j + NJ i)
+            if (sl < base || sl >= hi) continue;
+            const int k = e % NK, j = (e / NK) % NJ, i = e / (NK * NJ);
+            const int64_t c = B.sp + (k + 2) + (j + 2) * kk + (i + 2) * kk * jj;
+            double d = fak * ((u[c] - u[c - kk * jj]) * rddx[i] +
+                              (v[c] - v[c - kk]) * rddy[j] +
+                              (w[c] - w[c - 1]) * rddz[k]);
+            if (bp_sp) {
+                d = d + fak * 0.0;
+                d = bp_sp[c] * d;
+            }
+            st_dv[sl - base] = d;
         }
-        rhs64[B.wo + s] = d;
-        if (rhs_t) rhs_t[B.wo + s] = (T)d;
+        __syncthreads();
+        for (int q = threadIdx.x; q < hi - base; q += blockDim.x) {
+            const double d = st_dv[q];
+            rhs64[B.wo + base + q] = d;
+            if (rhs_t) rhs_t[B.wo + base + q] = (T)d;
+        }
+        __syncthreads();
     }
+}
+
+static size_t stage_bytes(const Topo &t) {
+    const int maxc = t.max_ni * t.max_nj * t.max_nk;
+    return (size_t)(maxc < kStage ? maxc : kStage) * sizeof(double);
 }
 
 template <typename T>
@@ -290,10 +313,8 @@ void launch_divcal(const Topo &t, const double *u, const double *v,
                    const double *w, const double *rddx, const double *rddy,
                    const double *rddz, const double *bp_sp, double fak,
                    double *rhs64, T *rhs_t, hipStream_t s) {
-    const int maxc = t.max_ni * t.max_nj * t.max_nk;
-    dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
-    k_divcal<T><<<grid, kThreads, 0, s>>>(t, u, v, w, rddx, rddy, rddz,
-                                          bp_sp, fak, rhs64, rhs_t);
+    k_divcal<T><<<t.nlist, kStageThreads, stage_bytes(t), s>>>(
+        t, u, v, w, rddx, rddy, rddz, bp_sp, fak, rhs64, rhs_t);
 }
 template void launch_divcal<double>(const Topo &, const double *,
                                     const double *, const double *,
@@ -421,28 +442,36 @@ void launch_check(Outer o, int nouter, int nouter_min, double epcorr,
 }
 
 // ---------------------------------------------------------------------
-// WO -> reference layout (interior only; ghosts are left as they are).
+// WO -> reference layout (interior only; ghosts are left as they are),
+// through LDS as k_divcal.
 template <typename T>
-__global__ void k_wo_to_sp(Topo t, const T *src, double *dst) {
-    const int b = t.blist[blockIdx.y];
+__global__ __launch_bounds__(kStageThreads) void k_wo_to_sp(Topo t, const T *src,
+                                                            double *dst) {
+    extern __shared__ double st_ws[];
+    const int b = t.blist[blockIdx.x];
     const Block &B = t.blocks[b];
     const Shape &S = t.shapes[B.shape];
-    const uint32_t *cells = t.cells + S.cell_off;
+    const int *slot_of = t.slot_of + S.slot_off;
+    const int NJ = B.nj, NK = B.nk, n = S.ncell;
     const int64_t kk = B.nk + 4, jj = B.nj + 4;
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S.ncell;
-         s += gridDim.x * blockDim.x) {
-        int i, j, k;
-        unpack(cells[s], i, j, k);
-        dst[B.sp + (k + 2) + (j + 2) * kk + (i + 2) * kk * jj] =
-            (double)src[B.wo + s];
+    for (int base = 0; base < n; base += kStage) {
+        const int hi = n < base + kStage ? n : base + kStage;
+        for (int q = threadIdx.x; q < hi - base; q += blockDim.x)
+            st_ws[q] = (double)src[B.wo + base + q];
+        __syncthreads();
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const int sl = slot_of[e];
+            if (sl < base || sl >= hi) continue;
+            const int k = e % NK, j = (e / NK) % NJ, i = e / (NK * NJ);
+            dst[B.sp + (k + 2) + (j + 2) * kk + (i + 2) * kk * jj] = st_ws[sl - base];
+        }
+        __syncthreads();
     }
 }
 template <typename T>
 void launch_wo_to_sp(const Topo &t, const T *src, double *dst,
                      hipStream_t s) {
-    const int maxc = t.max_ni * t.max_nj * t.max_nk;
-    dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
-    k_wo_to_sp<T><<<grid, kThreads, 0, s>>>(t, src, dst);
+    k_wo_to_sp<T><<<t.nlist, kStageThreads, stage_bytes(t), s>>>(t, src, dst);
 }
 template void launch_wo_to_sp<double>(const Topo &, const double *, double *,
                                       hipStream_t);
