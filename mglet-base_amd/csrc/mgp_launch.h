@@ -132,8 +132,11 @@ void launch_mgpcorr(const Topo &t, double *u, double *v, double *w,
                     const double *rdy, const double *rdz, double rfak,
                     const double *bp_sp, hipStream_t s);
 
-void launch_copy(const CopyTask *tasks, int ntask, double *const *fields,
-                 int nfield, hipStream_t s);
+// One stage of box copies: `ntask` tasks with `total` elements; first[w]
+// is the task holding element w * kCopyThreads.
+void launch_copy(const CopyTask *tasks, int ntask, const int *first,
+                 int64_t total, double *const *fields, int nfield,
+                 hipStream_t s);
 
 template <typename T>
 void launch_to_ref_factors(const Topo &t, const T *wo, double *ref,
