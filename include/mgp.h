@@ -179,6 +179,26 @@ int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
                       const mgletreal *v, const mgletreal *w,
                       mgletreal fak);
 
+/* The RK stage around the projection (device fields, reference layout).
+ *
+ * tstle4 (src/flow/tstle4_mod.F90:13-120, no immersed boundary): uo, vo, wo
+ * = convection + diffusion with viscosity field g (gmol the molecular
+ * viscosity in its harmonic means; ilesmodel != 0 adds the transpose part
+ * of the stress) - grad p / rho - gradp[3] / rho (gradp may be NULL), zero
+ * outside the loop ranges.  rho is the handle's.  Grids with NOS walls
+ * (Werner-Wengle wall stress) or PAR faces are rejected (MGP_ERR_TOPOLOGY).
+ * Replaces the call at src/flow/timeintegration_mod.F90:98. */
+int mgp_tstle4_device(mgp_handle *h, mgletreal *uo, mgletreal *vo, mgletreal *wo,
+                      const mgletreal *u, const mgletreal *v, const mgletreal *w,
+                      const mgletreal *p, const mgletreal *g, mgletreal gmol,
+                      mgletint ilesmodel, const mgletreal *gradp);
+
+/* rkstep (src/core/rungekutta_mod.F90:315-333) over every stored position
+ * of the handle's 3-D fields: dp = frhs dp + rhsp, p = p + dtfu dp.
+ * Replaces the calls at src/flow/timeintegration_mod.F90:105-107. */
+int mgp_rkstep_device(mgp_handle *h, mgletreal *p, mgletreal *dp,
+                      const mgletreal *rhsp, mgletreal frhs, mgletreal dtfu);
+
 /* Times `niter` inner SIP iterations on the current RHS with HIP events
  * around each kernel; writes the mean kernel durations in ms:
  * ms[0] forward sweep, ms[1] backward sweep, ms[2] one outer update. */

@@ -1744,6 +1744,53 @@ int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
     return MGP_OK;
 }
 
+int mgp_tstle4_device(mgp_handle *h, mgletreal *uo, mgletreal *vo, mgletreal *wo,
+                      const mgletreal *u, const mgletreal *v, const mgletreal *w,
+                      const mgletreal *p, const mgletreal *g, mgletreal gmol,
+                      mgletint ilesmodel, const mgletreal *gradp) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (!uo || !vo || !wo || !u || !v || !w || !p || !g) return fail(MGP_ERR_ARG, "null field");
+    for (int q = 0; q < h->ngrid; ++q)
+        for (int f = 0; f < 6; ++f)
+            if (h->bc[6 * q + f] == 5 || h->bc[6 * q + f] == PAR_BC)
+                return fail(MGP_ERR_TOPOLOGY, "grid %d face %d: tstle4 covers neither NOS walls "
+                            "(wall stress) nor PAR faces", q + 1, f + 1);
+    try {
+        MomArgs a;
+        a.uo = uo; a.vo = vo; a.wo = wo;
+        a.u = u; a.v = v; a.w = w; a.p = p; a.g = g;
+        a.dx = h->d_dx.as<double>(); a.dy = h->d_dy.as<double>(); a.dz = h->d_dz.as<double>();
+        a.ddx = h->d_ddx.as<double>(); a.ddy = h->d_ddy.as<double>(); a.ddz = h->d_ddz.as<double>();
+        a.rdx = h->d_rdx.as<double>(); a.rdy = h->d_rdy.as<double>(); a.rdz = h->d_rdz.as<double>();
+        a.rddx = h->d_rddx.as<double>(); a.rddy = h->d_rddy.as<double>();
+        a.rddz = h->d_rddz.as<double>();
+        a.rho = h->cfg.rho;
+        a.gmol = gmol;
+        a.iles = ilesmodel == 0 ? 0.0 : 1.0;
+        for (int q = 0; q < 3; ++q) a.gp[q] = gradp ? gradp[q] : 0.0;
+        launch_tstle4(h->topo(-1), a, h->stream);
+        HC(hipGetLastError());
+        HC(hipStreamSynchronize(h->stream));
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    return MGP_OK;
+}
+
+int mgp_rkstep_device(mgp_handle *h, mgletreal *p, mgletreal *dp,
+                      const mgletreal *rhsp, mgletreal frhs, mgletreal dtfu) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (!p || !dp || !rhsp) return fail(MGP_ERR_ARG, "null field");
+    try {
+        launch_rkstep(p, dp, rhsp, h->n3d, frhs, dtfu, h->stream);
+        HC(hipGetLastError());
+        HC(hipStreamSynchronize(h->stream));
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    return MGP_OK;
+}
+
 int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms) {
     if (!h || !ms || niter < 1) return fail(MGP_ERR_ARG, "bad argument");
     try {

@@ -132,6 +132,19 @@ void launch_mgpcorr(const Topo &t, double *u, double *v, double *w,
                     const double *rdy, const double *rdz, double rfak,
                     const double *bp_sp, hipStream_t s);
 
+// Arguments of the momentum right-hand side (reference layout, FP64).
+struct MomArgs {
+    double *uo, *vo, *wo;
+    const double *u, *v, *w, *p, *g;
+    const double *dx, *dy, *dz, *ddx, *ddy, *ddz;
+    const double *rdx, *rdy, *rdz, *rddx, *rddy, *rddz;
+    double rho, gmol, iles;
+    double gp[3];      // body-force pressure gradient (gradp)
+};
+void launch_tstle4(const Topo &t, const MomArgs &a, hipStream_t s);
+void launch_rkstep(double *p, double *dp, const double *rhsp, int64_t n,
+                   double frhs, double dtfu, hipStream_t s);
+
 // One stage of box copies: `ntask` tasks with `total` elements; first[w]
 // is the task holding element w * kCopyThreads.
 void launch_copy(const CopyTask *tasks, int ntask, const int *first,

@@ -1,0 +1,200 @@
+// mgp_momentum.hip -- the RK stage around the projection on gfx950: the
+// momentum right-hand side of tstle4 (convection, diffusion, pressure
+// gradient) and the 2N-storage Runge-Kutta update.
+//
+// Both work in the reference layout (the caller's fields), one thread per
+// stored position with k fastest, so every neighbour access of a wave is a
+// contiguous run.  Expressions keep the reference's association order
+// (Fortran evaluates operators of equal precedence left to right; a leading
+// minus applies to the whole product) and the build uses -ffp-contract=off,
+// so the result is bitwise that of the CPU restatement.
+#include <hip/hip_runtime.h>
+
+#include "mgp_launch.h"
+
+namespace mgp {
+
+constexpr int kMomThreads = 256;
+
+__device__ __forceinline__ double gpair(double a, double b, double gmol) {
+    const double s = a + b;
+    return a * b / (s > gmol ? s : gmol);
+}
+
+// tstle4 without immersed boundaries, PAR faces or NOS walls
+// (src/flow/tstle4_mod.F90:13-529): uo = convection (tstle4_kon :139-265)
+// + diffusion (tstle4_diff :268-458, the transpose part of the stress when
+// the LES model is on) - pressure gradient and body-force gradient
+// (tstle4_gradp :461-529); zero outside the loop ranges, as the reference
+// zeroes uo before filling it.  The transporting velocities are u, v, w.
+__global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a) {
+    const int b = t.blist[blockIdx.y];
+    const Block &B = t.blocks[b];
+    const int kk = B.nk + 4, jj = B.nj + 4, ii = B.ni + 4;
+    const int64_t n = (int64_t)kk * jj * ii;
+    const int64_t e = (int64_t)blockIdx.x * kMomThreads + threadIdx.x;
+    if (e >= n) return;
+    const int k = (int)(e % kk) + 1, j = (int)((e / kk) % jj) + 1,
+              i = (int)(e / ((int64_t)kk * jj)) + 1;
+    const int64_t sj = kk, si = (int64_t)kk * jj;
+    const int64_t c = B.sp + e;
+    const double *u = a.u, *v = a.v, *w = a.w, *p = a.p, *gg = a.g;
+    const double *ut = u, *vt = v, *wt = w;
+    // 1-based 1-D metrics of the block
+    const double *dx = a.dx + B.o1x - 1, *ddx = a.ddx + B.o1x - 1;
+    const double *dy = a.dy + B.o1y - 1, *ddy = a.ddy + B.o1y - 1;
+    const double *dz = a.dz + B.o1z - 1, *ddz = a.ddz + B.o1z - 1;
+    const double *rdx = a.rdx + B.o1x - 1, *rddx = a.rddx + B.o1x - 1;
+    const double *rdy = a.rdy + B.o1y - 1, *rddy = a.rddy + B.o1y - 1;
+    const double *rdz = a.rdz + B.o1z - 1, *rddz = a.rddz + B.o1z - 1;
+    // CON (7) and OP1 (3) faces extend the loops
+    const int nfu = B.bc[0] == 3, nbu = B.bc[1] == 7 || B.bc[1] == 3;
+    const int nrv = B.bc[2] == 3, nlv = B.bc[3] == 7 || B.bc[3] == 3;
+    const int nbw = B.bc[4] == 3, ntw = B.bc[5] == 7 || B.bc[5] == 3;
+    const double rho = a.rho, gmol = a.gmol, iles = a.iles;
+    const bool inj = j >= 3 && j <= jj - 2, ink = k >= 3 && k <= kk - 2,
+               ini = i >= 3 && i <= ii - 2;
+#define U(o) u[c + (o)]
+#define V(o) v[c + (o)]
+#define W(o) w[c + (o)]
+#define G(o) gg[c + (o)]
+    double ro = 0.0;
+    if (i >= 3 - nfu && i <= ii - 3 + nbu && inj && ink) {
+        const double ax = ddy[j] * ddz[k], ay = dx[i] * ddz[k], az = dx[i] * ddy[j];
+        const double fe = ax * (ut[c] + (ut[c + si] - ut[c]) * 0.5 * dx[i] / ddx[i + 1]);
+        const double fw = ax * (ut[c - si] + (ut[c] - ut[c - si]) * 0.5 * dx[i - 1] / ddx[i]);
+        const double fn = ay * (vt[c] + vt[c + si]) * 0.5;
+        const double fs = ay * (vt[c - sj] + vt[c - sj + si]) * 0.5;
+        const double ft = az * (wt[c] + wt[c + si]) * 0.5;
+        const double fb = az * (wt[c - 1] + wt[c - 1 + si]) * 0.5;
+        double qe = 0.5 * fe * (U(0) + U(si));
+        double qw = 0.5 * fw * (U(-si) + U(0));
+        double qn = 0.5 * fn * (U(0) + U(sj));
+        double qs = 0.5 * fs * (U(-sj) + U(0));
+        double qt = 0.5 * ft * (U(0) + U(1));
+        double qb = 0.5 * fb * (U(-1) + U(0));
+        ro = -((qe - qw + qn - qs + qt - qb) * rdx[i] * rddy[j] * rddz[k]);
+        const double ge = G(si), gw = G(0);
+        const double gn = gpair(G(0), G(sj), gmol) + gpair(G(si), G(si + sj), gmol);
+        const double gs = gpair(G(-sj), G(0), gmol) + gpair(G(si - sj), G(si), gmol);
+        const double gt = gpair(G(0), G(1), gmol) + gpair(G(si), G(si + 1), gmol);
+        const double gb = gpair(G(-1), G(0), gmol) + gpair(G(si - 1), G(si), gmol);
+        qe = -(ge * ax * rddx[i + 1] * (U(si) - U(0)));
+        qw = -(gw * ax * rddx[i] * (U(0) - U(-si)));
+        qn = -(gn * ay * rdy[j] * (U(sj) - U(0)));
+        qs = -(gs * ay * rdy[j - 1] * (U(0) - U(-sj)));
+        qt = -(gt * az * rdz[k] * (U(1) - U(0)));
+        qb = -(gb * az * rdz[k - 1] * (U(0) - U(-1)));
+        const double st = ((ge * (U(si) - U(0)) * rddx[i + 1]) - (gw * (U(0) - U(-si)) * rddx[i])) * ax
+                          + ((gn * (V(si) - V(0))) - (gs * (V(si - sj) - V(-sj)))) * ddz[k]
+                          + ((gt * (W(si) - W(0))) - (gb * (W(si - 1) - W(-1)))) * ddy[j];
+        const double qc = st * iles;
+        const double fak = 1.0 / rho * rddy[j] * rdx[i] * rddz[k];
+        ro = ro - fak * (qe - qw + qn - qs + qt - qb - qc);
+        ro = ro - 1.0 / (rho * dx[i]) * (p[c + si] - p[c] + a.gp[0] * dx[i]);
+    }
+    a.uo[c] = ro;
+    ro = 0.0;
+    if (ini && j >= 3 - nrv && j <= jj - 3 + nlv && ink) {
+        const double ax = dy[j] * ddz[k], ay = ddx[i] * ddz[k], az = ddx[i] * dy[j];
+        const double fe = ax * (ut[c] + ut[c + sj]) * 0.5;
+        const double fw = ax * (ut[c - si] + ut[c - si + sj]) * 0.5;
+        const double fn = ay * (vt[c] + (vt[c + sj] - vt[c]) * 0.5 * dy[j] / ddy[j + 1]);
+        const double fs = ay * (vt[c - sj] + (vt[c] - vt[c - sj]) * 0.5 * dy[j - 1] / ddy[j]);
+        const double ft = az * (wt[c] + wt[c + sj]) * 0.5;
+        const double fb = az * (wt[c - 1] + wt[c - 1 + sj]) * 0.5;
+        double qe = 0.5 * fe * (V(0) + V(si));
+        double qw = 0.5 * fw * (V(-si) + V(0));
+        double qn = 0.5 * fn * (V(0) + V(sj));
+        double qs = 0.5 * fs * (V(-sj) + V(0));
+        double qt = 0.5 * ft * (V(0) + V(1));
+        double qb = 0.5 * fb * (V(-1) + V(0));
+        ro = -((qe - qw + qn - qs + qt - qb) * rddx[i] * rdy[j] * rddz[k]);
+        const double ge = gpair(G(0), G(si), gmol) + gpair(G(sj), G(sj + si), gmol);
+        const double gw = gpair(G(-si), G(0), gmol) + gpair(G(sj - si), G(sj), gmol);
+        const double gn = G(sj), gs = G(0);
+        const double gt = gpair(G(0), G(1), gmol) + gpair(G(sj), G(sj + 1), gmol);
+        const double gb = gpair(G(-1), G(0), gmol) + gpair(G(sj - 1), G(sj), gmol);
+        qe = -(ge * ax * rdx[i] * (V(si) - V(0)));
+        qw = -(gw * ax * rdx[i - 1] * (V(0) - V(-si)));
+        qn = -(gn * ay * rddy[j + 1] * (V(sj) - V(0)));
+        qs = -(gs * ay * rddy[j] * (V(0) - V(-sj)));
+        qt = -(gt * az * rdz[k] * (V(1) - V(0)));
+        qb = -(gb * az * rdz[k - 1] * (V(0) - V(-1)));
+        const double st = ((ge * (U(sj) - U(0))) - (gw * (U(sj - si) - U(-si)))) * ddz[k]
+                          + ((gn * (V(sj) - V(0)) * rddy[j + 1]) - (gs * (V(0) - V(-sj)) * rddy[j])) * ay
+                          + ((gt * (W(sj) - W(0))) - (gb * (W(sj - 1) - W(-1)))) * ddx[i];
+        const double qc = st * iles;
+        const double fak = 1.0 / rho * rddx[i] * rdy[j] * rddz[k];
+        ro = ro - fak * (qe - qw + qn - qs + qt - qb - qc);
+        ro = ro - 1.0 / (rho * dy[j]) * (p[c + sj] - p[c] + a.gp[1] * dy[j]);
+    }
+    a.vo[c] = ro;
+    ro = 0.0;
+    if (ini && inj && k >= 3 - nbw && k <= kk - 3 + ntw) {
+        const double ax = ddy[j] * dz[k], ay = ddx[i] * dz[k], az = ddx[i] * ddy[j];
+        const double fe = ax * (ut[c] + ut[c + 1]) * 0.5;
+        const double fw = ax * (ut[c - si] + ut[c - si + 1]) * 0.5;
+        const double fn = ay * (vt[c] + vt[c + 1]) * 0.5;
+        const double fs = ay * (vt[c - sj] + vt[c - sj + 1]) * 0.5;
+        const double ft = az * (wt[c] + (wt[c + 1] - wt[c]) * 0.5 * dz[k] / ddz[k + 1]);
+        const double fb = az * (wt[c - 1] + (wt[c] - wt[c - 1]) * 0.5 * dz[k - 1] / ddz[k]);
+        double qe = 0.5 * fe * (W(0) + W(si));
+        double qw = 0.5 * fw * (W(-si) + W(0));
+        double qn = 0.5 * fn * (W(0) + W(sj));
+        double qs = 0.5 * fs * (W(-sj) + W(0));
+        double qt = 0.5 * ft * (W(0) + W(1));
+        double qb = 0.5 * fb * (W(-1) + W(0));
+        ro = -((qe - qw + qn - qs + qt - qb) * rddx[i] * rddy[j] * rdz[k]);
+        const double ge = gpair(G(0), G(si), gmol) + gpair(G(1), G(1 + si), gmol);
+        const double gw = gpair(G(-si), G(0), gmol) + gpair(G(1 - si), G(1), gmol);
+        const double gn = gpair(G(0), G(sj), gmol) + gpair(G(1), G(1 + sj), gmol);
+        const double gs = gpair(G(-sj), G(0), gmol) + gpair(G(1 - sj), G(1), gmol);
+        const double gt = G(1), gb = G(0);
+        qe = -(ge * ax * rdx[i] * (W(si) - W(0)));
+        qw = -(gw * ax * rdx[i - 1] * (W(0) - W(-si)));
+        qn = -(gn * ay * rdy[j] * (W(sj) - W(0)));
+        qs = -(gs * ay * rdy[j - 1] * (W(0) - W(-sj)));
+        qt = -(gt * az * rddz[k + 1] * (W(1) - W(0)));
+        qb = -(gb * az * rddz[k] * (W(0) - W(-1)));
+        const double st = ((ge * (U(1) - U(0))) - (gw * (U(1 - si) - U(-si)))) * ddy[j]
+                          + ((gn * (V(1) - V(0))) - (gs * (V(1 - sj) - V(-sj)))) * ddx[i]
+                          + ((gt * (W(1) - W(0)) * rddz[k + 1]) - (gb * (W(0) - W(-1)) * rddz[k])) * az;
+        const double qc = st * iles;
+        const double fak = 1.0 / rho * rddx[i] * rddy[j] * rdz[k];
+        ro = ro - fak * (qe - qw + qn - qs + qt - qb - qc);
+        ro = ro - 1.0 / (rho * dz[k]) * (p[c + 1] - p[c] + a.gp[2] * dz[k]);
+    }
+    a.wo[c] = ro;
+#undef U
+#undef V
+#undef W
+#undef G
+}
+
+void launch_tstle4(const Topo &t, const MomArgs &a, hipStream_t s) {
+    const int maxc = (t.max_ni + 4) * (t.max_nj + 4) * (t.max_nk + 4);
+    dim3 grid((maxc + kMomThreads - 1) / kMomThreads, t.nlist);
+    k_tstle4<<<grid, kMomThreads, 0, s>>>(t, a);
+}
+
+// src/core/rungekutta_mod.F90:315-333 over every stored position
+__global__ void k_rkstep(double *p, double *dp, const double *rhsp,
+                         int64_t n, double frhs, double dtfu) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const double d = frhs * dp[e] + rhsp[e];
+        dp[e] = d;
+        p[e] = p[e] + dtfu * d;
+    }
+}
+
+void launch_rkstep(double *p, double *dp, const double *rhsp, int64_t n,
+                   double frhs, double dtfu, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t nb = (n + kMomThreads - 1) / kMomThreads;
+    if (nb > 65536) nb = 65536;
+    k_rkstep<<<(unsigned)nb, kMomThreads, 0, s>>>(p, dp, rhsp, n, frhs, dtfu);
+}
+
+}  // namespace mgp

@@ -160,6 +160,28 @@ class PressureSolver:
                    "mgp_divcal_device")
         return div
 
+    def tstle4(self, u, v, w, p, g, gmol, ilesmodel=0, gradp=None):
+        """Momentum right-hand side of one RK stage (tstle4, no immersed
+        boundary) on CUDA float64 tensors; returns (uo, vo, wo)."""
+        import torch
+        self._bind_stream()
+        uo, vo, wo = (torch.empty_like(u) for _ in range(3))
+        gp = None if gradp is None else _ptr(np.asarray(gradp, dtype=np.float64))
+        _lib.check(self.lib.mgp_tstle4_device(self.h, uo.data_ptr(), vo.data_ptr(),
+                                              wo.data_ptr(), u.data_ptr(), v.data_ptr(),
+                                              w.data_ptr(), p.data_ptr(), g.data_ptr(),
+                                              gmol, ilesmodel, gp),
+                   "mgp_tstle4_device")
+        return uo, vo, wo
+
+    def rkstep(self, p, dp, rhsp, frhs, dtfu):
+        """2N-storage Runge-Kutta update in place: dp = frhs dp + rhsp,
+        p += dtfu dp (CUDA float64 tensors of the 3-D length)."""
+        self._bind_stream()
+        _lib.check(self.lib.mgp_rkstep_device(self.h, p.data_ptr(), dp.data_ptr(),
+                                              rhsp.data_ptr(), frhs, dtfu),
+                   "mgp_rkstep_device")
+
     def time_kernels(self, niter=10):
         ms = np.zeros(3)
         _lib.check(self.lib.mgp_time_kernels(self.h, niter, _ptr(ms)), "mgp_time_kernels")

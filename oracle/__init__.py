@@ -166,6 +166,27 @@ class Oracle:
                            _p(bp) if bp is not None else None,
                            _p(sdiv) if sdiv is not None else None)
 
+    def tstle4(self, u, v, w, p, g_visc, rho, gmol, ilesmodel=0, gradp=None):
+        """Momentum right-hand side (src/flow/tstle4_mod.F90:13-529, noib):
+        returns (uo, vo, wo)."""
+        g = self.g
+        uo, vo, wo = g.zeros(), g.zeros(), g.zeros()
+        gp = np.asarray(gradp, dtype=np.float64) if gradp is not None else None
+        rc = self.lib.og_tstle4(C.byref(self.topo), _p(uo), _p(vo), _p(wo), _p(u), _p(v),
+                                _p(w), _p(p), _p(g_visc), C.c_double(rho), C.c_double(gmol),
+                                ilesmodel, _p(gp) if gp is not None else None,
+                                _p(g.dx), _p(g.dy), _p(g.dz), _p(g.ddx), _p(g.ddy), _p(g.ddz),
+                                _p(g.rdx), _p(g.rdy), _p(g.rdz), _p(g.rddx), _p(g.rddy),
+                                _p(g.rddz))
+        if rc != 0:
+            raise ValueError("tstle4: NOS wall or PAR faces are not restated")
+        return uo, vo, wo
+
+    def rkstep(self, p, dp, rhsp, frhs, dtfu):
+        """src/core/rungekutta_mod.F90:315-333, in place on p and dp."""
+        self.lib.og_rkstep(C.c_int64(p.size), _p(p), _p(dp), _p(rhsp), C.c_double(frhs),
+                           C.c_double(dtfu))
+
     def mgpcorr(self, u, v, w, p, dp, rfak, bp=None):
         g = self.g
         self.lib.og_mgpcorr(C.byref(self.topo), _p(u), _p(v), _p(w), _p(p), _p(dp),

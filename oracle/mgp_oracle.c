@@ -1507,3 +1507,195 @@ double og_time_sip(const og_topo *t, const og_conn *c, int nconn, int niter,
 #endif
     return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
 }
+
+/* ------------------------------------------------------------------ */
+/* Momentum right-hand side of one RK stage, tstle4 without immersed
+ * boundaries and without PAR or slip faces (src/flow/tstle4_mod.F90:13-120):
+ * uo = convection (tstle4_kon :139-265) + diffusion (tstle4_diff :268-458)
+ * - pressure gradient (tstle4_gradp :461-529).  ut = u etc. (noib).  Each
+ * expression keeps the reference's association order (Fortran evaluates
+ * operators of equal precedence left to right; a leading minus applies to
+ * the whole product).  Returns -1 for a face type the restatement does not
+ * cover: NOS walls (5), whose Werner-Wengle wall stress (swcle3d,
+ * :1756-1845) is not restated, and PAR faces (8, tstle4_par). */
+static double gpair(double a, double b, double gmol)
+{
+    const double s = a + b;
+    return a * b / (s > gmol ? s : gmol);
+}
+
+int og_tstle4(const og_topo *t, double *uo_, double *vo_, double *wo_,
+              const double *u_, const double *v_, const double *w_,
+              const double *p_, const double *g_, double rho, double gmol,
+              int ilesmodel, const double *gradp,
+              const double *dx_, const double *dy_, const double *dz_,
+              const double *ddx_, const double *ddy_, const double *ddz_,
+              const double *rdx_, const double *rdy_, const double *rdz_,
+              const double *rddx_, const double *rddy_, const double *rddz_)
+{
+    for (int g = 1; g <= t->ngrid; ++g)
+        for (int f = 0; f < 6; ++f) {
+            const int c = t->bc[6 * (g - 1) + f];
+            if (c == 5 || c == 8) return -1;
+        }
+    const double iles = ilesmodel == 0 ? 0.0 : 1.0;
+    for (int g = 1; g <= t->ngrid; ++g) {
+        GDIMS(t, g);
+        const int64_t o3 = t->ip3[g - 1];
+        double *uo = uo_ + o3, *vo = vo_ + o3, *wo = wo_ + o3;
+        const double *u = u_ + o3, *v = v_ + o3, *w = w_ + o3;
+        const double *ut = u, *vt = v, *wt = w;
+        const double *p = p_ + o3, *gg = g_ + o3;
+        const double *dx = dx_ + t->ip1x[g - 1] - 1, *ddx = ddx_ + t->ip1x[g - 1] - 1;
+        const double *dy = dy_ + t->ip1y[g - 1] - 1, *ddy = ddy_ + t->ip1y[g - 1] - 1;
+        const double *dz = dz_ + t->ip1z[g - 1] - 1, *ddz = ddz_ + t->ip1z[g - 1] - 1;
+        const double *rdx = rdx_ + t->ip1x[g - 1] - 1, *rddx = rddx_ + t->ip1x[g - 1] - 1;
+        const double *rdy = rdy_ + t->ip1y[g - 1] - 1, *rddy = rddy_ + t->ip1y[g - 1] - 1;
+        const double *rdz = rdz_ + t->ip1z[g - 1] - 1, *rddz = rddz_ + t->ip1z[g - 1] - 1;
+        const int32_t *bc = t->bc + 6 * (g - 1);
+        /* nfro..ntop = faces 1..6; CON = 7 and OP1 = 3 extend the loops */
+        const int nfu = bc[0] == 3, nbu = bc[1] == 7 || bc[1] == 3;
+        const int nrv = bc[2] == 3, nlv = bc[3] == 7 || bc[3] == 3;
+        const int nbw = bc[4] == 3, ntw = bc[5] == 7 || bc[5] == 3;
+        const double gpx = gradp ? gradp[0] : 0.0, gpy = gradp ? gradp[1] : 0.0,
+                     gpz = gradp ? gradp[2] : 0.0;
+        for (int64_t c = 0; c < (int64_t)ii * jj * kk; ++c) uo[c] = vo[c] = wo[c] = 0.0;
+#define G(k, j, i) gg[IX(k, j, i)]
+        /* u momentum */
+        for (int i = 3 - nfu; i <= ii - 3 + nbu; ++i)
+            for (int j = 3; j <= jj - 2; ++j)
+                for (int k = 3; k <= kk - 2; ++k) {
+                    const int64_t c = IX(k, j, i);
+                    double ax = ddy[j] * ddz[k], ay = dx[i] * ddz[k], az = dx[i] * ddy[j];
+                    double fe = ax * (ut[c] + (ut[IX(k, j, i + 1)] - ut[c]) * 0.5 * dx[i] / ddx[i + 1]);
+                    double fw = ax * (ut[IX(k, j, i - 1)] + (ut[c] - ut[IX(k, j, i - 1)]) * 0.5 * dx[i - 1] / ddx[i]);
+                    double fn = ay * (vt[c] + vt[IX(k, j, i + 1)]) * 0.5;
+                    double fs = ay * (vt[IX(k, j - 1, i)] + vt[IX(k, j - 1, i + 1)]) * 0.5;
+                    double ft = az * (wt[c] + wt[IX(k, j, i + 1)]) * 0.5;
+                    double fb = az * (wt[IX(k - 1, j, i)] + wt[IX(k - 1, j, i + 1)]) * 0.5;
+                    double qe = 0.5 * fe * (u[c] + u[IX(k, j, i + 1)]);
+                    double qw = 0.5 * fw * (u[IX(k, j, i - 1)] + u[c]);
+                    double qn = 0.5 * fn * (u[c] + u[IX(k, j + 1, i)]);
+                    double qs = 0.5 * fs * (u[IX(k, j - 1, i)] + u[c]);
+                    double qt = 0.5 * ft * (u[c] + u[IX(k + 1, j, i)]);
+                    double qb = 0.5 * fb * (u[IX(k - 1, j, i)] + u[c]);
+                    uo[c] = -((qe - qw + qn - qs + qt - qb) * rdx[i] * rddy[j] * rddz[k]);
+                    /* diffusion */
+                    const double ge = G(k, j, i + 1), gw = G(k, j, i);
+                    const double gn = gpair(G(k, j, i), G(k, j + 1, i), gmol) + gpair(G(k, j, i + 1), G(k, j + 1, i + 1), gmol);
+                    const double gs = gpair(G(k, j - 1, i), G(k, j, i), gmol) + gpair(G(k, j - 1, i + 1), G(k, j, i + 1), gmol);
+                    const double gt = gpair(G(k, j, i), G(k + 1, j, i), gmol) + gpair(G(k, j, i + 1), G(k + 1, j, i + 1), gmol);
+                    const double gb = gpair(G(k - 1, j, i), G(k, j, i), gmol) + gpair(G(k - 1, j, i + 1), G(k, j, i + 1), gmol);
+                    qe = -(ge * ax * rddx[i + 1] * (u[IX(k, j, i + 1)] - u[c]));
+                    qw = -(gw * ax * rddx[i] * (u[c] - u[IX(k, j, i - 1)]));
+                    qn = -(gn * ay * rdy[j] * (u[IX(k, j + 1, i)] - u[c]));
+                    qs = -(gs * ay * rdy[j - 1] * (u[c] - u[IX(k, j - 1, i)]));
+                    qt = -(gt * az * rdz[k] * (u[IX(k + 1, j, i)] - u[c]));
+                    qb = -(gb * az * rdz[k - 1] * (u[c] - u[IX(k - 1, j, i)]));
+                    const double st = ((ge * (u[IX(k, j, i + 1)] - u[c]) * rddx[i + 1])
+                                       - (gw * (u[c] - u[IX(k, j, i - 1)]) * rddx[i])) * ax
+                                      + ((gn * (v[IX(k, j, i + 1)] - v[c]))
+                                         - (gs * (v[IX(k, j - 1, i + 1)] - v[IX(k, j - 1, i)]))) * ddz[k]
+                                      + ((gt * (w[IX(k, j, i + 1)] - w[c]))
+                                         - (gb * (w[IX(k - 1, j, i + 1)] - w[IX(k - 1, j, i)]))) * ddy[j];
+                    const double qc = st * iles;
+                    const double fak = 1.0 / rho * rddy[j] * rdx[i] * rddz[k];
+                    uo[c] = uo[c] - fak * (qe - qw + qn - qs + qt - qb - qc);
+                    uo[c] = uo[c] - 1.0 / (rho * dx[i]) * (p[IX(k, j, i + 1)] - p[c] + gpx * dx[i]);
+                }
+        /* v momentum */
+        for (int i = 3; i <= ii - 2; ++i)
+            for (int j = 3 - nrv; j <= jj - 3 + nlv; ++j)
+                for (int k = 3; k <= kk - 2; ++k) {
+                    const int64_t c = IX(k, j, i);
+                    double ax = dy[j] * ddz[k], ay = ddx[i] * ddz[k], az = ddx[i] * dy[j];
+                    double fe = ax * (ut[c] + ut[IX(k, j + 1, i)]) * 0.5;
+                    double fw = ax * (ut[IX(k, j, i - 1)] + ut[IX(k, j + 1, i - 1)]) * 0.5;
+                    double fn = ay * (vt[c] + (vt[IX(k, j + 1, i)] - vt[c]) * 0.5 * dy[j] / ddy[j + 1]);
+                    double fs = ay * (vt[IX(k, j - 1, i)] + (vt[c] - vt[IX(k, j - 1, i)]) * 0.5 * dy[j - 1] / ddy[j]);
+                    double ft = az * (wt[c] + wt[IX(k, j + 1, i)]) * 0.5;
+                    double fb = az * (wt[IX(k - 1, j, i)] + wt[IX(k - 1, j + 1, i)]) * 0.5;
+                    double qe = 0.5 * fe * (v[c] + v[IX(k, j, i + 1)]);
+                    double qw = 0.5 * fw * (v[IX(k, j, i - 1)] + v[c]);
+                    double qn = 0.5 * fn * (v[c] + v[IX(k, j + 1, i)]);
+                    double qs = 0.5 * fs * (v[IX(k, j - 1, i)] + v[c]);
+                    double qt = 0.5 * ft * (v[c] + v[IX(k + 1, j, i)]);
+                    double qb = 0.5 * fb * (v[IX(k - 1, j, i)] + v[c]);
+                    vo[c] = -((qe - qw + qn - qs + qt - qb) * rddx[i] * rdy[j] * rddz[k]);
+                    const double ge = gpair(G(k, j, i), G(k, j, i + 1), gmol) + gpair(G(k, j + 1, i), G(k, j + 1, i + 1), gmol);
+                    const double gw = gpair(G(k, j, i - 1), G(k, j, i), gmol) + gpair(G(k, j + 1, i - 1), G(k, j + 1, i), gmol);
+                    const double gn = G(k, j + 1, i), gs = G(k, j, i);
+                    const double gt = gpair(G(k, j, i), G(k + 1, j, i), gmol) + gpair(G(k, j + 1, i), G(k + 1, j + 1, i), gmol);
+                    const double gb = gpair(G(k - 1, j, i), G(k, j, i), gmol) + gpair(G(k - 1, j + 1, i), G(k, j + 1, i), gmol);
+                    qe = -(ge * ax * rdx[i] * (v[IX(k, j, i + 1)] - v[c]));
+                    qw = -(gw * ax * rdx[i - 1] * (v[c] - v[IX(k, j, i - 1)]));
+                    qn = -(gn * ay * rddy[j + 1] * (v[IX(k, j + 1, i)] - v[c]));
+                    qs = -(gs * ay * rddy[j] * (v[c] - v[IX(k, j - 1, i)]));
+                    qt = -(gt * az * rdz[k] * (v[IX(k + 1, j, i)] - v[c]));
+                    qb = -(gb * az * rdz[k - 1] * (v[c] - v[IX(k - 1, j, i)]));
+                    const double st = ((ge * (u[IX(k, j + 1, i)] - u[c]))
+                                       - (gw * (u[IX(k, j + 1, i - 1)] - u[IX(k, j, i - 1)]))) * ddz[k]
+                                      + ((gn * (v[IX(k, j + 1, i)] - v[c]) * rddy[j + 1])
+                                         - (gs * (v[c] - v[IX(k, j - 1, i)]) * rddy[j])) * ay
+                                      + ((gt * (w[IX(k, j + 1, i)] - w[c]))
+                                         - (gb * (w[IX(k - 1, j + 1, i)] - w[IX(k - 1, j, i)]))) * ddx[i];
+                    const double qc = st * iles;
+                    const double fak = 1.0 / rho * rddx[i] * rdy[j] * rddz[k];
+                    vo[c] = vo[c] - fak * (qe - qw + qn - qs + qt - qb - qc);
+                    vo[c] = vo[c] - 1.0 / (rho * dy[j]) * (p[IX(k, j + 1, i)] - p[c] + gpy * dy[j]);
+                }
+        /* w momentum */
+        for (int i = 3; i <= ii - 2; ++i)
+            for (int j = 3; j <= jj - 2; ++j)
+                for (int k = 3 - nbw; k <= kk - 3 + ntw; ++k) {
+                    const int64_t c = IX(k, j, i);
+                    double ax = ddy[j] * dz[k], ay = ddx[i] * dz[k], az = ddx[i] * ddy[j];
+                    double fe = ax * (ut[c] + ut[IX(k + 1, j, i)]) * 0.5;
+                    double fw = ax * (ut[IX(k, j, i - 1)] + ut[IX(k + 1, j, i - 1)]) * 0.5;
+                    double fn = ay * (vt[c] + vt[IX(k + 1, j, i)]) * 0.5;
+                    double fs = ay * (vt[IX(k, j - 1, i)] + vt[IX(k + 1, j - 1, i)]) * 0.5;
+                    double ft = az * (wt[c] + (wt[IX(k + 1, j, i)] - wt[c]) * 0.5 * dz[k] / ddz[k + 1]);
+                    double fb = az * (wt[IX(k - 1, j, i)] + (wt[c] - wt[IX(k - 1, j, i)]) * 0.5 * dz[k - 1] / ddz[k]);
+                    double qe = 0.5 * fe * (w[c] + w[IX(k, j, i + 1)]);
+                    double qw = 0.5 * fw * (w[IX(k, j, i - 1)] + w[c]);
+                    double qn = 0.5 * fn * (w[c] + w[IX(k, j + 1, i)]);
+                    double qs = 0.5 * fs * (w[IX(k, j - 1, i)] + w[c]);
+                    double qt = 0.5 * ft * (w[c] + w[IX(k + 1, j, i)]);
+                    double qb = 0.5 * fb * (w[IX(k - 1, j, i)] + w[c]);
+                    wo[c] = -((qe - qw + qn - qs + qt - qb) * rddx[i] * rddy[j] * rdz[k]);
+                    const double ge = gpair(G(k, j, i), G(k, j, i + 1), gmol) + gpair(G(k + 1, j, i), G(k + 1, j, i + 1), gmol);
+                    const double gw = gpair(G(k, j, i - 1), G(k, j, i), gmol) + gpair(G(k + 1, j, i - 1), G(k + 1, j, i), gmol);
+                    const double gn = gpair(G(k, j, i), G(k, j + 1, i), gmol) + gpair(G(k + 1, j, i), G(k + 1, j + 1, i), gmol);
+                    const double gs = gpair(G(k, j - 1, i), G(k, j, i), gmol) + gpair(G(k + 1, j - 1, i), G(k + 1, j, i), gmol);
+                    const double gt = G(k + 1, j, i), gb = G(k, j, i);
+                    qe = -(ge * ax * rdx[i] * (w[IX(k, j, i + 1)] - w[c]));
+                    qw = -(gw * ax * rdx[i - 1] * (w[c] - w[IX(k, j, i - 1)]));
+                    qn = -(gn * ay * rdy[j] * (w[IX(k, j + 1, i)] - w[c]));
+                    qs = -(gs * ay * rdy[j - 1] * (w[c] - w[IX(k, j - 1, i)]));
+                    qt = -(gt * az * rddz[k + 1] * (w[IX(k + 1, j, i)] - w[c]));
+                    qb = -(gb * az * rddz[k] * (w[c] - w[IX(k - 1, j, i)]));
+                    const double st = ((ge * (u[IX(k + 1, j, i)] - u[c]))
+                                       - (gw * (u[IX(k + 1, j, i - 1)] - u[IX(k, j, i - 1)]))) * ddy[j]
+                                      + ((gn * (v[IX(k + 1, j, i)] - v[c]))
+                                         - (gs * (v[IX(k + 1, j - 1, i)] - v[IX(k, j - 1, i)]))) * ddx[i]
+                                      + ((gt * (w[IX(k + 1, j, i)] - w[c]) * rddz[k + 1])
+                                         - (gb * (w[c] - w[IX(k - 1, j, i)]) * rddz[k])) * az;
+                    const double qc = st * iles;
+                    const double fak = 1.0 / rho * rddx[i] * rddy[j] * rdz[k];
+                    wo[c] = wo[c] - fak * (qe - qw + qn - qs + qt - qb - qc);
+                    wo[c] = wo[c] - 1.0 / (rho * dz[k]) * (p[IX(k + 1, j, i)] - p[c] + gpz * dz[k]);
+                }
+#undef G
+    }
+    return 0;
+}
+
+/* src/core/rungekutta_mod.F90:315-333 */
+void og_rkstep(int64_t n, double *p, double *dp, const double *rhsp,
+               double frhs, double dtfu)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        dp[i] = frhs * dp[i] + rhsp[i];
+        p[i] = p[i] + dtfu * dp[i];
+    }
+}

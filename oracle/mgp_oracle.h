@@ -193,6 +193,25 @@ int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
                double *dp, double *hilf, double *rhs, double *res,
                og_stats *st);
 
+/* Momentum right-hand side of one RK stage without immersed boundaries
+ * (src/flow/tstle4_mod.F90:13-529: convection, diffusion with viscosity
+ * field g, pressure gradient with the optional body-force gradient gradp[3]
+ * on every grid).  uo, vo, wo are zeroed and then filled.  Returns -1 when a
+ * grid has a NOS wall (5: Werner-Wengle wall stress, swcle3d) or a PAR face
+ * (8: tstle4_par), which this restatement omits. */
+int og_tstle4(const og_topo *t, double *uo, double *vo, double *wo,
+              const double *u, const double *v, const double *w,
+              const double *p, const double *g, double rho, double gmol,
+              int ilesmodel, const double *gradp,
+              const double *dx, const double *dy, const double *dz,
+              const double *ddx, const double *ddy, const double *ddz,
+              const double *rdx, const double *rdy, const double *rdz,
+              const double *rddx, const double *rddy, const double *rddz);
+
+/* src/core/rungekutta_mod.F90:315-333: dp = frhs dp + rhsp; p += dtfu dp */
+void og_rkstep(int64_t n, double *p, double *dp, const double *rhsp,
+               double frhs, double dtfu);
+
 /* Timed inner-iteration loop used as the CPU baseline: `niter` fused
  * (laplacephi + sipiter1 + conn + sipiter2 + conn) SIP iterations over all
  * grids with `nthreads` OpenMP threads.  Returns wall seconds. */

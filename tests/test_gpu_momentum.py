@@ -1,0 +1,173 @@
+"""GPU RK stage around the projection (tstle4, rkstep) against the oracle,
+and the Taylor-Green case: one self-periodic 32^3 block on [0, 2 pi]^3,
+u = sin x cos y cos z, v = -cos x sin y cos z, w = 0 plus N(0, 1e-3) noise
+per face value (seed 1), one Williamson RK3 stage at dt = 1e-3 with
+nu = gmol / rho = 1e-2, SIP to a 1e-10 relative residual.
+
+FP64 results are bitwise equal: both sides evaluate every expression in the
+reference's order without FMA."""
+import numpy as np
+import pytest
+
+from cases import topology, velocity
+from mgp.grids import FIX, uniform_box
+
+pytestmark = pytest.mark.gpu
+
+
+def _cuda(*arrs):
+    import torch
+    return [torch.from_numpy(a.copy()).cuda() for a in arrs]
+
+
+def _host(*ts):
+    return [t.cpu().numpy() for t in ts]
+
+
+def _case(name):
+    if name == "fixwalls":
+        return uniform_box((2, 1, 2), (6, 8, 6), periodic=(False, True, False), wall=FIX)
+    return topology(name)
+
+
+@pytest.mark.parametrize("name", ["self12", "box2", "odd", "fixwalls"])
+@pytest.mark.parametrize("iles", [0, 1])
+def test_tstle4_bitwise(oracle_mod, name, iles):
+    import mgp
+    g = _case(name)
+    o = oracle_mod.Oracle(g)
+    u, v, w = velocity(g, o, 21)
+    rng = np.random.default_rng(22)
+    p = rng.standard_normal(g.n3d)
+    visc = rng.uniform(1e-3, 2e-2, g.n3d)
+    gradp = [0.3, -0.2, 0.1]
+    rho, gmol = 1.2, 1e-3
+    ref = o.tstle4(u, v, w, p, visc, rho, gmol, ilesmodel=iles, gradp=gradp)
+    s = mgp.PressureSolver(g, rho=rho)
+    got = _host(*s.tstle4(*_cuda(u, v, w, p, visc), gmol, ilesmodel=iles, gradp=gradp))
+    for a, b, nm in zip(ref, got, ("uo", "vo", "wo")):
+        assert np.array_equal(a, b), nm
+    assert np.abs(ref[0]).max() > 0
+
+
+def test_rkstep_bitwise(oracle_mod):
+    import mgp
+    g = topology("box2")
+    o = oracle_mod.Oracle(g)
+    rng = np.random.default_rng(5)
+    p, dp, rhs = (rng.standard_normal(g.n3d) for _ in range(3))
+    s = mgp.PressureSolver(g)
+    tp, tdp, trhs = _cuda(p, dp, rhs)
+    for frhs, dtfu in ((0.0, 1e-3 / 3), (-5.0 / 9.0, 1e-3 * 15 / 16)):
+        o.rkstep(p, dp, rhs, frhs, dtfu)
+        s.rkstep(tp, tdp, trhs, frhs, dtfu)
+        assert np.array_equal(p, _host(tp)[0]) and np.array_equal(dp, _host(tdp)[0])
+
+
+def test_tstle4_rejects_nos_walls():
+    import mgp
+    g = topology("walls")        # NOS walls: the wall stress is not built
+    s = mgp.PressureSolver(g)
+    import torch
+    z = torch.zeros(g.n3d, dtype=torch.float64, device="cuda")
+    with pytest.raises(mgp.MgpError):
+        s.tstle4(z, z, z, z, z, 1e-2)
+
+
+def _taylor_green(g, o, seed=1, noise=1e-3):
+    u, v, w = g.zeros(), g.zeros(), g.zeros()
+    b = 1
+    sx = slice(g.ip1x[0], g.ip1x[0] + g.ii[0])
+    sy = slice(g.ip1y[0], g.ip1y[0] + g.jj[0])
+    sz = slice(g.ip1z[0], g.ip1z[0] + g.kk[0])
+    xs, ys = g.xstag[sx][:, None, None], g.ystag[sy][None, :, None]
+    xc, yc, zc = g.x[sx][:, None, None], g.y[sy][None, :, None], g.z[sz][None, None, :]
+    g.block_view(u, b)[...] = np.sin(xs) * np.cos(yc) * np.cos(zc)
+    g.block_view(v, b)[...] = -np.cos(xc) * np.sin(ys) * np.cos(zc)
+    inner = g.interior_mask()
+    rng = np.random.default_rng(seed)
+    for f in (u, v, w):
+        f += np.where(inner, rng.normal(0.0, noise, g.n3d), 0.0)
+    o.exchange([u, v, w], layers=2, corners=True)
+    return u, v, w
+
+
+def _oracle_stage(o, st, g, u, v, w, p, visc, dt, gmol, nouter, eps_rel):
+    """tstle4, the three 2N-storage updates, the forward exchange and
+    mgpoisl of stage 1 on the oracle; returns (stats, max |div| of the
+    predicted velocity)."""
+    from mgp.timeintegration import coeffs
+    frhs, fu, dtrk = coeffs(1)
+    uo, vo, wo = o.tstle4(u, v, w, p, visc, 1.0, gmol)
+    du, dv, dw = g.zeros(), g.zeros(), g.zeros()
+    o.rkstep(u, du, uo, frhs, dt * fu)
+    o.rkstep(v, dv, vo, frhs, dt * fu)
+    o.rkstep(w, dw, wo, frhs, dt * fu)
+    o.exchange([u, v, w], layers=1, forward=1)
+    div = g.zeros()
+    o.divcal(div, u, v, w, 1.0)
+    div_pre = np.abs(div[g.interior_mask()]).max()
+    return o.mgpoisl(st, u, v, w, p, dtrk * dt, ninner=5, nouter=nouter,
+                     eps_rel=eps_rel), div_pre
+
+
+def _gpu_stage(s, g, u, v, w, p, visc, dt, gmol):
+    from mgp.timeintegration import rk_stage
+    import torch
+    tu, tv, tw, tp, tg = _cuda(u, v, w, p, visc)
+    du, dv, dw = (torch.zeros_like(tu) for _ in range(3))
+    stats = rk_stage(s, tu, tv, tw, tp, du, dv, dw, tg, dt, 1, gmol)
+    return stats, _host(tu, tv, tw, tp)
+
+
+def test_taylor_green_stage_bitwise(oracle_mod):
+    """Config 1 at a fixed iteration count: FP64 GPU stage = oracle stage."""
+    import mgp
+    g = uniform_box((1, 1, 1), (32, 32, 32), length=(2 * np.pi,) * 3)
+    o = oracle_mod.Oracle(g)
+    u, v, w = _taylor_green(g, o)
+    p = g.zeros()
+    dt, gmol = 1e-3, 1e-2
+    visc = np.full(g.n3d, gmol)
+    st = o.setup()
+    ins = [a.copy() for a in (u, v, w, p)]
+    so, _ = _oracle_stage(o, st, g, u, v, w, p, visc, dt, gmol, nouter=6, eps_rel=0.0)
+    s = mgp.PressureSolver(g, ninner=5, nouter=6)
+    sg, got = _gpu_stage(s, g, *ins, visc, dt, gmol)
+    assert sg["ipc"] == so["ipc"] and sg["maxrhs"] == so["maxrhs"]
+    for a, b, nm in zip((u, v, w, p), got, "uvwp"):
+        assert np.array_equal(a, b), nm
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_taylor_green_stage_converged(oracle_mod, precision):
+    """Config 1 to a 1e-10 relative residual, all-FP64 and FP32-refined:
+    pressure (modulo its mean) and velocity within 1e-7 of the oracle's
+    converged stage (the two stop after different outer counts: the library
+    tests convergence every 4 outer iterations), and the divergence left by
+    the projection at most 1e-9 of the predicted velocity's."""
+    import mgp
+    g = uniform_box((1, 1, 1), (32, 32, 32), length=(2 * np.pi,) * 3)
+    o = oracle_mod.Oracle(g)
+    u, v, w = _taylor_green(g, o)
+    p = g.zeros()
+    dt, gmol = 1e-3, 1e-2
+    visc = np.full(g.n3d, gmol)
+    st = o.setup()
+    ins = [a.copy() for a in (u, v, w, p)]
+    so, div_pre = _oracle_stage(o, st, g, u, v, w, p, visc, dt, gmol, nouter=4000,
+                                eps_rel=1e-10)
+    assert so["maxrhs"] <= 1e-10 * so["maxrhs0"]
+    s = mgp.PressureSolver(g, ninner=5, nouter=4000, eps_rel=1e-10, precision=precision)
+    sg, got = _gpu_stage(s, g, *ins, visc, dt, gmol)
+    assert sg["converged"] and sg["maxrhs"] <= 1e-10 * sg["maxrhs0"]
+    inner = g.interior_mask()
+    a, b = p[inner], got[3][inner]
+    err_p = np.abs((a - a.mean()) - (b - b.mean())).max() / np.abs(a - a.mean()).max()
+    err_u = max(np.abs(x - y)[inner].max() for x, y in zip((u, v, w), got[:3])) / \
+        max(np.abs(x).max() for x in (u, v, w))
+    assert err_p <= 1e-7 and err_u <= 1e-7, (err_p, err_u, so["nouter_done"],
+                                             sg["nouter_done"])
+    div_g = g.zeros()
+    o.divcal(div_g, got[0], got[1], got[2], 1.0)
+    assert np.abs(div_g[inner]).max() <= 1e-9 * div_pre

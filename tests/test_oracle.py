@@ -14,7 +14,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from cases import apply_walls, topology, velocity
-from mgp.grids import NOS, uniform_box
+from mgp.grids import FIX, NOS, uniform_box
 
 
 def _ids(g):
@@ -674,3 +674,124 @@ def test_ghostcell_restriction_forms(oracle_mod):
     pv = g.block_view(a, 1)
     assert pv[2, 2, 2] == 0.0
     assert np.isclose(pv[3, 2, 2], fv[4, 2, 2] / 8.0, rtol=1e-14)
+
+
+# --- momentum right-hand side (tstle4) and the RK step -----------------
+def _self_block(n=12, length=2 * np.pi):
+    return uniform_box((1, 1, 1), (n, n, n), length=(length,) * 3)
+
+
+def _fill(g, o, fields_interior):
+    """Interior arrays (n, n, n) indexed [i, j, k] into block 1, ghosts by
+    the oracle's exchange."""
+    out = []
+    for a in fields_interior:
+        f = g.zeros()
+        g.block_view(f, 1)[2:-2, 2:-2, 2:-2] = a
+        out.append(f)
+    o.exchange(out, layers=2, corners=True)
+    return out
+
+
+def _curl_field(n, h, seed):
+    """Discrete curl of a random vector potential on a periodic n^3 grid:
+    exactly divergence free up to rounding."""
+    rng = np.random.default_rng(seed)
+    ax_, ay_, az_ = (rng.uniform(-1, 1, (n, n, n)) for _ in range(3))
+    d = lambda a, axis: (a - np.roll(a, 1, axis=axis)) / h    # backward difference
+    u = d(az_, 1) - d(ay_, 2)
+    v = d(ax_, 2) - d(az_, 0)
+    w = d(ay_, 0) - d(ax_, 1)
+    return u, v, w
+
+
+def test_tstle4_uniform_flow_is_steady(oracle_mod):
+    g = _self_block(8)
+    o = oracle_mod.Oracle(g)
+    n = 8
+    u, v, w = _fill(g, o, [np.full((n, n, n), 0.7), np.full((n, n, n), -0.3),
+                           np.full((n, n, n), 1.1)])
+    p = g.zeros()
+    visc = np.full(g.n3d, 1e-2)
+    for iles in (0, 1):
+        uo, vo, wo = o.tstle4(u, v, w, p, visc, 1.0, 1e-2, ilesmodel=iles)
+        assert np.abs(uo).max() == 0 and np.abs(vo).max() == 0 and np.abs(wo).max() == 0
+
+
+def test_tstle4_convection_conserves_energy(oracle_mod):
+    """The skew-symmetric convective form (tstle4_kon) does no work on a
+    divergence-free periodic field: sum of u . conv(u) over the momentum
+    cells vanishes (viscosity 0, pressure 0)."""
+    n = 12
+    g = _self_block(n)
+    o = oracle_mod.Oracle(g)
+    h = 2 * np.pi / n
+    ui, vi, wi = _curl_field(n, h, 11)
+    u, v, w = _fill(g, o, [ui, vi, wi])
+    div = g.zeros()
+    o.divcal(div, u, v, w, 1.0)
+    assert np.abs(div).max() < 1e-12 * np.abs(ui).max() / h
+    uo, vo, wo = o.tstle4(u, v, w, g.zeros(), g.zeros(), 1.0, 1e-30)
+    inner = g.interior_mask()
+    terms = np.concatenate([(u * uo)[inner], (v * vo)[inner], (w * wo)[inner]])
+    assert abs(terms.sum()) < 1e-13 * np.abs(terms).sum()
+    assert np.abs(uo[inner]).max() > 0.1      # the term itself is not trivial
+
+
+def test_tstle4_diffusion_eigenmode(oracle_mod):
+    """u = sin(m x) along x: the viscous part of the right-hand side (the
+    result minus the inviscid one) is the 3-point second difference
+    (ilesmodel 0) and twice that when the transpose part of the stress is on
+    (ilesmodel 1, u varies along x only)."""
+    n, m = 16, 3
+    g = _self_block(n)
+    o = oracle_mod.Oracle(g)
+    h = 2 * np.pi / n
+    xs = (np.arange(n) + 1.0) * h                # u faces of the interior cells
+    ui = np.broadcast_to(np.sin(m * xs)[:, None, None], (n, n, n)).copy()
+    zero = np.zeros((n, n, n))
+    u, v, w = _fill(g, o, [ui, zero, zero])
+    nu, rho = 2e-2, 1.3
+    visc = np.full(g.n3d, nu)
+    lam = -(nu / rho) * 4.0 / h ** 2 * np.sin(m * h / 2) ** 2
+    inner = g.interior_mask()
+    conv = o.tstle4(u, v, w, g.zeros(), g.zeros(), rho, 1e-30)   # viscosity 0
+    for iles, f in ((0, 1.0), (1, 2.0)):
+        uo, vo, wo = o.tstle4(u, v, w, g.zeros(), visc, rho, nu, ilesmodel=iles)
+        d = uo - conv[0]
+        assert np.allclose(d[inner], f * lam * u[inner], rtol=0, atol=1e-11 * abs(lam))
+        assert np.abs(vo - conv[1]).max() < 1e-12 and np.abs(wo - conv[2]).max() < 1e-12
+
+
+def test_tstle4_pressure_gradient(oracle_mod):
+    """A linear pressure p = a x on interior cells: -1/rho dp/dx on the u
+    faces between them, plus the body-force gradient gradp."""
+    n = 8
+    g = uniform_box((1, 1, 1), (n, n, n), length=(1.0, 1.0, 1.0), periodic=(False, True, True),
+                    wall=FIX)
+    o = oracle_mod.Oracle(g)
+    a, rho = 0.75, 2.0
+    xc = g.x[2:2 + n]
+    p = g.zeros()
+    g.block_view(p, 1)[2:-2, 2:-2, 2:-2] = (a * xc)[:, None, None]
+    o.exchange([p], layers=2, corners=True)     # periodic in y and z
+    z = g.zeros()
+    uo, vo, wo = o.tstle4(z, z, z, p, z, rho, 1e-30, gradp=[0.0, 0.25, 0.0])
+    # faces between two interior cells: i = 3 .. ii-3 (walls at x)
+    du = g.block_view(uo, 1)[2:n + 1, 2:-2, 2:-2]
+    assert np.allclose(du, -a / rho, rtol=1e-12)
+    dv = g.block_view(vo, 1)[2:-2, 2:-2, 2:-2]
+    assert np.allclose(dv, -0.25 / rho, rtol=1e-12)
+
+
+def test_rkstep_first_stage(oracle_mod):
+    g = _self_block(4)
+    o = oracle_mod.Oracle(g)
+    rng = np.random.default_rng(3)
+    p0, rhs = rng.standard_normal(g.n3d), rng.standard_normal(g.n3d)
+    p, dp = p0.copy(), np.full(g.n3d, 5.0)
+    o.rkstep(p, dp, rhs, 0.0, 0.25)           # Williamson stage 1: a = 0
+    assert np.array_equal(dp, rhs) and np.array_equal(p, p0 + 0.25 * rhs)
+    p2, dp2 = p.copy(), dp.copy()
+    o.rkstep(p2, dp2, rhs, -5.0 / 9.0, 0.5)
+    assert np.array_equal(dp2, -5.0 / 9.0 * dp + rhs)
