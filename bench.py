@@ -20,7 +20,10 @@ The line also carries:
   cpu_baseline -- the oracle (C restatement of the reference CPU path) timed
                   on this host on a bounded sample of the same workload;
   tts          -- time to a 1e-10 relative residual (FP32-refined and FP64)
-                  for the configuration named in `tts.config`.
+                  for the configuration named in `tts.config`;
+  rk_stage     -- (with --stage) the momentum right-hand side (tstle4) and
+                  the 2N-storage update (rkstep) on the same fields, host
+                  wall time per call over synchronised repetitions.
 """
 from __future__ import annotations
 
@@ -99,6 +102,43 @@ def run_steps(solver, fields0, steps, ninner):
     t1 = time.perf_counter()
     del u, v, w, p
     return t1 - t0, st
+
+
+# stored positions per launch move U, V, W, P, G in and UO, VO, WO out (tstle4);
+# P, DP, RHS in and P, DP out (rkstep)
+STAGE_BYTES = {"tstle4": 64, "rkstep": 40}
+
+
+def rk_stage_timing(solver, fields0, reps=5):
+    """Mean wall ms per call of tstle4 and rkstep on the workload's fields
+    (viscosity 1e-2, pressure 0); each call synchronises before returning."""
+    import torch
+    u, v, w = fields0
+    p = torch.zeros_like(u)
+    g = torch.full_like(u, 1e-2)
+    out = {}
+    solver.tstle4(u, v, w, p, g, 1e-2)          # warm
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        uo, vo, wo = solver.tstle4(u, v, w, p, g, 1e-2)
+        del uo, vo, wo
+    torch.cuda.synchronize()
+    out["tstle4_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    del g
+    q, dq, rhs = p, torch.zeros_like(u), torch.ones_like(u)
+    solver.rkstep(q, dq, rhs, -5.0 / 9.0, 1e-3)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        solver.rkstep(q, dq, rhs, -5.0 / 9.0, 1e-3)
+    torch.cuda.synchronize()
+    out["rkstep_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    n = u.numel()
+    for k in ("tstle4", "rkstep"):
+        out[f"{k}_gbs"] = round(STAGE_BYTES[k] * n / (out[f"{k}_ms"] * 1e-3) / 1e9, 1)
+        out[f"{k}_ms"] = round(out[f"{k}_ms"], 3)
+    out["stored_positions"] = n
+    out["alg_bytes_per_position"] = STAGE_BYTES
+    return out
 
 
 def time_to_solution(solver, fields0, ninner, eps, cap):
@@ -335,6 +375,8 @@ def main():
     ap.add_argument("--tts-cap", type=int, default=4000, help="max outer iterations")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp32,fp64")
+    ap.add_argument("--stage", action="store_true",
+                    help="also time the RK stage kernels (tstle4, rkstep) on the workload")
     ap.add_argument("--config", choices=["main", "c5"], default="main",
                     help="c5: the adjoint-projection check (not the headline line)")
     args = ap.parse_args()
@@ -360,6 +402,7 @@ def main():
     fields0 = make_velocity(g, nb, n, seed=4)
     forms = args.forms.split(",")
     res = {}
+    stage = None
     filled = False
     for form in forms:
         log(f"[bench] init {form}")
@@ -373,6 +416,10 @@ def main():
         wall, st = run_steps(s, fields0, args.steps, args.ninner)
         kt = s.time_kernels(niter=3)
         shape, shape_ms = s.plan_shape()
+        if args.stage and stage is None:
+            log("[bench] timing the RK stage kernels")
+            stage = rk_stage_timing(s, fields0)
+            log(f"[bench] rk stage: {stage}")
         res[form] = dict(wall=wall, stats=st, kernels=kt, info=s.info(),
                          autotune={"shape": shape,
                                    "ms_per_outer_by_shape": [round(x, 3) for x in shape_ms]})
@@ -430,6 +477,8 @@ def main():
                          (ALG_BYTES[main_form]["forward"] + ALG_BYTES[main_form]["backward"])
                          * cells / ((fw_ms + r["kernels"]["backward_ms"]) * 1e-3) / 1e9, 1)},
     }
+    if stage is not None:
+        line["rk_stage"] = stage
     if "fp32" in res and "fp64" in res:
         line["fp32_over_fp64_throughput"] = round(res["fp64"]["wall"] / res["fp32"]["wall"], 3)
 

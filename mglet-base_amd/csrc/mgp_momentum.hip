@@ -10,6 +10,8 @@
 // so the result is bitwise that of the CPU restatement.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "mgp_launch.h"
 
 namespace mgp {
@@ -27,12 +29,16 @@ __device__ __forceinline__ double gpair(double a, double b, double gmol) {
 // the LES model is on) - pressure gradient and body-force gradient
 // (tstle4_gradp :461-529); zero outside the loop ranges, as the reference
 // zeroes uo before filling it.  The transporting velocities are u, v, w.
-__global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a) {
-    const int b = t.blist[blockIdx.y];
+// One-dimensional grid of nchunk workgroups per block, consecutive
+// workgroups walking one block, so the i-1 plane a workgroup reads was
+// fetched by one a few workgroups earlier.
+__global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a, int nchunk,
+                                                         int first) {
+    const int b = t.blist[first + (int)(blockIdx.x / nchunk)];
     const Block &B = t.blocks[b];
     const int kk = B.nk + 4, jj = B.nj + 4, ii = B.ni + 4;
     const int64_t n = (int64_t)kk * jj * ii;
-    const int64_t e = (int64_t)blockIdx.x * kMomThreads + threadIdx.x;
+    const int64_t e = (int64_t)(blockIdx.x % nchunk) * kMomThreads + threadIdx.x;
     if (e >= n) return;
     const int k = (int)(e % kk) + 1, j = (int)((e / kk) % jj) + 1,
               i = (int)(e / ((int64_t)kk * jj)) + 1;
@@ -174,8 +180,13 @@ __global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a) {
 
 void launch_tstle4(const Topo &t, const MomArgs &a, hipStream_t s) {
     const int maxc = (t.max_ni + 4) * (t.max_nj + 4) * (t.max_nk + 4);
-    dim3 grid((maxc + kMomThreads - 1) / kMomThreads, t.nlist);
-    k_tstle4<<<grid, kMomThreads, 0, s>>>(t, a);
+    const int nchunk = (maxc + kMomThreads - 1) / kMomThreads;
+    // launches of at most 2^31 work-items
+    const int per = (int)std::max<int64_t>(1, (INT64_C(1) << 31) / ((int64_t)nchunk * kMomThreads));
+    for (int first = 0; first < t.nlist; first += per) {
+        const int nb = std::min(per, t.nlist - first);
+        k_tstle4<<<(unsigned)((int64_t)nchunk * nb), kMomThreads, 0, s>>>(t, a, nchunk, first);
+    }
 }
 
 // src/core/rungekutta_mod.F90:315-333 over every stored position

@@ -97,6 +97,32 @@ MODULE mgp_bindc_mod
             INTEGER(C_INT), VALUE :: ninner, nouter, nouter_min, check_every
             REAL(C_DOUBLE), VALUE :: epcorr, eps_rel
         END FUNCTION mgp_set_iterations
+
+        ! fields: C_LOC of an array of nfield device pointers
+        INTEGER(C_INT) FUNCTION mgp_connect_device(handle, level, layers, &
+                corners, forward, fields, nfield) &
+                BIND(C, name="mgp_connect_device")
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: handle, fields
+            INTEGER(C_INT), VALUE :: level, layers, corners, forward, nfield
+        END FUNCTION mgp_connect_device
+
+        ! tstle4 (noib) on device fields; gradp may be C_NULL_PTR
+        INTEGER(C_INT) FUNCTION mgp_tstle4_device(handle, uo, vo, wo, u, v, &
+                w, p, g, gmol, ilesmodel, gradp) &
+                BIND(C, name="mgp_tstle4_device")
+            IMPORT :: C_INT, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: handle, uo, vo, wo, u, v, w, p, g, gradp
+            REAL(C_DOUBLE), VALUE :: gmol
+            INTEGER(C_INT), VALUE :: ilesmodel
+        END FUNCTION mgp_tstle4_device
+
+        INTEGER(C_INT) FUNCTION mgp_rkstep_device(handle, p, dp, rhsp, frhs, &
+                dtfu) BIND(C, name="mgp_rkstep_device")
+            IMPORT :: C_INT, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: handle, p, dp, rhsp
+            REAL(C_DOUBLE), VALUE :: frhs, dtfu
+        END FUNCTION mgp_rkstep_device
     END INTERFACE
 
 CONTAINS
