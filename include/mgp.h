@@ -110,6 +110,13 @@ int mgp_init_mask(const mgp_grids *grids, const mgp_config *cfg,
                   const mgletreal *ddy, const mgletreal *ddz,
                   const int8_t *blocked, mgp_handle **out);
 
+/* The ghostcell divergence source SDIV (3-D field, reference layout, host
+ * memory), which the ghostcell divcal adds as fak*SDIV before the BP factor
+ * (src/ib/gc_mod.F90:278-347; the caller computes it from its STL cut cells,
+ * src/flow/gc_flowstencils_mod.F90:1486-1530).  NULL resets it to zero, the
+ * value without cut cells.  Only for handles with ibtype 1. */
+int mgp_set_sdiv(mgp_handle *h, const mgletreal *sdiv);
+
 /* mgpoisl(u, v, w, p, dt, ittot, irk) (src/flow/pressuresolver_mod.F90:
  * 160-357): projection of host arrays u, v, w, p in place. */
 int mgp_mgpoisl(mgp_handle *h, mgletreal *u, mgletreal *v, mgletreal *w,
@@ -174,7 +181,7 @@ int mgp_connect_device(mgp_handle *h, mgletint level, mgletint layers,
                        mgletreal **fields, mgletint nfield);
 
 /* ib%divcal (src/ib/noib_mod.F90:253-354): div = fak * div(u, v, w) over
- * the interior of device fields (x BP for ibtype 1). */
+ * the interior of device fields (+ fak*SDIV, then x BP, for ibtype 1). */
 int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
                       const mgletreal *v, const mgletreal *w,
                       mgletreal fak);
@@ -206,13 +213,15 @@ int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms);
 
 /* Pipeline shape of the plane kernels (the sweeps, SOR and the outer
  * update): 1 = four 64-cell items of loads in flight per wave, 2 = two
- * 128-cell items, 3 = three 128-cell items.  request -1 only reports;
- * 0 runs the autotuner:
+ * 128-cell items, 3 = three 128-cell items, 4 = six 128-cell items, 5 =
+ * eight 64-cell items.  request -1 only reports; 0 runs the autotuner:
  * it times one inner iteration per shape (ninner x forward + backward, or
  * SOR, plus one outer update) on h's own blocks and keeps the fastest;
- * 1..3 sets the shape.  Results do not depend on the shape.  chosen gets
- * the shape in use, ms[3] (may be NULL) the timed ms per shape (0 when not
- * timed).  mgp_init autotunes unless MGP_PLAN_SHAPE=1..3 is set. */
+ * 1..MGP_NUM_PLAN_SHAPES sets the shape.  Results do not depend on the
+ * shape.  chosen gets the shape in use, ms[MGP_NUM_PLAN_SHAPES] (may be
+ * NULL) the timed ms per shape (0 when not timed).  mgp_init autotunes
+ * unless MGP_PLAN_SHAPE=1..MGP_NUM_PLAN_SHAPES is set. */
+#define MGP_NUM_PLAN_SHAPES 5
 int mgp_plan_shape(mgp_handle *h, mgletint request, mgletint *chosen,
                    double *ms);
 

@@ -142,6 +142,7 @@ struct mgp_handle {
     // per-cell WO arrays
     DBuf d_ap64, d_apT, d_f[7], d_hilf, d_rhs64, d_rhsT, d_res, d_dp64;
     DBuf d_bpw, d_bpf, d_bpsp;
+    DBuf d_sdiv;               // ghostcell divergence source (reference layout)
     DBuf d_hface, d_rhi[2], d_rlo;
     // SOR: SOR_RAP and the second HILF face buffer (frozen ghosts)
     DBuf d_rapT, d_hface2;
@@ -175,7 +176,8 @@ struct mgp_handle {
     int graph_key[4] = {-1, -1, -1, -1};
     bool have_solution = false;
     int plan_shape = 1;          // pipeline shape of the plane kernels
-    double shape_ms[3] = {0, 0, 0};
+    double shape_ms[kNumShapes] = {};
+    static_assert(kNumShapes == MGP_NUM_PLAN_SHAPES, "shape count of the C ABI");
 
     ~mgp_handle() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -183,6 +185,9 @@ struct mgp_handle {
     }
 
     bool fp32() const { return cfg.precision == 1; }
+    const double *sdiv_sp() const {
+        return cfg.ibtype == 1 && d_sdiv.p ? d_sdiv.as<double>() : nullptr;
+    }
     size_t tsz() const { return fp32() ? 4 : 8; }
 
     Topo topo(int lvl) const {
@@ -1101,7 +1106,7 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
         HC(hipStreamSynchronize(h->stream));
         const char *ps = getenv("MGP_PLAN_SHAPE");
         const int forced = ps ? atoi(ps) : 0;
-        if (forced >= 1 && forced <= 3) {
+        if (forced >= 1 && forced <= kNumShapes) {
             h->plan_shape = forced;
         } else {
             if (h->fp32()) tune_shape<float>(*h);
@@ -1143,7 +1148,7 @@ void tune_shape(mgp_handle &h) {
         }
     };
     int best = 0;
-    for (int v = 1; v <= 3; ++v) {
+    for (int v = 1; v <= kNumShapes; ++v) {
         set_plan_shape(v);
         pass();  // warm-up
         float a = 0.f, b = 0.f;
@@ -1373,7 +1378,7 @@ int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
     reset_state<T>(h);
     launch_divcal<T>(tall, u, v, w, h.d_rddx.as<double>(), h.d_rddy.as<double>(),
                      h.d_rddz.as<double>(), c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr,
-                     prefak, h.d_rhs64.as<double>(), rhsT, s);
+                     h.sdiv_sp(), prefak, h.d_rhs64.as<double>(), rhsT, s);
     // RHS of the refined grids into their parents (:197-199)
     const double *dd[3] = {h.d_ddx.as<double>(), h.d_ddy.as<double>(), h.d_ddz.as<double>()};
     for (auto lv = h.level_blocks.rbegin(); lv != h.level_blocks.rend(); ++lv)
@@ -1734,9 +1739,28 @@ int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
         Topo t = h->topo(-1);
         launch_divcal<double>(t, u, v, w, h->d_rddx.as<double>(), h->d_rddy.as<double>(),
                               h->d_rddz.as<double>(), h->cfg.ibtype == 1 ? h->d_bpsp.as<double>() : nullptr,
-                              fak, tmp.as<double>(), nullptr, h->stream);
+                              h->sdiv_sp(), fak, tmp.as<double>(), nullptr, h->stream);
         launch_wo_to_sp<double>(t, tmp.as<double>(), div, h->stream);
         HC(hipGetLastError());
+        HC(hipStreamSynchronize(h->stream));
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    return MGP_OK;
+}
+
+int mgp_set_sdiv(mgp_handle *h, const mgletreal *sdiv) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (h->cfg.ibtype != 1)
+        return fail(MGP_ERR_ARG, "SDIV enters the ghostcell divergence only (ib = ghostcell)");
+    try {
+        if (!sdiv) {
+            h->d_sdiv.alloc(0);
+            return MGP_OK;
+        }
+        if (!h->d_sdiv.p) h->d_sdiv.alloc((size_t)h->n3d * sizeof(double));
+        HC(hipMemcpyAsync(h->d_sdiv.p, sdiv, (size_t)h->n3d * sizeof(double),
+                          hipMemcpyHostToDevice, h->stream));
         HC(hipStreamSynchronize(h->stream));
     } catch (const HipError &e) {
         return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
@@ -1847,7 +1871,7 @@ int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms) {
 
 int mgp_plan_shape(mgp_handle *h, mgletint request, mgletint *chosen,
                    double *ms) {
-    if (!h || request < -1 || request > 3) return fail(MGP_ERR_ARG, "bad argument");
+    if (!h || request < -1 || request > kNumShapes) return fail(MGP_ERR_ARG, "bad argument");
     try {
         if (request < 0) {
             // report only
@@ -1862,7 +1886,7 @@ int mgp_plan_shape(mgp_handle *h, mgletint request, mgletint *chosen,
     }
     if (chosen) *chosen = h->plan_shape;
     if (ms)
-        for (int v = 0; v < 3; ++v) ms[v] = h->shape_ms[v];
+        for (int v = 0; v < kNumShapes; ++v) ms[v] = h->shape_ms[v];
     return MGP_OK;
 }
 
@@ -1890,7 +1914,7 @@ int mgp_info(mgp_handle *h, int64_t *ncells, int64_t *device_bytes) {
         const DBuf *bs[] = {&h->d_blocks, &h->d_shapes, &h->d_cells, &h->d_pstart, &h->d_slot,
                             &h->d_items, &h->d_ents, &h->d_zero, &h->d_cellx,
                             &h->d_ap64, &h->d_apT, &h->d_hilf, &h->d_rhs64, &h->d_rhsT, &h->d_res,
-                            &h->d_dp64, &h->d_bpw, &h->d_bpf, &h->d_bpsp, &h->d_hface, &h->d_rhi[0],
+                            &h->d_dp64, &h->d_bpw, &h->d_bpf, &h->d_bpsp, &h->d_sdiv, &h->d_hface, &h->d_rhi[0],
                             &h->d_rhi[1], &h->d_rlo, &h->d_rapT, &h->d_hface2, &h->d_ft[0], &h->d_ft[1], &h->d_ft[2],
                             &h->d_ft[3], &h->d_ft[4], &h->d_ft[5], &h->d_u, &h->d_v, &h->d_w, &h->d_p, &h->d_dpsp};
         for (auto *b : bs) tot += b->n;

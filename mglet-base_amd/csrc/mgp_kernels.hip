@@ -261,13 +261,15 @@ constexpr int kStageThreads = 1024;
 constexpr int kStage = 8192;   // 64 KB of doubles
 
 // RHS = fak * div(u, v, w): src/ib/noib_mod.F90:304-354 (noib) and the
-// ghostcell variant src/ib/gc_mod.F90:278-347 (+ fak*SDIV with SDIV = 0,
-// then x BP); u, v, w in the reference layout, RHS in WO.
+// ghostcell variant src/ib/gc_mod.F90:278-347 (+ fak*SDIV, the caller's
+// geometry source or zero, then x BP); u, v, w, SDIV and BP in the reference
+// layout, RHS in WO.
 template <typename T>
 __global__ __launch_bounds__(kStageThreads) void k_divcal(
     Topo t, const double *u, const double *v, const double *w,
     const double *rddx_, const double *rddy_, const double *rddz_,
-    const double *bp_sp, double fak, double *rhs64, T *rhs_t) {
+    const double *bp_sp, const double *sdiv_sp, double fak, double *rhs64,
+    T *rhs_t) {
     extern __shared__ double st_dv[];
     const int b = t.blist[blockIdx.x];
     const Block &B = t.blocks[b];
@@ -288,7 +290,7 @@ __global__ __launch_bounds__(kStageThreads) void k_divcal(
                               (v[c] - v[c - kk]) * rddy[j] +
                               (w[c] - w[c - 1]) * rddz[k]);
             if (bp_sp) {
-                d = d + fak * 0.0;
+                d = d + fak * (sdiv_sp ? sdiv_sp[c] : 0.0);
                 d = bp_sp[c] * d;
             }
             st_dv[sl - base] = d;
@@ -311,21 +313,23 @@ static size_t stage_bytes(const Topo &t) {
 template <typename T>
 void launch_divcal(const Topo &t, const double *u, const double *v,
                    const double *w, const double *rddx, const double *rddy,
-                   const double *rddz, const double *bp_sp, double fak,
-                   double *rhs64, T *rhs_t, hipStream_t s) {
+                   const double *rddz, const double *bp_sp, const double *sdiv_sp,
+                   double fak, double *rhs64, T *rhs_t, hipStream_t s) {
     k_divcal<T><<<t.nlist, kStageThreads, stage_bytes(t), s>>>(
-        t, u, v, w, rddx, rddy, rddz, bp_sp, fak, rhs64, rhs_t);
+        t, u, v, w, rddx, rddy, rddz, bp_sp, sdiv_sp, fak, rhs64, rhs_t);
 }
 template void launch_divcal<double>(const Topo &, const double *,
                                     const double *, const double *,
                                     const double *, const double *,
-                                    const double *, const double *, double,
-                                    double *, double *, hipStream_t);
+                                    const double *, const double *,
+                                    const double *, double, double *, double *,
+                                    hipStream_t);
 template void launch_divcal<float>(const Topo &, const double *,
                                    const double *, const double *,
                                    const double *, const double *,
-                                   const double *, const double *, double,
-                                   double *, float *, hipStream_t);
+                                   const double *, const double *,
+                                   const double *, double, double *, float *,
+                                   hipStream_t);
 
 // ---------------------------------------------------------------------
 // Block-wide max of non-negative doubles folded into a global maximum.

@@ -88,8 +88,8 @@ void launch_convert(const double *src, T *dst, int64_t n, hipStream_t s);
 template <typename T>
 void launch_divcal(const Topo &t, const double *u, const double *v,
                    const double *w, const double *rddx, const double *rddy,
-                   const double *rddz, const double *bp_sp, double fak,
-                   double *rhs64, T *rhs_t, hipStream_t s);
+                   const double *rddz, const double *bp_sp, const double *sdiv_sp,
+                   double fak, double *rhs64, T *rhs_t, hipStream_t s);
 
 void launch_maxabs(const Topo &t, const double *rhs64, Outer o, hipStream_t s);
 
@@ -112,6 +112,8 @@ void launch_sor(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
 
 // pipeline shape of the plane kernels (1: 4 items of 64 cells in flight,
 // 2: 2 items of 128 cells, 3: 3 items of 128 cells)
+// pipeline shapes of the plane kernels (see set_plan_shape)
+constexpr int kNumShapes = 5;
 void set_plan_shape(int v);
 int plan_shape();
 

@@ -849,10 +849,12 @@ __global__ __launch_bounds__(kWave) void k_outer(
 // ---------------------------------------------------------------------
 // Launchers.  The pipeline shape is a tuning knob, set_plan_shape(): 1 =
 // 4 items of 64 cells in flight, 2 = 2 items of 128 cells (two cells per
-// lane, independent, so their latencies overlap), 3 = 3 items of 128.
+// lane, independent, so their latencies overlap), 3 = 3 items of 128,
+// 4 = 6 items of 128, 5 = 8 items of 64.  The deep shapes hide more memory
+// latency per wave, for launches with too few blocks to fill the chip.
 static int g_shape = 1;
 
-void set_plan_shape(int v) { g_shape = (v >= 1 && v <= 3) ? v : 1; }
+void set_plan_shape(int v) { g_shape = (v >= 1 && v <= kNumShapes) ? v : 1; }
 int plan_shape() { return g_shape; }
 
 template <typename T>
@@ -864,6 +866,8 @@ static void fwd_launch(const Topo &t, const Fields<T> &f, Mask m, int rd,
     switch (g_shape) {
     case 1: k_forward<T, CLASSIC, MASKED, LAP, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
     case 3: k_forward<T, CLASSIC, MASKED, LAP, 3, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    case 4: k_forward<T, CLASSIC, MASKED, LAP, 6, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    case 5: k_forward<T, CLASSIC, MASKED, LAP, 8, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
     default: k_forward<T, CLASSIC, MASKED, LAP, 2, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
     }
 }
@@ -903,6 +907,8 @@ static void bwd_launch(const Topo &t, const Fields<T> &f, const int *done,
     switch (g_shape) {
     case 1: k_backward<T, CLASSIC, HZ, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
     case 3: k_backward<T, CLASSIC, HZ, 3, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
+    case 4: k_backward<T, CLASSIC, HZ, 6, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
+    case 5: k_backward<T, CLASSIC, HZ, 8, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
     default: k_backward<T, CLASSIC, HZ, 2, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
     }
 }
@@ -932,6 +938,8 @@ static void sor_launch(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
     switch (g_shape) {
     case 1: k_sor<T, MASKED, HZ, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
     case 3: k_sor<T, MASKED, HZ, 3, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
+    case 4: k_sor<T, MASKED, HZ, 6, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
+    case 5: k_sor<T, MASKED, HZ, 8, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
     default: k_sor<T, MASKED, HZ, 2, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
     }
 }
@@ -966,6 +974,8 @@ static void out_launch(const Topo &t, const Fields<T> &f, Mask m, Outer o,
     switch (g_shape) {
     case 1: k_outer<T, MASKED, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
     case 3: k_outer<T, MASKED, 3, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    case 4: k_outer<T, MASKED, 6, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    case 5: k_outer<T, MASKED, 8, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
     default: k_outer<T, MASKED, 2, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
     }
 }

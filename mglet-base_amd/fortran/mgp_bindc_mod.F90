@@ -41,7 +41,8 @@ MODULE mgp_bindc_mod
     END TYPE mgp_stats_t
 
     PUBLIC :: mgp_init, mgp_init_mask, mgp_mgpoisl, mgp_mgpoisl_device, &
-        mgp_finish, mgp_last_error, mgp_set_iterations, mgp_error_message
+        mgp_finish, mgp_last_error, mgp_set_iterations, mgp_error_message, &
+        mgp_set_sdiv, mgp_connect_device, mgp_tstle4_device, mgp_rkstep_device
 
     INTERFACE
         INTEGER(C_INT) FUNCTION mgp_init(grids, cfg, dx, dy, dz, ddx, ddy, &
@@ -97,6 +98,12 @@ MODULE mgp_bindc_mod
             INTEGER(C_INT), VALUE :: ninner, nouter, nouter_min, check_every
             REAL(C_DOUBLE), VALUE :: epcorr, eps_rel
         END FUNCTION mgp_set_iterations
+
+        INTEGER(C_INT) FUNCTION mgp_set_sdiv(handle, sdiv) &
+                BIND(C, name="mgp_set_sdiv")
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: handle, sdiv
+        END FUNCTION mgp_set_sdiv
 
         ! fields: C_LOC of an array of nfield device pointers
         INTEGER(C_INT) FUNCTION mgp_connect_device(handle, level, layers, &

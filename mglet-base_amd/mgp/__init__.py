@@ -23,6 +23,7 @@ __all__ = ["PressureSolver", "MgpError", "Grids", "uniform_box"]
 _TYPES = {"sip": 0, "sor": 1, "classic": 2}
 _IB = {"noib": 0, "ghostcell": 1}
 _PREC = {"fp64": 0, "fp32": 1}
+NUM_PLAN_SHAPES = 5       # MGP_NUM_PLAN_SHAPES
 
 
 def _ptr(a, ct=C.c_double):
@@ -160,6 +161,17 @@ class PressureSolver:
                    "mgp_divcal_device")
         return div
 
+    def set_sdiv(self, sdiv):
+        """The ghostcell divergence source SDIV (flat float64 array of the
+        3-D length, copied to the device); None resets it to zero."""
+        if sdiv is None:
+            _lib.check(self.lib.mgp_set_sdiv(self.h, None), "mgp_set_sdiv")
+            return
+        a = np.ascontiguousarray(sdiv, dtype=np.float64)
+        if a.size != self.grids.n3d:
+            raise MgpError("SDIV must have the 3-D length")
+        _lib.check(self.lib.mgp_set_sdiv(self.h, _ptr(a)), "mgp_set_sdiv")
+
     def tstle4(self, u, v, w, p, g, gmol, ilesmodel=0, gradp=None):
         """Momentum right-hand side of one RK stage (tstle4, no immersed
         boundary) on CUDA float64 tensors; returns (uo, vo, wo)."""
@@ -189,10 +201,10 @@ class PressureSolver:
 
     def plan_shape(self, request=-1):
         """Pipeline shape of the plane kernels.  request -1: report; 0: run
-        the autotuner again; 1..3: set.  Returns (shape, ms per shape as
-        last timed by the autotuner, 0 where not timed)."""
+        the autotuner again; 1..NUM_PLAN_SHAPES: set.  Returns (shape, ms per
+        shape as last timed by the autotuner, 0 where not timed)."""
         chosen = C.c_int32()
-        ms = np.zeros(3)
+        ms = np.zeros(NUM_PLAN_SHAPES)
         _lib.check(self.lib.mgp_plan_shape(self.h, request, C.byref(chosen), _ptr(ms)),
                    "mgp_plan_shape")
         return chosen.value, ms

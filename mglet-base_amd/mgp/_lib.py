@@ -85,6 +85,7 @@ def load():
         "mgp_connect_device": [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                C.POINTER(vp), C.c_int32],
         "mgp_divcal_device": [vp, vp, vp, vp, vp, C.c_double],
+        "mgp_set_sdiv": [vp, _dp],
         "mgp_time_kernels": [vp, C.c_int32, _dp],
         "mgp_plan_shape": [vp, C.c_int32, _i32p, _dp],
         "mgp_tstle4_device": [vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_int32, _dp],

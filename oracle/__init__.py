@@ -40,7 +40,7 @@ class OgParams(C.Structure):
     _fields_ = [("ityp", C.c_int32), ("ninner", C.c_int32), ("nouter", C.c_int32),
                 ("nouter_min", C.c_int32), ("epcorr", C.c_double), ("eps_rel", C.c_double),
                 ("omega", C.c_double), ("rho", C.c_double), ("ibtype", C.c_int32),
-                ("sor_all_levels", C.c_int32)]
+                ("sor_all_levels", C.c_int32), ("sdiv", _dp)]
 
 
 class OgStats(C.Structure):
@@ -222,10 +222,11 @@ class Oracle:
 
     def mgpoisl(self, st, u, v, w, p, dt, ityp=0, ninner=5, nouter=9, nouter_min=0,
                 epcorr=0.0, eps_rel=0.0, omega=1.1, rho=1.0, ibtype=0,
-                sor_all_levels=0):
+                sor_all_levels=0, sdiv=None):
+        """sdiv: the ghostcell divergence source (ibtype 1), None = zero."""
         g = self.g
         prm = OgParams(ityp, ninner, nouter, nouter_min, epcorr, eps_rel, omega, rho, ibtype,
-                       sor_all_levels)
+                       sor_all_levels, _p(sdiv) if sdiv is not None else None)
         hist = np.zeros(max(nouter, 1))
         stats = OgStats(0, 0, 0.0, 0.0, _p(hist))
         work = [np.zeros(g.n3d) for _ in range(4)]

@@ -1411,11 +1411,13 @@ int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
     if (prm->ibtype == 0) {
         og_divcal(t, rhs, u, v, w, rddx, rddy, rddz, prefak, NULL, NULL);
     } else {
-        /* ghostcell divcal (src/ib/gc_mod.F90:278-347) passes SDIV, which
-         * is zero without STL stencils, then multiplies by BP */
-        double *sdiv = calloc((size_t)n, sizeof(double));
-        og_divcal(t, rhs, u, v, w, rddx, rddy, rddz, prefak, bp, sdiv);
-        free(sdiv);
+        /* ghostcell divcal (src/ib/gc_mod.F90:278-347) adds fak*SDIV (the
+         * caller's geometry source, zero without STL cut cells), then
+         * multiplies by BP */
+        double *zero = prm->sdiv ? NULL : calloc((size_t)n, sizeof(double));
+        og_divcal(t, rhs, u, v, w, rddx, rddy, rddz, prefak, bp,
+                  prm->sdiv ? prm->sdiv : zero);
+        free(zero);
     }
     const double *bpr = prm->ibtype ? bp : NULL;   /* gc_restrict with the IB */
     for (int l = t->maxlevel; l >= t->minlevel; --l)

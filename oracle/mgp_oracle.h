@@ -55,6 +55,8 @@ typedef struct og_params {
     double rho;
     int32_t ibtype;       /* 0 noib, 1 ghostcell (divcal*BP, masked pcorr)  */
     int32_t sor_all_levels; /* 1: run SOR also on minlevel                 */
+    const double *sdiv;   /* ghostcell divergence source SDIV (3-D field),  */
+                          /*   NULL = zero (no STL cut cells)              */
 } og_params;
 
 typedef struct og_stats {

@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _run(oracle_mod, g, seed, ityp=0, ninner=3, nouter=3, ibtype=0, blocked=None,
-         eps_rel=0.0, precision="fp64", omega=1.1):
+         eps_rel=0.0, precision="fp64", omega=1.1, sdiv_seed=None):
     import mgp
     o = oracle_mod.Oracle(g)
     bp = None
@@ -20,12 +20,18 @@ def _run(oracle_mod, g, seed, ityp=0, ninner=3, nouter=3, ibtype=0, blocked=None
     u, v, w = velocity_levels(g, o, seed, blocked=blocked)
     p = np.zeros(g.n3d)
     ins = [a.copy() for a in (u, v, w, p)]
+    sdiv = None
+    if sdiv_seed is not None:    # geometry source on interior cells of every level
+        rng = np.random.default_rng(sdiv_seed)
+        sdiv = np.where(g.interior_mask(), rng.standard_normal(g.n3d), 0.0)
     so = o.mgpoisl(st, u, v, w, p, 1e-3, ityp=ityp, ninner=ninner, nouter=nouter,
-                   eps_rel=eps_rel, ibtype=ibtype, omega=omega)
+                   eps_rel=eps_rel, ibtype=ibtype, omega=omega, sdiv=sdiv)
     s = mgp.PressureSolver(g, type={0: "sip", 1: "sor", 2: "classic"}[ityp], ninner=ninner,
                            nouter=nouter, eps_rel=eps_rel, omega=omega,
                            ib={0: "noib", 1: "ghostcell"}[ibtype], precision=precision,
                            bp=bp)
+    if sdiv is not None:
+        s.set_sdiv(sdiv)
     sg = s.mgpoisl(*ins, 1e-3)
     return so, sg, s, (u, v, w, p), ins
 
@@ -70,12 +76,15 @@ def test_three_level_mixed_sizes_bitwise(oracle_mod, ninner):
     _equal(so, sg, s, ref, got)
 
 
-def test_sphere_three_levels_masked_bitwise(oracle_mod):
+@pytest.mark.parametrize("sdiv_seed", [None, 88])
+def test_sphere_three_levels_masked_bitwise(oracle_mod, sdiv_seed):
     """Blocked cells inside a sphere (int8 mask from the signed distance)
-    across three levels, ghostcell operator."""
+    across three levels, ghostcell operator, with and without a geometry
+    source SDIV."""
     g = three_level()
     blocked = sphere_mask(g, radius=0.3, centre=(0.25, 0.25, 0.25))
-    so, sg, s, ref, got = _run(oracle_mod, g, 86, ibtype=1, blocked=blocked, nouter=2)
+    so, sg, s, ref, got = _run(oracle_mod, g, 86, ibtype=1, blocked=blocked, nouter=2,
+                               sdiv_seed=sdiv_seed)
     _equal(so, sg, s, ref, got)
 
 

@@ -196,11 +196,11 @@ def test_single_hip_runtime():
     s.close()
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("case", ["sip-walls", "classic-masked", "sor-odd", "sor-masked"])
 def test_plan_shapes_bitwise(oracle_mod, shape, case):
     """Every pipeline shape of the plane kernels (items of 64 or 128 cells,
-    2 to 4 in flight) gives the oracle's FP64 result bit for bit."""
+    2 to 8 in flight) gives the oracle's FP64 result bit for bit."""
     kind, name = case.split("-")
     ityp = {"sip": 0, "classic": 2, "sor": 1}[kind]
     topo = {"walls": "walls", "masked": "box2", "odd": "odd"}[name]
@@ -230,9 +230,74 @@ def test_plan_shapes_bitwise(oracle_mod, shape, case):
 def test_plan_shape_autotuner():
     """The autotuner times every shape on the handle's blocks and keeps the
     fastest."""
+    import mgp
     g = topology("box2")
     s = _solver(g)
     shape, ms = s.plan_shape(0)
-    assert shape in (1, 2, 3)
+    assert 1 <= shape <= mgp.NUM_PLAN_SHAPES
     assert np.all(ms > 0) and ms[shape - 1] == ms.min()
     assert s.plan_shape()[0] == shape
+
+
+@pytest.mark.parametrize("ibtype", [0, 1])
+def test_divcal_device_bitwise(oracle_mod, ibtype):
+    """mgp_divcal_device: noib, and ghostcell with BP and the source SDIV."""
+    import torch
+    g = topology("aniso")
+    o = oracle_mod.Oracle(g)
+    bp = sdiv = None
+    if ibtype:
+        bp = 1.0 - blocked_mask(g, 150, 0.1)
+        o.exchange([bp], layers=2, corners=True)
+        sdiv = np.where(g.interior_mask(), np.random.default_rng(151).standard_normal(g.n3d),
+                        0.0)
+    u, v, w = velocity(g, o, 152)
+    ref = g.zeros()
+    o.divcal(ref, u, v, w, 3.5, bp=bp, sdiv=sdiv)
+    s = _solver(g, ib={0: "noib", 1: "ghostcell"}[ibtype], bp=bp)
+    if ibtype:
+        s.set_sdiv(sdiv)
+    got = s.divcal(*[torch.from_numpy(a).cuda() for a in (u, v, w)], 3.5).cpu().numpy()
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("name", ["box2", "walls"])
+@pytest.mark.parametrize("ityp", [0, 2])
+def test_sdiv_source_bitwise(oracle_mod, name, ityp):
+    """Ghostcell projection with a nonzero geometry source SDIV (seeded
+    normal values on interior cells): FP64 bitwise equal to the oracle over
+    three outer iterations; resetting SDIV gives the solve without it."""
+    import mgp
+    g = topology(name)
+    o = oracle_mod.Oracle(g)
+    bp = 1.0 - blocked_mask(g, 140, 0.1)
+    o.exchange([bp], layers=2, corners=True)
+    st = o.setup(bp=bp, ityp=ityp)
+    u, v, w = velocity(g, o, 141)
+    sdiv = np.where(g.interior_mask(), np.random.default_rng(142).standard_normal(g.n3d), 0.0)
+    ins = [a.copy() for a in (u, v, w, np.zeros(g.n3d))]
+    ref = [a.copy() for a in ins]
+    so = o.mgpoisl(st, *ref, 1e-3, ityp=ityp, ninner=5, nouter=3, ibtype=1, sdiv=sdiv)
+    s = mgp.PressureSolver(g, type={0: "sip", 2: "classic"}[ityp], ninner=5, nouter=3,
+                           ib="ghostcell", bp=bp)
+    s.set_sdiv(sdiv)
+    got = [a.copy() for a in ins]
+    sg = s.mgpoisl(*got, 1e-3)
+    assert sg["maxrhs0"] == so["maxrhs0"] and sg["maxrhs"] == so["maxrhs"]
+    for a, b, nm in zip(ref, got, "uvwp"):
+        assert np.array_equal(a, b), nm
+    s.set_sdiv(None)
+    ref0 = [a.copy() for a in ins]
+    so0 = o.mgpoisl(st, *ref0, 1e-3, ityp=ityp, ninner=5, nouter=3, ibtype=1)
+    got0 = [a.copy() for a in ins]
+    sg0 = s.mgpoisl(*got0, 1e-3)
+    assert sg0["maxrhs0"] == so0["maxrhs0"] != so["maxrhs0"]
+    assert np.array_equal(ref0[3], got0[3])
+
+
+def test_sdiv_rejected_without_ghostcell():
+    import mgp
+    g = topology("box2")
+    s = _solver(g)
+    with pytest.raises(mgp.MgpError):
+        s.set_sdiv(np.zeros(g.n3d))

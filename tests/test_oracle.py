@@ -795,3 +795,29 @@ def test_rkstep_first_stage(oracle_mod):
     p2, dp2 = p.copy(), dp.copy()
     o.rkstep(p2, dp2, rhs, -5.0 / 9.0, 0.5)
     assert np.array_equal(dp2, -5.0 / 9.0 * dp + rhs)
+
+
+def test_sdiv_source_cancelling_the_divergence_is_a_no_op(oracle_mod):
+    """Ghostcell mgpoisl with the geometry source SDIV = -div(u) on every
+    interior cell: the right-hand side fak*(div u + SDIV)*BP is zero, so the
+    projection leaves U, V, W and P as they were; with SDIV = 0 it does not."""
+    from cases import blocked_mask, topology, velocity
+    g = topology("box2")
+    o = oracle_mod.Oracle(g)
+    bp = 1.0 - blocked_mask(g, 7, 0.1)
+    o.exchange([bp], layers=2, corners=True)
+    st = o.setup(bp=bp)
+    u, v, w = velocity(g, o, 8)
+    div = g.zeros()
+    o.divcal(div, u, v, w, 1.0)
+    sdiv = -div
+    p = np.full(g.n3d, 0.25)
+    ins = [a.copy() for a in (u, v, w, p)]
+    so = o.mgpoisl(st, u, v, w, p, 1e-3, ninner=5, nouter=2, ibtype=1, sdiv=sdiv)
+    assert so["maxrhs0"] == 0.0
+    inner = g.interior_mask()
+    for a, b in zip((u, v, w, p), ins):
+        assert np.array_equal(a[inner], b[inner])
+    ref = [a.copy() for a in ins]
+    o.mgpoisl(st, *ref, 1e-3, ninner=5, nouter=2, ibtype=1)
+    assert np.abs(ref[3] - ins[3])[inner].max() > 0
