@@ -145,24 +145,38 @@ __device__ __forceinline__ void ap_rows(double *c, const double *ax,
     }
 }
 
+// The block's wall fold-in terms, copied to registers before a plane loop:
+// read through the Block reference inside the loop they would be reloaded
+// at every step (the compiler cannot prove them unaliased by the kernel's
+// stores), and each reload would wait for every load in flight.
+struct WallFold {
+    uint32_t mask;
+    double w[6];
+    __device__ __forceinline__ void load(const Block &B) {
+        mask = (uint32_t)B.wallmask;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) w[f] = B.wall[f];
+    }
+};
+
 // GSAP of a cell of a block without blocked cells (BP = 1 on the block
 // and its ghost layer), in the order of src/ib/noib_mod.F90:154-244: the
 // separable part, the six a_nb (1 - BP BP_nb) terms (all +0 here, exact),
 // then the wall fold-ins of the boundary cells (a_nb BP BP_nb = a_nb).
 // Equal bit for bit to the stored GSAP.
 __device__ __forceinline__ double ap_cell(const double *ar, int nr, int i, int j,
-                                          int k, uint32_t fl, const Block &B) {
+                                          int k, uint32_t fl, const WallFold &B) {
     double a = -ar[i];
     a = a - ar[nr + j];
     a = a - ar[2 * nr + k];
-    const uint32_t w = fl & (uint32_t)B.wallmask;
+    const uint32_t w = fl & B.mask;
     if (w) {
-        if (w & 1) a = a + B.wall[0];
-        if (w & 2) a = a + B.wall[1];
-        if (w & 4) a = a + B.wall[2];
-        if (w & 8) a = a + B.wall[3];
-        if (w & 16) a = a + B.wall[4];
-        if (w & 32) a = a + B.wall[5];
+        if (w & 1) a = a + B.w[0];
+        if (w & 2) a = a + B.w[1];
+        if (w & 4) a = a + B.w[2];
+        if (w & 8) a = a + B.w[3];
+        if (w & 16) a = a + B.w[4];
+        if (w & 32) a = a + B.w[5];
     }
     return a;
 }
@@ -341,7 +355,7 @@ struct FwdK {
     T *res, *rlo, *rw;
     const T *cf;
     const double *ar;
-    const Block *blk;
+    WallFold wf;
     T *R, *H;
     uint8_t *BH;
     int NI, NJ, NK, HW, RW, RS, mHS, nr;
@@ -367,7 +381,7 @@ struct FwdK {
                       sp = ((p + 2) & 3) * mHS;
             const T *aw = cf, *ae = cf + nr, *as = cf + 2 * nr,
                     *an = cf + 3 * nr, *ab = cf + 4 * nr, *at = cf + 5 * nr;
-            const T apv = APC ? (T)ap_cell(ar, nr, i, j, k, d.c.y >> 24, *blk) : d.ap;
+            const T apv = APC ? (T)ap_cell(ar, nr, i, j, k, d.c.y >> 24, wf) : d.ap;
             if (MASKED && masked)
                 r = laplace_cell<T, T, true>(H + sm, H + s0, H + sp, BH + sm, BH + s0,
                                              BH + sp, c0, HW, aw[i], ae[i], as[j], an[j],
@@ -457,7 +471,7 @@ __global__ __launch_bounds__(kWave) void k_forward(
     k.rw = write_hi ? F.rhi[rd ^ 1] + fc : nullptr;
     k.cf = cf;
     k.ar = ar;
-    k.blk = &B;
+    k.wf.load(B);
     k.R = R;
     k.H = H;
     k.BH = BH;
@@ -609,7 +623,7 @@ struct SorK {
     T *hilf, *fout;
     const T *cf;
     const double *ar;
-    const Block *blk;
+    WallFold wf;
     T *H;
     uint8_t *BH;
     int NI, NJ, NK, HW, mHS, nr;
@@ -658,7 +672,7 @@ struct SorK {
                 rap = d.rap;
             } else {
                 // SOR_RAP = divide0(1, GSAP), src/flow/sor_mod.F90:18-49
-                const double a = ap_cell(ar, nr, i, j, k, d.c.y >> 24, *blk);
+                const double a = ap_cell(ar, nr, i, j, k, d.c.y >> 24, wf);
                 rap = (T)(a == 0.0 ? 0.0 : 1.0 / a);
             }
             r = (r - d.rhs) * rap;
@@ -715,7 +729,7 @@ __global__ __launch_bounds__(kWave) void k_sor(
     k.fout = fout + B.fc;
     k.cf = cf;
     k.ar = ar;
-    k.blk = &B;
+    k.wf.load(B);
     k.H = H;
     k.BH = BH;
     k.NI = NI;
@@ -750,7 +764,7 @@ struct OutK {
     };
     const uint2 *cells;
     const double *ar;
-    const Block *blk;
+    WallFold wf;
     const double *ap;
     double *rhs, *dp;
     T *rhs_t;
@@ -772,7 +786,7 @@ struct OutK {
         cell_ijk(d.c.x, i, j, k);
         const int c0 = (j + 1) + HW * (i + 1);
         const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS, sp = ((p + 2) & 3) * mHS;
-        const double apv = MASKED ? d.ap : ap_cell(ar, nr, i, j, k, d.c.y >> 24, *blk);
+        const double apv = MASKED ? d.ap : ap_cell(ar, nr, i, j, k, d.c.y >> 24, wf);
         const double *aw = cf, *ae = cf + nr, *as = cf + 2 * nr,
                      *an = cf + 3 * nr, *ab = cf + 4 * nr, *at = cf + 5 * nr;
         double r;
@@ -823,7 +837,7 @@ __global__ __launch_bounds__(kWave) void k_outer(
     OutK<T, MASKED> k;
     k.cells = t.cellx + S.cell_off;
     k.ar = ar;
-    k.blk = &B;
+    k.wf.load(B);
     k.ap = o.ap64 + wo;
     k.rhs = o.rhs64 + wo;
     k.dp = o.dp64 + wo;
