@@ -373,6 +373,9 @@ def main():
     ap.add_argument("--tts", choices=["none", "c2", "main"], default="c2",
                     help="time-to-1e-10 on the 8x8x8-block case (c2), the main case, or skip")
     ap.add_argument("--tts-cap", type=int, default=4000, help="max outer iterations")
+    ap.add_argument("--tts-history", default=None,
+                    help="write each form's max|RHS| per outer iteration of the tts solve "
+                         "to this .npz file")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp32,fp64")
     ap.add_argument("--stage", action="store_true",
@@ -488,7 +491,7 @@ def main():
         tf = make_velocity(tg, tnb, n, seed=4) if args.tts == "c2" else fields0
         tts = {"config": f"{tg.ngrid} blocks of {n[0]}^3 ({tnb[0]}x{tnb[1]}x{tnb[2]} periodic)",
                "eps_rel": 1e-10}
-        ps = {}
+        ps, hist = {}, {}
         for form in forms:
             s = mgp.PressureSolver(tg, ninner=args.ninner, nouter=1, precision=form)
             if form == forms[0] and args.tts == "c2":
@@ -496,6 +499,8 @@ def main():
             time_to_solution(s, tf, args.ninner, 1e-10, 2)        # warm
             t, st, p = time_to_solution(s, tf, args.ninner, 1e-10, args.tts_cap)
             ps[form] = p
+            if args.tts_history:
+                hist[form] = s.history()
             tts[form] = {"seconds": round(t, 4), "inner_iterations": st["ipc"],
                          "converged": st["converged"],
                          "rel_residual": st["maxrhs"] / st["maxrhs0"]}
@@ -508,6 +513,9 @@ def main():
             tts["fp32_pressure_err_vs_fp64"] = float((a - b).abs().max() / b.abs().max())
             tts["speedup_fp32_over_fp64"] = round(tts["fp64"]["seconds"] / tts["fp32"]["seconds"], 3)
         line["tts"] = tts
+        if args.tts_history:
+            import numpy as np
+            np.savez(args.tts_history, **hist)
 
     if not args.no_cpu:
         log("[bench] cpu baseline")

@@ -138,6 +138,7 @@ struct mgp_handle {
     // 1-D fields (FP64 and working precision)
     DBuf d_dx, d_dy, d_dz, d_rdx, d_rdy, d_rdz, d_rddx, d_rddy, d_rddz;
     DBuf d_ddx, d_ddy, d_ddz;
+    DBuf d_rpd[3];       // 1 / (rho dx), 1 / (rho dy), 1 / (rho dz) (tstle4)
     DBuf d_raw[6], d_c64[6], d_cT[6], d_apd[3];
     // per-cell WO arrays
     DBuf d_ap64, d_apT, d_f[7], d_hilf, d_rhs64, d_rhsT, d_res, d_dp64;
@@ -985,6 +986,15 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
         upload(h->d_rddx, recip(ddx, h->n1x));
         upload(h->d_rddy, recip(ddy, h->n1y));
         upload(h->d_rddz, recip(ddz, h->n1z));
+        {
+            const double *d[3] = {dx, dy, dz};
+            const int64_t m[3] = {h->n1x, h->n1y, h->n1z};
+            for (int a = 0; a < 3; ++a) {
+                std::vector<double> r(m[a]);
+                for (int64_t q = 0; q < m[a]; ++q) r[q] = 1.0 / (h->cfg.rho * d[a][q]);
+                upload(h->d_rpd[a], r);
+            }
+        }
         std::vector<double> raw[6], fold[6];
         raw[0].assign(h->n1x, 0.0); raw[1].assign(h->n1x, 0.0);
         raw[2].assign(h->n1y, 0.0); raw[3].assign(h->n1y, 0.0);
@@ -1788,7 +1798,9 @@ int mgp_tstle4_device(mgp_handle *h, mgletreal *uo, mgletreal *vo, mgletreal *wo
         a.rdx = h->d_rdx.as<double>(); a.rdy = h->d_rdy.as<double>(); a.rdz = h->d_rdz.as<double>();
         a.rddx = h->d_rddx.as<double>(); a.rddy = h->d_rddy.as<double>();
         a.rddz = h->d_rddz.as<double>();
-        a.rho = h->cfg.rho;
+        a.rpdx = h->d_rpd[0].as<double>(); a.rpdy = h->d_rpd[1].as<double>();
+        a.rpdz = h->d_rpd[2].as<double>();
+        a.rrho = 1.0 / h->cfg.rho;
         a.gmol = gmol;
         a.iles = ilesmodel == 0 ? 0.0 : 1.0;
         for (int q = 0; q < 3; ++q) a.gp[q] = gradp ? gradp[q] : 0.0;

@@ -140,7 +140,8 @@ struct MomArgs {
     const double *u, *v, *w, *p, *g;
     const double *dx, *dy, *dz, *ddx, *ddy, *ddz;
     const double *rdx, *rdy, *rdz, *rddx, *rddy, *rddz;
-    double rho, gmol, iles;
+    const double *rpdx, *rpdy, *rpdz;   // 1 / (rho dx) etc.
+    double rrho, gmol, iles;            // 1 / rho
     double gp[3];      // body-force pressure gradient (gradp)
 };
 void launch_tstle4(const Topo &t, const MomArgs &a, hipStream_t s);

@@ -29,6 +29,8 @@ __device__ __forceinline__ double gpair(double a, double b, double gmol) {
 // the LES model is on) - pressure gradient and body-force gradient
 // (tstle4_gradp :461-529); zero outside the loop ranges, as the reference
 // zeroes uo before filling it.  The transporting velocities are u, v, w.
+// 1/rho and 1/(rho dx) come from host tables: the same correctly rounded
+// quotients, without a division per position.
 // One-dimensional grid of nchunk workgroups per block, consecutive
 // workgroups walking one block, so the i-1 plane a workgroup reads was
 // fetched by one a few workgroups earlier.
@@ -57,7 +59,9 @@ __global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a, int n
     const int nfu = B.bc[0] == 3, nbu = B.bc[1] == 7 || B.bc[1] == 3;
     const int nrv = B.bc[2] == 3, nlv = B.bc[3] == 7 || B.bc[3] == 3;
     const int nbw = B.bc[4] == 3, ntw = B.bc[5] == 7 || B.bc[5] == 3;
-    const double rho = a.rho, gmol = a.gmol, iles = a.iles;
+    const double rrho = a.rrho, gmol = a.gmol, iles = a.iles;
+    const double *rpx = a.rpdx + B.o1x - 1, *rpy = a.rpdy + B.o1y - 1,
+                 *rpz = a.rpdz + B.o1z - 1;
     const bool inj = j >= 3 && j <= jj - 2, ink = k >= 3 && k <= kk - 2,
                ini = i >= 3 && i <= ii - 2;
 #define U(o) u[c + (o)]
@@ -95,9 +99,9 @@ __global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a, int n
                           + ((gn * (V(si) - V(0))) - (gs * (V(si - sj) - V(-sj)))) * ddz[k]
                           + ((gt * (W(si) - W(0))) - (gb * (W(si - 1) - W(-1)))) * ddy[j];
         const double qc = st * iles;
-        const double fak = 1.0 / rho * rddy[j] * rdx[i] * rddz[k];
+        const double fak = rrho * rddy[j] * rdx[i] * rddz[k];
         ro = ro - fak * (qe - qw + qn - qs + qt - qb - qc);
-        ro = ro - 1.0 / (rho * dx[i]) * (p[c + si] - p[c] + a.gp[0] * dx[i]);
+        ro = ro - rpx[i] * (p[c + si] - p[c] + a.gp[0] * dx[i]);
     }
     a.uo[c] = ro;
     ro = 0.0;
@@ -131,9 +135,9 @@ __global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a, int n
                           + ((gn * (V(sj) - V(0)) * rddy[j + 1]) - (gs * (V(0) - V(-sj)) * rddy[j])) * ay
                           + ((gt * (W(sj) - W(0))) - (gb * (W(sj - 1) - W(-1)))) * ddx[i];
         const double qc = st * iles;
-        const double fak = 1.0 / rho * rddx[i] * rdy[j] * rddz[k];
+        const double fak = rrho * rddx[i] * rdy[j] * rddz[k];
         ro = ro - fak * (qe - qw + qn - qs + qt - qb - qc);
-        ro = ro - 1.0 / (rho * dy[j]) * (p[c + sj] - p[c] + a.gp[1] * dy[j]);
+        ro = ro - rpy[j] * (p[c + sj] - p[c] + a.gp[1] * dy[j]);
     }
     a.vo[c] = ro;
     ro = 0.0;
@@ -167,9 +171,9 @@ __global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a, int n
                           + ((gn * (V(1) - V(0))) - (gs * (V(1 - sj) - V(-sj)))) * ddx[i]
                           + ((gt * (W(1) - W(0)) * rddz[k + 1]) - (gb * (W(0) - W(-1)) * rddz[k])) * az;
         const double qc = st * iles;
-        const double fak = 1.0 / rho * rddx[i] * rddy[j] * rdz[k];
+        const double fak = rrho * rddx[i] * rddy[j] * rdz[k];
         ro = ro - fak * (qe - qw + qn - qs + qt - qb - qc);
-        ro = ro - 1.0 / (rho * dz[k]) * (p[c + 1] - p[c] + a.gp[2] * dz[k]);
+        ro = ro - rpz[k] * (p[c + 1] - p[c] + a.gp[2] * dz[k]);
     }
     a.wo[c] = ro;
 #undef U
