@@ -214,14 +214,16 @@ int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms);
 /* Pipeline shape of the plane kernels (the sweeps, SOR and the outer
  * update): 1 = four 64-cell items of loads in flight per wave, 2 = two
  * 128-cell items, 3 = three 128-cell items, 4 = six 128-cell items, 5 =
- * eight 64-cell items.  request -1 only reports; 0 runs the autotuner:
+ * eight 64-cell items, all one wave per block; 6 and 7 = four 64-cell
+ * items per wave with 4 and 8 waves sharing each block (SOR keeps one
+ * wave).  request -1 only reports; 0 runs the autotuner:
  * it times one inner iteration per shape (ninner x forward + backward, or
  * SOR, plus one outer update) on h's own blocks and keeps the fastest;
  * 1..MGP_NUM_PLAN_SHAPES sets the shape.  Results do not depend on the
  * shape.  chosen gets the shape in use, ms[MGP_NUM_PLAN_SHAPES] (may be
  * NULL) the timed ms per shape (0 when not timed).  mgp_init autotunes
  * unless MGP_PLAN_SHAPE=1..MGP_NUM_PLAN_SHAPES is set. */
-#define MGP_NUM_PLAN_SHAPES 5
+#define MGP_NUM_PLAN_SHAPES 7
 int mgp_plan_shape(mgp_handle *h, mgletint request, mgletint *chosen,
                    double *ms);
 

@@ -642,23 +642,40 @@ void build_plans(mgp_handle &h, const std::vector<uint32_t> &cells,
                 v.y = (uint32_t)epos | (uint32_t)(p + 8) << 21;
                 return v;
             };
-            for (int wi = 0; wi < 2; ++wi) {
-                const int W = 64 << wi;
-                S.plan_item[wi][kind] = (int)items.size();
-                for (const Step &st : steps) {
-                    const int n = std::max((st.ccnt + W - 1) / W, (st.ecnt + W - 1) / W);
-                    for (int c = 0; c < n; ++c) {
-                        const int cc = std::min(std::max(st.ccnt - W * c, 0), W);
-                        const int ec = std::min(std::max(st.ecnt - W * c, 0), W);
-                        items.push_back(pack(cc ? st.cpos + W * c : 0, cc, ec,
-                                             ec ? st.epos + W * c : 0, st.p));
+            for (int nwi = 0; nwi < 3; ++nwi) {
+                const int NW = kPlanWaves[nwi];
+                for (int wi = 0; wi < 2; ++wi) {
+                    const int W = 64 << wi;
+                    const size_t base = items.size();
+                    S.plan_item[nwi][wi][kind] = (int)base;
+                    for (const Step &st : steps) {
+                        const int n = std::max((st.ccnt + W - 1) / W, (st.ecnt + W - 1) / W);
+                        const int rows = (n + NW - 1) / NW;
+                        for (int c = 0; c < rows * NW; ++c) {
+                            if (c >= n) {   // padding: a no-op of the same step
+                                items.push_back(pack(0, 0, 0, 0, st.p));
+                                continue;
+                            }
+                            const int cc = std::min(std::max(st.ccnt - W * c, 0), W);
+                            const int ec = std::min(std::max(st.ecnt - W * c, 0), W);
+                            items.push_back(pack(cc ? st.cpos + W * c : 0, cc, ec,
+                                                 ec ? st.epos + W * c : 0, st.p));
+                        }
                     }
+                    // closing no-op items (the engine clamps item indices to
+                    // the last one)
+                    do
+                        for (int w = 0; w < NW; ++w) items.push_back(pack(0, 0, 0, 0, 0));
+                    while (((items.size() - base) / NW) % kPlanPad);
+                    const int per = (int)((items.size() - base) / NW);
+                    S.plan_nitem[nwi][wi][kind] = per;
+                    // every wave must meet the same steps (its barriers)
+                    for (int q = 0; q < per; ++q)
+                        for (int w = 1; w < NW; ++w)
+                            if ((items[base + (size_t)q * NW + w].y >> 21) !=
+                                (items[base + (size_t)q * NW].y >> 21))
+                                throw HipError{"plan steps differ between waves"};
                 }
-                // a closing no-op item (the engine clamps item indices to it)
-                do
-                    items.push_back(pack(0, 0, 0, 0, 0));
-                while ((items.size() - S.plan_item[wi][kind]) % kPlanPad);
-                S.plan_nitem[wi][kind] = (int)items.size() - S.plan_item[wi][kind];
             }
         }
     }

@@ -29,20 +29,26 @@ struct Shape {
     int64_t pstart_off;// offset into the plane start table (np + 1 entries)
     int64_t slot_off;  // offset into the spatial->slot table (ncell entries)
     int fsize;         // face buffer entries per block: 2(NK NJ + NK NI + NJ NI)
-    int plan_item[2][5];  // per item width (64, 128 cells) and plan kind:
-                          // offset of the shape's items
-    int plan_nitem[2][5]; // items (a multiple of kPlanPad)
+    // per wave count (kPlanWaves), item width (64, 128 cells) and plan
+    // kind: offset of the shape's items, and items per wave (a multiple of
+    // kPlanPad)
+    int plan_item[3][2][5];
+    int plan_nitem[3][2][5];
     int plan_ent[5];      // per plan kind: offset of the shape's entries
 };
 
-// Plans of the single-wave plane kernels (built per shape on the host).
-// A plan is a sequence of items; item n = {x, y} with x = cpos | ccnt << 16
-// | ecnt << 24 and y = epos | (p + 8) << 21: the compute part handles WO
-// cells [cpos, cpos + ccnt) of one hyperplane, the entry part moves
-// entries [epos, epos + ecnt) of the plan's entry table into LDS, and p is
-// the step the item belongs to (it selects the LDS ring slots).
+// Plans of the plane kernels (built per shape on the host).  A plan is a
+// sequence of items; item n = {x, y} with x = cpos | ccnt << 16 | ecnt <<
+// 24 and y = epos | (p + 8) << 21: the compute part handles WO cells
+// [cpos, cpos + ccnt) of one hyperplane, the entry part moves entries
+// [epos, epos + ecnt) of the plan's entry table into LDS, and p is the step
+// the item belongs to (it selects the LDS ring slots).  With NW waves per
+// block the items of each step are dealt to the waves in turn, and steps are
+// padded with no-op items of the same p so that every wave sees the same
+// sequence of steps; item q of wave w is stored at q NW + w.
 enum PlanKind { kPlanFwdLap = 0, kPlanFwdZero, kPlanBwd, kPlanOut, kPlanSor, kPlanKinds };
 constexpr int kPlanPad = 8;
+constexpr int kPlanWaves[3] = {1, 4, 8};
 // Entry: LDS index (12 bits), source kind (3 bits: own face 0-5 of the
 // ghost, 6 = the block's WO cells), target (bit 15: 0 = window ring slot,
 // 1 = recurrence buffer) and the offset into the source (16 bits).

@@ -113,7 +113,7 @@ void launch_sor(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
 // pipeline shape of the plane kernels (1: 4 items of 64 cells in flight,
 // 2: 2 items of 128 cells, 3: 3 items of 128 cells)
 // pipeline shapes of the plane kernels (see set_plan_shape)
-constexpr int kNumShapes = 5;
+constexpr int kNumShapes = 7;
 void set_plan_shape(int v);
 int plan_shape();
 

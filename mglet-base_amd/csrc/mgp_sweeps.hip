@@ -63,6 +63,22 @@ __device__ __forceinline__ T gld(const T *p) {
     return *(const __attribute__((address_space(1))) T *)p;
 }
 
+// Index of a per-block wave count in the plan tables (kPlanWaves).
+template <int NW>
+constexpr int plan_nw() {
+    static_assert(NW == 1 || NW == 4 || NW == 8, "no plans for this wave count");
+    return NW == 1 ? 0 : (NW == 4 ? 1 : 2);
+}
+
+// Barrier of the waves of a block over LDS only: orders LDS accesses
+// across it without waiting for the global loads in flight (a full
+// __syncthreads would drain the prefetch pipeline at every plane).
+__device__ __forceinline__ void block_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Per-wave table of entry sources, by entry kind.  Absent sources point at
 // the zero buffer, so every entry load is valid and yields 0 there.
 template <typename T>
@@ -121,7 +137,7 @@ __device__ __forceinline__ void coef_rows(C *c, const C *aw, const C *ae,
                                           const C *as, const C *an,
                                           const C *ab, const C *at,
                                           const Block &B, int nr) {
-    for (int e = threadIdx.x; e < 6 * nr; e += kWave) {
+    for (int e = threadIdx.x; e < 6 * nr; e += blockDim.x) {
         const int r = e / nr, q = e % nr;
         const int n = r < 2 ? B.ni : (r < 4 ? B.nj : B.nk);
         if (q >= n) continue;
@@ -136,7 +152,7 @@ __device__ __forceinline__ void coef_rows(C *c, const C *aw, const C *ae,
 __device__ __forceinline__ void ap_rows(double *c, const double *ax,
                                         const double *ay, const double *az,
                                         const Block &B, int nr) {
-    for (int e = threadIdx.x; e < 3 * nr; e += kWave) {
+    for (int e = threadIdx.x; e < 3 * nr; e += blockDim.x) {
         const int r = e / nr, q = e % nr;
         const int n = r == 0 ? B.ni : (r == 1 ? B.nj : B.nk);
         if (q >= n) continue;
@@ -241,13 +257,13 @@ __device__ __forceinline__ void fill_tab(SrcTab<T> *tab, const Topo &t,
 // items instead of a prologue, so it is entered with no load outstanding
 // and the compiler's wait counts at the loop head are those of the steady
 // state.  Indices outside the plan are clamped to its closing no-op item.
-template <int D, int M, bool EBP, typename T, class K>
+template <int D, int M, int NW, bool EBP, typename T, class K>
 __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
                                          int nitem,
                                          const uint32_t *__restrict__ ents,
                                          const SrcTab<T> *tab) {
     constexpr int W = 2 * D;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
     const int last = nitem - 1;
     typename K::Data cd[D][M];
     uint32_t ew[W][M];
@@ -257,7 +273,8 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
     uint32_t ix[D], iy[D];
     const T *pn[M];
     const uint8_t *bn[M];
-    auto ldi = [&](int q) -> uint2 { return ld(items, (q < 0 || q > last) ? last : q); };
+    // this wave's items: every NW-th of the plan, from its own first one
+    auto ldi = [&](int q) -> uint2 { return ld(items, ((q < 0 || q > last) ? last : q) * NW); };
     auto lookup = [&](int m, uint32_t e) {
         const int kind = (e >> 12) & 7;
         pn[m] = (e & 0x8000u) ? tab->r[kind] : tab->h[kind];
@@ -326,7 +343,11 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
                 if (lane + 64 * m < ec)
                     k.store(ew[w][m], ev[u][m], EBP ? eb[u][m] : (uint8_t)0, p);
             wave_sync();
+            // several waves per block: the next step reads what all waves
+            // wrote in this one (the next item was issued D items ago)
+            const bool step_ends = NW > 1 && (iy[(w + 1) % D] >> 21) != (y >> 21);
             issue(u, (w + D) % W, q0 + w + D);
+            if (step_ends) block_sync_lds();
         }
     }
 }
@@ -431,8 +452,8 @@ struct FwdK {
     }
 };
 
-template <typename T, bool CLASSIC, bool MASKED, bool LAP, int D, int M>
-__global__ __launch_bounds__(kWave) void k_forward(
+template <typename T, bool CLASSIC, bool MASKED, bool LAP, int D, int M, int NW>
+__global__ __launch_bounds__(kWave * NW) void k_forward(
     const uint2 *__restrict__ items, Topo t, Fields<T> F, Mask m, int rd,
     int write_hi, const int *done) {
     if (*done) return;
@@ -455,7 +476,7 @@ __global__ __launch_bounds__(kWave) void k_forward(
                 0x15, m);
     if (LAP) coef_rows<T>(cf, F.aw, F.ae, F.as, F.an, F.ab, F.at, B, nr);
     if (APC) ap_rows(ar, F.apx, F.apy, F.apz, B, nr);
-    wave_sync();
+    if (NW > 1) __syncthreads(); else wave_sync();
     const int NI = B.ni, NJ = B.nj, NK = B.nk;
     const int64_t wo = B.wo, fc = B.fc;
     FwdK<T, CLASSIC, MASKED, LAP> k;
@@ -491,8 +512,10 @@ __global__ __launch_bounds__(kWave) void k_forward(
     k.fb5 = face_base(5, NI, NJ, NK);
     k.masked = MASKED && !(B.flags & 1);
     const int kind = LAP ? kPlanFwdLap : kPlanFwdZero;
-    run_plan<D, M, MASKED && LAP, T>(k, items + S.plan_item[M - 1][kind],
-                                     S.plan_nitem[M - 1][kind], t.ents + S.plan_ent[kind], tab);
+    constexpr int nw = plan_nw<NW>();
+    run_plan<D, M, NW, MASKED && LAP, T>(k, items + S.plan_item[nw][M - 1][kind] + threadIdx.x / kWave,
+                                         S.plan_nitem[nw][M - 1][kind], t.ents + S.plan_ent[kind],
+                                         tab);
 }
 
 // ---------------------------------------------------------------------
@@ -558,8 +581,8 @@ struct BwdK {
     }
 };
 
-template <typename T, bool CLASSIC, bool HZ, int D, int M>
-__global__ __launch_bounds__(kWave) void k_backward(
+template <typename T, bool CLASSIC, bool HZ, int D, int M, int NW>
+__global__ __launch_bounds__(kWave * NW) void k_backward(
     const uint2 *__restrict__ items, Topo t, Fields<T> F, const int *done) {
     if (*done) return;
     extern __shared__ __align__(16) unsigned char lds_bw[];
@@ -570,7 +593,7 @@ __global__ __launch_bounds__(kWave) void k_backward(
     T *R = (T *)(lds_bw + sizeof(SrcTab<T>));
     // R sources: the neighbours' lower planes feed faces 1, 3, 5
     fill_tab<T>(tab, t, B, nullptr, nullptr, F.hilf, F.rlo, 0x2a, Mask{});
-    wave_sync();
+    if (NW > 1) __syncthreads(); else wave_sync();
     const int NI = B.ni, NJ = B.nj, NK = B.nk;
     const int64_t wo = B.wo;
     BwdK<T, CLASSIC, HZ> k;
@@ -593,8 +616,10 @@ __global__ __launch_bounds__(kWave) void k_backward(
     k.fb3 = face_base(3, NI, NJ, NK);
     k.fb4 = face_base(4, NI, NJ, NK);
     k.fb5 = face_base(5, NI, NJ, NK);
-    run_plan<D, M, false, T>(k, items + S.plan_item[M - 1][kPlanBwd],
-                             S.plan_nitem[M - 1][kPlanBwd], t.ents + S.plan_ent[kPlanBwd], tab);
+    constexpr int nw = plan_nw<NW>();
+    run_plan<D, M, NW, false, T>(k, items + S.plan_item[nw][M - 1][kPlanBwd] + threadIdx.x / kWave,
+                                 S.plan_nitem[nw][M - 1][kPlanBwd], t.ents + S.plan_ent[kPlanBwd],
+                                 tab);
 }
 
 // ---------------------------------------------------------------------
@@ -748,8 +773,9 @@ __global__ __launch_bounds__(kWave) void k_sor(
     k.om1 = (T)(1.0 - omega);
     k.masked = MASKED && !(B.flags & 1);
     k.keep_top = (NK & 1) != 0;
-    run_plan<D, M, MASKED, T>(k, items + S.plan_item[M - 1][kPlanSor],
-                              S.plan_nitem[M - 1][kPlanSor], t.ents + S.plan_ent[kPlanSor], tab);
+    run_plan<D, M, 1, MASKED, T>(k, items + S.plan_item[0][M - 1][kPlanSor],
+                                 S.plan_nitem[0][M - 1][kPlanSor], t.ents + S.plan_ent[kPlanSor],
+                                 tab);
 }
 
 // ---------------------------------------------------------------------
@@ -813,8 +839,8 @@ struct OutK {
     }
 };
 
-template <typename T, bool MASKED, int D, int M>
-__global__ __launch_bounds__(kWave) void k_outer(
+template <typename T, bool MASKED, int D, int M, int NW>
+__global__ __launch_bounds__(kWave * NW) void k_outer(
     const uint2 *__restrict__ items, Topo t, Fields<T> F, Mask m, Outer o,
     T *rhs_t) {
     if (*o.done) return;
@@ -832,7 +858,7 @@ __global__ __launch_bounds__(kWave) void k_outer(
     fill_tab<T>(tab, t, B, F.hface, F.hpar, F.hilf, nullptr, 0, m);
     coef_rows<double>(cf, o.aw, o.ae, o.as, o.an, o.ab, o.at, B, nr);
     if (!MASKED) ap_rows(ar, F.apx, F.apy, F.apz, B, nr);
-    wave_sync();
+    if (NW > 1) __syncthreads(); else wave_sync();
     const int64_t wo = B.wo;
     OutK<T, MASKED> k;
     k.cells = t.cellx + S.cell_off;
@@ -850,22 +876,30 @@ __global__ __launch_bounds__(kWave) void k_outer(
     k.nr = nr;
     k.masked = MASKED && !(B.flags & 1);
     k.mx = 0.0;
-    run_plan<D, M, MASKED, T>(k, items + S.plan_item[M - 1][kPlanOut],
-                              S.plan_nitem[M - 1][kPlanOut], t.ents + S.plan_ent[kPlanOut], tab);
+    constexpr int nw = plan_nw<NW>();
+    run_plan<D, M, NW, MASKED, T>(k, items + S.plan_item[nw][M - 1][kPlanOut] + threadIdx.x / kWave,
+                                  S.plan_nitem[nw][M - 1][kPlanOut], t.ents + S.plan_ent[kPlanOut],
+                                  tab);
     double mx = k.mx * ldexp(1.0, t.maxlevel - B.level);
     for (int off = 32; off > 0; off >>= 1) {
         const double x = __shfl_down(mx, off, kWave);
         mx = (x > mx || mx != mx) ? x : mx;
     }
-    if (threadIdx.x == 0) atomicMax(o.gmax, (unsigned long long)__double_as_longlong(mx));
+    if ((threadIdx.x & (kWave - 1)) == 0)   // one per wave
+        atomicMax(o.gmax, (unsigned long long)__double_as_longlong(mx));
 }
 
 // ---------------------------------------------------------------------
 // Launchers.  The pipeline shape is a tuning knob, set_plan_shape(): 1 =
 // 4 items of 64 cells in flight, 2 = 2 items of 128 cells (two cells per
 // lane, independent, so their latencies overlap), 3 = 3 items of 128,
-// 4 = 6 items of 128, 5 = 8 items of 64.  The deep shapes hide more memory
-// latency per wave, for launches with too few blocks to fill the chip.
+// 4 = 6 items of 128, 5 = 8 items of 64, all one wave per block; 6 and 7 =
+// 4 items of 64 in flight per wave with 4 and 8 waves per block, which
+// share each hyperplane's items and meet at an LDS barrier per plane.
+// Launches with too few blocks to fill the chip are bound by one wave's
+// instruction stream; the deep and the multi-wave shapes are for them.
+// SOR relaxes HILF in place in global memory, which a prefetching wave
+// must not race, so it runs one wave per block in every shape.
 static int g_shape = 1;
 
 void set_plan_shape(int v) { g_shape = (v >= 1 && v <= kNumShapes) ? v : 1; }
@@ -878,11 +912,13 @@ template <typename T, bool CLASSIC, bool MASKED, bool LAP>
 static void fwd_launch(const Topo &t, const Fields<T> &f, Mask m, int rd,
                        int write_hi, const int *done, size_t lds, hipStream_t s) {
     switch (g_shape) {
-    case 1: k_forward<T, CLASSIC, MASKED, LAP, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
-    case 3: k_forward<T, CLASSIC, MASKED, LAP, 3, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
-    case 4: k_forward<T, CLASSIC, MASKED, LAP, 6, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
-    case 5: k_forward<T, CLASSIC, MASKED, LAP, 8, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
-    default: k_forward<T, CLASSIC, MASKED, LAP, 2, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    case 1: k_forward<T, CLASSIC, MASKED, LAP, 4, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    case 3: k_forward<T, CLASSIC, MASKED, LAP, 3, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    case 4: k_forward<T, CLASSIC, MASKED, LAP, 6, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    case 5: k_forward<T, CLASSIC, MASKED, LAP, 8, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    case 6: k_forward<T, CLASSIC, MASKED, LAP, 4, 1, 4><<<t.nlist, kWave * 4, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    case 7: k_forward<T, CLASSIC, MASKED, LAP, 4, 1, 8><<<t.nlist, kWave * 8, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    default: k_forward<T, CLASSIC, MASKED, LAP, 2, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
     }
 }
 
@@ -919,11 +955,13 @@ template <typename T, bool CLASSIC, bool HZ>
 static void bwd_launch(const Topo &t, const Fields<T> &f, const int *done,
                        size_t lds, hipStream_t s) {
     switch (g_shape) {
-    case 1: k_backward<T, CLASSIC, HZ, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
-    case 3: k_backward<T, CLASSIC, HZ, 3, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
-    case 4: k_backward<T, CLASSIC, HZ, 6, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
-    case 5: k_backward<T, CLASSIC, HZ, 8, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
-    default: k_backward<T, CLASSIC, HZ, 2, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
+    case 1: k_backward<T, CLASSIC, HZ, 4, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
+    case 3: k_backward<T, CLASSIC, HZ, 3, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
+    case 4: k_backward<T, CLASSIC, HZ, 6, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
+    case 5: k_backward<T, CLASSIC, HZ, 8, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
+    case 6: k_backward<T, CLASSIC, HZ, 4, 1, 4><<<t.nlist, kWave * 4, lds, s>>>(t.items, t, f, done); break;
+    case 7: k_backward<T, CLASSIC, HZ, 4, 1, 8><<<t.nlist, kWave * 8, lds, s>>>(t.items, t, f, done); break;
+    default: k_backward<T, CLASSIC, HZ, 2, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
     }
 }
 
@@ -954,6 +992,8 @@ static void sor_launch(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
     case 3: k_sor<T, MASKED, HZ, 3, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
     case 4: k_sor<T, MASKED, HZ, 6, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
     case 5: k_sor<T, MASKED, HZ, 8, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
+    case 6:   // several waves per block: SOR relaxes in place, one wave
+    case 7: k_sor<T, MASKED, HZ, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
     default: k_sor<T, MASKED, HZ, 2, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
     }
 }
@@ -986,11 +1026,13 @@ template <typename T, bool MASKED>
 static void out_launch(const Topo &t, const Fields<T> &f, Mask m, Outer o,
                        T *rhs_t, size_t lds, hipStream_t s) {
     switch (g_shape) {
-    case 1: k_outer<T, MASKED, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
-    case 3: k_outer<T, MASKED, 3, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
-    case 4: k_outer<T, MASKED, 6, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
-    case 5: k_outer<T, MASKED, 8, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
-    default: k_outer<T, MASKED, 2, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    case 1: k_outer<T, MASKED, 4, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    case 3: k_outer<T, MASKED, 3, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    case 4: k_outer<T, MASKED, 6, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    case 5: k_outer<T, MASKED, 8, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    case 6: k_outer<T, MASKED, 4, 1, 4><<<t.nlist, kWave * 4, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    case 7: k_outer<T, MASKED, 4, 1, 8><<<t.nlist, kWave * 8, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    default: k_outer<T, MASKED, 2, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
     }
 }
 

@@ -23,7 +23,7 @@ __all__ = ["PressureSolver", "MgpError", "Grids", "uniform_box"]
 _TYPES = {"sip": 0, "sor": 1, "classic": 2}
 _IB = {"noib": 0, "ghostcell": 1}
 _PREC = {"fp64": 0, "fp32": 1}
-NUM_PLAN_SHAPES = 5       # MGP_NUM_PLAN_SHAPES
+NUM_PLAN_SHAPES = 7       # MGP_NUM_PLAN_SHAPES
 
 
 def _ptr(a, ct=C.c_double):
