@@ -228,3 +228,61 @@ def mask_levels(g, o, blocked):
                                          else p0[ax] + (n[ax] - 4) // 2])
                     fv[np.ix_(*idx)] = pv[np.ix_(*cidx)]
     return bp, (bp == 0.0).astype(np.int8)
+
+
+# ---------------------------------------------------------------- reference topologies
+BC_CODES = {"FIX": 2, "OP1": 3, "NOS": 5, "SLI": 6, "CON": 7, "PAR": 8, "GRA": 15,
+            "SCA": 16, "REF": 17, "NRE": 18, "CO1": 19, "NIX": 99}
+# cases of the reference's tests/*/grids.h5 whose blocks the plane kernels
+# take (every interior count >= 2, ni*nj <= 1024)
+SUPPORTED_TOPOLOGIES = ["Checkblock", "GeostrophicWind", "HotAndColdSpheres", "Parker_micro",
+                        "Santarelli", "ScalarDecay", "Statistics", "TaylorGreen"]
+
+
+def golden_topologies():
+    import json
+    from pathlib import Path
+    with open(Path(__file__).resolve().parent / "golden" / "topologies.json") as fh:
+        return json.load(fh)
+
+
+def reference_topology(name):
+    """Grids of the reference's test case `name` (tests/<name>/grids.h5 as
+    kept in tests/golden/topologies.json): dims, levels, parents, positions,
+    NBRGRID and the first boundary condition of each face (the one the
+    pressure path reads, itypboconds(1, iface, igrid)); uniform spacing per
+    block from its bounding box."""
+    from mgp.grids import Grids
+    recs = golden_topologies()[name]["grids"]
+    assert [r["IGRID"] for r in recs] == list(range(1, len(recs) + 1))
+    dims = [(r["II"] - 4, r["JJ"] - 4, r["KK"] - 4) for r in recs]
+    box = np.array([r["BBOX"] for r in recs]).reshape(-1, 3, 2)
+    spacing = (box[:, :, 1] - box[:, :, 0]) / np.array(dims)
+    levels = [r["LEVEL"] for r in recs]
+    iparent = [max(r["IPARENT"], 0) for r in recs]
+    position = [(max(r["IPOSITION"], 0), max(r["JPOSITION"], 0), max(r["KPOSITION"], 0))
+                for r in recs]
+    bc = [[BC_CODES[t] for t in r["BC"]] for r in recs]
+    return Grids(dims, levels, [r["NBRGRID"] for r in recs], bc, spacing, box[:, :, 0],
+                 iparent=iparent, position=position)
+
+
+def lattice_ids(g):
+    """Per stored position: 1 + X + NX*(Y + NY*Z), (X, Y, Z) the cell's index
+    on its level's global lattice, wrapped periodically over the domain the
+    coarsest level spans; interior cells hold it, ghosts hold 0.  Returns
+    (field, expected) where expected holds the id at every position."""
+    lo = np.array([g.origin[b] for b in range(g.ngrid) if g.level[b] == g.minlevel]).min(axis=0)
+    hi = np.array([g.origin[b] + g.spacing[b] * (g.ii[b] - 4, g.jj[b] - 4, g.kk[b] - 4)
+                   for b in range(g.ngrid) if g.level[b] == g.minlevel]).max(axis=0)
+    field, expect = g.zeros(), g.zeros()
+    for b in range(1, g.ngrid + 1):
+        h = g.spacing[b - 1]
+        N = np.rint((hi - lo) / h).astype(np.int64)
+        o = np.rint((g.origin[b - 1] - lo) / h).astype(np.int64)
+        n = (g.ii[b - 1], g.jj[b - 1], g.kk[b - 1])
+        X, Y, Z = ((o[a] + np.arange(n[a]) - 2) % N[a] for a in range(3))
+        ids = 1 + X[:, None, None] + N[0] * (Y[None, :, None] + N[1] * Z[None, None, :])
+        g.block_view(expect, b)[...] = ids
+        g.block_view(field, b)[2:-2, 2:-2, 2:-2] = ids[2:-2, 2:-2, 2:-2]
+    return field, expect
