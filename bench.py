@@ -380,12 +380,16 @@ def main():
     ap.add_argument("--forms", default="fp32,fp64")
     ap.add_argument("--stage", action="store_true",
                     help="also time the RK stage kernels (tstle4, rkstep) on the workload")
+    ap.add_argument("--unshared", action="store_true",
+                    help="keep one copy of the SIP factors per block (MGP_SHARED_FACTORS=0)")
     ap.add_argument("--config", choices=["main", "c5"], default="main",
                     help="c5: the adjoint-projection check (not the headline line)")
     args = ap.parse_args()
     if args.config == "c5":
         adjoint_config()
         return
+    if args.unshared:
+        os.environ["MGP_SHARED_FACTORS"] = "0"
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (every block of the projection "
@@ -424,6 +428,7 @@ def main():
             stage = rk_stage_timing(s, fields0)
             log(f"[bench] rk stage: {stage}")
         res[form] = dict(wall=wall, stats=st, kernels=kt, info=s.info(),
+                         factor_classes=s.factor_classes(),
                          autotune={"shape": shape,
                                    "ms_per_outer_by_shape": [round(x, 3) for x in shape_ms]})
         log(f"[bench] {form}: {wall:.3f} s, device loop {st['solve_ms']:.1f} ms, kernels {kt}")
@@ -467,6 +472,7 @@ def main():
                       "ms_per_step": round(res[f]["wall"] / args.steps * 1e3, 3),
                       "kernel_ms": {k: round(v, 4) for k, v in res[f]["kernels"].items()},
                       "device_gb": round(res[f]["info"]["device_bytes"] / 1e9, 2),
+                      "factor_classes": res[f]["factor_classes"][0],
                       "autotune": res[f]["autotune"]}
                   for f in res},
         "roofline": {"bound": "hbm", "kernel": f"k_forward ({main_form})",
@@ -480,6 +486,13 @@ def main():
                          (ALG_BYTES[main_form]["forward"] + ALG_BYTES[main_form]["backward"])
                          * cells / ((fw_ms + r["kernels"]["backward_ms"]) * 1e-3) / 1e9, 1)},
     }
+    # whole inner iteration against the fused algorithmic bytes per
+    # cell-update (DP, RHS, AP, 7 factors, mask read, DP written: 45 B in
+    # FP32, 89 B in FP64), whatever the kernels actually move
+    fused = {"fp32": 45, "fp64": 89}[main_form]
+    line["roofline"]["fused_update"] = {
+        "bytes_per_update": fused, "achieved": round(value * fused, 1),
+        "frac": round(value * fused / HBM_PEAK_GBS, 4)}
     if stage is not None:
         line["rk_stage"] = stage
     if "fp32" in res and "fp64" in res:

@@ -236,6 +236,14 @@ int mgp_get_history(mgp_handle *h, mgletreal *out, mgletint cap,
 /* Number of interior cells and bytes of device memory held by h. */
 int mgp_info(mgp_handle *h, int64_t *ncells, int64_t *device_bytes);
 
+/* Shared SIP factor storage (no reference counterpart; the reference keeps
+ * LW..UT per grid, src/flow/sip_hyperplane_mod.F90:106-257): mgp_init
+ * groups the blocks whose factors (and BP) are equal bit for bit and keeps
+ * one copy per group.  nclasses gets the number of groups (= the number of
+ * grids when nothing is shared or MGP_SHARED_FACTORS=0 was set at init),
+ * ncells_stored the factor cells kept per factor array. */
+int mgp_factor_classes(mgp_handle *h, mgletint *nclasses, int64_t *ncells_stored);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,18 +58,21 @@ __global__ void k_tfactors(Topo t, Mask m, const T *lw, const T *ls,
     const uint32_t *cells = t.cells + S.cell_off;
     const int *slot = t.slot_of + S.slot_off;
     const int NI = B.ni, NJ = B.nj, NK = B.nk;
-    const int64_t wo = B.wo;
+    // factors and transposed factors at the block's factor offset (blocks
+    // with equal factors share them, and their transposed factors are then
+    // equal too: they involve the block's own cells only)
+    const int64_t fo = B.fo;
     const bool masked = m.bp != nullptr && !(B.flags & 1);
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < S.ncell;
          s += gridDim.x * blockDim.x) {
         int i, j, k;
         unpack_cell(cells[s], i, j, k);
-        const int64_t o = wo + s;
+        const int64_t o = fo + s;
         double a = 0.0, bs = 0.0, c = 0.0, d = 0.0, e = 0.0, f = 0.0;
-        if (!(masked && m.bp[o] == 0)) {
+        if (!(masked && m.bp[B.wo + s] == 0)) {
             const double p = (double)lpr[o];
             auto at = [&](int ii, int jj, int kk) {
-                return wo + slot[kk + NK * (jj + NJ * ii)];
+                return fo + slot[kk + NK * (jj + NJ * ii)];
             };
             if (i > 0) { const int64_t q = at(i - 1, j, k); a = p * (double)ue[q] / (double)lpr[q]; }
             if (j > 0) { const int64_t q = at(i, j - 1, k); bs = p * (double)un[q] / (double)lpr[q]; }

@@ -142,6 +142,12 @@ struct mgp_handle {
     DBuf d_raw[6], d_c64[6], d_cT[6], d_apd[3];
     // per-cell WO arrays
     DBuf d_ap64, d_apT, d_f[7], d_hilf, d_rhs64, d_rhsT, d_res, d_dp64;
+    // shared factor storage: classes of blocks with equal factors, their
+    // representatives (device list) and the factor cells stored
+    int nfclass = 0;
+    int64_t nfcell = 0;
+    std::vector<int> frep;
+    DBuf d_blist_rep;
     DBuf d_bpw, d_bpf, d_bpsp;
     DBuf d_sdiv;               // ghostcell divergence source (reference layout)
     DBuf d_hface, d_rhi[2], d_rlo;
@@ -215,6 +221,16 @@ struct mgp_handle {
             t.blist = it->second.as<int>();
             t.nlist = (int)level_blocks.at(lvl).size();
         }
+        return t;
+    }
+
+    // the representatives of the factor classes (one block per class)
+    Topo topo_reps() const {
+        if (d_blist_rep.n != (size_t)nfclass * sizeof(int))
+            throw HipError{"factor class list not built"};
+        Topo t = topo(-1);
+        t.blist = d_blist_rep.as<int>();
+        t.nlist = nfclass;
         return t;
     }
 
@@ -753,6 +769,7 @@ void build_shapes(mgp_handle &h, std::vector<uint32_t> &cells,
             if (B.bc[f] == PAR_BC) B.par |= 1 << f;
         }
         B.wo = wo;
+        B.fo = wo;
         B.sp = h.ip3[g];
         B.fc = fc;
         B.sh = sh;
@@ -806,6 +823,83 @@ void setup_factors(mgp_handle &h) {
                  h.d_c64[3].as<double>(), h.d_c64[4].as<double>(),
                  h.d_c64[5].as<double>(), h.mask(), f, h.stream);
     HC(hipGetLastError());
+}
+
+// Shared factor storage: groups the blocks whose SIP factors (and BP
+// bytes) are equal bit for bit, keeps one copy per group and points each
+// block's factor offset (Block::fo) at it.  The factors are the same
+// values, so every result is unchanged; the sweeps read a shared copy from
+// L2 instead of per-block copies from HBM.  MGP_SHARED_FACTORS=0 keeps one
+// copy per block.
+template <typename T>
+void share_factors(mgp_handle &h) {
+    const hipStream_t s = h.stream;
+    const int n = h.ngrid;
+    h.frep.resize(n);
+    for (int b = 0; b < n; ++b) h.frep[b] = b;
+    h.nfclass = n;
+    h.nfcell = h.ncell;
+    const char *env = getenv("MGP_SHARED_FACTORS");
+    const bool on = !(env && atoi(env) == 0);
+    Topo t = h.topo(-1);
+    const T *f[7];
+    for (int q = 0; q < 7; ++q) f[q] = h.d_f[q].as<T>();
+    Mask m = h.mask();
+    if (on && n > 1) {
+        DBuf dh, drep, dbad;
+        dh.alloc((size_t)n * 8);
+        launch_fhash<T>(t, f, m, dh.as<unsigned long long>(), s);
+        HC(hipGetLastError());
+        std::vector<unsigned long long> hash(n);
+        HC(hipMemcpyAsync(hash.data(), dh.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+        HC(hipStreamSynchronize(s));
+        std::map<std::pair<unsigned long long, int>, int> first;
+        for (int b = 0; b < n; ++b) {
+            auto key = std::make_pair(hash[b], h.blocks[b].shape);
+            auto it = first.find(key);
+            if (it == first.end()) first[key] = b;
+            else h.frep[b] = it->second;
+        }
+        // confirm every member against its representative, bit for bit
+        upload(drep, h.frep);
+        std::vector<int> bad(n, 0);
+        upload(dbad, bad);
+        launch_fcmp<T>(t, f, m, drep.as<int>(), dbad.as<int>(), s);
+        HC(hipGetLastError());
+        HC(hipMemcpyAsync(bad.data(), dbad.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s));
+        HC(hipStreamSynchronize(s));
+        for (int b = 0; b < n; ++b)
+            if (bad[b]) h.frep[b] = b;      // hash collision: a class of its own
+    }
+    std::vector<int> reps;
+    for (int b = 0; b < n; ++b)
+        if (h.frep[b] == b) reps.push_back(b);
+    upload(h.d_blist_rep, reps);
+    h.nfclass = (int)reps.size();
+    if (h.nfclass == n) return;         // nothing shared: fo stays wo
+    // new arrays: the representatives' factors, class after class
+    int64_t off = 0;
+    for (int r : reps) {
+        h.blocks[r].fo = off;
+        off += h.shapes[h.blocks[r].shape].ncell;
+    }
+    for (int b = 0; b < n; ++b) h.blocks[b].fo = h.blocks[h.frep[b]].fo;
+    h.nfcell = off;
+    DBuf nf[7];
+    T *g[7];
+    for (int q = 0; q < 7; ++q) {
+        nf[q].alloc((size_t)off * sizeof(T));
+        g[q] = nf[q].as<T>();
+    }
+    // the gather reads Block::wo and writes Block::fo of the new blocks table
+    upload(h.d_blocks, h.blocks);
+    launch_fgather<T>(h.topo_reps(), f, g, s);
+    HC(hipGetLastError());
+    HC(hipStreamSynchronize(s));
+    for (int q = 0; q < 7; ++q) {   // the old arrays go with nf
+        std::swap(h.d_f[q].p, nf[q].p);
+        std::swap(h.d_f[q].n, nf[q].n);
+    }
 }
 
 template <typename T>
@@ -1121,9 +1215,11 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
         HC(hipGetLastError());
         if (h->fp32()) {
             setup_factors<float>(*h);
+            share_factors<float>(*h);
             launch_convert<float>(h->d_ap64.as<double>(), h->d_apT.as<float>(), h->ncell, h->stream);
         } else {
             setup_factors<double>(*h);
+            share_factors<double>(*h);
         }
         if (h->cfg.ityp == 1) {
             if (h->fp32()) launch_rap<float>(h->d_ap64.as<double>(), h->d_rapT.as<float>(), h->ncell, h->stream);
@@ -1487,12 +1583,12 @@ void ensure_adjoint(mgp_handle &h) {
         }
     }
     const size_t tsz = h.tsz();
-    for (int q = 0; q < 6; ++q) h.d_ft[q].alloc(h.ncell * tsz);
+    for (int q = 0; q < 6; ++q) h.d_ft[q].alloc(h.nfcell * tsz);
     const T *f[7];
     T *g[6];
     for (int q = 0; q < 7; ++q) f[q] = h.d_f[q].as<T>();
     for (int q = 0; q < 6; ++q) g[q] = h.d_ft[q].as<T>();
-    launch_tfactors<T>(h.topo(-1), h.mask(), f, g, h.stream);
+    launch_tfactors<T>(h.topo_reps(), h.mask(), f, g, h.stream);
     HC(hipGetLastError());
     HC(hipStreamSynchronize(h.stream));
     h.adj_ready = true;
@@ -1932,6 +2028,13 @@ int mgp_get_history(mgp_handle *h, mgletreal *out, mgletint cap,
     } catch (const HipError &e) {
         return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
     }
+    return MGP_OK;
+}
+
+int mgp_factor_classes(mgp_handle *h, mgletint *nclasses, int64_t *ncells_stored) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (nclasses) *nclasses = (mgletint)h->nfclass;
+    if (ncells_stored) *ncells_stored = h->nfcell;
     return MGP_OK;
 }
 

@@ -69,6 +69,9 @@ struct Block {
     int pos[3];        // iposition, jposition, kposition in the parent
     int bc[6];         // first BC code per face
     int64_t wo;        // offset of the block's interior cells in WO arrays
+    // offset of the block's SIP factors (and transposed factors) in WO
+    // order: blocks whose factors are equal bit for bit share one copy
+    int64_t fo;
     int64_t sp;        // offset of the block in reference-layout 3-D fields
     int64_t fc;        // offset of the block's face buffers
     int64_t sh;        // offset of the block's ghost-shell buffers

@@ -252,6 +252,126 @@ template void launch_lu<float>(const Topo &, const double *, const double *,
                                float *const[7], hipStream_t);
 
 // ---------------------------------------------------------------------
+// Shared factor storage.  The SIP factors of a block depend only on its
+// shape, 1-D coefficients, GSAP and BP; blocks whose factors come out equal
+// bit for bit (every interior block of a uniform grid, for instance) keep
+// one copy, which the sweeps then read from L2 instead of HBM.  Classes are
+// found by a hash of each block's factor bits (and BP bytes), confirmed by
+// an exact comparison with the class representative.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t fbits(double v) { return (uint64_t)__double_as_longlong(v); }
+__device__ __forceinline__ uint64_t fbits(float v) { return (uint64_t)__float_as_uint(v); }
+
+template <typename T>
+struct F7 {
+    const T *f[7];
+};
+
+// One workgroup per block: order-independent sum of mixed (position,
+// factor, bits) words over the block's factors at Block::wo.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fhash(Topo t, F7<T> F, Mask m,
+                                                   unsigned long long *out) {
+    __shared__ unsigned long long red[kThreads];
+    const int b = t.blist[blockIdx.x];
+    const Block &B = t.blocks[b];
+    const int n = t.shapes[B.shape].ncell;
+    uint64_t acc = 0;
+    for (int s = threadIdx.x; s < n; s += kThreads) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+            acc += mix64(fbits(F.f[q][B.wo + s]) ^ mix64((uint64_t)s * 8 + q + 1));
+        if (m.bp) acc += mix64((uint64_t)m.bp[B.wo + s] ^ mix64((uint64_t)s * 8 + 8));
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[b] = red[0] ^ mix64((uint64_t)B.shape + 0x9e3779b97f4a7c15ull);
+}
+
+// bad[b] = 1 when block b's factors (or BP bytes) differ from those of its
+// representative rep[b] (both at Block::wo); blocks that represent
+// themselves are skipped.
+template <typename T>
+__global__ void k_fcmp(Topo t, F7<T> F, Mask m, const int *rep, int *bad) {
+    const int b = t.blist[blockIdx.y];
+    const int r = rep[b];
+    if (r == b) return;
+    const Block &B = t.blocks[b];
+    const Block &R = t.blocks[r];
+    const int n = t.shapes[B.shape].ncell;
+    bool diff = B.shape != R.shape;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n && !diff;
+         s += gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+            diff = diff || fbits(F.f[q][B.wo + s]) != fbits(F.f[q][R.wo + s]);
+        if (m.bp) diff = diff || m.bp[B.wo + s] != m.bp[R.wo + s];
+    }
+    if (diff) bad[b] = 1;
+}
+
+// Copies each representative's factors from Block::wo of the old arrays to
+// Block::fo of the new ones.
+template <typename T>
+__global__ void k_fgather(Topo t, F7<T> F, F7<T> G) {
+    const int b = t.blist[blockIdx.y];
+    const Block &B = t.blocks[b];
+    const int n = t.shapes[B.shape].ncell;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n;
+         s += gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) const_cast<T *>(G.f[q])[B.fo + s] = F.f[q][B.wo + s];
+    }
+}
+
+template <typename T>
+void launch_fhash(const Topo &t, const T *const f[7], Mask m,
+                  unsigned long long *out, hipStream_t s) {
+    F7<T> F;
+    for (int q = 0; q < 7; ++q) F.f[q] = f[q];
+    k_fhash<T><<<t.nlist, kThreads, 0, s>>>(t, F, m, out);
+}
+template <typename T>
+void launch_fcmp(const Topo &t, const T *const f[7], Mask m, const int *rep,
+                 int *bad, hipStream_t s) {
+    F7<T> F;
+    for (int q = 0; q < 7; ++q) F.f[q] = f[q];
+    const int maxc = t.max_ni * t.max_nj * t.max_nk;
+    dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
+    k_fcmp<T><<<grid, kThreads, 0, s>>>(t, F, m, rep, bad);
+}
+template <typename T>
+void launch_fgather(const Topo &t, const T *const f[7], T *const g[7],
+                    hipStream_t s) {
+    F7<T> F, G;
+    for (int q = 0; q < 7; ++q) {
+        F.f[q] = f[q];
+        G.f[q] = g[q];
+    }
+    const int maxc = t.max_ni * t.max_nj * t.max_nk;
+    dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
+    k_fgather<T><<<grid, kThreads, 0, s>>>(t, F, G);
+}
+#define MGP_INST_FSHARE(T)                                                          \
+    template void launch_fhash<T>(const Topo &, const T *const[7], Mask,              \
+                                  unsigned long long *, hipStream_t);                 \
+    template void launch_fcmp<T>(const Topo &, const T *const[7], Mask, const int *,   \
+                                 int *, hipStream_t);                                 \
+    template void launch_fgather<T>(const Topo &, const T *const[7], T *const[7],       \
+                                    hipStream_t);
+MGP_INST_FSHARE(double)
+MGP_INST_FSHARE(float)
+#undef MGP_INST_FSHARE
+
+// ---------------------------------------------------------------------
 // Layout changes between the reference layout and WO go through LDS, one
 // workgroup per block: the reference-layout side is read or written in its
 // own order (k fastest, coalesced), the WO side in WO order, and LDS holds
@@ -501,7 +621,7 @@ __global__ void k_to_ref_factors(Topo t, const T *wo, double *ref,
         unpack(cells[s], i, j, k);
         const int64_t d = spatial ? (k + 2) + (j + 2) * kk + (i + 2) * kk * jj
                                   : (int64_t)s;
-        ref[B.sp + d] = (double)wo[B.wo + s];
+        ref[B.sp + d] = (double)wo[B.fo + s];
     }
 }
 template <typename T>

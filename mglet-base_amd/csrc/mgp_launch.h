@@ -34,7 +34,7 @@ struct Fields {
     T *res;                     // forward-sweep result RES
     const T *ap;                // GSAP
     const T *rap;               // SOR_RAP = divide0(1, GSAP) (SOR only)
-    const T *f[7];              // LW LS LB LPR UE UN UT
+    const T *f[7];              // LW LS LB LPR UE UN UT, at Block::fo
     const T *aw, *ae, *as, *an, *ab, *at;  // 1-D coefficients (ref. layout)
     // 2 / (d(i-1) d(i)) per direction (ref. 1-D layout, FP64): the separable
     // part of GSAP, from which the plane kernels rebuild GSAP of blocks
@@ -81,6 +81,19 @@ void launch_lu(const Topo &t, const double *ap, const double *aw,
                const double *ae, const double *as, const double *an,
                const double *ab, const double *at, Mask m, T *const f[7],
                hipStream_t s);
+
+// shared factor storage: per-block hash of the factors at Block::wo, exact
+// comparison with the representative rep[b], copy of the representatives'
+// factors to Block::fo of new arrays
+template <typename T>
+void launch_fhash(const Topo &t, const T *const f[7], Mask m,
+                  unsigned long long *out, hipStream_t s);
+template <typename T>
+void launch_fcmp(const Topo &t, const T *const f[7], Mask m, const int *rep,
+                 int *bad, hipStream_t s);
+template <typename T>
+void launch_fgather(const Topo &t, const T *const f[7], T *const g[7],
+                    hipStream_t s);
 
 template <typename T>
 void launch_convert(const double *src, T *dst, int64_t n, hipStream_t s);

@@ -482,10 +482,10 @@ __global__ __launch_bounds__(kWave * NW) void k_forward(
     FwdK<T, CLASSIC, MASKED, LAP> k;
     k.cells = t.cellx + S.cell_off;
     k.rhs = F.rhs + wo;
-    k.lw = F.f[0] + wo;
-    k.ls = F.f[1] + wo;
-    k.lb = F.f[2] + wo;
-    k.lpr = F.f[3] + wo;
+    k.lw = F.f[0] + B.fo;
+    k.ls = F.f[1] + B.fo;
+    k.lb = F.f[2] + B.fo;
+    k.lpr = F.f[3] + B.fo;
     k.ap = F.ap + wo;
     k.res = F.res + wo;
     k.rlo = F.rlo + fc;
@@ -599,9 +599,9 @@ __global__ __launch_bounds__(kWave * NW) void k_backward(
     BwdK<T, CLASSIC, HZ> k;
     k.cells = t.cellx + S.cell_off;
     k.res = F.res + wo;
-    k.ue = F.f[4] + wo;
-    k.un = F.f[5] + wo;
-    k.ut = F.f[6] + wo;
+    k.ue = F.f[4] + B.fo;
+    k.un = F.f[5] + B.fo;
+    k.ut = F.f[6] + B.fo;
     k.hilf = F.hilf + wo;
     k.hface = F.hface + B.fc;
     k.R = R;

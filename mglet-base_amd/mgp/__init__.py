@@ -223,6 +223,14 @@ class PressureSolver:
         _lib.check(self.lib.mgp_info(self.h, C.byref(a), C.byref(b)), "mgp_info")
         return dict(ncells=a.value, device_bytes=b.value)
 
+    def factor_classes(self):
+        """(groups of blocks sharing one copy of the SIP factors, factor
+        cells stored per factor array)."""
+        a, b = C.c_int32(), C.c_int64()
+        _lib.check(self.lib.mgp_factor_classes(self.h, C.byref(a), C.byref(b)),
+                   "mgp_factor_classes")
+        return a.value, b.value
+
     def close(self):
         if getattr(self, "h", None):
             _lib.check(self.lib.mgp_finish(self.h), "mgp_finish")

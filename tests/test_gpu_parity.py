@@ -301,3 +301,47 @@ def test_sdiv_rejected_without_ghostcell():
     s = _solver(g)
     with pytest.raises(mgp.MgpError):
         s.set_sdiv(np.zeros(g.n3d))
+
+
+@pytest.mark.parametrize("name,classes", [("box2", 1), ("aniso", 2), ("walls", 8), ("c2", 1)])
+def test_shared_factor_classes(name, classes):
+    """Blocks with equal SIP factors share one copy: every block of a
+    periodic uniform box is alike; walls make classes of their own."""
+    g = topology(name)
+    s = _solver(g)
+    n, cells = s.factor_classes()
+    assert n == classes, (name, n)
+    per_block = (g.ii - 4) * (g.jj - 4) * (g.kk - 4)
+    assert cells == per_block[:classes].sum()
+    s.close()
+
+
+def test_shared_factor_classes_masked():
+    """Random blocked cells make every block's factors differ."""
+    g = topology("box2")
+    bp = 1.0 - blocked_mask(g, 7, 0.1)
+    s = _solver(g, bp=bp)
+    assert s.factor_classes() == (g.ngrid, g.ncells)
+    s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_shared_factors_change_nothing(oracle_mod, monkeypatch, precision):
+    """One copy per block (MGP_SHARED_FACTORS=0) and shared copies give the
+    same bits."""
+    g = topology("box2")
+    o = oracle_mod.Oracle(g)
+    u, v, w = velocity(g, o, 61)
+    outs = []
+    for env in ("0", "1"):
+        monkeypatch.setenv("MGP_SHARED_FACTORS", env)
+        s = _solver(g, ninner=5, nouter=6, precision=precision)
+        assert s.factor_classes()[0] == (g.ngrid if env == "0" else 1)
+        f = [a.copy() for a in (u, v, w, np.zeros(g.n3d))]
+        s.mgpoisl(*f, 1e-3)
+        outs.append((f, s.history()))
+        s.close()
+    (a, ha), (b, hb) = outs
+    assert np.array_equal(ha, hb)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
