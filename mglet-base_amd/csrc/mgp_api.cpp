@@ -148,6 +148,8 @@ struct mgp_handle {
     int64_t nfcell = 0;
     std::vector<int> frep;
     DBuf d_blist_rep;
+    DBuf d_aps64, d_apsT;      // GSAP kept with the shared factors (FP64, FP32)
+    bool fshared() const { return d_aps64.p != nullptr; }
     DBuf d_bpw, d_bpf, d_bpsp;
     DBuf d_sdiv;               // ghostcell divergence source (reference layout)
     DBuf d_hface, d_rhi[2], d_rlo;
@@ -280,6 +282,7 @@ struct mgp_handle {
         F.apx = d_apd[0].as<double>();
         F.apy = d_apd[1].as<double>();
         F.apz = d_apd[2].as<double>();
+        F.aps = !fshared() ? nullptr : fp32() ? d_apsT.as<T>() : d_aps64.as<T>();
         F.hface = d_hface.as<T>();
         F.hpar = d_hpar.as<T>();
         F.rhi[0] = d_rhi[0].as<T>();
@@ -825,12 +828,12 @@ void setup_factors(mgp_handle &h) {
     HC(hipGetLastError());
 }
 
-// Shared factor storage: groups the blocks whose SIP factors (and BP
-// bytes) are equal bit for bit, keeps one copy per group and points each
-// block's factor offset (Block::fo) at it.  The factors are the same
-// values, so every result is unchanged; the sweeps read a shared copy from
-// L2 instead of per-block copies from HBM.  MGP_SHARED_FACTORS=0 keeps one
-// copy per block.
+// Shared factor storage: groups the blocks whose SIP factors, GSAP (and BP
+// bytes) are equal bit for bit, keeps one copy of the factors and GSAP per
+// group and points each block's factor offset (Block::fo) at it.  The
+// values are the same, so every result is unchanged; the plane kernels read
+// the shared copy from L2 instead of per-block copies from HBM.
+// MGP_SHARED_FACTORS=0 keeps one copy per block.
 template <typename T>
 void share_factors(mgp_handle &h) {
     const hipStream_t s = h.stream;
@@ -845,10 +848,11 @@ void share_factors(mgp_handle &h) {
     const T *f[7];
     for (int q = 0; q < 7; ++q) f[q] = h.d_f[q].as<T>();
     Mask m = h.mask();
+    const double *ap = h.d_ap64.as<double>();
     if (on && n > 1) {
         DBuf dh, drep, dbad;
         dh.alloc((size_t)n * 8);
-        launch_fhash<T>(t, f, m, dh.as<unsigned long long>(), s);
+        launch_fhash<T>(t, f, ap, m, dh.as<unsigned long long>(), s);
         HC(hipGetLastError());
         std::vector<unsigned long long> hash(n);
         HC(hipMemcpyAsync(hash.data(), dh.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
@@ -864,7 +868,7 @@ void share_factors(mgp_handle &h) {
         upload(drep, h.frep);
         std::vector<int> bad(n, 0);
         upload(dbad, bad);
-        launch_fcmp<T>(t, f, m, drep.as<int>(), dbad.as<int>(), s);
+        launch_fcmp<T>(t, f, ap, m, drep.as<int>(), dbad.as<int>(), s);
         HC(hipGetLastError());
         HC(hipMemcpyAsync(bad.data(), dbad.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s));
         HC(hipStreamSynchronize(s));
@@ -891,9 +895,12 @@ void share_factors(mgp_handle &h) {
         nf[q].alloc((size_t)off * sizeof(T));
         g[q] = nf[q].as<T>();
     }
+    h.d_aps64.alloc((size_t)off * sizeof(double));
+    if (h.fp32()) h.d_apsT.alloc((size_t)off * sizeof(float));
     // the gather reads Block::wo and writes Block::fo of the new blocks table
     upload(h.d_blocks, h.blocks);
-    launch_fgather<T>(h.topo_reps(), f, g, s);
+    launch_fgather<T>(h.topo_reps(), f, ap, g, h.d_aps64.as<double>(),
+                      h.fp32() ? h.d_apsT.as<float>() : nullptr, s);
     HC(hipGetLastError());
     HC(hipStreamSynchronize(s));
     for (int q = 0; q < 7; ++q) {   // the old arrays go with nf
@@ -1345,6 +1352,10 @@ float iterate(mgp_handle &h, const Fields<T> &F, const Outer &o,
         const double *dd[3] = {mt[3], mt[4], mt[5]};
         T *pbuf = h.d_pbuf.as<T>();
         const int bpv = c.ibtype == 1;
+        // one level of SIP sweeps: the outer update runs fused with the next
+        // outer iteration's first forward sweep (MGP_FUSED_OUTER=0: apart)
+        const char *fe = getenv("MGP_FUSED_OUTER");
+        const bool fuse = !multi && !(c.ityp == 1 && c.sor_all_levels) && !(fe && atoi(fe) == 0);
         auto enqueue_chunk = [&]() {
             // RES buffer parity per level (each level exchanges its own RES)
             std::map<int, int> rd;
@@ -1380,8 +1391,10 @@ float iterate(mgp_handle &h, const Fields<T> &F, const Outer &o,
                         for (int il = 1; il <= c.ninner; ++il) {
                             const int last = il == c.ninner;
                             if (par) launch_bound<T>(t, F, pbuf, m, bpv, mt, o.done, s);
-                            launch_forward<T>(t, F, m, r, !last, il == 1 && first, classic,
-                                              o.done, s);
+                            // fused: the previous outer update ran this sweep
+                            if (!(fuse && il == 1))
+                                launch_forward<T>(t, F, m, r, !last, il == 1 && first, classic,
+                                                  o.done, s);
                             launch_backward<T>(t, F, il == 1 && first, classic, o.done, s);
                             if (!last) r ^= 1;
                         }
@@ -1394,7 +1407,11 @@ float iterate(mgp_handle &h, const Fields<T> &F, const Outer &o,
                     if (lv->first != coarsest)
                         launch_restrict<T, T>(h.topo(lv->first), F.hilf, F.hilf, F.hface,
                                               nullptr, m, bpv ? 1 : 0, dd, o.done, s);
-                launch_outer<T>(tall, F, m, o_inner, rhsT, s);
+                if (fuse)   // with the next iteration's first sweep (RES parity rd)
+                    launch_outer_fwd<T>(tall, F, m, o_inner, rhsT, rd[coarsest], c.ninner > 1,
+                                        classic, s);
+                else
+                    launch_outer<T>(tall, F, m, o_inner, rhsT, s);
                 if (multi) {
                     for (auto lv = h.level_blocks.rbegin(); lv != h.level_blocks.rend(); ++lv)
                         if (lv->first != coarsest)
@@ -1424,6 +1441,10 @@ float iterate(mgp_handle &h, const Fields<T> &F, const Outer &o,
         const double every = pe ? atof(pe) : 0.0;
         auto t_last = std::chrono::steady_clock::now();
         int hstate[2] = {0, 0};   // outer iterations done, done flag
+        // fused: the first forward sweep of the first outer iteration
+        if (fuse)
+            launch_forward<T>(h.topo(coarsest), F, h.mask(), 0, c.ninner > 1, 1, classic,
+                              o.done, s);
         while (launched < c.nouter) {
             if (exec) HC(hipGraphLaunch(exec, s));
             else enqueue_chunk();
@@ -2048,7 +2069,8 @@ int mgp_info(mgp_handle *h, int64_t *ncells, int64_t *device_bytes) {
                             &h->d_ap64, &h->d_apT, &h->d_hilf, &h->d_rhs64, &h->d_rhsT, &h->d_res,
                             &h->d_dp64, &h->d_bpw, &h->d_bpf, &h->d_bpsp, &h->d_sdiv, &h->d_hface, &h->d_rhi[0],
                             &h->d_rhi[1], &h->d_rlo, &h->d_rapT, &h->d_hface2, &h->d_ft[0], &h->d_ft[1], &h->d_ft[2],
-                            &h->d_ft[3], &h->d_ft[4], &h->d_ft[5], &h->d_u, &h->d_v, &h->d_w, &h->d_p, &h->d_dpsp};
+                            &h->d_ft[3], &h->d_ft[4], &h->d_ft[5], &h->d_u, &h->d_v, &h->d_w, &h->d_p, &h->d_dpsp,
+                            &h->d_aps64, &h->d_apsT, &h->d_blist_rep};
         for (auto *b : bs) tot += b->n;
         for (auto &b : h->d_f) tot += b.n;
         *device_bytes = (int64_t)tot;

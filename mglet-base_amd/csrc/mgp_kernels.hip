@@ -253,11 +253,12 @@ template void launch_lu<float>(const Topo &, const double *, const double *,
 
 // ---------------------------------------------------------------------
 // Shared factor storage.  The SIP factors of a block depend only on its
-// shape, 1-D coefficients, GSAP and BP; blocks whose factors come out equal
-// bit for bit (every interior block of a uniform grid, for instance) keep
-// one copy, which the sweeps then read from L2 instead of HBM.  Classes are
-// found by a hash of each block's factor bits (and BP bytes), confirmed by
-// an exact comparison with the class representative.
+// shape, 1-D coefficients, GSAP and BP; blocks whose factors and GSAP come
+// out equal bit for bit (every interior block of a uniform grid, for
+// instance) keep one copy of both, which the plane kernels then read from
+// L2 instead of HBM.  Classes are found by a hash of each block's factor
+// and GSAP bits (and BP bytes), confirmed by an exact comparison with the
+// class representative.
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
     z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
@@ -269,6 +270,7 @@ __device__ __forceinline__ uint64_t fbits(float v) { return (uint64_t)__float_as
 template <typename T>
 struct F7 {
     const T *f[7];
+    const double *ap;   // GSAP (FP64)
 };
 
 // One workgroup per block: order-independent sum of mixed (position,
@@ -284,8 +286,9 @@ __global__ __launch_bounds__(kThreads) void k_fhash(Topo t, F7<T> F, Mask m,
     for (int s = threadIdx.x; s < n; s += kThreads) {
 #pragma unroll
         for (int q = 0; q < 7; ++q)
-            acc += mix64(fbits(F.f[q][B.wo + s]) ^ mix64((uint64_t)s * 8 + q + 1));
-        if (m.bp) acc += mix64((uint64_t)m.bp[B.wo + s] ^ mix64((uint64_t)s * 8 + 8));
+            acc += mix64(fbits(F.f[q][B.wo + s]) ^ mix64((uint64_t)s * 9 + q + 1));
+        acc += mix64(fbits(F.ap[B.wo + s]) ^ mix64((uint64_t)s * 9 + 8));
+        if (m.bp) acc += mix64((uint64_t)m.bp[B.wo + s] ^ mix64((uint64_t)s * 9 + 9));
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -313,15 +316,17 @@ __global__ void k_fcmp(Topo t, F7<T> F, Mask m, const int *rep, int *bad) {
 #pragma unroll
         for (int q = 0; q < 7; ++q)
             diff = diff || fbits(F.f[q][B.wo + s]) != fbits(F.f[q][R.wo + s]);
+        diff = diff || fbits(F.ap[B.wo + s]) != fbits(F.ap[R.wo + s]);
         if (m.bp) diff = diff || m.bp[B.wo + s] != m.bp[R.wo + s];
     }
     if (diff) bad[b] = 1;
 }
 
-// Copies each representative's factors from Block::wo of the old arrays to
-// Block::fo of the new ones.
+// Copies each representative's factors and GSAP from Block::wo of the old
+// arrays to Block::fo of the new ones (GSAP in FP64, and in the working
+// precision when that is FP32).
 template <typename T>
-__global__ void k_fgather(Topo t, F7<T> F, F7<T> G) {
+__global__ void k_fgather(Topo t, F7<T> F, F7<T> G, float *ap32) {
     const int b = t.blist[blockIdx.y];
     const Block &B = t.blocks[b];
     const int n = t.shapes[B.shape].ncell;
@@ -329,44 +334,51 @@ __global__ void k_fgather(Topo t, F7<T> F, F7<T> G) {
          s += gridDim.x * blockDim.x) {
 #pragma unroll
         for (int q = 0; q < 7; ++q) const_cast<T *>(G.f[q])[B.fo + s] = F.f[q][B.wo + s];
+        const double a = F.ap[B.wo + s];
+        const_cast<double *>(G.ap)[B.fo + s] = a;
+        if (ap32) ap32[B.fo + s] = (float)a;
     }
 }
 
 template <typename T>
-void launch_fhash(const Topo &t, const T *const f[7], Mask m,
+void launch_fhash(const Topo &t, const T *const f[7], const double *ap, Mask m,
                   unsigned long long *out, hipStream_t s) {
     F7<T> F;
     for (int q = 0; q < 7; ++q) F.f[q] = f[q];
+    F.ap = ap;
     k_fhash<T><<<t.nlist, kThreads, 0, s>>>(t, F, m, out);
 }
 template <typename T>
-void launch_fcmp(const Topo &t, const T *const f[7], Mask m, const int *rep,
-                 int *bad, hipStream_t s) {
+void launch_fcmp(const Topo &t, const T *const f[7], const double *ap, Mask m,
+                 const int *rep, int *bad, hipStream_t s) {
     F7<T> F;
     for (int q = 0; q < 7; ++q) F.f[q] = f[q];
+    F.ap = ap;
     const int maxc = t.max_ni * t.max_nj * t.max_nk;
     dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
     k_fcmp<T><<<grid, kThreads, 0, s>>>(t, F, m, rep, bad);
 }
 template <typename T>
-void launch_fgather(const Topo &t, const T *const f[7], T *const g[7],
-                    hipStream_t s) {
+void launch_fgather(const Topo &t, const T *const f[7], const double *ap,
+                    T *const g[7], double *gap, float *gap32, hipStream_t s) {
     F7<T> F, G;
     for (int q = 0; q < 7; ++q) {
         F.f[q] = f[q];
         G.f[q] = g[q];
     }
+    F.ap = ap;
+    G.ap = gap;
     const int maxc = t.max_ni * t.max_nj * t.max_nk;
     dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
-    k_fgather<T><<<grid, kThreads, 0, s>>>(t, F, G);
+    k_fgather<T><<<grid, kThreads, 0, s>>>(t, F, G, gap32);
 }
 #define MGP_INST_FSHARE(T)                                                          \
-    template void launch_fhash<T>(const Topo &, const T *const[7], Mask,              \
+    template void launch_fhash<T>(const Topo &, const T *const[7], const double *, Mask, \
                                   unsigned long long *, hipStream_t);                 \
-    template void launch_fcmp<T>(const Topo &, const T *const[7], Mask, const int *,   \
-                                 int *, hipStream_t);                                 \
-    template void launch_fgather<T>(const Topo &, const T *const[7], T *const[7],       \
-                                    hipStream_t);
+    template void launch_fcmp<T>(const Topo &, const T *const[7], const double *, Mask, \
+                                 const int *, int *, hipStream_t);                    \
+    template void launch_fgather<T>(const Topo &, const T *const[7], const double *,    \
+                                    T *const[7], double *, float *, hipStream_t);
 MGP_INST_FSHARE(double)
 MGP_INST_FSHARE(float)
 #undef MGP_INST_FSHARE

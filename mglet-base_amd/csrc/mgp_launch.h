@@ -40,6 +40,9 @@ struct Fields {
     // part of GSAP, from which the plane kernels rebuild GSAP of blocks
     // without blocked cells
     const double *apx, *apy, *apz;
+    // GSAP kept with the shared factors (at Block::fo), or null when no
+    // factors are shared; the plane kernels then rebuild it on chip
+    const T *aps;
     T *hface;                   // HILF boundary planes (own faces)
     T *hpar;                    // HILF ghosts of the block's PAR faces
     T *rhi[2];                  // RES upper planes, double-buffered
@@ -82,18 +85,19 @@ void launch_lu(const Topo &t, const double *ap, const double *aw,
                const double *ab, const double *at, Mask m, T *const f[7],
                hipStream_t s);
 
-// shared factor storage: per-block hash of the factors at Block::wo, exact
-// comparison with the representative rep[b], copy of the representatives'
-// factors to Block::fo of new arrays
+// shared factor storage: per-block hash of the factors and GSAP at
+// Block::wo, exact comparison with the representative rep[b], copy of the
+// representatives' factors and GSAP (FP64, and FP32 when gap32 is given) to
+// Block::fo of new arrays
 template <typename T>
-void launch_fhash(const Topo &t, const T *const f[7], Mask m,
+void launch_fhash(const Topo &t, const T *const f[7], const double *ap, Mask m,
                   unsigned long long *out, hipStream_t s);
 template <typename T>
-void launch_fcmp(const Topo &t, const T *const f[7], Mask m, const int *rep,
-                 int *bad, hipStream_t s);
+void launch_fcmp(const Topo &t, const T *const f[7], const double *ap, Mask m,
+                 const int *rep, int *bad, hipStream_t s);
 template <typename T>
-void launch_fgather(const Topo &t, const T *const f[7], T *const g[7],
-                    hipStream_t s);
+void launch_fgather(const Topo &t, const T *const f[7], const double *ap,
+                    T *const g[7], double *gap, float *gap32, hipStream_t s);
 
 template <typename T>
 void launch_convert(const double *src, T *dst, int64_t n, hipStream_t s);
@@ -137,6 +141,13 @@ void launch_outer(const Topo &t, const Fields<T> &f, Mask m, Outer o,
 void launch_check(Outer o, int nouter, int nouter_min, double epcorr,
                   double eps_rel, double prefak, double *hist, int hist_cap,
                   hipStream_t s);
+
+// the outer update of one outer iteration fused with the first forward
+// sweep (HILF = 0) of the next (mgp_fused.hip); rd and write_hi as for
+// launch_forward
+template <typename T>
+void launch_outer_fwd(const Topo &t, const Fields<T> &f, Mask m, Outer o, T *rhs_t,
+                      int rd, int write_hi, int classic, hipStream_t s);
 
 template <typename T>
 void launch_wo_to_sp(const Topo &t, const T *src, double *dst,

@@ -1,0 +1,335 @@
+// mgp_plane.h -- device helpers and the plan engine shared by the plane
+// kernels (mgp_sweeps.hip, mgp_fused.hip): one wave64 owns one block and
+// walks the items of a host-built plan over the block's i+j+k hyperplanes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "mgp_device.h"
+#include "mgp_launch.h"
+
+namespace mgp {
+
+constexpr int kWave = 64;
+
+// Loads at a 32-bit element offset from a uniform base: the byte offset
+// stays an unsigned 32-bit value, so the load takes the base from SGPRs.
+template <typename T>
+__device__ __forceinline__ T ld(const T *base, int s) {
+    return *(const T *)((const char *)base + (uint32_t)s * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st(T *base, int s, T v) {
+    *(T *)((char *)base + (uint32_t)s * (uint32_t)sizeof(T)) = v;
+}
+
+// Cell word of the plane kernels (per shape and WO cell, built on the
+// host): x = i | j << 6 | k << 12 | fz << 18, y = fx | fy << 12 |
+// flags << 24, with fx, fy, fz the cell's position in the x, y and z face
+// buffers and flags bit 0..5 = (i == 0, i == NI-1, j == 0, j == NJ-1,
+// k == 0, k == NK-1).
+__device__ __forceinline__ void cell_ijk(uint32_t w, int &i, int &j, int &k) {
+    i = w & 63;
+    j = (w >> 6) & 63;
+    k = (w >> 12) & 63;
+}
+
+// Load through a pointer read from LDS: marks it as global memory (a
+// generic pointer would become a flat load, which the compiler can only
+// wait for by draining every outstanding load).
+template <typename T>
+__device__ __forceinline__ T gld(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+
+// Index of a per-block wave count in the plan tables (kPlanWaves).
+template <int NW>
+constexpr int plan_nw() {
+    static_assert(NW == 1 || NW == 4 || NW == 8, "no plans for this wave count");
+    return NW == 1 ? 0 : (NW == 4 ? 1 : 2);
+}
+
+// Barrier of the waves of a block over LDS only: orders LDS accesses
+// across it without waiting for the global loads in flight (a full
+// __syncthreads would drain the prefetch pipeline at every plane).
+__device__ __forceinline__ void block_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Per-wave table of entry sources, by entry kind.  Absent sources point at
+// the zero buffer, so every entry load is valid and yields 0 there.
+template <typename T>
+struct SrcTab {
+    const T *h[8];        // window (target 0) values
+    const T *r[8];        // recurrence buffer (target 1) values
+    const uint8_t *b[8];  // window BP
+};
+
+// res = -(A phi) at one cell, src/flow/laplacephi_mod.F90:149-162, with
+// phi and BP read from the window planes p-1, p, p+1.
+template <typename T, typename A, bool MASKED>
+__device__ __forceinline__ A laplace_cell(const T *Hm, const T *H0,
+                                          const T *Hp, const uint8_t *Bm,
+                                          const uint8_t *B0,
+                                          const uint8_t *Bp, int c0, int HW,
+                                          A aw, A ae, A as, A an, A ab, A at,
+                                          A ap) {
+    const A pw = (A)Hm[c0 - HW], pe = (A)Hp[c0 + HW];
+    const A psv = (A)Hm[c0 - 1], pn = (A)Hp[c0 + 1];
+    const A pb = (A)Hm[c0], pt = (A)Hp[c0], pc = (A)H0[c0];
+    A r;
+    if (MASKED) {
+        const A bc = (A)B0[c0];
+        const A bw = (A)Bm[c0 - HW], be = (A)Bp[c0 + HW];
+        const A bs = (A)Bm[c0 - 1], bn = (A)Bp[c0 + 1];
+        const A bb = (A)Bm[c0], bt = (A)Bp[c0];
+        r = -(aw * pw * bw * bc);
+        r = r - ae * pe * bc * be;
+        r = r - as * psv * bs * bc;
+        r = r - an * pn * bc * bn;
+        r = r - ab * pb * bb * bc;
+        r = r - at * pt * bc * bt;
+    } else {
+        r = -(aw * pw);
+        r = r - ae * pe;
+        r = r - as * psv;
+        r = r - an * pn;
+        r = r - ab * pb;
+        r = r - at * pt;
+    }
+    r = r - ap * pc;
+    return r;
+}
+
+// Length of the LDS rows of 1-D data: the largest block dimension, even
+// (so that FP32 rows keep the FP64 rows behind them 8-byte aligned).
+__host__ __device__ inline int row_len(const Topo &t) {
+    const int m = t.max_ni > t.max_nj ? t.max_ni : t.max_nj;
+    return ((m > t.max_nk ? m : t.max_nk) + 1) & ~1;
+}
+
+// 1-D coefficient rows of a block in LDS: [w e s n b t][nr]
+template <typename C>
+__device__ __forceinline__ void coef_rows(C *c, const C *aw, const C *ae,
+                                          const C *as, const C *an,
+                                          const C *ab, const C *at,
+                                          const Block &B, int nr) {
+    for (int e = threadIdx.x; e < 6 * nr; e += blockDim.x) {
+        const int r = e / nr, q = e % nr;
+        const int n = r < 2 ? B.ni : (r < 4 ? B.nj : B.nk);
+        if (q >= n) continue;
+        const C *src = r == 0 ? aw + B.o1x : r == 1 ? ae + B.o1x
+                     : r == 2 ? as + B.o1y : r == 3 ? an + B.o1y
+                     : r == 4 ? ab + B.o1z : at + B.o1z;
+        c[e] = src[q + 2];
+    }
+}
+
+// Separable part of GSAP in LDS: [x y z][nr] of 2 / (d(i-1) d(i)).
+__device__ __forceinline__ void ap_rows(double *c, const double *ax,
+                                        const double *ay, const double *az,
+                                        const Block &B, int nr) {
+    for (int e = threadIdx.x; e < 3 * nr; e += blockDim.x) {
+        const int r = e / nr, q = e % nr;
+        const int n = r == 0 ? B.ni : (r == 1 ? B.nj : B.nk);
+        if (q >= n) continue;
+        const double *src = r == 0 ? ax + B.o1x : r == 1 ? ay + B.o1y : az + B.o1z;
+        c[e] = src[q + 2];
+    }
+}
+
+// The block's wall fold-in terms, copied to registers before a plane loop:
+// read through the Block reference inside the loop they would be reloaded
+// at every step (the compiler cannot prove them unaliased by the kernel's
+// stores), and each reload would wait for every load in flight.
+struct WallFold {
+    uint32_t mask;
+    double w[6];
+    __device__ __forceinline__ void load(const Block &B) {
+        mask = (uint32_t)B.wallmask;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) w[f] = B.wall[f];
+    }
+};
+
+// GSAP of a cell of a block without blocked cells (BP = 1 on the block
+// and its ghost layer), in the order of src/ib/noib_mod.F90:154-244: the
+// separable part, the six a_nb (1 - BP BP_nb) terms (all +0 here, exact),
+// then the wall fold-ins of the boundary cells (a_nb BP BP_nb = a_nb).
+// Equal bit for bit to the stored GSAP.
+__device__ __forceinline__ double ap_cell(const double *ar, int nr, int i, int j,
+                                          int k, uint32_t fl, const WallFold &B) {
+    double a = -ar[i];
+    a = a - ar[nr + j];
+    a = a - ar[2 * nr + k];
+    const uint32_t w = fl & B.mask;
+    if (w) {
+        if (w & 1) a = a + B.w[0];
+        if (w & 2) a = a + B.w[1];
+        if (w & 4) a = a + B.w[2];
+        if (w & 8) a = a + B.w[3];
+        if (w & 16) a = a + B.w[4];
+        if (w & 32) a = a + B.w[5];
+    }
+    return a;
+}
+
+// Window sources of HILF for the 6 faces: the neighbour's boundary plane
+// (CON), the block's own PAR ghosts from bound_pressure, or none.
+template <typename T>
+__device__ __forceinline__ const T *window_face(const T *hface, const T *hpar,
+                                                const Block *blocks,
+                                                const Block &B, int f) {
+    const T *g = nbr_face(hface, blocks, B.nbr[f], f ^ 1);
+    if (!g && ((B.par >> f) & 1))
+        g = hpar + B.fc + face_base(f, B.ni, B.nj, B.nk);
+    return g;
+}
+
+// Fills the source table: lanes 0-7 window values, 8-15 recurrence
+// values, 16-23 window BP.
+template <typename T>
+__device__ __forceinline__ void fill_tab(SrcTab<T> *tab, const Topo &t,
+                                         const Block &B, const T *hface,
+                                         const T *hpar, const T *hilf,
+                                         const T *rsrc, int rmask, Mask m) {
+    const T *zero = (const T *)t.zero;
+    const int l = threadIdx.x;
+    if (l < 8) {
+        const T *g = nullptr;
+        if (l < 6 && hface) g = window_face(hface, hpar, t.blocks, B, l);
+        if (l == 6) g = hilf + B.wo;
+        tab->h[l] = g ? g : zero;
+    } else if (l < 16) {
+        const int f = l - 8;
+        const T *g = nullptr;
+        if (f < 6 && rsrc && ((rmask >> f) & 1))
+            g = nbr_face(rsrc, t.blocks, B.nbr[f], f ^ 1);
+        tab->r[f] = g ? g : zero;
+    } else if (l < 24) {
+        const int f = l - 16;
+        const uint8_t *g = nullptr;
+        if (f < 6 && m.bpf) g = m.bpf + B.fc + face_base(f, B.ni, B.nj, B.nk);
+        if (f == 6 && m.bp) g = m.bp + B.wo;
+        tab->b[f] = g ? g : (const uint8_t *)t.zero;
+    }
+}
+
+// The plan engine: runs the items of one block through functor K.  An
+// item covers up to 64 M cells (M per lane) and 64 M entries.  K provides
+// Data, load(Data&, s), compute(const Data&, s, p) and
+// store(entry, value, bp, p).
+//
+// Pipeline (W = 2D items per loop trip, every ring slot named statically,
+// so no register holding a load in flight is ever copied -- a copy would
+// make the wave wait for that load):
+//   * descriptor of item q: vector load at issue(q - 2D), ring iv[q mod W];
+//   * entry words of item q: loaded at issue(q - D), ring ew[q mod W];
+//   * data and entry values of item q: loaded at issue(q), ring q mod D,
+//     consumed by compute(q), which runs D items later;
+//   * the LDS source pointers of item q's entries are looked up at the end
+//     of issue(q - 1).
+// All loads go through the in-order vector memory counter, so each item
+// waits only for its own loads.  The loop starts W items early with no-op
+// items instead of a prologue, so it is entered with no load outstanding
+// and the compiler's wait counts at the loop head are those of the steady
+// state.  Indices outside the plan are clamped to its closing no-op item.
+template <int D, int M, int NW, bool EBP, typename T, class K>
+__device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
+                                         int nitem,
+                                         const uint32_t *__restrict__ ents,
+                                         const SrcTab<T> *tab) {
+    constexpr int W = 2 * D;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int last = nitem - 1;
+    typename K::Data cd[D][M];
+    uint32_t ew[W][M];
+    T ev[D][M];
+    uint8_t eb[D][M];
+    uint2 iv[W];
+    uint32_t ix[D], iy[D];
+    const T *pn[M];
+    const uint8_t *bn[M];
+    // this wave's items: every NW-th of the plan, from its own first one
+    auto ldi = [&](int q) -> uint2 { return ld(items, ((q < 0 || q > last) ? last : q) * NW); };
+    auto lookup = [&](int m, uint32_t e) {
+        const int kind = (e >> 12) & 7;
+        pn[m] = (e & 0x8000u) ? tab->r[kind] : tab->h[kind];
+        if (EBP) bn[m] = tab->b[kind];
+    };
+    // issue(q), q = q0 + w + D: r = q mod W, u = q mod D
+    auto issue = [&](int u, int r, int q) {
+        const uint32_t x = __builtin_amdgcn_readfirstlane(iv[r].x);
+        const uint32_t y = __builtin_amdgcn_readfirstlane(iv[r].y);
+        ix[u] = x;
+        iy[u] = y;
+        const int cpos = x & 0xffff, cc = (x >> 16) & 0xff;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int l = lane + 64 * m;
+            k.load(cd[u][m], cpos + (l < cc ? l : (cc > 0 ? cc - 1 : 0)));
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int off = (int)(ew[r][m] >> 16);
+            ev[u][m] = gld(pn[m] + off);
+            if (EBP) eb[u][m] = gld(bn[m] + off);
+        }
+        const int r2 = (r + D) % W;
+        const uint32_t x2 = __builtin_amdgcn_readfirstlane(iv[r2].x);
+        const uint32_t y2 = __builtin_amdgcn_readfirstlane(iv[r2].y);
+        const int ec2 = x2 >> 24, ep2 = y2 & 0x1fffff;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int l = lane + 64 * m;
+            ew[r2][m] = ld(ents, ep2 + (l < ec2 ? l : (ec2 > 0 ? ec2 - 1 : 0)));
+        }
+        iv[r] = ldi(q + W);
+        const int r1 = (r + 1) % W;
+#pragma unroll
+        for (int m = 0; m < M; ++m) lookup(m, ew[r1][m]);
+    };
+#pragma unroll
+    for (int r = 0; r < W; ++r) {
+        uint2 v = ldi(r < D ? r : -1);
+        v.x = __builtin_amdgcn_readfirstlane(v.x);  // completes the load here
+        v.y = __builtin_amdgcn_readfirstlane(v.y);
+        iv[r] = v;
+#pragma unroll
+        for (int m = 0; m < M; ++m) ew[r][m] = 0u;  // entry word 0: a valid address
+    }
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+        ix[u] = 0u;
+        iy[u] = 8u << 21;
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m) lookup(m, 0u);
+    for (int q0 = -W; q0 < nitem; q0 += W) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const int u = w % D;
+            const uint32_t x = ix[u], y = iy[u];
+            const int cpos = x & 0xffff, cc = (x >> 16) & 0xff, ec = x >> 24;
+            const int p = (int)(y >> 21) - 8;
+#pragma unroll
+            for (int m = 0; m < M; ++m)
+                if (lane + 64 * m < cc) k.compute(cd[u][m], cpos + lane + 64 * m, p);
+#pragma unroll
+            for (int m = 0; m < M; ++m)
+                if (lane + 64 * m < ec)
+                    k.store(ew[w][m], ev[u][m], EBP ? eb[u][m] : (uint8_t)0, p);
+            wave_sync();
+            // several waves per block: the next step reads what all waves
+            // wrote in this one (the next item was issued D items ago)
+            const bool step_ends = NW > 1 && (iy[(w + 1) % D] >> 21) != (y >> 21);
+            issue(u, (w + D) % W, q0 + w + D);
+            if (step_ends) block_sync_lds();
+        }
+    }
+}
+
+}  // namespace mgp
