@@ -14,7 +14,13 @@ The line also carries:
                   the library's stream; algorithmic bytes per launch from the
                   per-cell figures in `ALG_BYTES` below, against the 8 TB/s
                   peak and against a measured device copy (torch copy_ of
-                  2 x 1.8 GB) as the practical streaming rate;
+                  2 x 1.8 GB) as the practical streaming rate; `traffic` is
+                  the HBM bytes per launch from the committed rocprofv3
+                  FETCH_SIZE/WRITE_SIZE summary and `hbm_gbs` that over the
+                  launch time (factors shared between blocks come from L2,
+                  so HBM carries less than the algorithmic bytes);
+                  `fused_update` prices the whole inner iteration at the
+                  fused algorithmic bytes per cell-update (45 B FP32);
   autotune     -- the pipeline shape the library's autotuner picked for the
                   plane kernels, with its per-shape timings;
   cpu_baseline -- the oracle (C restatement of the reference CPU path) timed
@@ -480,6 +486,9 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_unit": "GB per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
                      "alg_bytes_per_cell": ALG_BYTES[main_form]["forward"],
+                     # HBM bytes actually moved per launch over the launch time:
+                     # factors and GSAP shared between blocks come from L2
+                     "hbm_gbs": (round(traffic / (fw_ms * 1e-3), 1) if traffic else None),
                      "copy_gbs": round(copy_gbs, 1),
                      "frac_of_copy": round(achieved / copy_gbs, 4),
                      "inner_iteration_gbs": round(
