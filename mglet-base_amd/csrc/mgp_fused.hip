@@ -68,17 +68,16 @@ struct OutFwdK {
         const int c0 = (j + 1) + __mul24(HW, i + 1);
         const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS, sp = ((p + 2) & 3) * mHS;
         const double apv = MASKED ? d.ap : ap_cell(ar, nr, i, j, k, d.c.y >> 24, wf);
-        const double *aw = cf, *ae = cf + nr, *as = cf + 2 * nr,
-                     *an = cf + 3 * nr, *ab = cf + 4 * nr, *at = cf + 5 * nr;
+        const Coef6<double> co(cf, nr, i, j, k);
         double q;
         if (MASKED && masked)
             q = laplace_cell<T, double, true>(H + sm, H + s0, H + sp, BH + sm, BH + s0,
-                                              BH + sp, c0, HW, aw[i], ae[i], as[j], an[j],
-                                              ab[k], at[k], apv);
+                                              BH + sp, c0, HW, co.w, co.e, co.s, co.n, co.b, co.t,
+                                              apv);
         else
             q = laplace_cell<T, double, false>(H + sm, H + s0, H + sp, nullptr, nullptr,
-                                               nullptr, c0, HW, aw[i], ae[i], as[j], an[j],
-                                               ab[k], at[k], apv);
+                                               nullptr, c0, HW, co.w, co.e, co.s, co.n, co.b, co.t,
+                                               apv);
         const double rn = d.rhs + q;
         st(rhs, s, rn);
         if (rhs_t) st(rhs_t, s, (T)rn);

@@ -111,7 +111,9 @@ __host__ __device__ inline int row_len(const Topo &t) {
     return ((m > t.max_nk ? m : t.max_nk) + 1) & ~1;
 }
 
-// 1-D coefficient rows of a block in LDS: [w e s n b t][nr]
+// 1-D coefficients of a block in LDS as pairs per direction,
+// [x: (w, e)][y: (s, n)][z: (b, t)] with nr pairs each, so that a cell reads
+// its six coefficients with three paired LDS loads.
 template <typename C>
 __device__ __forceinline__ void coef_rows(C *c, const C *aw, const C *ae,
                                           const C *as, const C *an,
@@ -124,9 +126,25 @@ __device__ __forceinline__ void coef_rows(C *c, const C *aw, const C *ae,
         const C *src = r == 0 ? aw + B.o1x : r == 1 ? ae + B.o1x
                      : r == 2 ? as + B.o1y : r == 3 ? an + B.o1y
                      : r == 4 ? ab + B.o1z : at + B.o1z;
-        c[e] = src[q + 2];
+        c[2 * (nr * (r >> 1) + q) + (r & 1)] = src[q + 2];
     }
 }
+
+template <typename C>
+struct alignas(2 * sizeof(C)) CoefPair {
+    C lo, hi;
+};
+// The six coefficients of cell (i, j, k) from the pairs of coef_rows.
+template <typename C>
+struct Coef6 {
+    C w, e, s, n, b, t;
+    __device__ __forceinline__ Coef6(const C *c, int nr, int i, int j, int k) {
+        const CoefPair<C> x = ((const CoefPair<C> *)c)[i];
+        const CoefPair<C> y = ((const CoefPair<C> *)c)[nr + j];
+        const CoefPair<C> z = ((const CoefPair<C> *)c)[2 * nr + k];
+        w = x.lo; e = x.hi; s = y.lo; n = y.hi; b = z.lo; t = z.hi;
+    }
+};
 
 // Separable part of GSAP in LDS: [x y z][nr] of 2 / (d(i-1) d(i)).
 __device__ __forceinline__ void ap_rows(double *c, const double *ax,

@@ -84,17 +84,16 @@ struct FwdK {
             const int c0 = (j + 1) + __mul24(HW, i + 1);
             const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS,
                       sp = ((p + 2) & 3) * mHS;
-            const T *aw = cf, *ae = cf + nr, *as = cf + 2 * nr,
-                    *an = cf + 3 * nr, *ab = cf + 4 * nr, *at = cf + 5 * nr;
+            const Coef6<T> co(cf, nr, i, j, k);
             const T apv = (APC && !apsh) ? (T)ap_cell(ar, nr, i, j, k, d.c.y >> 24, wf) : d.ap;
             if (MASKED && masked)
                 r = laplace_cell<T, T, true>(H + sm, H + s0, H + sp, BH + sm, BH + s0,
-                                             BH + sp, c0, HW, aw[i], ae[i], as[j], an[j],
-                                             ab[k], at[k], apv);
+                                             BH + sp, c0, HW, co.w, co.e, co.s, co.n, co.b, co.t,
+                                             apv);
             else
                 r = laplace_cell<T, T, false>(H + sm, H + s0, H + sp, nullptr, nullptr,
-                                              nullptr, c0, HW, aw[i], ae[i], as[j], an[j],
-                                              ab[k], at[k], apv);
+                                              nullptr, c0, HW, co.w, co.e, co.s, co.n, co.b, co.t,
+                                              apv);
         }
         const int c1 = (j + 1) + __mul24(RW, i + 1);
         const T *Rp = R + ((p - 1) & 1) * RS;
@@ -356,8 +355,7 @@ struct SorK {
         const T old = H0[c0];
         T nv = old;
         if (!(keep_top && k == NK - 1)) {
-            const T *aw = cf, *ae = cf + nr, *as = cf + 2 * nr,
-                    *an = cf + 3 * nr, *ab = cf + 4 * nr, *at = cf + 5 * nr;
+            const Coef6<T> co(cf, nr, i, j, k);
             const T pw = Hm[c0 - HW], pe = Hp[c0 + HW];
             const T psv = Hm[c0 - 1], pn = Hp[c0 + 1];
             const T pb = Hm[c0], pt = Hp[c0];
@@ -366,16 +364,15 @@ struct SorK {
                 const uint8_t *B0 = BH + ((u + 1) & 7) * mHS;
                 const uint8_t *Bm = BH + (u & 7) * mHS, *Bp = BH + ((u + 2) & 7) * mHS;
                 const T bc = (T)B0[c0];
-                const T cw = (T)Bm[c0 - HW] * bc * aw[i];
-                const T ce = bc * (T)Bp[c0 + HW] * ae[i];
-                const T cs = (T)Bm[c0 - 1] * bc * as[j];
-                const T cn = bc * (T)Bp[c0 + 1] * an[j];
-                const T cb = (T)Bm[c0] * bc * ab[k];
-                const T ct = bc * (T)Bp[c0] * at[k];
+                const T cw = (T)Bm[c0 - HW] * bc * co.w;
+                const T ce = bc * (T)Bp[c0 + HW] * co.e;
+                const T cs = (T)Bm[c0 - 1] * bc * co.s;
+                const T cn = bc * (T)Bp[c0 + 1] * co.n;
+                const T cb = (T)Bm[c0] * bc * co.b;
+                const T ct = bc * (T)Bp[c0] * co.t;
                 r = cw * pw + ce * pe + cs * psv + cn * pn + cb * pb + ct * pt;
             } else {
-                r = aw[i] * pw + ae[i] * pe + as[j] * psv + an[j] * pn +
-                    ab[k] * pb + at[k] * pt;
+                r = co.w * pw + co.e * pe + co.s * psv + co.n * pn + co.b * pb + co.t * pt;
             }
             T rap;
             if (MASKED) {
@@ -498,17 +495,16 @@ struct OutK {
         const int c0 = (j + 1) + __mul24(HW, i + 1);
         const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS, sp = ((p + 2) & 3) * mHS;
         const double apv = MASKED ? d.ap : ap_cell(ar, nr, i, j, k, d.c.y >> 24, wf);
-        const double *aw = cf, *ae = cf + nr, *as = cf + 2 * nr,
-                     *an = cf + 3 * nr, *ab = cf + 4 * nr, *at = cf + 5 * nr;
+        const Coef6<double> co(cf, nr, i, j, k);
         double r;
         if (MASKED && masked)
             r = laplace_cell<T, double, true>(H + sm, H + s0, H + sp, BH + sm, BH + s0,
-                                              BH + sp, c0, HW, aw[i], ae[i], as[j], an[j],
-                                              ab[k], at[k], apv);
+                                              BH + sp, c0, HW, co.w, co.e, co.s, co.n, co.b, co.t,
+                                              apv);
         else
             r = laplace_cell<T, double, false>(H + sm, H + s0, H + sp, nullptr, nullptr,
-                                               nullptr, c0, HW, aw[i], ae[i], as[j], an[j],
-                                               ab[k], at[k], apv);
+                                               nullptr, c0, HW, co.w, co.e, co.s, co.n, co.b, co.t,
+                                               apv);
         const double rn = d.rhs + r;
         st(rhs, s, rn);
         if (rhs_t) st(rhs_t, s, (T)rn);
