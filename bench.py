@@ -175,11 +175,24 @@ def cpu_baseline(target_s=12.0):
     t1 = o.time_sip(st, rhs, 1, threads)
     niter = max(2, int(target_s / max(t1, 1e-3)))
     t = o.time_sip(st, rhs, niter, threads)
+    # one thread, about 3 s
+    s1 = o.time_sip(st, rhs, 1, 1)
+    n1 = max(2, int(3.0 / max(s1, 1e-3)))
+    t_one = o.time_sip(st, rhs, n1, 1)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh
+                          if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return dict(value=round(g.ncells * niter / t / 1e9, 4), unit="GCell-updates/s",
-                cores=threads, kind="port",
+                cores=threads, kind="port", cpu_model=model,
+                single_thread=round(g.ncells * n1 / t_one / 1e9, 4),
                 sample=f"{g.ngrid} blocks of 24^3 ({g.ncells} cells), {niter} hyperplane-SIP "
                        f"inner iterations (laplacephi+sipiter1+conn+sipiter2+conn), "
-                       f"OpenMP over blocks, {t:.1f} s")
+                       f"OpenMP over blocks, {t:.1f} s; single_thread: {n1} iterations "
+                       f"on 1 thread, {t_one:.1f} s")
 
 
 def adjoint_config():
