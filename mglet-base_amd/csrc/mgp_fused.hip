@@ -91,13 +91,13 @@ struct OutFwdK {
         T r = T(0);
         r = ((T)rn + r) * d.lpr;
         if (CLASSIC) {
-            r = r - d.lw * Rp[c1 - RW];
-            r = r - d.ls * Rp[c1 - 1];
-            r = r - d.lb * Rp[c1];
+            r = fms(r, d.lw, Rp[c1 - RW]);
+            r = fms(r, d.ls, Rp[c1 - 1]);
+            r = fms(r, d.lb, Rp[c1]);
         } else {
-            r = r - d.lb * Rp[c1];
-            r = r - d.ls * Rp[c1 - 1];
-            r = r - d.lw * Rp[c1 - RW];
+            r = fms(r, d.lb, Rp[c1]);
+            r = fms(r, d.ls, Rp[c1 - 1]);
+            r = fms(r, d.lw, Rp[c1 - RW]);
         }
         Rc[c1] = r;
         st(res, s, r);

@@ -68,6 +68,14 @@ struct SrcTab {
     const uint8_t *b[8];  // window BP
 };
 
+// r - a b.  FP64 keeps the reference's separate multiply and subtract (the
+// build compiles without contraction, for bitwise parity); the FP32 sweeps
+// of the mixed-precision form, which the FP64 defect correction makes
+// converge to the FP64 solution and which are not held to bitwise parity,
+// use one fused multiply-add.
+__device__ __forceinline__ double fms(double r, double a, double b) { return r - a * b; }
+__device__ __forceinline__ float fms(float r, float a, float b) { return __builtin_fmaf(-a, b, r); }
+
 // res = -(A phi) at one cell, src/flow/laplacephi_mod.F90:149-162, with
 // phi and BP read from the window planes p-1, p, p+1.
 template <typename T, typename A, bool MASKED>
@@ -94,13 +102,13 @@ __device__ __forceinline__ A laplace_cell(const T *Hm, const T *H0,
         r = r - at * pt * bc * bt;
     } else {
         r = -(aw * pw);
-        r = r - ae * pe;
-        r = r - as * psv;
-        r = r - an * pn;
-        r = r - ab * pb;
-        r = r - at * pt;
+        r = fms(r, ae, pe);
+        r = fms(r, as, psv);
+        r = fms(r, an, pn);
+        r = fms(r, ab, pb);
+        r = fms(r, at, pt);
     }
-    r = r - ap * pc;
+    r = fms(r, ap, pc);
     return r;
 }
 

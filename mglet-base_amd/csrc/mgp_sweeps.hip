@@ -100,13 +100,13 @@ struct FwdK {
         T *Rc = R + (p & 1) * RS;
         r = (d.rhs + r) * d.lpr;
         if (CLASSIC) {
-            r = r - d.lw * Rp[c1 - RW];
-            r = r - d.ls * Rp[c1 - 1];
-            r = r - d.lb * Rp[c1];
+            r = fms(r, d.lw, Rp[c1 - RW]);
+            r = fms(r, d.ls, Rp[c1 - 1]);
+            r = fms(r, d.lb, Rp[c1]);
         } else {
-            r = r - d.lb * Rp[c1];
-            r = r - d.ls * Rp[c1 - 1];
-            r = r - d.lw * Rp[c1 - RW];
+            r = fms(r, d.lb, Rp[c1]);
+            r = fms(r, d.ls, Rp[c1 - 1]);
+            r = fms(r, d.lw, Rp[c1 - RW]);
         }
         Rc[c1] = r;
         st(res, s, r);
@@ -238,13 +238,13 @@ struct BwdK {
         T *Rc = R + (p & 1) * RS;
         T r = d.res;
         if (CLASSIC) {
-            r = r - d.un * Rp[c1 + 1];
-            r = r - d.ue * Rp[c1 + RW];
-            r = r - d.ut * Rp[c1];
+            r = fms(r, d.un, Rp[c1 + 1]);
+            r = fms(r, d.ue, Rp[c1 + RW]);
+            r = fms(r, d.ut, Rp[c1]);
         } else {
-            r = r - d.ut * Rp[c1];
-            r = r - d.un * Rp[c1 + 1];
-            r = r - d.ue * Rp[c1 + RW];
+            r = fms(r, d.ut, Rp[c1]);
+            r = fms(r, d.un, Rp[c1 + 1]);
+            r = fms(r, d.ue, Rp[c1 + RW]);
         }
         Rc[c1] = r;
         const T h = (HZ ? T(0) : d.h) + r;
