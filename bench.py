@@ -7,7 +7,10 @@ pressure solve.  A step is one outer (defect-correction) iteration of the
 projection: `ninner` SIP inner iterations over every cell plus the FP64
 residual/RHS/pressure update.  `value` = interior cells x inner iterations /
 wall time of the K timed steps (GCell-updates/s) for the FP32-refined form;
-the all-FP64 form is timed the same way and reported beside it.
+the all-FP64 form is timed the same way and reported beside it, and so is
+the FP32-refined form with one copy of the SIP factors per block
+(`fp32-unshared`; the library shares one copy between blocks whose factors
+are equal bit for bit, every block here).
 
 The line also carries:
   roofline     -- dominant kernel (forward sweep) timed with HIP events on
@@ -396,7 +399,10 @@ def main():
                     help="write each form's max|RHS| per outer iteration of the tts solve "
                          "to this .npz file")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp32,fp64")
+    ap.add_argument("--forms", default="fp32,fp64,fp32-unshared",
+                    help="fp32 / fp64: the FP32-refined and all-FP64 forms; "
+                         "fp32-unshared: FP32-refined with one copy of the SIP factors "
+                         "per block (MGP_SHARED_FACTORS=0), timed the same way")
     ap.add_argument("--stage", action="store_true",
                     help="also time the RK stage kernels (tstle4, rkstep) on the workload")
     ap.add_argument("--unshared", action="store_true",
@@ -432,7 +438,12 @@ def main():
     filled = False
     for form in forms:
         log(f"[bench] init {form}")
-        s = mgp.PressureSolver(g, ninner=args.ninner, nouter=1, precision=form, check_every=4)
+        prec = form.split("-")[0]
+        if form.endswith("-unshared"):
+            os.environ["MGP_SHARED_FACTORS"] = "0"
+        s = mgp.PressureSolver(g, ninner=args.ninner, nouter=1, precision=prec, check_every=4)
+        if form.endswith("-unshared") and not args.unshared:
+            os.environ.pop("MGP_SHARED_FACTORS", None)
         if not filled:      # the caller's connect: ghosts of the input velocity
             s.connect(fields0, layers=2, corners=True)
             filled = True
@@ -458,16 +469,17 @@ def main():
     copy_gbs = copy_bandwidth()
     cells = g.ncells
     main_form = "fp32" if "fp32" in res else forms[0]
+    mp = main_form.split("-")[0]     # its precision
     r = res[main_form]
     upd = cells * args.ninner * args.steps
     value = upd / r["wall"] / 1e9
     fw_ms = r["kernels"]["forward_ms"]
-    alg = ALG_BYTES[main_form]["forward"] * cells
+    alg = ALG_BYTES[mp]["forward"] * cells
     achieved = alg / (fw_ms * 1e-3) / 1e9
     traffic = None
     tfile = ROOT / "profiles" / "traffic_sip_32768x24.json"
     if tfile.exists() and nb == (32, 32, 32) and n == (24, 24, 24):
-        key = (f"void mgp::k_forward<{'float' if main_form == 'fp32' else 'double'}, "
+        key = (f"void mgp::k_forward<{'float' if mp == 'fp32' else 'double'}, "
                f"false, false, true, ")
         per = json.loads(tfile.read_text())["per_launch"]
         hit = [k for k in per if k.startswith(key)]
@@ -480,7 +492,7 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 inner sweeps, f64 outer" if main_form == "fp32" else "f64",
+        "dtype": "f32 inner sweeps, f64 outer" if mp == "fp32" else "f64",
         "data": "synthetic (seeded k^-5/3 Gaussian velocity, rms 1e-3)",
         "config": {"workload": f"SIP projection, {g.ngrid} blocks of {n[0]}^3 "
                                f"({nb[0]}x{nb[1]}x{nb[2]} periodic), {cells} cells",
@@ -498,20 +510,20 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_unit": "GB per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
-                     "alg_bytes_per_cell": ALG_BYTES[main_form]["forward"],
+                     "alg_bytes_per_cell": ALG_BYTES[mp]["forward"],
                      # HBM bytes actually moved per launch over the launch time:
                      # factors and GSAP shared between blocks come from L2
                      "hbm_gbs": (round(traffic / (fw_ms * 1e-3), 1) if traffic else None),
                      "copy_gbs": round(copy_gbs, 1),
                      "frac_of_copy": round(achieved / copy_gbs, 4),
                      "inner_iteration_gbs": round(
-                         (ALG_BYTES[main_form]["forward"] + ALG_BYTES[main_form]["backward"])
+                         (ALG_BYTES[mp]["forward"] + ALG_BYTES[mp]["backward"])
                          * cells / ((fw_ms + r["kernels"]["backward_ms"]) * 1e-3) / 1e9, 1)},
     }
     # whole inner iteration against the fused algorithmic bytes per
     # cell-update (DP, RHS, AP, 7 factors, mask read, DP written: 45 B in
     # FP32, 89 B in FP64), whatever the kernels actually move
-    fused = {"fp32": 45, "fp64": 89}[main_form]
+    fused = {"fp32": 45, "fp64": 89}[mp]
     line["roofline"]["fused_update"] = {
         "bytes_per_update": fused, "achieved": round(value * fused, 1),
         "frac": round(value * fused / HBM_PEAK_GBS, 4)}
@@ -527,9 +539,10 @@ def main():
         tts = {"config": f"{tg.ngrid} blocks of {n[0]}^3 ({tnb[0]}x{tnb[1]}x{tnb[2]} periodic)",
                "eps_rel": 1e-10}
         ps, hist = {}, {}
-        for form in forms:
+        tforms = [f for f in forms if not f.endswith("-unshared")]
+        for form in tforms:
             s = mgp.PressureSolver(tg, ninner=args.ninner, nouter=1, precision=form)
-            if form == forms[0] and args.tts == "c2":
+            if form == tforms[0] and args.tts == "c2":
                 s.connect(tf, layers=2, corners=True)
             time_to_solution(s, tf, args.ninner, 1e-10, 2)        # warm
             t, st, p = time_to_solution(s, tf, args.ninner, 1e-10, args.tts_cap)
