@@ -175,9 +175,6 @@ struct mgp_handle {
     struct TaskSet {
         DBuf d;
         std::vector<int> stage_start;  // per stage offsets (+ end)
-        DBuf dfirst;                   // first task of each copy workgroup
-        std::vector<int> wg_start;     // per stage offsets into dfirst
-        std::vector<int64_t> total;    // elements per stage
     };
     std::map<std::tuple<int, int, int, int>, std::unique_ptr<TaskSet>> tasks;
     // captured outer-iteration graphs per level
@@ -466,36 +463,16 @@ mgp_handle::TaskSet &task_set(mgp_handle &h, int level, int layers,
         rdl[list[q].sg].push_back((int)q);
     }
     std::vector<CopyTask> sorted;
-    std::vector<int> first;
     auto ts = std::make_unique<mgp_handle::TaskSet>();
     for (int s = 0; s < nst; ++s) {
-        const int t0 = (int)sorted.size();
-        ts->stage_start.push_back(t0);
-        ts->wg_start.push_back((int)first.size());
-        int64_t e = 0;
+        ts->stage_start.push_back((int)sorted.size());
         for (size_t q = 0; q < list.size(); ++q)
-            if (stage[q] == s) {
-                CopyTask t = list[q].t;
-                t.eoff = e;
-                e += (int64_t)t.ni * t.nj * t.nk;
-                sorted.push_back(t);
-            }
-        ts->total.push_back(e);
-        // first task (index within the stage) of each workgroup's elements
-        int cur = t0;
-        for (int64_t w = 0; w * kCopyThreads < e; ++w) {
-            while (cur + 1 < (int)sorted.size() && sorted[cur + 1].eoff <= w * kCopyThreads) ++cur;
-            first.push_back(cur - t0);
-        }
+            if (stage[q] == s) sorted.push_back(list[q].t);
     }
     ts->stage_start.push_back((int)sorted.size());
     ts->d.alloc(sorted.size() * sizeof(CopyTask));
     if (!sorted.empty())
         HC(hipMemcpy(ts->d.p, sorted.data(), sorted.size() * sizeof(CopyTask),
-                     hipMemcpyHostToDevice));
-    ts->dfirst.alloc(first.size() * sizeof(int));
-    if (!first.empty())
-        HC(hipMemcpy(ts->dfirst.p, first.data(), first.size() * sizeof(int),
                      hipMemcpyHostToDevice));
     auto &ref = *ts;
     h.tasks[key] = std::move(ts);
@@ -508,8 +485,8 @@ void exchange(mgp_handle &h, int level, int layers, int corners, int forward,
     for (size_t s = 0; s + 1 < ts.stage_start.size(); ++s) {
         const int a = ts.stage_start[s], b = ts.stage_start[s + 1];
         for (int f0 = 0; f0 < nfield; f0 += 6)
-            launch_copy(ts.d.as<CopyTask>() + a, b - a, ts.dfirst.as<int>() + ts.wg_start[s],
-                        ts.total[s], fields + f0, std::min(6, nfield - f0), h.stream);
+            launch_copy(ts.d.as<CopyTask>() + a, b - a, fields + f0, std::min(6, nfield - f0),
+                        h.stream);
     }
 }
 

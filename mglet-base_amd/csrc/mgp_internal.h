@@ -138,8 +138,7 @@ struct CopyTask {
     int ni, nj, nk;    // box extent
     int s_sj, s_si;    // source strides of j and i (k stride is 1)
     int d_sj, d_si;    // destination strides
-    int64_t eoff;      // first element of the box in its stage's element range
 };
-constexpr int kCopyThreads = 256;   // elements per workgroup of k_copy
+constexpr int kCopyThreads = 256;   // threads per workgroup of k_copy (a box per wave)
 
 }  // namespace mgp

@@ -703,45 +703,54 @@ void launch_mgpcorr(const Topo &t, double *u, double *v, double *w,
 
 // ---------------------------------------------------------------------
 // Ghost-layer box copies (connect_self_single, src/core/connect/
-// conn1_mod.F90:674-704), all fields at once.  The boxes of one stage form
-// one element range; each workgroup takes kCopyThreads consecutive
-// elements (k fastest inside a box, so copies stay coalesced) and finds
-// their boxes by a binary search from the first box of its range, which
-// keeps the many small edge and corner boxes from costing a workgroup each.
+// conn1_mod.F90:674-704), all fields at once.  One wave per box: the box
+// record comes in through scalar loads, and the wave walks the box 64
+// elements at a time, k fastest (contiguous in both the source and the
+// destination layout), so there is no search for a thread's box.  Element
+// indices are split with a float reciprocal and one correction step
+// (exact for the small box sizes of a ghost layer).
 struct FieldList {
     double *f[6];
     int n;
 };
 
-__global__ __launch_bounds__(kCopyThreads) void k_copy(const CopyTask *tasks, int ntask,
-                                                       const int *first, int64_t total,
-                                                       FieldList fl) {
-    const int64_t g = (int64_t)blockIdx.x * kCopyThreads + threadIdx.x;
-    if (g >= total) return;
-    int lo = first[blockIdx.x];
-    int hi = blockIdx.x + 1 < gridDim.x ? first[blockIdx.x + 1] : ntask - 1;
-    while (lo < hi) {   // last task with eoff <= g
-        const int mid = (lo + hi + 1) >> 1;
-        if (tasks[mid].eoff <= g) lo = mid;
-        else hi = mid - 1;
-    }
-    const CopyTask T = tasks[lo];
-    const int e = (int)(g - T.eoff);
-    const int k = e % T.nk, j = (e / T.nk) % T.nj, i = e / (T.nk * T.nj);
-    const int64_t so = T.src + k + (int64_t)j * T.s_sj + (int64_t)i * T.s_si;
-    const int64_t d = T.dst + k + (int64_t)j * T.d_sj + (int64_t)i * T.d_si;
-    for (int f = 0; f < fl.n; ++f) fl.f[f][d] = fl.f[f][so];
+__device__ __forceinline__ int div_small(int e, int d, float rd) {
+    int q = (int)((float)e * rd);
+    const int r = e - q * d;
+    if (r < 0) --q;
+    else if (r >= d) ++q;
+    return q;
 }
 
-void launch_copy(const CopyTask *tasks, int ntask, const int *first,
-                 int64_t total, double *const *fields, int nfield,
+constexpr int kCopyWaves = kCopyThreads / 64;
+
+__global__ __launch_bounds__(kCopyThreads) void k_copy(const CopyTask *tasks, int ntask,
+                                                       FieldList fl) {
+    const int b = __builtin_amdgcn_readfirstlane(blockIdx.x * kCopyWaves + (threadIdx.x >> 6));
+    if (b >= ntask) return;
+    const int lane = threadIdx.x & 63;
+    const CopyTask T = tasks[b];
+    const int nk = T.nk, njk = T.nj * T.nk, n = T.ni * njk;
+    const float rnk = 1.0f / (float)nk, rnjk = 1.0f / (float)njk;
+    for (int e = lane; e < n; e += 64) {
+        const int i = div_small(e, njk, rnjk);
+        const int r = e - i * njk;
+        const int j = div_small(r, nk, rnk);
+        const int k = r - j * nk;
+        const int64_t so = T.src + k + (int64_t)j * T.s_sj + (int64_t)i * T.s_si;
+        const int64_t d = T.dst + k + (int64_t)j * T.d_sj + (int64_t)i * T.d_si;
+        for (int f = 0; f < fl.n; ++f) fl.f[f][d] = fl.f[f][so];
+    }
+}
+
+void launch_copy(const CopyTask *tasks, int ntask, double *const *fields, int nfield,
                  hipStream_t s) {
-    if (ntask <= 0 || total <= 0) return;
+    if (ntask <= 0) return;
     FieldList fl;
     fl.n = nfield;
     for (int f = 0; f < nfield && f < 6; ++f) fl.f[f] = fields[f];
-    const unsigned nwg = (unsigned)((total + kCopyThreads - 1) / kCopyThreads);
-    k_copy<<<nwg, kCopyThreads, 0, s>>>(tasks, ntask, first, total, fl);
+    const unsigned nwg = (unsigned)((ntask + kCopyWaves - 1) / kCopyWaves);
+    k_copy<<<nwg, kCopyThreads, 0, s>>>(tasks, ntask, fl);
 }
 
 }  // namespace mgp

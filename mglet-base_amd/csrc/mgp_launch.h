@@ -172,10 +172,8 @@ void launch_tstle4(const Topo &t, const MomArgs &a, hipStream_t s);
 void launch_rkstep(double *p, double *dp, const double *rhsp, int64_t n,
                    double frhs, double dtfu, hipStream_t s);
 
-// One stage of box copies: `ntask` tasks with `total` elements; first[w]
-// is the task holding element w * kCopyThreads.
-void launch_copy(const CopyTask *tasks, int ntask, const int *first,
-                 int64_t total, double *const *fields, int nfield,
+// One stage of box copies: `ntask` boxes, one wave each.
+void launch_copy(const CopyTask *tasks, int ntask, double *const *fields, int nfield,
                  hipStream_t s);
 
 template <typename T>
