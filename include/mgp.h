@@ -192,8 +192,9 @@ int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
  * = convection + diffusion with viscosity field g (gmol the molecular
  * viscosity in its harmonic means; ilesmodel != 0 adds the transpose part
  * of the stress) - grad p / rho - gradp[3] / rho (gradp may be NULL), zero
- * outside the loop ranges.  rho is the handle's.  Grids with NOS walls
- * (Werner-Wengle wall stress) or PAR faces are rejected (MGP_ERR_TOPOLOGY).
+ * outside the loop ranges; next to NOS walls the Werner-Wengle wall stress
+ * of swcle3d (:1756-1845) is subtracted from the tangential components.
+ * rho is the handle's.  Grids with PAR faces are rejected (MGP_ERR_TOPOLOGY).
  * Replaces the call at src/flow/timeintegration_mod.F90:98. */
 int mgp_tstle4_device(mgp_handle *h, mgletreal *uo, mgletreal *vo, mgletreal *wo,
                       const mgletreal *u, const mgletreal *v, const mgletreal *w,

@@ -1889,6 +1889,18 @@ int mgp_set_sdiv(mgp_handle *h, const mgletreal *sdiv) {
     return MGP_OK;
 }
 
+// Werner-Wengle constants (init_wernerwengle, src/flow/wernerwengle_mod.F90:
+// 13-29): cwb, cpo4, cpo5, cpo6, cpo8 for the wall stress next to NOS walls
+static void ww_constants(double rho, double gmol, double ww[5]) {
+    const double cwa = 8.3, cwb = 1.0 / 7.0;
+    const double cpo1 = 1.0 - cwb, cpo2 = 1.0 + cwb;
+    ww[0] = cwb;
+    ww[1] = cpo2 / cwa * std::pow(gmol / rho, cwb);
+    ww[2] = 0.5 * cpo1 * std::pow(cwa, cpo2 / cpo1) * std::pow(gmol / rho, cpo2);
+    ww[3] = 0.5 * gmol / rho * std::pow(cwa, 2.0 / cpo1);
+    ww[4] = 2.0 / cpo2;
+}
+
 int mgp_tstle4_device(mgp_handle *h, mgletreal *uo, mgletreal *vo, mgletreal *wo,
                       const mgletreal *u, const mgletreal *v, const mgletreal *w,
                       const mgletreal *p, const mgletreal *g, mgletreal gmol,
@@ -1897,9 +1909,9 @@ int mgp_tstle4_device(mgp_handle *h, mgletreal *uo, mgletreal *vo, mgletreal *wo
     if (!uo || !vo || !wo || !u || !v || !w || !p || !g) return fail(MGP_ERR_ARG, "null field");
     for (int q = 0; q < h->ngrid; ++q)
         for (int f = 0; f < 6; ++f)
-            if (h->bc[6 * q + f] == 5 || h->bc[6 * q + f] == PAR_BC)
-                return fail(MGP_ERR_TOPOLOGY, "grid %d face %d: tstle4 covers neither NOS walls "
-                            "(wall stress) nor PAR faces", q + 1, f + 1);
+            if (h->bc[6 * q + f] == PAR_BC)
+                return fail(MGP_ERR_TOPOLOGY, "grid %d face %d: tstle4 does not cover PAR faces "
+                            "(tstle4_par)", q + 1, f + 1);
     try {
         MomArgs a;
         a.uo = uo; a.vo = vo; a.wo = wo;
@@ -1914,6 +1926,8 @@ int mgp_tstle4_device(mgp_handle *h, mgletreal *uo, mgletreal *vo, mgletreal *wo
         a.rrho = 1.0 / h->cfg.rho;
         a.gmol = gmol;
         a.iles = ilesmodel == 0 ? 0.0 : 1.0;
+        a.rho = h->cfg.rho;
+        ww_constants(h->cfg.rho, gmol, a.ww);
         for (int q = 0; q < 3; ++q) a.gp[q] = gradp ? gradp[q] : 0.0;
         launch_tstle4(h->topo(-1), a, h->stream);
         HC(hipGetLastError());

@@ -167,6 +167,8 @@ struct MomArgs {
     const double *rpdx, *rpdy, *rpdz;   // 1 / (rho dx) etc.
     double rrho, gmol, iles;            // 1 / rho
     double gp[3];      // body-force pressure gradient (gradp)
+    double rho;
+    double ww[5];      // Werner-Wengle constants cwb, cpo4, cpo5, cpo6, cpo8
 };
 void launch_tstle4(const Topo &t, const MomArgs &a, hipStream_t s);
 void launch_rkstep(double *p, double *dp, const double *rhsp, int64_t n,

@@ -23,12 +23,55 @@ __device__ __forceinline__ double gpair(double a, double b, double gmol) {
     return a * b / (s > gmol ? s : gmol);
 }
 
-// tstle4 without immersed boundaries, PAR faces or NOS walls
-// (src/flow/tstle4_mod.F90:13-529): uo = convection (tstle4_kon :139-265)
-// + diffusion (tstle4_diff :268-458, the transpose part of the stress when
-// the LES model is on) - pressure gradient and body-force gradient
-// (tstle4_gradp :461-529); zero outside the loop ranges, as the reference
-// zeroes uo before filling it.  The transporting velocities are u, v, w.
+// swcle3d_one (src/flow/tstle4_mod.F90:1848-1859) with tauwin
+// (src/flow/wernerwengle_mod.F90:64-87): the Werner-Wengle wall shear stress
+// of velocity uq at wall distance dd, over rho dd.  pow is the device
+// library's, so these terms agree with a CPU evaluation to rounding, not
+// bit for bit.
+__device__ __forceinline__ double swc(double dd, double uq, const MomArgs &a) {
+    const double vz = copysign(1.0, uq), un = fabs(uq);
+    double tau;
+    if (un >= a.ww[3] / dd) {
+        const double ddsb = pow(dd, -a.ww[0]);
+        tau = vz * a.rho * pow(ddsb * (a.ww[1] * un + a.ww[2] / dd), a.ww[4]);
+    } else {
+        tau = vz * 2.0 * a.gmol * un / dd;
+    }
+    return tau / a.rho / dd;
+}
+
+// The swcle3d terms (src/flow/tstle4_mod.F90:1756-1845) of velocity
+// component comp (0 u, 1 v, 2 w) at (k, j, i): NOS walls act on the
+// tangential components of the first cell, each face loop running over
+// 2..n-1 of the tangential indices, in the reference's face order.
+__device__ __forceinline__ double wall_terms(double val, int comp, int k, int j, int i,
+                                             const int *bc, int kk, int jj, int ii,
+                                             double vel, const double *ddx, const double *ddy,
+                                             const double *ddz, const MomArgs &a) {
+    const bool tk = k >= 2 && k <= kk - 1, tj = j >= 2 && j <= jj - 1, ti = i >= 2 && i <= ii - 1;
+    if (comp != 0) {
+        if (bc[0] == 5 && i == 3 && tj && tk) val = val - swc(ddx[i], vel, a);
+        if (bc[1] == 5 && i == ii - 2 && tj && tk) val = val - swc(ddx[i], vel, a);
+    }
+    if (comp != 1) {
+        if (bc[2] == 5 && j == 3 && ti && tk) val = val - swc(ddy[j], vel, a);
+        if (bc[3] == 5 && j == jj - 2 && ti && tk) val = val - swc(ddy[j], vel, a);
+    }
+    if (comp != 2) {
+        if (bc[4] == 5 && k == 3 && ti && tj) val = val - swc(ddz[k], vel, a);
+        if (bc[5] == 5 && k == kk - 2 && ti && tj) val = val - swc(ddz[k], vel, a);
+    }
+    return val;
+}
+
+// tstle4 without immersed boundaries or PAR faces (src/flow/tstle4_mod.F90:
+// 13-529): uo = convection (tstle4_kon :139-265), minus the wall stress next
+// to NOS walls (swcle3d, called first in tstle4_diff :315), + diffusion
+// (tstle4_diff :268-458, the transpose part of the stress when the LES
+// model is on) - pressure gradient and body-force gradient (tstle4_gradp
+// :461-529); zero outside the loop ranges (but for the wall terms, whose
+// loops include ghost rows), as the reference zeroes uo before filling it.
+// The transporting velocities are u, v, w.
 // 1/rho and 1/(rho dx) come from host tables: the same correctly rounded
 // quotients, without a division per position.
 // One-dimensional grid of nchunk workgroups per block, consecutive
@@ -60,6 +103,8 @@ __global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a, int n
     const int nrv = B.bc[2] == 3, nlv = B.bc[3] == 7 || B.bc[3] == 3;
     const int nbw = B.bc[4] == 3, ntw = B.bc[5] == 7 || B.bc[5] == 3;
     const double rrho = a.rrho, gmol = a.gmol, iles = a.iles;
+    const bool nos = B.bc[0] == 5 || B.bc[1] == 5 || B.bc[2] == 5 || B.bc[3] == 5 ||
+                     B.bc[4] == 5 || B.bc[5] == 5;
     const double *rpx = a.rpdx + B.o1x - 1, *rpy = a.rpdy + B.o1y - 1,
                  *rpz = a.rpdz + B.o1z - 1;
     const bool inj = j >= 3 && j <= jj - 2, ink = k >= 3 && k <= kk - 2,
@@ -84,6 +129,7 @@ __global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a, int n
         double qt = 0.5 * ft * (U(0) + U(1));
         double qb = 0.5 * fb * (U(-1) + U(0));
         ro = -((qe - qw + qn - qs + qt - qb) * rdx[i] * rddy[j] * rddz[k]);
+        if (nos) ro = wall_terms(ro, 0, k, j, i, B.bc, kk, jj, ii, U(0), ddx, ddy, ddz, a);
         const double ge = G(si), gw = G(0);
         const double gn = gpair(G(0), G(sj), gmol) + gpair(G(si), G(si + sj), gmol);
         const double gs = gpair(G(-sj), G(0), gmol) + gpair(G(si - sj), G(si), gmol);
@@ -102,6 +148,8 @@ __global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a, int n
         const double fak = rrho * rddy[j] * rdx[i] * rddz[k];
         ro = ro - fak * (qe - qw + qn - qs + qt - qb - qc);
         ro = ro - rpx[i] * (p[c + si] - p[c] + a.gp[0] * dx[i]);
+    } else if (nos) {
+        ro = wall_terms(ro, 0, k, j, i, B.bc, kk, jj, ii, U(0), ddx, ddy, ddz, a);
     }
     a.uo[c] = ro;
     ro = 0.0;
@@ -120,6 +168,7 @@ __global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a, int n
         double qt = 0.5 * ft * (V(0) + V(1));
         double qb = 0.5 * fb * (V(-1) + V(0));
         ro = -((qe - qw + qn - qs + qt - qb) * rddx[i] * rdy[j] * rddz[k]);
+        if (nos) ro = wall_terms(ro, 1, k, j, i, B.bc, kk, jj, ii, V(0), ddx, ddy, ddz, a);
         const double ge = gpair(G(0), G(si), gmol) + gpair(G(sj), G(sj + si), gmol);
         const double gw = gpair(G(-si), G(0), gmol) + gpair(G(sj - si), G(sj), gmol);
         const double gn = G(sj), gs = G(0);
@@ -138,6 +187,8 @@ __global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a, int n
         const double fak = rrho * rddx[i] * rdy[j] * rddz[k];
         ro = ro - fak * (qe - qw + qn - qs + qt - qb - qc);
         ro = ro - rpy[j] * (p[c + sj] - p[c] + a.gp[1] * dy[j]);
+    } else if (nos) {
+        ro = wall_terms(ro, 1, k, j, i, B.bc, kk, jj, ii, V(0), ddx, ddy, ddz, a);
     }
     a.vo[c] = ro;
     ro = 0.0;
@@ -156,6 +207,7 @@ __global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a, int n
         double qt = 0.5 * ft * (W(0) + W(1));
         double qb = 0.5 * fb * (W(-1) + W(0));
         ro = -((qe - qw + qn - qs + qt - qb) * rddx[i] * rddy[j] * rdz[k]);
+        if (nos) ro = wall_terms(ro, 2, k, j, i, B.bc, kk, jj, ii, W(0), ddx, ddy, ddz, a);
         const double ge = gpair(G(0), G(si), gmol) + gpair(G(1), G(1 + si), gmol);
         const double gw = gpair(G(-si), G(0), gmol) + gpair(G(1 - si), G(1), gmol);
         const double gn = gpair(G(0), G(sj), gmol) + gpair(G(1), G(1 + sj), gmol);
@@ -174,6 +226,8 @@ __global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a, int n
         const double fak = rrho * rddx[i] * rddy[j] * rdz[k];
         ro = ro - fak * (qe - qw + qn - qs + qt - qb - qc);
         ro = ro - rpz[k] * (p[c + 1] - p[c] + a.gp[2] * dz[k]);
+    } else if (nos) {
+        ro = wall_terms(ro, 2, k, j, i, B.bc, kk, jj, ii, W(0), ddx, ddy, ddz, a);
     }
     a.wo[c] = ro;
 #undef U

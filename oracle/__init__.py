@@ -62,7 +62,25 @@ def lib():
         _lib.og_maxabscal.restype = C.c_double
         _lib.og_time_sip.restype = C.c_double
         _lib.og_par_buffers.restype = C.c_int64
+        _lib.og_swc.restype = C.c_double
+        _lib.og_swc.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+        _lib.og_ww_constants.argtypes = [C.c_double, C.c_double, _dp]
     return _lib
+
+
+def ww_constants(rho, gmol):
+    """Werner-Wengle constants (cwb, cpo4, cpo5, cpo6, cpo8),
+    src/flow/wernerwengle_mod.F90:13-29."""
+    ww = np.zeros(5)
+    lib().og_ww_constants(rho, gmol, _p(ww))
+    return ww
+
+
+def wall_term(dd, u, rho, gmol):
+    """swcle3d_one (src/flow/tstle4_mod.F90:1848-1859): Werner-Wengle wall
+    shear stress (tauwin) of velocity u at wall distance dd, over rho dd."""
+    ww = ww_constants(rho, gmol)
+    return lib().og_swc(dd, u, rho, gmol, _p(ww))
 
 
 def _p(a, ct=_dp):
@@ -167,8 +185,9 @@ class Oracle:
                            _p(sdiv) if sdiv is not None else None)
 
     def tstle4(self, u, v, w, p, g_visc, rho, gmol, ilesmodel=0, gradp=None):
-        """Momentum right-hand side (src/flow/tstle4_mod.F90:13-529, noib):
-        returns (uo, vo, wo)."""
+        """Momentum right-hand side (src/flow/tstle4_mod.F90:13-529, noib,
+        with the Werner-Wengle wall stress next to NOS walls): returns
+        (uo, vo, wo)."""
         g = self.g
         uo, vo, wo = g.zeros(), g.zeros(), g.zeros()
         gp = np.asarray(gradp, dtype=np.float64) if gradp is not None else None
@@ -179,7 +198,7 @@ class Oracle:
                                 _p(g.rdx), _p(g.rdy), _p(g.rdz), _p(g.rddx), _p(g.rddy),
                                 _p(g.rddz))
         if rc != 0:
-            raise ValueError("tstle4: NOS wall or PAR faces are not restated")
+            raise ValueError("tstle4: PAR faces are not restated")
         return uo, vo, wo
 
     def rkstep(self, p, dp, rhsp, frhs, dtfu):

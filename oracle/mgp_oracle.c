@@ -1512,18 +1512,72 @@ double og_time_sip(const og_topo *t, const og_conn *c, int nconn, int niter,
 
 /* ------------------------------------------------------------------ */
 /* Momentum right-hand side of one RK stage, tstle4 without immersed
- * boundaries and without PAR or slip faces (src/flow/tstle4_mod.F90:13-120):
- * uo = convection (tstle4_kon :139-265) + diffusion (tstle4_diff :268-458)
- * - pressure gradient (tstle4_gradp :461-529).  ut = u etc. (noib).  Each
- * expression keeps the reference's association order (Fortran evaluates
- * operators of equal precedence left to right; a leading minus applies to
- * the whole product).  Returns -1 for a face type the restatement does not
- * cover: NOS walls (5), whose Werner-Wengle wall stress (swcle3d,
- * :1756-1845) is not restated, and PAR faces (8, tstle4_par). */
+ * boundaries and without PAR faces (src/flow/tstle4_mod.F90:13-120):
+ * uo = convection (tstle4_kon :139-265), minus the Werner-Wengle wall
+ * stress next to NOS walls (swcle3d, :1756-1845, called at the start of
+ * tstle4_diff :315), + diffusion (tstle4_diff :268-458) - pressure gradient
+ * (tstle4_gradp :461-529).  ut = u etc. (noib).  Each expression keeps the
+ * reference's association order (Fortran evaluates operators of equal
+ * precedence left to right; a leading minus applies to the whole product).
+ * Returns -1 for PAR faces (8, tstle4_par), which are not restated. */
 static double gpair(double a, double b, double gmol)
 {
     const double s = a + b;
     return a * b / (s > gmol ? s : gmol);
+}
+
+/* Werner-Wengle constants (init_wernerwengle, src/flow/wernerwengle_mod.F90:
+ * 13-29): ww[0..4] = cwb, cpo4, cpo5, cpo6, cpo8 */
+void og_ww_constants(double rho, double gmol, double ww[5])
+{
+    const double cwa = 8.3, cwb = 1.0 / 7.0;
+    const double cpo1 = 1.0 - cwb, cpo2 = 1.0 + cwb;
+    ww[0] = cwb;
+    ww[1] = cpo2 / cwa * pow(gmol / rho, cwb);
+    ww[2] = 0.5 * cpo1 * pow(cwa, cpo2 / cpo1) * pow(gmol / rho, cpo2);
+    ww[3] = 0.5 * gmol / rho * pow(cwa, 2.0 / cpo1);
+    ww[4] = 2.0 / cpo2;
+}
+
+/* swcle3d_one (src/flow/tstle4_mod.F90:1848-1859) with tauwin
+ * (src/flow/wernerwengle_mod.F90:64-87): the wall shear stress of the
+ * velocity uq at wall distance dd, over rho dd */
+double og_swc(double dd, double uq, double rho, double gmol, const double ww[5])
+{
+    const double vz = copysign(1.0, uq), un = fabs(uq);
+    double tau;
+    if (un >= ww[3] / dd) {            /* fully turbulent */
+        const double ddsb = pow(dd, -ww[0]);
+        tau = vz * rho * pow(ddsb * (ww[1] * un + ww[2] / dd), ww[4]);
+    } else {                           /* laminar sublayer */
+        tau = vz * 2.0 * gmol * un / dd;
+    }
+    return tau / rho / dd;
+}
+
+/* The swcle3d terms that velocity component comp (0 u, 1 v, 2 w) at
+ * (k, j, i) receives from the NOS walls of its grid, in the reference's
+ * face order (front, back, right, left, bottom, top; each face loop runs
+ * over 2..n-1 of the two tangential indices). */
+static double swc_terms(double val, int comp, int k, int j, int i, const int32_t *bc,
+                        int kk, int jj, int ii, double vel, const double *ddx,
+                        const double *ddy, const double *ddz, double rho, double gmol,
+                        const double ww[5])
+{
+    const int tk = k >= 2 && k <= kk - 1, tj = j >= 2 && j <= jj - 1, ti = i >= 2 && i <= ii - 1;
+    if (comp != 0) {                   /* v and w on x walls */
+        if (bc[0] == 5 && i == 3 && tj && tk) val = val - og_swc(ddx[i], vel, rho, gmol, ww);
+        if (bc[1] == 5 && i == ii - 2 && tj && tk) val = val - og_swc(ddx[i], vel, rho, gmol, ww);
+    }
+    if (comp != 1) {                   /* u and w on y walls */
+        if (bc[2] == 5 && j == 3 && ti && tk) val = val - og_swc(ddy[j], vel, rho, gmol, ww);
+        if (bc[3] == 5 && j == jj - 2 && ti && tk) val = val - og_swc(ddy[j], vel, rho, gmol, ww);
+    }
+    if (comp != 2) {                   /* u and v on z walls */
+        if (bc[4] == 5 && k == 3 && ti && tj) val = val - og_swc(ddz[k], vel, rho, gmol, ww);
+        if (bc[5] == 5 && k == kk - 2 && ti && tj) val = val - og_swc(ddz[k], vel, rho, gmol, ww);
+    }
+    return val;
 }
 
 int og_tstle4(const og_topo *t, double *uo_, double *vo_, double *wo_,
@@ -1536,11 +1590,11 @@ int og_tstle4(const og_topo *t, double *uo_, double *vo_, double *wo_,
               const double *rddx_, const double *rddy_, const double *rddz_)
 {
     for (int g = 1; g <= t->ngrid; ++g)
-        for (int f = 0; f < 6; ++f) {
-            const int c = t->bc[6 * (g - 1) + f];
-            if (c == 5 || c == 8) return -1;
-        }
+        for (int f = 0; f < 6; ++f)
+            if (t->bc[6 * (g - 1) + f] == 8) return -1;
     const double iles = ilesmodel == 0 ? 0.0 : 1.0;
+    double ww[5];
+    og_ww_constants(rho, gmol, ww);
     for (int g = 1; g <= t->ngrid; ++g) {
         GDIMS(t, g);
         const int64_t o3 = t->ip3[g - 1];
@@ -1582,6 +1636,7 @@ int og_tstle4(const og_topo *t, double *uo_, double *vo_, double *wo_,
                     double qt = 0.5 * ft * (u[c] + u[IX(k + 1, j, i)]);
                     double qb = 0.5 * fb * (u[IX(k - 1, j, i)] + u[c]);
                     uo[c] = -((qe - qw + qn - qs + qt - qb) * rdx[i] * rddy[j] * rddz[k]);
+                    uo[c] = swc_terms(uo[c], 0, k, j, i, bc, kk, jj, ii, u[c], ddx, ddy, ddz, rho, gmol, ww);
                     /* diffusion */
                     const double ge = G(k, j, i + 1), gw = G(k, j, i);
                     const double gn = gpair(G(k, j, i), G(k, j + 1, i), gmol) + gpair(G(k, j, i + 1), G(k, j + 1, i + 1), gmol);
@@ -1624,6 +1679,7 @@ int og_tstle4(const og_topo *t, double *uo_, double *vo_, double *wo_,
                     double qt = 0.5 * ft * (v[c] + v[IX(k + 1, j, i)]);
                     double qb = 0.5 * fb * (v[IX(k - 1, j, i)] + v[c]);
                     vo[c] = -((qe - qw + qn - qs + qt - qb) * rddx[i] * rdy[j] * rddz[k]);
+                    vo[c] = swc_terms(vo[c], 1, k, j, i, bc, kk, jj, ii, v[c], ddx, ddy, ddz, rho, gmol, ww);
                     const double ge = gpair(G(k, j, i), G(k, j, i + 1), gmol) + gpair(G(k, j + 1, i), G(k, j + 1, i + 1), gmol);
                     const double gw = gpair(G(k, j, i - 1), G(k, j, i), gmol) + gpair(G(k, j + 1, i - 1), G(k, j + 1, i), gmol);
                     const double gn = G(k, j + 1, i), gs = G(k, j, i);
@@ -1665,6 +1721,7 @@ int og_tstle4(const og_topo *t, double *uo_, double *vo_, double *wo_,
                     double qt = 0.5 * ft * (w[c] + w[IX(k + 1, j, i)]);
                     double qb = 0.5 * fb * (w[IX(k - 1, j, i)] + w[c]);
                     wo[c] = -((qe - qw + qn - qs + qt - qb) * rddx[i] * rddy[j] * rdz[k]);
+                    wo[c] = swc_terms(wo[c], 2, k, j, i, bc, kk, jj, ii, w[c], ddx, ddy, ddz, rho, gmol, ww);
                     const double ge = gpair(G(k, j, i), G(k, j, i + 1), gmol) + gpair(G(k + 1, j, i), G(k + 1, j, i + 1), gmol);
                     const double gw = gpair(G(k, j, i - 1), G(k, j, i), gmol) + gpair(G(k + 1, j, i - 1), G(k + 1, j, i), gmol);
                     const double gn = gpair(G(k, j, i), G(k, j + 1, i), gmol) + gpair(G(k + 1, j, i), G(k + 1, j + 1, i), gmol);
@@ -1688,6 +1745,21 @@ int og_tstle4(const og_topo *t, double *uo_, double *vo_, double *wo_,
                     wo[c] = wo[c] - 1.0 / (rho * dz[k]) * (p[IX(k + 1, j, i)] - p[c] + gpz * dz[k]);
                 }
 #undef G
+        /* swcle3d at the positions outside the loop ranges above (its face
+         * loops include the ghost rows 2 and n-1), where uo is still zero */
+        for (int i = 1; i <= ii; ++i)
+            for (int j = 1; j <= jj; ++j)
+                for (int k = 1; k <= kk; ++k) {
+                    const int64_t c = IX(k, j, i);
+                    const int ik = k >= 3 && k <= kk - 2, ij = j >= 3 && j <= jj - 2,
+                              iI = i >= 3 && i <= ii - 2;
+                    if (!(i >= 3 - nfu && i <= ii - 3 + nbu && ij && ik))
+                        uo[c] = swc_terms(uo[c], 0, k, j, i, bc, kk, jj, ii, u[c], ddx, ddy, ddz, rho, gmol, ww);
+                    if (!(iI && j >= 3 - nrv && j <= jj - 3 + nlv && ik))
+                        vo[c] = swc_terms(vo[c], 1, k, j, i, bc, kk, jj, ii, v[c], ddx, ddy, ddz, rho, gmol, ww);
+                    if (!(iI && ij && k >= 3 - nbw && k <= kk - 3 + ntw))
+                        wo[c] = swc_terms(wo[c], 2, k, j, i, bc, kk, jj, ii, w[c], ddx, ddy, ddz, rho, gmol, ww);
+                }
     }
     return 0;
 }

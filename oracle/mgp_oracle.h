@@ -196,11 +196,17 @@ int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
                og_stats *st);
 
 /* Momentum right-hand side of one RK stage without immersed boundaries
- * (src/flow/tstle4_mod.F90:13-529: convection, diffusion with viscosity
- * field g, pressure gradient with the optional body-force gradient gradp[3]
- * on every grid).  uo, vo, wo are zeroed and then filled.  Returns -1 when a
- * grid has a NOS wall (5: Werner-Wengle wall stress, swcle3d) or a PAR face
+ * (src/flow/tstle4_mod.F90:13-529: convection, Werner-Wengle wall stress
+ * next to NOS walls, diffusion with viscosity field g, pressure gradient
+ * with the optional body-force gradient gradp[3] on every grid).  uo, vo,
+ * wo are zeroed and then filled.  Returns -1 when a grid has a PAR face
  * (8: tstle4_par), which this restatement omits. */
+/* Werner-Wengle constants cwb, cpo4, cpo5, cpo6, cpo8
+ * (src/flow/wernerwengle_mod.F90:13-29) and the wall term swcle3d_one
+ * (src/flow/tstle4_mod.F90:1848-1859, tauwin :64-87 of wernerwengle_mod). */
+void og_ww_constants(double rho, double gmol, double ww[5]);
+double og_swc(double dd, double uq, double rho, double gmol, const double ww[5]);
+
 int og_tstle4(const og_topo *t, double *uo, double *vo, double *wo,
               const double *u, const double *v, const double *w,
               const double *p, const double *g, double rho, double gmol,

@@ -821,3 +821,75 @@ def test_sdiv_source_cancelling_the_divergence_is_a_no_op(oracle_mod):
     ref = [a.copy() for a in ins]
     o.mgpoisl(st, *ref, 1e-3, ninner=5, nouter=2, ibtype=1)
     assert np.abs(ref[3] - ins[3])[inner].max() > 0
+
+
+# ---------------------------------------------------------------- wall stress
+@pytest.mark.parametrize("rho,gmol", [(1.0, 1e-2), (1.2, 1.8e-5), (1000.0, 1e-3)])
+@pytest.mark.parametrize("dd", [1e-3, 0.01, 0.37])
+def test_wall_law_continuous_at_regime_switch(oracle_mod, rho, gmol, dd):
+    """The Werner-Wengle wall stress (tauwin, src/flow/wernerwengle_mod.F90:
+    64-87) switches from the laminar to the power-law form at
+    |u| = cpo6/dd; the two forms meet there only with the constants of
+    init_wernerwengle (:13-29), so a wrong constant shows as a jump."""
+    ww = oracle_mod.ww_constants(rho, gmol)
+    us = ww[3] / dd
+    lo = oracle_mod.wall_term(dd, us * (1 - 1e-12), rho, gmol)
+    hi = oracle_mod.wall_term(dd, us * (1 + 1e-12), rho, gmol)
+    assert abs(hi - lo) <= 1e-10 * abs(lo)
+    # laminar sublayer: 2 nu u / dd^2, odd in u
+    u = 0.25 * us
+    assert np.isclose(oracle_mod.wall_term(dd, u, rho, gmol), 2 * gmol / rho * u / dd**2,
+                      rtol=1e-14, atol=0)
+    assert oracle_mod.wall_term(dd, -u, rho, gmol) == -oracle_mod.wall_term(dd, u, rho, gmol)
+    # far in the turbulent regime tau grows as |u|^(2/(1+1/7)) = |u|^1.75
+    big = oracle_mod.wall_term(dd, 1e8 * us, rho, gmol)
+    assert abs(oracle_mod.wall_term(dd, 2e8 * us, rho, gmol) / big - 2**1.75) < 1e-7
+
+
+def test_tstle4_nos_wall_terms(oracle_mod):
+    """NOS walls (swcle3d, src/flow/tstle4_mod.F90:1756-1845) change tstle4
+    only by the wall stress of the tangential components in the first cell
+    next to each wall (including its ghost rows 2 and n-1), each wall
+    subtracting wall_term(wall distance, velocity) once: NOS minus FIX walls
+    (same loops, no wall stress)."""
+    from mgp.grids import FIX, NOS, uniform_box
+    gn = uniform_box((2, 2, 2), (6, 8, 6), periodic=(False, False, False), wall=NOS)
+    gf = uniform_box((2, 2, 2), (6, 8, 6), periodic=(False, False, False), wall=FIX)
+    on, of = oracle_mod.Oracle(gn), oracle_mod.Oracle(gf)
+    rng = np.random.default_rng(41)
+    u, v, w = (rng.uniform(-3, 3, gn.n3d) for _ in range(3))
+    p = rng.standard_normal(gn.n3d)
+    visc = rng.uniform(1e-3, 2e-2, gn.n3d)
+    rho, gmol = 1.2, 1e-3
+    rn = on.tstle4(u, v, w, p, visc, rho, gmol)
+    rf = of.tstle4(u, v, w, p, visc, rho, gmol)
+    hits = 0
+    for comp, (a, b, vel) in enumerate(zip(rn, rf, (u, v, w))):
+        for blk in range(1, gn.ngrid + 1):
+            an, af, vv = gn.block_view(a, blk), gn.block_view(b, blk), gn.block_view(vel, blk)
+            ii, jj, kk = an.shape
+            bc = gn.bc[blk - 1]
+            expect = np.zeros_like(an)
+            dx = gn.ddx[gn.ip1x[blk - 1]:gn.ip1x[blk - 1] + ii]
+            dy = gn.ddy[gn.ip1y[blk - 1]:gn.ip1y[blk - 1] + jj]
+            dz = gn.ddz[gn.ip1z[blk - 1]:gn.ip1z[blk - 1] + kk]
+            for ax, (lo_f, hi_f) in enumerate(((0, 1), (2, 3), (4, 5))):
+                if ax == comp:
+                    continue        # the wall-normal component has no wall stress
+                for f, plane in ((lo_f, 2), (hi_f, an.shape[ax] - 3)):
+                    if bc[f] != NOS:
+                        continue
+                    sl = [slice(1, n - 1) for n in an.shape]
+                    sl[ax] = plane
+                    dd = (dx, dy, dz)[ax][plane]
+                    for idx in zip(*np.nonzero(np.ones_like(an[tuple(sl)], dtype=bool))):
+                        full = list(idx)
+                        full.insert(ax, None)
+                        pos = tuple(plane if q is None else s.start + q
+                                    for q, s in zip(full, sl))
+                        expect[pos] -= oracle_mod.wall_term(dd, vv[pos], rho, gmol)
+                        hits += 1
+            diff = an - af
+            assert np.allclose(diff, expect, rtol=1e-12, atol=1e-12 * np.abs(expect).max()), \
+                (comp, blk)
+    assert hits > 0
