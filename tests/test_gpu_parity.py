@@ -345,3 +345,26 @@ def test_shared_factors_change_nothing(oracle_mod, monkeypatch, precision):
     assert np.array_equal(ha, hb)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def test_config2_sor_omega_17_bitwise(oracle_mod):
+    """The 8x8x8-block case (192^3) with SOR at omega = 1.7 on every level:
+    the reference's block-lagged red-black SOR diverges there (the residual
+    grows about 5x per outer iteration); the GPU follows the oracle bit for
+    bit while it does."""
+    g = topology("c2")
+    o = oracle_mod.Oracle(g)
+    st = o.setup(ityp=1)
+    u, v, w = velocity(g, o, 7)
+    ins = [a.copy() for a in (u, v, w)] + [np.zeros(g.n3d)]
+    ref = [a.copy() for a in ins]
+    so = o.mgpoisl(st, *ref, 1e-3, ityp=1, ninner=5, nouter=4, omega=1.7, sor_all_levels=1)
+    s = _solver(g, type="sor", ninner=5, nouter=4, omega=1.7, sor_all_levels=1)
+    got = [a.copy() for a in ins]
+    sg = s.mgpoisl(*got, 1e-3)
+    h = s.history()
+    print(f"[config2 sor 1.7] max|RHS| {so['maxrhs0']:.4e} -> {h}", flush=True)
+    assert np.array_equal(h, so["history"]) and sg["maxrhs"] == so["maxrhs"]
+    for a, b, nm in zip(ref, got, "uvwp"):
+        assert np.array_equal(a, b), nm
+    assert h[-1] > 10 * h[0]

@@ -893,3 +893,22 @@ def test_tstle4_nos_wall_terms(oracle_mod):
             assert np.allclose(diff, expect, rtol=1e-12, atol=1e-12 * np.abs(expect).max()), \
                 (comp, blk)
     assert hits > 0
+
+
+def test_sor_lagged_ghosts_diverge_at_high_omega(oracle_mod):
+    """sor relaxes every grid completely (both colours) before conn exchanges
+    the ghosts (src/flow/pressuresolver_mod.F90:403-414, sor_mod.F90:52-101),
+    so the blocks see their neighbours' values of the previous relaxation.
+    On a periodic 2x2x2-block box this block-lagged red-black SOR converges
+    at omega = 1.2 and diverges at omega = 1.7."""
+    g = topology("box2")
+    o = oracle_mod.Oracle(g)
+    st = o.setup(ityp=1)
+    u, v, w = velocity(g, o, 7)
+    hist = {}
+    for om in (1.2, 1.7):
+        f = [a.copy() for a in (u, v, w)] + [np.zeros(g.n3d)]
+        hist[om] = o.mgpoisl(st, *f, 1e-3, ityp=1, ninner=5, nouter=20, omega=om,
+                             sor_all_levels=1)["history"]
+    assert hist[1.2][-1] < 1e-3 * hist[1.2][0]
+    assert hist[1.7][-1] > 1e6 * hist[1.7][0]
