@@ -501,6 +501,10 @@ def main():
         "forms": {f: {"gcell_updates_per_s": round(cells * args.ninner * args.steps
                                                    / res[f]["wall"] / 1e9, 3),
                       "ms_per_step": round(res[f]["wall"] / args.steps * 1e3, 3),
+                      # the outer-iteration loop alone (device time of the
+                      # solve, without the per-call divergence, correction
+                      # and exchanges)
+                      "loop_ms_per_step": round(res[f]["stats"]["solve_ms"] / args.steps, 3),
                       "kernel_ms": {k: round(v, 4) for k, v in res[f]["kernels"].items()},
                       "device_gb": round(res[f]["info"]["device_bytes"] / 1e9, 2),
                       "factor_classes": res[f]["factor_classes"][0],
