@@ -418,7 +418,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (every block of the projection "
-              "lives on one MI355X); multi-GPU sharding is described in DESIGN.md",
+              "lives on one MI355X)",
               file=sys.stderr)
         sys.exit(2)
 
