@@ -79,6 +79,10 @@ typedef struct mgp_config {
                           /*   iteration 0 instead of the epcorr test       */
     mgletint sor_all_levels; /* 1: "sor" also relaxes the coarsest level    */
     mgletint check_every; /* outer iterations per captured graph (>= 1)     */
+    mgletreal alfa;       /* SIP alfa, 0 <= alfa <= 1; 0 selects the        */
+                          /*   reference's hard-coded default-real 0.92,    */
+                          /*   (double)0.92f (src/flow/sip_hyperplane_mod.  */
+                          /*   F90:123, 202); a negative value gives 0      */
 } mgp_config;
 
 typedef struct mgp_stats {
@@ -171,6 +175,13 @@ int mgp_set_stream(mgp_handle *h, void *stream);
  * accumulated injected values elsewhere).  1-D fields
  * "GSAW", "GSAE", "GSAS", "GSAN", "GSAB", "GSAT" use the 1-D layout. */
 int mgp_get_field(mgp_handle *h, const char *name, mgletreal *out);
+
+/* The same for one grid (igrid 1-based; 0 = every grid, as mgp_get_field):
+ * out gets the kk x jj x ii values of the grid, the block the reference's
+ * field_t%get_ptr(arr, igrid) points at (src/core/field_t/realfield_mod.F90:
+ * 113-135).  3-D fields only. */
+int mgp_get_field_grid(mgp_handle *h, const char *name, mgletint igrid,
+                       mgletreal *out);
 
 /* Ghost-layer exchange of device fields in the reference layout
  * (conn / connect: src/core/connect/conn1_mod.F90:31-129,

@@ -789,6 +789,7 @@ int check_cfg(const mgp_config &c) {
     if (c.precision < 0 || c.precision > 1) return fail(MGP_ERR_ARG, "invalid precision %d", c.precision);
     if (c.ibtype < 0 || c.ibtype > 1) return fail(MGP_ERR_ARG, "invalid ibtype %d", c.ibtype);
     if (!(c.rho > 0.0)) return fail(MGP_ERR_ARG, "invalid rho %g", c.rho);
+    if (!(c.alfa <= 1.0)) return fail(MGP_ERR_ARG, "invalid SIP alfa %g", c.alfa);
     return MGP_OK;
 }
 
@@ -801,7 +802,7 @@ void setup_factors(mgp_handle &h) {
     launch_lu<T>(t, h.d_ap64.as<double>(), h.d_c64[0].as<double>(),
                  h.d_c64[1].as<double>(), h.d_c64[2].as<double>(),
                  h.d_c64[3].as<double>(), h.d_c64[4].as<double>(),
-                 h.d_c64[5].as<double>(), h.mask(), f, h.stream);
+                 h.d_c64[5].as<double>(), h.mask(), h.cfg.alfa, f, h.stream);
     HC(hipGetLastError());
 }
 
@@ -902,6 +903,9 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
     auto h = std::make_unique<mgp_handle>();
     h->cfg = *cfg;
     if (h->cfg.check_every < 1) h->cfg.check_every = 1;
+    // alfa 0 is the reference's default-real literal 0.92 in an FP64 build
+    if (h->cfg.alfa == 0.0) h->cfg.alfa = (double)0.92f;
+    else if (h->cfg.alfa < 0.0) h->cfg.alfa = 0.0;
     h->ngrid = G->ngrid;
     h->minlevel = G->minlevel;
     h->maxlevel = G->maxlevel;
@@ -1778,13 +1782,19 @@ int mgp_set_stream(mgp_handle *h, void *stream) {
 }
 
 int mgp_get_field(mgp_handle *h, const char *name, mgletreal *out) {
+    return mgp_get_field_grid(h, name, 0, out);
+}
+
+int mgp_get_field_grid(mgp_handle *h, const char *name, mgletint igrid, mgletreal *out) {
     if (!h || !name || !out) return fail(MGP_ERR_ARG, "null argument");
+    if (igrid < 0 || igrid > h->ngrid) return fail(MGP_ERR_ARG, "invalid grid %d", igrid);
     const std::string nm(name);
     try {
         const hipStream_t s = h->stream;
         const char *c1[6] = {"GSAW", "GSAE", "GSAS", "GSAN", "GSAB", "GSAT"};
         for (int q = 0; q < 6; ++q)
             if (nm == c1[q]) {
+                if (igrid > 0) return fail(MGP_ERR_ARG, "1-D fields are copied whole");
                 HC(hipMemcpyAsync(out, h->d_c64[q].p, h->d_c64[q].n, hipMemcpyDeviceToHost, s));
                 HC(hipStreamSynchronize(s));
                 return MGP_OK;
@@ -1824,7 +1834,13 @@ int mgp_get_field(mgp_handle *h, const char *name, mgletreal *out) {
             found = true;
         }
         if (!found) return fail(MGP_ERR_ARG, "unknown field %s", name);
-        HC(hipMemcpyAsync(out, tmp.p, (size_t)h->n3d * 8, hipMemcpyDeviceToHost, s));
+        if (igrid > 0) {
+            const int b = igrid - 1;
+            const size_t n = (size_t)h->kk[b] * h->jj[b] * h->ii[b];
+            HC(hipMemcpyAsync(out, tmp.as<double>() + h->ip3[b], n * 8, hipMemcpyDeviceToHost, s));
+        } else {
+            HC(hipMemcpyAsync(out, tmp.p, (size_t)h->n3d * 8, hipMemcpyDeviceToHost, s));
+        }
         HC(hipStreamSynchronize(s));
     } catch (const HipError &e) {
         return fail(MGP_ERR_HIP, "%s", e.msg.c_str());

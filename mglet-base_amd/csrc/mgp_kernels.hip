@@ -17,7 +17,6 @@
 
 namespace mgp {
 
-#define ALFA ((double)0.92f)
 #define TINY ((double)1.0e-20f)
 
 constexpr int kThreads = 256;
@@ -157,8 +156,8 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void k_lu(
     Topo t, const double *ap, const double *aw_, const double *ae_,
     const double *as_, const double *an_, const double *ab_,
-    const double *at_, Mask m, T *lw_o, T *ls_o, T *lb_o, T *lpr_o, T *ue_o,
-    T *un_o, T *ut_o) {
+    const double *at_, Mask m, double ALFA, T *lw_o, T *ls_o, T *lb_o,
+    T *lpr_o, T *ue_o, T *un_o, T *ut_o) {
     extern __shared__ double lds_lu[];
     const int b = t.blist[blockIdx.x];
     const Block &B = t.blocks[b];
@@ -234,21 +233,21 @@ __global__ __launch_bounds__(kThreads) void k_lu(
 template <typename T>
 void launch_lu(const Topo &t, const double *ap, const double *aw,
                const double *ae, const double *as, const double *an,
-               const double *ab, const double *at, Mask m, T *const f[7],
-               hipStream_t s) {
+               const double *ab, const double *at, Mask m, double alfa,
+               T *const f[7], hipStream_t s) {
     const size_t lds = 6 * (size_t)(t.max_nj + 1) * (t.max_ni + 1) *
                        sizeof(double);
     k_lu<T><<<t.nlist, kThreads, lds, s>>>(t, ap, aw, ae, as, an, ab, at, m,
-                                           f[0], f[1], f[2], f[3], f[4], f[5],
+                                           alfa, f[0], f[1], f[2], f[3], f[4], f[5],
                                            f[6]);
 }
 template void launch_lu<double>(const Topo &, const double *, const double *,
                                 const double *, const double *, const double *,
-                                const double *, const double *, Mask,
+                                const double *, const double *, Mask, double,
                                 double *const[7], hipStream_t);
 template void launch_lu<float>(const Topo &, const double *, const double *,
                                const double *, const double *, const double *,
-                               const double *, const double *, Mask,
+                               const double *, const double *, Mask, double,
                                float *const[7], hipStream_t);
 
 // ---------------------------------------------------------------------
@@ -470,7 +469,7 @@ __device__ void block_max_to_global(double m, unsigned long long *g) {
     __shared__ double red[kThreads / 64];
     for (int off = 32; off > 0; off >>= 1) {
         const double o = __shfl_down(m, off, 64);
-        m = (o > m || m != m) ? o : m;
+        m = (o > m || o != o) ? o : m;   // a NaN from either side stays
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) red[wv] = m;
@@ -478,7 +477,7 @@ __device__ void block_max_to_global(double m, unsigned long long *g) {
     if (threadIdx.x == 0) {
         double r = red[0];
         for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
-            r = (red[q] > r || r != r) ? red[q] : r;
+            r = (red[q] > r || red[q] != red[q]) ? red[q] : r;
         atomicMax(g, (unsigned long long)__double_as_longlong(r));
     }
 }

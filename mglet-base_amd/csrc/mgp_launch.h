@@ -82,8 +82,8 @@ void launch_ap(const Topo &t, const double *dx, const double *dy,
 template <typename T>
 void launch_lu(const Topo &t, const double *ap, const double *aw,
                const double *ae, const double *as, const double *an,
-               const double *ab, const double *at, Mask m, T *const f[7],
-               hipStream_t s);
+               const double *ab, const double *at, Mask m, double alfa,
+               T *const f[7], hipStream_t s);
 
 // shared factor storage: per-block hash of the factors and GSAP at
 // Block::wo, exact comparison with the representative rep[b], copy of the

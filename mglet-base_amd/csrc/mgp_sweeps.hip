@@ -564,7 +564,7 @@ __global__ __launch_bounds__(kWave * NW) void k_outer(
     double mx = k.mx * ldexp(1.0, t.maxlevel - B.level);
     for (int off = 32; off > 0; off >>= 1) {
         const double x = __shfl_down(mx, off, kWave);
-        mx = (x > mx || mx != mx) ? x : mx;
+        mx = (x > mx || x != x) ? x : mx;   // a NaN from either side stays
     }
     if ((threadIdx.x & (kWave - 1)) == 0)   // one per wave
         atomicMax(o.gmax, (unsigned long long)__double_as_longlong(mx));

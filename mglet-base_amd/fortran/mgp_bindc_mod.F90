@@ -27,11 +27,16 @@ MODULE mgp_bindc_mod
     ! src/flow/pressuresolver_mod.F90:57-142)
     TYPE, BIND(C), PUBLIC :: mgp_config_t
         INTEGER(C_INT) :: ityp = 0, ninner = 5, nouter = 9, nouter_min = 0
-        REAL(C_DOUBLE) :: epcorr = 0.0_C_DOUBLE, omega = 1.1_C_DOUBLE
+        ! omega: the reference's default_value=1.1 is a default-real literal
+        ! converted with REAL(..., real64) (src/flow/pressuresolver_mod.F90:135)
+        REAL(C_DOUBLE) :: epcorr = 0.0_C_DOUBLE
+        REAL(C_DOUBLE) :: omega = REAL(1.1, C_DOUBLE)
         REAL(C_DOUBLE) :: rho = 1.0_C_DOUBLE
         INTEGER(C_INT) :: ibtype = 0, precision = 0
         REAL(C_DOUBLE) :: eps_rel = 0.0_C_DOUBLE
         INTEGER(C_INT) :: sor_all_levels = 0, check_every = 4
+        ! SIP alfa; 0 = the reference's hard-coded default-real 0.92
+        REAL(C_DOUBLE) :: alfa = 0.0_C_DOUBLE
     END TYPE mgp_config_t
 
     TYPE, BIND(C), PUBLIC :: mgp_stats_t

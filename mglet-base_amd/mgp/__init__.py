@@ -30,10 +30,39 @@ def _ptr(a, ct=C.c_double):
     return a.ctypes.data_as(C.POINTER(ct))
 
 
+# The reference reads omega with default_value=1.1, a default-real (real32)
+# literal that config_mod widens with REAL(default_value, real64)
+# (src/flow/pressuresolver_mod.F90:135): the FP64 default is (double)1.1f.
+OMEGA_DEFAULT = float(np.float32(1.1))
+
+
+def _check_host(arrays, n):
+    for a in arrays:
+        if (not isinstance(a, np.ndarray) or a.dtype != np.float64
+                or not a.flags.c_contiguous or a.size != n):
+            raise MgpError("fields must be contiguous float64 arrays of the 3-D length")
+
+
+def _check_device(tensors, n):
+    """Every tensor handed to a device entry point must be a contiguous
+    CUDA float64 tensor of the 3-D length: the kernels index n values from
+    the raw pointer, so anything else would be an out-of-bounds or host
+    access on the GPU.  Raises before any library call."""
+    try:
+        import torch
+    except ImportError as e:  # pragma: no cover - torch is part of the image
+        raise MgpError("device fields need torch") from e
+    for a in tensors:
+        if (not isinstance(a, torch.Tensor) or a.dtype != torch.float64 or not a.is_cuda
+                or not a.is_contiguous() or a.numel() != n):
+            raise MgpError("fields must be contiguous CUDA float64 tensors of the 3-D length")
+
+
 class PressureSolver:
     def __init__(self, grids: Grids, type="sip", ninner=5, nouter=9, nouter_min=0,
-                 epcorr=0.0, omega=1.1, rho=1.0, ib="noib", precision="fp64",
-                 eps_rel=0.0, check_every=4, bp=None, blocked=None, sor_all_levels=False):
+                 epcorr=0.0, omega=OMEGA_DEFAULT, rho=1.0, ib="noib", precision="fp64",
+                 eps_rel=0.0, check_every=4, bp=None, blocked=None, sor_all_levels=False,
+                 alfa=0.0):
         self.lib = _lib.load()
         self.grids = g = grids
         t = type.lower()
@@ -55,7 +84,7 @@ class PressureSolver:
                                _ptr(k[11], C.c_int32) if nested else None)
         self.cfg = MgpConfig(_TYPES[t], ninner, nouter, nouter_min, epcorr, omega, rho,
                              _IB[ib], _PREC[precision], eps_rel, int(bool(sor_all_levels)),
-                             check_every)
+                             check_every, alfa)
         h = C.c_void_p()
         if blocked is not None:
             blocked = np.ascontiguousarray(blocked, dtype=np.int8)
@@ -89,17 +118,11 @@ class PressureSolver:
         """Project (u, v, w) and update p in place; returns the stats."""
         st = MgpStats()
         if isinstance(u, np.ndarray):
-            for a in (u, v, w, p):
-                if a.dtype != np.float64 or not a.flags.c_contiguous or a.size != self.grids.n3d:
-                    raise MgpError("fields must be contiguous float64 of the 3-D length")
+            _check_host((u, v, w, p), self.grids.n3d)
             rc = self.lib.mgp_mgpoisl(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(p), dt, ittot,
                                       irk, C.byref(st))
         else:
-            import torch
-            for a in (u, v, w, p):
-                if (a.dtype != torch.float64 or not a.is_cuda or not a.is_contiguous()
-                        or a.numel() != self.grids.n3d):
-                    raise MgpError("fields must be contiguous CUDA float64 tensors")
+            _check_device((u, v, w, p), self.grids.n3d)
             self._bind_stream()
             rc = self.lib.mgp_mgpoisl_device(self.h, u.data_ptr(), v.data_ptr(), w.data_ptr(),
                                              p.data_ptr(), dt, ittot, irk, C.byref(st))
@@ -113,17 +136,11 @@ class PressureSolver:
         adjoint Poisson problem uses this solver's iteration settings."""
         st = MgpStats()
         if isinstance(ubar, np.ndarray):
-            for a in (ubar, vbar, wbar):
-                if a.dtype != np.float64 or not a.flags.c_contiguous or a.size != self.grids.n3d:
-                    raise MgpError("fields must be contiguous float64 of the 3-D length")
+            _check_host((ubar, vbar, wbar), self.grids.n3d)
             rc = self.lib.mgp_adjoint(self.h, _ptr(ubar), _ptr(vbar), _ptr(wbar), dt,
                                       C.byref(st))
         else:
-            import torch
-            for a in (ubar, vbar, wbar):
-                if (a.dtype != torch.float64 or not a.is_cuda or not a.is_contiguous()
-                        or a.numel() != self.grids.n3d):
-                    raise MgpError("fields must be contiguous CUDA float64 tensors")
+            _check_device((ubar, vbar, wbar), self.grids.n3d)
             self._bind_stream()
             rc = self.lib.mgp_adjoint_device(self.h, ubar.data_ptr(), vbar.data_ptr(),
                                              wbar.data_ptr(), dt, C.byref(st))
@@ -137,8 +154,18 @@ class PressureSolver:
         import torch
         torch.cuda.current_stream().synchronize()
 
-    def get_field(self, name):
+    def get_field(self, name, igrid=None):
+        """A library field in the reference layout; with igrid (1-based)
+        only that grid's kk x jj x ii block of a 3-D field."""
         g = self.grids
+        if igrid is not None:
+            if not 1 <= igrid <= g.ngrid:
+                raise MgpError(f"invalid grid {igrid}")
+            b = igrid - 1
+            out = np.zeros(int(g.kk[b]) * int(g.jj[b]) * int(g.ii[b]))
+            _lib.check(self.lib.mgp_get_field_grid(self.h, name.encode(), igrid, _ptr(out)),
+                       "mgp_get_field_grid")
+            return out
         n = {"GSAW": g.n1x, "GSAE": g.n1x, "GSAS": g.n1y, "GSAN": g.n1y,
              "GSAB": g.n1z, "GSAT": g.n1z}.get(name, g.n3d)
         out = np.zeros(n)
@@ -147,6 +174,7 @@ class PressureSolver:
 
     def connect(self, fields, level=-1, layers=1, corners=False, forward=0):
         """Ghost-layer exchange of CUDA float64 tensors (reference layout)."""
+        _check_device(fields, self.grids.n3d)
         self._bind_stream()
         arr = (C.c_void_p * len(fields))(*[f.data_ptr() for f in fields])
         _lib.check(self.lib.mgp_connect_device(self.h, level, layers, int(corners), forward,
@@ -154,6 +182,7 @@ class PressureSolver:
 
     def divcal(self, u, v, w, fak=1.0):
         import torch
+        _check_device((u, v, w), self.grids.n3d)
         self._bind_stream()
         div = torch.zeros_like(u)
         _lib.check(self.lib.mgp_divcal_device(self.h, div.data_ptr(), u.data_ptr(),
@@ -176,9 +205,15 @@ class PressureSolver:
         """Momentum right-hand side of one RK stage (tstle4, no immersed
         boundary) on CUDA float64 tensors; returns (uo, vo, wo)."""
         import torch
+        _check_device((u, v, w, p, g), self.grids.n3d)
+        gp = None
+        if gradp is not None:
+            gradp = np.ascontiguousarray(gradp, dtype=np.float64)
+            if gradp.size != 3:
+                raise MgpError("gradp must hold three values")
+            gp = _ptr(gradp)
         self._bind_stream()
         uo, vo, wo = (torch.empty_like(u) for _ in range(3))
-        gp = None if gradp is None else _ptr(np.asarray(gradp, dtype=np.float64))
         _lib.check(self.lib.mgp_tstle4_device(self.h, uo.data_ptr(), vo.data_ptr(),
                                               wo.data_ptr(), u.data_ptr(), v.data_ptr(),
                                               w.data_ptr(), p.data_ptr(), g.data_ptr(),
@@ -189,6 +224,7 @@ class PressureSolver:
     def rkstep(self, p, dp, rhsp, frhs, dtfu):
         """2N-storage Runge-Kutta update in place: dp = frhs dp + rhsp,
         p += dtfu dp (CUDA float64 tensors of the 3-D length)."""
+        _check_device((p, dp, rhsp), self.grids.n3d)
         self._bind_stream()
         _lib.check(self.lib.mgp_rkstep_device(self.h, p.data_ptr(), dp.data_ptr(),
                                               rhsp.data_ptr(), frhs, dtfu),

@@ -31,7 +31,7 @@ class MgpConfig(C.Structure):
                 ("nouter_min", C.c_int32), ("epcorr", C.c_double), ("omega", C.c_double),
                 ("rho", C.c_double), ("ibtype", C.c_int32), ("precision", C.c_int32),
                 ("eps_rel", C.c_double), ("sor_all_levels", C.c_int32),
-                ("check_every", C.c_int32)]
+                ("check_every", C.c_int32), ("alfa", C.c_double)]
 
 
 class MgpStats(C.Structure):
@@ -82,6 +82,7 @@ def load():
                                C.c_int32],
         "mgp_set_stream": [vp, vp],
         "mgp_get_field": [vp, C.c_char_p, _dp],
+        "mgp_get_field_grid": [vp, C.c_char_p, C.c_int32, _dp],
         "mgp_connect_device": [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                C.POINTER(vp), C.c_int32],
         "mgp_divcal_device": [vp, vp, vp, vp, vp, C.c_double],

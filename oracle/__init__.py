@@ -19,6 +19,10 @@ _HERE = Path(__file__).resolve().parent
 _LIB_PATH = _HERE / "_build" / "libmgp_oracle.so"
 _lib = None
 
+# The reference's default omega: default_value=1.1 is a real32 literal
+# widened to real64 (src/flow/pressuresolver_mod.F90:135).
+OMEGA_DEFAULT = float(np.float32(1.1))
+
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
@@ -240,7 +244,7 @@ class Oracle:
         return dict(bp=bp, coef=c, fac=f, mip=mip, idx=idx, rap=rap)
 
     def mgpoisl(self, st, u, v, w, p, dt, ityp=0, ninner=5, nouter=9, nouter_min=0,
-                epcorr=0.0, eps_rel=0.0, omega=1.1, rho=1.0, ibtype=0,
+                epcorr=0.0, eps_rel=0.0, omega=OMEGA_DEFAULT, rho=1.0, ibtype=0,
                 sor_all_levels=0, sdiv=None):
         """sdiv: the ghostcell divergence source (ibtype 1), None = zero."""
         g = self.g

@@ -162,12 +162,13 @@ def velocity_levels(g, o, seed, amp=1.0, blocked=None):
     return u, v, w
 
 
-def config3(width=0.1):
+def config3(width=0.25):
     """Multi-level batch with an immersed sphere: a level-1 box of 4x4x4
     blocks of 24^3 on the unit cube (periodic), refined everywhere into
     16^3 blocks (level 2), and around the sphere of radius 0.2 at the centre
     into 32^3 blocks (level 3, one per level-2 block whose centre lies
-    within `width` of the sphere surface)."""
+    within `width` of the sphere surface).  The default width gives 2464
+    blocks (64 of 24^3, 1728 of 16^3, 672 of 32^3; 29,982,720 cells)."""
     from mgp.grids import refined_box
     shell = lambda B: abs(np.linalg.norm(B["centre"] - 0.5) - 0.2) < width
     return refined_box((4, 4, 4), 24, [(16, lambda B: True), (32, shell)])

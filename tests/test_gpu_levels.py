@@ -106,14 +106,14 @@ def test_multilevel_converged_fp32_refined(oracle_mod, masked):
 
 def test_config3_sphere_full_size(oracle_mod):
     """Multi-level batch with the immersed sphere at full size (level-1 box of
-    4x4x4 blocks of 24^3, 1728 blocks of 16^3, 200 of 32^3 around the sphere
-    of radius 0.2): FP64 bitwise equal to the oracle over two outer
-    iterations; the FP32-refined solve converged to 1e-10 against the FP64
-    one (error reported in the assertion message)."""
+    4x4x4 blocks of 24^3, 1728 blocks of 16^3, 672 of 32^3 around the sphere
+    of radius 0.2: 2464 blocks): FP64 bitwise equal to the oracle over two
+    outer iterations; the FP32-refined solve converged to 1e-10 against the
+    FP64 one (error reported in the assertion message)."""
     import mgp
     from cases import config3
     g = config3()
-    assert g.ngrid == 1992
+    assert g.ngrid == 2464
     o = oracle_mod.Oracle(g)
     bp, blocked = mask_levels(g, o, sphere_mask(g))
     st = o.setup(bp=bp)
@@ -144,3 +144,38 @@ def test_config3_sphere_full_size(oracle_mod):
           f"iteration): pressure {err_p:.3e}, velocity {err_u:.3e} (relative max)", flush=True)
     assert err_p < 1e-7 and err_u < 1e-7, (err_p, err_u, st64["nouter_done"],
                                            st32["nouter_done"])
+
+
+def test_config3_int8_mask_entry(oracle_mod):
+    """The int8 blocked-cell entry (mgp_init_mask) on the full config-3
+    sphere batch: the caller passes 1 = blocked at every stored position,
+    with the parent cells under refined grids and the ghost layers already
+    set the way the immersed-boundary setup leaves BP (a parent cell is
+    fluid when any child is, ghosts exchanged per level, PAR ghosts from
+    the parent: src/ib/gc_blockbp_mod.F90:299).  The library's BP, its
+    factors and two outer iterations equal the float-BP entry's and the
+    oracle's bit for bit."""
+    import mgp
+    from cases import config3
+    g = config3()
+    o = oracle_mod.Oracle(g)
+    bp, blocked = mask_levels(g, o, sphere_mask(g))
+    assert blocked.dtype == np.int8 and np.array_equal(1.0 - blocked, bp)
+    st = o.setup(bp=bp)
+    u, v, w = velocity_levels(g, o, 3, blocked=blocked)
+    ins = [a.copy() for a in (u, v, w, np.zeros(g.n3d))]
+    ref = [a.copy() for a in ins]
+    so = o.mgpoisl(st, *ref, 1e-3, ninner=5, nouter=2, ibtype=1)
+    s8 = mgp.PressureSolver(g, ninner=5, nouter=2, ib="ghostcell", blocked=blocked)
+    sf = mgp.PressureSolver(g, ninner=5, nouter=2, ib="ghostcell", bp=bp)
+    inner = g.interior_mask()
+    assert np.array_equal(s8.get_field("BP")[inner], bp[inner])
+    for k in ("BP", "GSAP", "SIPLPR", "SIPUT"):
+        assert np.array_equal(s8.get_field(k), sf.get_field(k)), k
+    got = [a.copy() for a in ins]
+    sg = s8.mgpoisl(*got, 1e-3)
+    _equal(so, sg, s8, ref, got)
+    gf = [a.copy() for a in ins]
+    sf.mgpoisl(*gf, 1e-3)
+    for a, b, nm in zip(gf, got, "uvwp"):
+        assert np.array_equal(a, b), nm

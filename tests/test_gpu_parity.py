@@ -368,3 +368,73 @@ def test_config2_sor_omega_17_bitwise(oracle_mod):
     for a, b, nm in zip(ref, got, "uvwp"):
         assert np.array_equal(a, b), nm
     assert h[-1] > 10 * h[0]
+
+
+def test_default_omega_sor_bitwise(oracle_mod):
+    """SOR with the default omega: the library's default is the reference's
+    (double)1.1f (src/flow/pressuresolver_mod.F90:135), and a run that names
+    no omega on either side is bitwise the oracle's."""
+    g = topology("box2")
+    o = oracle_mod.Oracle(g)
+    st = o.setup(ityp=1)
+    u, v, w = velocity(g, o, 71)
+    ins = [a.copy() for a in (u, v, w)] + [np.zeros(g.n3d)]
+    ref = [a.copy() for a in ins]
+    so = o.mgpoisl(st, *ref, 1e-3, ityp=1, ninner=4, nouter=3, sor_all_levels=1)
+    s = _solver(g, type="sor", ninner=4, nouter=3, sor_all_levels=True)
+    assert s.cfg.omega == float(np.float32(1.1))
+    got = [a.copy() for a in ins]
+    s.mgpoisl(*got, 1e-3)
+    assert np.array_equal(s.history(), so["history"])
+    for a, b, nm in zip(ref, got, "uvwp"):
+        assert np.array_equal(a, b), nm
+
+
+@pytest.mark.parametrize("alfa", [0.9, 0.5])
+def test_alfa_config_bitwise(oracle_mod, alfa):
+    """The SIP alfa of the configuration reaches the factorisation: factors
+    and a fixed-count solve equal the oracle's run at the same alfa.  (The
+    reference hard-codes (double)0.92f; other values are the build's own
+    setting, pinned here only against the oracle's restatement.)"""
+    import ctypes as C
+    g = topology("walls")
+    o = oracle_mod.Oracle(g)
+    lib = oracle_mod.lib()
+    lib.og_set_alfa(C.c_double(alfa))
+    try:
+        st = o.setup()
+        u, v, w = velocity(g, o, 72)
+        ins = [a.copy() for a in (u, v, w)] + [np.zeros(g.n3d)]
+        ref = [a.copy() for a in ins]
+        so = o.mgpoisl(st, *ref, 1e-3, ninner=5, nouter=3)
+    finally:
+        lib.og_set_alfa(C.c_double(float(np.float32(0.92))))
+    s = _solver(g, ninner=5, nouter=3, alfa=alfa)
+    for k, f in (("SIPLW", "lw"), ("SIPLPR", "lpr"), ("SIPUT", "ut")):
+        assert np.array_equal(s.get_field(k), st["fac"][f]), k
+    got = [a.copy() for a in ins]
+    s.mgpoisl(*got, 1e-3)
+    assert np.array_equal(s.history(), so["history"])
+    for a, b, nm in zip(ref, got, "uvwp"):
+        assert np.array_equal(a, b), nm
+    s0 = _solver(g, ninner=5, nouter=3)
+    assert not np.array_equal(s0.get_field("SIPLW"), st["fac"]["lw"])
+
+
+@pytest.mark.parametrize("where", ["nan", "inf"])
+def test_nonfinite_velocity_reports_error(where):
+    """A non-finite velocity makes the residual norm non-finite: mgpoisl
+    stops with MGP_ERR_NONFINITE (status 4) instead of iterating on it,
+    and the handle stays usable for the next call."""
+    import mgp
+    g = topology("box2")
+    import oracle
+    o = oracle.Oracle(g)
+    u, v, w = velocity(g, o, 73)
+    bad = u.copy()
+    bad[np.flatnonzero(g.interior_mask())[100]] = np.nan if where == "nan" else np.inf
+    s = _solver(g, ninner=3, nouter=5)
+    with pytest.raises(mgp.MgpError, match="status 4"):
+        s.mgpoisl(bad, v.copy(), w.copy(), np.zeros(g.n3d), 1e-3)
+    r = s.mgpoisl(u.copy(), v.copy(), w.copy(), np.zeros(g.n3d), 1e-3)
+    assert np.isfinite(r["maxrhs"]) and r["nouter_done"] == 5

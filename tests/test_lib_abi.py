@@ -34,7 +34,8 @@ def test_invalid_config_rejected():
     from mgp import _lib
     g = mgp.uniform_box((1, 1, 1), (4, 4, 4))
     for kw, msg in ((dict(ninner=-1), "ninner"), (dict(nouter=-1), "nouter"),
-                    (dict(type="sor", omega=2.0), "omega"), (dict(epcorr=-1.0), "epcorr")):
+                    (dict(type="sor", omega=2.0), "omega"), (dict(epcorr=-1.0), "epcorr"),
+                    (dict(alfa=1.5), "alfa")):
         with pytest.raises(_lib.MgpError, match=msg):
             mgp.PressureSolver(g, **kw)
     with pytest.raises(_lib.MgpError, match="type"):
