@@ -13,17 +13,18 @@ the FP32-refined form with one copy of the SIP factors per block
 are equal bit for bit, every block here).
 
 The line also carries:
-  roofline     -- dominant kernel (forward sweep) timed with HIP events on
-                  the library's stream; algorithmic bytes per launch from the
-                  per-cell figures in `ALG_BYTES` below, against the 8 TB/s
-                  peak and against a measured device copy (torch copy_ of
-                  2 x 1.8 GB) as the practical streaming rate; `traffic` is
-                  the HBM bytes per launch from the committed rocprofv3
-                  FETCH_SIZE/WRITE_SIZE summary and `hbm_gbs` that over the
-                  launch time (factors shared between blocks come from L2,
-                  so HBM carries less than the algorithmic bytes);
-                  `fused_update` prices the whole inner iteration at the
-                  fused algorithmic bytes per cell-update (45 B FP32);
+  roofline     -- one SIP inner iteration (forward + backward sweep, timed
+                  with HIP events on the library's stream) priced at the
+                  fused algorithmic bytes per cell-update (45 B FP32: DP,
+                  RHS, AP, seven factors, mask read, DP written), against
+                  the 8 TB/s peak and a measured device copy (torch copy_
+                  of 2 x 1.8 GB); `step` prices the whole timed step
+                  (ninner x 45 + 49 B per cell) over ms_per_step;
+                  `traffic` is the HBM bytes per inner iteration from the
+                  committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, and
+                  `per_kernel` gives each sweep at its own bytes (factors
+                  shared between blocks come from L2, so HBM carries less
+                  than the algorithmic bytes);
   autotune     -- the pipeline shape the library's autotuner picked for the
                   plane kernels, with its per-shape timings;
   cpu_baseline -- the oracle (C restatement of the reference CPU path) timed
@@ -51,6 +52,12 @@ sys.path[:0] = [str(ROOT), str(ROOT / "mglet-base_amd")]
 #   backward sweep: RES, UE, UN, UT, HILF reads, HILF write          (6)
 ALG_BYTES = {"fp32": {"forward": 32, "backward": 24},
              "fp64": {"forward": 64, "backward": 48}}
+# fused algorithmic bytes per cell-update of one inner iteration (DP, RHS,
+# AP and the seven SIP factors read at 4 or 8 B, the int8 mask read, DP
+# written) and of the FP64 outer update (DP, HILF, RHS, AP, mask read; DP,
+# RHS written), whatever the kernels move on chip
+FUSED_BYTES = {"fp32": 45, "fp64": 89}
+OUTER_BYTES = 49
 HBM_PEAK_GBS = 8000.0
 
 
@@ -198,125 +205,6 @@ def cpu_baseline(target_s=12.0):
                        f"on 1 thread, {t_one:.1f} s")
 
 
-def adjoint_config():
-    """Adjoint projection at 256 blocks of 24^3 (8x8x4, periodic): U(-1,1)
-    velocity (numpy default_rng(5)), weights N(0,1) (rng 6), J = sum w u',
-    dJ/du through torch.autograd (the library's adjoint) against central
-    differences (h = 1e-6) in 64 random directions (rng 7) at a fixed
-    iteration count, for FP64 and FP32-refined; then a 20-step rollout
-    u <- P(u + 0.1 tanh u) differentiated with checkpoints every 5 steps and
-    without, and checked by central differences in 2 directions."""
-    import numpy as np
-    import torch
-    import mgp
-    import mgp.autograd as ag
-    from mgp.grids import uniform_box
-    g = uniform_box((8, 8, 4), (24, 24, 24), length=(1.0, 1.0, 0.5))
-    inner = torch.from_numpy(g.interior_mask()).cuda()
-
-    def field(rng, kind):
-        a = rng.uniform(-1, 1, g.n3d) if kind == "u" else rng.standard_normal(g.n3d)
-        return torch.where(inner, torch.from_numpy(a).cuda(), 0.0)
-
-    r5, r6, r7 = (np.random.default_rng(k) for k in (5, 6, 7))
-    u0 = [field(r5, "u") for _ in range(3)]
-    wts = [field(r6, "w") for _ in range(3)]
-    dirs = [[field(r7, "w") for _ in range(3)] for _ in range(64)]
-    out = {"config": "256 blocks of 24^3 (8x8x4 periodic), 3538944 cells", "h": 1e-6}
-    g64 = None
-    for form in ("fp64", "fp32"):
-        s = mgp.PressureSolver(g, ninner=5, nouter=4000, eps_rel=1e-12, precision=form,
-                               check_every=8)
-        x = [t.clone().requires_grad_(True) for t in u0]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        res = ag.project(s, *x, 1e-3)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        sum((a * b).sum() for a, b in zip(wts, res[:3])).backward()
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        fwd, adj = s.last_stats, s.last_adjoint_stats
-        nfix = fwd["nouter_done"]
-        s.set_iterations(nouter=nfix, eps_rel=0.0)
-
-        def J(f):
-            with torch.no_grad():
-                o = ag.project(s, *f, 1e-3)
-            return float(sum((a * b).sum() for a, b in zip(wts, o[:3])))
-
-        errs = []
-        for d in dirs:
-            fd = (J([a + 1e-6 * b for a, b in zip(u0, d)]) -
-                  J([a - 1e-6 * b for a, b in zip(u0, d)])) / 2e-6
-            ad = float(sum((t.grad * b).sum() for t, b in zip(x, d)))
-            errs.append(abs(fd - ad) / max(abs(ad), 1e-300))
-            if len(errs) % 8 == 0:
-                log(f"[c5] {form}: {len(errs)} directions, max rel err {max(errs):.2e}")
-        log(f"[c5] {form}: fwd {t1 - t0:.2f} s ({nfix} outer), adjoint {t2 - t1:.2f} s "
-            f"({adj['nouter_done']} outer), FD rel err max {max(errs):.2e}")
-        out[form] = {"forward_s": round(t1 - t0, 3), "adjoint_s": round(t2 - t1, 3),
-                     "outer_iterations": nfix, "adjoint_outer_iterations": adj["nouter_done"],
-                     "fd_rel_err_max": float(max(errs)),
-                     "fd_rel_err_median": float(np.median(errs)), "directions": len(dirs)}
-        grads = [t.grad.clone() for t in x]
-        if g64 is None:
-            g64 = grads
-        else:
-            out["fp32_grad_rel_err_vs_fp64"] = (
-                max(float((a - b).abs().max()) for a, b in zip(grads, g64))
-                / max(float(b.abs().max()) for b in g64))
-        s.close()
-
-    # rollout with checkpointing
-    s = mgp.PressureSolver(g, ninner=5, nouter=4000, eps_rel=1e-12, check_every=8)
-
-    def step(a, b, c):
-        p = ag.project(s, a + 0.1 * torch.tanh(a), b + 0.1 * torch.tanh(b),
-                       c + 0.1 * torch.tanh(c), 1e-3)
-        return p[0], p[1], p[2]
-
-    grads, mem, tim = [], [], []
-    for every in (0, 5):
-        torch.cuda.reset_peak_memory_stats()
-        base = torch.cuda.memory_allocated()
-        x = [t.clone().requires_grad_(True) for t in u0]
-        t0 = time.perf_counter()
-        o = ag.rollout(step, x, 20, checkpoint_every=every)
-        sum((a * b).sum() for a, b in zip(wts, o)).backward()
-        torch.cuda.synchronize()
-        tim.append(time.perf_counter() - t0)
-        mem.append((torch.cuda.max_memory_allocated() - base) / 2**20)
-        grads.append([t.grad.clone() for t in x])
-        log(f"[c5] rollout checkpoint_every={every}: {tim[-1]:.1f} s, peak +{mem[-1]:.0f} MiB")
-    diff = max(float((a - b).abs().max()) for a, b in zip(*grads))
-    scale = max(float(a.abs().max()) for a in grads[0])
-    nfix = s.last_stats["nouter_done"]
-    s.set_iterations(nouter=nfix, eps_rel=0.0)
-
-    def Jr(f):
-        with torch.no_grad():
-            o = ag.rollout(step, f, 20)
-        return float(sum((a * b).sum() for a, b in zip(wts, o)))
-
-    fd_err = []
-    for d in dirs[:2]:
-        fd = (Jr([a + 1e-6 * b for a, b in zip(u0, d)]) -
-              Jr([a - 1e-6 * b for a, b in zip(u0, d)])) / 2e-6
-        ad = float(sum((t * b).sum() for t, b in zip(grads[1], d)))
-        fd_err.append(abs(fd - ad) / max(abs(ad), 1e-300))
-        log(f"[c5] rollout FD direction: rel err {fd_err[-1]:.2e}")
-    out["rollout"] = {"steps": 20, "checkpoint_every": 5,
-                      "grad_rel_diff_checkpointed_vs_plain": diff / scale,
-                      "seconds": {"plain": round(tim[0], 2), "checkpointed": round(tim[1], 2)},
-                      "peak_extra_mib": {"plain": round(mem[0], 1),
-                                         "checkpointed": round(mem[1], 1)},
-                      "fd_outer_iterations_fixed": nfix,
-                      "fd_rel_err_max": float(max(fd_err)), "fd_directions": len(fd_err)}
-    s.close()
-    print(json.dumps({"adjoint_projection": out}))
-
-
 def copy_bandwidth(nbytes=1 << 31, reps=5):
     """Device copy rate in GB/s (bytes read + written) of torch's copy_ on
     two float32 tensors of `nbytes` each: the practical streaming rate the
@@ -407,12 +295,7 @@ def main():
                     help="also time the RK stage kernels (tstle4, rkstep) on the workload")
     ap.add_argument("--unshared", action="store_true",
                     help="keep one copy of the SIP factors per block (MGP_SHARED_FACTORS=0)")
-    ap.add_argument("--config", choices=["main", "c5"], default="main",
-                    help="c5: the adjoint-projection check (not the headline line)")
     args = ap.parse_args()
-    if args.config == "c5":
-        adjoint_config()
-        return
     if args.unshared:
         os.environ["MGP_SHARED_FACTORS"] = "0"
 
@@ -474,17 +357,34 @@ def main():
     upd = cells * args.ninner * args.steps
     value = upd / r["wall"] / 1e9
     fw_ms = r["kernels"]["forward_ms"]
-    alg = ALG_BYTES[mp]["forward"] * cells
-    achieved = alg / (fw_ms * 1e-3) / 1e9
-    traffic = None
+    bw_ms = r["kernels"]["backward_ms"]
+    # per-launch HBM bytes of the sweeps from the committed PMC summary
+    traffic = {}
     tfile = ROOT / "profiles" / "traffic_sip_32768x24.json"
     if tfile.exists() and nb == (32, 32, 32) and n == (24, 24, 24):
-        key = (f"void mgp::k_forward<{'float' if mp == 'fp32' else 'double'}, "
-               f"false, false, true, ")
         per = json.loads(tfile.read_text())["per_launch"]
-        hit = [k for k in per if k.startswith(key)]
-        if hit:
-            traffic = round(per[hit[0]]["total_bytes"] / 1e9, 3)
+        tname = "float" if mp == "fp32" else "double"
+        for kern, key in (("forward", f"void mgp::k_forward<{tname}, false, false, true, "),
+                          ("backward", f"void mgp::k_backward<{tname}, false, false, ")):
+            hit = [k for k in per if k.startswith(key)]
+            if hit:
+                traffic[kern] = per[hit[0]]["total_bytes"] / 1e9
+    upd_bytes = FUSED_BYTES[mp]
+    inner_ms = fw_ms + bw_ms
+    achieved = upd_bytes * cells / (inner_ms * 1e-3) / 1e9
+    step_bytes = args.ninner * upd_bytes + OUTER_BYTES
+    step_ms = r["wall"] / args.steps * 1e3
+    step_gbs = step_bytes * cells / (step_ms * 1e-3) / 1e9
+    per_kernel = {}
+    for kern, ms in (("forward", fw_ms), ("backward", bw_ms)):
+        a = ALG_BYTES[mp][kern] * cells / (ms * 1e-3) / 1e9
+        t = traffic.get(kern)
+        per_kernel[kern] = {"ms": round(ms, 4), "alg_bytes_per_cell": ALG_BYTES[mp][kern],
+                            "achieved": round(a, 1), "frac": round(a / HBM_PEAK_GBS, 4),
+                            "traffic": round(t, 3) if t else None,
+                            # HBM bytes actually moved over the launch time:
+                            # factors shared between blocks come from L2
+                            "hbm_gbs": round(t / (ms * 1e-3), 1) if t else None}
     line = {
         "metric": "GCell-updates/s + time-to-1e-10 residual, FP32-refined vs FP64, 1 MI355X",
         "value": round(value, 3),
@@ -510,27 +410,28 @@ def main():
                       "factor_classes": res[f]["factor_classes"][0],
                       "autotune": res[f]["autotune"]}
                   for f in res},
-        "roofline": {"bound": "hbm", "kernel": f"k_forward ({main_form})",
+        # one SIP inner iteration (forward + backward sweep) priced at the
+        # fused algorithmic bytes per cell-update: DP, RHS, AP, the seven
+        # factors and the mask read, DP written (45 B FP32, 89 B FP64); the
+        # RES exchange and halo traffic are overhead, not counted
+        "roofline": {"bound": "hbm",
+                     "kernel": f"SIP inner iteration: k_forward + k_backward ({main_form})",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_unit": "GB per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
-                     "alg_bytes_per_cell": ALG_BYTES[mp]["forward"],
-                     # HBM bytes actually moved per launch over the launch time:
-                     # factors and GSAP shared between blocks come from L2
-                     "hbm_gbs": (round(traffic / (fw_ms * 1e-3), 1) if traffic else None),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (round(sum(traffic.values()), 3)
+                                 if len(traffic) == 2 else None),
+                     "traffic_unit": "GB per inner iteration (forward + backward launch; "
+                                     "rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
+                     "alg_bytes_per_cell_update": upd_bytes,
+                     "inner_iteration_ms": round(inner_ms, 4),
                      "copy_gbs": round(copy_gbs, 1),
                      "frac_of_copy": round(achieved / copy_gbs, 4),
-                     "inner_iteration_gbs": round(
-                         (ALG_BYTES[mp]["forward"] + ALG_BYTES[mp]["backward"])
-                         * cells / ((fw_ms + r["kernels"]["backward_ms"]) * 1e-3) / 1e9, 1)},
+                     # the whole timed step: ninner inner iterations plus the
+                     # FP64 outer update (49 B per cell) over ms_per_step
+                     "step": {"bytes_per_cell": step_bytes, "achieved": round(step_gbs, 1),
+                              "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+                     "per_kernel": per_kernel},
     }
-    # whole inner iteration against the fused algorithmic bytes per
-    # cell-update (DP, RHS, AP, 7 factors, mask read, DP written: 45 B in
-    # FP32, 89 B in FP64), whatever the kernels actually move
-    fused = {"fp32": 45, "fp64": 89}[mp]
-    line["roofline"]["fused_update"] = {
-        "bytes_per_update": fused, "achieved": round(value * fused, 1),
-        "frac": round(value * fused / HBM_PEAK_GBS, 4)}
     if stage is not None:
         line["rk_stage"] = stage
     if "fp32" in res and "fp64" in res:
@@ -555,7 +456,9 @@ def main():
                 hist[form] = s.history()
             tts[form] = {"seconds": round(t, 4), "inner_iterations": st["ipc"],
                          "converged": st["converged"],
-                         "rel_residual": st["maxrhs"] / st["maxrhs0"]}
+                         "rel_residual": st["maxrhs"] / st["maxrhs0"],
+                         # pipeline shape the autotuner picked for this batch
+                         "autotune_shape": s.plan_shape()[0]}
             log(f"[bench] tts {form}: {t:.3f} s, {st['ipc']} inner, conv {st['converged']}")
             s.close()
         if "fp32" in ps and "fp64" in ps:

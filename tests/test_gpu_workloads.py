@@ -212,13 +212,23 @@ def _c5_gradient(c5, prec):
     import mgp.autograd as ag
     g, u0, wts, _, cache = c5
     if prec not in cache:
+        import time
         s = _solver(g, ninner=5, nouter=4000, eps_rel=1e-12, precision=prec, check_every=8)
         x = [t.clone().requires_grad_(True) for t in u0]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         out = ag.project(s, *x, 1e-3)
         J = sum((a * b).sum() for a, b in zip(wts, out[:3]))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
         J.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
         assert s.last_stats["converged"] and s.last_adjoint_stats["converged"]
-        cache[prec] = (s, float(J), [t.grad.detach().clone() for t in x])
+        print(f"[c5 {prec}] forward {s.last_stats['nouter_done']} outer {t1 - t0:.2f} s, "
+              f"adjoint {s.last_adjoint_stats['nouter_done']} outer {t2 - t1:.2f} s",
+              flush=True)
+        cache[prec] = (s, float(J.detach()), [t.grad.detach().clone() for t in x])
     return cache[prec]
 
 
