@@ -44,7 +44,8 @@ class OgParams(C.Structure):
     _fields_ = [("ityp", C.c_int32), ("ninner", C.c_int32), ("nouter", C.c_int32),
                 ("nouter_min", C.c_int32), ("epcorr", C.c_double), ("eps_rel", C.c_double),
                 ("omega", C.c_double), ("rho", C.c_double), ("ibtype", C.c_int32),
-                ("sor_all_levels", C.c_int32), ("sdiv", _dp)]
+                ("sor_all_levels", C.c_int32), ("sdiv", _dp), ("sync_by_caller", C.c_int32),
+                ("ubuf", _dp), ("vbuf", _dp), ("wbuf", _dp), ("pinf", _dp)]
 
 
 class OgStats(C.Structure):
@@ -66,6 +67,7 @@ def lib():
         _lib.og_maxabscal.restype = C.c_double
         _lib.og_time_sip.restype = C.c_double
         _lib.og_par_buffers.restype = C.c_int64
+        _lib.og_bound_buffers.restype = C.c_int64
         _lib.og_swc.restype = C.c_double
         _lib.og_swc.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp]
         _lib.og_ww_constants.argtypes = [C.c_double, C.c_double, _dp]
@@ -107,6 +109,9 @@ class Oracle:
                            _p(g.position, _i32p))
         self.ipbb = np.zeros(6 * g.ngrid, dtype=np.int64)
         self.nbuf = int(self.lib.og_par_buffers(C.byref(self.topo), _p(self.ipbb, _i64p)))
+        # velocity boundary buffers of the FIX, OP1 and PAR faces
+        self.bbo = np.zeros(6 * g.ngrid, dtype=np.int64)
+        self.nbb = int(self.lib.og_bound_buffers(C.byref(self.topo), _p(self.bbo, _i64p)))
         cap = 26 * g.ngrid + 26
         self.conns = (OgConn * cap)()
         n = self.lib.og_connect_build(C.byref(self.topo), self.conns, cap)
@@ -245,11 +250,21 @@ class Oracle:
 
     def mgpoisl(self, st, u, v, w, p, dt, ityp=0, ninner=5, nouter=9, nouter_min=0,
                 epcorr=0.0, eps_rel=0.0, omega=OMEGA_DEFAULT, rho=1.0, ibtype=0,
-                sor_all_levels=0, sdiv=None):
-        """sdiv: the ghostcell divergence source (ibtype 1), None = zero."""
+                sor_all_levels=0, sdiv=None, sync_by_caller=0, bound=None):
+        """sdiv: the ghostcell divergence source (ibtype 1), None = zero.
+        sync_by_caller: stop after ftoc(u, v, w, p) instead of running the
+        closing parent, bound_flow and connect.  bound: dict with the FIX /
+        OP1 boundary values "u", "v", "w" (bound_buffer() layout) and
+        "pinf" ([ngrid*6] OP1 pressure), any of them optional."""
         g = self.g
+        bd = bound or {}
+        bu, bv, bw = (np.ascontiguousarray(bd[k], dtype=np.float64) if k in bd else None
+                      for k in ("u", "v", "w"))
+        pf = np.ascontiguousarray(bd["pinf"], dtype=np.float64) if "pinf" in bd else None
         prm = OgParams(ityp, ninner, nouter, nouter_min, epcorr, eps_rel, omega, rho, ibtype,
-                       sor_all_levels, _p(sdiv) if sdiv is not None else None)
+                       sor_all_levels, _p(sdiv) if sdiv is not None else None,
+                       int(sync_by_caller), *[_p(a) if a is not None else None
+                                              for a in (bu, bv, bw, pf)])
         hist = np.zeros(max(nouter, 1))
         stats = OgStats(0, 0, 0.0, 0.0, _p(hist))
         work = [np.zeros(g.n3d) for _ in range(4)]
@@ -272,6 +287,23 @@ class Oracle:
     # -- multi-level coupling ------------------------------------------
     def par_buffer(self):
         return np.zeros(max(self.nbuf, 1))
+
+    def bound_buffer(self):
+        """A zeroed velocity boundary buffer (FIX, OP1 and PAR faces)."""
+        return np.zeros(max(self.nbb, 1))
+
+    def parent_uvw(self, level, u, v, w, ub, vb, wb):
+        self.lib.og_parent_uvw(C.byref(self.topo), level, _p(u), _p(v), _p(w), _p(ub),
+                               _p(vb), _p(wb), _p(self.bbo, _i64p))
+
+    def bound_flow(self, level, u, v, w, p, ub, vb, wb, pinf=None, bp=None, rho=1.0):
+        g = self.g
+        pf = None if pinf is None else np.ascontiguousarray(pinf, dtype=np.float64)
+        self.lib.og_bound_flow(C.byref(self.topo), level, _p(u), _p(v), _p(w), _p(p), _p(ub),
+                               _p(vb), _p(wb), _p(self.bbo, _i64p),
+                               _p(pf) if pf is not None else None,
+                               _p(bp) if bp is not None else None, _p(g.ddx), _p(g.ddy),
+                               _p(g.ddz), C.c_double(rho))
 
     def ctof(self, level, f):
         self.lib.og_ctof(C.byref(self.topo), level, _p(f), _p(f))

@@ -1321,6 +1321,540 @@ void og_ftoc_uvwp(const og_topo *t, int level, double *u_, double *v_,
 }
 
 /* ------------------------------------------------------------------ */
+/* ------------------------------------------------------------------ */
+/* Closing sync of mgpoisl: parent(u, v, w), bound_flow, connect per level
+ * (src/flow/pressuresolver_mod.F90:338-348). */
+
+#define FIX_BC 2
+#define OP1_BC 3
+#define NOS_BC 5
+#define SLI_BC 6
+
+/* Velocity boundary buffers (the field buffers at ipbb, src/core/
+ * pointers_mod.F90:64-92) of every face whose flow boundary reads them:
+ * FIX, OP1 (the caller's prescribed values) and PAR (filled by parent),
+ * kk*jj / kk*ii / jj*ii values per face, first index fastest.  bbo[6*ngrid]
+ * gets the offsets (-1 = none); returns the total length. */
+int64_t og_bound_buffers(const og_topo *t, int64_t *bbo)
+{
+    int64_t n = 0;
+    for (int g = 1; g <= t->ngrid; ++g) {
+        GDIMS(t, g);
+        for (int f = 1; f <= 6; ++f) {
+            const int b = t->bc[6 * (g - 1) + f - 1];
+            int64_t *o = &bbo[6 * (g - 1) + f - 1];
+            if (b != FIX_BC && b != OP1_BC && b != PAR_BC) { *o = -1; continue; }
+            *o = n;
+            n += f <= 2 ? (int64_t)kk * jj : f <= 4 ? (int64_t)kk * ii : (int64_t)jj * ii;
+        }
+    }
+    return n;
+}
+
+/* prolong1 / prolong2 (src/ib/parent/parent1_mod.F90:515-583) on a packed
+ * coarse face `in` (jjc x iic) into the fine face `out` (jj x ii) through
+ * `tmp` (jj x iic). */
+static void prolong12(int jjc, int iic, int jj, int ii, const double *in,
+                      double *tmp, double *out, int stag1, int stag2)
+{
+#define IN(j, i) in[((j)-1) + (int64_t)((i)-1) * jjc]
+#define TM(j, i) tmp[((j)-1) + (int64_t)((i)-1) * jj]
+#define OU(j, i) out[((j)-1) + (int64_t)((i)-1) * jj]
+    for (int ic = 1; ic <= iic; ++ic)
+        for (int j = 1; j <= jj; j += 2) {
+            const int jc = 2 + (j - 1) / 2;
+            TM(j, ic) = stag1 ? 0.5 * (IN(jc, ic) + IN(jc - 1, ic)) : IN(jc, ic);
+            TM(j + 1, ic) = IN(jc, ic);
+        }
+    for (int i = 1; i <= ii; i += 2) {
+        const int ic = 2 + (i - 1) / 2;
+        for (int j = 1; j <= jj; ++j) {
+            OU(j, i) = stag2 ? 0.5 * (TM(j, ic) + TM(j, ic - 1)) : TM(j, ic);
+            OU(j, i + 1) = TM(j, ic);
+        }
+    }
+#undef IN
+#undef TM
+#undef OU
+}
+
+/* parent(ilevel, u, v, w) into the PAR face buffers: the coarse slab of
+ * start_and_stop with icomp (the normal component at a back face starts one
+ * index lower, src/ib/parent/parent_core_mod.F90:213-267), packed k, j, i
+ * fastest first (pack_single, src/ib/parent/parent1_mod.F90:296-338) and
+ * prolonged with the staggering of stag (:185-210): at x faces (k, j) v is
+ * staggered in the second, w in the first direction; at y faces (k, i) u in
+ * the second, w in the first; at z faces (j, i) u in the second, v in the
+ * first. */
+void og_parent_uvw(const og_topo *t, int level, const double *u_, const double *v_,
+                   const double *w_, double *ubuf, double *vbuf, double *wbuf,
+                   const int64_t *bbo)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        if (t->level[g - 1] != level) continue;
+        const int c = t->iparent[g - 1];
+        if (c == 0) continue;
+        GDIMS(t, g);
+        const int ipos = POS(g, 0), jpos = POS(g, 1), kpos = POS(g, 2);
+        const int64_t kkc = t->kk[c - 1], jjc = t->jj[c - 1];
+        const int kkc2 = kk / 2 + 2, jjc2 = jj / 2 + 2, iic2 = ii / 2 + 2;
+        const int mx = (kk > jj ? kk : jj) > ii ? (kk > jj ? kk : jj) : ii;
+        const int mxc = mx / 2 + 2;
+        double *in = malloc(sizeof(double) * (size_t)mxc * mxc);
+        double *tmp = malloc(sizeof(double) * (size_t)mx * mxc);
+        for (int f = 1; f <= 6; ++f) {
+            if (t->bc[6 * (g - 1) + f - 1] != PAR_BC) continue;
+            const int64_t o = bbo[6 * (g - 1) + f - 1];
+            for (int comp = 1; comp <= 3; ++comp) {
+                const double *fc = (comp == 1 ? u_ : comp == 2 ? v_ : w_) + t->ip3[c - 1];
+                double *buf = (comp == 1 ? ubuf : comp == 2 ? vbuf : wbuf) + o;
+                const int icomp = (f <= 2 && comp == 1) || (f > 2 && f <= 4 && comp == 2) ||
+                                  (f > 4 && comp == 3) ? comp : 0;
+                int ks = kpos - 2, js = jpos - 2, is = ipos - 2;
+                switch (f) {
+                case 1: is = ipos - 1; break;
+                case 2: is = ipos + (ii - 4) / 2 - (icomp == 1); break;
+                case 3: js = jpos - 1; break;
+                case 4: js = jpos + (jj - 4) / 2 - (icomp == 2); break;
+                case 5: ks = kpos - 1; break;
+                default: ks = kpos + (kk - 4) / 2 - (icomp == 3); break;
+                }
+                int n1, n2, f1, f2, s1 = 0, s2 = 0;
+                if (f <= 2) {
+                    n1 = kkc2; n2 = jjc2; f1 = kk; f2 = jj;
+                    s1 = comp == 3; s2 = comp == 2;
+                } else if (f <= 4) {
+                    n1 = kkc2; n2 = iic2; f1 = kk; f2 = ii;
+                    s1 = comp == 3; s2 = comp == 1;
+                } else {
+                    n1 = jjc2; n2 = iic2; f1 = jj; f2 = ii;
+                    s1 = comp == 2; s2 = comp == 1;
+                }
+#define FC(k, j, i) fc[((k)-1) + ((j)-1) * kkc + ((i)-1) * kkc * jjc]
+                for (int b = 1; b <= n2; ++b)
+                    for (int a = 1; a <= n1; ++a)
+                        in[(a - 1) + (int64_t)(b - 1) * n1] =
+                            f <= 2 ? FC(ks + a - 1, js + b - 1, is)
+                            : f <= 4 ? FC(ks + a - 1, js, is + b - 1)
+                                     : FC(ks, js + a - 1, is + b - 1);
+#undef FC
+                prolong12(n1, n2, f1, f2, in, tmp, buf, s1, s2);
+            }
+        }
+        free(in);
+        free(tmp);
+    }
+}
+
+#define BPV(k, j, i) (bp ? bp[IX(k, j, i)] : 1.0)
+
+/* bfront (src/flow/bound_flow_mod.F90:23-212): faces 1 and 2 */
+static void bfront(int f, int ctyp, int kk, int jj, int ii, double *u, double *v,
+                   double *w, double *p, const double *ub, const double *vb,
+                   const double *wb, double pinf, const double *bp, const double *ddy,
+                   const double *ddz, double rho)
+{
+    const int i2 = f == 1 ? 2 : ii - 1, i3 = f == 1 ? 3 : ii - 2, i4 = f == 1 ? 4 : ii - 3;
+    const int istag1 = f == 1 ? 1 : ii - 1, istag2 = f == 1 ? 2 : ii - 2;
+    const double dir = f == 1 ? -1.0 : 1.0;
+#define BF(b, k, j) b[((k)-1) + (int64_t)((j)-1) * kk]
+#define DY(j) ddy[(j)-1]
+#define DZ(k) ddz[(k)-1]
+    switch (ctyp) {
+    case FIX_BC:
+        for (int j = 1; j <= jj; ++j)
+            for (int k = 1; k <= kk; ++k) {
+                u[IX(k, j, istag2)] = BF(ub, k, j);
+                v[IX(k, j, i2)] = 2.0 * BF(vb, k, j) - v[IX(k, j, i3)];
+                w[IX(k, j, i2)] = 2.0 * BF(wb, k, j) - w[IX(k, j, i3)];
+            }
+        break;
+    case OP1_BC:
+        for (int j = 1; j <= jj - 1; ++j)
+            for (int k = 1; k <= kk - 1; ++k) {
+                u[IX(k, j, istag1)] = u[IX(k, j, istag2)];
+                double flag = copysign(1.0, dir * (u[IX(k, j, istag2)] + u[IX(k, j + 1, istag2)]));
+                flag = 0.5 * (flag + 1.0);
+                v[IX(k, j, i2)] = flag * v[IX(k, j, i3)] +
+                                  (1.0 - flag) * (2.0 * BF(vb, k, j) - v[IX(k, j, i3)]);
+                flag = copysign(1.0, dir * (u[IX(k, j, istag2)] + u[IX(k + 1, j, istag2)]));
+                flag = 0.5 * (flag + 1.0);
+                w[IX(k, j, i2)] = flag * w[IX(k, j, i3)] +
+                                  (1.0 - flag) * (2.0 * BF(wb, k, j) - w[IX(k, j, i3)]);
+            }
+        break;
+    case NOS_BC:
+        for (int j = 1; j <= jj; ++j)
+            for (int k = 1; k <= kk; ++k) {
+                u[IX(k, j, istag2)] = 0.0;
+                v[IX(k, j, i2)] = 0.0;
+                w[IX(k, j, i2)] = 0.0;
+            }
+        break;
+    case SLI_BC:
+        for (int j = 1; j <= jj; ++j)
+            for (int k = 1; k <= kk; ++k) {
+                u[IX(k, j, istag2)] = 0.0;
+                v[IX(k, j, i2)] = v[IX(k, j, i3)];
+                w[IX(k, j, i2)] = w[IX(k, j, i3)];
+            }
+        break;
+    case PAR_BC:
+        for (int j = 2; j <= jj - 1; ++j)
+            for (int k = 2; k <= kk - 1; ++k) {
+                const double sbv = BPV(k, j, i2) * BPV(k, j + 1, i2);
+                const double sbw = BPV(k, j, i2) * BPV(k + 1, j, i2);
+                v[IX(k, j, i2)] = (BF(vb, k, j) - 0.5 * (v[IX(k, j, i3)] - v[IX(k, j, i4)])) * sbv +
+                                  (1.0 - sbv) * v[IX(k, j, i2)];
+                w[IX(k, j, i2)] = (BF(wb, k, j) - 0.5 * (w[IX(k, j, i3)] - w[IX(k, j, i4)])) * sbw +
+                                  (1.0 - sbw) * w[IX(k, j, i2)];
+            }
+        for (int j = 3; j <= jj - 2; j += 2)
+            for (int k = 3; k <= kk - 2; k += 2) {
+                const double sbu = BPV(k, j, i2);
+                double sb11, sb12, sb13, sb14;
+                if (sbu > 0.5) {
+                    sb11 = BPV(k, j, i2) * BPV(k, j, i3);
+                    sb12 = BPV(k, j + 1, i2) * BPV(k, j + 1, i3);
+                    sb13 = BPV(k + 1, j, i2) * BPV(k + 1, j, i3);
+                    sb14 = BPV(k + 1, j + 1, i2) * BPV(k + 1, j + 1, i3);
+                } else {
+                    sb11 = fmin(BPV(k, j, i3) + BPV(k, j, i2) + BPV(k, j, i4) + BPV(k, j - 1, i3) +
+                                BPV(k, j + 1, i3) + BPV(k - 1, j, i3) + BPV(k + 1, j, i3), 1.0);
+                    sb12 = fmin(BPV(k, j + 1, i3) + BPV(k, j + 1, i2) + BPV(k, j + 1, i4) +
+                                BPV(k, j, i3) + BPV(k, j + 2, i3) + BPV(k - 1, j + 1, i3) +
+                                BPV(k + 1, j + 1, i3), 1.0);
+                    sb13 = fmin(BPV(k + 1, j, i3) + BPV(k + 1, j, i2) + BPV(k + 1, j, i4) +
+                                BPV(k + 1, j - 1, i3) + BPV(k + 1, j + 1, i3) + BPV(k, j, i3) +
+                                BPV(k + 2, j, i3), 1.0);
+                    sb14 = fmin(BPV(k + 1, j + 1, i3) + BPV(k + 1, j + 1, i2) + BPV(k + 1, j + 1, i4) +
+                                BPV(k + 1, j, i3) + BPV(k + 1, j + 2, i3) + BPV(k, j + 1, i3) +
+                                BPV(k + 2, j + 1, i3), 1.0);
+                }
+                double fak = 1.0 / ((DY(j) + DY(j + 1)) * (DZ(k) + DZ(k + 1))) *
+                             (sb11 * DY(j) * DZ(k) + sb12 * DY(j + 1) * DZ(k) +
+                              sb13 * DY(j) * DZ(k + 1) + sb14 * DY(j + 1) * DZ(k + 1));
+                if (fak < 0.1) fak = 1.0;
+                fak = 1.0 / fak;
+                const double u11 = u[IX(k, j, istag2)], u12 = u[IX(k, j + 1, istag2)];
+                const double u13 = u[IX(k + 1, j, istag2)], u14 = u[IX(k + 1, j + 1, istag2)];
+                const double ubfine = (u11 * (1.0 - sb11) * DY(j) * DZ(k) +
+                                       u12 * (1.0 - sb12) * DY(j + 1) * DZ(k) +
+                                       u13 * (1.0 - sb13) * DY(j) * DZ(k + 1) +
+                                       u14 * (1.0 - sb14) * DY(j + 1) * DZ(k + 1)) /
+                                      ((DY(j) + DY(j + 1)) * (DZ(k) + DZ(k + 1)));
+                u[IX(k, j, istag2)] = (BF(ub, k, j) - ubfine) * fak * sb11 + (1.0 - sb11) * u11 * sbu;
+                u[IX(k, j + 1, istag2)] = (BF(ub, k, j + 1) - ubfine) * fak * sb12 + (1.0 - sb12) * u12 * sbu;
+                u[IX(k + 1, j, istag2)] = (BF(ub, k + 1, j) - ubfine) * fak * sb13 + (1.0 - sb13) * u13 * sbu;
+                u[IX(k + 1, j + 1, istag2)] = (BF(ub, k + 1, j + 1) - ubfine) * fak * sb14 +
+                                              (1.0 - sb14) * u14 * sbu;
+            }
+        break;
+    }
+    if (ctyp == OP1_BC) {
+        for (int j = 2; j <= jj; ++j)
+            for (int k = 2; k <= kk; ++k) {
+                const double vi = 0.25 * (v[IX(k, j - 1, i2)] + v[IX(k, j, i2)] +
+                                          v[IX(k, j - 1, i3)] + v[IX(k, j, i3)]);
+                const double wi = 0.25 * (w[IX(k - 1, j, i2)] + w[IX(k, j, i2)] +
+                                          w[IX(k - 1, j, i3)] + w[IX(k, j, i3)]);
+                const double us = u[IX(k, j, istag2)];
+                const double umagsqr = us * us + vi * vi + wi * wi;
+                p[IX(k, j, i2)] = pinf + fmin(0.0, 0.5 * rho * copysign(umagsqr, dir * us));
+            }
+    } else if (ctyp == FIX_BC || ctyp == NOS_BC || ctyp == SLI_BC) {
+        for (int j = 1; j <= jj; ++j)
+            for (int k = 1; k <= kk; ++k) p[IX(k, j, i2)] = p[IX(k, j, i3)];
+    }
+#undef BF
+#undef DY
+#undef DZ
+}
+
+/* bright (src/flow/bound_flow_mod.F90:215-494): faces 3 and 4.  The
+ * tangential w of a PAR face is extrapolated from w(j2), not w(j3) as in
+ * bfront and bbottom: restated as written. */
+static void bright(int f, int ctyp, int kk, int jj, int ii, double *u, double *v,
+                   double *w, double *p, const double *ub, const double *vb,
+                   const double *wb, double pinf, const double *bp, const double *ddx,
+                   const double *ddz, double rho)
+{
+    const int j2 = f == 3 ? 2 : jj - 1, j3 = f == 3 ? 3 : jj - 2, j4 = f == 3 ? 4 : jj - 3;
+    const int jstag1 = f == 3 ? 1 : jj - 1, jstag2 = f == 3 ? 2 : jj - 2;
+    const double dir = f == 3 ? -1.0 : 1.0;
+#define BF(b, k, i) b[((k)-1) + (int64_t)((i)-1) * kk]
+#define DX(i) ddx[(i)-1]
+#define DZ(k) ddz[(k)-1]
+    switch (ctyp) {
+    case FIX_BC:
+        for (int i = 1; i <= ii; ++i)
+            for (int k = 1; k <= kk; ++k) {
+                u[IX(k, j2, i)] = 2.0 * BF(ub, k, i) - u[IX(k, j3, i)];
+                v[IX(k, jstag2, i)] = BF(vb, k, i);
+                w[IX(k, j2, i)] = 2.0 * BF(wb, k, i) - w[IX(k, j3, i)];
+            }
+        break;
+    case OP1_BC:
+        for (int i = 1; i <= ii - 1; ++i)
+            for (int k = 1; k <= kk - 1; ++k) {
+                double flag = copysign(1.0, dir * (v[IX(k, jstag2, i)] + v[IX(k, jstag2, i + 1)]));
+                flag = 0.5 * (flag + 1.0);
+                u[IX(k, j2, i)] = flag * u[IX(k, j3, i)] +
+                                  (1.0 - flag) * (2.0 * BF(ub, k, i) - u[IX(k, j3, i)]);
+                v[IX(k, jstag1, i)] = v[IX(k, jstag2, i)];
+                flag = copysign(1.0, dir * (v[IX(k, jstag2, i)] + v[IX(k + 1, jstag2, i)]));
+                flag = 0.5 * (flag + 1.0);
+                w[IX(k, j2, i)] = flag * w[IX(k, j3, i)] +
+                                  (1.0 - flag) * (2.0 * BF(wb, k, i) - w[IX(k, j3, i)]);
+            }
+        break;
+    case NOS_BC:
+        for (int i = 1; i <= ii; ++i)
+            for (int k = 1; k <= kk; ++k) {
+                u[IX(k, j2, i)] = 0.0;
+                v[IX(k, jstag2, i)] = 0.0;
+                w[IX(k, j2, i)] = 0.0;
+            }
+        break;
+    case SLI_BC:
+        for (int i = 1; i <= ii; ++i)
+            for (int k = 1; k <= kk; ++k) {
+                u[IX(k, j2, i)] = u[IX(k, j3, i)];
+                v[IX(k, jstag2, i)] = 0.0;
+                w[IX(k, j2, i)] = w[IX(k, j3, i)];
+            }
+        break;
+    case PAR_BC:
+        for (int i = 2; i <= ii - 1; ++i)
+            for (int k = 2; k <= kk - 1; ++k) {
+                const double sbu = BPV(k, j2, i) * BPV(k, j2, i + 1);
+                const double sbw = BPV(k, j2, i) * BPV(k + 1, j2, i);
+                u[IX(k, j2, i)] = (BF(ub, k, i) - 0.5 * (u[IX(k, j3, i)] - u[IX(k, j4, i)])) * sbu +
+                                  (1.0 - sbu) * u[IX(k, j2, i)];
+                w[IX(k, j2, i)] = (BF(wb, k, i) - 0.5 * (w[IX(k, j2, i)] - w[IX(k, j4, i)])) * sbw +
+                                  (1.0 - sbw) * w[IX(k, j2, i)];
+            }
+        for (int i = 3; i <= ii - 2; i += 2)
+            for (int k = 3; k <= kk - 2; k += 2) {
+                const double sbv = BPV(k, j2, i);
+                double sb11, sb12, sb13, sb14;
+                if (sbv > 0.5) {
+                    sb11 = BPV(k, j2, i) * BPV(k, j3, i);
+                    sb12 = BPV(k, j2, i + 1) * BPV(k, j3, i + 1);
+                    sb13 = BPV(k + 1, j2, i) * BPV(k + 1, j3, i);
+                    sb14 = BPV(k + 1, j2, i + 1) * BPV(k + 1, j3, i + 1);
+                } else {
+                    sb11 = fmin(BPV(k, j3, i) + BPV(k, j2, i) + BPV(k, j4, i) + BPV(k, j3, i - 1) +
+                                BPV(k, j3, i + 1) + BPV(k - 1, j3, i) + BPV(k + 1, j3, i), 1.0);
+                    sb12 = fmin(BPV(k, j3, i + 1) + BPV(k, j2, i + 1) + BPV(k, j4, i + 1) +
+                                BPV(k, j3, i) + BPV(k, j3, i + 2) + BPV(k - 1, j3, i + 1) +
+                                BPV(k + 1, j3, i + 1), 1.0);
+                    sb13 = fmin(BPV(k + 1, j3, i) + BPV(k + 1, j2, i) + BPV(k + 1, j4, i) +
+                                BPV(k + 1, j3, i - 1) + BPV(k + 1, j3, i + 1) + BPV(k, j3, i) +
+                                BPV(k + 2, j3, i), 1.0);
+                    sb14 = fmin(BPV(k + 1, j3, i + 1) + BPV(k + 1, j2, i + 1) + BPV(k + 1, j4, i + 1) +
+                                BPV(k + 1, j3, i) + BPV(k + 1, j3, i + 2) + BPV(k, j3, i + 1) +
+                                BPV(k + 2, j3, i + 1), 1.0);
+                }
+                double fak = 1.0 / ((DZ(k) + DZ(k + 1)) * (DX(i) + DX(i + 1))) *
+                             (sb11 * DZ(k) * DX(i) + sb12 * DZ(k) * DX(i + 1) +
+                              sb13 * DZ(k + 1) * DX(i) + sb14 * DZ(k + 1) * DX(i + 1));
+                if (fak < 0.1) fak = 1.0;
+                fak = 1.0 / fak;
+                const double v11 = v[IX(k, jstag2, i)], v12 = v[IX(k, jstag2, i + 1)];
+                const double v13 = v[IX(k + 1, jstag2, i)], v14 = v[IX(k + 1, jstag2, i + 1)];
+                const double vbfine = (v11 * (1.0 - sb11) * DZ(k) * DX(i) +
+                                       v12 * (1.0 - sb12) * DZ(k) * DX(i + 1) +
+                                       v13 * (1.0 - sb13) * DZ(k + 1) * DX(i) +
+                                       v14 * (1.0 - sb14) * DZ(k + 1) * DX(i + 1)) /
+                                      ((DZ(k) + DZ(k + 1)) * (DX(i) + DX(i + 1)));
+                v[IX(k, jstag2, i)] = (BF(vb, k, i) - vbfine) * fak * sb11 + v11 * (1.0 - sb11) * sbv;
+                v[IX(k, jstag2, i + 1)] = (BF(vb, k, i + 1) - vbfine) * fak * sb12 + v12 * (1.0 - sb12) * sbv;
+                v[IX(k + 1, jstag2, i)] = (BF(vb, k + 1, i) - vbfine) * fak * sb13 + v13 * (1.0 - sb13) * sbv;
+                v[IX(k + 1, jstag2, i + 1)] = (BF(vb, k + 1, i + 1) - vbfine) * fak * sb14 +
+                                              v14 * (1.0 - sb14) * sbv;
+            }
+        break;
+    }
+    if (ctyp == OP1_BC) {
+        for (int i = 2; i <= ii; ++i)
+            for (int k = 2; k <= kk; ++k) {
+                const double ui = 0.25 * (u[IX(k, j2, i - 1)] + u[IX(k, j2, i)] +
+                                          u[IX(k, j3, i - 1)] + u[IX(k, j3, i)]);
+                const double wi = 0.25 * (w[IX(k - 1, j2, i)] + w[IX(k, j2, i)] +
+                                          w[IX(k - 1, j3, i)] + w[IX(k, j3, i)]);
+                const double vs = v[IX(k, jstag2, i)];
+                const double umagsqr = ui * ui + vs * vs + wi * wi;
+                p[IX(k, j2, i)] = pinf + fmin(0.0, 0.5 * rho * copysign(umagsqr, dir * vs));
+            }
+    } else if (ctyp == FIX_BC || ctyp == NOS_BC || ctyp == SLI_BC) {
+        for (int i = 1; i <= ii; ++i)
+            for (int k = 1; k <= kk; ++k) p[IX(k, j2, i)] = p[IX(k, j3, i)];
+    }
+#undef BF
+#undef DX
+#undef DZ
+}
+
+/* bbottom (src/flow/bound_flow_mod.F90:497-731): faces 5 and 6 */
+static void bbottom(int f, int ctyp, int kk, int jj, int ii, double *u, double *v,
+                    double *w, double *p, const double *ub, const double *vb,
+                    const double *wb, double pinf, const double *bp, const double *ddx,
+                    const double *ddy, double rho)
+{
+    const int k2 = f == 5 ? 2 : kk - 1, k3 = f == 5 ? 3 : kk - 2, k4 = f == 5 ? 4 : kk - 3;
+    const int kstag1 = f == 5 ? 1 : kk - 1, kstag2 = f == 5 ? 2 : kk - 2;
+    const double dir = f == 5 ? -1.0 : 1.0;
+#define BF(b, j, i) b[((j)-1) + (int64_t)((i)-1) * jj]
+#define DX(i) ddx[(i)-1]
+#define DY(j) ddy[(j)-1]
+    switch (ctyp) {
+    case FIX_BC:
+        for (int i = 1; i <= ii; ++i)
+            for (int j = 1; j <= jj; ++j) {
+                u[IX(k2, j, i)] = 2.0 * BF(ub, j, i) - u[IX(k3, j, i)];
+                v[IX(k2, j, i)] = 2.0 * BF(vb, j, i) - v[IX(k3, j, i)];
+                w[IX(kstag2, j, i)] = BF(wb, j, i);
+            }
+        break;
+    case OP1_BC:
+        for (int i = 1; i <= ii - 1; ++i)
+            for (int j = 1; j <= jj - 1; ++j) {
+                double flag = copysign(1.0, dir * (w[IX(kstag2, j, i)] + w[IX(kstag2, j, i + 1)]));
+                flag = 0.5 * (flag + 1.0);
+                u[IX(k2, j, i)] = flag * u[IX(k3, j, i)] +
+                                  (1.0 - flag) * (2 * BF(ub, j, i) - u[IX(k3, j, i)]);
+                flag = copysign(1.0, dir * (w[IX(kstag2, j, i)] + w[IX(kstag2, j + 1, i)]));
+                flag = 0.5 * (flag + 1.0);
+                v[IX(k2, j, i)] = flag * v[IX(k3, j, i)] +
+                                  (1.0 - flag) * (2.0 * BF(vb, j, i) - v[IX(k3, j, i)]);
+                w[IX(kstag1, j, i)] = w[IX(kstag2, j, i)];
+            }
+        break;
+    case NOS_BC:
+        for (int i = 1; i <= ii; ++i)
+            for (int j = 1; j <= jj; ++j) {
+                u[IX(k2, j, i)] = 0.0;
+                v[IX(k2, j, i)] = 0.0;
+                w[IX(kstag2, j, i)] = 0.0;
+            }
+        break;
+    case SLI_BC:
+        for (int i = 1; i <= ii; ++i)
+            for (int j = 1; j <= jj; ++j) {
+                u[IX(k2, j, i)] = u[IX(k3, j, i)];
+                v[IX(k2, j, i)] = v[IX(k3, j, i)];
+                w[IX(kstag2, j, i)] = 0.0;
+            }
+        break;
+    case PAR_BC:
+        for (int i = 2; i <= ii - 1; ++i)
+            for (int j = 2; j <= jj - 1; ++j) {
+                const double sbu = BPV(k2, j, i) * BPV(k2, j, i + 1);
+                const double sbv = BPV(k2, j, i) * BPV(k2, j + 1, i);
+                u[IX(k2, j, i)] = (BF(ub, j, i) - 0.5 * (u[IX(k3, j, i)] - u[IX(k4, j, i)])) * sbu +
+                                  (1.0 - sbu) * u[IX(k2, j, i)];
+                v[IX(k2, j, i)] = (BF(vb, j, i) - 0.5 * (v[IX(k3, j, i)] - v[IX(k4, j, i)])) * sbv +
+                                  (1.0 - sbv) * v[IX(k2, j, i)];
+            }
+        for (int i = 3; i <= ii - 2; i += 2)
+            for (int j = 3; j <= jj - 2; j += 2) {
+                const double sbw = BPV(k2, j, i);
+                double sb11, sb12, sb13, sb14;
+                if (sbw > 0.5) {
+                    sb11 = BPV(k2, j, i) * BPV(k3, j, i);
+                    sb12 = BPV(k2, j, i + 1) * BPV(k3, j, i + 1);
+                    sb13 = BPV(k2, j + 1, i) * BPV(k3, j + 1, i);
+                    sb14 = BPV(k2, j + 1, i + 1) * BPV(k3, j + 1, i + 1);
+                } else {
+                    sb11 = fmin(BPV(k3, j, i) + BPV(k2, j, i) + BPV(k4, j, i) + BPV(k3, j, i - 1) +
+                                BPV(k3, j, i + 1) + BPV(k3, j - 1, i) + BPV(k3, j + 1, i), 1.0);
+                    sb12 = fmin(BPV(k3, j, i + 1) + BPV(k2, j, i + 1) + BPV(k4, j, i + 1) +
+                                BPV(k3, j, i) + BPV(k3, j, i + 2) + BPV(k3, j - 1, i + 1) +
+                                BPV(k3, j + 1, i + 1), 1.0);
+                    sb13 = fmin(BPV(k3, j + 1, i) + BPV(k2, j + 1, i) + BPV(k4, j + 1, i) +
+                                BPV(k3, j + 1, i - 1) + BPV(k3, j + 1, i + 1) + BPV(k3, j, i) +
+                                BPV(k3, j + 2, i), 1.0);
+                    sb14 = fmin(BPV(k3, j + 1, i + 1) + BPV(k2, j + 1, i + 1) + BPV(k4, j + 1, i + 1) +
+                                BPV(k3, j + 1, i) + BPV(k3, j + 1, i + 2) + BPV(k3, j, i + 1) +
+                                BPV(k3, j + 2, i + 1), 1.0);
+                }
+                double fak = 1. / ((DY(j) + DY(j + 1)) * (DX(i) + DX(i + 1))) *
+                             (sb11 * DY(j) * DX(i) + sb12 * DY(j) * DX(i + 1) +
+                              sb13 * DY(j + 1) * DX(i) + sb14 * DY(j + 1) * DX(i + 1));
+                if (fak < 0.1) fak = 1.0;
+                fak = 1.0 / fak;
+                const double w11 = w[IX(kstag2, j, i)], w12 = w[IX(kstag2, j, i + 1)];
+                const double w13 = w[IX(kstag2, j + 1, i)], w14 = w[IX(kstag2, j + 1, i + 1)];
+                const double wbfine = (w11 * (1.0 - sb11) * DY(j) * DX(i) +
+                                       w12 * (1.0 - sb12) * DY(j) * DX(i + 1) +
+                                       w13 * (1.0 - sb13) * DY(j + 1) * DX(i) +
+                                       w14 * (1.0 - sb14) * DY(j + 1) * DX(i + 1)) /
+                                      ((DY(j) + DY(j + 1)) * (DX(i) + DX(i + 1)));
+                w[IX(kstag2, j, i)] = (BF(wb, j, i) - wbfine) * fak * sb11 + w11 * (1.0 - sb11) * sbw;
+                w[IX(kstag2, j, i + 1)] = (BF(wb, j, i + 1) - wbfine) * fak * sb12 + w12 * (1.0 - sb12) * sbw;
+                w[IX(kstag2, j + 1, i)] = (BF(wb, j + 1, i) - wbfine) * fak * sb13 + w13 * (1.0 - sb13) * sbw;
+                w[IX(kstag2, j + 1, i + 1)] = (BF(wb, j + 1, i + 1) - wbfine) * fak * sb14 +
+                                              w14 * (1.0 - sb14) * sbw;
+            }
+        break;
+    }
+    if (ctyp == OP1_BC) {
+        for (int i = 2; i <= ii; ++i)
+            for (int j = 2; j <= jj; ++j) {
+                const double ui = 0.25 * (u[IX(k2, j, i - 1)] + u[IX(k2, j, i)] +
+                                          u[IX(k3, j, i - 1)] + u[IX(k3, j, i)]);
+                const double vi = 0.25 * (v[IX(k2, j - 1, i)] + v[IX(k2, j, i)] +
+                                          v[IX(k3, j - 1, i)] + v[IX(k3, j, i)]);
+                const double ws = w[IX(kstag2, j, i)];
+                const double umagsqr = ui * ui + vi * vi + ws * ws;
+                p[IX(k2, j, i)] = pinf + fmin(0.0, 0.5 * rho * copysign(umagsqr, dir * ws));
+            }
+    } else if (ctyp == FIX_BC || ctyp == NOS_BC || ctyp == SLI_BC) {
+        for (int i = 1; i <= ii; ++i)
+            for (int j = 1; j <= jj; ++j) p[IX(k2, j, i)] = p[IX(k3, j, i)];
+    }
+#undef BF
+#undef DX
+#undef DY
+}
+
+/* bound_flow%bound(ilevel, u, v, w, p): src/core/bound_mod.F90:36-76 (the
+ * faces of each grid in order 1..6; the first boundary condition of each
+ * face).  pinf[6*ngrid]: the OP1 pressure parameter (NULL = 0). */
+void og_bound_flow(const og_topo *t, int level, double *u_, double *v_, double *w_,
+                   double *p_, const double *ubuf, const double *vbuf, const double *wbuf,
+                   const int64_t *bbo, const double *pinf, const double *bp_,
+                   const double *ddx_, const double *ddy_, const double *ddz_, double rho)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        if (level >= 0 && t->level[g - 1] != level) continue;
+        GDIMS(t, g);
+        const int64_t o3 = t->ip3[g - 1];
+        double *u = u_ + o3, *v = v_ + o3, *w = w_ + o3, *p = p_ + o3;
+        const double *bp = bp_ ? bp_ + o3 : NULL;
+        const double *ddx = ddx_ + t->ip1x[g - 1], *ddy = ddy_ + t->ip1y[g - 1],
+                     *ddz = ddz_ + t->ip1z[g - 1];
+        for (int f = 1; f <= 6; ++f) {
+            const int ctyp = t->bc[6 * (g - 1) + f - 1];
+            if (ctyp != FIX_BC && ctyp != OP1_BC && ctyp != NOS_BC && ctyp != SLI_BC &&
+                ctyp != PAR_BC)
+                continue;
+            const int64_t o = bbo[6 * (g - 1) + f - 1];
+            const double *ub = o >= 0 ? ubuf + o : NULL, *vb = o >= 0 ? vbuf + o : NULL,
+                         *wb = o >= 0 ? wbuf + o : NULL;
+            const double pf = pinf ? pinf[6 * (g - 1) + f - 1] : 0.0;
+            if (f <= 2)
+                bfront(f, ctyp, kk, jj, ii, u, v, w, p, ub, vb, wb, pf, bp, ddy, ddz, rho);
+            else if (f <= 4)
+                bright(f, ctyp, kk, jj, ii, u, v, w, p, ub, vb, wb, pf, bp, ddx, ddz, rho);
+            else
+                bbottom(f, ctyp, kk, jj, ii, u, v, w, p, ub, vb, wb, pf, bp, ddx, ddy, rho);
+        }
+    }
+}
+#undef BPV
+
 static int64_t field_len(const og_topo *t)
 {
     const int g = t->ngrid;
@@ -1467,9 +2001,31 @@ int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
                prm->ibtype ? bp : NULL);
     for (int l = t->maxlevel; l >= t->minlevel; --l)
         og_ftoc_uvwp(t, l, u, v, w, p, ddx, ddy, ddz, bpr);
-    double *fs[4] = {u, v, w, p};
-    for (int l = t->minlevel; l <= t->maxlevel; ++l)
-        og_exchange(t, c, nconn, l, 2, 1, 0, fs, 4);
+    /* per level, coarse to fine: parent, then bound_flow to fill the
+     * ghost layers at PAR and wall faces, then the exchange inside the
+     * level, which uses them (src/flow/pressuresolver_mod.F90:338-348); a
+     * caller that runs these itself stops after ftoc */
+    if (!prm->sync_by_caller) {
+        int64_t *bbo = malloc(sizeof(int64_t) * 6 * (size_t)t->ngrid);
+        const int64_t nb = og_bound_buffers(t, bbo);
+        const size_t nbz = (size_t)(nb > 0 ? nb : 1);
+        double *ub = calloc(nbz, sizeof(double)), *vb = calloc(nbz, sizeof(double)),
+               *wb = calloc(nbz, sizeof(double));
+        if (prm->ubuf) memcpy(ub, prm->ubuf, (size_t)nb * sizeof(double));
+        if (prm->vbuf) memcpy(vb, prm->vbuf, (size_t)nb * sizeof(double));
+        if (prm->wbuf) memcpy(wb, prm->wbuf, (size_t)nb * sizeof(double));
+        double *fs[4] = {u, v, w, p};
+        for (int l = t->minlevel; l <= t->maxlevel; ++l) {
+            og_parent_uvw(t, l, u, v, w, ub, vb, wb, bbo);
+            og_bound_flow(t, l, u, v, w, p, ub, vb, wb, bbo, prm->pinf, bp, ddx, ddy, ddz,
+                          prm->rho);
+            og_exchange(t, c, nconn, l, 2, 1, 0, fs, 4);
+        }
+        free(ub);
+        free(vb);
+        free(wb);
+        free(bbo);
+    }
     free(pbh);
     free(pbd);
     free(ipbb);

@@ -57,6 +57,11 @@ typedef struct og_params {
     int32_t sor_all_levels; /* 1: run SOR also on minlevel                 */
     const double *sdiv;   /* ghostcell divergence source SDIV (3-D field),  */
                           /*   NULL = zero (no STL cut cells)              */
+    int32_t sync_by_caller; /* 1: stop after ftoc(u, v, w, p); 0: run the  */
+                          /*   closing parent, bound_flow, connect         */
+    const double *ubuf, *vbuf, *wbuf; /* FIX/OP1 boundary values in the   */
+                          /*   og_bound_buffers layout (NULL = zero)       */
+    const double *pinf;   /* [ngrid*6] OP1 pressure parameter (NULL = 0)   */
 } og_params;
 
 typedef struct og_stats {
@@ -175,10 +180,29 @@ void og_ftoc_uvwp(const og_topo *t, int level, double *u, double *v,
                   double *w, double *p, const double *ddx, const double *ddy,
                   const double *ddz, const double *bp);
 
-/* src/flow/pressuresolver_mod.F90:160-357, single rank, all levels (the
- * final per-level parent/bound_flow of U, V, W, P belong to the caller;
- * the per-level 2-layer connect with corners is done here).  Work arrays
- * dp, hilf, rhs, res are full 3-D fields. */
+/* Velocity boundary buffers of the FIX, OP1 and PAR faces (the field
+ * buffers at ipbb, src/core/pointers_mod.F90:64-92): kk*jj / kk*ii / jj*ii
+ * values per face, first index fastest; bbo[6*ngrid] offsets (-1 = none);
+ * returns the total length. */
+int64_t og_bound_buffers(const og_topo *t, int64_t *bbo);
+/* parent(ilevel, u, v, w): src/ib/parent/parent1_mod.F90:296-583 with the
+ * staggered prolongation (prolong1/2, istag 1) and start_and_stop's icomp
+ * (src/ib/parent/parent_core_mod.F90:185-267), into the PAR face buffers */
+void og_parent_uvw(const og_topo *t, int level, const double *u, const double *v,
+                   const double *w, double *ubuf, double *vbuf, double *wbuf,
+                   const int64_t *bbo);
+/* bound_flow%bound(ilevel, u, v, w, p): src/flow/bound_flow_mod.F90 (FIX,
+ * OP1, NOS, SLI, PAR) over src/core/bound_mod.F90:36-76; level < 0: all;
+ * bp NULL = 1 */
+void og_bound_flow(const og_topo *t, int level, double *u, double *v, double *w,
+                   double *p, const double *ubuf, const double *vbuf, const double *wbuf,
+                   const int64_t *bbo, const double *pinf, const double *bp,
+                   const double *ddx, const double *ddy, const double *ddz, double rho);
+
+/* src/flow/pressuresolver_mod.F90:160-357, single rank, all levels,
+ * ending per level with parent(u, v, w), bound_flow and the 2-layer
+ * connect with corners (unless prm->sync_by_caller).  Work arrays dp,
+ * hilf, rhs, res are full 3-D fields. */
 int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
                const og_params *prm, double *u, double *v, double *w,
                double *p, double dt, const double *bp,

@@ -912,3 +912,259 @@ def test_sor_lagged_ghosts_diverge_at_high_omega(oracle_mod):
                              sor_all_levels=1)["history"]
     assert hist[1.2][-1] < 1e-3 * hist[1.2][0]
     assert hist[1.7][-1] > 1e6 * hist[1.7][0]
+
+
+# ---------------------------------------------------------------- closing sync
+def _linear_uvw(g, cu=(3.0, 5.0), cv=(-2.0, 7.0), cw=(1.0, 3.0)):
+    """u, v, w linear along their own staggered axis (u = a + b x at the
+    u-face positions, constant in y and z, ...), on every grid and every
+    stored position; dyadic grids keep the arithmetic exact."""
+    u, v, w = g.zeros(), g.zeros(), g.zeros()
+    for b in range(1, g.ngrid + 1):
+        sx = slice(g.ip1x[b - 1], g.ip1x[b - 1] + g.ii[b - 1])
+        sy = slice(g.ip1y[b - 1], g.ip1y[b - 1] + g.jj[b - 1])
+        sz = slice(g.ip1z[b - 1], g.ip1z[b - 1] + g.kk[b - 1])
+        g.block_view(u, b)[...] = (cu[0] + cu[1] * g.xstag[sx])[:, None, None]
+        g.block_view(v, b)[...] = (cv[0] + cv[1] * g.ystag[sy])[None, :, None]
+        g.block_view(w, b)[...] = (cw[0] + cw[1] * g.zstag[sz])[None, None, :]
+    return u, v, w
+
+
+def _face_buf(o, g, buf, b, f):
+    """The 2-D view of grid b's buffer of face f (1-based): (k, j), (k, i) or
+    (j, i), first index fastest."""
+    off = int(o.bbo[6 * (b - 1) + f - 1])
+    assert off >= 0
+    kk, jj, ii = int(g.kk[b - 1]), int(g.jj[b - 1]), int(g.ii[b - 1])
+    n1, n2 = ((kk, jj), (kk, jj), (kk, ii), (kk, ii), (jj, ii), (jj, ii))[f - 1]
+    return buf[off:off + n1 * n2].reshape(n2, n1).T
+
+
+@pytest.mark.parametrize("topo", ["two_level", "three_level"])
+def test_parent_uvw_staggered_prolongation_exact(oracle_mod, topo):
+    """parent(u, v, w) (src/ib/parent/parent1_mod.F90:296-583): for fields
+    linear along their own staggered axis, the prolonged PAR face buffers
+    hold the field's value at the fine face positions exactly -- the normal
+    component the value on the block's boundary plane (start_and_stop's
+    icomp shift at back faces), a tangential component staggered along the
+    face the linear interpolation at the fine faces (prolong istag 1)."""
+    from cases import three_level, two_level
+    g = two_level(periodic=False) if topo == "two_level" else three_level()
+    o = oracle_mod.Oracle(g)
+    u, v, w = _linear_uvw(g)
+    nchecked = 0
+    for lvl in range(g.minlevel + 1, g.maxlevel + 1):
+        ub, vb, wb = o.bound_buffer(), o.bound_buffer(), o.bound_buffer()
+        o.parent_uvw(lvl, u, v, w, ub, vb, wb)
+        for b in range(1, g.ngrid + 1):
+            if g.level[b - 1] != lvl:
+                continue
+            xs = g.xstag[g.ip1x[b - 1]:g.ip1x[b - 1] + g.ii[b - 1]]
+            ys = g.ystag[g.ip1y[b - 1]:g.ip1y[b - 1] + g.jj[b - 1]]
+            zs = g.zstag[g.ip1z[b - 1]:g.ip1z[b - 1] + g.kk[b - 1]]
+            ii, jj, kk = len(xs), len(ys), len(zs)
+            for f in range(1, 7):
+                if g.bc[b - 1, f - 1] != 8:
+                    continue
+                U, V, W = (_face_buf(o, g, x, b, f) for x in (ub, vb, wb))
+                if f <= 2:
+                    xb = xs[1] if f == 1 else xs[ii - 3]
+                    assert np.all(U == 3.0 + 5.0 * xb)
+                    assert np.array_equal(V, np.broadcast_to((-2.0 + 7.0 * ys)[None, :], V.shape))
+                    assert np.array_equal(W, np.broadcast_to((1.0 + 3.0 * zs)[:, None], W.shape))
+                elif f <= 4:
+                    yb = ys[1] if f == 3 else ys[jj - 3]
+                    assert np.array_equal(U, np.broadcast_to((3.0 + 5.0 * xs)[None, :], U.shape))
+                    assert np.all(V == -2.0 + 7.0 * yb)
+                    assert np.array_equal(W, np.broadcast_to((1.0 + 3.0 * zs)[:, None], W.shape))
+                else:
+                    zb = zs[1] if f == 5 else zs[kk - 3]
+                    assert np.array_equal(U, np.broadcast_to((3.0 + 5.0 * xs)[None, :], U.shape))
+                    assert np.array_equal(V, np.broadcast_to((-2.0 + 7.0 * ys)[:, None], V.shape))
+                    assert np.all(W == 1.0 + 3.0 * zb)
+                nchecked += 1
+    assert nchecked >= 8
+
+
+def test_par_tangential_ghosts_linear_exact(oracle_mod):
+    """parent then bound_flow at PAR faces (src/flow/bound_flow_mod.F90,
+    PAR branches): with fluid everywhere, a tangential component linear
+    along the face normal gets its exact value at the ghost cell centre
+    (the buffer holds the coarse value one fine cell further out, the
+    extrapolation goes half a cell back) -- except the y faces' w, which the
+    reference extrapolates from w(j2) instead of w(j3) (bright): that ghost
+    gets the value on the face plane instead.  One run per face
+    orientation, the two tangential components linear along its normal."""
+    from cases import two_level
+    g = two_level(periodic=False)
+    o = oracle_mod.Oracle(g)
+    nchecked = 0
+    for ax in range(3):
+        u, v, w, p = g.zeros(), g.zeros(), g.zeros(), g.zeros()
+        for b in range(1, g.ngrid + 1):
+            c = (g.x, g.y, g.z)[ax]
+            off = (g.ip1x, g.ip1y, g.ip1z)[ax][b - 1]
+            n = (g.ii, g.jj, g.kk)[ax][b - 1]
+            shape = [1, 1, 1]
+            shape[ax] = n
+            t = c[off:off + n].reshape(shape)
+            comps = [(u, 1.0, 2.0), (v, -3.0, 6.0), (w, 5.0, -4.0)]
+            for q, (arr, a0, a1) in enumerate(comps):
+                if q != ax:
+                    g.block_view(arr, b)[...] = a0 + a1 * t
+        before = [a.copy() for a in (u, v, w)]
+        ub, vb, wb = o.bound_buffer(), o.bound_buffer(), o.bound_buffer()
+        o.parent_uvw(2, u, v, w, ub, vb, wb)
+        o.bound_flow(2, u, v, w, p, ub, vb, wb)
+        for b in range(1, g.ngrid + 1):
+            if g.level[b - 1] != 2:
+                continue
+            F = [g.block_view(a, b) for a in (u, v, w)]
+            F0 = [g.block_view(a, b) for a in before]
+            n = F[0].shape[ax]
+            for hi in (0, 1):
+                if g.bc[b - 1, 2 * ax + hi] != 8:
+                    continue
+                gi = n - 2 if hi else 1          # ghost layer i2 (0-based)
+                # interior tangential positions, without the normal-velocity
+                # planes of the other faces (their PAR correction runs first
+                # for faces with a smaller number and moves those values)
+                sl = [slice(2, -3)] * 3
+                sl[ax] = gi
+                sl = tuple(sl)
+                for q in range(3):
+                    if q == ax:
+                        continue
+                    if ax == 1 and q == 2:       # bright's w: the face-plane value
+                        nb = list(sl)
+                        nb[ax] = gi + (-1 if hi else 1)
+                        want = 0.5 * (F0[q][sl] + F0[q][tuple(nb)])
+                    else:
+                        want = F0[q][sl]
+                    assert np.array_equal(F[q][sl], want), (ax, hi, q)
+                nchecked += 1
+    assert nchecked >= 6
+
+
+def test_par_normal_correction_conserves_flux(oracle_mod):
+    """The PAR normal-velocity correction (bound_flow_mod, PAR branches):
+    over each 2x2 group of fine boundary faces whose ghost cell is fluid,
+    the area-weighted sum of the corrected normal velocities equals the
+    parent's (prolonged, piecewise constant) value times the group's area,
+    whichever of the four faces are blocked -- the blocked ones keep their
+    values, the open ones absorb the difference."""
+    from cases import two_level
+    g = two_level(periodic=False)
+    o = oracle_mod.Oracle(g)
+    rng = np.random.default_rng(301)
+    u, v, w, p = (rng.uniform(-1, 1, g.n3d) for _ in range(4))
+    bp = (rng.random(g.n3d) > 0.3).astype(np.float64)
+    ub, vb, wb = o.bound_buffer(), o.bound_buffer(), o.bound_buffer()
+    o.parent_uvw(2, u, v, w, ub, vb, wb)
+    o.bound_flow(2, u, v, w, p, ub, vb, wb, bp=bp)
+    ngroups = 0
+    for b in range(1, g.ngrid + 1):
+        if g.level[b - 1] != 2:
+            continue
+        ii, jj, kk = int(g.ii[b - 1]), int(g.jj[b - 1]), int(g.kk[b - 1])
+        d = [g.ddx[g.ip1x[b - 1]:][:ii], g.ddy[g.ip1y[b - 1]:][:jj], g.ddz[g.ip1z[b - 1]:][:kk]]
+        B = g.block_view(bp, b)
+        for f in range(1, 7):
+            if g.bc[b - 1, f - 1] != 8:
+                continue
+            ax, hi = (f - 1) // 2, (f - 1) % 2
+            F = g.block_view((u, v, w)[ax], b)
+            buf = _face_buf(o, g, (ub, vb, wb)[ax], b, f)
+            n = (ii, jj, kk)[ax]
+            c2, c3 = (n - 2, n - 3) if hi else (1, 2)     # ghost i2, interior i3
+            cs = n - 3 if hi else 1                       # normal-velocity plane
+            t1, t2 = [a for a in range(3) if a != ax]     # face axes, buffer (t2-fast? no)
+            # buffer index order: x faces (k, j), y (k, i), z (j, i)
+            first, second = (2, 1) if ax == 0 else ((2, 0) if ax == 1 else (1, 0))
+            for s in range(2, (ii, jj, kk)[second] - 3, 2):
+                for r in range(2, (ii, jj, kk)[first] - 3, 2):
+                    def at(arr, nrm, rr, ss):
+                        idx = [0, 0, 0]
+                        idx[ax], idx[first], idx[second] = nrm, rr, ss
+                        return arr[tuple(idx)]
+                    if at(B, c2, r, s) <= 0.5:
+                        continue
+                    cells = [(r, s), (r, s + 1), (r + 1, s), (r + 1, s + 1)]
+                    if not any(at(B, c2, rr, ss) * at(B, c3, rr, ss) for rr, ss in cells):
+                        continue
+                    area = [d[first][rr] * d[second][ss] for rr, ss in cells]
+                    flux = sum(a * at(F, cs, rr, ss) for a, (rr, ss) in zip(area, cells))
+                    assert buf[r, s] == buf[r + 1, s + 1]
+                    want = buf[r, s] * sum(area)
+                    assert abs(flux - want) <= 1e-13 * max(1.0, abs(want) / sum(area)) * sum(area)
+                    ngroups += 1
+    assert ngroups > 50
+
+
+def test_bound_flow_wall_types(oracle_mod):
+    """bound_flow at the wall faces of one closed block, faces FIX, OP1,
+    NOS, SLI, FIX, OP1 (src/flow/bound_flow_mod.F90): FIX puts the
+    prescribed normal velocity on the wall and makes the tangential ghost
+    and interior average to the prescribed value; OP1 copies the normal
+    velocity outward, keeps tangential ghosts zero-gradient at outflow and
+    averaging to the prescribed value at inflow, and sets the ghost
+    pressure to pinf at outflow and below it at inflow; NOS zeroes the
+    wall velocities, SLI the normal one and copies the tangential ones; the
+    ghost pressure of FIX, NOS and SLI copies the interior.  Dyadic values
+    keep every relation exact; positions next to edges are left out
+    (faces later in the order 1..6 overwrite them)."""
+    from mgp.grids import uniform_box
+    g = uniform_box((1, 1, 1), (8, 8, 8), periodic=(False, False, False))
+    g.bc[0, :] = [2, 3, 5, 6, 2, 3]
+    o = oracle_mod.Oracle(g)
+    rng = np.random.default_rng(302)
+    dy = lambda n: rng.integers(-32, 33, n) / 8.0
+    u, v, w, p = (dy(g.n3d) for _ in range(4))
+    ub, vb, wb = (dy(o.nbb) for _ in range(3))
+    pinf = dy(6)
+    rho = 1.0
+    o.bound_flow(-1, u, v, w, p, ub, vb, wb, pinf=pinf, rho=rho)
+    U, V, W, P = (g.block_view(a, 1) for a in (u, v, w, p))
+    n = U.shape[0]
+    s = slice(2, n - 3)     # tangential range without the faces' normal planes
+    # face 1 (FIX, -x): i2 = 1, i3 = 2, istag2 = 1 (0-based)
+    B = [_face_buf(o, g, x, 1, 1) for x in (ub, vb, wb)]   # (k, j)
+    assert np.array_equal(U[1, s, s], B[0][s, s].T)
+    assert np.array_equal(0.5 * (V[1, s, s] + V[2, s, s]), B[1][s, s].T)
+    assert np.array_equal(0.5 * (W[1, s, s] + W[2, s, s]), B[2][s, s].T)
+    assert np.array_equal(P[1, s, s], P[2, s, s])
+    # face 2 (OP1, +x): istag1 = n-2, istag2 = n-3, i2 = n-2, i3 = n-3
+    assert np.array_equal(U[n - 2, s, s], U[n - 3, s, s])
+    B = [_face_buf(o, g, x, 1, 2) for x in (ub, vb, wb)]
+    for j in range(2, n - 3):
+        for k in range(2, n - 3):
+            # SIGN(1, +0.0) is +1: a zero flux counts as outflow here
+            out_v = U[n - 3, j, k] + U[n - 3, j + 1, k] >= 0
+            out_w = U[n - 3, j, k] + U[n - 3, j, k + 1] >= 0
+            if out_v:
+                assert V[n - 2, j, k] == V[n - 3, j, k]
+            else:
+                assert 0.5 * (V[n - 2, j, k] + V[n - 3, j, k]) == B[1][k, j]
+            if out_w:
+                assert W[n - 2, j, k] == W[n - 3, j, k]
+            else:
+                assert 0.5 * (W[n - 2, j, k] + W[n - 3, j, k]) == B[2][k, j]
+            if U[n - 3, j, k] >= 0:
+                assert P[n - 2, j, k] == pinf[1]
+            elif U[n - 3, j, k] < 0:
+                assert P[n - 2, j, k] < pinf[1]
+    # face 3 (NOS, -y): jstag2 = j2 = 1
+    assert not V[s, 1, s].any() and not U[s, 1, s].any() and not W[s, 1, s].any()
+    assert np.array_equal(P[s, 1, s], P[s, 2, s])
+    # face 4 (SLI, +y): jstag2 = n-3, j2 = n-2
+    assert not V[s, n - 3, s].any()
+    assert np.array_equal(U[s, n - 2, s], U[s, n - 3, s])
+    assert np.array_equal(W[s, n - 2, s], W[s, n - 3, s])
+    assert np.array_equal(P[s, n - 2, s], P[s, n - 3, s])
+    # face 5 (FIX, -z): kstag2 = k2 = 1; buffer (j, i)
+    B = [_face_buf(o, g, x, 1, 5) for x in (ub, vb, wb)]
+    assert np.array_equal(W[s, s, 1], B[2][s, s].T)
+    assert np.array_equal(0.5 * (U[s, s, 1] + U[s, s, 2]), B[0][s, s].T)
+    assert np.array_equal(0.5 * (V[s, s, 1] + V[s, s, 2]), B[1][s, s].T)
+    # face 6 (OP1, +z): normal copied outward
+    assert np.array_equal(W[s, s, n - 2], W[s, s, n - 3])
