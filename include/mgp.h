@@ -83,6 +83,11 @@ typedef struct mgp_config {
                           /*   reference's hard-coded default-real 0.92,    */
                           /*   (double)0.92f (src/flow/sip_hyperplane_mod.  */
                           /*   F90:123, 202); a negative value gives 0      */
+    mgletint sync_by_caller; /* 0: mgpoisl ends with the reference's       */
+                          /*   closing sync per level (parent of U, V, W,   */
+                          /*   bound_flow, connect; src/flow/              */
+                          /*   pressuresolver_mod.F90:338-348); 1: it stops */
+                          /*   after ftoc and the caller runs them          */
 } mgp_config;
 
 typedef struct mgp_stats {
@@ -120,6 +125,23 @@ int mgp_init_mask(const mgp_grids *grids, const mgp_config *cfg,
  * src/flow/gc_flowstencils_mod.F90:1486-1530).  NULL resets it to zero, the
  * value without cut cells.  Only for handles with ibtype 1. */
 int mgp_set_sdiv(mgp_handle *h, const mgletreal *sdiv);
+
+/* Velocity boundary buffers that bound_flow reads (the field buffers at
+ * ipbb, src/core/pointers_mod.F90:64-92, read by f1%get_buffer in
+ * src/flow/bound_flow_mod.F90:62, 298, 536): one kk*jj (faces 1, 2), kk*ii
+ * (3, 4) or jj*ii (5, 6) array per FIX, OP1 or PAR face, first index
+ * fastest, faces in grid and face order.  offsets[6*(igrid-1) + iface-1]
+ * (may be NULL) gets each face's start, -1 for faces without a buffer;
+ * *n the total length. */
+int mgp_bound_layout(mgp_handle *h, int64_t *offsets, int64_t *n);
+
+/* The caller's FIX / OP1 boundary values in that layout (host memory, the
+ * values U%buffers, V%buffers, W%buffers hold) and the OP1 pressure per
+ * grid and face (pinf of get_bcprms, src/flow/bound_flow_mod.F90:229-231;
+ * [ngrid*6]).  NULL sets an array to zero (the start value).  PAR entries
+ * are rewritten by the closing parent of every mgpoisl. */
+int mgp_set_bound(mgp_handle *h, const mgletreal *ubuf, const mgletreal *vbuf,
+                  const mgletreal *wbuf, const mgletreal *pinf);
 
 /* mgpoisl(u, v, w, p, dt, ittot, irk) (src/flow/pressuresolver_mod.F90:
  * 160-357): projection of host arrays u, v, w, p in place. */

@@ -100,6 +100,11 @@ const int RESCUE_NBR[26][7] = {
 
 bool is_con(int c) { return c == 7 || c == 19; }
 constexpr int PAR_BC = 8;
+// flow boundary codes bound_flow acts on (src/flow/bound_flow_mod.F90)
+constexpr int FIX_BC = 2, OP1_BC = 3, NOS_BC = 5, SLI_BC = 6;
+bool is_flow_bc(int c) {
+    return c == FIX_BC || c == OP1_BC || c == NOS_BC || c == SLI_BC || c == PAR_BC;
+}
 bool is_wall_lo(int c) { return c == 2 || c == 5 || c == 6 || c == 19; }
 bool is_wall_hi(int c) { return c == 2 || c == 5 || c == 6; }
 
@@ -163,6 +168,13 @@ struct mgp_handle {
     // which DP keeps in the ghost cells no exchange or bound_pressure sets
     DBuf d_hsh, d_dsh;
     std::set<int> level_par;   // levels with PAR faces
+    // closing sync of mgpoisl: velocity boundary buffers of the FIX, OP1
+    // and PAR faces (offsets per grid and face, -1 none), OP1 pressure per
+    // grid and face, and the levels with a face bound_flow acts on
+    std::vector<int64_t> bbo;
+    int64_t nbb = 0;
+    DBuf d_bbo, d_ubnd, d_vbnd, d_wbnd, d_pinf;
+    std::set<int> level_flow;
     // adjoint: transposed factors and 1-D coefficients (built on first use)
     DBuf d_ft[6], d_c64t[6], d_cTt[6];
     std::vector<double> fold_host[6];
@@ -790,6 +802,8 @@ int check_cfg(const mgp_config &c) {
     if (c.ibtype < 0 || c.ibtype > 1) return fail(MGP_ERR_ARG, "invalid ibtype %d", c.ibtype);
     if (!(c.rho > 0.0)) return fail(MGP_ERR_ARG, "invalid rho %g", c.rho);
     if (!(c.alfa <= 1.0)) return fail(MGP_ERR_ARG, "invalid SIP alfa %g", c.alfa);
+    if (c.sync_by_caller < 0 || c.sync_by_caller > 1)
+        return fail(MGP_ERR_ARG, "invalid sync_by_caller %d", c.sync_by_caller);
     return MGP_OK;
 }
 
@@ -1063,7 +1077,31 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
             upload(h->d_bpw, bpw);
             upload(h->d_bpf, bpf);
         }
-        if (!bpd.empty() && (h->cfg.ibtype == 1)) upload(h->d_bpsp, bpd);
+        // BP in the reference layout: the ghostcell operators and the PAR
+        // faces of bound_flow read it
+        if (!bpd.empty()) upload(h->d_bpsp, bpd);
+
+        // velocity boundary buffers (field buffers at ipbb, src/core/
+        // pointers_mod.F90:64-92): kk*jj, kk*ii, jj*ii values per FIX, OP1
+        // or PAR face, first index fastest, in grid and face order
+        h->bbo.assign(6 * (size_t)n, -1);
+        h->nbb = 0;
+        for (int g = 0; g < n; ++g)
+            for (int f = 0; f < 6; ++f) {
+                const int c = h->bc[6 * g + f];
+                if (is_flow_bc(c)) h->level_flow.insert(h->level[g]);
+                if (c != FIX_BC && c != OP1_BC && c != PAR_BC) continue;
+                h->bbo[6 * g + f] = h->nbb;
+                h->nbb += f < 2 ? (int64_t)h->kk[g] * h->jj[g]
+                        : f < 4 ? (int64_t)h->kk[g] * h->ii[g] : (int64_t)h->jj[g] * h->ii[g];
+            }
+        upload(h->d_bbo, h->bbo);
+        for (DBuf *d : {&h->d_ubnd, &h->d_vbnd, &h->d_wbnd}) {
+            d->alloc(std::max<int64_t>(h->nbb, 1) * sizeof(double));
+            HC(hipMemsetAsync(d->p, 0, d->n, h->stream));
+        }
+        h->d_pinf.alloc(6 * (size_t)n * sizeof(double));
+        HC(hipMemsetAsync(h->d_pinf.p, 0, h->d_pinf.n, h->stream));
 
         // 1-D metrics and coefficients (src/ib/noib_mod.F90:141-152,
         // fold-in zeroing :178-244; reciprocals src/core/corefields_mod.F90:
@@ -1534,9 +1572,26 @@ int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
         if (lv->first != h.level_blocks.begin()->first)
             launch_ftoc_uvwp(h.topo(lv->first), u, v, w, p, c.ibtype == 1,
                              c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr, dd, s);
-    double *fs[4] = {u, v, w, p};
-    for (int l = h.minlevel; l <= h.maxlevel; ++l)
-        if (h.level_blocks.count(l)) exchange(h, l, 2, 1, 0, fs, 4);
+    // per level, coarse to fine: U, V, W at the PAR faces from the parent,
+    // bound_flow, and the exchange inside the level, which takes the ghost
+    // values bound_flow set (:338-348); a caller that runs these itself
+    // asks to stop after ftoc
+    if (!c.sync_by_caller) {
+        double *fs[4] = {u, v, w, p};
+        for (int l = h.minlevel; l <= h.maxlevel; ++l) {
+            if (!h.level_blocks.count(l)) continue;
+            const Topo tl = h.topo(l);
+            if (h.level_par.count(l))
+                launch_parent_uvw(tl, u, v, w, h.d_ubnd.as<double>(), h.d_vbnd.as<double>(),
+                                  h.d_wbnd.as<double>(), h.d_bbo.as<int64_t>(), s);
+            if (h.level_flow.count(l))
+                launch_bound_flow(tl, u, v, w, p, h.d_ubnd.as<double>(), h.d_vbnd.as<double>(),
+                                  h.d_wbnd.as<double>(), h.d_bbo.as<int64_t>(),
+                                  h.d_pinf.as<double>(), h.d_bpsp.as<double>(), dd[0], dd[1],
+                                  dd[2], c.rho, s);
+            exchange(h, l, 2, 1, 0, fs, 4);
+        }
+    }
     HC(hipGetLastError());
     return finish_stats(h, ms, st);
 }
@@ -1898,6 +1953,39 @@ int mgp_set_sdiv(mgp_handle *h, const mgletreal *sdiv) {
         if (!h->d_sdiv.p) h->d_sdiv.alloc((size_t)h->n3d * sizeof(double));
         HC(hipMemcpyAsync(h->d_sdiv.p, sdiv, (size_t)h->n3d * sizeof(double),
                           hipMemcpyHostToDevice, h->stream));
+        HC(hipStreamSynchronize(h->stream));
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    return MGP_OK;
+}
+
+int mgp_bound_layout(mgp_handle *h, int64_t *offsets, int64_t *n) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (!n) return fail(MGP_ERR_ARG, "n is NULL");
+    *n = h->nbb;
+    if (offsets) std::copy(h->bbo.begin(), h->bbo.end(), offsets);
+    return MGP_OK;
+}
+
+int mgp_set_bound(mgp_handle *h, const mgletreal *ubuf, const mgletreal *vbuf,
+                  const mgletreal *wbuf, const mgletreal *pinf) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    try {
+        const mgletreal *src[3] = {ubuf, vbuf, wbuf};
+        DBuf *dst[3] = {&h->d_ubnd, &h->d_vbnd, &h->d_wbnd};
+        for (int q = 0; q < 3; ++q) {
+            if (src[q] && h->nbb > 0)
+                HC(hipMemcpyAsync(dst[q]->p, src[q], (size_t)h->nbb * sizeof(double),
+                                  hipMemcpyHostToDevice, h->stream));
+            else if (!src[q])
+                HC(hipMemsetAsync(dst[q]->p, 0, dst[q]->n, h->stream));
+        }
+        if (pinf)
+            HC(hipMemcpyAsync(h->d_pinf.p, pinf, 6 * (size_t)h->ngrid * sizeof(double),
+                              hipMemcpyHostToDevice, h->stream));
+        else
+            HC(hipMemsetAsync(h->d_pinf.p, 0, h->d_pinf.n, h->stream));
         HC(hipStreamSynchronize(h->stream));
     } catch (const HipError &e) {
         return fail(MGP_ERR_HIP, "%s", e.msg.c_str());

@@ -206,6 +206,18 @@ void launch_parent_sp(const Topo &t, const double *f, double *pbuf,
 void launch_bound_sp(const Topo &t, double *p, const double *pbuf,
                      const double *bp, int bpv, const double *const metric[6],
                      hipStream_t s);
+// closing sync of mgpoisl (reference layout): U, V, W of the parents into
+// the PAR face buffers, then the ghost values of U, V, W, P at the FIX,
+// OP1, NOS, SLI and PAR faces (bbo: buffer offset per grid and face, -1
+// none; pinf: OP1 pressure per grid and face; bp may be null: all fluid)
+void launch_parent_uvw(const Topo &t, const double *u, const double *v, const double *w,
+                       double *ubuf, double *vbuf, double *wbuf, const int64_t *bbo,
+                       hipStream_t s);
+void launch_bound_flow(const Topo &t, double *u, double *v, double *w, double *p,
+                       const double *ubuf, const double *vbuf, const double *wbuf,
+                       const int64_t *bbo, const double *pinf, const double *bp,
+                       const double *ddx, const double *ddy, const double *ddz, double rho,
+                       hipStream_t s);
 // gc: ghostcell restriction of P (bp may be null: all fluid)
 void launch_ftoc_uvwp(const Topo &t, double *u, double *v, double *w,
                       double *p, int gc, const double *bp,

@@ -37,6 +37,9 @@ MODULE mgp_bindc_mod
         INTEGER(C_INT) :: sor_all_levels = 0, check_every = 4
         ! SIP alfa; 0 = the reference's hard-coded default-real 0.92
         REAL(C_DOUBLE) :: alfa = 0.0_C_DOUBLE
+        ! 1: mgpoisl stops after ftoc, the caller runs parent, bound_flow
+        ! and connect of U, V, W, P itself
+        INTEGER(C_INT) :: sync_by_caller = 0
     END TYPE mgp_config_t
 
     TYPE, BIND(C), PUBLIC :: mgp_stats_t
@@ -47,7 +50,8 @@ MODULE mgp_bindc_mod
 
     PUBLIC :: mgp_init, mgp_init_mask, mgp_mgpoisl, mgp_mgpoisl_device, &
         mgp_finish, mgp_last_error, mgp_set_iterations, mgp_error_message, &
-        mgp_set_sdiv, mgp_connect_device, mgp_tstle4_device, mgp_rkstep_device
+        mgp_set_sdiv, mgp_connect_device, mgp_tstle4_device, mgp_rkstep_device, &
+        mgp_bound_layout, mgp_set_bound
 
     INTERFACE
         INTEGER(C_INT) FUNCTION mgp_init(grids, cfg, dx, dy, dz, ddx, ddy, &
@@ -109,6 +113,20 @@ MODULE mgp_bindc_mod
             IMPORT :: C_INT, C_PTR
             TYPE(C_PTR), VALUE :: handle, sdiv
         END FUNCTION mgp_set_sdiv
+
+        ! offsets: C_LOC of INTEGER(C_INT64_T) (6*ngrid) or C_NULL_PTR
+        INTEGER(C_INT) FUNCTION mgp_bound_layout(handle, offsets, n) &
+                BIND(C, name="mgp_bound_layout")
+            IMPORT :: C_INT, C_PTR, C_INT64_T
+            TYPE(C_PTR), VALUE :: handle, offsets
+            INTEGER(C_INT64_T), INTENT(out) :: n
+        END FUNCTION mgp_bound_layout
+
+        INTEGER(C_INT) FUNCTION mgp_set_bound(handle, ubuf, vbuf, wbuf, pinf) &
+                BIND(C, name="mgp_set_bound")
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: handle, ubuf, vbuf, wbuf, pinf
+        END FUNCTION mgp_set_bound
 
         ! fields: C_LOC of an array of nfield device pointers
         INTEGER(C_INT) FUNCTION mgp_connect_device(handle, level, layers, &

@@ -62,7 +62,7 @@ class PressureSolver:
     def __init__(self, grids: Grids, type="sip", ninner=5, nouter=9, nouter_min=0,
                  epcorr=0.0, omega=OMEGA_DEFAULT, rho=1.0, ib="noib", precision="fp64",
                  eps_rel=0.0, check_every=4, bp=None, blocked=None, sor_all_levels=False,
-                 alfa=0.0):
+                 alfa=0.0, sync_by_caller=False):
         self.lib = _lib.load()
         self.grids = g = grids
         t = type.lower()
@@ -84,7 +84,7 @@ class PressureSolver:
                                _ptr(k[11], C.c_int32) if nested else None)
         self.cfg = MgpConfig(_TYPES[t], ninner, nouter, nouter_min, epcorr, omega, rho,
                              _IB[ib], _PREC[precision], eps_rel, int(bool(sor_all_levels)),
-                             check_every, alfa)
+                             check_every, alfa, int(bool(sync_by_caller)))
         h = C.c_void_p()
         if blocked is not None:
             blocked = np.ascontiguousarray(blocked, dtype=np.int8)
@@ -200,6 +200,35 @@ class PressureSolver:
         if a.size != self.grids.n3d:
             raise MgpError("SDIV must have the 3-D length")
         _lib.check(self.lib.mgp_set_sdiv(self.h, _ptr(a)), "mgp_set_sdiv")
+
+    def bound_layout(self):
+        """(offsets [ngrid*6] int64, n): where the FIX, OP1 and PAR faces'
+        velocity boundary values start in the buffers of set_bound (-1:
+        none) and their total length."""
+        n = C.c_int64()
+        _lib.check(self.lib.mgp_bound_layout(self.h, None, C.byref(n)), "mgp_bound_layout")
+        off = np.zeros(6 * self.grids.ngrid, dtype=np.int64)
+        _lib.check(self.lib.mgp_bound_layout(self.h, _ptr(off, C.c_int64), C.byref(n)),
+                   "mgp_bound_layout")
+        return off, n.value
+
+    def set_bound(self, u=None, v=None, w=None, pinf=None):
+        """FIX / OP1 boundary values of U, V, W (bound_layout() order, float64
+        of its length) and the OP1 pressure per grid and face ([ngrid*6]);
+        None sets an array to zero."""
+        _, n = self.bound_layout()
+        arrs = []
+        for a, m, nm in ((u, n, "u"), (v, n, "v"), (w, n, "w"),
+                         (pinf, 6 * self.grids.ngrid, "pinf")):
+            if a is None:
+                arrs.append(None)
+                continue
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.size != m:
+                raise MgpError(f"set_bound: {nm} must have {m} values")
+            arrs.append(a)
+        _lib.check(self.lib.mgp_set_bound(self.h, *[_ptr(a) if a is not None else None
+                                                    for a in arrs]), "mgp_set_bound")
 
     def tstle4(self, u, v, w, p, g, gmol, ilesmodel=0, gradp=None):
         """Momentum right-hand side of one RK stage (tstle4, no immersed

@@ -31,7 +31,8 @@ class MgpConfig(C.Structure):
                 ("nouter_min", C.c_int32), ("epcorr", C.c_double), ("omega", C.c_double),
                 ("rho", C.c_double), ("ibtype", C.c_int32), ("precision", C.c_int32),
                 ("eps_rel", C.c_double), ("sor_all_levels", C.c_int32),
-                ("check_every", C.c_int32), ("alfa", C.c_double)]
+                ("check_every", C.c_int32), ("alfa", C.c_double),
+                ("sync_by_caller", C.c_int32)]
 
 
 class MgpStats(C.Structure):
@@ -87,6 +88,8 @@ def load():
                                C.POINTER(vp), C.c_int32],
         "mgp_divcal_device": [vp, vp, vp, vp, vp, C.c_double],
         "mgp_set_sdiv": [vp, _dp],
+        "mgp_bound_layout": [vp, _i64p, _i64p],
+        "mgp_set_bound": [vp, _dp, _dp, _dp, _dp],
         "mgp_time_kernels": [vp, C.c_int32, _dp],
         "mgp_plan_shape": [vp, C.c_int32, _i32p, _dp],
         "mgp_tstle4_device": [vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_int32, _dp],

@@ -60,12 +60,17 @@ def test_omega_default_is_the_real32_literal():
     assert "omega = REAL(1.1, C_DOUBLE)" in src
 
 
-def test_alfa_config_field_layout():
-    """mgp_config ends with alfa (0 = the reference's (double)0.92f); the
-    ctypes and Fortran mirrors carry it."""
+def test_config_tail_field_layout():
+    """mgp_config ends with alfa (0 = the reference's (double)0.92f) and
+    sync_by_caller (0 = the library runs the closing sync); the ctypes and
+    Fortran mirrors carry both in that order."""
+    import ctypes
     from mgp import _lib
-    assert _lib.MgpConfig._fields_[-1] == ("alfa", __import__("ctypes").c_double)
+    assert _lib.MgpConfig._fields_[-2:] == [("alfa", ctypes.c_double),
+                                            ("sync_by_caller", ctypes.c_int32)]
     src = (ROOT / "mglet-base_amd" / "fortran" / "mgp_bindc_mod.F90").read_text()
-    assert "REAL(C_DOUBLE) :: alfa = 0.0_C_DOUBLE" in src
+    assert src.index("REAL(C_DOUBLE) :: alfa = 0.0_C_DOUBLE") < \
+        src.index("INTEGER(C_INT) :: sync_by_caller = 0") < src.index("END TYPE mgp_config_t")
     hdr = (ROOT / "include" / "mgp.h").read_text()
-    assert "mgletreal alfa;" in hdr
+    assert hdr.index("mgletreal alfa;") < hdr.index("mgletint sync_by_caller;") < \
+        hdr.index("} mgp_config;")

@@ -25,6 +25,9 @@ def test_errors_without_handle():
     assert lib.mgp_finish(None) == 5
     assert b"not initialised" in lib.mgp_last_error()
     assert lib.mgp_mgpoisl(None, None, None, None, None, 1.0, 0, 0, None) == 5
+    n = C.c_int64(-1)
+    assert lib.mgp_bound_layout(None, None, C.byref(n)) == 5
+    assert lib.mgp_set_bound(None, None, None, None, None) == 5
 
 
 def test_invalid_config_rejected():
