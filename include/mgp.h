@@ -219,6 +219,17 @@ int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
                       const mgletreal *v, const mgletreal *w,
                       mgletreal fak);
 
+/* mgp_divcal_device without allocating: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * intermediate divergence lives in a static device array of 8,388,608 FP64
+ * values (64 MiB, one per GPU, held until the process ends); a handle with
+ * more interior cells gets MGP_ERR_ARG.  Every call on a GPU shares that
+ * array: calls that overlap on one GPU (other streams, other threads)
+ * race. */
+int mgp_divcal_device_noalloc(mgp_handle *h, mgletreal *div, const mgletreal *u,
+                              const mgletreal *v, const mgletreal *w, mgletreal fak);
+
 /* The RK stage around the projection (device fields, reference layout).
  *
  * tstle4 (src/flow/tstle4_mod.F90:13-120, no immersed boundary): uo, vo, wo

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <set>
 #include <memory>
 #include <string>
@@ -118,6 +119,35 @@ struct Box {
 
 }  // namespace
 
+// Scratch of the allocation-free entry point mgp_divcal_device_noalloc: the
+// divergence in WO order before it is scattered into the caller's field.
+// One array per GPU, held until the process ends.  Its host-side name is a
+// stand-in, not a device address: host code passes the address that
+// hipGetSymbolAddress returns for the current device (scratch_device_ptr).
+constexpr size_t kScratchCells = size_t(1) << 23;   // 8 Mi FP64 values, 64 MiB
+static __device__ __align__(256) double scratch[kScratchCells];
+
+namespace {
+
+// Device address of `scratch` on the current device, looked up once per
+// device: each GPU holds its own copy at its own address.
+void *scratch_device_ptr() {
+    static std::mutex mu;
+    static std::map<int, void *> ptrs;
+    int dev = 0;
+    HC(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = ptrs.find(dev);
+    if (it != ptrs.end()) return it->second;
+    void *ptr = nullptr;
+    HC(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw HipError{"hipGetSymbolAddress returned no address for the scratch array"};
+    ptrs[dev] = ptr;
+    return ptr;
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------
 struct mgp_handle {
     // configuration and topology (host)
@@ -181,6 +211,8 @@ struct mgp_handle {
     bool adj_ready = false;
     // reference-layout work arrays
     DBuf d_u, d_v, d_w, d_p, d_dpsp;
+    // this device's copy of the static scratch array (set at init)
+    double *scratch = nullptr;
     // scalars and history
     DBuf d_scal, d_hist, d_never;
     // exchange tasks: key (level, layers, corners, forward)
@@ -993,6 +1025,7 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
     try {
         HC(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking));
         h->stream = h->own_stream;
+        h->scratch = static_cast<double *>(scratch_device_ptr());
         std::vector<uint32_t> cells;
         std::vector<int> pstart, slot;
         build_shapes(*h, cells, pstart, slot);
@@ -1921,6 +1954,19 @@ int mgp_connect_device(mgp_handle *h, mgletint level, mgletint layers,
     return MGP_OK;
 }
 
+// divcal into `tmp` (WO order, ncell values: every one written), then
+// scattered into the interior of `div`
+static void divcal_via(mgp_handle *h, double *div, const double *u, const double *v,
+                       const double *w, double fak, double *tmp) {
+    Topo t = h->topo(-1);
+    launch_divcal<double>(t, u, v, w, h->d_rddx.as<double>(), h->d_rddy.as<double>(),
+                          h->d_rddz.as<double>(), h->cfg.ibtype == 1 ? h->d_bpsp.as<double>() : nullptr,
+                          h->sdiv_sp(), fak, tmp, nullptr, h->stream);
+    launch_wo_to_sp<double>(t, tmp, div, h->stream);
+    HC(hipGetLastError());
+    HC(hipStreamSynchronize(h->stream));
+}
+
 int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
                       const mgletreal *v, const mgletreal *w, mgletreal fak) {
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
@@ -1928,13 +1974,24 @@ int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
     try {
         DBuf tmp;
         tmp.alloc((size_t)h->ncell * 8);
-        Topo t = h->topo(-1);
-        launch_divcal<double>(t, u, v, w, h->d_rddx.as<double>(), h->d_rddy.as<double>(),
-                              h->d_rddz.as<double>(), h->cfg.ibtype == 1 ? h->d_bpsp.as<double>() : nullptr,
-                              h->sdiv_sp(), fak, tmp.as<double>(), nullptr, h->stream);
-        launch_wo_to_sp<double>(t, tmp.as<double>(), div, h->stream);
-        HC(hipGetLastError());
-        HC(hipStreamSynchronize(h->stream));
+        divcal_via(h, div, u, v, w, fak, tmp.as<double>());
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    return MGP_OK;
+}
+
+int mgp_divcal_device_noalloc(mgp_handle *h, mgletreal *div, const mgletreal *u,
+                              const mgletreal *v, const mgletreal *w, mgletreal fak) {
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (!div || !u || !v || !w) return fail(MGP_ERR_ARG, "null field");
+    if ((size_t)h->ncell > kScratchCells)
+        return fail(MGP_ERR_ARG, "%lld interior cells exceed the %zu-cell scratch of "
+                    "mgp_divcal_device_noalloc (use mgp_divcal_device)",
+                    (long long)h->ncell, kScratchCells);
+    if (!h->scratch) return fail(MGP_ERR_STATE, "scratch array not set up");
+    try {
+        divcal_via(h, div, u, v, w, fak, h->scratch);
     } catch (const HipError &e) {
         return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
     }

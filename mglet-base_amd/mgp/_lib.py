@@ -87,6 +87,7 @@ def load():
         "mgp_connect_device": [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                C.POINTER(vp), C.c_int32],
         "mgp_divcal_device": [vp, vp, vp, vp, vp, C.c_double],
+        "mgp_divcal_device_noalloc": [vp, vp, vp, vp, vp, C.c_double],
         "mgp_set_sdiv": [vp, _dp],
         "mgp_bound_layout": [vp, _i64p, _i64p],
         "mgp_set_bound": [vp, _dp, _dp, _dp, _dp],
