@@ -217,8 +217,9 @@ struct mgp_handle {
     DBuf d_scal, d_hist, d_never;
     // exchange tasks: key (level, layers, corners, forward)
     struct TaskSet {
-        DBuf d;
-        std::vector<int> stage_start;  // per stage offsets (+ end)
+        DBuf blocks, ents;             // PullBlock records of all stages, entry tables
+        std::vector<int> stage_start;  // per stage offsets into blocks (+ end)
+        bool pairs = false;            // entries move two values along k
     };
     std::map<std::tuple<int, int, int, int>, std::unique_ptr<TaskSet>> tasks;
     // captured outer-iteration graphs per level
@@ -441,6 +442,9 @@ bool overlap(const Box &a, const Box &b) {
            a.ks <= b.ke && b.ks <= a.ke;
 }
 
+template <typename T>
+void upload(DBuf &d, const std::vector<T> &v);
+
 // Copy tasks of one exchange, grouped into stages that keep the
 // reference's sequential order wherever two copies touch the same cells
 // (only possible when a source box reaches into a ghost layer).
@@ -506,18 +510,100 @@ mgp_handle::TaskSet &task_set(mgp_handle &h, int level, int layers,
         wr[list[q].dg].push_back((int)q);
         rdl[list[q].sg].push_back((int)q);
     }
-    std::vector<CopyTask> sorted;
+    // entry tables: per stage and receiving block, the box copies become
+    // slots and their elements entries sorted by destination offset (no two
+    // copies of a stage write the same position).  Pairs of values along k
+    // when every box has an even k extent and even offsets and strides.
     auto ts = std::make_unique<mgp_handle::TaskSet>();
-    for (int s = 0; s < nst; ++s) {
-        ts->stage_start.push_back((int)sorted.size());
-        for (size_t q = 0; q < list.size(); ++q)
-            if (stage[q] == s) sorted.push_back(list[q].t);
+    bool pairs = true;
+    for (const T2 &q : list) {
+        const CopyTask &t = q.t;
+        pairs = pairs && t.nk % 2 == 0 && t.src % 2 == 0 && t.dst % 2 == 0 && t.s_sj % 2 == 0 &&
+                t.s_si % 2 == 0 && t.d_sj % 2 == 0 && t.d_si % 2 == 0;
     }
-    ts->stage_start.push_back((int)sorted.size());
-    ts->d.alloc(sorted.size() * sizeof(CopyTask));
-    if (!sorted.empty())
-        HC(hipMemcpy(ts->d.p, sorted.data(), sorted.size() * sizeof(CopyTask),
-                     hipMemcpyHostToDevice));
+    ts->pairs = pairs;
+    const int step = pairs ? 2 : 1;
+    std::vector<PullBlock> pbs;
+    std::vector<uint2> ents;
+    std::map<std::vector<int64_t>, std::pair<int, int>> tables;  // pattern -> (tab, n)
+    for (int st = 0; st < nst; ++st) {
+        ts->stage_start.push_back((int)pbs.size());
+        std::map<int, std::vector<int>> byrecv;
+        for (size_t q = 0; q < list.size(); ++q)
+            if (stage[q] == st) byrecv[list[q].dg].push_back((int)q);
+        // receiving blocks in the order of nested neighbour chains: along
+        // +x (face 2) outermost, then +y (face 4), then +z (face 6)
+        // innermost.  Consecutive workgroups then work on z-neighbours
+        // (the thin z-ghost rows) and neighbouring columns on y-neighbours,
+        // so that partial-line reads and writes of the same lines meet in
+        // the memory-side cache: 1.5x faster than block-id order on a
+        // 32^3 lattice numbered x fastest.
+        // Every block starts at most one walk per direction: O(ngrid).
+        std::vector<int> order;
+        {
+            std::vector<char> zdone(h.ngrid + 1, 0), ydone(h.ngrid + 1, 0),
+                xdone(h.ngrid + 1, 0);
+            auto next = [&](int g, int face) { return h.nbr[26 * (g - 1) + face - 1]; };
+            auto zwalk = [&](int b) {
+                for (int g = b; g > 0 && !zdone[g]; g = next(g, 6)) {
+                    zdone[g] = 1;
+                    if (byrecv.count(g)) order.push_back(g);
+                }
+            };
+            auto ywalk = [&](int a) {
+                for (int g = a; g > 0 && !ydone[g]; g = next(g, 4)) {
+                    ydone[g] = 1;
+                    zwalk(g);
+                }
+            };
+            for (auto &kv : byrecv)
+                for (int g = kv.first; g > 0 && !xdone[g]; g = next(g, 2)) {
+                    xdone[g] = 1;
+                    ywalk(g);
+                }
+        }
+        if (order.size() != byrecv.size()) throw HipError{"exchange block order incomplete"};
+        for (int recv : order) {
+            const std::vector<int> &qs = byrecv.at(recv);
+            if (qs.size() > 26) throw HipError{"more than 26 ghost copies into one block"};
+            PullBlock pb{};
+            pb.dst = h.ip3[recv - 1];
+            std::vector<int64_t> pat;
+            for (size_t sl = 0; sl < qs.size(); ++sl) {
+                const T2 &q = list[qs[sl]];
+                const CopyTask &t = q.t;
+                pb.src[sl] = h.ip3[q.sg - 1];
+                pat.insert(pat.end(), {t.dst - pb.dst, t.src - pb.src[sl], t.ni, t.nj, t.nk,
+                                       t.s_sj, t.s_si, t.d_sj, t.d_si});
+            }
+            auto it = tables.find(pat);
+            if (it == tables.end()) {
+                std::vector<uint2> e;
+                for (size_t sl = 0; sl < qs.size(); ++sl) {
+                    const CopyTask &t = list[qs[sl]].t;
+                    const int64_t d0 = t.dst - pb.dst, s0 = t.src - pb.src[sl];
+                    for (int i = 0; i < t.ni; ++i)
+                        for (int j = 0; j < t.nj; ++j)
+                            for (int k = 0; k < t.nk; k += step) {
+                                const int64_t d = d0 + k + (int64_t)j * t.d_sj + (int64_t)i * t.d_si;
+                                const int64_t so = s0 + k + (int64_t)j * t.s_sj + (int64_t)i * t.s_si;
+                                if (so < 0 || so >= (1 << kPullSlotBits) || d < 0 || d > INT32_MAX)
+                                    throw HipError{"ghost copy offsets beyond the entry format"};
+                                e.push_back(make_uint2((uint32_t)d, (uint32_t)so | (uint32_t)sl << kPullSlotBits));
+                            }
+                }
+                std::sort(e.begin(), e.end(), [](const uint2 &a, const uint2 &b) { return a.x < b.x; });
+                it = tables.emplace(pat, std::make_pair((int)ents.size(), (int)e.size())).first;
+                ents.insert(ents.end(), e.begin(), e.end());
+            }
+            pb.tab = it->second.first;
+            pb.n = it->second.second;
+            pbs.push_back(pb);
+        }
+    }
+    ts->stage_start.push_back((int)pbs.size());
+    upload(ts->blocks, pbs);
+    upload(ts->ents, ents);
     auto &ref = *ts;
     h.tasks[key] = std::move(ts);
     return ref;
@@ -526,11 +612,19 @@ mgp_handle::TaskSet &task_set(mgp_handle &h, int level, int layers,
 void exchange(mgp_handle &h, int level, int layers, int corners, int forward,
               double *const *fields, int nfield) {
     auto &ts = task_set(h, level, layers, corners, forward);
-    for (size_t s = 0; s + 1 < ts.stage_start.size(); ++s) {
-        const int a = ts.stage_start[s], b = ts.stage_start[s + 1];
-        for (int f0 = 0; f0 < nfield; f0 += 6)
-            launch_copy(ts.d.as<CopyTask>() + a, b - a, fields + f0, std::min(6, nfield - f0),
-                        h.stream);
+    for (int f0 = 0; f0 < nfield; f0 += 6) {
+        const int nf = std::min(6, nfield - f0);
+        int mode = 0;
+        if (ts.pairs) {
+            mode = 2;
+            for (int f = 0; f < nf; ++f)
+                if ((uintptr_t)fields[f0 + f] % 16) mode = 1;
+        }
+        for (size_t s = 0; s + 1 < ts.stage_start.size(); ++s) {
+            const int a = ts.stage_start[s], b = ts.stage_start[s + 1];
+            launch_pull(ts.blocks.as<PullBlock>() + a, b - a, ts.ents.as<uint2>(), mode,
+                        fields + f0, nf, h.stream);
+        }
     }
 }
 

@@ -139,6 +139,19 @@ struct CopyTask {
     int s_sj, s_si;    // source strides of j and i (k stride is 1)
     int d_sj, d_si;    // destination strides
 };
-constexpr int kCopyThreads = 256;   // threads per workgroup of k_copy (a box per wave)
+
+// The exchange by receiving block: one workgroup per receiving block of a
+// stage copies all of the block's ghost entries of that stage.  An entry
+// is (destination offset in the receiving block, source offset in the
+// block of its slot | slot << 24); slot q is the source block of the
+// block's q-th box copy.  In pair tables an entry moves two values
+// consecutive along k.  Blocks with the same copy pattern share a table.
+struct PullBlock {
+    int64_t dst;       // offset of the receiving block (reference layout)
+    int tab, n;        // first entry of its table, number of entries
+    int64_t src[26];   // offset of the source block of each slot
+};
+constexpr int kPullThreads = 256;
+constexpr int kPullSlotBits = 24;   // source offsets below 2^24 (blocks <= 68^3)
 
 }  // namespace mgp

@@ -701,55 +701,81 @@ void launch_mgpcorr(const Topo &t, double *u, double *v, double *w,
 }
 
 // ---------------------------------------------------------------------
-// Ghost-layer box copies (connect_self_single, src/core/connect/
-// conn1_mod.F90:674-704), all fields at once.  One wave per box: the box
-// record comes in through scalar loads, and the wave walks the box 64
-// elements at a time, k fastest (contiguous in both the source and the
-// destination layout), so there is no search for a thread's box.  Element
-// indices are split with a float reciprocal and one correction step
-// (exact for the small box sizes of a ghost layer).
+// Ghost-layer exchange (connect_self_single, src/core/connect/conn1_mod.
+// F90:674-704, all fields at once) by receiving block: the workgroup of a
+// block reads its slots' source offsets into LDS and walks the block's
+// entry table, which lists the ghost positions in memory order, so
+// consecutive lanes write consecutive addresses wherever the ghost region
+// is contiguous (rows along k).  Every load of an entry is issued before
+// its stores.
 struct FieldList {
     double *f[6];
-    int n;
 };
 
-__device__ __forceinline__ int div_small(int e, int d, float rd) {
-    int q = (int)((float)e * rd);
-    const int r = e - q * d;
-    if (r < 0) --q;
-    else if (r >= d) ++q;
-    return q;
-}
-
-constexpr int kCopyWaves = kCopyThreads / 64;
-
-__global__ __launch_bounds__(kCopyThreads) void k_copy(const CopyTask *tasks, int ntask,
-                                                       FieldList fl) {
-    const int b = __builtin_amdgcn_readfirstlane(blockIdx.x * kCopyWaves + (threadIdx.x >> 6));
-    if (b >= ntask) return;
-    const int lane = threadIdx.x & 63;
-    const CopyTask T = tasks[b];
-    const int nk = T.nk, njk = T.nj * T.nk, n = T.ni * njk;
-    const float rnk = 1.0f / (float)nk, rnjk = 1.0f / (float)njk;
-    for (int e = lane; e < n; e += 64) {
-        const int i = div_small(e, njk, rnjk);
-        const int r = e - i * njk;
-        const int j = div_small(r, nk, rnk);
-        const int k = r - j * nk;
-        const int64_t so = T.src + k + (int64_t)j * T.s_sj + (int64_t)i * T.s_si;
-        const int64_t d = T.dst + k + (int64_t)j * T.d_sj + (int64_t)i * T.d_si;
-        for (int f = 0; f < fl.n; ++f) fl.f[f][d] = fl.f[f][so];
+template <int NF, int MODE>
+__global__ __launch_bounds__(kPullThreads) void k_pull(const PullBlock *pb, const uint2 *ents,
+                                                      FieldList fl) {
+    __shared__ int64_t src[26];
+    const PullBlock &B = pb[blockIdx.x];
+    if (threadIdx.x < 26) src[threadIdx.x] = B.src[threadIdx.x];
+    const int64_t dst = B.dst;
+    const int n = B.n;
+    const uint2 *e = ents + B.tab;
+    __syncthreads();
+    constexpr uint32_t kMask = (1u << kPullSlotBits) - 1;
+    for (int q = threadIdx.x; q < n; q += kPullThreads) {
+        const uint2 E = e[q];
+        const int64_t so = src[E.y >> kPullSlotBits] + (E.y & kMask);
+        const int64_t d = dst + E.x;
+        if (MODE == 2) {
+            double2 v[NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) v[f] = *(const double2 *)(fl.f[f] + so);
+#pragma unroll
+            for (int f = 0; f < NF; ++f) *(double2 *)(fl.f[f] + d) = v[f];
+        } else if (MODE == 1) {
+            double v[NF][2];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                v[f][0] = fl.f[f][so];
+                v[f][1] = fl.f[f][so + 1];
+            }
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                fl.f[f][d] = v[f][0];
+                fl.f[f][d + 1] = v[f][1];
+            }
+        } else {
+            double v[NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) v[f] = fl.f[f][so];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) fl.f[f][d] = v[f];
+        }
     }
 }
 
-void launch_copy(const CopyTask *tasks, int ntask, double *const *fields, int nfield,
-                 hipStream_t s) {
-    if (ntask <= 0) return;
-    FieldList fl;
-    fl.n = nfield;
+template <int NF>
+void launch_pull_nf(const PullBlock *blocks, int nblock, const uint2 *ents, int mode,
+                    const FieldList &fl, hipStream_t s) {
+    if (mode == 2) k_pull<NF, 2><<<nblock, kPullThreads, 0, s>>>(blocks, ents, fl);
+    else if (mode == 1) k_pull<NF, 1><<<nblock, kPullThreads, 0, s>>>(blocks, ents, fl);
+    else k_pull<NF, 0><<<nblock, kPullThreads, 0, s>>>(blocks, ents, fl);
+}
+
+void launch_pull(const PullBlock *blocks, int nblock, const uint2 *ents, int mode,
+                 double *const *fields, int nfield, hipStream_t s) {
+    if (nblock <= 0 || nfield <= 0) return;
+    FieldList fl{};
     for (int f = 0; f < nfield && f < 6; ++f) fl.f[f] = fields[f];
-    const unsigned nwg = (unsigned)((ntask + kCopyWaves - 1) / kCopyWaves);
-    k_copy<<<nwg, kCopyThreads, 0, s>>>(tasks, ntask, fl);
+    switch (nfield) {
+    case 1: launch_pull_nf<1>(blocks, nblock, ents, mode, fl, s); break;
+    case 2: launch_pull_nf<2>(blocks, nblock, ents, mode, fl, s); break;
+    case 3: launch_pull_nf<3>(blocks, nblock, ents, mode, fl, s); break;
+    case 4: launch_pull_nf<4>(blocks, nblock, ents, mode, fl, s); break;
+    case 5: launch_pull_nf<5>(blocks, nblock, ents, mode, fl, s); break;
+    default: launch_pull_nf<6>(blocks, nblock, ents, mode, fl, s); break;
+    }
 }
 
 }  // namespace mgp

@@ -174,9 +174,11 @@ void launch_tstle4(const Topo &t, const MomArgs &a, hipStream_t s);
 void launch_rkstep(double *p, double *dp, const double *rhsp, int64_t n,
                    double frhs, double dtfu, hipStream_t s);
 
-// One stage of box copies: `ntask` boxes, one wave each.
-void launch_copy(const CopyTask *tasks, int ntask, double *const *fields, int nfield,
-                 hipStream_t s);
+// One stage of the exchange: `nblock` receiving blocks.  mode 0: an entry
+// moves one value; 1: two values (scalar accesses); 2: two values as one
+// 16-byte access (every field pointer 16-byte aligned).
+void launch_pull(const PullBlock *blocks, int nblock, const uint2 *ents, int mode,
+                 double *const *fields, int nfield, hipStream_t s);
 
 template <typename T>
 void launch_to_ref_factors(const Topo &t, const T *wo, double *ref,
