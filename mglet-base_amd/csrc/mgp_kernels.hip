@@ -649,44 +649,77 @@ template void launch_to_ref_factors<float>(const Topo &, const float *,
 
 // ---------------------------------------------------------------------
 // Velocity and pressure correction (src/flow/pressuresolver_mod.F90:
-// 787-879) on the reference layout; one thread per stored cell.
-__global__ void k_mgpcorr(Topo t, double *u_, double *v_, double *w_,
-                          double *p_, const double *dp_, const double *rdx_,
-                          const double *rdy_, const double *rdz_,
-                          double rfak, const double *bp_) {
+// 787-879) on the reference layout.  The threads of a block walk its rows
+// (i, j) in 2..ii-1 x 2..jj-1, all kk positions of a row, so every HBM
+// line a row touches is used whole: a field is read and written back over
+// the full rows it updates anywhere (positions outside the update range
+// keep their value), and rows it never updates are not touched.  Element
+// indices are split with a float reciprocal and one correction step
+// (exact for the row and block sizes here).
+__device__ __forceinline__ int div_small(int e, int d, float rd) {
+    int q = (int)((float)e * rd);
+    const int r = e - q * d;
+    if (r < 0) --q;
+    else if (r >= d) ++q;
+    return q;
+}
+
+__global__ __launch_bounds__(kThreads) void k_mgpcorr(Topo t, double *u_, double *v_,
+                                                     double *w_, double *p_, const double *dp_,
+                                                     const double *rdx_, const double *rdy_,
+                                                     const double *rdz_, double rfak,
+                                                     const double *bp_) {
     const int b = t.blist[blockIdx.y];
     const Block &B = t.blocks[b];
     const int kk = B.nk + 4, jj = B.nj + 4, ii = B.ni + 4;
-    const int64_t n = (int64_t)kk * jj * ii;
+    const int nrj = jj - 2, nel = (ii - 2) * nrj * kk;
+    const int e = blockIdx.x * kThreads + threadIdx.x;
+    if (e >= nel) return;
+    const int row = div_small(e, kk, 1.0f / (float)kk);
+    const int k = e - row * kk + 1;
+    const int ri = div_small(row, nrj, 1.0f / (float)nrj);
+    const int i = ri + 2, j = row - ri * nrj + 2;
+    const int64_t sj = kk, si = (int64_t)kk * jj;
+    const int64_t c = (int64_t)(k - 1) + (int64_t)(j - 1) * sj + (int64_t)(i - 1) * si;
     double *u = u_ + B.sp, *v = v_ + B.sp, *w = w_ + B.sp, *p = p_ + B.sp;
     const double *dp = dp_ + B.sp;
     const double *bp = bp_ ? bp_ + B.sp : nullptr;
-    const double *rdx = rdx_ + B.o1x - 1, *rdy = rdy_ + B.o1y - 1,
-                 *rdz = rdz_ + B.o1z - 1;  // 1-based
-    const int64_t sj = kk, si = (int64_t)kk * jj;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int k = (int)(e % kk) + 1;
-        const int j = (int)((e / kk) % jj) + 1;
-        const int i = (int)(e / si) + 1;
-        if (i >= 2 && i <= ii - 1 && j >= 2 && j <= jj - 1 && k >= 2 &&
-            k <= kk - 1)
-            p[e] = bp ? p[e] + dp[e] * bp[e] : p[e] + dp[e];
-        if (i >= 2 && i <= ii - 2 && j >= 3 && j <= jj - 2 && k >= 3 &&
-            k <= kk - 2)
-            u[e] = bp ? u[e] + (dp[e] - dp[e + si]) * bp[e] * bp[e + si] *
-                                   rdx[i] * rfak
-                      : u[e] + (dp[e] - dp[e + si]) * rdx[i] * rfak;
-        if (i >= 3 && i <= ii - 2 && j >= 2 && j <= jj - 2 && k >= 3 &&
-            k <= kk - 2)
-            v[e] = bp ? v[e] + (dp[e] - dp[e + sj]) * bp[e] * bp[e + sj] *
-                                   rdy[j] * rfak
-                      : v[e] + (dp[e] - dp[e + sj]) * rdy[j] * rfak;
-        if (i >= 3 && i <= ii - 2 && j >= 3 && j <= jj - 2 && k >= 2 &&
-            k <= kk - 2)
-            w[e] = bp ? w[e] + (dp[e] - dp[e + 1]) * bp[e] * bp[e + 1] *
-                                   rdz[k] * rfak
-                      : w[e] + (dp[e] - dp[e + 1]) * rdz[k] * rfak;
+    const bool kin = k >= 3 && k <= kk - 2;
+    {   // P: i, j, k in 2..n-1
+        const double pv = p[c];
+        double r = pv;
+        if (k >= 2 && k <= kk - 1) r = bp ? pv + dp[c] * bp[c] : pv + dp[c];
+        p[c] = r;
+    }
+    if (i <= ii - 2 && j >= 3 && j <= jj - 2) {   // U rows
+        const double uv = u[c];
+        double r = uv;
+        if (kin) {
+            const double rdx = rdx_[B.o1x + i - 1];
+            r = bp ? uv + (dp[c] - dp[c + si]) * bp[c] * bp[c + si] * rdx * rfak
+                   : uv + (dp[c] - dp[c + si]) * rdx * rfak;
+        }
+        u[c] = r;
+    }
+    if (i >= 3 && i <= ii - 2 && j <= jj - 2) {   // V rows
+        const double vv = v[c];
+        double r = vv;
+        if (kin) {
+            const double rdy = rdy_[B.o1y + j - 1];
+            r = bp ? vv + (dp[c] - dp[c + sj]) * bp[c] * bp[c + sj] * rdy * rfak
+                   : vv + (dp[c] - dp[c + sj]) * rdy * rfak;
+        }
+        v[c] = r;
+    }
+    if (i >= 3 && i <= ii - 2 && j >= 3 && j <= jj - 2) {   // W rows
+        const double wv = w[c];
+        double r = wv;
+        if (k >= 2 && k <= kk - 2) {
+            const double rdz = rdz_[B.o1z + k - 1];
+            r = bp ? wv + (dp[c] - dp[c + 1]) * bp[c] * bp[c + 1] * rdz * rfak
+                   : wv + (dp[c] - dp[c + 1]) * rdz * rfak;
+        }
+        w[c] = r;
     }
 }
 
@@ -694,10 +727,9 @@ void launch_mgpcorr(const Topo &t, double *u, double *v, double *w,
                     double *p, const double *dp, const double *rdx,
                     const double *rdy, const double *rdz, double rfak,
                     const double *bp, hipStream_t s) {
-    const int maxc = (t.max_ni + 4) * (t.max_nj + 4) * (t.max_nk + 4);
-    dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
-    k_mgpcorr<<<grid, kThreads, 0, s>>>(t, u, v, w, p, dp, rdx, rdy, rdz,
-                                        rfak, bp);
+    const int maxe = (t.max_ni + 2) * (t.max_nj + 2) * (t.max_nk + 4);
+    dim3 grid((maxe + kThreads - 1) / kThreads, t.nlist);
+    k_mgpcorr<<<grid, kThreads, 0, s>>>(t, u, v, w, p, dp, rdx, rdy, rdz, rfak, bp);
 }
 
 // ---------------------------------------------------------------------
