@@ -283,6 +283,8 @@ def main():
     ap.add_argument("--tts", choices=["none", "c2", "main"], default="c2",
                     help="time-to-1e-10 on the 8x8x8-block case (c2), the main case, or skip")
     ap.add_argument("--tts-cap", type=int, default=4000, help="max outer iterations")
+    ap.add_argument("--tts-eps", type=float, default=1e-10,
+                    help="relative residual the time-to-solution runs stop at")
     ap.add_argument("--tts-history", default=None,
                     help="write each form's max|RHS| per outer iteration of the tts solve "
                          "to this .npz file")
@@ -442,19 +444,20 @@ def main():
         tg = uniform_box(tnb, n) if args.tts == "c2" else g
         tf = make_velocity(tg, tnb, n, seed=4) if args.tts == "c2" else fields0
         tts = {"config": f"{tg.ngrid} blocks of {n[0]}^3 ({tnb[0]}x{tnb[1]}x{tnb[2]} periodic)",
-               "eps_rel": 1e-10}
+               "eps_rel": args.tts_eps}
         ps, hist = {}, {}
         tforms = [f for f in forms if not f.endswith("-unshared")]
         for form in tforms:
             s = mgp.PressureSolver(tg, ninner=args.ninner, nouter=1, precision=form)
             if form == tforms[0] and args.tts == "c2":
                 s.connect(tf, layers=2, corners=True)
-            time_to_solution(s, tf, args.ninner, 1e-10, 2)        # warm
-            t, st, p = time_to_solution(s, tf, args.ninner, 1e-10, args.tts_cap)
+            time_to_solution(s, tf, args.ninner, args.tts_eps, 2)        # warm
+            t, st, p = time_to_solution(s, tf, args.ninner, args.tts_eps, args.tts_cap)
             ps[form] = p
             if args.tts_history:
                 hist[form] = s.history()
             tts[form] = {"seconds": round(t, 4), "inner_iterations": st["ipc"],
+                         "outer_iterations": st["nouter_done"],
                          "converged": st["converged"],
                          "rel_residual": st["maxrhs"] / st["maxrhs0"],
                          # pipeline shape the autotuner picked for this batch
