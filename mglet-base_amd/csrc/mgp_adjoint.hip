@@ -206,6 +206,7 @@ __global__ void k_divT(Topo t, double *u_, double *v_, double *w_,
 template <typename T>
 void launch_tfactors(const Topo &t, Mask m, const T *const f[7],
                      T *const g[6], hipStream_t s) {
+    MGP_RANGE("mgp:tfactors");
     const int maxc = t.max_ni * t.max_nj * t.max_nk;
     dim3 grid((maxc + kThreadsAdj - 1) / kThreadsAdj, t.nlist);
     k_tfactors<T><<<grid, kThreadsAdj, 0, s>>>(t, m, f[0], f[1], f[2], f[3],
@@ -219,6 +220,7 @@ template void launch_tfactors<float>(const Topo &, Mask, const float *const[7],
 
 void launch_adj_clean(const Topo &t, double *u, double *v, double *w,
                       const double *bp, hipStream_t s) {
+    MGP_RANGE("mgp:adj_clean");
     const int maxc = (t.max_ni + 4) * (t.max_nj + 4) * (t.max_nk + 4);
     dim3 grid((maxc + kThreadsAdj - 1) / kThreadsAdj, t.nlist);
     k_adj_clean<<<grid, kThreadsAdj, 0, s>>>(t, u, v, w, bp);
@@ -229,6 +231,7 @@ void launch_gradT(const Topo &t, const double *u, const double *v,
                   const double *w, const double *rdx, const double *rdy,
                   const double *rdz, const double *bp, double *rhs64,
                   T *rhs_t, hipStream_t s) {
+    MGP_RANGE("mgp:gradT");
     const int maxc = t.max_ni * t.max_nj * t.max_nk;
     dim3 grid((maxc + kThreadsAdj - 1) / kThreadsAdj, t.nlist);
     k_gradT<T><<<grid, kThreadsAdj, 0, s>>>(t, u, v, w, rdx, rdy, rdz, bp,
@@ -246,6 +249,7 @@ template void launch_gradT<float>(const Topo &, const double *, const double *,
 void launch_divT(const Topo &t, double *u, double *v, double *w,
                  const double *y, const double *rddx, const double *rddy,
                  const double *rddz, const double *bp, hipStream_t s) {
+    MGP_RANGE("mgp:divT");
     const int maxc = t.max_ni * t.max_nj * t.max_nk;
     dim3 grid((maxc + kThreadsAdj - 1) / kThreadsAdj, t.nlist);
     k_divT<<<grid, kThreadsAdj, 0, s>>>(t, u, v, w, y, rddx, rddy, rddz, bp);

@@ -1848,6 +1848,7 @@ int mgp_init(const mgp_grids *grids, const mgp_config *cfg,
              const mgletreal *dx, const mgletreal *dy, const mgletreal *dz,
              const mgletreal *ddx, const mgletreal *ddy, const mgletreal *ddz,
              const mgletreal *bp, mgp_handle **out) {
+    MGP_RANGE("mgp_init");
     return init_common(grids, cfg, dx, dy, dz, ddx, ddy, ddz, bp, nullptr, out);
 }
 
@@ -1856,6 +1857,7 @@ int mgp_init_mask(const mgp_grids *grids, const mgp_config *cfg,
                   const mgletreal *dz, const mgletreal *ddx,
                   const mgletreal *ddy, const mgletreal *ddz,
                   const int8_t *blocked, mgp_handle **out) {
+    MGP_RANGE("mgp_init_mask");
     return init_common(grids, cfg, dx, dy, dz, ddx, ddy, ddz, nullptr,
                        blocked, out);
 }
@@ -1863,6 +1865,7 @@ int mgp_init_mask(const mgp_grids *grids, const mgp_config *cfg,
 int mgp_mgpoisl_device(mgp_handle *h, mgletreal *u, mgletreal *v,
                        mgletreal *w, mgletreal *p, mgletreal dt,
                        mgletint ittot, mgletint irk, mgp_stats *stats) {
+    MGP_RANGE("mgp_mgpoisl_device");
     (void)ittot;
     (void)irk;
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
@@ -1873,6 +1876,7 @@ int mgp_mgpoisl_device(mgp_handle *h, mgletreal *u, mgletreal *v,
 int mgp_mgpoisl(mgp_handle *h, mgletreal *u, mgletreal *v, mgletreal *w,
                 mgletreal *p, mgletreal dt, mgletint ittot, mgletint irk,
                 mgp_stats *stats) {
+    MGP_RANGE("mgp_mgpoisl");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     if (!u || !v || !w || !p) return fail(MGP_ERR_ARG, "null field");
     try {
@@ -1896,6 +1900,7 @@ int mgp_mgpoisl(mgp_handle *h, mgletreal *u, mgletreal *v, mgletreal *w,
 
 int mgp_adjoint_device(mgp_handle *h, mgletreal *ubar, mgletreal *vbar,
                        mgletreal *wbar, mgletreal dt, mgp_stats *stats) {
+    MGP_RANGE("mgp_adjoint_device");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     if (!ubar || !vbar || !wbar) return fail(MGP_ERR_ARG, "null field");
     return adjoint_any(h, ubar, vbar, wbar, dt, stats);
@@ -1903,6 +1908,7 @@ int mgp_adjoint_device(mgp_handle *h, mgletreal *ubar, mgletreal *vbar,
 
 int mgp_adjoint(mgp_handle *h, mgletreal *ubar, mgletreal *vbar,
                 mgletreal *wbar, mgletreal dt, mgp_stats *stats) {
+    MGP_RANGE("mgp_adjoint");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     if (!ubar || !vbar || !wbar) return fail(MGP_ERR_ARG, "null field");
     try {
@@ -1925,6 +1931,7 @@ int mgp_adjoint(mgp_handle *h, mgletreal *ubar, mgletreal *vbar,
 }
 
 int mgp_finish(mgp_handle *h) {
+    MGP_RANGE("mgp_finish");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     (void)hipStreamSynchronize(h->stream);
     delete h;
@@ -1937,6 +1944,7 @@ const char *mgp_last_error(void) { return g_err.c_str(); }
 int mgp_set_iterations(mgp_handle *h, mgletint ninner, mgletint nouter,
                        mgletint nouter_min, mgletreal epcorr,
                        mgletreal eps_rel, mgletint check_every) {
+    MGP_RANGE("mgp_set_iterations");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     mgp_config c = h->cfg;
     c.ninner = ninner;
@@ -1958,16 +1966,19 @@ int mgp_set_iterations(mgp_handle *h, mgletint ninner, mgletint nouter,
 }
 
 int mgp_set_stream(mgp_handle *h, void *stream) {
+    MGP_RANGE("mgp_set_stream");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     h->stream = stream ? (hipStream_t)stream : h->own_stream;
     return MGP_OK;
 }
 
 int mgp_get_field(mgp_handle *h, const char *name, mgletreal *out) {
+    MGP_RANGE("mgp_get_field");
     return mgp_get_field_grid(h, name, 0, out);
 }
 
 int mgp_get_field_grid(mgp_handle *h, const char *name, mgletint igrid, mgletreal *out) {
+    MGP_RANGE("mgp_get_field_grid");
     if (!h || !name || !out) return fail(MGP_ERR_ARG, "null argument");
     if (igrid < 0 || igrid > h->ngrid) return fail(MGP_ERR_ARG, "invalid grid %d", igrid);
     const std::string nm(name);
@@ -2033,6 +2044,7 @@ int mgp_get_field_grid(mgp_handle *h, const char *name, mgletint igrid, mgletrea
 int mgp_connect_device(mgp_handle *h, mgletint level, mgletint layers,
                        mgletint corners, mgletint forward,
                        mgletreal **fields, mgletint nfield) {
+    MGP_RANGE("mgp_connect_device");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     if (layers < 1 || layers > 2) return fail(MGP_ERR_ARG, "layers must be 1 or 2");
     if (forward != 0 && corners) return fail(MGP_ERR_ARG, "forward with corners");
@@ -2063,6 +2075,7 @@ static void divcal_via(mgp_handle *h, double *div, const double *u, const double
 
 int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
                       const mgletreal *v, const mgletreal *w, mgletreal fak) {
+    MGP_RANGE("mgp_divcal_device");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     if (!div || !u || !v || !w) return fail(MGP_ERR_ARG, "null field");
     try {
@@ -2077,6 +2090,7 @@ int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
 
 int mgp_divcal_device_noalloc(mgp_handle *h, mgletreal *div, const mgletreal *u,
                               const mgletreal *v, const mgletreal *w, mgletreal fak) {
+    MGP_RANGE("mgp_divcal_device_noalloc");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     if (!div || !u || !v || !w) return fail(MGP_ERR_ARG, "null field");
     if ((size_t)h->ncell > kScratchCells)
@@ -2093,6 +2107,7 @@ int mgp_divcal_device_noalloc(mgp_handle *h, mgletreal *div, const mgletreal *u,
 }
 
 int mgp_set_sdiv(mgp_handle *h, const mgletreal *sdiv) {
+    MGP_RANGE("mgp_set_sdiv");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     if (h->cfg.ibtype != 1)
         return fail(MGP_ERR_ARG, "SDIV enters the ghostcell divergence only (ib = ghostcell)");
@@ -2112,6 +2127,7 @@ int mgp_set_sdiv(mgp_handle *h, const mgletreal *sdiv) {
 }
 
 int mgp_bound_layout(mgp_handle *h, int64_t *offsets, int64_t *n) {
+    MGP_RANGE("mgp_bound_layout");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     if (!n) return fail(MGP_ERR_ARG, "n is NULL");
     *n = h->nbb;
@@ -2121,6 +2137,7 @@ int mgp_bound_layout(mgp_handle *h, int64_t *offsets, int64_t *n) {
 
 int mgp_set_bound(mgp_handle *h, const mgletreal *ubuf, const mgletreal *vbuf,
                   const mgletreal *wbuf, const mgletreal *pinf) {
+    MGP_RANGE("mgp_set_bound");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     try {
         const mgletreal *src[3] = {ubuf, vbuf, wbuf};
@@ -2160,6 +2177,7 @@ int mgp_tstle4_device(mgp_handle *h, mgletreal *uo, mgletreal *vo, mgletreal *wo
                       const mgletreal *u, const mgletreal *v, const mgletreal *w,
                       const mgletreal *p, const mgletreal *g, mgletreal gmol,
                       mgletint ilesmodel, const mgletreal *gradp) {
+    MGP_RANGE("mgp_tstle4_device");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     if (!uo || !vo || !wo || !u || !v || !w || !p || !g) return fail(MGP_ERR_ARG, "null field");
     for (int q = 0; q < h->ngrid; ++q)
@@ -2195,6 +2213,7 @@ int mgp_tstle4_device(mgp_handle *h, mgletreal *uo, mgletreal *vo, mgletreal *wo
 
 int mgp_rkstep_device(mgp_handle *h, mgletreal *p, mgletreal *dp,
                       const mgletreal *rhsp, mgletreal frhs, mgletreal dtfu) {
+    MGP_RANGE("mgp_rkstep_device");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     if (!p || !dp || !rhsp) return fail(MGP_ERR_ARG, "null field");
     try {
@@ -2208,6 +2227,7 @@ int mgp_rkstep_device(mgp_handle *h, mgletreal *p, mgletreal *dp,
 }
 
 int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms) {
+    MGP_RANGE("mgp_time_kernels");
     if (!h || !ms || niter < 1) return fail(MGP_ERR_ARG, "bad argument");
     try {
         set_plan_shape(h->plan_shape);
@@ -2263,6 +2283,7 @@ int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms) {
 
 int mgp_plan_shape(mgp_handle *h, mgletint request, mgletint *chosen,
                    double *ms) {
+    MGP_RANGE("mgp_plan_shape");
     if (!h || request < -1 || request > kNumShapes) return fail(MGP_ERR_ARG, "bad argument");
     try {
         if (request < 0) {
@@ -2284,6 +2305,7 @@ int mgp_plan_shape(mgp_handle *h, mgletint request, mgletint *chosen,
 
 int mgp_get_history(mgp_handle *h, mgletreal *out, mgletint cap,
                     mgletint *count) {
+    MGP_RANGE("mgp_get_history");
     if (!h || !out || !count || cap < 0) return fail(MGP_ERR_ARG, "bad argument");
     if (!h->have_solution) return fail(MGP_ERR_STATE, "no mgpoisl call yet");
     try {
@@ -2299,6 +2321,7 @@ int mgp_get_history(mgp_handle *h, mgletreal *out, mgletint cap,
 }
 
 int mgp_factor_classes(mgp_handle *h, mgletint *nclasses, int64_t *ncells_stored) {
+    MGP_RANGE("mgp_factor_classes");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     if (nclasses) *nclasses = (mgletint)h->nfclass;
     if (ncells_stored) *ncells_stored = h->nfcell;
@@ -2306,6 +2329,7 @@ int mgp_factor_classes(mgp_handle *h, mgletint *nclasses, int64_t *ncells_stored
 }
 
 int mgp_info(mgp_handle *h, int64_t *ncells, int64_t *device_bytes) {
+    MGP_RANGE("mgp_info");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     if (ncells) *ncells = h->ncell;
     if (device_bytes) {

@@ -519,6 +519,7 @@ __global__ __launch_bounds__(kBoundThreads) void k_parent_uvw(
 void launch_parent_uvw(const Topo &t, const double *u, const double *v, const double *w,
                        double *ubuf, double *vbuf, double *wbuf, const int64_t *bbo,
                        hipStream_t s) {
+    MGP_RANGE("mgp:parent_uvw");
     if (t.nlist <= 0) return;
     k_parent_uvw<<<t.nlist, kBoundThreads, 0, s>>>(t, u, v, w, ubuf, vbuf, wbuf, bbo);
 }
@@ -528,6 +529,7 @@ void launch_bound_flow(const Topo &t, double *u, double *v, double *w, double *p
                        const int64_t *bbo, const double *pinf, const double *bp,
                        const double *ddx, const double *ddy, const double *ddz, double rho,
                        hipStream_t s) {
+    MGP_RANGE("mgp:bound_flow");
     if (t.nlist <= 0) return;
     k_bound_flow<<<t.nlist, kBoundThreads, 0, s>>>(t, u, v, w, p, ubuf, vbuf, wbuf, bbo, pinf,
                                                    bp, ddx, ddy, ddz, rho);

@@ -218,6 +218,7 @@ static void ofw_launch(const Topo &t, const Fields<T> &f, Mask m, Outer o, T *rh
 template <typename T>
 void launch_outer_fwd(const Topo &t, const Fields<T> &f, Mask m, Outer o, T *rhs_t,
                       int rd, int write_hi, int classic, hipStream_t s) {
+    MGP_RANGE("mgp:outer_fwd");
     const size_t mRS = (size_t)(t.max_nj + 1) * (t.max_ni + 1);
     const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
     const bool masked = m.bp != nullptr;

@@ -11,11 +11,24 @@
 // face 2/3 (-y/+y) (k, i), face 4/5 (-z/+z) (j, i), each anti-diagonal-
 // major (see face_index).
 #pragma once
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
 namespace mgp {
+
+// roctx range around a host call (a C-ABI entry point or a kernel launch):
+// shown by rocprofv3 --marker-trace, a few tens of ns without a profiler.
+struct RoctxRange {
+    explicit RoctxRange(const char *name) { roctxRangePushA(name); }
+    ~RoctxRange() { roctxRangePop(); }
+    RoctxRange(const RoctxRange &) = delete;
+    RoctxRange &operator=(const RoctxRange &) = delete;
+};
+#define MGP_RANGE_CAT2(a, b) a##b
+#define MGP_RANGE_CAT(a, b) MGP_RANGE_CAT2(a, b)
+#define MGP_RANGE(name) ::mgp::RoctxRange MGP_RANGE_CAT(mgp_range_, __LINE__)(name)
 
 constexpr int kMaxN = 1000;   // interior cells per direction (10-bit packing)
 constexpr int kMaxN1 = 64;    // ... supported by the plane kernels (LDS rows)

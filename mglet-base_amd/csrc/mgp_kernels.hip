@@ -31,12 +31,14 @@ __global__ void k_fill(T *p, int64_t n, T v) {
 
 template <typename T>
 void launch_fill(T *p, int64_t n, T v, hipStream_t s) {
+    MGP_RANGE("mgp:fill");
     if (n <= 0) return;
     int64_t nb = (n + 255) / 256;
     if (nb > 65536) nb = 65536;
     k_fill<T><<<(unsigned)nb, 256, 0, s>>>(p, n, v);
 }
 void launch_fill_u8(uint8_t *p, int64_t n, uint8_t v, hipStream_t s) {
+    MGP_RANGE("mgp:fill_u8");
     launch_fill<uint8_t>(p, n, v, s);
 }
 template void launch_fill<double>(double *, int64_t, double, hipStream_t);
@@ -51,6 +53,7 @@ __global__ void k_convert(const double *src, T *dst, int64_t n) {
 }
 template <typename T>
 void launch_convert(const double *src, T *dst, int64_t n, hipStream_t s) {
+    MGP_RANGE("mgp:convert");
     if (n <= 0) return;
     int64_t nb = (n + 255) / 256;
     if (nb > 65536) nb = 65536;
@@ -142,6 +145,7 @@ void launch_ap(const Topo &t, const double *dx, const double *dy,
                const double *dz, const double *aw, const double *ae,
                const double *as, const double *an, const double *ab,
                const double *at, Mask m, double *ap, hipStream_t s) {
+    MGP_RANGE("mgp:ap");
     const int maxc = t.max_ni * t.max_nj * t.max_nk;
     dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
     k_ap<<<grid, kThreads, 0, s>>>(t, dx, dy, dz, aw, ae, as, an, ab, at, m,
@@ -235,6 +239,7 @@ void launch_lu(const Topo &t, const double *ap, const double *aw,
                const double *ae, const double *as, const double *an,
                const double *ab, const double *at, Mask m, double alfa,
                T *const f[7], hipStream_t s) {
+    MGP_RANGE("mgp:lu");
     const size_t lds = 6 * (size_t)(t.max_nj + 1) * (t.max_ni + 1) *
                        sizeof(double);
     k_lu<T><<<t.nlist, kThreads, lds, s>>>(t, ap, aw, ae, as, an, ab, at, m,
@@ -342,6 +347,7 @@ __global__ void k_fgather(Topo t, F7<T> F, F7<T> G, float *ap32) {
 template <typename T>
 void launch_fhash(const Topo &t, const T *const f[7], const double *ap, Mask m,
                   unsigned long long *out, hipStream_t s) {
+    MGP_RANGE("mgp:fhash");
     F7<T> F;
     for (int q = 0; q < 7; ++q) F.f[q] = f[q];
     F.ap = ap;
@@ -350,6 +356,7 @@ void launch_fhash(const Topo &t, const T *const f[7], const double *ap, Mask m,
 template <typename T>
 void launch_fcmp(const Topo &t, const T *const f[7], const double *ap, Mask m,
                  const int *rep, int *bad, hipStream_t s) {
+    MGP_RANGE("mgp:fcmp");
     F7<T> F;
     for (int q = 0; q < 7; ++q) F.f[q] = f[q];
     F.ap = ap;
@@ -360,6 +367,7 @@ void launch_fcmp(const Topo &t, const T *const f[7], const double *ap, Mask m,
 template <typename T>
 void launch_fgather(const Topo &t, const T *const f[7], const double *ap,
                     T *const g[7], double *gap, float *gap32, hipStream_t s) {
+    MGP_RANGE("mgp:fgather");
     F7<T> F, G;
     for (int q = 0; q < 7; ++q) {
         F.f[q] = f[q];
@@ -446,6 +454,7 @@ void launch_divcal(const Topo &t, const double *u, const double *v,
                    const double *w, const double *rddx, const double *rddy,
                    const double *rddz, const double *bp_sp, const double *sdiv_sp,
                    double fak, double *rhs64, T *rhs_t, hipStream_t s) {
+    MGP_RANGE("mgp:divcal");
     k_divcal<T><<<t.nlist, kStageThreads, stage_bytes(t), s>>>(
         t, u, v, w, rddx, rddy, rddz, bp_sp, sdiv_sp, fak, rhs64, rhs_t);
 }
@@ -510,6 +519,7 @@ __global__ void k_maxabs_store(Outer o) {
 
 void launch_maxabs(const Topo &t, const double *rhs64, Outer o,
                    hipStream_t s) {
+    MGP_RANGE("mgp:maxabs");
     launch_fill<unsigned long long>(o.gmax, 1, 0ull, s);
     k_maxabs<<<t.nlist, kThreads, 0, s>>>(t, rhs64, o);
     k_maxabs_store<<<1, 1, 0, s>>>(o);
@@ -522,6 +532,7 @@ template void launch_fill<unsigned long long>(unsigned long long *, int64_t,
 // 288-298)
 void launch_maxabs_acc(const Topo &t, const double *rhs64, Outer o,
                        hipStream_t s) {
+    MGP_RANGE("mgp:maxabs_acc");
     launch_fill<unsigned long long>(o.gmax, 1, 0ull, s);
     k_maxabs<<<t.nlist, kThreads, 0, s>>>(t, rhs64, o);
 }
@@ -540,6 +551,7 @@ __global__ void k_rap(const double *ap, T *rap, int64_t n) {
 
 template <typename T>
 void launch_rap(const double *ap, T *rap, int64_t n, hipStream_t s) {
+    MGP_RANGE("mgp:rap");
     if (n <= 0) return;
     int64_t nb = (n + 255) / 256;
     if (nb > 65536) nb = 65536;
@@ -572,6 +584,7 @@ __global__ void k_check(Outer o, int nouter, int nouter_min, double epcorr,
 void launch_check(Outer o, int nouter, int nouter_min, double epcorr,
                   double eps_rel, double prefak, double *hist, int hist_cap,
                   hipStream_t s) {
+    MGP_RANGE("mgp:check");
     k_check<<<1, 1, 0, s>>>(o, nouter, nouter_min, epcorr, eps_rel, prefak,
                             hist, hist_cap);
 }
@@ -606,6 +619,7 @@ __global__ __launch_bounds__(kStageThreads) void k_wo_to_sp(Topo t, const T *src
 template <typename T>
 void launch_wo_to_sp(const Topo &t, const T *src, double *dst,
                      hipStream_t s) {
+    MGP_RANGE("mgp:wo_to_sp");
     k_wo_to_sp<T><<<t.nlist, kStageThreads, stage_bytes(t), s>>>(t, src, dst);
 }
 template void launch_wo_to_sp<double>(const Topo &, const double *, double *,
@@ -638,6 +652,7 @@ __global__ void k_to_ref_factors(Topo t, const T *wo, double *ref,
 template <typename T>
 void launch_to_ref_factors(const Topo &t, const T *wo, double *ref,
                            int spatial, hipStream_t s) {
+    MGP_RANGE("mgp:to_ref_factors");
     const int maxc = t.max_ni * t.max_nj * t.max_nk;
     dim3 grid((maxc + kThreads - 1) / kThreads, t.nlist);
     k_to_ref_factors<T><<<grid, kThreads, 0, s>>>(t, wo, ref, spatial);
@@ -727,6 +742,7 @@ void launch_mgpcorr(const Topo &t, double *u, double *v, double *w,
                     double *p, const double *dp, const double *rdx,
                     const double *rdy, const double *rdz, double rfak,
                     const double *bp, hipStream_t s) {
+    MGP_RANGE("mgp:mgpcorr");
     const int maxe = (t.max_ni + 2) * (t.max_nj + 2) * (t.max_nk + 4);
     dim3 grid((maxe + kThreads - 1) / kThreads, t.nlist);
     k_mgpcorr<<<grid, kThreads, 0, s>>>(t, u, v, w, p, dp, rdx, rdy, rdz, rfak, bp);
@@ -797,6 +813,7 @@ void launch_pull_nf(const PullBlock *blocks, int nblock, const uint2 *ents, int 
 
 void launch_pull(const PullBlock *blocks, int nblock, const uint2 *ents, int mode,
                  double *const *fields, int nfield, hipStream_t s) {
+    MGP_RANGE("mgp:pull");
     if (nblock <= 0 || nfield <= 0) return;
     FieldList fl{};
     for (int f = 0; f < nfield && f < 6; ++f) fl.f[f] = fields[f];

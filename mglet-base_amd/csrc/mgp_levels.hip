@@ -668,6 +668,7 @@ __global__ void k_ftoc_uvwp(Topo t, double *u, double *v, double *w,
 template <typename T>
 void launch_ctof(const Topo &t, const Fields<T> &f, T *fout, T *hsh,
                  double *dsh, const int *done, hipStream_t s) {
+    MGP_RANGE("mgp:ctof");
     const int maxc = t.max_ni * t.max_nj * t.max_nk;
     dim3 grid((maxc + kThreadsLv - 1) / kThreadsLv, t.nlist);
     k_ctof<T><<<grid, kThreadsLv, 0, s>>>(t, f, fout, hsh, dsh, done);
@@ -703,6 +704,7 @@ __global__ void k_shell_to_sp(Topo t, const double *dsh, double *sp) {
 
 void launch_shell_to_sp(const Topo &t, const double *dsh, double *sp,
                         hipStream_t s) {
+    MGP_RANGE("mgp:shell_to_sp");
     const int maxs = shell_size(t.max_ni, t.max_nj, t.max_nk);
     dim3 grid((maxs + kThreadsLv - 1) / kThreadsLv, t.nlist);
     k_shell_to_sp<<<grid, kThreadsLv, 0, s>>>(t, dsh, sp);
@@ -711,6 +713,7 @@ void launch_shell_to_sp(const Topo &t, const double *dsh, double *sp,
 template <typename T>
 void launch_parent(const Topo &t, const Fields<T> &f, T *pbuf, const int *done,
                    hipStream_t s) {
+    MGP_RANGE("mgp:parent");
     k_parent<T><<<t.nlist, kThreadsLv, 0, s>>>(t, f.hilf, f.hface, f.hpar, pbuf, done);
 }
 
@@ -718,6 +721,7 @@ template <typename T>
 void launch_bound(const Topo &t, const Fields<T> &f, const T *pbuf, Mask m,
                   int bpv, const double *const metric[6], const int *done,
                   hipStream_t s) {
+    MGP_RANGE("mgp:bound");
     if (bpv)
         k_bound<T, true><<<t.nlist, kThreadsLv, 0, s>>>(
             t, f.hilf, f.hpar, pbuf, m, metric[0], metric[1], metric[2],
@@ -732,6 +736,7 @@ template <typename S, typename D>
 void launch_restrict(const Topo &t, const S *src, D *dst, D *face,
                      float *dst2, Mask m, int mode, const double *const dd[3],
                      const int *done, hipStream_t s) {
+    MGP_RANGE("mgp:restrict");
     const int maxg = (t.max_ni / 2) * (t.max_nj / 2) * (t.max_nk / 2);
     dim3 grid((maxg + kThreadsLv - 1) / kThreadsLv, t.nlist);
     if (mode == 1)
@@ -747,12 +752,14 @@ void launch_restrict(const Topo &t, const S *src, D *dst, D *face,
 
 void launch_parent_sp(const Topo &t, const double *f, double *pbuf,
                       hipStream_t s) {
+    MGP_RANGE("mgp:parent_sp");
     k_parent_sp<<<t.nlist, kThreadsLv, 0, s>>>(t, f, pbuf);
 }
 
 void launch_bound_sp(const Topo &t, double *p, const double *pbuf,
                      const double *bp, int bpv, const double *const metric[6],
                      hipStream_t s) {
+    MGP_RANGE("mgp:bound_sp");
     if (bpv)
         k_bound_sp<true><<<t.nlist, kThreadsLv, 0, s>>>(
             t, p, pbuf, bp, metric[0], metric[1], metric[2], metric[3],
@@ -766,6 +773,7 @@ void launch_bound_sp(const Topo &t, double *p, const double *pbuf,
 void launch_ftoc_uvwp(const Topo &t, double *u, double *v, double *w,
                       double *p, int gc, const double *bp,
                       const double *const dd[3], hipStream_t s) {
+    MGP_RANGE("mgp:ftoc_uvwp");
     const int maxn = 4 * (t.max_ni / 2 + 1) * (t.max_nj / 2 + 1) * (t.max_nk / 2 + 1);
     dim3 grid((maxn + kThreadsLv - 1) / kThreadsLv, t.nlist);
     k_ftoc_uvwp<<<grid, kThreadsLv, 0, s>>>(t, u, v, w, p, dd[0], dd[1], dd[2], gc, bp);

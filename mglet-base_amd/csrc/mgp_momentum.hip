@@ -237,6 +237,7 @@ __global__ __launch_bounds__(kMomThreads) void k_tstle4(Topo t, MomArgs a, int n
 }
 
 void launch_tstle4(const Topo &t, const MomArgs &a, hipStream_t s) {
+    MGP_RANGE("mgp:tstle4");
     const int maxc = (t.max_ni + 4) * (t.max_nj + 4) * (t.max_nk + 4);
     const int nchunk = (maxc + kMomThreads - 1) / kMomThreads;
     // launches of at most 2^31 work-items
@@ -260,6 +261,7 @@ __global__ void k_rkstep(double *p, double *dp, const double *rhsp,
 
 void launch_rkstep(double *p, double *dp, const double *rhsp, int64_t n,
                    double frhs, double dtfu, hipStream_t s) {
+    MGP_RANGE("mgp:rkstep");
     if (n <= 0) return;
     int64_t nb = (n + kMomThreads - 1) / kMomThreads;
     if (nb > 65536) nb = 65536;

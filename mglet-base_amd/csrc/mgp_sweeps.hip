@@ -607,6 +607,7 @@ template <typename T>
 void launch_forward(const Topo &t, const Fields<T> &f, Mask m, int rd,
                     int write_hi, int hilf_zero, int classic, const int *done,
                     hipStream_t s) {
+    MGP_RANGE("mgp:forward");
     const size_t mRS = (size_t)(t.max_nj + 1) * (t.max_ni + 1);
     const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
     const bool masked = m.bp != nullptr;
@@ -649,6 +650,7 @@ static void bwd_launch(const Topo &t, const Fields<T> &f, const int *done,
 template <typename T>
 void launch_backward(const Topo &t, const Fields<T> &f, int hilf_zero,
                      int classic, const int *done, hipStream_t s) {
+    MGP_RANGE("mgp:backward");
     const size_t mRS = (size_t)(t.max_nj + 1) * (t.max_ni + 1);
     const size_t lds = tab_bytes<T>() + 2 * mRS * sizeof(T);
     if (classic) {
@@ -683,6 +685,7 @@ template <typename T>
 void launch_sor(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
                 T *fout, double omega, int hilf_zero, const int *done,
                 hipStream_t s) {
+    MGP_RANGE("mgp:sor");
     const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
     const bool masked = m.bp != nullptr;
     const size_t nr = row_len(t);
@@ -720,6 +723,7 @@ static void out_launch(const Topo &t, const Fields<T> &f, Mask m, Outer o,
 template <typename T>
 void launch_outer(const Topo &t, const Fields<T> &f, Mask m, Outer o,
                   T *rhs_t, hipStream_t s) {
+    MGP_RANGE("mgp:outer");
     const size_t mHS = (size_t)(t.max_nj + 2) * (t.max_ni + 2);
     const bool masked = m.bp != nullptr;
     const size_t nr = row_len(t);

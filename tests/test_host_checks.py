@@ -74,3 +74,29 @@ def test_config_tail_field_layout():
     hdr = (ROOT / "include" / "mgp.h").read_text()
     assert hdr.index("mgletreal alfa;") < hdr.index("mgletint sync_by_caller;") < \
         hdr.index("} mgp_config;")
+
+
+def test_roctx_ranges_on_entry_points_and_launchers():
+    """Every C-ABI entry point that does work and every kernel launcher opens
+    a roctx range (src/core/timer_mod.F90:114 times the same calls), and the
+    library links the roctx runtime."""
+    import re
+    import subprocess
+    csrc = ROOT / "mglet-base_amd" / "csrc"
+    api = (csrc / "mgp_api.cpp").read_text()
+    body = api[api.index('extern "C" {'):]
+    entries = re.findall(r'^int (mgp_\w+)\([^{]*\{\s*\n\s*MGP_RANGE\("(\w+)"\)', body, re.M)
+    declared = re.findall(r'^int (mgp_\w+)\(', body, re.M)
+    assert declared and [e[0] for e in entries] == declared
+    assert all(a == b for a, b in entries)
+    for f in csrc.glob("*.hip"):
+        s = f.read_text()
+        for m in re.finditer(r'^void (launch_\w+)\(', s, re.M):
+            if m.group(1) == "launch_pull_nf":      # called by launch_pull only
+                continue
+            b = s.index("{", m.end())
+            assert "MGP_RANGE(" in s[b:b + 80], (f.name, m.group(1))
+    from mgp import _lib
+    out = subprocess.run(["readelf", "-d", str(_lib.LIB_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    assert "roctx" in out
