@@ -194,9 +194,9 @@ class Oracle:
                            _p(sdiv) if sdiv is not None else None)
 
     def tstle4(self, u, v, w, p, g_visc, rho, gmol, ilesmodel=0, gradp=None):
-        """Momentum right-hand side (src/flow/tstle4_mod.F90:13-529, noib,
-        with the Werner-Wengle wall stress next to NOS walls): returns
-        (uo, vo, wo)."""
+        """Momentum right-hand side (src/flow/tstle4_mod.F90:13-1753, noib,
+        with the Werner-Wengle wall stress next to NOS walls and tstle4_par
+        at PAR faces): returns (uo, vo, wo)."""
         g = self.g
         uo, vo, wo = g.zeros(), g.zeros(), g.zeros()
         gp = np.asarray(gradp, dtype=np.float64) if gradp is not None else None
@@ -207,8 +207,17 @@ class Oracle:
                                 _p(g.rdx), _p(g.rdy), _p(g.rdz), _p(g.rddx), _p(g.rddy),
                                 _p(g.rddz))
         if rc != 0:
-            raise ValueError("tstle4: PAR faces are not restated")
+            raise ValueError(f"og_tstle4 failed ({rc})")
         return uo, vo, wo
+
+    def tstle4_par(self, uo, vo, wo, u, v, w):
+        """tstle4_par alone (src/flow/tstle4_mod.F90:532-1753): adds the PAR
+        face convection terms to uo, vo, wo in place."""
+        g = self.g
+        self.lib.og_tstle4_par(C.byref(self.topo), _p(uo), _p(vo), _p(wo), _p(u), _p(v),
+                               _p(w), _p(g.dx), _p(g.dy), _p(g.dz), _p(g.ddx), _p(g.ddy),
+                               _p(g.ddz), _p(g.rdx), _p(g.rdy), _p(g.rdz), _p(g.rddx),
+                               _p(g.rddy), _p(g.rddz))
 
     def rkstep(self, p, dp, rhsp, frhs, dtfu):
         """src/core/rungekutta_mod.F90:315-333, in place on p and dp."""

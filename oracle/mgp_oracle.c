@@ -2072,10 +2072,10 @@ double og_time_sip(const og_topo *t, const og_conn *c, int nconn, int niter,
  * uo = convection (tstle4_kon :139-265), minus the Werner-Wengle wall
  * stress next to NOS walls (swcle3d, :1756-1845, called at the start of
  * tstle4_diff :315), + diffusion (tstle4_diff :268-458) - pressure gradient
- * (tstle4_gradp :461-529).  ut = u etc. (noib).  Each expression keeps the
+ * (tstle4_gradp :461-529), then tstle4_par at PAR faces (:532-1753, see
+ * og_tstle4_par).  ut = u etc. (noib).  Each expression keeps the
  * reference's association order (Fortran evaluates operators of equal
- * precedence left to right; a leading minus applies to the whole product).
- * Returns -1 for PAR faces (8, tstle4_par), which are not restated. */
+ * precedence left to right; a leading minus applies to the whole product). */
 static double gpair(double a, double b, double gmol)
 {
     const double s = a + b;
@@ -2136,6 +2136,212 @@ static double swc_terms(double val, int comp, int k, int j, int i, const int32_t
     return val;
 }
 
+/* ------------------------------------------------------------------ */
+/* tstle4_par (src/flow/tstle4_mod.F90:532-1753): the convection terms next
+ * to PAR faces, added to uo, vo, wo of one grid after kon, diff and gradp.
+ * The reference writes each face out by hand; every instance has the same
+ * form, restated here once for a face with normal axis n (0 x, 1 y, 2 z),
+ * side lo (faces 1, 3, 5) or hi (2, 4, 6), and a tangential momentum
+ * component c, with o the third axis.  Positions along n: L the first
+ * interior layer (3, or N-2), G the ghost layer beside it (2, or N-1), T
+ * the face between them (G at lo faces, L at hi faces), where the normal
+ * transport velocity lives.
+ *
+ * 1. Upwind at the second layer (:571-776), faces in the order 1..6, both
+ *    tangential components over a, b in 3..N-2: with i = 4 (lo) or N-2
+ *    (hi) along n, f = area*(t(i-1; a) + t(i-1; a+1))*0.5 and
+ *    q = 0.5*(f -|f|)*(0.5*c(i) - 1.5*c(i-1) + c(i-2))*0.5*0.5 (lo), or
+ *    q = 0.5*(f +|f|)*(0.5*c(i-1) - 1.5*c(i) + c(i+1))*0.5*0.5 (hi);
+ *    co(i) += F*(-q), co(i-1) += F*q, with F of position i for both.
+ * 2. The momentum-conserving flux (:778-1750), faces in the order 2, 1, 6,
+ *    5, 4, 3; x faces w then v, y faces u then w, z faces u then v.  Over
+ *    a in 2..Nc-2 (the component's own axis) and b in 3..No-2: the standard
+ *    flux qs = 0.5*f*(c(L) + c(G)), f = area*(t(T; a) + t(T; a+1))*0.5;
+ *    the coarse-consistent flux qn at even a, shared by the pair (b, b+1)
+ *    (b = 3, 5, ..): f = area(a, b)*S*0.125 with S the sum of t(T) over
+ *    a-1..a+2 and b, b+1, the outermost a-1 or a+2 replaced by doubling the
+ *    inner end at a = 2 and a = Nc-2; odd a take the mean of the even
+ *    neighbours; then co(L) += F*(qs - qn) (lo) or F*(qn - qs) (hi).
+ *    (The reference's LEFT/RIGHT w cells also compute qn at odd a, which
+ *    the mean then overwrites.)
+ * area = d_c(a)*dd_o(b) (the momentum cell's face); F = -(rX*rY)*rZ of the
+ * updated cell, rX = rdx for u else rddx, rY = rdy for v else rddy, rZ =
+ * rdz for w else rddz, grouped (x*y)*z as the reference's fkdt*rz.  t is
+ * the transport velocity, u, v, w themselves (no immersed boundary). */
+typedef struct par_grid {
+    int kk, jj, ii, n[3];
+    double *out[3];
+    const double *f[3];
+    const double *d[3], *dd[3], *rd[3], *rdd[3];   /* 1-based */
+} par_grid;
+
+static int64_t par_ix(const par_grid *P, int ax0, int v0, int ax1, int v1, int ax2, int v2)
+{
+    int x[3];
+    x[ax0] = v0;
+    x[ax1] = v1;
+    x[ax2] = v2;
+    return (int64_t)(x[2] - 1) + (int64_t)(x[1] - 1) * P->kk + (int64_t)(x[0] - 1) * P->kk * P->jj;
+}
+
+static double par_cellfac(const par_grid *P, int comp, int ax0, int v0, int ax1, int v1,
+                          int ax2, int v2)
+{
+    int x[3];
+    x[ax0] = v0;
+    x[ax1] = v1;
+    x[ax2] = v2;
+    const double rx = comp == 0 ? P->rd[0][x[0]] : P->rdd[0][x[0]];
+    const double ry = comp == 1 ? P->rd[1][x[1]] : P->rdd[1][x[1]];
+    const double rz = comp == 2 ? P->rd[2][x[2]] : P->rdd[2][x[2]];
+    double fk = -1.0 * rx * ry;
+    fk = fk * 1.0;                 /* wkon */
+    return fk * rz;
+}
+
+static void par_upwind(par_grid *P, int n, int hi)
+{
+    const int N = P->n[n];
+    const int iu = hi ? N - 2 : 4;
+    const double *tn = P->f[n];
+    for (int c = 0; c < 3; ++c) {
+        if (c == n) continue;
+        const int o = 3 - n - c;
+        const double *cf = P->f[c];
+        double *co = P->out[c];
+        for (int a = 3; a <= P->n[c] - 2; ++a)
+            for (int b = 3; b <= P->n[o] - 2; ++b) {
+                const double area = P->d[c][a] * P->dd[o][b];
+                const double f = area * (tn[par_ix(P, n, iu - 1, c, a, o, b)] +
+                                         tn[par_ix(P, n, iu - 1, c, a + 1, o, b)]) * 0.5;
+                double q;
+                if (!hi)
+                    q = 0.5 * (f - fabs(f)) *
+                        (0.5 * cf[par_ix(P, n, iu, c, a, o, b)] - 1.5 * cf[par_ix(P, n, iu - 1, c, a, o, b)] +
+                         cf[par_ix(P, n, iu - 2, c, a, o, b)]) * 0.5 * 0.5;
+                else
+                    q = 0.5 * (f + fabs(f)) *
+                        (0.5 * cf[par_ix(P, n, iu - 1, c, a, o, b)] - 1.5 * cf[par_ix(P, n, iu, c, a, o, b)] +
+                         cf[par_ix(P, n, iu + 1, c, a, o, b)]) * 0.5 * 0.5;
+                const double F = par_cellfac(P, c, n, iu, c, a, o, b);
+                const int64_t x0 = par_ix(P, n, iu, c, a, o, b), x1 = par_ix(P, n, iu - 1, c, a, o, b);
+                co[x0] = co[x0] + F * (-q);
+                co[x1] = co[x1] + F * q;
+            }
+    }
+}
+
+static void par_conserve(par_grid *P, int n, int hi, int c)
+{
+    const int N = P->n[n], o = 3 - n - c;
+    const int L = hi ? N - 2 : 3, G = hi ? N - 1 : 2, T = hi ? L : G;
+    const int Nc = P->n[c], No = P->n[o];
+    const double *tn = P->f[n], *cf = P->f[c];
+    double *co = P->out[c];
+    double *qs = calloc((size_t)(Nc + 1) * (No + 1), sizeof(double));
+    double *qn = calloc((size_t)(Nc + 1) * (No + 1), sizeof(double));
+#define Q(arr, a, b) arr[(size_t)(a) * (No + 1) + (b)]
+#define TN(a, b) tn[par_ix(P, n, T, c, a, o, b)]
+#define CL(a, b) cf[par_ix(P, n, L, c, a, o, b)]
+#define CG(a, b) cf[par_ix(P, n, G, c, a, o, b)]
+    for (int a = 2; a <= Nc - 2; ++a)
+        for (int b = 3; b <= No - 2; ++b) {
+            const double f = P->d[c][a] * P->dd[o][b] * (TN(a, b) + TN(a + 1, b)) * 0.5;
+            Q(qs, a, b) = 0.5 * f * (CL(a, b) + CG(a, b));
+        }
+    for (int a = 2; a <= Nc - 2; a += 2)
+        for (int b = 3; b <= No - 3; b += 2) {
+            const double area = P->d[c][a] * P->dd[o][b];
+            double sum;
+            if (a == 2)
+                sum = 2.0 * TN(a, b) + 2.0 * TN(a, b + 1) + TN(a + 1, b) + TN(a + 1, b + 1) +
+                      TN(a + 2, b) + TN(a + 2, b + 1);
+            else if (a == Nc - 2)
+                sum = TN(a - 1, b) + TN(a - 1, b + 1) + TN(a, b) + TN(a, b + 1) +
+                      2.0 * TN(a + 1, b) + 2.0 * TN(a + 1, b + 1);
+            else
+                sum = TN(a - 1, b) + TN(a - 1, b + 1) + TN(a, b) + TN(a, b + 1) + TN(a + 1, b) +
+                      TN(a + 1, b + 1) + TN(a + 2, b) + TN(a + 2, b + 1);
+            const double f = area * sum * 0.125;
+            Q(qn, a, b) = 0.5 * f * (CL(a, b) + CG(a, b));
+            Q(qn, a, b + 1) = 0.5 * f * (CL(a, b + 1) + CG(a, b + 1));
+        }
+    for (int a = 3; a <= Nc - 3; a += 2)
+        for (int b = 3; b <= No - 2; ++b)
+            Q(qn, a, b) = 0.5 * (Q(qn, a - 1, b) + Q(qn, a + 1, b));
+    for (int a = 2; a <= Nc - 2; ++a)
+        for (int b = 3; b <= No - 2; ++b) {
+            const int64_t x = par_ix(P, n, L, c, a, o, b);
+            const double F = par_cellfac(P, c, n, L, c, a, o, b);
+            const double dq = hi ? Q(qn, a, b) - Q(qs, a, b) : Q(qs, a, b) - Q(qn, a, b);
+            co[x] = co[x] + F * dq;
+        }
+#undef Q
+#undef TN
+#undef CL
+#undef CG
+    free(qs);
+    free(qn);
+}
+
+static void par_grid_run(par_grid *P, const int32_t *bc)
+{
+    /* faces 1..6 = (n, hi): (0,0) (0,1) (1,0) (1,1) (2,0) (2,1) */
+    for (int f = 0; f < 6; ++f)
+        if (bc[f] == 8) par_upwind(P, f / 2, f & 1);
+    static const int order[6] = {1, 0, 5, 4, 3, 2};
+    static const int comps[3][2] = {{2, 1}, {0, 2}, {0, 1}};
+    for (int q = 0; q < 6; ++q) {
+        const int f = order[q];
+        if (bc[f] != 8) continue;
+        par_conserve(P, f / 2, f & 1, comps[f / 2][0]);
+        par_conserve(P, f / 2, f & 1, comps[f / 2][1]);
+    }
+}
+
+static void par_setup(par_grid *P, const og_topo *t, int g, double *uo_, double *vo_,
+                      double *wo_, const double *u_, const double *v_, const double *w_,
+                      const double *dx_, const double *dy_, const double *dz_,
+                      const double *ddx_, const double *ddy_, const double *ddz_,
+                      const double *rdx_, const double *rdy_, const double *rdz_,
+                      const double *rddx_, const double *rddy_, const double *rddz_)
+{
+    GDIMS(t, g);
+    const int64_t o3 = t->ip3[g - 1];
+    const int64_t ox = t->ip1x[g - 1] - 1, oy = t->ip1y[g - 1] - 1, oz = t->ip1z[g - 1] - 1;
+    P->kk = kk;
+    P->jj = jj;
+    P->ii = ii;
+    P->n[0] = ii;
+    P->n[1] = jj;
+    P->n[2] = kk;
+    P->out[0] = uo_ + o3;
+    P->out[1] = vo_ + o3;
+    P->out[2] = wo_ + o3;
+    P->f[0] = u_ + o3;
+    P->f[1] = v_ + o3;
+    P->f[2] = w_ + o3;
+    P->d[0] = dx_ + ox; P->d[1] = dy_ + oy; P->d[2] = dz_ + oz;
+    P->dd[0] = ddx_ + ox; P->dd[1] = ddy_ + oy; P->dd[2] = ddz_ + oz;
+    P->rd[0] = rdx_ + ox; P->rd[1] = rdy_ + oy; P->rd[2] = rdz_ + oz;
+    P->rdd[0] = rddx_ + ox; P->rdd[1] = rddy_ + oy; P->rdd[2] = rddz_ + oz;
+}
+
+void og_tstle4_par(const og_topo *t, double *uo, double *vo, double *wo,
+                   const double *u, const double *v, const double *w,
+                   const double *dx, const double *dy, const double *dz,
+                   const double *ddx, const double *ddy, const double *ddz,
+                   const double *rdx, const double *rdy, const double *rdz,
+                   const double *rddx, const double *rddy, const double *rddz)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        par_grid P;
+        par_setup(&P, t, g, uo, vo, wo, u, v, w, dx, dy, dz, ddx, ddy, ddz, rdx, rdy, rdz,
+                  rddx, rddy, rddz);
+        par_grid_run(&P, t->bc + 6 * (g - 1));
+    }
+}
+
 int og_tstle4(const og_topo *t, double *uo_, double *vo_, double *wo_,
               const double *u_, const double *v_, const double *w_,
               const double *p_, const double *g_, double rho, double gmol,
@@ -2145,9 +2351,6 @@ int og_tstle4(const og_topo *t, double *uo_, double *vo_, double *wo_,
               const double *rdx_, const double *rdy_, const double *rdz_,
               const double *rddx_, const double *rddy_, const double *rddz_)
 {
-    for (int g = 1; g <= t->ngrid; ++g)
-        for (int f = 0; f < 6; ++f)
-            if (t->bc[6 * (g - 1) + f] == 8) return -1;
     const double iles = ilesmodel == 0 ? 0.0 : 1.0;
     double ww[5];
     og_ww_constants(rho, gmol, ww);
@@ -2316,6 +2519,11 @@ int og_tstle4(const og_topo *t, double *uo_, double *vo_, double *wo_,
                     if (!(iI && ij && k >= 3 - nbw && k <= kk - 3 + ntw))
                         wo[c] = swc_terms(wo[c], 2, k, j, i, bc, kk, jj, ii, w[c], ddx, ddy, ddz, rho, gmol, ww);
                 }
+        /* tstle4_par after kon, diff and gradp of the grid (:114-116) */
+        par_grid P;
+        par_setup(&P, t, g, uo_, vo_, wo_, u_, v_, w_, dx_, dy_, dz_, ddx_, ddy_, ddz_, rdx_,
+                  rdy_, rdz_, rddx_, rddy_, rddz_);
+        par_grid_run(&P, bc);
     }
     return 0;
 }

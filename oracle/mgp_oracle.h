@@ -223,8 +223,7 @@ int og_mgpoisl(const og_topo *t, const og_conn *c, int nconn,
  * (src/flow/tstle4_mod.F90:13-529: convection, Werner-Wengle wall stress
  * next to NOS walls, diffusion with viscosity field g, pressure gradient
  * with the optional body-force gradient gradp[3] on every grid).  uo, vo,
- * wo are zeroed and then filled.  Returns -1 when a grid has a PAR face
- * (8: tstle4_par), which this restatement omits. */
+ * wo are zeroed and then filled, tstle4_par included at PAR faces. */
 /* Werner-Wengle constants cwb, cpo4, cpo5, cpo6, cpo8
  * (src/flow/wernerwengle_mod.F90:13-29) and the wall term swcle3d_one
  * (src/flow/tstle4_mod.F90:1848-1859, tauwin :64-87 of wernerwengle_mod). */
@@ -241,6 +240,17 @@ int og_tstle4(const og_topo *t, double *uo, double *vo, double *wo,
               const double *rddx, const double *rddy, const double *rddz);
 
 /* src/core/rungekutta_mod.F90:315-333: dp = frhs dp + rhsp; p += dtfu dp */
+/* tstle4_par alone (src/flow/tstle4_mod.F90:532-1753): adds the
+ * upwind and momentum-conserving convection terms at the PAR faces of every
+ * grid to uo, vo, wo (no immersed boundary: the transport velocity is u, v,
+ * w).  Metrics 1-D, reference layout. */
+void og_tstle4_par(const og_topo *t, double *uo, double *vo, double *wo,
+                   const double *u, const double *v, const double *w,
+                   const double *dx, const double *dy, const double *dz,
+                   const double *ddx, const double *ddy, const double *ddz,
+                   const double *rdx, const double *rdy, const double *rdz,
+                   const double *rddx, const double *rddy, const double *rddz);
+
 void og_rkstep(int64_t n, double *p, double *dp, const double *rhsp,
                double frhs, double dtfu);
 
