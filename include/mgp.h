@@ -230,6 +230,20 @@ int mgp_divcal_device(mgp_handle *h, mgletreal *div, const mgletreal *u,
 int mgp_divcal_device_noalloc(mgp_handle *h, mgletreal *div, const mgletreal *u,
                               const mgletreal *v, const mgletreal *w, mgletreal fak);
 
+/* The velocity boundary conditions of one level (device fields, reference
+ * layout): parent (src/ib/parent/parent_mod.F90:16-52; prolong1/prolong2
+ * with the staggered positions, src/ib/parent/parent1_mod.F90:515-583)
+ * fills the library's PAR face buffers of the level's refined grids from
+ * their parents' U, V, W, then bound_flow%bound (src/flow/bound_flow_mod.F90)
+ * sets the ghost values at FIX, OP1, NOS, SLI and PAR faces (FIX / OP1
+ * values and OP1 pressures from mgp_set_bound).  The pair the reference
+ * runs per level before the divergence of an RK stage, after the normal
+ * exchange (src/flow/timeintegration_mod.F90:119-123), and inside
+ * mgpoisl's closing sync, before the two-layer exchange
+ * (src/flow/pressuresolver_mod.F90:338-348). */
+int mgp_flow_bc_device(mgp_handle *h, mgletint level, mgletreal *u, mgletreal *v,
+                       mgletreal *w, mgletreal *p);
+
 /* The RK stage around the projection (device fields, reference layout).
  *
  * tstle4 (src/flow/tstle4_mod.F90:13-120, no immersed boundary): uo, vo, wo
@@ -238,7 +252,8 @@ int mgp_divcal_device_noalloc(mgp_handle *h, mgletreal *div, const mgletreal *u,
  * of the stress) - grad p / rho - gradp[3] / rho (gradp may be NULL), zero
  * outside the loop ranges; next to NOS walls the Werner-Wengle wall stress
  * of swcle3d (:1756-1845) is subtracted from the tangential components.
- * rho is the handle's.  Grids with PAR faces are rejected (MGP_ERR_TOPOLOGY).
+ * rho is the handle's.  At the PAR faces of refined grids tstle4_par
+ * (:532-1753) adds its upwind and momentum-conserving convection terms.
  * Replaces the call at src/flow/timeintegration_mod.F90:98. */
 int mgp_tstle4_device(mgp_handle *h, mgletreal *uo, mgletreal *vo, mgletreal *wo,
                       const mgletreal *u, const mgletreal *v, const mgletreal *w,
@@ -260,15 +275,16 @@ int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms);
  * update): 1 = four 64-cell items of loads in flight per wave, 2 = two
  * 128-cell items, 3 = three 128-cell items, 4 = six 128-cell items, 5 =
  * eight 64-cell items, all one wave per block; 6 and 7 = four 64-cell
- * items per wave with 4 and 8 waves sharing each block (SOR keeps one
- * wave).  request -1 only reports; 0 runs the autotuner:
+ * items per wave with 4 and 8 waves sharing each block; 8 and 9 = three
+ * 128-cell and four 64-cell items per wave with 2 waves sharing each block
+ * (SOR keeps one wave).  request -1 only reports; 0 runs the autotuner:
  * it times one inner iteration per shape (ninner x forward + backward, or
  * SOR, plus one outer update) on h's own blocks and keeps the fastest;
  * 1..MGP_NUM_PLAN_SHAPES sets the shape.  Results do not depend on the
  * shape.  chosen gets the shape in use, ms[MGP_NUM_PLAN_SHAPES] (may be
  * NULL) the timed ms per shape (0 when not timed).  mgp_init autotunes
  * unless MGP_PLAN_SHAPE=1..MGP_NUM_PLAN_SHAPES is set. */
-#define MGP_NUM_PLAN_SHAPES 7
+#define MGP_NUM_PLAN_SHAPES 9
 int mgp_plan_shape(mgp_handle *h, mgletint request, mgletint *chosen,
                    double *ms);
 

@@ -198,6 +198,8 @@ struct mgp_handle {
     // which DP keeps in the ghost cells no exchange or bound_pressure sets
     DBuf d_hsh, d_dsh;
     std::set<int> level_par;   // levels with PAR faces
+    DBuf d_parlist;            // blocks with PAR faces (tstle4_par)
+    int npar = 0;
     // closing sync of mgpoisl: velocity boundary buffers of the FIX, OP1
     // and PAR faces (offsets per grid and face, -1 none), OP1 pressure per
     // grid and face, and the levels with a face bound_flow acts on
@@ -776,7 +778,7 @@ void build_plans(mgp_handle &h, const std::vector<uint32_t> &cells,
                 v.y = (uint32_t)epos | (uint32_t)(p + 8) << 21;
                 return v;
             };
-            for (int nwi = 0; nwi < 3; ++nwi) {
+            for (int nwi = 0; nwi < 4; ++nwi) {
                 const int NW = kPlanWaves[nwi];
                 for (int wi = 0; wi < 2; ++wi) {
                     const int W = 64 << wi;
@@ -1341,8 +1343,14 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
             h->d_rapT.alloc(h->ncell * tsz);
             h->d_hface2.alloc(h->nface * tsz);
         }
-        for (const auto &B : h->blocks)
-            if (B.par) h->level_par.insert(B.level);
+        std::vector<int> parlist;
+        for (int b = 0; b < (int)h->blocks.size(); ++b)
+            if (h->blocks[b].par) {
+                h->level_par.insert(h->blocks[b].level);
+                parlist.push_back(b);
+            }
+        h->npar = (int)parlist.size();
+        if (h->npar) upload(h->d_parlist, parlist);
         if (!h->level_par.empty()) {
             h->d_hpar.alloc(h->nface * tsz);
             h->d_pbuf.alloc(h->nface * tsz);
@@ -2060,6 +2068,33 @@ int mgp_connect_device(mgp_handle *h, mgletint level, mgletint layers,
     return MGP_OK;
 }
 
+int mgp_flow_bc_device(mgp_handle *h, mgletint level, mgletreal *u, mgletreal *v,
+                       mgletreal *w, mgletreal *p) {
+    MGP_RANGE("mgp_flow_bc_device");
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (!u || !v || !w || !p) return fail(MGP_ERR_ARG, "null field");
+    if (level < h->minlevel || level > h->maxlevel) return fail(MGP_ERR_ARG, "level out of range");
+    try {
+        if (h->level_blocks.count(level)) {
+            const Topo tl = h->topo(level);
+            if (h->level_par.count(level))
+                launch_parent_uvw(tl, u, v, w, h->d_ubnd.as<double>(), h->d_vbnd.as<double>(),
+                                  h->d_wbnd.as<double>(), h->d_bbo.as<int64_t>(), h->stream);
+            if (h->level_flow.count(level))
+                launch_bound_flow(tl, u, v, w, p, h->d_ubnd.as<double>(), h->d_vbnd.as<double>(),
+                                  h->d_wbnd.as<double>(), h->d_bbo.as<int64_t>(),
+                                  h->d_pinf.as<double>(), h->d_bpsp.as<double>(),
+                                  h->d_ddx.as<double>(), h->d_ddy.as<double>(),
+                                  h->d_ddz.as<double>(), h->cfg.rho, h->stream);
+        }
+        HC(hipGetLastError());
+        HC(hipStreamSynchronize(h->stream));
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    return MGP_OK;
+}
+
 // divcal into `tmp` (WO order, ncell values: every one written), then
 // scattered into the interior of `div`
 static void divcal_via(mgp_handle *h, double *div, const double *u, const double *v,
@@ -2180,11 +2215,6 @@ int mgp_tstle4_device(mgp_handle *h, mgletreal *uo, mgletreal *vo, mgletreal *wo
     MGP_RANGE("mgp_tstle4_device");
     if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
     if (!uo || !vo || !wo || !u || !v || !w || !p || !g) return fail(MGP_ERR_ARG, "null field");
-    for (int q = 0; q < h->ngrid; ++q)
-        for (int f = 0; f < 6; ++f)
-            if (h->bc[6 * q + f] == PAR_BC)
-                return fail(MGP_ERR_TOPOLOGY, "grid %d face %d: tstle4 does not cover PAR faces "
-                            "(tstle4_par)", q + 1, f + 1);
     try {
         MomArgs a;
         a.uo = uo; a.vo = vo; a.wo = wo;
@@ -2203,6 +2233,8 @@ int mgp_tstle4_device(mgp_handle *h, mgletreal *uo, mgletreal *vo, mgletreal *wo
         ww_constants(h->cfg.rho, gmol, a.ww);
         for (int q = 0; q < 3; ++q) a.gp[q] = gradp ? gradp[q] : 0.0;
         launch_tstle4(h->topo(-1), a, h->stream);
+        HC(hipGetLastError());
+        launch_tstle4_par(h->topo(-1), a, h->d_parlist.as<int>(), h->npar, h->stream);
         HC(hipGetLastError());
         HC(hipStreamSynchronize(h->stream));
     } catch (const HipError &e) {
@@ -2340,7 +2372,7 @@ int mgp_info(mgp_handle *h, int64_t *ncells, int64_t *device_bytes) {
                             &h->d_dp64, &h->d_bpw, &h->d_bpf, &h->d_bpsp, &h->d_sdiv, &h->d_hface, &h->d_rhi[0],
                             &h->d_rhi[1], &h->d_rlo, &h->d_rapT, &h->d_hface2, &h->d_ft[0], &h->d_ft[1], &h->d_ft[2],
                             &h->d_ft[3], &h->d_ft[4], &h->d_ft[5], &h->d_u, &h->d_v, &h->d_w, &h->d_p, &h->d_dpsp,
-                            &h->d_aps64, &h->d_apsT, &h->d_blist_rep};
+                            &h->d_aps64, &h->d_apsT, &h->d_blist_rep, &h->d_parlist};
         for (auto *b : bs) tot += b->n;
         for (auto &b : h->d_f) tot += b.n;
         *device_bytes = (int64_t)tot;

@@ -210,6 +210,8 @@ static void ofw_launch(const Topo &t, const Fields<T> &f, Mask m, Outer o, T *rh
     case 5: MGP_OFW(8, 1, 1); break;
     case 6: MGP_OFW(4, 1, 4); break;
     case 7: MGP_OFW(4, 1, 8); break;
+    case 8: MGP_OFW(3, 2, 2); break;
+    case 9: MGP_OFW(4, 1, 2); break;
     default: MGP_OFW(2, 2, 1); break;
     }
 #undef MGP_OFW

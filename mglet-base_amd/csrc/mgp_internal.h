@@ -45,8 +45,8 @@ struct Shape {
     // per wave count (kPlanWaves), item width (64, 128 cells) and plan
     // kind: offset of the shape's items, and items per wave (a multiple of
     // kPlanPad)
-    int plan_item[3][2][5];
-    int plan_nitem[3][2][5];
+    int plan_item[4][2][5];
+    int plan_nitem[4][2][5];
     int plan_ent[5];      // per plan kind: offset of the shape's entries
 };
 
@@ -61,7 +61,7 @@ struct Shape {
 // sequence of steps; item q of wave w is stored at q NW + w.
 enum PlanKind { kPlanFwdLap = 0, kPlanFwdZero, kPlanBwd, kPlanOut, kPlanSor, kPlanKinds };
 constexpr int kPlanPad = 8;
-constexpr int kPlanWaves[3] = {1, 4, 8};
+constexpr int kPlanWaves[4] = {1, 2, 4, 8};
 // Entry: LDS index (12 bits), source kind (3 bits: own face 0-5 of the
 // ghost, 6 = the block's WO cells), target (bit 15: 0 = window ring slot,
 // 1 = recurrence buffer) and the offset into the source (16 bits).

@@ -127,10 +127,11 @@ void launch_sor(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
                 T *fout, double omega, int hilf_zero, const int *done,
                 hipStream_t s);
 
-// pipeline shape of the plane kernels (1: 4 items of 64 cells in flight,
-// 2: 2 items of 128 cells, 3: 3 items of 128 cells)
-// pipeline shapes of the plane kernels (see set_plan_shape)
-constexpr int kNumShapes = 7;
+// pipeline shapes of the plane kernels (items in flight per wave x cells
+// per item, waves per block): 1: 4x64, 2: 2x128, 3: 3x128, 4: 6x128,
+// 5: 8x64 (one wave); 6: 4x64 (4 waves), 7: 4x64 (8 waves), 8: 3x128
+// (2 waves), 9: 4x64 (2 waves)
+constexpr int kNumShapes = 9;
 void set_plan_shape(int v);
 int plan_shape();
 
@@ -171,6 +172,9 @@ struct MomArgs {
     double ww[5];      // Werner-Wengle constants cwb, cpo4, cpo5, cpo6, cpo8
 };
 void launch_tstle4(const Topo &t, const MomArgs &a, hipStream_t s);
+// tstle4_par at the PAR faces of the npar blocks of plist (after tstle4)
+void launch_tstle4_par(const Topo &t, const MomArgs &a, const int *plist, int npar,
+                       hipStream_t s);
 void launch_rkstep(double *p, double *dp, const double *rhsp, int64_t n,
                    double frhs, double dtfu, hipStream_t s);
 

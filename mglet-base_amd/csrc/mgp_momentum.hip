@@ -248,6 +248,181 @@ void launch_tstle4(const Topo &t, const MomArgs &a, hipStream_t s) {
     }
 }
 
+// ---------------------------------------------------------------------
+// tstle4_par (src/flow/tstle4_mod.F90:532-1753): the convection terms next
+// to the PAR faces of a refined grid, added to uo, vo, wo after tstle4 (the
+// reference calls it per grid after kon, diff and gradp, :114-116; it reads
+// only u, v, w and writes only the grid's own uo, vo, wo, so it can follow
+// the whole tstle4 launch).  The reference writes every face out by hand;
+// all instances share one form, evaluated here for a face with normal axis
+// n, side lo (faces 1, 3, 5) or hi (2, 4, 6) and tangential momentum
+// component c, with o the third axis:
+//  1. upwind at the second layer (:571-776), faces 1..6: along n the
+//     positions iu = 4 (lo) or N-2 (hi), f = area (t(iu-1; a) +
+//     t(iu-1; a+1)) 0.5, q = 0.5 (f -|f|) (0.5 c(iu) - 1.5 c(iu-1) +
+//     c(iu-2)) 0.5 0.5 (lo) or 0.5 (f +|f|) (0.5 c(iu-1) - 1.5 c(iu) +
+//     c(iu+1)) 0.5 0.5 (hi); co(iu) += F (-q), co(iu-1) += F q;
+//  2. the momentum-conserving flux (:778-1750), faces 2, 1, 6, 5, 4, 3:
+//     the standard flux qs = 0.5 f (c(L) + c(G)) minus the coarse-consistent
+//     flux qn, whose f is the mean of the normal transport velocity over the
+//     2 x 4 fine faces of one coarse face (pairs (b, b+1) = (3, 4), (5, 6),
+//     ..; even a; the outermost a-1 or a+2 replaced by doubling the inner
+//     end at a = 2 and Nc-2), odd a taking the mean of its even neighbours;
+//     co(L) += F (qs - qn) (lo) or F (qn - qs) (hi).
+// area = d_c(a) dd_o(b), F = -(rX rY) rZ of the updated cell (rX = rdx for
+// u else rddx, ...), t the normal component itself (no immersed boundary).
+// One workgroup per grid with PAR faces walks the faces in the reference's
+// order, a barrier between faces (edge cells take terms of several faces,
+// in that order); inside a face every position updates its own cells.
+struct ParGrid {
+    int n[3];                          // ii, jj, kk
+    int64_t st[3];                     // strides of i, j, k
+    double *out[3];
+    const double *f[3];
+    const double *d[3], *dd[3], *rd[3], *rdd[3];   // 1-based
+    __device__ __forceinline__ int64_t ix(int a0, int v0, int a1, int v1, int a2, int v2) const {
+        return (int64_t)(v0 - 1) * st[a0] + (int64_t)(v1 - 1) * st[a1] + (int64_t)(v2 - 1) * st[a2];
+    }
+    // F of component comp at the position x[a0] = v0, x[a1] = v1, x[a2] = v2
+    __device__ __forceinline__ double fac(int comp, int a0, int v0, int a1, int v1, int a2,
+                                          int v2) const {
+        int x[3];
+        x[a0] = v0;
+        x[a1] = v1;
+        x[a2] = v2;
+        const double rx = comp == 0 ? rd[0][x[0]] : rdd[0][x[0]];
+        const double ry = comp == 1 ? rd[1][x[1]] : rdd[1][x[1]];
+        const double rz = comp == 2 ? rd[2][x[2]] : rdd[2][x[2]];
+        return -1.0 * rx * ry * rz;
+    }
+};
+
+__device__ void par_upwind(const ParGrid &P, int n, int hi) {
+    const int N = P.n[n], iu = hi ? N - 2 : 4;
+    const int c0 = n == 0 ? 1 : 0, c1 = n == 2 ? 1 : 2;   // the tangential components
+    const int na0 = P.n[c0] - 4, na1 = P.n[c1] - 4;
+    const int nb0 = P.n[3 - n - c0] - 4, nb1 = P.n[3 - n - c1] - 4;
+    const int tot0 = na0 * nb0, tot = tot0 + na1 * nb1;
+    const double *tn = P.f[n];
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const bool second = e >= tot0;
+        const int c = second ? c1 : c0, o = 3 - n - c;
+        const int r = second ? e - tot0 : e, nb = second ? nb1 : nb0;
+        const int a = 3 + r / nb, b = 3 + r % nb;
+        const double *cf = P.f[c];
+        const double area = P.d[c][a] * P.dd[o][b];
+        const double f = area * (tn[P.ix(n, iu - 1, c, a, o, b)] +
+                                 tn[P.ix(n, iu - 1, c, a + 1, o, b)]) * 0.5;
+        double q;
+        if (!hi)
+            q = 0.5 * (f - fabs(f)) *
+                (0.5 * cf[P.ix(n, iu, c, a, o, b)] - 1.5 * cf[P.ix(n, iu - 1, c, a, o, b)] +
+                 cf[P.ix(n, iu - 2, c, a, o, b)]) * 0.5 * 0.5;
+        else
+            q = 0.5 * (f + fabs(f)) *
+                (0.5 * cf[P.ix(n, iu - 1, c, a, o, b)] - 1.5 * cf[P.ix(n, iu, c, a, o, b)] +
+                 cf[P.ix(n, iu + 1, c, a, o, b)]) * 0.5 * 0.5;
+        const double F = P.fac(c, n, iu, c, a, o, b);
+        double *co = P.out[c];
+        const int64_t x0 = P.ix(n, iu, c, a, o, b), x1 = P.ix(n, iu - 1, c, a, o, b);
+        co[x0] = co[x0] + F * (-q);
+        co[x1] = co[x1] + F * q;
+    }
+}
+
+// qn at an even a (the pair of b containing b; 0 where no pair reaches b)
+__device__ __forceinline__ double par_qn_even(const ParGrid &P, int n, int c, int o, int L,
+                                              int G, int T, int a, int b) {
+    const int Nc = P.n[c], No = P.n[o];
+    const int b0 = ((b - 3) & 1) ? b - 1 : b;
+    if (b0 > No - 3) return 0.0;
+    const double *tn = P.f[n], *cf = P.f[c];
+#define TN(x, y) tn[P.ix(n, T, c, x, o, y)]
+    double sum;
+    if (a == 2)
+        sum = 2.0 * TN(a, b0) + 2.0 * TN(a, b0 + 1) + TN(a + 1, b0) + TN(a + 1, b0 + 1) +
+              TN(a + 2, b0) + TN(a + 2, b0 + 1);
+    else if (a == Nc - 2)
+        sum = TN(a - 1, b0) + TN(a - 1, b0 + 1) + TN(a, b0) + TN(a, b0 + 1) +
+              2.0 * TN(a + 1, b0) + 2.0 * TN(a + 1, b0 + 1);
+    else
+        sum = TN(a - 1, b0) + TN(a - 1, b0 + 1) + TN(a, b0) + TN(a, b0 + 1) + TN(a + 1, b0) +
+              TN(a + 1, b0 + 1) + TN(a + 2, b0) + TN(a + 2, b0 + 1);
+#undef TN
+    const double f = P.d[c][a] * P.dd[o][b0] * sum * 0.125;
+    return 0.5 * f * (cf[P.ix(n, L, c, a, o, b)] + cf[P.ix(n, G, c, a, o, b)]);
+}
+
+__device__ void par_conserve(const ParGrid &P, int n, int hi, int c0, int c1) {
+    const int N = P.n[n];
+    const int L = hi ? N - 2 : 3, G = hi ? N - 1 : 2, T = hi ? L : G;
+    const int na0 = P.n[c0] - 3, na1 = P.n[c1] - 3;
+    const int nb0 = P.n[3 - n - c0] - 4, nb1 = P.n[3 - n - c1] - 4;
+    const int tot0 = na0 * nb0, tot = tot0 + na1 * nb1;
+    const double *tn = P.f[n];
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const bool second = e >= tot0;
+        const int c = second ? c1 : c0, o = 3 - n - c;
+        const int r = second ? e - tot0 : e, nb = second ? nb1 : nb0;
+        const int a = 2 + r / nb, b = 3 + r % nb;
+        const double *cf = P.f[c];
+        const double f = P.d[c][a] * P.dd[o][b] *
+                         (tn[P.ix(n, T, c, a, o, b)] + tn[P.ix(n, T, c, a + 1, o, b)]) * 0.5;
+        const double qs = 0.5 * f * (cf[P.ix(n, L, c, a, o, b)] + cf[P.ix(n, G, c, a, o, b)]);
+        const double qn = (a & 1) ? 0.5 * (par_qn_even(P, n, c, o, L, G, T, a - 1, b) +
+                                           par_qn_even(P, n, c, o, L, G, T, a + 1, b))
+                                  : par_qn_even(P, n, c, o, L, G, T, a, b);
+        const double F = P.fac(c, n, L, c, a, o, b);
+        const double dq = hi ? qn - qs : qs - qn;
+        double *co = P.out[c];
+        const int64_t x = P.ix(n, L, c, a, o, b);
+        co[x] = co[x] + F * dq;
+    }
+}
+
+__global__ __launch_bounds__(kMomThreads) void k_tstle4_par(Topo t, MomArgs a,
+                                                            const int *plist) {
+    const Block &B = t.blocks[plist[blockIdx.x]];
+    ParGrid P;
+    P.n[0] = B.ni + 4;
+    P.n[1] = B.nj + 4;
+    P.n[2] = B.nk + 4;
+    P.st[2] = 1;
+    P.st[1] = P.n[2];
+    P.st[0] = (int64_t)P.n[2] * P.n[1];
+    P.out[0] = a.uo + B.sp;
+    P.out[1] = a.vo + B.sp;
+    P.out[2] = a.wo + B.sp;
+    P.f[0] = a.u + B.sp;
+    P.f[1] = a.v + B.sp;
+    P.f[2] = a.w + B.sp;
+    P.d[0] = a.dx + B.o1x - 1;  P.dd[0] = a.ddx + B.o1x - 1;
+    P.d[1] = a.dy + B.o1y - 1;  P.dd[1] = a.ddy + B.o1y - 1;
+    P.d[2] = a.dz + B.o1z - 1;  P.dd[2] = a.ddz + B.o1z - 1;
+    P.rd[0] = a.rdx + B.o1x - 1;  P.rdd[0] = a.rddx + B.o1x - 1;
+    P.rd[1] = a.rdy + B.o1y - 1;  P.rdd[1] = a.rddy + B.o1y - 1;
+    P.rd[2] = a.rdz + B.o1z - 1;  P.rdd[2] = a.rddz + B.o1z - 1;
+    for (int f = 0; f < 6; ++f)
+        if ((B.par >> f) & 1) {
+            par_upwind(P, f >> 1, f & 1);
+            __syncthreads();
+        }
+    // faces 2, 1, 6, 5, 4, 3 with the component pairs (w, v), (u, w), (u, v)
+    const int order[6] = {1, 0, 5, 4, 3, 2};
+    for (int q = 0; q < 6; ++q) {
+        const int f = order[q], n = f >> 1;
+        if (!((B.par >> f) & 1)) continue;
+        par_conserve(P, n, f & 1, n == 0 ? 2 : 0, n == 2 ? 1 : (n == 0 ? 1 : 2));
+        __syncthreads();
+    }
+}
+
+void launch_tstle4_par(const Topo &t, const MomArgs &a, const int *plist, int npar,
+                       hipStream_t s) {
+    MGP_RANGE("mgp:tstle4_par");
+    if (npar > 0) k_tstle4_par<<<npar, kMomThreads, 0, s>>>(t, a, plist);
+}
+
 // src/core/rungekutta_mod.F90:315-333 over every stored position
 __global__ void k_rkstep(double *p, double *dp, const double *rhsp,
                          int64_t n, double frhs, double dtfu) {

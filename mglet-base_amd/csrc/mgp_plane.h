@@ -46,8 +46,8 @@ __device__ __forceinline__ T gld(const T *p) {
 // Index of a per-block wave count in the plan tables (kPlanWaves).
 template <int NW>
 constexpr int plan_nw() {
-    static_assert(NW == 1 || NW == 4 || NW == 8, "no plans for this wave count");
-    return NW == 1 ? 0 : (NW == 4 ? 1 : 2);
+    static_assert(NW == 1 || NW == 2 || NW == 4 || NW == 8, "no plans for this wave count");
+    return NW == 1 ? 0 : (NW == 2 ? 1 : (NW == 4 ? 2 : 3));
 }
 
 // Barrier of the waves of a block over LDS only: orders LDS accesses

@@ -599,6 +599,8 @@ static void fwd_launch(const Topo &t, const Fields<T> &f, Mask m, int rd,
     case 5: k_forward<T, CLASSIC, MASKED, LAP, 8, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
     case 6: k_forward<T, CLASSIC, MASKED, LAP, 4, 1, 4><<<t.nlist, kWave * 4, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
     case 7: k_forward<T, CLASSIC, MASKED, LAP, 4, 1, 8><<<t.nlist, kWave * 8, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    case 8: k_forward<T, CLASSIC, MASKED, LAP, 3, 2, 2><<<t.nlist, kWave * 2, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    case 9: k_forward<T, CLASSIC, MASKED, LAP, 4, 1, 2><<<t.nlist, kWave * 2, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
     default: k_forward<T, CLASSIC, MASKED, LAP, 2, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
     }
 }
@@ -643,6 +645,8 @@ static void bwd_launch(const Topo &t, const Fields<T> &f, const int *done,
     case 5: k_backward<T, CLASSIC, HZ, 8, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
     case 6: k_backward<T, CLASSIC, HZ, 4, 1, 4><<<t.nlist, kWave * 4, lds, s>>>(t.items, t, f, done); break;
     case 7: k_backward<T, CLASSIC, HZ, 4, 1, 8><<<t.nlist, kWave * 8, lds, s>>>(t.items, t, f, done); break;
+    case 8: k_backward<T, CLASSIC, HZ, 3, 2, 2><<<t.nlist, kWave * 2, lds, s>>>(t.items, t, f, done); break;
+    case 9: k_backward<T, CLASSIC, HZ, 4, 1, 2><<<t.nlist, kWave * 2, lds, s>>>(t.items, t, f, done); break;
     default: k_backward<T, CLASSIC, HZ, 2, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
     }
 }
@@ -672,11 +676,13 @@ static void sor_launch(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
                        hipStream_t s) {
     switch (g_shape) {
     case 1: k_sor<T, MASKED, HZ, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
-    case 3: k_sor<T, MASKED, HZ, 3, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
+    case 3:
+    case 8: k_sor<T, MASKED, HZ, 3, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
     case 4: k_sor<T, MASKED, HZ, 6, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
     case 5: k_sor<T, MASKED, HZ, 8, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
     case 6:   // several waves per block: SOR relaxes in place, one wave
-    case 7: k_sor<T, MASKED, HZ, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
+    case 7:
+    case 9: k_sor<T, MASKED, HZ, 4, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
     default: k_sor<T, MASKED, HZ, 2, 2><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, fin, fout, omega, done); break;
     }
 }
@@ -716,6 +722,8 @@ static void out_launch(const Topo &t, const Fields<T> &f, Mask m, Outer o,
     case 5: k_outer<T, MASKED, 8, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
     case 6: k_outer<T, MASKED, 4, 1, 4><<<t.nlist, kWave * 4, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
     case 7: k_outer<T, MASKED, 4, 1, 8><<<t.nlist, kWave * 8, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    case 8: k_outer<T, MASKED, 3, 2, 2><<<t.nlist, kWave * 2, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    case 9: k_outer<T, MASKED, 4, 1, 2><<<t.nlist, kWave * 2, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
     default: k_outer<T, MASKED, 2, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
     }
 }

@@ -51,7 +51,7 @@ MODULE mgp_bindc_mod
     PUBLIC :: mgp_init, mgp_init_mask, mgp_mgpoisl, mgp_mgpoisl_device, &
         mgp_finish, mgp_last_error, mgp_set_iterations, mgp_error_message, &
         mgp_set_sdiv, mgp_connect_device, mgp_tstle4_device, mgp_rkstep_device, &
-        mgp_bound_layout, mgp_set_bound
+        mgp_bound_layout, mgp_set_bound, mgp_flow_bc_device
 
     INTERFACE
         INTEGER(C_INT) FUNCTION mgp_init(grids, cfg, dx, dy, dz, ddx, ddy, &
@@ -136,6 +136,14 @@ MODULE mgp_bindc_mod
             TYPE(C_PTR), VALUE :: handle, fields
             INTEGER(C_INT), VALUE :: level, layers, corners, forward, nfield
         END FUNCTION mgp_connect_device
+
+        ! parent, then bound_flow%bound, of one level (device fields)
+        INTEGER(C_INT) FUNCTION mgp_flow_bc_device(handle, level, u, v, w, p) &
+                BIND(C, name="mgp_flow_bc_device")
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: handle, u, v, w, p
+            INTEGER(C_INT), VALUE :: level
+        END FUNCTION mgp_flow_bc_device
 
         ! tstle4 (noib) on device fields; gradp may be C_NULL_PTR
         INTEGER(C_INT) FUNCTION mgp_tstle4_device(handle, uo, vo, wo, u, v, &

@@ -23,7 +23,7 @@ __all__ = ["PressureSolver", "MgpError", "Grids", "uniform_box"]
 _TYPES = {"sip": 0, "sor": 1, "classic": 2}
 _IB = {"noib": 0, "ghostcell": 1}
 _PREC = {"fp64": 0, "fp32": 1}
-NUM_PLAN_SHAPES = 7       # MGP_NUM_PLAN_SHAPES
+NUM_PLAN_SHAPES = 9       # MGP_NUM_PLAN_SHAPES
 
 
 def _ptr(a, ct=C.c_double):
@@ -180,6 +180,18 @@ class PressureSolver:
         _lib.check(self.lib.mgp_connect_device(self.h, level, layers, int(corners), forward,
                                                arr, len(fields)), "mgp_connect_device")
 
+    def flow_bc(self, u, v, w, p, level):
+        """parent, then bound_flow%bound, of one level on CUDA float64
+        tensors (reference layout): U, V, W at the PAR faces of the level's
+        refined grids from their parents, ghost values at FIX, OP1, NOS,
+        SLI and PAR faces (src/ib/parent/parent_mod.F90:16-52,
+        src/flow/bound_flow_mod.F90)."""
+        _check_device((u, v, w, p), self.grids.n3d)
+        self._bind_stream()
+        _lib.check(self.lib.mgp_flow_bc_device(self.h, level, u.data_ptr(), v.data_ptr(),
+                                               w.data_ptr(), p.data_ptr()),
+                   "mgp_flow_bc_device")
+
     def divcal(self, u, v, w, fak=1.0):
         import torch
         _check_device((u, v, w), self.grids.n3d)
@@ -231,8 +243,9 @@ class PressureSolver:
                                                     for a in arrs]), "mgp_set_bound")
 
     def tstle4(self, u, v, w, p, g, gmol, ilesmodel=0, gradp=None):
-        """Momentum right-hand side of one RK stage (tstle4, no immersed
-        boundary) on CUDA float64 tensors; returns (uo, vo, wo)."""
+        """Momentum right-hand side of one RK stage (tstle4 with tstle4_par
+        at PAR faces, no immersed boundary) on CUDA float64 tensors; returns
+        (uo, vo, wo)."""
         import torch
         _check_device((u, v, w, p, g), self.grids.n3d)
         gp = None

@@ -1,13 +1,14 @@
 """One Runge-Kutta stage of the flow solver around the projection
 (src/flow/timeintegration_mod.F90:22-142), on the library's device kernels:
-tstle4, the 2N-storage update of U, V, W, the velocity exchange the
-divergence needs, then mgpoisl.
+tstle4 (with tstle4_par at the PAR faces of refined grids), the 2N-storage
+update of U, V, W, then per level the velocity exchange the divergence needs,
+parent and bound_flow (src/flow/timeintegration_mod.F90:119-123), then
+mgpoisl.
 
-Covered: one or more grids of one level, no immersed boundary, no LES model
-(the viscosity field G is the caller's), no buoyancy or Coriolis term, no
-time-dependent inflow; `bound_flow` and `parent` of the velocities belong to
-the caller (nothing to do on periodic grids).  The reference exchanges only
-the face-normal component of each velocity before the divergence
+Covered: nested grids, walls and periodic faces, no immersed boundary, no
+LES model (the viscosity field G is the caller's), no buoyancy or Coriolis
+term, no time-dependent inflow.  The reference exchanges only the
+face-normal component of each velocity before the divergence
 (`connect(..., normal=.true., forward=1)`); here all three go through the
 forward exchange, which fills the same ghosts the divergence reads and ghosts
 that mgpoisl's closing two-layer exchange overwrites.
@@ -44,5 +45,7 @@ def rk_stage(solver, u, v, w, p, du, dv, dw, g, dt, irk, gmol, ilesmodel=0,
     solver.rkstep(u, du, uo, frhs, dt * fu)
     solver.rkstep(v, dv, vo, frhs, dt * fu)
     solver.rkstep(w, dw, wo, frhs, dt * fu)
-    solver.connect([u, v, w], layers=1, forward=1)
+    for level in range(solver.grids.minlevel, solver.grids.maxlevel + 1):
+        solver.connect([u, v, w], level=level, layers=1, forward=1)
+        solver.flow_bc(u, v, w, p, level)
     return solver.mgpoisl(u, v, w, p, dtrk * dt, ittot=ittot, irk=irk)

@@ -91,6 +91,95 @@ def test_tstle4_nos_walls(oracle_mod, name):
     assert any((a != f).any() for a, f in zip(ref, rf))
 
 
+def _nested(name):
+    from cases import config3, reference_topology, three_level, two_level
+    if name == "two_level":
+        return two_level(periodic=False)
+    if name == "two_level_periodic":
+        return two_level(periodic=True)
+    if name == "three_level":
+        return three_level()
+    if name == "config3":
+        return config3()
+    return reference_topology(name)
+
+
+@pytest.mark.parametrize("name", ["two_level", "two_level_periodic", "three_level",
+                                  "Parker_micro", "Checkblock", "config3"])
+def test_tstle4_par_bitwise(oracle_mod, name):
+    """tstle4 on nested grids: at the PAR faces of refined grids tstle4_par
+    (src/flow/tstle4_mod.F90:532-1753) adds the second-layer upwind terms and
+    the momentum-conserving coarse-consistent flux.  Equal to the oracle bit
+    for bit away from NOS walls (whose Werner-Wengle stress takes real
+    powers: 1e-13 there); the same grids with the PAR faces taken as FIX
+    faces (the same loop ranges, no tstle4_par) differ, so the terms are
+    there."""
+    import mgp
+    g = _nested(name)
+    assert (g.bc == 8).any()
+    print(f"[tstle4_par] {name}: {g.ngrid} grids, {g.n3d} positions", flush=True)
+    o = oracle_mod.Oracle(g)
+    rng = np.random.default_rng(25)
+    u, v, w, p = (rng.uniform(-1.0, 1.0, g.n3d) for _ in range(4))
+    visc = rng.uniform(1e-3, 2e-2, g.n3d)
+    rho, gmol = 1.2, 1e-3
+    ref = o.tstle4(u, v, w, p, visc, rho, gmol, ilesmodel=1)
+    gw = _nested(name)
+    gw.bc[gw.bc == 5] = 2
+    nowall = oracle_mod.Oracle(gw).tstle4(u, v, w, p, visc, rho, gmol, ilesmodel=1)
+    gp = _nested(name)
+    gp.bc[gp.bc == 8] = 2
+    nopar = oracle_mod.Oracle(gp).tstle4(u, v, w, p, visc, rho, gmol, ilesmodel=1)
+    s = mgp.PressureSolver(g, rho=rho)
+    got = _host(*s.tstle4(*_cuda(u, v, w, p, visc), gmol, ilesmodel=1))
+    s.close()
+    for a, b, nw, nm in zip(ref, got, nowall, ("uo", "vo", "wo")):
+        wall = a != nw
+        assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max(), nm
+        assert np.array_equal(a[~wall], b[~wall]), nm
+    assert all((a != n).any() for a, n in zip(ref, nopar))
+
+
+@pytest.mark.parametrize("name", ["two_level", "three_level"])
+def test_rk_stage_nested_bitwise(oracle_mod, name):
+    """One RK stage on nested grids with FIX walls at a fixed iteration
+    count: tstle4 with tstle4_par, the 2N-storage update, per level the
+    forward exchange, parent and bound_flow (src/flow/timeintegration_mod.F90:
+    96-123), then mgpoisl; the GPU stage (mgp.timeintegration.rk_stage)
+    equals the oracle's bit for bit, ghosts included."""
+    import mgp
+    from cases import velocity_levels
+    from mgp.timeintegration import coeffs
+    g = _nested(name)
+    g.bc[g.bc == 5] = 2
+    o = oracle_mod.Oracle(g)
+    u, v, w = velocity_levels(g, o, 311)
+    p = g.zeros()
+    rng = np.random.default_rng(312)
+    visc = rng.uniform(1e-3, 2e-2, g.n3d)
+    dt, gmol = 1e-3, 1e-3
+    ins = [a.copy() for a in (u, v, w, p)]
+    st = o.setup()
+    frhs, fu, dtrk = coeffs(1)
+    uo, vo, wo = o.tstle4(u, v, w, p, visc, 1.0, gmol)
+    du, dv, dw = g.zeros(), g.zeros(), g.zeros()
+    o.rkstep(u, du, uo, frhs, dt * fu)
+    o.rkstep(v, dv, vo, frhs, dt * fu)
+    o.rkstep(w, dw, wo, frhs, dt * fu)
+    ub, vb, wb = o.bound_buffer(), o.bound_buffer(), o.bound_buffer()
+    for lvl in range(g.minlevel, g.maxlevel + 1):
+        o.exchange([u, v, w], level=lvl, layers=1, forward=1)
+        o.parent_uvw(lvl, u, v, w, ub, vb, wb)
+        o.bound_flow(lvl, u, v, w, p, ub, vb, wb)
+    so = o.mgpoisl(st, u, v, w, p, dtrk * dt, ninner=3, nouter=4)
+    s = mgp.PressureSolver(g, ninner=3, nouter=4)
+    sg, got = _gpu_stage(s, g, *ins, visc, dt, gmol)
+    s.close()
+    assert sg["ipc"] == so["ipc"] and sg["maxrhs"] == so["maxrhs"]
+    for a, b, nm in zip((u, v, w, p), got, "uvwp"):
+        assert np.array_equal(a, b), (nm, float(np.abs(a - b).max()))
+
+
 def _taylor_green(g, o, seed=1, noise=1e-3):
     u, v, w = g.zeros(), g.zeros(), g.zeros()
     b = 1

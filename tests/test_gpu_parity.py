@@ -196,11 +196,11 @@ def test_single_hip_runtime():
     s.close()
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 6, 7, 8, 9])
 @pytest.mark.parametrize("case", ["sip-walls", "classic-masked", "sor-odd", "sor-masked"])
 def test_plan_shapes_bitwise(oracle_mod, shape, case):
     """Every pipeline shape of the plane kernels (items of 64 or 128 cells,
-    2 to 8 in flight, 1, 4 or 8 waves per block) gives the oracle's FP64 result bit for bit."""
+    2 to 8 in flight, 1, 2, 4 or 8 waves per block) gives the oracle's FP64 result bit for bit."""
     kind, name = case.split("-")
     ityp = {"sip": 0, "classic": 2, "sor": 1}[kind]
     topo = {"walls": "walls", "masked": "box2", "odd": "odd"}[name]
