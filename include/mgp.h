@@ -244,6 +244,16 @@ int mgp_divcal_device_noalloc(mgp_handle *h, mgletreal *div, const mgletreal *u,
 int mgp_flow_bc_device(mgp_handle *h, mgletint level, mgletreal *u, mgletreal *v,
                        mgletreal *w, mgletreal *p);
 
+/* par_ftoc_norm(ilevel, u, v, w, sum) (src/ib/par_ftoc_mod.F90:35-475) on
+ * device fields: the face-normal velocity at the PAR faces of the level's
+ * grids, per 2x2 group of fine faces the area-weighted mean (sum = 0) or
+ * the plain sum (sum != 0), into the coincident coarse face positions of
+ * the parent, and of the parent's neighbour where the face lies on the
+ * parent's boundary.  Called from the ghostcell flux setup
+ * (src/flow/gc_flowstencils_mod.F90:1420) after ftoc, fine to coarse. */
+int mgp_par_ftoc_device(mgp_handle *h, mgletint level, mgletreal *u, mgletreal *v,
+                        mgletreal *w, mgletint sum);
+
 /* The RK stage around the projection (device fields, reference layout).
  *
  * tstle4 (src/flow/tstle4_mod.F90:13-120, no immersed boundary): uo, vo, wo

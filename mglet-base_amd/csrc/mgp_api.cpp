@@ -200,6 +200,10 @@ struct mgp_handle {
     std::set<int> level_par;   // levels with PAR faces
     DBuf d_parlist;            // blocks with PAR faces (tstle4_par)
     int npar = 0;
+    // par_ftoc_norm transfers, ordered by the fine grid's level, and the
+    // range of each level in that table
+    DBuf d_pftask;
+    std::map<int, std::pair<int, int>> pf_range;
     // closing sync of mgpoisl: velocity boundary buffers of the FIX, OP1
     // and PAR faces (offsets per grid and face, -1 none), OP1 pressure per
     // grid and face, and the levels with a face bound_flow acts on
@@ -390,6 +394,56 @@ void get_nbrs(const mgp_handle &h, int g, int iface, int &ng, int &nf) {
             break;
         }
     }
+}
+
+// Transfers of par_ftoc_norm (the connection table of init_par_ftoc,
+// src/ib/par_ftoc_mod.F90:513-745): every PAR face of a fine grid sends to
+// its parent, at the parent's own boundary face when the fine face lies on
+// it (the fine grid starts at the parent's first cell, pos == 3, for a low
+// face; ends at its last, for a high face: pos > 3, or pos == 3 with the
+// fine grid spanning the whole parent) and otherwise inside (the coarse
+// face before the first / after the last covered cell); a face on the
+// parent's boundary also goes to the parent's neighbour across it (NBRGRID
+// of the parent), at that grid's opposite boundary face.
+template <typename T>
+void upload(DBuf &d, const std::vector<T> &v);
+
+void build_par_ftoc(mgp_handle &h) {
+    std::vector<ParFtocTask> tasks;
+    for (int L = h.minlevel; L <= h.maxlevel; ++L) {
+        const int first = (int)tasks.size();
+        for (int g = 0; g < h.ngrid; ++g) {
+            const Block &B = h.blocks[g];
+            if (B.level != L || !B.par) continue;
+            const int c = B.parent;
+            const int nf[3] = {h.ii[g], h.jj[g], h.kk[g]};
+            const int nc[3] = {h.ii[c], h.jj[c], h.kk[c]};
+            for (int f = 0; f < 6; ++f) {
+                if (!((B.par >> f) & 1)) continue;
+                const int ax = f >> 1, pos = B.pos[ax];
+                const bool low = (f & 1) == 0, one_connect = !(nf[ax] > nc[ax]);
+                bool inside = false;
+                if (pos == 3 && one_connect) inside = !low;
+                else if (pos > 3) inside = low;
+                ParFtocTask t;
+                t.fine = g;
+                t.face = f;
+                t.dest = c;
+                if (inside) t.dnorm = low ? pos - 1 : pos + (nf[ax] - 4) / 2 - 1;
+                else t.dnorm = low ? 2 : nc[ax] - 2;
+                tasks.push_back(t);
+                if (inside) continue;
+                const int nb = h.nbr[26 * c + f];   // 1-based, 0 none
+                if (nb <= 0) continue;
+                const int nn[3] = {h.ii[nb - 1], h.jj[nb - 1], h.kk[nb - 1]};
+                t.dest = nb - 1;
+                t.dnorm = low ? nn[ax] - 2 : 2;
+                tasks.push_back(t);
+            }
+        }
+        if ((int)tasks.size() > first) h.pf_range[L] = {first, (int)tasks.size() - first};
+    }
+    if (!tasks.empty()) upload(h.d_pftask, tasks);
 }
 
 void build_conns(mgp_handle &h) {
@@ -1351,6 +1405,7 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
             }
         h->npar = (int)parlist.size();
         if (h->npar) upload(h->d_parlist, parlist);
+        build_par_ftoc(*h);
         if (!h->level_par.empty()) {
             h->d_hpar.alloc(h->nface * tsz);
             h->d_pbuf.alloc(h->nface * tsz);
@@ -2060,6 +2115,28 @@ int mgp_connect_device(mgp_handle *h, mgletint level, mgletint layers,
     if (!fields || nfield < 1) return fail(MGP_ERR_ARG, "no fields");
     try {
         exchange(*h, level, layers, corners, forward, fields, nfield);
+        HC(hipGetLastError());
+        HC(hipStreamSynchronize(h->stream));
+    } catch (const HipError &e) {
+        return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
+    }
+    return MGP_OK;
+}
+
+int mgp_par_ftoc_device(mgp_handle *h, mgletint level, mgletreal *u, mgletreal *v,
+                        mgletreal *w, mgletint sum) {
+    MGP_RANGE("mgp_par_ftoc_device");
+    if (!h) return fail(MGP_ERR_STATE, "handle not initialised");
+    if (!u || !v || !w) return fail(MGP_ERR_ARG, "null field");
+    if (level < h->minlevel || level > h->maxlevel) return fail(MGP_ERR_ARG, "level out of range");
+    try {
+        const auto it = h->pf_range.find(level);
+        if (it != h->pf_range.end()) {
+            const double *dd[3] = {h->d_ddx.as<double>(), h->d_ddy.as<double>(),
+                                   h->d_ddz.as<double>()};
+            launch_par_ftoc(h->d_blocks.as<Block>(), h->d_pftask.as<ParFtocTask>() + it->second.first,
+                            it->second.second, u, v, w, dd, sum != 0, h->stream);
+        }
         HC(hipGetLastError());
         HC(hipStreamSynchronize(h->stream));
     } catch (const HipError &e) {

@@ -225,6 +225,16 @@ void launch_bound_flow(const Topo &t, double *u, double *v, double *w, double *p
                        const double *ddx, const double *ddy, const double *ddz, double rho,
                        hipStream_t s);
 // gc: ghostcell restriction of P (bp may be null: all fluid)
+// One transfer of par_ftoc_norm: the face-normal velocity on PAR face
+// `face` (0..5) of block `fine` into block `dest` (its parent, or the
+// parent's neighbour across that face) at the 1-based normal index dnorm.
+struct ParFtocTask {
+    int fine, face, dest, dnorm;
+};
+void launch_par_ftoc(const Block *blocks, const ParFtocTask *tasks, int ntask, double *u,
+                     double *v, double *w, const double *const dd[3], int sum,
+                     hipStream_t s);
+
 void launch_ftoc_uvwp(const Topo &t, double *u, double *v, double *w,
                       double *p, int gc, const double *bp,
                       const double *const dd[3], hipStream_t s);

@@ -779,6 +779,60 @@ void launch_ftoc_uvwp(const Topo &t, double *u, double *v, double *w,
     k_ftoc_uvwp<<<grid, kThreadsLv, 0, s>>>(t, u, v, w, p, dd[0], dd[1], dd[2], gc, bp);
 }
 
+// par_ftoc_norm (src/ib/par_ftoc_mod.F90:35-475): one workgroup per task
+// (a PAR face of a fine grid and one coarse grid receiving it).  Each
+// coarse face position (a, b) takes its 2 x 2 group of the fine face plane:
+// the plain sum, or the area-weighted mean (pack_single :234-337, terms in
+// its order: along t1 then t2 on x and y faces, t2 then t1 on z faces).
+__global__ void k_par_ftoc(const Block *blocks, const ParFtocTask *tasks, double *u,
+                           double *v, double *w, const double *ddx_, const double *ddy_,
+                           const double *ddz_, int sum) {
+    const ParFtocTask T = tasks[blockIdx.x];
+    const Block &F = blocks[T.fine], &C = blocks[T.dest];
+    const int n = T.face >> 1;
+    const int t1 = n == 0 ? 1 : 0, t2 = n == 2 ? 1 : 2;
+    const int nf[3] = {F.ni + 4, F.nj + 4, F.nk + 4};
+    const int nc[3] = {C.ni + 4, C.nj + 4, C.nk + 4};
+    const int64_t sf[3] = {(int64_t)nf[2] * nf[1], nf[2], 1};
+    const int64_t sc[3] = {(int64_t)nc[2] * nc[1], nc[2], 1};
+    const double *dd[3] = {ddx_ + F.o1x - 1, ddy_ + F.o1y - 1, ddz_ + F.o1z - 1};
+    const double *w1 = dd[t1], *w2 = dd[t2];
+    const double *ff = (n == 0 ? u : n == 1 ? v : w) + F.sp;
+    double *fc = (n == 0 ? u : n == 1 ? v : w) + C.sp;
+    const int fn = (T.face & 1) ? nf[n] - 2 : 2;   // fine face plane
+    const int na = (nf[t1] - 4) / 2, nb = (nf[t2] - 4) / 2;
+    for (int e = threadIdx.x; e < na * nb; e += blockDim.x) {
+        const int a = e / nb, b = e % nb;
+        const int p1 = 3 + 2 * a, p2 = 3 + 2 * b;
+        const int64_t x0 = (int64_t)(fn - 1) * sf[n] + (int64_t)(p1 - 1) * sf[t1] +
+                           (int64_t)(p2 - 1) * sf[t2];
+        const double f00 = ff[x0], f10 = ff[x0 + sf[t1]], f01 = ff[x0 + sf[t2]],
+                     f11 = ff[x0 + sf[t1] + sf[t2]];
+        double val;
+        if (sum) {
+            val = n == 2 ? f00 + f01 + f10 + f11 : f00 + f10 + f01 + f11;
+        } else {
+            const double a00 = f00 * w1[p1] * w2[p2], a10 = f10 * w1[p1 + 1] * w2[p2],
+                         a01 = f01 * w1[p1] * w2[p2 + 1], a11 = f11 * w1[p1 + 1] * w2[p2 + 1];
+            const double ua = n == 2 ? a00 + a01 + a10 + a11 : a00 + a10 + a01 + a11;
+            val = ua / ((w1[p1] + w1[p1 + 1]) * (w2[p2] + w2[p2 + 1]));
+        }
+        const int64_t xc = (int64_t)(T.dnorm - 1) * sc[n] +
+                           (int64_t)(F.pos[t1] + a - 1) * sc[t1] +
+                           (int64_t)(F.pos[t2] + b - 1) * sc[t2];
+        fc[xc] = val;
+    }
+}
+
+void launch_par_ftoc(const Block *blocks, const ParFtocTask *tasks, int ntask, double *u,
+                     double *v, double *w, const double *const dd[3], int sum,
+                     hipStream_t s) {
+    MGP_RANGE("mgp:par_ftoc");
+    if (ntask > 0)
+        k_par_ftoc<<<ntask, kThreadsLv, 0, s>>>(blocks, tasks, u, v, w, dd[0], dd[1], dd[2],
+                                                sum);
+}
+
 template void launch_ctof<double>(const Topo &, const Fields<double> &, double *, double *,
                                   double *, const int *, hipStream_t);
 template void launch_ctof<float>(const Topo &, const Fields<float> &, float *, float *,

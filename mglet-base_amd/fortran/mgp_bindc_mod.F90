@@ -51,7 +51,7 @@ MODULE mgp_bindc_mod
     PUBLIC :: mgp_init, mgp_init_mask, mgp_mgpoisl, mgp_mgpoisl_device, &
         mgp_finish, mgp_last_error, mgp_set_iterations, mgp_error_message, &
         mgp_set_sdiv, mgp_connect_device, mgp_tstle4_device, mgp_rkstep_device, &
-        mgp_bound_layout, mgp_set_bound, mgp_flow_bc_device
+        mgp_bound_layout, mgp_set_bound, mgp_flow_bc_device, mgp_par_ftoc_device
 
     INTERFACE
         INTEGER(C_INT) FUNCTION mgp_init(grids, cfg, dx, dy, dz, ddx, ddy, &
@@ -144,6 +144,14 @@ MODULE mgp_bindc_mod
             TYPE(C_PTR), VALUE :: handle, u, v, w, p
             INTEGER(C_INT), VALUE :: level
         END FUNCTION mgp_flow_bc_device
+
+        ! par_ftoc_norm of one level (device fields); sum /= 0: plain sums
+        INTEGER(C_INT) FUNCTION mgp_par_ftoc_device(handle, level, u, v, w, &
+                sum) BIND(C, name="mgp_par_ftoc_device")
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: handle, u, v, w
+            INTEGER(C_INT), VALUE :: level, sum
+        END FUNCTION mgp_par_ftoc_device
 
         ! tstle4 (noib) on device fields; gradp may be C_NULL_PTR
         INTEGER(C_INT) FUNCTION mgp_tstle4_device(handle, uo, vo, wo, u, v, &

@@ -192,6 +192,17 @@ class PressureSolver:
                                                w.data_ptr(), p.data_ptr()),
                    "mgp_flow_bc_device")
 
+    def par_ftoc_norm(self, u, v, w, level, sum=False):
+        """par_ftoc_norm of one level on CUDA float64 tensors (reference
+        layout): the face-normal velocity at the PAR faces of the level's
+        grids into the coincident coarse faces (src/ib/par_ftoc_mod.F90:
+        35-475); area-weighted mean of each 2x2 group, or the sum."""
+        _check_device((u, v, w), self.grids.n3d)
+        self._bind_stream()
+        _lib.check(self.lib.mgp_par_ftoc_device(self.h, level, u.data_ptr(), v.data_ptr(),
+                                                w.data_ptr(), int(bool(sum))),
+                   "mgp_par_ftoc_device")
+
     def divcal(self, u, v, w, fak=1.0):
         import torch
         _check_device((u, v, w), self.grids.n3d)

@@ -96,6 +96,7 @@ def load():
         "mgp_tstle4_device": [vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_int32, _dp],
         "mgp_rkstep_device": [vp, vp, vp, vp, C.c_double, C.c_double],
         "mgp_flow_bc_device": [vp, C.c_int32, vp, vp, vp, vp],
+        "mgp_par_ftoc_device": [vp, C.c_int32, vp, vp, vp, C.c_int32],
         "mgp_info": [vp, _i64p, _i64p],
         "mgp_factor_classes": [vp, _i32p, _i64p],
         "mgp_get_history": [vp, _dp, C.c_int32, _i32p],

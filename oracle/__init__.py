@@ -339,6 +339,11 @@ class Oracle:
                               _p(g.ddx), _p(g.ddy), _p(g.ddz),
                               _p(bp) if bp is not None else None)
 
+    def par_ftoc_norm(self, level, u, v, w, sum=False):
+        g = self.g
+        self.lib.og_par_ftoc_norm(C.byref(self.topo), level, _p(u), _p(v), _p(w),
+                                  _p(g.ddx), _p(g.ddy), _p(g.ddz), int(bool(sum)))
+
     def time_sip(self, st, rhs, niter, nthreads=0):
         """Seconds for `niter` hyperplane-SIP inner iterations (CPU baseline)."""
         g = self.g

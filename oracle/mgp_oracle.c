@@ -1446,6 +1446,161 @@ void og_parent_uvw(const og_topo *t, int level, const double *u_, const double *
     }
 }
 
+/* par_ftoc_norm(ilevel, u, v, w, sum) (src/ib/par_ftoc_mod.F90:35-60): for
+ * every PAR face of the grids of `level`, the face-normal velocity on the
+ * fine face plane (start_and_stop :764-810: position 2 or n-2 along the
+ * normal, 3..n-2 across) packed per 2x2 group (pack_single :234-337: the
+ * plain sum, or the ddx/ddy/ddz-area-weighted mean) and unpacked into the
+ * coarse face positions (unpack_single :383-475) of the parent, at the
+ * parent's own boundary face when the fine face lies on it, else inside
+ * the parent (ipos-1 for a low face, the last covered index for a high
+ * face); a fine face on the parent's boundary is also sent to the parent's
+ * neighbour across it, at that grid's opposite boundary face (the
+ * connection table of init_par_ftoc :513-745: pos == 3 and one_connect,
+ * pos > 3).  Each coarse position is written by one message, so the
+ * reference's arrival order does not matter; here grids and faces go in
+ * order. */
+static void par_ftoc_pack(const og_topo *t, int g, int f, const double *u_, const double *v_,
+                          const double *w_, const double *ddx_, const double *ddy_,
+                          const double *ddz_, int sum, double *buf)
+{
+    GDIMS(t, g);
+    const double *ff = (f <= 2 ? u_ : f <= 4 ? v_ : w_) + t->ip3[g - 1];
+    const double *ddx = ddx_ + t->ip1x[g - 1] - 1, *ddy = ddy_ + t->ip1y[g - 1] - 1,
+                 *ddz = ddz_ + t->ip1z[g - 1] - 1;
+    int ks = 3, ke = kk - 2, js = 3, je = jj - 2, is = 3, ie = ii - 2;
+    switch (f) {
+    case 1: is = ie = 2; break;
+    case 2: is = ie = ii - 2; break;
+    case 3: js = je = 2; break;
+    case 4: js = je = jj - 2; break;
+    case 5: ks = ke = 2; break;
+    default: ks = ke = kk - 2; break;
+    }
+    int64_t n = 0;
+    if (f <= 2) {
+        const int i = is;
+        for (int j = js; j <= je; j += 2)
+            for (int k = ks; k <= ke; k += 2) {
+                if (sum)
+                    buf[n++] = ff[IX(k, j, i)] + ff[IX(k, j + 1, i)] + ff[IX(k + 1, j, i)] +
+                               ff[IX(k + 1, j + 1, i)];
+                else {
+                    const double ua = ff[IX(k, j, i)] * ddy[j] * ddz[k] +
+                                      ff[IX(k, j + 1, i)] * ddy[j + 1] * ddz[k] +
+                                      ff[IX(k + 1, j, i)] * ddy[j] * ddz[k + 1] +
+                                      ff[IX(k + 1, j + 1, i)] * ddy[j + 1] * ddz[k + 1];
+                    const double a = (ddy[j] + ddy[j + 1]) * (ddz[k] + ddz[k + 1]);
+                    buf[n++] = ua / a;
+                }
+            }
+    } else if (f <= 4) {
+        const int j = js;
+        for (int i = is; i <= ie; i += 2)
+            for (int k = ks; k <= ke; k += 2) {
+                if (sum)
+                    buf[n++] = ff[IX(k, j, i)] + ff[IX(k, j, i + 1)] + ff[IX(k + 1, j, i)] +
+                               ff[IX(k + 1, j, i + 1)];
+                else {
+                    const double ua = ff[IX(k, j, i)] * ddx[i] * ddz[k] +
+                                      ff[IX(k, j, i + 1)] * ddx[i + 1] * ddz[k] +
+                                      ff[IX(k + 1, j, i)] * ddx[i] * ddz[k + 1] +
+                                      ff[IX(k + 1, j, i + 1)] * ddx[i + 1] * ddz[k + 1];
+                    const double a = (ddx[i] + ddx[i + 1]) * (ddz[k] + ddz[k + 1]);
+                    buf[n++] = ua / a;
+                }
+            }
+    } else {
+        const int k = ks;
+        for (int i = is; i <= ie; i += 2)
+            for (int j = js; j <= je; j += 2) {
+                if (sum)
+                    buf[n++] = ff[IX(k, j, i)] + ff[IX(k, j + 1, i)] + ff[IX(k, j, i + 1)] +
+                               ff[IX(k, j + 1, i + 1)];
+                else {
+                    const double ua = ff[IX(k, j, i)] * ddx[i] * ddy[j] +
+                                      ff[IX(k, j + 1, i)] * ddx[i] * ddy[j + 1] +
+                                      ff[IX(k, j, i + 1)] * ddx[i + 1] * ddy[j] +
+                                      ff[IX(k, j + 1, i + 1)] * ddx[i + 1] * ddy[j + 1];
+                    const double a = (ddx[i] + ddx[i + 1]) * (ddy[j] + ddy[j + 1]);
+                    buf[n++] = ua / a;
+                }
+            }
+    }
+}
+
+static void par_ftoc_unpack(const og_topo *t, int gf, int gc, int f, int frecv, double *u_,
+                            double *v_, double *w_, const double *buf)
+{
+    const int iif = t->ii[gf - 1], jjf = t->jj[gf - 1], kkf = t->kk[gf - 1];
+    const int ipos = POS(gf, 0), jpos = POS(gf, 1), kpos = POS(gf, 2);
+    int is = ipos, ie = ipos + (iif - 4) / 2 - 1;
+    int js = jpos, je = jpos + (jjf - 4) / 2 - 1;
+    int ks = kpos, ke = kpos + (kkf - 4) / 2 - 1;
+    if (frecv < 0) {
+        switch (f) {
+        case 1: is = ie = ipos - 1; break;
+        case 2: is = ie; break;
+        case 3: js = je = jpos - 1; break;
+        case 4: js = je; break;
+        case 5: ks = ke = kpos - 1; break;
+        default: ks = ke; break;
+        }
+    } else {
+        const int iic = t->ii[gc - 1], jjc = t->jj[gc - 1], kkc = t->kk[gc - 1];
+        switch (frecv) {
+        case 1: is = ie = 2; break;
+        case 2: is = ie = iic - 2; break;
+        case 3: js = je = 2; break;
+        case 4: js = je = jjc - 2; break;
+        case 5: ks = ke = 2; break;
+        default: ks = ke = kkc - 2; break;
+        }
+    }
+    double *fc = (f <= 2 ? u_ : f <= 4 ? v_ : w_) + t->ip3[gc - 1];
+    const int64_t kk = t->kk[gc - 1], jj = t->jj[gc - 1];
+    int64_t n = 0;
+    for (int i = is; i <= ie; ++i)
+        for (int j = js; j <= je; ++j)
+            for (int k = ks; k <= ke; ++k) fc[IX(k, j, i)] = buf[n++];
+}
+
+void og_par_ftoc_norm(const og_topo *t, int level, double *u, double *v, double *w,
+                      const double *ddx, const double *ddy, const double *ddz, int sum)
+{
+    for (int g = 1; g <= t->ngrid; ++g) {
+        if (t->level[g - 1] != level) continue;
+        for (int f = 1; f <= 6; ++f) {
+            if (t->bc[6 * (g - 1) + f - 1] != PAR_BC) continue;
+            const int c = t->iparent[g - 1];
+            const int ax = (f - 1) / 2, odd = (f & 1) != 0;
+            const int pos = POS(g, ax);
+            const int nf = ax == 0 ? t->ii[g - 1] : ax == 1 ? t->jj[g - 1] : t->kk[g - 1];
+            const int nc = ax == 0 ? t->ii[c - 1] : ax == 1 ? t->jj[c - 1] : t->kk[c - 1];
+            const int one_connect = !(nf > nc);
+            int frecv = f;
+            if (pos == 3 && one_connect) {
+                if (!odd) frecv = -1;
+            } else if (pos > 3) {
+                if (odd) frecv = -1;
+            }
+            const int64_t area = (int64_t)((t->ii[g - 1] - 4) / 2 + 2) *
+                                 ((t->jj[g - 1] - 4) / 2 + 2) * ((t->kk[g - 1] - 4) / 2 + 2);
+            double *buf = malloc(sizeof(double) * (size_t)area);
+            par_ftoc_pack(t, g, f, u, v, w, ddx, ddy, ddz, sum, buf);
+            par_ftoc_unpack(t, g, c, f, frecv, u, v, w, buf);
+            if (frecv > 0) {
+                /* the parent's neighbour across face f, receiving at its
+                 * opposite face */
+                const int nb = t->nbr[26 * (c - 1) + f - 1];
+                if (nb > 0) par_ftoc_unpack(t, g, nb, f, odd ? f + 1 : f - 1, u, v, w, buf);
+            }
+            free(buf);
+        }
+    }
+}
+
+
 #define BPV(k, j, i) (bp ? bp[IX(k, j, i)] : 1.0)
 
 /* bfront (src/flow/bound_flow_mod.F90:23-212): faces 1 and 2 */

@@ -191,6 +191,13 @@ int64_t og_bound_buffers(const og_topo *t, int64_t *bbo);
 void og_parent_uvw(const og_topo *t, int level, const double *u, const double *v,
                    const double *w, double *ubuf, double *vbuf, double *wbuf,
                    const int64_t *bbo);
+/* par_ftoc_norm(ilevel, u, v, w, sum): src/ib/par_ftoc_mod.F90:35-475, the
+ * face-normal velocity at the PAR faces of the grids of `level` into the
+ * coarse face positions of the parent (and of the parent's neighbour where
+ * the face lies on the parent's boundary): area-weighted mean (sum = 0) or
+ * plain sum of each 2x2 group */
+void og_par_ftoc_norm(const og_topo *t, int level, double *u, double *v, double *w,
+                      const double *ddx, const double *ddy, const double *ddz, int sum);
 /* bound_flow%bound(ilevel, u, v, w, p): src/flow/bound_flow_mod.F90 (FIX,
  * OP1, NOS, SLI, PAR) over src/core/bound_mod.F90:36-76; level < 0: all;
  * bp NULL = 1 */
