@@ -168,7 +168,7 @@ struct mgp_handle {
 
     // device topology
     DBuf d_blocks, d_shapes, d_cells, d_pstart, d_slot, d_blist_all;
-    DBuf d_items, d_ents, d_zero, d_cellx;   // plans of the plane kernels
+    DBuf d_items, d_ents, d_zero, d_cellx, d_fents;   // plans of the plane kernels
     std::map<int, DBuf> d_blist_lvl;
     // 1-D fields (FP64 and working precision)
     DBuf d_dx, d_dy, d_dz, d_rdx, d_rdy, d_rdz, d_rddx, d_rddy, d_rddz;
@@ -262,6 +262,7 @@ struct mgp_handle {
         t.items = d_items.as<uint2>();
         t.ents = d_ents.as<uint32_t>();
         t.cellx = d_cellx.as<uint2>();
+        t.fents = d_fents.as<uint32_t>();
         t.zero = d_zero.p;
         if (lvl < 0) {
             t.blist = d_blist_all.as<int>();
@@ -693,7 +694,8 @@ void exchange(mgp_handle &h, int level, int layers, int corners, int forward,
 // buffer p mod 2.  Steps before the first plane only fill LDS.
 void build_plans(mgp_handle &h, const std::vector<uint32_t> &cells,
                  const std::vector<int> &pstart, std::vector<uint2> &items,
-                 std::vector<uint32_t> &ents, std::vector<uint2> &cellx) {
+                 std::vector<uint32_t> &ents, std::vector<uint2> &cellx,
+                 std::vector<uint32_t> &fents) {
     cellx.resize(cells.size());
     for (Shape &S : h.shapes) {
         const int NI = S.ni, NJ = S.nj, NK = S.nk, NP = S.np;
@@ -867,6 +869,46 @@ void build_plans(mgp_handle &h, const std::vector<uint32_t> &cells,
                                 throw HipError{"plan steps differ between waves"};
                 }
             }
+        }
+    }
+    // face-store entries, 64 per item (parallel to the item table): the
+    // boundary cells of the item's compute chunk, one entry per (cell, own
+    // face) -- the forward sweeps' RES from the recurrence plane (lower
+    // faces, and upper faces flagged), the backward sweep's HILF from the
+    // item's staging row, SOR's new value from its window slot
+    fents.assign(items.size() * kFaceSlots, kFaceNone);
+    for (const Shape &S : h.shapes) {
+        const int NI = S.ni, NJ = S.nj, NK = S.nk;
+        const uint32_t *cl = cells.data() + S.cell_off;
+        for (int kind = 0; kind < kPlanKinds; ++kind) {
+            if (kind == kPlanOut) continue;
+            for (int nwi = 0; nwi < 4; ++nwi)
+                for (int wi = 0; wi < 2; ++wi) {
+                    const size_t base = S.plan_item[nwi][wi][kind];
+                    const size_t n = (size_t)S.plan_nitem[nwi][wi][kind] * kPlanWaves[nwi];
+                    for (size_t q = base; q < base + n; ++q) {
+                        const int cpos = items[q].x & 0xffff, cc = (items[q].x >> 16) & 0xff;
+                        int e = 0;
+                        for (int l = 0; l < cc; ++l) {
+                            const uint32_t w = cl[cpos + l];
+                            const int i = w & 1023, j = (w >> 10) & 1023, k = w >> 20;
+                            const bool on[6] = {i == 0, i == NI - 1, j == 0, j == NJ - 1,
+                                                k == 0, k == NK - 1};
+                            const int pos[3] = {fidx_x(k, j, NJ, NK), fidx_y(k, i, NI, NK),
+                                                fidx_z(j, i, NI, NJ)};
+                            const int src = kind == kPlanBwd ? l
+                                          : kind == kPlanSor ? (j + 1) + (NJ + 2) * (i + 1)
+                                                             : (j + 1) + (NJ + 1) * (i + 1);
+                            for (int f = 0; f < 6; ++f) {
+                                if (!on[f]) continue;
+                                if (e == kFaceSlots)
+                                    throw HipError{"more face cells in a plan item than lanes"};
+                                fents[q * kFaceSlots + e++] =
+                                    face_entry(src, face_base(f, NI, NJ, NK) + pos[f >> 1], f & 1);
+                            }
+                        }
+                    }
+                }
         }
     }
 }
@@ -1182,7 +1224,8 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
         std::vector<uint2> items;
         std::vector<uint32_t> ents;
         std::vector<uint2> cellx;
-        build_plans(*h, cells, pstart, items, ents, cellx);
+        std::vector<uint32_t> fents;
+        build_plans(*h, cells, pstart, items, ents, cellx, fents);
         build_conns(*h);
 
         // BP: per WO cell, own ghost layer per face, reference layout
@@ -1250,6 +1293,7 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
         upload(h->d_items, items);
         upload(h->d_ents, ents);
         upload(h->d_cellx, cellx);
+        upload(h->d_fents, fents);
         h->d_zero.alloc(kZeroBytes);
         HC(hipMemsetAsync(h->d_zero.p, 0, kZeroBytes, h->stream));
         std::vector<int> all(n);
@@ -2444,7 +2488,7 @@ int mgp_info(mgp_handle *h, int64_t *ncells, int64_t *device_bytes) {
     if (device_bytes) {
         size_t tot = 0;
         const DBuf *bs[] = {&h->d_blocks, &h->d_shapes, &h->d_cells, &h->d_pstart, &h->d_slot,
-                            &h->d_items, &h->d_ents, &h->d_zero, &h->d_cellx,
+                            &h->d_items, &h->d_ents, &h->d_zero, &h->d_cellx, &h->d_fents,
                             &h->d_ap64, &h->d_apT, &h->d_hilf, &h->d_rhs64, &h->d_rhsT, &h->d_res,
                             &h->d_dp64, &h->d_bpw, &h->d_bpf, &h->d_bpsp, &h->d_sdiv, &h->d_hface, &h->d_rhi[0],
                             &h->d_rhi[1], &h->d_rlo, &h->d_rapT, &h->d_hface2, &h->d_ft[0], &h->d_ft[1], &h->d_ft[2],

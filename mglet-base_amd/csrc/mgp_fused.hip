@@ -28,6 +28,7 @@ namespace mgp {
 
 template <typename T, bool CLASSIC, bool MASKED>
 struct OutFwdK {
+    static constexpr bool kFaces = true;
     struct Data {
         uint2 c;
         double ap, rhs, dp;
@@ -47,7 +48,6 @@ struct OutFwdK {
     T *R, *H;
     uint8_t *BH;
     int HW, RW, RS, mHS, nr;
-    int fb0, fb1, fb2, fb3, fb4, fb5;
     bool masked;
     double mx;
 
@@ -61,7 +61,7 @@ struct OutFwdK {
         d.ls = ld(ls, s);
         d.lw = ld(lw, s);
     }
-    __device__ __forceinline__ void compute(const Data &d, int s, int p) {
+    __device__ __forceinline__ void compute(const Data &d, int s, int p, int) {
         int i, j, k;
         cell_ijk(d.c.x, i, j, k);
         // outer update of iteration n (as k_outer)
@@ -101,18 +101,14 @@ struct OutFwdK {
         }
         Rc[c1] = r;
         st(res, s, r);
-        const uint32_t fl = d.c.y >> 24;
-        if (fl) {
-            const int fx = d.c.y & 4095, fy = (d.c.y >> 12) & 4095, fz = d.c.x >> 18;
-            if (fl & 1) st(rlo, fb0 + fx, r);
-            if (fl & 4) st(rlo, fb2 + fy, r);
-            if (fl & 16) st(rlo, fb4 + fz, r);
-            if (rw) {
-                if (fl & 2) st(rw, fb1 + fx, r);
-                if (fl & 8) st(rw, fb3 + fy, r);
-                if (fl & 32) st(rw, fb5 + fz, r);
-            }
-        }
+    }
+    // RES of a boundary cell into the lower or upper planes (as k_forward)
+    __device__ __forceinline__ T face_get(uint32_t f, int p) const {
+        return R[(p & 1) * RS + face_src(f)];
+    }
+    __device__ __forceinline__ void face_put(uint32_t f, T v) const {
+        T *dst = face_hi(f) ? rw : rlo;
+        if (dst) st(dst, face_dst(f), v);
     }
     __device__ __forceinline__ void store(uint32_t e, T v, uint8_t b, int p) const {
         const int x = e & 0xfff;
@@ -176,18 +172,13 @@ __global__ __launch_bounds__(kWave * NW) void k_outer_fwd(
     k.RS = (NJ + 1) * (NI + 1);
     k.mHS = mHS;
     k.nr = nr;
-    k.fb0 = face_base(0, NI, NJ, NK);
-    k.fb1 = face_base(1, NI, NJ, NK);
-    k.fb2 = face_base(2, NI, NJ, NK);
-    k.fb3 = face_base(3, NI, NJ, NK);
-    k.fb4 = face_base(4, NI, NJ, NK);
-    k.fb5 = face_base(5, NI, NJ, NK);
     k.masked = MASKED && !(B.flags & 1);
     k.mx = 0.0;
     constexpr int nw = plan_nw<NW>();
-    run_plan<D, M, NW, MASKED, T>(k, items + S.plan_item[nw][M - 1][kPlanFwdLap] + threadIdx.x / kWave,
-                                  S.plan_nitem[nw][M - 1][kPlanFwdLap],
-                                  t.ents + S.plan_ent[kPlanFwdLap], tab);
+    const int ib = S.plan_item[nw][M - 1][kPlanFwdLap] + threadIdx.x / kWave;
+    run_plan<D, M, NW, MASKED, T>(k, items + ib, S.plan_nitem[nw][M - 1][kPlanFwdLap],
+                                  t.ents + S.plan_ent[kPlanFwdLap], tab,
+                                  t.fents + (int64_t)ib * kWave);
     double mx = k.mx * ldexp(1.0, t.maxlevel - B.level);
     for (int off = 32; off > 0; off >>= 1) {
         const double x = __shfl_down(mx, off, kWave);

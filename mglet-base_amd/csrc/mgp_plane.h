@@ -244,10 +244,21 @@ __device__ __forceinline__ void fill_tab(SrcTab<T> *tab, const Topo &t,
     }
 }
 
+// Fields of a face-store entry (face_entry, mgp_internal.h).
+__device__ __forceinline__ int face_src(uint32_t f) { return f & 0xfff; }
+__device__ __forceinline__ int face_dst(uint32_t f) { return (f >> 12) & 0x1fff; }
+__device__ __forceinline__ bool face_hi(uint32_t f) { return (f >> 25) & 1; }
+
 // The plan engine: runs the items of one block through functor K.  An
 // item covers up to 64 M cells (M per lane) and 64 M entries.  K provides
-// Data, load(Data&, s), compute(const Data&, s, p) and
-// store(entry, value, bp, p).
+// Data, load(Data&, s), compute(const Data&, s, p, l) (l = the cell's row
+// in the item, lane + 64 m), store(entry, value, bp, p), and, with
+// K::kFaces, face_get(entry, p) and face_put(entry, value): the boundary
+// values the item computed, read back from LDS by the lanes of its
+// face-store entries at the end of the item and stored to the face planes,
+// once each, at the end of the next item, so the store does not wait for
+// the read (a per-cell store behind six flag tests costs about as much as
+// the rest of a backward sweep).
 //
 // Pipeline (W = 2D items per loop trip, every ring slot named statically,
 // so no register holding a load in flight is ever copied -- a copy would
@@ -267,7 +278,8 @@ template <int D, int M, int NW, bool EBP, typename T, class K>
 __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
                                          int nitem,
                                          const uint32_t *__restrict__ ents,
-                                         const SrcTab<T> *tab) {
+                                         const SrcTab<T> *tab,
+                                         const uint32_t *__restrict__ fents = nullptr) {
     constexpr int W = 2 * D;
     const int lane = threadIdx.x & (kWave - 1);
     const int last = nitem - 1;
@@ -277,6 +289,9 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
     uint8_t eb[D][M];
     uint2 iv[W];
     uint32_t ix[D], iy[D];
+    uint32_t fw[D];   // face-store entries of the items in flight
+    uint32_t fwp = kFaceNone;   // ... of the previous item, and its value
+    T fvp = T(0);
     const T *pn[M];
     const uint8_t *bn[M];
     // this wave's items: every NW-th of the plan, from its own first one
@@ -303,6 +318,10 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
             const int off = (int)(ew[r][m] >> 16);
             ev[u][m] = gld(pn[m] + off);
             if (EBP) eb[u][m] = gld(bn[m] + off);
+        }
+        if constexpr (K::kFaces) {
+            const int qc = (q < 0 || q > last) ? last : q;
+            fw[u] = ld(fents, (qc * NW) * kFaceSlots + lane);
         }
         const int r2 = (r + D) % W;
         const uint32_t x2 = __builtin_amdgcn_readfirstlane(iv[r2].x);
@@ -331,6 +350,7 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
     for (int u = 0; u < D; ++u) {
         ix[u] = 0u;
         iy[u] = 8u << 21;
+        fw[u] = kFaceNone;
     }
 #pragma unroll
     for (int m = 0; m < M; ++m) lookup(m, 0u);
@@ -343,11 +363,17 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
             const int p = (int)(y >> 21) - 8;
 #pragma unroll
             for (int m = 0; m < M; ++m)
-                if (lane + 64 * m < cc) k.compute(cd[u][m], cpos + lane + 64 * m, p);
+                if (lane + 64 * m < cc) k.compute(cd[u][m], cpos + lane + 64 * m, p, lane + 64 * m);
 #pragma unroll
             for (int m = 0; m < M; ++m)
                 if (lane + 64 * m < ec)
                     k.store(ew[w][m], ev[u][m], EBP ? eb[u][m] : (uint8_t)0, p);
+            if constexpr (K::kFaces) {
+                if (fwp != kFaceNone) k.face_put(fwp, fvp);
+                fwp = fw[u];
+                if (fwp != kFaceNone) fvp = k.face_get(fwp, p);
+            }
+            static_assert(kFaceSlots == kWave, "one face-store entry per lane");
             wave_sync();
             // several waves per block: the next step reads what all waves
             // wrote in this one (the next item was issued D items ago)

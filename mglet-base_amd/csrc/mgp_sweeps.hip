@@ -48,6 +48,7 @@ struct FwdK {
     // factors (an L2 hit), or rebuild it from its separable part when no
     // factors are shared (saves an HBM load)
     static constexpr bool APC = LAP && !MASKED;
+    static constexpr bool kFaces = true;
     struct Data {
         uint2 c;
         T rhs, lpr, lb, ls, lw, ap;
@@ -61,7 +62,6 @@ struct FwdK {
     T *R, *H;
     uint8_t *BH;
     int NI, NJ, NK, HW, RW, RS, mHS, nr;
-    int fb0, fb1, fb2, fb3, fb4, fb5;
     bool masked, apsh;
 
     // every item issues the same loads (the GSAP load reads zeros when the
@@ -75,7 +75,7 @@ struct FwdK {
         d.lw = ld(lw, s);
         if (LAP) d.ap = ld(ap, s);
     }
-    __device__ __forceinline__ void compute(const Data &d, int s, int p) const {
+    __device__ __forceinline__ void compute(const Data &d, int s, int p, int) const {
         int i, j, k;
         cell_ijk(d.c.x, i, j, k);
         T r = T(0);
@@ -110,18 +110,6 @@ struct FwdK {
         }
         Rc[c1] = r;
         st(res, s, r);
-        const uint32_t fl = d.c.y >> 24;
-        if (fl) {
-            const int fx = d.c.y & 4095, fy = (d.c.y >> 12) & 4095, fz = d.c.x >> 18;
-            if (fl & 1) st(rlo, fb0 + fx, r);
-            if (fl & 4) st(rlo, fb2 + fy, r);
-            if (fl & 16) st(rlo, fb4 + fz, r);
-            if (rw) {
-                if (fl & 2) st(rw, fb1 + fx, r);
-                if (fl & 8) st(rw, fb3 + fy, r);
-                if (fl & 32) st(rw, fb5 + fz, r);
-            }
-        }
     }
     __device__ __forceinline__ void store(uint32_t e, T v, uint8_t b, int p) const {
         const int x = e & 0xfff;
@@ -132,6 +120,16 @@ struct FwdK {
             H[sl + x] = v;
             if (MASKED && masked) BH[sl + x] = b;
         }
+    }
+    // RES of a boundary cell (in the step's recurrence plane) into the
+    // lower planes, or the upper planes unless this is the last inner
+    // iteration
+    __device__ __forceinline__ T face_get(uint32_t f, int p) const {
+        return R[(p & 1) * RS + face_src(f)];
+    }
+    __device__ __forceinline__ void face_put(uint32_t f, T v) const {
+        T *dst = face_hi(f) ? rw : rlo;
+        if (dst) st(dst, face_dst(f), v);
     }
 };
 
@@ -188,18 +186,13 @@ __global__ __launch_bounds__(kWave * NW) void k_forward(
     k.RS = (NJ + 1) * (NI + 1);
     k.mHS = mHS;
     k.nr = nr;
-    k.fb0 = face_base(0, NI, NJ, NK);
-    k.fb1 = face_base(1, NI, NJ, NK);
-    k.fb2 = face_base(2, NI, NJ, NK);
-    k.fb3 = face_base(3, NI, NJ, NK);
-    k.fb4 = face_base(4, NI, NJ, NK);
-    k.fb5 = face_base(5, NI, NJ, NK);
     k.masked = MASKED && !(B.flags & 1);
     const int kind = LAP ? kPlanFwdLap : kPlanFwdZero;
     constexpr int nw = plan_nw<NW>();
-    run_plan<D, M, NW, MASKED && LAP, T>(k, items + S.plan_item[nw][M - 1][kind] + threadIdx.x / kWave,
-                                         S.plan_nitem[nw][M - 1][kind], t.ents + S.plan_ent[kind],
-                                         tab);
+    const int ib = S.plan_item[nw][M - 1][kind] + threadIdx.x / kWave;
+    run_plan<D, M, NW, MASKED && LAP, T>(k, items + ib, S.plan_nitem[nw][M - 1][kind],
+                                         t.ents + S.plan_ent[kind], tab,
+                                         t.fents + (int64_t)ib * kWave);
 }
 
 // ---------------------------------------------------------------------
@@ -211,6 +204,7 @@ __global__ __launch_bounds__(kWave * NW) void k_forward(
 // HZ: HILF is zero on entry (first inner iteration of an outer iteration).
 template <typename T, bool CLASSIC, bool HZ>
 struct BwdK {
+    static constexpr bool kFaces = true;
     struct Data {
         uint2 c;
         T res, ue, un, ut, h;
@@ -218,9 +212,8 @@ struct BwdK {
     const uint2 *cells;
     const T *res, *ue, *un, *ut;
     T *hilf, *hface;
-    T *R;
+    T *R, *stg;   // recurrence planes; this wave's staging row of HILF
     int NI, NJ, NK, RW, RS;
-    int fb0, fb1, fb2, fb3, fb4, fb5;
 
     __device__ __forceinline__ void load(Data &d, int s) const {
         d.c = ld(cells, s);
@@ -230,7 +223,7 @@ struct BwdK {
         d.ut = ld(ut, s);
         if (!HZ) d.h = ld((const T *)hilf, s);
     }
-    __device__ __forceinline__ void compute(const Data &d, int s, int p) const {
+    __device__ __forceinline__ void compute(const Data &d, int s, int p, int l) const {
         int i, j, k;
         cell_ijk(d.c.x, i, j, k);
         const int c1 = j + __mul24(RW, i);
@@ -249,20 +242,14 @@ struct BwdK {
         Rc[c1] = r;
         const T h = (HZ ? T(0) : d.h) + r;
         st(hilf, s, h);
-        const uint32_t fl = d.c.y >> 24;
-        if (fl) {
-            const int fx = d.c.y & 4095, fy = (d.c.y >> 12) & 4095, fz = d.c.x >> 18;
-            if (fl & 1) st(hface, fb0 + fx, h);
-            if (fl & 2) st(hface, fb1 + fx, h);
-            if (fl & 4) st(hface, fb2 + fy, h);
-            if (fl & 8) st(hface, fb3 + fy, h);
-            if (fl & 16) st(hface, fb4 + fz, h);
-            if (fl & 32) st(hface, fb5 + fz, h);
-        }
+        stg[l] = h;
     }
     __device__ __forceinline__ void store(uint32_t e, T v, uint8_t, int p) const {
         R[(p & 1) * RS + (e & 0xfff)] = v;
     }
+    // HILF of a boundary cell into the block's boundary planes
+    __device__ __forceinline__ T face_get(uint32_t f, int) const { return stg[face_src(f)]; }
+    __device__ __forceinline__ void face_put(uint32_t f, T v) const { st(hface, face_dst(f), v); }
 };
 
 template <typename T, bool CLASSIC, bool HZ, int D, int M, int NW>
@@ -275,6 +262,7 @@ __global__ __launch_bounds__(kWave * NW) void k_backward(
     const Shape &S = t.shapes[B.shape];
     SrcTab<T> *tab = (SrcTab<T> *)lds_bw;
     T *R = (T *)(lds_bw + sizeof(SrcTab<T>));
+    const int mRS = (t.max_nj + 1) * (t.max_ni + 1);
     // R sources: the neighbours' lower planes feed faces 1, 3, 5
     fill_tab<T>(tab, t, B, nullptr, nullptr, F.hilf, F.rlo, 0x2a, Mask{});
     if (NW > 1) __syncthreads(); else wave_sync();
@@ -289,21 +277,17 @@ __global__ __launch_bounds__(kWave * NW) void k_backward(
     k.hilf = F.hilf + wo;
     k.hface = F.hface + B.fc;
     k.R = R;
+    k.stg = R + 2 * mRS + (threadIdx.x / kWave) * (kWave * M);
     k.NI = NI;
     k.NJ = NJ;
     k.NK = NK;
     k.RW = NJ + 1;
     k.RS = (NJ + 1) * (NI + 1);
-    k.fb0 = face_base(0, NI, NJ, NK);
-    k.fb1 = face_base(1, NI, NJ, NK);
-    k.fb2 = face_base(2, NI, NJ, NK);
-    k.fb3 = face_base(3, NI, NJ, NK);
-    k.fb4 = face_base(4, NI, NJ, NK);
-    k.fb5 = face_base(5, NI, NJ, NK);
     constexpr int nw = plan_nw<NW>();
-    run_plan<D, M, NW, false, T>(k, items + S.plan_item[nw][M - 1][kPlanBwd] + threadIdx.x / kWave,
-                                 S.plan_nitem[nw][M - 1][kPlanBwd], t.ents + S.plan_ent[kPlanBwd],
-                                 tab);
+    const int ib = S.plan_item[nw][M - 1][kPlanBwd] + threadIdx.x / kWave;
+    run_plan<D, M, NW, false, T>(k, items + ib, S.plan_nitem[nw][M - 1][kPlanBwd],
+                                 t.ents + S.plan_ent[kPlanBwd], tab,
+                                 t.fents + (int64_t)ib * kWave);
 }
 
 // ---------------------------------------------------------------------
@@ -323,6 +307,7 @@ __global__ __launch_bounds__(kWave * NW) void k_backward(
 // HZ: HILF is zero on entry (first inner iteration of an outer iteration).
 template <typename T, bool MASKED, bool HZ>
 struct SorK {
+    static constexpr bool kFaces = true;
     struct Data {
         uint2 c;
         T rhs, rap;
@@ -336,7 +321,6 @@ struct SorK {
     T *H;
     uint8_t *BH;
     int NI, NJ, NK, HW, mHS, nr;
-    int fb0, fb1, fb2, fb3, fb4, fb5;
     T om, om1;
     bool masked, keep_top;
 
@@ -345,7 +329,7 @@ struct SorK {
         d.rhs = ld(rhs, s);
         if (MASKED) d.rap = ld(rap, s);
     }
-    __device__ __forceinline__ void compute(const Data &d, int s, int p) const {
+    __device__ __forceinline__ void compute(const Data &d, int s, int p, int) const {
         const int u = (p & 1) ? p : p - 2;
         int i, j, k;
         cell_ijk(d.c.x, i, j, k);
@@ -387,17 +371,14 @@ struct SorK {
         }
         H0[c0] = nv;
         st(hilf, s, nv);
-        const uint32_t fl = d.c.y >> 24;
-        if (fl) {
-            const int fx = d.c.y & 4095, fy = (d.c.y >> 12) & 4095, fz = d.c.x >> 18;
-            if (fl & 1) st(fout, fb0 + fx, nv);
-            if (fl & 2) st(fout, fb1 + fx, nv);
-            if (fl & 4) st(fout, fb2 + fy, nv);
-            if (fl & 8) st(fout, fb3 + fy, nv);
-            if (fl & 16) st(fout, fb4 + fz, nv);
-            if (fl & 32) st(fout, fb5 + fz, nv);
-        }
     }
+    // the relaxed value of a boundary cell (in its window slot) into the
+    // new boundary planes
+    __device__ __forceinline__ T face_get(uint32_t f, int p) const {
+        const int u = (p & 1) ? p : p - 2;
+        return H[((u + 1) & 7) * mHS + face_src(f)];
+    }
+    __device__ __forceinline__ void face_put(uint32_t f, T v) const { st(fout, face_dst(f), v); }
     __device__ __forceinline__ void store(uint32_t e, T v, uint8_t b, int p) const {
         const int x = e & 0xfff;
         const int sl = ((p + 3) & 7) * mHS;
@@ -445,19 +426,14 @@ __global__ __launch_bounds__(kWave) void k_sor(
     k.HW = NJ + 2;
     k.mHS = mHS;
     k.nr = nr;
-    k.fb0 = face_base(0, NI, NJ, NK);
-    k.fb1 = face_base(1, NI, NJ, NK);
-    k.fb2 = face_base(2, NI, NJ, NK);
-    k.fb3 = face_base(3, NI, NJ, NK);
-    k.fb4 = face_base(4, NI, NJ, NK);
-    k.fb5 = face_base(5, NI, NJ, NK);
     k.om = (T)omega;
     k.om1 = (T)(1.0 - omega);
     k.masked = MASKED && !(B.flags & 1);
     k.keep_top = (NK & 1) != 0;
-    run_plan<D, M, 1, MASKED, T>(k, items + S.plan_item[0][M - 1][kPlanSor],
-                                 S.plan_nitem[0][M - 1][kPlanSor], t.ents + S.plan_ent[kPlanSor],
-                                 tab);
+    const int ib = S.plan_item[0][M - 1][kPlanSor];
+    run_plan<D, M, 1, MASKED, T>(k, items + ib, S.plan_nitem[0][M - 1][kPlanSor],
+                                 t.ents + S.plan_ent[kPlanSor], tab,
+                                 t.fents + (int64_t)ib * kWave);
 }
 
 // ---------------------------------------------------------------------
@@ -466,6 +442,7 @@ __global__ __launch_bounds__(kWave) void k_sor(
 // In the mixed-precision mode the sweeps' RHS copy is refreshed here.
 template <typename T, bool MASKED>
 struct OutK {
+    static constexpr bool kFaces = false;
     struct Data {
         uint2 c;
         double ap, rhs, dp;
@@ -489,7 +466,7 @@ struct OutK {
         d.rhs = ld((const double *)rhs, s);
         d.dp = ld((const double *)dp, s);
     }
-    __device__ __forceinline__ void compute(const Data &d, int s, int p) {
+    __device__ __forceinline__ void compute(const Data &d, int s, int p, int) {
         int i, j, k;
         cell_ijk(d.c.x, i, j, k);
         const int c0 = (j + 1) + __mul24(HW, i + 1);
@@ -638,17 +615,22 @@ template void launch_forward<float>(const Topo &, const Fields<float> &, Mask,
 template <typename T, bool CLASSIC, bool HZ>
 static void bwd_launch(const Topo &t, const Fields<T> &f, const int *done,
                        size_t lds, hipStream_t s) {
+    // each wave stages its item's HILF values (64 M) for the face stores
+#define MGP_BWD(D, M, NW) \
+    k_backward<T, CLASSIC, HZ, D, M, NW><<<t.nlist, kWave * NW, \
+        lds + (size_t)NW * kWave * M * sizeof(T), s>>>(t.items, t, f, done)
     switch (g_shape) {
-    case 1: k_backward<T, CLASSIC, HZ, 4, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
-    case 3: k_backward<T, CLASSIC, HZ, 3, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
-    case 4: k_backward<T, CLASSIC, HZ, 6, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
-    case 5: k_backward<T, CLASSIC, HZ, 8, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
-    case 6: k_backward<T, CLASSIC, HZ, 4, 1, 4><<<t.nlist, kWave * 4, lds, s>>>(t.items, t, f, done); break;
-    case 7: k_backward<T, CLASSIC, HZ, 4, 1, 8><<<t.nlist, kWave * 8, lds, s>>>(t.items, t, f, done); break;
-    case 8: k_backward<T, CLASSIC, HZ, 3, 2, 2><<<t.nlist, kWave * 2, lds, s>>>(t.items, t, f, done); break;
-    case 9: k_backward<T, CLASSIC, HZ, 4, 1, 2><<<t.nlist, kWave * 2, lds, s>>>(t.items, t, f, done); break;
-    default: k_backward<T, CLASSIC, HZ, 2, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, done); break;
+    case 1: MGP_BWD(4, 1, 1); break;
+    case 3: MGP_BWD(3, 2, 1); break;
+    case 4: MGP_BWD(6, 2, 1); break;
+    case 5: MGP_BWD(8, 1, 1); break;
+    case 6: MGP_BWD(4, 1, 4); break;
+    case 7: MGP_BWD(4, 1, 8); break;
+    case 8: MGP_BWD(3, 2, 2); break;
+    case 9: MGP_BWD(4, 1, 2); break;
+    default: MGP_BWD(2, 2, 1); break;
     }
+#undef MGP_BWD
 }
 
 template <typename T>
