@@ -261,7 +261,7 @@ struct mgp_handle {
         t.maxlevel = maxlevel;
         t.items = d_items.as<uint2>();
         t.ents = d_ents.as<uint32_t>();
-        t.cellx = d_cellx.as<uint2>();
+        t.cellx = d_cellx.as<uint32_t>();
         t.fents = d_fents.as<uint32_t>();
         t.zero = d_zero.p;
         if (lvl < 0) {
@@ -694,7 +694,7 @@ void exchange(mgp_handle &h, int level, int layers, int corners, int forward,
 // buffer p mod 2.  Steps before the first plane only fill LDS.
 void build_plans(mgp_handle &h, const std::vector<uint32_t> &cells,
                  const std::vector<int> &pstart, std::vector<uint2> &items,
-                 std::vector<uint32_t> &ents, std::vector<uint2> &cellx,
+                 std::vector<uint32_t> &ents, std::vector<uint32_t> &cellx,
                  std::vector<uint32_t> &fents) {
     cellx.resize(cells.size());
     for (Shape &S : h.shapes) {
@@ -702,16 +702,12 @@ void build_plans(mgp_handle &h, const std::vector<uint32_t> &cells,
         const int HW = NJ + 2, RW = NJ + 1;
         const uint32_t *cl = cells.data() + S.cell_off;
         const int *ps = pstart.data() + S.pstart_off;
-        // cell words: indices, face positions and boundary flags
+        // cell words: indices and boundary flags
         for (int s = 0; s < S.ncell; ++s) {
             const int i = cl[s] & 1023, j = (cl[s] >> 10) & 1023, k = cl[s] >> 20;
             const uint32_t fl = (i == 0) | (i == NI - 1) << 1 | (j == 0) << 2 |
                                 (j == NJ - 1) << 3 | (k == 0) << 4 | (k == NK - 1) << 5;
-            uint2 w;
-            w.x = (uint32_t)i | (uint32_t)j << 6 | (uint32_t)k << 12 |
-                  (uint32_t)fidx_z(j, i, NI, NJ) << 18;
-            w.y = (uint32_t)fidx_x(k, j, NJ, NK) | (uint32_t)fidx_y(k, i, NI, NK) << 12 | fl << 24;
-            cellx[S.cell_off + s] = w;
+            cellx[S.cell_off + s] = (uint32_t)i | (uint32_t)j << 6 | (uint32_t)k << 12 | fl << 18;
         }
         using Ents = std::vector<uint32_t>;
         // HILF window plane q: its interior cells, then the face ghosts
@@ -1223,7 +1219,7 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
         build_shapes(*h, cells, pstart, slot);
         std::vector<uint2> items;
         std::vector<uint32_t> ents;
-        std::vector<uint2> cellx;
+        std::vector<uint32_t> cellx;
         std::vector<uint32_t> fents;
         build_plans(*h, cells, pstart, items, ents, cellx, fents);
         build_conns(*h);

@@ -30,11 +30,11 @@ template <typename T, bool CLASSIC, bool MASKED>
 struct OutFwdK {
     static constexpr bool kFaces = true;
     struct Data {
-        uint2 c;
+        uint32_t c;
         double ap, rhs, dp;
         T lpr, lb, ls, lw;
     };
-    const uint2 *cells;
+    const uint32_t *cells;
     // outer update
     const double *ar;
     WallFold wf;
@@ -63,11 +63,11 @@ struct OutFwdK {
     }
     __device__ __forceinline__ void compute(const Data &d, int s, int p, int) {
         int i, j, k;
-        cell_ijk(d.c.x, i, j, k);
+        cell_ijk(d.c, i, j, k);
         // outer update of iteration n (as k_outer)
         const int c0 = (j + 1) + __mul24(HW, i + 1);
         const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS, sp = ((p + 2) & 3) * mHS;
-        const double apv = MASKED ? d.ap : ap_cell(ar, nr, i, j, k, d.c.y >> 24, wf);
+        const double apv = MASKED ? d.ap : ap_cell(ar, nr, i, j, k, cell_flags(d.c), wf);
         const Coef6<double> co(cf, nr, i, j, k);
         double q;
         if (MASKED && masked)

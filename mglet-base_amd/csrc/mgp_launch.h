@@ -21,7 +21,7 @@ struct Topo {
     int maxlevel;
     const uint2 *items;    // plan items of all shapes, widths and kinds
     const uint32_t *ents;  // plan entries
-    const uint2 *cellx;    // cell words of the plane kernels (per WO slot)
+    const uint32_t *cellx; // cell words of the plane kernels (per WO slot)
     const uint32_t *fents; // face-store entries, 64 per plan item (parallel to items)
     const void *zero;      // kZeroBytes of zeros: source of absent ghosts
 };

@@ -25,15 +25,14 @@ __device__ __forceinline__ void st(T *base, int s, T v) {
 }
 
 // Cell word of the plane kernels (per shape and WO cell, built on the
-// host): x = i | j << 6 | k << 12 | fz << 18, y = fx | fy << 12 |
-// flags << 24, with fx, fy, fz the cell's position in the x, y and z face
-// buffers and flags bit 0..5 = (i == 0, i == NI-1, j == 0, j == NJ-1,
-// k == 0, k == NK-1).
+// host): i | j << 6 | k << 12 | flags << 18 with flags bit 0..5 = (i == 0,
+// i == NI-1, j == 0, j == NJ-1, k == 0, k == NK-1).
 __device__ __forceinline__ void cell_ijk(uint32_t w, int &i, int &j, int &k) {
     i = w & 63;
     j = (w >> 6) & 63;
     k = (w >> 12) & 63;
 }
+__device__ __forceinline__ uint32_t cell_flags(uint32_t w) { return w >> 18; }
 
 // Load through a pointer read from LDS: marks it as global memory (a
 // generic pointer would become a flat load, which the compiler can only

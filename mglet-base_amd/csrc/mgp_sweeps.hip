@@ -50,10 +50,10 @@ struct FwdK {
     static constexpr bool APC = LAP && !MASKED;
     static constexpr bool kFaces = true;
     struct Data {
-        uint2 c;
+        uint32_t c;
         T rhs, lpr, lb, ls, lw, ap;
     };
-    const uint2 *cells;
+    const uint32_t *cells;
     const T *rhs, *lpr, *lb, *ls, *lw, *ap;
     T *res, *rlo, *rw;
     const T *cf;
@@ -77,7 +77,7 @@ struct FwdK {
     }
     __device__ __forceinline__ void compute(const Data &d, int s, int p, int) const {
         int i, j, k;
-        cell_ijk(d.c.x, i, j, k);
+        cell_ijk(d.c, i, j, k);
         T r = T(0);
         if (LAP) {
             // LDS positions stay below 2^24: full-rate 24-bit multiplies
@@ -85,7 +85,7 @@ struct FwdK {
             const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS,
                       sp = ((p + 2) & 3) * mHS;
             const Coef6<T> co(cf, nr, i, j, k);
-            const T apv = (APC && !apsh) ? (T)ap_cell(ar, nr, i, j, k, d.c.y >> 24, wf) : d.ap;
+            const T apv = (APC && !apsh) ? (T)ap_cell(ar, nr, i, j, k, cell_flags(d.c), wf) : d.ap;
             if (MASKED && masked)
                 r = laplace_cell<T, T, true>(H + sm, H + s0, H + sp, BH + sm, BH + s0,
                                              BH + sp, c0, HW, co.w, co.e, co.s, co.n, co.b, co.t,
@@ -206,10 +206,10 @@ template <typename T, bool CLASSIC, bool HZ>
 struct BwdK {
     static constexpr bool kFaces = true;
     struct Data {
-        uint2 c;
+        uint32_t c;
         T res, ue, un, ut, h;
     };
-    const uint2 *cells;
+    const uint32_t *cells;
     const T *res, *ue, *un, *ut;
     T *hilf, *hface;
     T *R, *stg;   // recurrence planes; this wave's staging row of HILF
@@ -225,7 +225,7 @@ struct BwdK {
     }
     __device__ __forceinline__ void compute(const Data &d, int s, int p, int l) const {
         int i, j, k;
-        cell_ijk(d.c.x, i, j, k);
+        cell_ijk(d.c, i, j, k);
         const int c1 = j + __mul24(RW, i);
         const T *Rp = R + ((p + 1) & 1) * RS;
         T *Rc = R + (p & 1) * RS;
@@ -309,10 +309,10 @@ template <typename T, bool MASKED, bool HZ>
 struct SorK {
     static constexpr bool kFaces = true;
     struct Data {
-        uint2 c;
+        uint32_t c;
         T rhs, rap;
     };
-    const uint2 *cells;
+    const uint32_t *cells;
     const T *rhs, *rap;
     T *hilf, *fout;
     const T *cf;
@@ -332,7 +332,7 @@ struct SorK {
     __device__ __forceinline__ void compute(const Data &d, int s, int p, int) const {
         const int u = (p & 1) ? p : p - 2;
         int i, j, k;
-        cell_ijk(d.c.x, i, j, k);
+        cell_ijk(d.c, i, j, k);
         const int c0 = (j + 1) + __mul24(HW, i + 1);
         T *H0 = H + ((u + 1) & 7) * mHS;
         const T *Hm = H + (u & 7) * mHS, *Hp = H + ((u + 2) & 7) * mHS;
@@ -363,7 +363,7 @@ struct SorK {
                 rap = d.rap;
             } else {
                 // SOR_RAP = divide0(1, GSAP), src/flow/sor_mod.F90:18-49
-                const double a = ap_cell(ar, nr, i, j, k, d.c.y >> 24, wf);
+                const double a = ap_cell(ar, nr, i, j, k, cell_flags(d.c), wf);
                 rap = (T)(a == 0.0 ? 0.0 : 1.0 / a);
             }
             r = (r - d.rhs) * rap;
@@ -444,10 +444,10 @@ template <typename T, bool MASKED>
 struct OutK {
     static constexpr bool kFaces = false;
     struct Data {
-        uint2 c;
+        uint32_t c;
         double ap, rhs, dp;
     };
-    const uint2 *cells;
+    const uint32_t *cells;
     const double *ar;
     WallFold wf;
     const double *ap;
@@ -468,10 +468,10 @@ struct OutK {
     }
     __device__ __forceinline__ void compute(const Data &d, int s, int p, int) {
         int i, j, k;
-        cell_ijk(d.c.x, i, j, k);
+        cell_ijk(d.c, i, j, k);
         const int c0 = (j + 1) + __mul24(HW, i + 1);
         const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS, sp = ((p + 2) & 3) * mHS;
-        const double apv = MASKED ? d.ap : ap_cell(ar, nr, i, j, k, d.c.y >> 24, wf);
+        const double apv = MASKED ? d.ap : ap_cell(ar, nr, i, j, k, cell_flags(d.c), wf);
         const Coef6<double> co(cf, nr, i, j, k);
         double r;
         if (MASKED && masked)
