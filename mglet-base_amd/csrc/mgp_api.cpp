@@ -822,13 +822,56 @@ void build_plans(mgp_handle &h, const std::vector<uint32_t> &cells,
             }
             // one valid entry for the clamped loads of empty items
             ents.push_back(plan_entry(0, 7, 0, 0));
-            if ((int)ents.size() - ebase0 >= (1 << 21))
-                throw HipError{"plan entry table too large"};
             auto pack = [](int cpos, int cc, int ec, int epos, int p) {
                 uint2 v;
                 v.x = (uint32_t)cpos | (uint32_t)cc << 16 | (uint32_t)ec << 24;
                 v.y = (uint32_t)epos | (uint32_t)(p + 8) << 21;
                 return v;
+            };
+            // One wave per block keeps three window slots in the FP64 kernels
+            // that read HILF's stencil (win_slots): an entry of window(p+2) lands in
+            // the slot plane p-1 occupies, so it goes to an item no earlier
+            // than the one computing the last cell of plane p that reads
+            // plane p-1 at its position (the cell itself, its j+1 and its i+1
+            // neighbour: Hm[c0], Hm[c0-1], Hm[c0-HW]); entries are reordered
+            // per step by that item and spill into entry-only items of the
+            // same step where the step's compute items cannot take them.
+            const bool three = (kind == kPlanFwdLap || kind == kPlanOut) && !h.fp32();
+            const int HS = (NI + 2) * HW;
+            auto scheduled = [&](int W) {
+                std::vector<int> cellitem(HS);
+                for (const Step &st : steps) {
+                    std::fill(cellitem.begin(), cellitem.end(), -1);
+                    for (int t = 0; t < st.ccnt; ++t) {
+                        const uint32_t w = cl[st.cpos + t];
+                        cellitem[((w >> 10) & 1023) + 1 + HW * ((w & 1023) + 1)] = t / W;
+                    }
+                    std::vector<std::pair<int, uint32_t>> es;
+                    for (int q = 0; q < st.ecnt; ++q) {
+                        const uint32_t e = ents[ebase0 + st.epos + q];
+                        int last = 0;
+                        if (!(e & 0x8000u)) {
+                            const int x = e & 0xfff;
+                            for (const int c : {x, x + 1, x + HW})
+                                if (c < HS && cellitem[c] > last) last = cellitem[c];
+                        }
+                        es.push_back({last, e});
+                    }
+                    std::stable_sort(es.begin(), es.end(),
+                                     [](const auto &a, const auto &b) { return a.first < b.first; });
+                    const int nc = (st.ccnt + W - 1) / W;
+                    size_t idx = 0;
+                    for (int c = 0; c < nc || idx < es.size(); ++c) {
+                        const int epos = (int)ents.size() - ebase0;
+                        int ec = 0;
+                        while (idx < es.size() && ec < W && es[idx].first <= c) {
+                            ents.push_back(es[idx++].second);
+                            ++ec;
+                        }
+                        const int cc = c < nc ? std::min(st.ccnt - W * c, W) : 0;
+                        items.push_back(pack(cc ? st.cpos + W * c : 0, cc, ec, ec ? epos : 0, st.p));
+                    }
+                }
             };
             for (int nwi = 0; nwi < 4; ++nwi) {
                 const int NW = kPlanWaves[nwi];
@@ -836,7 +879,8 @@ void build_plans(mgp_handle &h, const std::vector<uint32_t> &cells,
                     const int W = 64 << wi;
                     const size_t base = items.size();
                     S.plan_item[nwi][wi][kind] = (int)base;
-                    for (const Step &st : steps) {
+                    if (three && NW == 1) scheduled(W);
+                    else for (const Step &st : steps) {
                         const int n = std::max((st.ccnt + W - 1) / W, (st.ecnt + W - 1) / W);
                         const int rows = (n + NW - 1) / NW;
                         for (int c = 0; c < rows * NW; ++c) {
@@ -850,6 +894,8 @@ void build_plans(mgp_handle &h, const std::vector<uint32_t> &cells,
                                                  ec ? st.epos + W * c : 0, st.p));
                         }
                     }
+                    if ((int)ents.size() - ebase0 >= (1 << 21))
+                        throw HipError{"plan entry table too large"};
                     // closing no-op items (the engine clamps item indices to
                     // the last one)
                     do

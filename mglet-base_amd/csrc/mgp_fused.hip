@@ -26,7 +26,7 @@
 
 namespace mgp {
 
-template <typename T, bool CLASSIC, bool MASKED>
+template <typename T, bool CLASSIC, bool MASKED, int NS>
 struct OutFwdK {
     static constexpr bool kFaces = true;
     struct Data {
@@ -66,7 +66,8 @@ struct OutFwdK {
         cell_ijk(d.c, i, j, k);
         // outer update of iteration n (as k_outer)
         const int c0 = (j + 1) + __mul24(HW, i + 1);
-        const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS, sp = ((p + 2) & 3) * mHS;
+        const int sm = win_slot<NS>(p - 1) * mHS, s0 = win_slot<NS>(p) * mHS,
+                  sp = win_slot<NS>(p + 1) * mHS;
         const double apv = MASKED ? d.ap : ap_cell(ar, nr, i, j, k, cell_flags(d.c), wf);
         const Coef6<double> co(cf, nr, i, j, k);
         double q;
@@ -115,7 +116,7 @@ struct OutFwdK {
         if (e & 0x8000u) {
             R[(p & 1) * RS + x] = v;
         } else {
-            const int sl = ((p + 3) & 3) * mHS;
+            const int sl = win_slot<NS>(p + 2) * mHS;
             H[sl + x] = v;
             if (MASKED && masked) BH[sl + x] = b;
         }
@@ -139,7 +140,8 @@ __global__ __launch_bounds__(kWave * NW) void k_outer_fwd(
     double *ar = cf + 6 * nr;
     T *R = (T *)(ar + (MASKED ? 0 : 3 * nr));
     T *H = R + 2 * mRS;
-    uint8_t *BH = (uint8_t *)(H + 4 * mHS);
+    constexpr int NS = win_slots<T, NW>();
+    uint8_t *BH = (uint8_t *)(H + NS * mHS);
     // window sources (HILF and its face ghosts) and the recurrence sources
     // (the neighbours' upper RES planes feed faces 0, 2, 4)
     fill_tab<T>(tab, t, B, F.hface, F.hpar, F.hilf, F.rhi[rd], 0x15, m);
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(kWave * NW) void k_outer_fwd(
     if (NW > 1) __syncthreads(); else wave_sync();
     const int NI = B.ni, NJ = B.nj, NK = B.nk;
     const int64_t wo = B.wo, fc = B.fc;
-    OutFwdK<T, CLASSIC, MASKED> k;
+    OutFwdK<T, CLASSIC, MASKED, NS> k;
     k.cells = t.cellx + S.cell_off;
     k.ar = ar;
     k.wf.load(B);
@@ -190,9 +192,10 @@ __global__ __launch_bounds__(kWave * NW) void k_outer_fwd(
 
 template <typename T, bool CLASSIC, bool MASKED>
 static void ofw_launch(const Topo &t, const Fields<T> &f, Mask m, Outer o, T *rhs_t,
-                       int rd, int write_hi, size_t lds, hipStream_t s) {
+                       int rd, int write_hi, size_t lds, size_t wslot, hipStream_t s) {
 #define MGP_OFW(D, M, NW) \
-    k_outer_fwd<T, CLASSIC, MASKED, D, M, NW><<<t.nlist, kWave * NW, lds, s>>>( \
+    k_outer_fwd<T, CLASSIC, MASKED, D, M, NW><<<t.nlist, kWave * NW, \
+        lds + win_slots<T, NW>() * wslot, s>>>( \
         t.items, t, f, m, o, rhs_t, rd, write_hi)
     switch (plan_shape()) {
     case 1: MGP_OFW(4, 1, 1); break;
@@ -217,14 +220,14 @@ void launch_outer_fwd(const Topo &t, const Fields<T> &f, Mask m, Outer o, T *rhs
     const bool masked = m.bp != nullptr;
     const size_t nr = row_len(t);
     const size_t lds = sizeof(SrcTab<T>) + 6 * nr * sizeof(double) +
-                       (masked ? 0 : 3 * nr * sizeof(double)) + 2 * mRS * sizeof(T) +
-                       4 * mHS * sizeof(T) + (masked ? 4 * mHS : 0);
+                       (masked ? 0 : 3 * nr * sizeof(double)) + 2 * mRS * sizeof(T);
+    const size_t wslot = mHS * sizeof(T) + (masked ? mHS : 0);
     if (classic) {
-        if (masked) ofw_launch<T, true, true>(t, f, m, o, rhs_t, rd, write_hi, lds, s);
-        else ofw_launch<T, true, false>(t, f, m, o, rhs_t, rd, write_hi, lds, s);
+        if (masked) ofw_launch<T, true, true>(t, f, m, o, rhs_t, rd, write_hi, lds, wslot, s);
+        else ofw_launch<T, true, false>(t, f, m, o, rhs_t, rd, write_hi, lds, wslot, s);
     } else {
-        if (masked) ofw_launch<T, false, true>(t, f, m, o, rhs_t, rd, write_hi, lds, s);
-        else ofw_launch<T, false, false>(t, f, m, o, rhs_t, rd, write_hi, lds, s);
+        if (masked) ofw_launch<T, false, true>(t, f, m, o, rhs_t, rd, write_hi, lds, wslot, s);
+        else ofw_launch<T, false, false>(t, f, m, o, rhs_t, rd, write_hi, lds, wslot, s);
     }
 }
 template void launch_outer_fwd<double>(const Topo &, const Fields<double> &, Mask, Outer,

@@ -49,6 +49,25 @@ constexpr int plan_nw() {
     return NW == 1 ? 0 : (NW == 2 ? 1 : (NW == 4 ? 2 : 3));
 }
 
+// HILF window ring of the kernels that read its 7-point stencil: plane q
+// lives in slot (q + 1) mod NS.  With one wave per block three slots
+// suffice when the plan stores each window entry of plane p+2 (slot of
+// plane p-1) only after the item that computes the last cell of plane p
+// reading plane p-1 at that position; the FP64 kernels use them (one more
+// resident wave per CU: FP64 forward sweep 5.89 -> 5.54 ms on 32768 blocks
+// of 24^3).  The FP32 kernels keep four slots and the plain entry order,
+// which their window loads coalesce better (the scheduled order cost them
+// 3 %, more than the extra waves gained).  Several waves per block keep
+// four slots.
+template <typename T, int NW>
+constexpr int win_slots() { return NW == 1 && sizeof(T) == 8 ? 3 : 4; }
+template <int NS>
+__device__ __forceinline__ int win_slot(int q) {
+    // q >= -4: an unsigned remainder by a constant (a multiply-high), which
+    // the kernels' four uses per step share
+    return NS == 4 ? ((q + 1) & 3) : (int)((unsigned)(q + 1 + 3 * NS) % (unsigned)NS);
+}
+
 // Barrier of the waves of a block over LDS only: orders LDS accesses
 // across it without waiting for the global loads in flight (a full
 // __syncthreads would drain the prefetch pipeline at every plane).

@@ -42,7 +42,7 @@ namespace mgp {
 // 1/3/5), the lower planes always go to rlo for the backward sweep.
 // LAP = false is the first inner iteration of an outer iteration, where
 // HILF = 0 and the Laplacian term vanishes.
-template <typename T, bool CLASSIC, bool MASKED, bool LAP>
+template <typename T, bool CLASSIC, bool MASKED, bool LAP, int NS>
 struct FwdK {
     // launches without blocked cells take GSAP from the copy shared with the
     // factors (an L2 hit), or rebuild it from its separable part when no
@@ -82,8 +82,8 @@ struct FwdK {
         if (LAP) {
             // LDS positions stay below 2^24: full-rate 24-bit multiplies
             const int c0 = (j + 1) + __mul24(HW, i + 1);
-            const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS,
-                      sp = ((p + 2) & 3) * mHS;
+            const int sm = win_slot<NS>(p - 1) * mHS, s0 = win_slot<NS>(p) * mHS,
+                      sp = win_slot<NS>(p + 1) * mHS;
             const Coef6<T> co(cf, nr, i, j, k);
             const T apv = (APC && !apsh) ? (T)ap_cell(ar, nr, i, j, k, cell_flags(d.c), wf) : d.ap;
             if (MASKED && masked)
@@ -116,7 +116,7 @@ struct FwdK {
         if (e & 0x8000u) {
             R[(p & 1) * RS + x] = v;
         } else {
-            const int sl = ((p + 3) & 3) * mHS;
+            const int sl = win_slot<NS>(p + 2) * mHS;
             H[sl + x] = v;
             if (MASKED && masked) BH[sl + x] = b;
         }
@@ -144,14 +144,15 @@ __global__ __launch_bounds__(kWave * NW) void k_forward(
     const Shape &S = t.shapes[B.shape];
     const int mRS = (t.max_nj + 1) * (t.max_ni + 1);
     const int mHS = (t.max_nj + 2) * (t.max_ni + 2);
-    constexpr bool APC = FwdK<T, CLASSIC, MASKED, LAP>::APC;
+    constexpr int NS = win_slots<T, NW>();
+    constexpr bool APC = FwdK<T, CLASSIC, MASKED, LAP, NS>::APC;
     SrcTab<T> *tab = (SrcTab<T> *)lds_fw;
     T *cf = (T *)(lds_fw + sizeof(SrcTab<T>));
     const int nr = row_len(t);
     double *ar = (double *)(cf + 6 * nr);
     T *R = (T *)(ar + (APC ? 3 * nr : 0));
     T *H = R + 2 * mRS;
-    uint8_t *BH = (uint8_t *)(H + 4 * mHS);
+    uint8_t *BH = (uint8_t *)(H + NS * mHS);
     // R sources: the neighbours' upper planes feed faces 0, 2, 4
     fill_tab<T>(tab, t, B, LAP ? F.hface : nullptr, F.hpar, F.hilf, F.rhi[rd],
                 0x15, m);
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(kWave * NW) void k_forward(
     if (NW > 1) __syncthreads(); else wave_sync();
     const int NI = B.ni, NJ = B.nj, NK = B.nk;
     const int64_t wo = B.wo, fc = B.fc;
-    FwdK<T, CLASSIC, MASKED, LAP> k;
+    FwdK<T, CLASSIC, MASKED, LAP, NS> k;
     k.cells = t.cellx + S.cell_off;
     k.rhs = F.rhs + wo;
     k.lw = F.f[0] + B.fo;
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(kWave) void k_sor(
 // FP64 outer update (src/flow/pressuresolver_mod.F90:258-279): RES =
 // -(A HILF) with the FP64 operator, RHS += RES, max |RHS|, DP += HILF.
 // In the mixed-precision mode the sweeps' RHS copy is refreshed here.
-template <typename T, bool MASKED>
+template <typename T, bool MASKED, int NS>
 struct OutK {
     static constexpr bool kFaces = false;
     struct Data {
@@ -470,7 +471,8 @@ struct OutK {
         int i, j, k;
         cell_ijk(d.c, i, j, k);
         const int c0 = (j + 1) + __mul24(HW, i + 1);
-        const int sm = (p & 3) * mHS, s0 = ((p + 1) & 3) * mHS, sp = ((p + 2) & 3) * mHS;
+        const int sm = win_slot<NS>(p - 1) * mHS, s0 = win_slot<NS>(p) * mHS,
+                  sp = win_slot<NS>(p + 1) * mHS;
         const double apv = MASKED ? d.ap : ap_cell(ar, nr, i, j, k, cell_flags(d.c), wf);
         const Coef6<double> co(cf, nr, i, j, k);
         double r;
@@ -491,7 +493,7 @@ struct OutK {
     }
     __device__ __forceinline__ void store(uint32_t e, T v, uint8_t b, int p) const {
         const int x = e & 0xfff;
-        const int sl = ((p + 3) & 3) * mHS;
+        const int sl = win_slot<NS>(p + 2) * mHS;
         H[sl + x] = v;
         if (MASKED && masked) BH[sl + x] = b;
     }
@@ -511,14 +513,15 @@ __global__ __launch_bounds__(kWave * NW) void k_outer(
     double *cf = (double *)(lds_ou + sizeof(SrcTab<T>));
     const int nr = row_len(t);
     double *ar = cf + 6 * nr;
+    constexpr int NS = win_slots<T, NW>();
     T *H = (T *)(ar + (MASKED ? 0 : 3 * nr));
-    uint8_t *BH = (uint8_t *)(H + 4 * mHS);
+    uint8_t *BH = (uint8_t *)(H + NS * mHS);
     fill_tab<T>(tab, t, B, F.hface, F.hpar, F.hilf, nullptr, 0, m);
     coef_rows<double>(cf, o.aw, o.ae, o.as, o.an, o.ab, o.at, B, nr);
     if (!MASKED) ap_rows(ar, F.apx, F.apy, F.apz, B, nr);
     if (NW > 1) __syncthreads(); else wave_sync();
     const int64_t wo = B.wo;
-    OutK<T, MASKED> k;
+    OutK<T, MASKED, NS> k;
     k.cells = t.cellx + S.cell_off;
     k.ar = ar;
     k.wf.load(B);
@@ -566,20 +569,27 @@ int plan_shape() { return g_shape; }
 template <typename T>
 static size_t tab_bytes() { return sizeof(SrcTab<T>); }
 
+// lds: the kernel's LDS without the HILF window ring; wslot: bytes of one
+// ring slot (win_slots<T, NW>() of them, 0 without a window)
 template <typename T, bool CLASSIC, bool MASKED, bool LAP>
 static void fwd_launch(const Topo &t, const Fields<T> &f, Mask m, int rd,
-                       int write_hi, const int *done, size_t lds, hipStream_t s) {
+                       int write_hi, const int *done, size_t lds, size_t wslot,
+                       hipStream_t s) {
+#define MGP_FWD1(D, M, NW) \
+    k_forward<T, CLASSIC, MASKED, LAP, D, M, NW><<<t.nlist, kWave * NW, \
+        lds + win_slots<T, NW>() * wslot, s>>>(t.items, t, f, m, rd, write_hi, done)
     switch (g_shape) {
-    case 1: k_forward<T, CLASSIC, MASKED, LAP, 4, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
-    case 3: k_forward<T, CLASSIC, MASKED, LAP, 3, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
-    case 4: k_forward<T, CLASSIC, MASKED, LAP, 6, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
-    case 5: k_forward<T, CLASSIC, MASKED, LAP, 8, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
-    case 6: k_forward<T, CLASSIC, MASKED, LAP, 4, 1, 4><<<t.nlist, kWave * 4, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
-    case 7: k_forward<T, CLASSIC, MASKED, LAP, 4, 1, 8><<<t.nlist, kWave * 8, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
-    case 8: k_forward<T, CLASSIC, MASKED, LAP, 3, 2, 2><<<t.nlist, kWave * 2, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
-    case 9: k_forward<T, CLASSIC, MASKED, LAP, 4, 1, 2><<<t.nlist, kWave * 2, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
-    default: k_forward<T, CLASSIC, MASKED, LAP, 2, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, rd, write_hi, done); break;
+    case 1: MGP_FWD1(4, 1, 1); break;
+    case 3: MGP_FWD1(3, 2, 1); break;
+    case 4: MGP_FWD1(6, 2, 1); break;
+    case 5: MGP_FWD1(8, 1, 1); break;
+    case 6: MGP_FWD1(4, 1, 4); break;
+    case 7: MGP_FWD1(4, 1, 8); break;
+    case 8: MGP_FWD1(3, 2, 2); break;
+    case 9: MGP_FWD1(4, 1, 2); break;
+    default: MGP_FWD1(2, 2, 1); break;
     }
+#undef MGP_FWD1
 }
 
 template <typename T>
@@ -593,9 +603,9 @@ void launch_forward(const Topo &t, const Fields<T> &f, Mask m, int rd,
     const bool lap = !hilf_zero;
     const size_t nr = row_len(t);
     const size_t lds = tab_bytes<T>() + 6 * nr * sizeof(T) + 2 * mRS * sizeof(T) +
-                       (lap ? 4 * mHS * sizeof(T) + (masked ? 4 * mHS : 0) : 0) +
                        (lap && !masked ? 3 * nr * sizeof(double) : 0);
-#define MGP_FWD(C, M, L) fwd_launch<T, C, M, L>(t, f, m, rd, write_hi, done, lds, s)
+    const size_t wslot = lap ? mHS * sizeof(T) + (masked ? mHS : 0) : 0;
+#define MGP_FWD(C, M, L) fwd_launch<T, C, M, L>(t, f, m, rd, write_hi, done, lds, wslot, s)
     if (classic) {
         if (masked) { if (lap) MGP_FWD(true, true, true); else MGP_FWD(true, true, false); }
         else { if (lap) MGP_FWD(true, false, true); else MGP_FWD(true, false, false); }
@@ -696,18 +706,22 @@ template void launch_sor<float>(const Topo &, const Fields<float> &, Mask,
 
 template <typename T, bool MASKED>
 static void out_launch(const Topo &t, const Fields<T> &f, Mask m, Outer o,
-                       T *rhs_t, size_t lds, hipStream_t s) {
+                       T *rhs_t, size_t lds, size_t wslot, hipStream_t s) {
+#define MGP_OUT(D, M, NW) \
+    k_outer<T, MASKED, D, M, NW><<<t.nlist, kWave * NW, lds + win_slots<T, NW>() * wslot, s>>>( \
+        t.items, t, f, m, o, rhs_t)
     switch (g_shape) {
-    case 1: k_outer<T, MASKED, 4, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
-    case 3: k_outer<T, MASKED, 3, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
-    case 4: k_outer<T, MASKED, 6, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
-    case 5: k_outer<T, MASKED, 8, 1, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
-    case 6: k_outer<T, MASKED, 4, 1, 4><<<t.nlist, kWave * 4, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
-    case 7: k_outer<T, MASKED, 4, 1, 8><<<t.nlist, kWave * 8, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
-    case 8: k_outer<T, MASKED, 3, 2, 2><<<t.nlist, kWave * 2, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
-    case 9: k_outer<T, MASKED, 4, 1, 2><<<t.nlist, kWave * 2, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
-    default: k_outer<T, MASKED, 2, 2, 1><<<t.nlist, kWave, lds, s>>>(t.items, t, f, m, o, rhs_t); break;
+    case 1: MGP_OUT(4, 1, 1); break;
+    case 3: MGP_OUT(3, 2, 1); break;
+    case 4: MGP_OUT(6, 2, 1); break;
+    case 5: MGP_OUT(8, 1, 1); break;
+    case 6: MGP_OUT(4, 1, 4); break;
+    case 7: MGP_OUT(4, 1, 8); break;
+    case 8: MGP_OUT(3, 2, 2); break;
+    case 9: MGP_OUT(4, 1, 2); break;
+    default: MGP_OUT(2, 2, 1); break;
     }
+#undef MGP_OUT
 }
 
 template <typename T>
@@ -718,9 +732,10 @@ void launch_outer(const Topo &t, const Fields<T> &f, Mask m, Outer o,
     const bool masked = m.bp != nullptr;
     const size_t nr = row_len(t);
     const size_t lds = tab_bytes<T>() + 6 * nr * sizeof(double) +
-                       4 * mHS * sizeof(T) + (masked ? 4 * mHS : 3 * nr * sizeof(double));
-    if (masked) out_launch<T, true>(t, f, m, o, rhs_t, lds, s);
-    else out_launch<T, false>(t, f, m, o, rhs_t, lds, s);
+                       (masked ? 0 : 3 * nr * sizeof(double));
+    const size_t wslot = mHS * sizeof(T) + (masked ? mHS : 0);
+    if (masked) out_launch<T, true>(t, f, m, o, rhs_t, lds, wslot, s);
+    else out_launch<T, false>(t, f, m, o, rhs_t, lds, wslot, s);
 }
 template void launch_outer<double>(const Topo &, const Fields<double> &, Mask,
                                    Outer, double *, hipStream_t);
