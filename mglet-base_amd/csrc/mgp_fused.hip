@@ -177,7 +177,7 @@ __global__ __launch_bounds__(kWave * NW) void k_outer_fwd(
     k.masked = MASKED && !(B.flags & 1);
     k.mx = 0.0;
     constexpr int nw = plan_nw<NW>();
-    const int ib = S.plan_item[nw][M - 1][kPlanFwdLap] + threadIdx.x / kWave;
+    const int ib = S.plan_item[nw][M - 1][kPlanFwdLap] + wave_index();
     run_plan<D, M, NW, MASKED, T>(k, items + ib, S.plan_nitem[nw][M - 1][kPlanFwdLap],
                                   t.ents + S.plan_ent[kPlanFwdLap], tab,
                                   t.fents + (int64_t)ib * kWave);

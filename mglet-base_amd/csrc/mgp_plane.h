@@ -42,6 +42,13 @@ __device__ __forceinline__ T gld(const T *p) {
     return *(const __attribute__((address_space(1))) T *)p;
 }
 
+// The wave's index in its workgroup, as a uniform value: the plan item
+// table offset by it is a uniform address, so the item descriptors load
+// through the scalar cache instead of a vector register ring.
+__device__ __forceinline__ int wave_index() {
+    return (int)(__builtin_amdgcn_readfirstlane(threadIdx.x) / kWave);
+}
+
 // Index of a per-block wave count in the plan tables (kPlanWaves).
 template <int NW>
 constexpr int plan_nw() {

@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kWave * NW) void k_forward(
     k.masked = MASKED && !(B.flags & 1);
     const int kind = LAP ? kPlanFwdLap : kPlanFwdZero;
     constexpr int nw = plan_nw<NW>();
-    const int ib = S.plan_item[nw][M - 1][kind] + threadIdx.x / kWave;
+    const int ib = S.plan_item[nw][M - 1][kind] + wave_index();
     run_plan<D, M, NW, MASKED && LAP, T>(k, items + ib, S.plan_nitem[nw][M - 1][kind],
                                          t.ents + S.plan_ent[kind], tab,
                                          t.fents + (int64_t)ib * kWave);
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kWave * NW) void k_backward(
     k.RW = NJ + 1;
     k.RS = (NJ + 1) * (NI + 1);
     constexpr int nw = plan_nw<NW>();
-    const int ib = S.plan_item[nw][M - 1][kPlanBwd] + threadIdx.x / kWave;
+    const int ib = S.plan_item[nw][M - 1][kPlanBwd] + wave_index();
     run_plan<D, M, NW, false, T>(k, items + ib, S.plan_nitem[nw][M - 1][kPlanBwd],
                                  t.ents + S.plan_ent[kPlanBwd], tab,
                                  t.fents + (int64_t)ib * kWave);
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(kWave * NW) void k_outer(
     k.masked = MASKED && !(B.flags & 1);
     k.mx = 0.0;
     constexpr int nw = plan_nw<NW>();
-    run_plan<D, M, NW, MASKED, T>(k, items + S.plan_item[nw][M - 1][kPlanOut] + threadIdx.x / kWave,
+    run_plan<D, M, NW, MASKED, T>(k, items + S.plan_item[nw][M - 1][kPlanOut] + wave_index(),
                                   S.plan_nitem[nw][M - 1][kPlanOut], t.ents + S.plan_ent[kPlanOut],
                                   tab);
     double mx = k.mx * ldexp(1.0, t.maxlevel - B.level);
