@@ -29,6 +29,7 @@ namespace mgp {
 template <typename T, bool CLASSIC, bool MASKED, int NS>
 struct OutFwdK {
     static constexpr bool kFaces = true;
+    static constexpr bool kSplit = false;
     struct Data {
         uint32_t c;
         double ap, rhs, dp;

@@ -101,6 +101,32 @@ struct SrcTab {
 __device__ __forceinline__ double fms(double r, double a, double b) { return r - a * b; }
 __device__ __forceinline__ float fms(float r, float a, float b) { return __builtin_fmaf(-a, b, r); }
 
+// The 7-point stencil of HILF (and BP) at one cell, read from the window
+// planes p-1, p, p+1, then res = -(A phi) from it, src/flow/
+// laplacephi_mod.F90:149-162 (laplace_cell below in two steps, so a kernel
+// can issue the reads of several cells before it waits for any).
+template <typename T, typename A, bool MASKED>
+struct Stencil {
+    A pw, pe, ps, pn, pb, pt, pc;
+    A bw, be, bs, bn, bb, bt, bc;
+    bool bp;   // with BP (a block with blocked cells)
+    __device__ __forceinline__ void read(const T *Hm, const T *H0, const T *Hp,
+                                         const uint8_t *Bm, const uint8_t *B0,
+                                         const uint8_t *Bp, int c0, int HW, bool withbp) {
+        pw = (A)Hm[c0 - HW]; pe = (A)Hp[c0 + HW];
+        ps = (A)Hm[c0 - 1]; pn = (A)Hp[c0 + 1];
+        pb = (A)Hm[c0]; pt = (A)Hp[c0]; pc = (A)H0[c0];
+        bp = MASKED && withbp;
+        if (bp) {
+            bc = (A)B0[c0];
+            bw = (A)Bm[c0 - HW]; be = (A)Bp[c0 + HW];
+            bs = (A)Bm[c0 - 1]; bn = (A)Bp[c0 + 1];
+            bb = (A)Bm[c0]; bt = (A)Bp[c0];
+        }
+    }
+    __device__ __forceinline__ A eval(A aw, A ae, A as, A an, A ab, A at, A ap) const;
+};
+
 // res = -(A phi) at one cell, src/flow/laplacephi_mod.F90:149-162, with
 // phi and BP read from the window planes p-1, p, p+1.
 template <typename T, typename A, bool MASKED>
@@ -129,6 +155,29 @@ __device__ __forceinline__ A laplace_cell(const T *Hm, const T *H0,
         r = -(aw * pw);
         r = fms(r, ae, pe);
         r = fms(r, as, psv);
+        r = fms(r, an, pn);
+        r = fms(r, ab, pb);
+        r = fms(r, at, pt);
+    }
+    r = fms(r, ap, pc);
+    return r;
+}
+
+template <typename T, typename A, bool MASKED>
+__device__ __forceinline__ A Stencil<T, A, MASKED>::eval(A aw, A ae, A as, A an, A ab, A at,
+                                                         A ap) const {
+    A r;
+    if (MASKED && bp) {
+        r = -(aw * pw * bw * bc);
+        r = r - ae * pe * bc * be;
+        r = r - as * ps * bs * bc;
+        r = r - an * pn * bc * bn;
+        r = r - ab * pb * bb * bc;
+        r = r - at * pt * bc * bt;
+    } else {
+        r = -(aw * pw);
+        r = fms(r, ae, pe);
+        r = fms(r, as, ps);
         r = fms(r, an, pn);
         r = fms(r, ab, pb);
         r = fms(r, at, pt);
@@ -386,9 +435,26 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
             const uint32_t x = ix[u], y = iy[u];
             const int cpos = x & 0xffff, cc = (x >> 16) & 0xff, ec = x >> 24;
             const int p = (int)(y >> 21) - 8;
+            bool split = false;
+            if constexpr (K::kSplit && M > 1) {
+                // every row holds cells: all LDS reads of the item first,
+                // then the arithmetic (stores of lanes past the chunk off)
+                split = cc > 64 * (M - 1);
+                if (split) {
+                    typename K::G g[M];
 #pragma unroll
-            for (int m = 0; m < M; ++m)
-                if (lane + 64 * m < cc) k.compute(cd[u][m], cpos + lane + 64 * m, p, lane + 64 * m);
+                    for (int m = 0; m < M; ++m) g[m] = k.gather(cd[u][m], p);
+#pragma unroll
+                    for (int m = 0; m < M; ++m)
+                        k.finish(cd[u][m], g[m], cpos + lane + 64 * m, p, lane + 64 * m,
+                                 lane + 64 * m < cc);
+                }
+            }
+            if (!split) {
+#pragma unroll
+                for (int m = 0; m < M; ++m)
+                    if (lane + 64 * m < cc) k.compute(cd[u][m], cpos + lane + 64 * m, p, lane + 64 * m);
+            }
 #pragma unroll
             for (int m = 0; m < M; ++m)
                 if (lane + 64 * m < ec)

@@ -49,9 +49,18 @@ struct FwdK {
     // factors are shared (saves an HBM load)
     static constexpr bool APC = LAP && !MASKED;
     static constexpr bool kFaces = true;
+    static constexpr bool kSplit = true;
     struct Data {
         uint32_t c;
         T rhs, lpr, lb, ls, lw, ap;
+    };
+    // LDS values of one cell (gather), then its arithmetic and stores
+    // (finish), so the engine can issue two cells' reads before waiting
+    struct G {
+        Stencil<T, T, MASKED> st;
+        T w, e, s, n, b, t, ap;   // 1-D coefficients, GSAP
+        T rk, rj, ri;             // recurrence neighbours k-1, j-1, i-1
+        int c1;
     };
     const uint32_t *cells;
     const T *rhs, *lpr, *lb, *ls, *lw, *ap;
@@ -75,41 +84,49 @@ struct FwdK {
         d.lw = ld(lw, s);
         if (LAP) d.ap = ld(ap, s);
     }
-    __device__ __forceinline__ void compute(const Data &d, int s, int p, int) const {
+    __device__ __forceinline__ G gather(const Data &d, int p) const {
+        G g;
         int i, j, k;
         cell_ijk(d.c, i, j, k);
-        T r = T(0);
         if (LAP) {
             // LDS positions stay below 2^24: full-rate 24-bit multiplies
             const int c0 = (j + 1) + __mul24(HW, i + 1);
             const int sm = win_slot<NS>(p - 1) * mHS, s0 = win_slot<NS>(p) * mHS,
                       sp = win_slot<NS>(p + 1) * mHS;
             const Coef6<T> co(cf, nr, i, j, k);
-            const T apv = (APC && !apsh) ? (T)ap_cell(ar, nr, i, j, k, cell_flags(d.c), wf) : d.ap;
-            if (MASKED && masked)
-                r = laplace_cell<T, T, true>(H + sm, H + s0, H + sp, BH + sm, BH + s0,
-                                             BH + sp, c0, HW, co.w, co.e, co.s, co.n, co.b, co.t,
-                                             apv);
-            else
-                r = laplace_cell<T, T, false>(H + sm, H + s0, H + sp, nullptr, nullptr,
-                                              nullptr, c0, HW, co.w, co.e, co.s, co.n, co.b, co.t,
-                                              apv);
+            g.w = co.w; g.e = co.e; g.s = co.s; g.n = co.n; g.b = co.b; g.t = co.t;
+            g.ap = (APC && !apsh) ? (T)ap_cell(ar, nr, i, j, k, cell_flags(d.c), wf) : d.ap;
+            g.st.read(H + sm, H + s0, H + sp, BH + sm, BH + s0, BH + sp, c0, HW,
+                      MASKED && masked);
         }
-        const int c1 = (j + 1) + __mul24(RW, i + 1);
+        g.c1 = (j + 1) + __mul24(RW, i + 1);
         const T *Rp = R + ((p - 1) & 1) * RS;
-        T *Rc = R + (p & 1) * RS;
+        g.rk = Rp[g.c1];
+        g.rj = Rp[g.c1 - 1];
+        g.ri = Rp[g.c1 - RW];
+        return g;
+    }
+    __device__ __forceinline__ void finish(const Data &d, const G &g, int s, int p, int,
+                                           bool ok) const {
+        T r = T(0);
+        if (LAP) r = g.st.eval(g.w, g.e, g.s, g.n, g.b, g.t, g.ap);
         r = (d.rhs + r) * d.lpr;
         if (CLASSIC) {
-            r = fms(r, d.lw, Rp[c1 - RW]);
-            r = fms(r, d.ls, Rp[c1 - 1]);
-            r = fms(r, d.lb, Rp[c1]);
+            r = fms(r, d.lw, g.ri);
+            r = fms(r, d.ls, g.rj);
+            r = fms(r, d.lb, g.rk);
         } else {
-            r = fms(r, d.lb, Rp[c1]);
-            r = fms(r, d.ls, Rp[c1 - 1]);
-            r = fms(r, d.lw, Rp[c1 - RW]);
+            r = fms(r, d.lb, g.rk);
+            r = fms(r, d.ls, g.rj);
+            r = fms(r, d.lw, g.ri);
         }
-        Rc[c1] = r;
-        st(res, s, r);
+        if (ok) {
+            R[(p & 1) * RS + g.c1] = r;
+            st(res, s, r);
+        }
+    }
+    __device__ __forceinline__ void compute(const Data &d, int s, int p, int l) const {
+        finish(d, gather(d, p), s, p, l, true);
     }
     __device__ __forceinline__ void store(uint32_t e, T v, uint8_t b, int p) const {
         const int x = e & 0xfff;
@@ -206,9 +223,14 @@ __global__ __launch_bounds__(kWave * NW) void k_forward(
 template <typename T, bool CLASSIC, bool HZ>
 struct BwdK {
     static constexpr bool kFaces = true;
+    static constexpr bool kSplit = true;
     struct Data {
         uint32_t c;
         T res, ue, un, ut, h;
+    };
+    struct G {
+        T rt, rn, re;   // recurrence neighbours k+1, j+1, i+1
+        int c1;
     };
     const uint32_t *cells;
     const T *res, *ue, *un, *ut;
@@ -224,26 +246,38 @@ struct BwdK {
         d.ut = ld(ut, s);
         if (!HZ) d.h = ld((const T *)hilf, s);
     }
-    __device__ __forceinline__ void compute(const Data &d, int s, int p, int l) const {
+    __device__ __forceinline__ G gather(const Data &d, int p) const {
+        G g;
         int i, j, k;
         cell_ijk(d.c, i, j, k);
-        const int c1 = j + __mul24(RW, i);
+        g.c1 = j + __mul24(RW, i);
         const T *Rp = R + ((p + 1) & 1) * RS;
-        T *Rc = R + (p & 1) * RS;
+        g.rt = Rp[g.c1];
+        g.rn = Rp[g.c1 + 1];
+        g.re = Rp[g.c1 + RW];
+        return g;
+    }
+    __device__ __forceinline__ void finish(const Data &d, const G &g, int s, int p, int l,
+                                           bool ok) const {
         T r = d.res;
         if (CLASSIC) {
-            r = fms(r, d.un, Rp[c1 + 1]);
-            r = fms(r, d.ue, Rp[c1 + RW]);
-            r = fms(r, d.ut, Rp[c1]);
+            r = fms(r, d.un, g.rn);
+            r = fms(r, d.ue, g.re);
+            r = fms(r, d.ut, g.rt);
         } else {
-            r = fms(r, d.ut, Rp[c1]);
-            r = fms(r, d.un, Rp[c1 + 1]);
-            r = fms(r, d.ue, Rp[c1 + RW]);
+            r = fms(r, d.ut, g.rt);
+            r = fms(r, d.un, g.rn);
+            r = fms(r, d.ue, g.re);
         }
-        Rc[c1] = r;
         const T h = (HZ ? T(0) : d.h) + r;
-        st(hilf, s, h);
-        stg[l] = h;
+        if (ok) {
+            R[(p & 1) * RS + g.c1] = r;
+            st(hilf, s, h);
+            stg[l] = h;
+        }
+    }
+    __device__ __forceinline__ void compute(const Data &d, int s, int p, int l) const {
+        finish(d, gather(d, p), s, p, l, true);
     }
     __device__ __forceinline__ void store(uint32_t e, T v, uint8_t, int p) const {
         R[(p & 1) * RS + (e & 0xfff)] = v;
@@ -309,6 +343,7 @@ __global__ __launch_bounds__(kWave * NW) void k_backward(
 template <typename T, bool MASKED, bool HZ>
 struct SorK {
     static constexpr bool kFaces = true;
+    static constexpr bool kSplit = false;
     struct Data {
         uint32_t c;
         T rhs, rap;
@@ -444,6 +479,7 @@ __global__ __launch_bounds__(kWave) void k_sor(
 template <typename T, bool MASKED, int NS>
 struct OutK {
     static constexpr bool kFaces = false;
+    static constexpr bool kSplit = false;
     struct Data {
         uint32_t c;
         double ap, rhs, dp;
