@@ -226,8 +226,17 @@ struct mgp_handle {
         DBuf blocks, ents;             // PullBlock records of all stages, entry tables
         std::vector<int> stage_start;  // per stage offsets into blocks (+ end)
         bool pairs = false;            // entries move two values along k
+        bool valid = true;             // false: a kept copy touches a pushed region
     };
-    std::map<std::tuple<int, int, int, int>, std::unique_ptr<TaskSet>> tasks;
+    std::map<std::tuple<int, int, int, int, int>, std::unique_ptr<TaskSet>> tasks;
+    // mgpoisl's velocity and pressure correction along z-chains, which
+    // writes the k-face ghosts of interior rows itself (single-level grids):
+    // 0 not decided, 1 on, -1 off; the chains (CorrChains) and per grid the
+    // faces whose interior-row ghosts it writes (bit 0 k-low, bit 1 k-high)
+    int kpush_state = 0;
+    DBuf d_cchain, d_cchains;
+    int ncchain = 0, ccnx = 0;
+    std::vector<int> kpush_faces;
     // captured outer-iteration graphs per level
     hipGraphExec_t graph = nullptr;
     int graph_key[4] = {-1, -1, -1, -1};
@@ -502,12 +511,46 @@ bool overlap(const Box &a, const Box &b) {
 template <typename T>
 void upload(DBuf &d, const std::vector<T> &v);
 
+// The grids of a set (1-based ids, ascending) in nested neighbour chains:
+// along +x (face 2) outermost, then +y (face 4), then +z (face 6)
+// innermost, so that consecutive entries are z-neighbours and
+// neighbouring columns y-neighbours.  Every grid starts at most one walk
+// per direction: O(ngrid).
+std::vector<int> chain_order(const mgp_handle &h, const std::vector<int> &ids) {
+    std::vector<char> in(h.ngrid + 1, 0), zdone(h.ngrid + 1, 0), ydone(h.ngrid + 1, 0),
+        xdone(h.ngrid + 1, 0);
+    for (int g : ids) in[g] = 1;
+    std::vector<int> order;
+    auto next = [&](int g, int face) { return h.nbr[26 * (g - 1) + face - 1]; };
+    auto zwalk = [&](int b) {
+        for (int g = b; g > 0 && !zdone[g]; g = next(g, 6)) {
+            zdone[g] = 1;
+            if (in[g]) order.push_back(g);
+        }
+    };
+    auto ywalk = [&](int a) {
+        for (int g = a; g > 0 && !ydone[g]; g = next(g, 4)) {
+            ydone[g] = 1;
+            zwalk(g);
+        }
+    };
+    for (int g0 : ids)
+        for (int g = g0; g > 0 && !xdone[g]; g = next(g, 2)) {
+            xdone[g] = 1;
+            ywalk(g);
+        }
+    if (order.size() != ids.size()) throw HipError{"block chain order incomplete"};
+    return order;
+}
+
 // Copy tasks of one exchange, grouped into stages that keep the
 // reference's sequential order wherever two copies touch the same cells
 // (only possible when a source box reaches into a ghost layer).
+// kpush: the k-face copies into the interior-row ghosts that the pushing
+// velocity and pressure correction fills (h.kpush_faces) are left out.
 mgp_handle::TaskSet &task_set(mgp_handle &h, int level, int layers,
-                              int corners, int forward) {
-    auto key = std::make_tuple(level, layers, corners, forward);
+                              int corners, int forward, bool kpush = false) {
+    auto key = std::make_tuple(level, layers, corners, forward, (int)kpush);
     auto it = h.tasks.find(key);
     if (it != h.tasks.end()) return *it->second;
     const int minl = level >= 0 ? level : h.minlevel;
@@ -518,6 +561,7 @@ mgp_handle::TaskSet &task_set(mgp_handle &h, int level, int layers,
         Box sb, db;
     };
     std::vector<T2> list;
+    std::vector<std::vector<Box>> pushed(kpush ? h.ngrid + 1 : 0);   // per grid, left out
     for (const Conn &c : h.conns) {
         const int fr = c.frecv;
         const int lvl = h.level[c.recv - 1];
@@ -525,6 +569,11 @@ mgp_handle::TaskSet &task_set(mgp_handle &h, int level, int layers,
         if (!corners && fr > 6) continue;
         if (forward == 1 && (fr == 2 || fr == 4 || fr == 6)) continue;
         if (forward == -1 && (fr == 1 || fr == 3 || fr == 5)) continue;
+        if (kpush && (fr == 5 || fr == 6) &&
+            (h.kpush_faces[c.recv - 1] >> (fr - 5) & 1)) {
+            pushed[c.recv].push_back(ss(h, c.recv, fr, layers, false));
+            continue;
+        }
         Box sb = ss(h, c.send, FACENBR[fr - 1], layers, true);
         if (fr > 6)
             for (int a = 1; a <= FACELIST[c.fsend - 1][0]; ++a)
@@ -550,6 +599,14 @@ mgp_handle::TaskSet &task_set(mgp_handle &h, int level, int layers,
         t.d_sj = (int)kkd; t.d_si = (int)(kkd * jjd);
         list.push_back({t, c.send, c.recv, sb, db});
     }
+    // a kept copy that reads or writes a pushed ghost region would see it
+    // filled out of the reference's order
+    bool valid = true;
+    if (kpush)
+        for (const T2 &q : list) {
+            for (const Box &b : pushed[q.sg]) valid = valid && !overlap(q.sb, b);
+            for (const Box &b : pushed[q.dg]) valid = valid && !overlap(q.db, b);
+        }
     // staging
     std::vector<int> stage(list.size(), 0);
     std::vector<std::vector<int>> wr(h.ngrid + 1), rdl(h.ngrid + 1);
@@ -572,6 +629,7 @@ mgp_handle::TaskSet &task_set(mgp_handle &h, int level, int layers,
     // copies of a stage write the same position).  Pairs of values along k
     // when every box has an even k extent and even offsets and strides.
     auto ts = std::make_unique<mgp_handle::TaskSet>();
+    ts->valid = valid;
     bool pairs = true;
     for (const T2 &q : list) {
         const CopyTask &t = q.t;
@@ -588,38 +646,15 @@ mgp_handle::TaskSet &task_set(mgp_handle &h, int level, int layers,
         std::map<int, std::vector<int>> byrecv;
         for (size_t q = 0; q < list.size(); ++q)
             if (stage[q] == st) byrecv[list[q].dg].push_back((int)q);
-        // receiving blocks in the order of nested neighbour chains: along
-        // +x (face 2) outermost, then +y (face 4), then +z (face 6)
-        // innermost.  Consecutive workgroups then work on z-neighbours
-        // (the thin z-ghost rows) and neighbouring columns on y-neighbours,
-        // so that partial-line reads and writes of the same lines meet in
-        // the memory-side cache: 1.5x faster than block-id order on a
-        // 32^3 lattice numbered x fastest.
-        // Every block starts at most one walk per direction: O(ngrid).
-        std::vector<int> order;
-        {
-            std::vector<char> zdone(h.ngrid + 1, 0), ydone(h.ngrid + 1, 0),
-                xdone(h.ngrid + 1, 0);
-            auto next = [&](int g, int face) { return h.nbr[26 * (g - 1) + face - 1]; };
-            auto zwalk = [&](int b) {
-                for (int g = b; g > 0 && !zdone[g]; g = next(g, 6)) {
-                    zdone[g] = 1;
-                    if (byrecv.count(g)) order.push_back(g);
-                }
-            };
-            auto ywalk = [&](int a) {
-                for (int g = a; g > 0 && !ydone[g]; g = next(g, 4)) {
-                    ydone[g] = 1;
-                    zwalk(g);
-                }
-            };
-            for (auto &kv : byrecv)
-                for (int g = kv.first; g > 0 && !xdone[g]; g = next(g, 2)) {
-                    xdone[g] = 1;
-                    ywalk(g);
-                }
-        }
-        if (order.size() != byrecv.size()) throw HipError{"exchange block order incomplete"};
+        // receiving blocks in nested neighbour chains (z innermost):
+        // consecutive workgroups then work on z-neighbours (the thin
+        // z-ghost rows) and neighbouring columns on y-neighbours, so that
+        // partial-line reads and writes of the same lines meet in the
+        // memory-side cache: 1.5x faster than block-id order on a 32^3
+        // lattice numbered x fastest.
+        std::vector<int> recvs;
+        for (auto &kv : byrecv) recvs.push_back(kv.first);
+        const std::vector<int> order = chain_order(h, recvs);
         for (int recv : order) {
             const std::vector<int> &qs = byrecv.at(recv);
             if (qs.size() > 26) throw HipError{"more than 26 ghost copies into one block"};
@@ -667,8 +702,8 @@ mgp_handle::TaskSet &task_set(mgp_handle &h, int level, int layers,
 }
 
 void exchange(mgp_handle &h, int level, int layers, int corners, int forward,
-              double *const *fields, int nfield) {
-    auto &ts = task_set(h, level, layers, corners, forward);
+              double *const *fields, int nfield, bool kpush = false) {
+    auto &ts = task_set(h, level, layers, corners, forward, kpush);
     for (int f0 = 0; f0 < nfield; f0 += 6) {
         const int nf = std::min(6, nfield - f0);
         int mode = 0;
@@ -683,6 +718,71 @@ void exchange(mgp_handle &h, int level, int layers, int corners, int forward,
                         fields + f0, nf, h.stream);
         }
     }
+}
+
+// Decided once per handle: whether mgpoisl's velocity and pressure
+// correction runs along z-chains and writes the k-face ghosts of interior
+// rows for the closing exchange (k_mgpcorr_chain).  It does on single-level
+// grids with no PAR face and no face bound_flow acts on (so nothing runs
+// between the correction and the exchange), when both k-faces of every
+// block are connected to blocks of its size with nk >= 2, the k-high
+// neighbour of each block takes its k-low ghosts from that block (so the
+// connections form cycles along z), and no copy left in the exchange reads
+// or writes a region the correction fills.  MGP_KPUSH=0 in the environment
+// keeps the plain path.
+bool kpush_on(mgp_handle &h) {
+    if (h.kpush_state) return h.kpush_state > 0;
+    h.kpush_state = -1;
+    const char *env = getenv("MGP_KPUSH");
+    if (env && atoi(env) == 0) return false;
+    if (h.level_blocks.size() != 1 || !h.level_par.empty() || !h.level_flow.empty())
+        return false;
+    // up[g]: the block whose k = 2, 3 fill g's k-high ghosts; down[g]: the
+    // block whose k = nk, nk + 1 fill g's k-low ghosts (0-based, -1 none)
+    std::vector<int> up(h.ngrid, -1), down(h.ngrid, -1);
+    for (const Conn &c : h.conns) {
+        if (c.frecv != 5 && c.frecv != 6) continue;
+        const int d = c.recv - 1, sg = c.send - 1;
+        if (h.ii[d] != h.ii[sg] || h.jj[d] != h.jj[sg] || h.kk[d] != h.kk[sg] || h.kk[d] < 6 ||
+            h.kk[d] > kCorrThreads)
+            return false;
+        int &nb = c.frecv == 6 ? up[d] : down[d];
+        if (nb >= 0) return false;
+        nb = sg;
+    }
+    for (int g = 0; g < h.ngrid; ++g)
+        if (up[g] < 0 || down[g] < 0 || down[up[g]] != g) return false;
+    h.kpush_faces.assign(h.ngrid, 3);
+    if (!task_set(h, h.level_blocks.begin()->first, 2, 1, 0, true).valid) {
+        h.kpush_faces.clear();
+        return false;
+    }
+    // cycles along +z, started from the blocks in nested chain order
+    std::vector<int> ids(h.ngrid);
+    for (int g = 0; g < h.ngrid; ++g) ids[g] = g + 1;
+    std::vector<char> seen(h.ngrid, 0);
+    std::vector<int> chain;
+    std::vector<int2> chains;
+    int nx = 0;
+    for (int g1 : chain_order(h, ids)) {
+        if (seen[g1 - 1]) continue;
+        const int start = (int)chain.size();
+        for (int g = g1 - 1; !seen[g]; g = up[g]) {
+            seen[g] = 1;
+            chain.push_back(g);
+        }
+        if ((int)chain.size() - start > kCorrChainMax) return false;
+        chains.push_back(make_int2(start, (int)chain.size() - start));
+        const int nrow = (h.ii[g1 - 1] - 2) * (h.jj[g1 - 1] - 2);
+        const int rows = kCorrThreads / h.kk[g1 - 1];
+        nx = std::max(nx, (nrow + rows - 1) / rows);
+    }
+    upload(h.d_cchain, chain);
+    upload(h.d_cchains, chains);
+    h.ncchain = (int)chains.size();
+    h.ccnx = nx;
+    h.kpush_state = 1;
+    return true;
 }
 
 // ------------------------------------------------------------------
@@ -1841,9 +1941,13 @@ int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
         launch_bound_sp(h.topo(L), h.d_dpsp.as<double>(), h.d_pbuf64.as<double>(),
                         c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr, c.ibtype == 1, mt, s);
     }
+    // on single-level grids the correction can write the k-face ghosts of
+    // the closing exchange itself (kpush_on)
+    const bool kpush = !c.sync_by_caller && kpush_on(h);
+    const CorrChains cc{h.d_cchain.as<int>(), h.d_cchains.as<int2>(), h.ncchain, h.ccnx};
     launch_mgpcorr(tall, u, v, w, p, h.d_dpsp.as<double>(), h.d_rdx.as<double>(),
                    h.d_rdy.as<double>(), h.d_rdz.as<double>(), dt / c.rho,
-                   c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr, s);
+                   c.ibtype == 1 ? h.d_bpsp.as<double>() : nullptr, kpush ? &cc : nullptr, s);
     for (auto lv = h.level_blocks.rbegin(); lv != h.level_blocks.rend(); ++lv)
         if (lv->first != h.level_blocks.begin()->first)
             launch_ftoc_uvwp(h.topo(lv->first), u, v, w, p, c.ibtype == 1,
@@ -1865,7 +1969,7 @@ int solve(mgp_handle &h, double *u, double *v, double *w, double *p,
                                   h.d_wbnd.as<double>(), h.d_bbo.as<int64_t>(),
                                   h.d_pinf.as<double>(), h.d_bpsp.as<double>(), dd[0], dd[1],
                                   dd[2], c.rho, s);
-            exchange(h, l, 2, 1, 0, fs, 4);
+            exchange(h, l, 2, 1, 0, fs, 4, kpush);
         }
     }
     HC(hipGetLastError());

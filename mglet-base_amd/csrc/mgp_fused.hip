@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kWave * NW) void k_outer_fwd(
     coef_rows<double>(cf, o.aw, o.ae, o.as, o.an, o.ab, o.at, B, nr);
     if (!MASKED) ap_rows(ar, F.apx, F.apy, F.apz, B, nr);
     if (NW > 1) __syncthreads(); else wave_sync();
-    const int NI = B.ni, NJ = B.nj, NK = B.nk;
+    const int NI = B.ni, NJ = B.nj;
     const int64_t wo = B.wo, fc = B.fc;
     OutFwdK<T, CLASSIC, MASKED, NS> k;
     k.cells = t.cellx + S.cell_off;

@@ -679,11 +679,55 @@ __device__ __forceinline__ int div_small(int e, int d, float rd) {
     return q;
 }
 
-__global__ __launch_bounds__(kThreads) void k_mgpcorr(Topo t, double *u_, double *v_,
-                                                     double *w_, double *p_, const double *dp_,
-                                                     const double *rdx_, const double *rdy_,
-                                                     const double *rdz_, double rfak,
-                                                     const double *bp_) {
+// The update of one position (i, j, k 1-based, c its offset in the block)
+// of a block (sp, ox, oy, oz: its offsets in the 3-D and 1-D fields):
+// r = P, U, V, W, each the stored value where that field is not updated
+// along k; rows outside a field's row range are not stored (corr_store).
+// Every operand is loaded before any is used, whatever the position, so a
+// wave waits for memory once: all of them lie inside the block (rows i + 1
+// and j + 1 exist for i <= ii - 1, j <= jj - 1; the k + 1 neighbour is
+// clamped at the end of the row, where W is not updated).
+struct CorrArgs {
+    double *u, *v, *w, *p;
+    const double *dp, *rdx, *rdy, *rdz, *bp;
+    double rfak;
+};
+
+__device__ __forceinline__ void corr_cell(const CorrArgs &a, int64_t sp, int ox, int oy,
+                                          int oz, int i, int j, int k, int64_t c, int ii,
+                                          int jj, int kk, double r[4]) {
+    const int64_t sj = kk, si = (int64_t)kk * jj;
+    const int64_t ck = c + (k < kk ? 1 : 0);
+    const double *dp = a.dp + sp;
+    const double pv = a.p[sp + c], uv = a.u[sp + c], vv = a.v[sp + c], wv = a.w[sp + c];
+    const double d0 = dp[c], di = dp[c + si], dj = dp[c + sj], dk = dp[ck];
+    const double rdx = a.rdx[ox + i - 1], rdy = a.rdy[oy + j - 1], rdz = a.rdz[oz + k - 1];
+    const double rfak = a.rfak;
+    const bool kin = k >= 3 && k <= kk - 2;
+    if (a.bp) {
+        const double *bp = a.bp + sp;
+        const double b0 = bp[c], bi = bp[c + si], bj = bp[c + sj], bk = bp[ck];
+        r[0] = k >= 2 && k <= kk - 1 ? pv + d0 * b0 : pv;
+        r[1] = kin ? uv + (d0 - di) * b0 * bi * rdx * rfak : uv;
+        r[2] = kin ? vv + (d0 - dj) * b0 * bj * rdy * rfak : vv;
+        r[3] = k >= 2 && k <= kk - 2 ? wv + (d0 - dk) * b0 * bk * rdz * rfak : wv;
+    } else {
+        r[0] = k >= 2 && k <= kk - 1 ? pv + d0 : pv;
+        r[1] = kin ? uv + (d0 - di) * rdx * rfak : uv;
+        r[2] = kin ? vv + (d0 - dj) * rdy * rfak : vv;
+        r[3] = k >= 2 && k <= kk - 2 ? wv + (d0 - dk) * rdz * rfak : wv;
+    }
+}
+
+__device__ __forceinline__ void corr_store(const CorrArgs &a, int64_t sp, int i, int j,
+                                           int64_t c, int ii, int jj, const double r[4]) {
+    a.p[sp + c] = r[0];
+    if (i <= ii - 2 && j >= 3 && j <= jj - 2) a.u[sp + c] = r[1];      // U rows
+    if (i >= 3 && i <= ii - 2 && j <= jj - 2) a.v[sp + c] = r[2];      // V rows
+    if (i >= 3 && i <= ii - 2 && j >= 3 && j <= jj - 2) a.w[sp + c] = r[3];   // W rows
+}
+
+__global__ __launch_bounds__(kThreads) void k_mgpcorr(Topo t, CorrArgs a) {
     const int b = t.blist[blockIdx.y];
     const Block &B = t.blocks[b];
     const int kk = B.nk + 4, jj = B.nj + 4, ii = B.ni + 4;
@@ -694,58 +738,102 @@ __global__ __launch_bounds__(kThreads) void k_mgpcorr(Topo t, double *u_, double
     const int k = e - row * kk + 1;
     const int ri = div_small(row, nrj, 1.0f / (float)nrj);
     const int i = ri + 2, j = row - ri * nrj + 2;
-    const int64_t sj = kk, si = (int64_t)kk * jj;
-    const int64_t c = (int64_t)(k - 1) + (int64_t)(j - 1) * sj + (int64_t)(i - 1) * si;
-    double *u = u_ + B.sp, *v = v_ + B.sp, *w = w_ + B.sp, *p = p_ + B.sp;
-    const double *dp = dp_ + B.sp;
-    const double *bp = bp_ ? bp_ + B.sp : nullptr;
-    const bool kin = k >= 3 && k <= kk - 2;
-    {   // P: i, j, k in 2..n-1
-        const double pv = p[c];
-        double r = pv;
-        if (k >= 2 && k <= kk - 1) r = bp ? pv + dp[c] * bp[c] : pv + dp[c];
-        p[c] = r;
+    const int64_t c = (int64_t)(k - 1) + (int64_t)(j - 1) * kk + (int64_t)(i - 1) * kk * jj;
+    double r[4];
+    corr_cell(a, B.sp, B.o1x, B.o1y, B.o1z, i, j, k, c, ii, jj, kk, r);
+    corr_store(a, B.sp, i, j, c, ii, jj, r);
+}
+
+// The update along z-chains of blocks (closing exchange of single-level
+// grids whose k-faces join blocks of one size in cycles): workgroup
+// (x, chain) takes the same whole rows of every block of its chain in
+// order, and writes the k-face ghosts of interior rows (k = 0, 1 and
+// nk + 2, nk + 3, 0-based) as part of those rows, with the values the
+// exchange would copy there: the k-high ghosts are the next block's update
+// at k = 2, 3, evaluated from its cells before the workgroup reaches it
+// (the same expressions, so the same bits), the k-low ghosts the previous
+// block's results at nk, nk + 1, carried in LDS (the chain's first block
+// evaluates its last block's, and the last block takes the first block's
+// results at 2, 3 kept from the start).  Every store is part of a whole
+// row, and the exchange that follows leaves these cells out.
+static_assert(kThreads == kCorrThreads, "workgroup of the correction kernels");
+constexpr int kCorrRows = kThreads / 6;   // rows per workgroup for kk >= 6
+
+__global__ __launch_bounds__(kThreads) void k_mgpcorr_chain(Topo t, const int *chain,
+                                                           const int2 *chains, CorrArgs a) {
+    __shared__ double carry[2][4][2 * kCorrRows];   // previous block at nk, nk + 1
+    __shared__ double first[4][2 * kCorrRows];      // first block at 2, 3
+    __shared__ int64_t rsp[kCorrChainMax];          // the chain's block offsets
+    __shared__ int rox[kCorrChainMax], roy[kCorrChainMax], roz[kCorrChainMax];
+    const int2 ch = chains[blockIdx.y];
+    const Block &B0 = t.blocks[chain[ch.x]];
+    const int nk = B0.nk, kk = nk + 4, jj = B0.nj + 4, ii = B0.ni + 4;
+    const int nrj = jj - 2, nrow = (ii - 2) * nrj;
+    const int R = kThreads / kk;
+    if (blockIdx.x * R >= nrow) return;   // whole workgroup
+    const int lr = div_small(threadIdx.x, kk, 1.0f / (float)kk);
+    const int k = threadIdx.x - lr * kk + 1;
+    const int row = blockIdx.x * R + lr;
+    const bool active = lr < R && row < nrow;
+    const int ri = div_small(row, nrj, 1.0f / (float)nrj);
+    const int i = ri + 2, j = row - ri * nrj + 2;
+    const int64_t c = (int64_t)(k - 1) + (int64_t)(j - 1) * kk + (int64_t)(i - 1) * kk * jj;
+    const bool inner = active && i >= 3 && i <= ii - 2 && j >= 3 && j <= jj - 2;
+    const bool glo = inner && k <= 2, ghi = inner && k >= kk - 1;
+    const int slot = 2 * lr;
+    // block records of the chain in LDS, read before any store (the loop's
+    // global stores would otherwise keep them out of the scalar cache)
+    for (int q = threadIdx.x; q < ch.y; q += kThreads) {
+        const Block &Bq = t.blocks[chain[ch.x + q]];
+        rsp[q] = Bq.sp;
+        rox[q] = Bq.o1x;
+        roy[q] = Bq.o1y;
+        roz[q] = Bq.o1z;
     }
-    if (i <= ii - 2 && j >= 3 && j <= jj - 2) {   // U rows
-        const double uv = u[c];
-        double r = uv;
-        if (kin) {
-            const double rdx = rdx_[B.o1x + i - 1];
-            r = bp ? uv + (dp[c] - dp[c + si]) * bp[c] * bp[c + si] * rdx * rfak
-                   : uv + (dp[c] - dp[c + si]) * rdx * rfak;
+    __syncthreads();
+    for (int q = 0; q < ch.y; ++q) {
+        // one evaluation per lane: its own position, or for a k-face ghost
+        // the position of the neighbour it copies (the last block for k-low
+        // ghosts of the first, the next block for k-high ghosts)
+        const bool elo = glo && q == 0, ehi = ghi && q + 1 < ch.y;
+        const int qe = elo ? ch.y - 1 : ehi ? q + 1 : q;
+        const int64_t sp = rsp[qe];
+        const int ox = rox[qe], oy = roy[qe], oz = roz[qe];
+        const int ke = elo ? k + nk : ehi ? k - nk : k;
+        const int64_t ce = elo ? c + nk : ehi ? c - nk : c;
+        double r[4];
+        if (active) {
+            corr_cell(a, sp, ox, oy, oz, i, j, ke, ce, ii, jj, kk, r);
+            if (inner && (k == kk - 3 || k == kk - 2))
+                for (int f = 0; f < 4; ++f) carry[q & 1][f][slot + k - (kk - 3)] = r[f];
+            if (inner && q == 0 && (k == 3 || k == 4))
+                for (int f = 0; f < 4; ++f) first[f][slot + k - 3] = r[f];
         }
-        u[c] = r;
-    }
-    if (i >= 3 && i <= ii - 2 && j <= jj - 2) {   // V rows
-        const double vv = v[c];
-        double r = vv;
-        if (kin) {
-            const double rdy = rdy_[B.o1y + j - 1];
-            r = bp ? vv + (dp[c] - dp[c + sj]) * bp[c] * bp[c + sj] * rdy * rfak
-                   : vv + (dp[c] - dp[c + sj]) * rdy * rfak;
+        // every load of this block (and of the next one's k = 2, 3) is
+        // done before any store of it; LDS written above is read after
+        __syncthreads();
+        if (active) {
+            if (glo && q > 0)
+                for (int f = 0; f < 4; ++f) r[f] = carry[(q - 1) & 1][f][slot + k - 1];
+            if (ghi && q + 1 == ch.y)
+                for (int f = 0; f < 4; ++f) r[f] = first[f][slot + k - (kk - 1)];
+            corr_store(a, rsp[q], i, j, c, ii, jj, r);
         }
-        v[c] = r;
-    }
-    if (i >= 3 && i <= ii - 2 && j >= 3 && j <= jj - 2) {   // W rows
-        const double wv = w[c];
-        double r = wv;
-        if (k >= 2 && k <= kk - 2) {
-            const double rdz = rdz_[B.o1z + k - 1];
-            r = bp ? wv + (dp[c] - dp[c + 1]) * bp[c] * bp[c + 1] * rdz * rfak
-                   : wv + (dp[c] - dp[c + 1]) * rdz * rfak;
-        }
-        w[c] = r;
     }
 }
 
 void launch_mgpcorr(const Topo &t, double *u, double *v, double *w,
                     double *p, const double *dp, const double *rdx,
                     const double *rdy, const double *rdz, double rfak,
-                    const double *bp, hipStream_t s) {
+                    const double *bp, const CorrChains *cc, hipStream_t s) {
     MGP_RANGE("mgp:mgpcorr");
-    const int maxe = (t.max_ni + 2) * (t.max_nj + 2) * (t.max_nk + 4);
-    dim3 grid((maxe + kThreads - 1) / kThreads, t.nlist);
-    k_mgpcorr<<<grid, kThreads, 0, s>>>(t, u, v, w, p, dp, rdx, rdy, rdz, rfak, bp);
+    const CorrArgs a{u, v, w, p, dp, rdx, rdy, rdz, bp, rfak};
+    if (cc) {
+        k_mgpcorr_chain<<<dim3(cc->nx, cc->n), kThreads, 0, s>>>(t, cc->chain, cc->chains, a);
+    } else {
+        const int maxe = (t.max_ni + 2) * (t.max_nj + 2) * (t.max_nk + 4);
+        k_mgpcorr<<<dim3((maxe + kThreads - 1) / kThreads, t.nlist), kThreads, 0, s>>>(t, a);
+    }
 }
 
 // ---------------------------------------------------------------------

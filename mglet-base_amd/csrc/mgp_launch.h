@@ -155,10 +155,26 @@ template <typename T>
 void launch_wo_to_sp(const Topo &t, const T *src, double *dst,
                      hipStream_t s);
 
+// z-chains of blocks for the velocity and pressure correction that also
+// writes the k-face ghosts of interior rows (k_mgpcorr_chain): chain[] holds
+// block indices, chains[c] = (first entry, length) of chain c, each a cycle
+// along +z of at most kCorrChainMax blocks of one size with nk >= 2; nx
+// workgroups per chain.
+constexpr int kCorrThreads = 256;   // workgroup of the correction kernels
+constexpr int kCorrChainMax = 256;  // longest chain
+struct CorrChains {
+    const int *chain;
+    const int2 *chains;
+    int n, nx;
+};
+
+// cc = nullptr: plain update of the blocks of t.blist; otherwise the chain
+// update, after which the exchange must leave out the k-face copies of
+// interior rows.
 void launch_mgpcorr(const Topo &t, double *u, double *v, double *w,
                     double *p, const double *dp, const double *rdx,
                     const double *rdy, const double *rdz, double rfak,
-                    const double *bp_sp, hipStream_t s);
+                    const double *bp_sp, const CorrChains *cc, hipStream_t s);
 
 // Arguments of the momentum right-hand side (reference layout, FP64).
 struct MomArgs {

@@ -438,3 +438,36 @@ def test_nonfinite_velocity_reports_error(where):
         s.mgpoisl(bad, v.copy(), w.copy(), np.zeros(g.n3d), 1e-3)
     r = s.mgpoisl(u.copy(), v.copy(), w.copy(), np.zeros(g.n3d), 1e-3)
     assert np.isfinite(r["maxrhs"]) and r["nouter_done"] == 5
+
+
+@pytest.mark.parametrize("nb,n", [((1, 1, 1), (12, 12, 12)), ((2, 2, 2), (8, 8, 8)),
+                                  ((2, 1, 2), (7, 6, 5)), ((2, 1, 3), (6, 5, 2))])
+def test_closing_exchange_push_bitwise(oracle_mod, monkeypatch, nb, n):
+    """On single-level periodic grids the velocity and pressure correction
+    stores the k-face ghosts of interior rows for the closing exchange
+    itself (src/flow/pressuresolver_mod.F90:323-348: mgpcorr, then
+    connect); MGP_KPUSH=0 keeps the plain correction and full exchange.
+    Every ghost starts with a value no exchange made, so a ghost the push
+    misses keeps it.  Cases: one self-periodic block (a block pushes into
+    itself), 2x2x2 blocks, odd sizes, nk = 2 (one cell feeds both
+    neighbours)."""
+    import mgp
+    from mgp.grids import uniform_box
+    g = uniform_box(nb, n)
+    o = oracle_mod.Oracle(g)
+    st = o.setup()
+    rng = np.random.default_rng(21)
+    u, v, w, p = (rng.uniform(-1, 1, g.n3d) for _ in range(4))
+    ins = [a.copy() for a in (u, v, w, p)]
+    o.mgpoisl(st, u, v, w, p, 1e-3, ninner=3, nouter=2)
+    outs = []
+    for push in ("1", "0"):
+        monkeypatch.setenv("MGP_KPUSH", push)
+        s = _solver(g, ninner=3, nouter=2)
+        f = [a.copy() for a in ins]
+        s.mgpoisl(*f, 1e-3)
+        s.close()
+        outs.append(f)
+    for a, b, c, nm in zip((u, v, w, p), outs[0], outs[1], "uvwp"):
+        assert np.array_equal(a, b), ("push", nm, float(np.abs(a - b).max()))
+        assert np.array_equal(a, c), ("plain", nm)
