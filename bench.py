@@ -344,7 +344,7 @@ def main():
             log(f"[bench] rk stage: {stage}")
         res[form] = dict(wall=wall, stats=st, kernels=kt, info=s.info(),
                          factor_classes=s.factor_classes(),
-                         autotune={"shape": shape,
+                         autotune={"shape": shape, "backward_shape": s.plan_shape_backward(),
                                    "ms_per_outer_by_shape": [round(x, 3) for x in shape_ms]})
         log(f"[bench] {form}: {wall:.3f} s, device loop {st['solve_ms']:.1f} ms, kernels {kt}")
         s.close()
@@ -461,7 +461,8 @@ def main():
                          "converged": st["converged"],
                          "rel_residual": st["maxrhs"] / st["maxrhs0"],
                          # pipeline shape the autotuner picked for this batch
-                         "autotune_shape": s.plan_shape()[0]}
+                         "autotune_shape": s.plan_shape()[0],
+                         "autotune_backward_shape": s.plan_shape_backward()}
             log(f"[bench] tts {form}: {t:.3f} s, {st['ipc']} inner, conv {st['converged']}")
             s.close()
         if "fp32" in ps and "fp64" in ps:

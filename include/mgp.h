@@ -292,11 +292,18 @@ int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms);
  * SOR, plus one outer update) on h's own blocks and keeps the fastest;
  * 1..MGP_NUM_PLAN_SHAPES sets the shape.  Results do not depend on the
  * shape.  chosen gets the shape in use, ms[MGP_NUM_PLAN_SHAPES] (may be
- * NULL) the timed ms per shape (0 when not timed).  mgp_init autotunes
- * unless MGP_PLAN_SHAPE=1..MGP_NUM_PLAN_SHAPES is set. */
+ * NULL) the timed ms per shape with every kernel in that shape (0 when not
+ * timed).  The autotuner gives the backward sweep, timed on its own, the
+ * shape fastest for it (mgp_plan_shape_backward); a request of 1..N sets
+ * both.  mgp_init autotunes unless MGP_PLAN_SHAPE=1..MGP_NUM_PLAN_SHAPES
+ * is set. */
 #define MGP_NUM_PLAN_SHAPES 9
 int mgp_plan_shape(mgp_handle *h, mgletint request, mgletint *chosen,
                    double *ms);
+
+/* The backward sweep's pipeline shape in use (a tuning query with no
+ * counterpart in the reference). */
+int mgp_plan_shape_backward(mgp_handle *h, mgletint *chosen);
 
 /* Level-weighted max |RHS| after each outer iteration of the last mgpoisl
  * call (the quantity of the exit test, src/flow/pressuresolver_mod.F90:

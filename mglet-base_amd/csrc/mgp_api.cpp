@@ -242,6 +242,7 @@ struct mgp_handle {
     int graph_key[4] = {-1, -1, -1, -1};
     bool have_solution = false;
     int plan_shape = 1;          // pipeline shape of the plane kernels
+    int plan_shape_bwd = 1;      // ... of the backward sweep
     double shape_ms[kNumShapes] = {};
     static_assert(kNumShapes == MGP_NUM_PLAN_SHAPES, "shape count of the C ABI");
 
@@ -757,14 +758,13 @@ bool kpush_on(mgp_handle &h) {
         h.kpush_faces.clear();
         return false;
     }
-    // cycles along +z, started from the blocks in nested chain order
-    std::vector<int> ids(h.ngrid);
-    for (int g = 0; g < h.ngrid; ++g) ids[g] = g + 1;
+    // cycles along +z, started from the blocks in id order (chains that
+    // run at the same time then hold blocks close in memory)
     std::vector<char> seen(h.ngrid, 0);
     std::vector<int> chain;
     std::vector<int2> chains;
     int nx = 0;
-    for (int g1 : chain_order(h, ids)) {
+    for (int g1 = 1; g1 <= h.ngrid; ++g1) {
         if (seen[g1 - 1]) continue;
         const int start = (int)chain.size();
         for (int g = g1 - 1; !seen[g]; g = up[g]) {
@@ -774,7 +774,7 @@ bool kpush_on(mgp_handle &h) {
         if ((int)chain.size() - start > kCorrChainMax) return false;
         chains.push_back(make_int2(start, (int)chain.size() - start));
         const int nrow = (h.ii[g1 - 1] - 2) * (h.jj[g1 - 1] - 2);
-        const int rows = kCorrThreads / h.kk[g1 - 1];
+        const int rows = std::min(kCorrThreads / h.kk[g1 - 1], kCorrRows);
         nx = std::max(nx, (nrow + rows - 1) / rows);
     }
     upload(h.d_cchain, chain);
@@ -1632,7 +1632,7 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
         const char *ps = getenv("MGP_PLAN_SHAPE");
         const int forced = ps ? atoi(ps) : 0;
         if (forced >= 1 && forced <= kNumShapes) {
-            h->plan_shape = forced;
+            h->plan_shape = h->plan_shape_bwd = forced;
         } else {
             if (h->fp32()) tune_shape<float>(*h);
             else tune_shape<double>(*h);
@@ -1647,7 +1647,9 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
 
 // Autotuner of the plane kernels' pipeline shape: times, per shape, the
 // kernels of one outer iteration (ninner sweeps or relaxations and one
-// outer update) on the handle's blocks and keeps the fastest.  Runs on
+// outer update) on the handle's blocks and keeps the fastest; the backward
+// sweep, timed on its own, gets the shape fastest for it.  shape_ms holds
+// the outer iteration's time with every kernel in one shape.  Runs on
 // whatever the work arrays hold (before the first solve: scratch); the
 // solver state is reset at the start of every solve.
 template <typename T>
@@ -1665,36 +1667,43 @@ void tune_shape(mgp_handle &h) {
     hipEvent_t e0, e1;
     HC(hipEventCreate(&e0));
     HC(hipEventCreate(&e1));
-    auto pass = [&]() {
+    auto fwd = [&]() {
         if (sor) launch_sor<T>(tall, F, m, F.hface, h.d_hface2.as<T>(), h.cfg.omega, 0, never, s);
-        else {
-            launch_forward<T>(tall, F, m, 0, 1, 0, classic, never, s);
-            launch_backward<T>(tall, F, 0, classic, never, s);
-        }
+        else launch_forward<T>(tall, F, m, 0, 1, 0, classic, never, s);
     };
-    int best = 0;
+    auto bwd = [&]() {
+        if (!sor) launch_backward<T>(tall, F, 0, classic, never, s);
+    };
+    auto timed = [&](auto &&f, int reps) {
+        float ms = 0.f;
+        HC(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) f();
+        HC(hipEventRecord(e1, s));
+        HC(hipEventSynchronize(e1));
+        HC(hipEventElapsedTime(&ms, e0, e1));
+        return (double)ms;
+    };
+    const int nin = std::max(1, h.cfg.ninner);
+    int best = 0, best_b = 0;
+    double tf[kNumShapes], tb[kNumShapes];
     for (int v = 1; v <= kNumShapes; ++v) {
         set_plan_shape(v);
-        pass();  // warm-up
-        float a = 0.f, b = 0.f;
-        HC(hipEventRecord(e0, s));
-        for (int r = 0; r < 2; ++r) pass();
-        HC(hipEventRecord(e1, s));
-        HC(hipEventSynchronize(e1));
-        HC(hipEventElapsedTime(&a, e0, e1));
-        HC(hipEventRecord(e0, s));
-        launch_outer<T>(tall, F, m, o, rhsT, s);
-        HC(hipEventRecord(e1, s));
-        HC(hipEventSynchronize(e1));
-        HC(hipEventElapsedTime(&b, e0, e1));
-        h.shape_ms[v - 1] = 0.5 * a * std::max(1, h.cfg.ninner) + b;
-        if (!best || h.shape_ms[v - 1] < h.shape_ms[best - 1]) best = v;
+        fwd();  // warm-up
+        bwd();
+        const double a = timed(fwd, 2), ab = timed(bwd, 2);
+        const double b = timed([&]() { launch_outer<T>(tall, F, m, o, rhsT, s); }, 1);
+        tf[v - 1] = 0.5 * a * nin + b;
+        tb[v - 1] = 0.5 * ab * nin;
+        h.shape_ms[v - 1] = tf[v - 1] + tb[v - 1];
+        if (!best || tf[v - 1] < tf[best - 1]) best = v;
+        if (!best_b || tb[v - 1] < tb[best_b - 1]) best_b = v;
     }
     HC(hipGetLastError());
     HC(hipEventDestroy(e0));
     HC(hipEventDestroy(e1));
     h.plan_shape = best;
-    set_plan_shape(best);
+    h.plan_shape_bwd = sor ? best : best_b;
+    set_plan_shapes(h.plan_shape, h.plan_shape_bwd);
 }
 
 // Pool fields start at zero (src/core/fieldpool_mod.F90:113-122).
@@ -1725,7 +1734,7 @@ float iterate(mgp_handle &h, const Fields<T> &F, const Outer &o,
     static_assert(sizeof(T) == 4 || sizeof(T) == 8, "T");
     const mgp_config &c = h.cfg;
     const hipStream_t s = h.stream;
-    set_plan_shape(h.plan_shape);
+    set_plan_shapes(h.plan_shape, h.plan_shape_bwd);
     Topo tall = h.topo(-1);
     T *rhsT = h.fp32() ? h.d_rhsT.as<T>() : nullptr;
     hipEvent_t e0, e1;
@@ -2529,7 +2538,7 @@ int mgp_time_kernels(mgp_handle *h, mgletint niter, double *ms) {
     MGP_RANGE("mgp_time_kernels");
     if (!h || !ms || niter < 1) return fail(MGP_ERR_ARG, "bad argument");
     try {
-        set_plan_shape(h->plan_shape);
+        set_plan_shapes(h->plan_shape, h->plan_shape_bwd);
         const hipStream_t s = h->stream;
         hipEvent_t ev[4];
         for (auto &e : ev) HC(hipEventCreate(&e));
@@ -2591,7 +2600,7 @@ int mgp_plan_shape(mgp_handle *h, mgletint request, mgletint *chosen,
             if (h->fp32()) tune_shape<float>(*h);
             else tune_shape<double>(*h);
         } else {
-            h->plan_shape = request;
+            h->plan_shape = h->plan_shape_bwd = request;
         }
     } catch (const HipError &e) {
         return fail(MGP_ERR_HIP, "%s", e.msg.c_str());
@@ -2599,6 +2608,13 @@ int mgp_plan_shape(mgp_handle *h, mgletint request, mgletint *chosen,
     if (chosen) *chosen = h->plan_shape;
     if (ms)
         for (int v = 0; v < kNumShapes; ++v) ms[v] = h->shape_ms[v];
+    return MGP_OK;
+}
+
+int mgp_plan_shape_backward(mgp_handle *h, mgletint *chosen) {
+    MGP_RANGE("mgp_plan_shape_backward");
+    if (!h || !chosen) return fail(MGP_ERR_ARG, "bad argument");
+    *chosen = h->plan_shape_bwd;
     return MGP_OK;
 }
 

@@ -682,40 +682,113 @@ __device__ __forceinline__ int div_small(int e, int d, float rd) {
 // The update of one position (i, j, k 1-based, c its offset in the block)
 // of a block (sp, ox, oy, oz: its offsets in the 3-D and 1-D fields):
 // r = P, U, V, W, each the stored value where that field is not updated
-// along k; rows outside a field's row range are not stored (corr_store).
-// Every operand is loaded before any is used, whatever the position, so a
-// wave waits for memory once: all of them lie inside the block (rows i + 1
-// and j + 1 exist for i <= ii - 1, j <= jj - 1; the k + 1 neighbour is
-// clamped at the end of the row, where W is not updated).
+// along k.  corr_load_t reads every operand whatever the position, so a
+// wave waits for memory once and takes one path: all of them lie inside
+// the block (rows i + 1 and j + 1 exist for i <= ii - 1, j <= jj - 1; the
+// k + 1 neighbour is clamped at the end of the row, where W is not
+// updated).
 struct CorrArgs {
     double *u, *v, *w, *p;
     const double *dp, *rdx, *rdy, *rdz, *bp;
     double rfak;
 };
 
+// the operands of one position, and the update from them
+struct CorrIn {
+    double pv, uv, vv, wv, d0, di, dj, dk, rdx, rdy, rdz, b0, bi, bj, bk;
+};
+
+template <bool BP>
+__device__ __forceinline__ CorrIn corr_load_t(const CorrArgs &a, int64_t sp, int ox, int oy,
+                                              int oz, int i, int j, int k, int64_t c, int jj,
+                                              int kk) {
+    const int64_t sj = kk, si = (int64_t)kk * jj;
+    const int64_t ck = c + (k < kk ? 1 : 0);
+    const double *dp = a.dp + sp;
+    CorrIn x;
+    x.pv = a.p[sp + c];
+    x.uv = a.u[sp + c];
+    x.vv = a.v[sp + c];
+    x.wv = a.w[sp + c];
+    x.d0 = dp[c];
+    x.di = dp[c + si];
+    x.dj = dp[c + sj];
+    x.dk = dp[ck];
+    x.rdx = a.rdx[ox + i - 1];
+    x.rdy = a.rdy[oy + j - 1];
+    x.rdz = a.rdz[oz + k - 1];
+    if (BP) {
+        const double *bp = a.bp + sp;
+        x.b0 = bp[c];
+        x.bi = bp[c + si];
+        x.bj = bp[c + sj];
+        x.bk = bp[ck];
+    } else {
+        x.b0 = x.bi = x.bj = x.bk = 1.0;
+    }
+    return x;
+}
+
+template <bool BP>
+__device__ __forceinline__ void corr_eval_t(const CorrArgs &a, const CorrIn &x, int k, int kk,
+                                            double r[4]) {
+    const double rfak = a.rfak;
+    const bool kin = k >= 3 && k <= kk - 2;
+    if (BP) {
+        r[0] = k >= 2 && k <= kk - 1 ? x.pv + x.d0 * x.b0 : x.pv;
+        r[1] = kin ? x.uv + (x.d0 - x.di) * x.b0 * x.bi * x.rdx * rfak : x.uv;
+        r[2] = kin ? x.vv + (x.d0 - x.dj) * x.b0 * x.bj * x.rdy * rfak : x.vv;
+        r[3] = k >= 2 && k <= kk - 2 ? x.wv + (x.d0 - x.dk) * x.b0 * x.bk * x.rdz * rfak : x.wv;
+    } else {
+        r[0] = k >= 2 && k <= kk - 1 ? x.pv + x.d0 : x.pv;
+        r[1] = kin ? x.uv + (x.d0 - x.di) * x.rdx * rfak : x.uv;
+        r[2] = kin ? x.vv + (x.d0 - x.dj) * x.rdy * rfak : x.vv;
+        r[3] = k >= 2 && k <= kk - 2 ? x.wv + (x.d0 - x.dk) * x.rdz * rfak : x.wv;
+    }
+}
+
+// The plain update of one position: each field read only on the rows it
+// updates (corr_store's row ranges), the same expressions as corr_eval_t
 __device__ __forceinline__ void corr_cell(const CorrArgs &a, int64_t sp, int ox, int oy,
                                           int oz, int i, int j, int k, int64_t c, int ii,
                                           int jj, int kk, double r[4]) {
     const int64_t sj = kk, si = (int64_t)kk * jj;
-    const int64_t ck = c + (k < kk ? 1 : 0);
+    const double *u = a.u + sp, *v = a.v + sp, *w = a.w + sp, *p = a.p + sp;
     const double *dp = a.dp + sp;
-    const double pv = a.p[sp + c], uv = a.u[sp + c], vv = a.v[sp + c], wv = a.w[sp + c];
-    const double d0 = dp[c], di = dp[c + si], dj = dp[c + sj], dk = dp[ck];
-    const double rdx = a.rdx[ox + i - 1], rdy = a.rdy[oy + j - 1], rdz = a.rdz[oz + k - 1];
+    const double *bp = a.bp ? a.bp + sp : nullptr;
     const double rfak = a.rfak;
     const bool kin = k >= 3 && k <= kk - 2;
-    if (a.bp) {
-        const double *bp = a.bp + sp;
-        const double b0 = bp[c], bi = bp[c + si], bj = bp[c + sj], bk = bp[ck];
-        r[0] = k >= 2 && k <= kk - 1 ? pv + d0 * b0 : pv;
-        r[1] = kin ? uv + (d0 - di) * b0 * bi * rdx * rfak : uv;
-        r[2] = kin ? vv + (d0 - dj) * b0 * bj * rdy * rfak : vv;
-        r[3] = k >= 2 && k <= kk - 2 ? wv + (d0 - dk) * b0 * bk * rdz * rfak : wv;
-    } else {
-        r[0] = k >= 2 && k <= kk - 1 ? pv + d0 : pv;
-        r[1] = kin ? uv + (d0 - di) * rdx * rfak : uv;
-        r[2] = kin ? vv + (d0 - dj) * rdy * rfak : vv;
-        r[3] = k >= 2 && k <= kk - 2 ? wv + (d0 - dk) * rdz * rfak : wv;
+    {   // P: i, j, k in 2..n-1
+        const double pv = p[c];
+        r[0] = pv;
+        if (k >= 2 && k <= kk - 1) r[0] = bp ? pv + dp[c] * bp[c] : pv + dp[c];
+    }
+    if (i <= ii - 2 && j >= 3 && j <= jj - 2) {   // U rows
+        const double uv = u[c];
+        r[1] = uv;
+        if (kin) {
+            const double rdx = a.rdx[ox + i - 1];
+            r[1] = bp ? uv + (dp[c] - dp[c + si]) * bp[c] * bp[c + si] * rdx * rfak
+                      : uv + (dp[c] - dp[c + si]) * rdx * rfak;
+        }
+    }
+    if (i >= 3 && i <= ii - 2 && j <= jj - 2) {   // V rows
+        const double vv = v[c];
+        r[2] = vv;
+        if (kin) {
+            const double rdy = a.rdy[oy + j - 1];
+            r[2] = bp ? vv + (dp[c] - dp[c + sj]) * bp[c] * bp[c + sj] * rdy * rfak
+                      : vv + (dp[c] - dp[c + sj]) * rdy * rfak;
+        }
+    }
+    if (i >= 3 && i <= ii - 2 && j >= 3 && j <= jj - 2) {   // W rows
+        const double wv = w[c];
+        r[3] = wv;
+        if (k >= 2 && k <= kk - 2) {
+            const double rdz = a.rdz[oz + k - 1];
+            r[3] = bp ? wv + (dp[c] - dp[c + 1]) * bp[c] * bp[c + 1] * rdz * rfak
+                      : wv + (dp[c] - dp[c + 1]) * rdz * rfak;
+        }
     }
 }
 
@@ -749,40 +822,48 @@ __global__ __launch_bounds__(kThreads) void k_mgpcorr(Topo t, CorrArgs a) {
 // (x, chain) takes the same whole rows of every block of its chain in
 // order, and writes the k-face ghosts of interior rows (k = 0, 1 and
 // nk + 2, nk + 3, 0-based) as part of those rows, with the values the
-// exchange would copy there: the k-high ghosts are the next block's update
-// at k = 2, 3, evaluated from its cells before the workgroup reaches it
-// (the same expressions, so the same bits), the k-low ghosts the previous
-// block's results at nk, nk + 1, carried in LDS (the chain's first block
-// evaluates its last block's, and the last block takes the first block's
-// results at 2, 3 kept from the start).  Every store is part of a whole
-// row, and the exchange that follows leaves these cells out.
-static_assert(kThreads == kCorrThreads, "workgroup of the correction kernels");
-constexpr int kCorrRows = kThreads / 6;   // rows per workgroup for kk >= 6
-
+// exchange would copy there.  Step q stores block q; it has computed block
+// q + 1 (from loads issued one step earlier, so they arrive during the
+// previous step's stores) and issued the loads of block q + 2.  Block q's
+// k-high ghosts are block q + 1's results at k = 2, 3 and its k-low ghosts
+// block q - 1's at nk, nk + 1, both passed through LDS; the first block's
+// k-low ghosts are evaluated from the last block's cells before anything
+// is stored, and the last block's k-high ghosts are the first block's
+// results.  Every value is the update the exchange would copy, the same
+// expressions on the same operands, and every store is part of a whole
+// row; the exchange that follows leaves these cells out.  Lanes load and
+// store only their own positions (the k-low ghost lanes of the first
+// block aside, whose source block is stored last), so the barrier orders
+// LDS only.  Lanes beyond the workgroup's rows repeat the work of its
+// first lane (same loads, same LDS slots, the same values stored to the
+// same places): every lane takes one path, with a fixed number of loads
+// and stores per step, so the wait for a block's loads leaves the
+// previous block's stores and the next block's loads in flight.
+template <bool BP>
 __global__ __launch_bounds__(kThreads) void k_mgpcorr_chain(Topo t, const int *chain,
                                                            const int2 *chains, CorrArgs a) {
-    __shared__ double carry[2][4][2 * kCorrRows];   // previous block at nk, nk + 1
-    __shared__ double first[4][2 * kCorrRows];      // first block at 2, 3
-    __shared__ int64_t rsp[kCorrChainMax];          // the chain's block offsets
+    constexpr int S = 2 * kCorrRows;
+    __shared__ double carry[4][4][S];   // block b's results at nk, nk + 1: [b mod 4]
+    __shared__ double nxt[2][4][S];     // block b's results at 2, 3: [b mod 2]
+    __shared__ double first[4][S];      // block 0's results at 2, 3
+    __shared__ int64_t rsp[kCorrChainMax];   // the chain's block offsets
     __shared__ int rox[kCorrChainMax], roy[kCorrChainMax], roz[kCorrChainMax];
     const int2 ch = chains[blockIdx.y];
     const Block &B0 = t.blocks[chain[ch.x]];
     const int nk = B0.nk, kk = nk + 4, jj = B0.nj + 4, ii = B0.ni + 4;
     const int nrj = jj - 2, nrow = (ii - 2) * nrj;
-    const int R = kThreads / kk;
+    const int R = min(kThreads / kk, kCorrRows);
     if (blockIdx.x * R >= nrow) return;   // whole workgroup
-    const int lr = div_small(threadIdx.x, kk, 1.0f / (float)kk);
-    const int k = threadIdx.x - lr * kk + 1;
+    int lr = div_small(threadIdx.x, kk, 1.0f / (float)kk);
+    int k = threadIdx.x - lr * kk + 1;
+    if (lr >= R || blockIdx.x * R + lr >= nrow) lr = 0, k = 1;   // repeat the first lane
     const int row = blockIdx.x * R + lr;
-    const bool active = lr < R && row < nrow;
     const int ri = div_small(row, nrj, 1.0f / (float)nrj);
     const int i = ri + 2, j = row - ri * nrj + 2;
     const int64_t c = (int64_t)(k - 1) + (int64_t)(j - 1) * kk + (int64_t)(i - 1) * kk * jj;
-    const bool inner = active && i >= 3 && i <= ii - 2 && j >= 3 && j <= jj - 2;
+    const bool inner = i >= 3 && i <= ii - 2 && j >= 3 && j <= jj - 2;
     const bool glo = inner && k <= 2, ghi = inner && k >= kk - 1;
     const int slot = 2 * lr;
-    // block records of the chain in LDS, read before any store (the loop's
-    // global stores would otherwise keep them out of the scalar cache)
     for (int q = threadIdx.x; q < ch.y; q += kThreads) {
         const Block &Bq = t.blocks[chain[ch.x + q]];
         rsp[q] = Bq.sp;
@@ -791,33 +872,50 @@ __global__ __launch_bounds__(kThreads) void k_mgpcorr_chain(Topo t, const int *c
         roz[q] = Bq.o1z;
     }
     __syncthreads();
+    auto load = [&](int qb, int ke, int64_t ce) {
+        return corr_load_t<BP>(a, rsp[qb], rox[qb], roy[qb], roz[qb], i, j, ke, ce, jj, kk);
+    };
+    auto save = [&](int qb, const double r[4]) {
+        if (inner && (k == kk - 3 || k == kk - 2))
+            for (int f = 0; f < 4; ++f) carry[qb & 3][f][slot + k - (kk - 3)] = r[f];
+        if (inner && (k == 3 || k == 4))
+            for (int f = 0; f < 4; ++f) {
+                nxt[qb & 1][f][slot + k - 3] = r[f];
+                if (qb == 0) first[f][slot + k - 3] = r[f];
+            }
+    };
+    double rc[4], rn[4] = {0.0, 0.0, 0.0, 0.0};
+    CorrIn x;
+    {   // block 0; its k-low ghost lanes evaluate the last block at k + nk
+        const int ke = glo ? k + nk : k;
+        const CorrIn x0 = glo ? load(ch.y - 1, ke, c + nk) : load(0, k, c);
+        corr_eval_t<BP>(a, x0, ke, kk, rc);
+        save(0, rc);
+        if (ch.y > 1) x = load(1, k, c);
+    }
     for (int q = 0; q < ch.y; ++q) {
-        // one evaluation per lane: its own position, or for a k-face ghost
-        // the position of the neighbour it copies (the last block for k-low
-        // ghosts of the first, the next block for k-high ghosts)
-        const bool elo = glo && q == 0, ehi = ghi && q + 1 < ch.y;
-        const int qe = elo ? ch.y - 1 : ehi ? q + 1 : q;
-        const int64_t sp = rsp[qe];
-        const int ox = rox[qe], oy = roy[qe], oz = roz[qe];
-        const int ke = elo ? k + nk : ehi ? k - nk : k;
-        const int64_t ce = elo ? c + nk : ehi ? c - nk : c;
-        double r[4];
-        if (active) {
-            corr_cell(a, sp, ox, oy, oz, i, j, ke, ce, ii, jj, kk, r);
-            if (inner && (k == kk - 3 || k == kk - 2))
-                for (int f = 0; f < 4; ++f) carry[q & 1][f][slot + k - (kk - 3)] = r[f];
-            if (inner && q == 0 && (k == 3 || k == 4))
-                for (int f = 0; f < 4; ++f) first[f][slot + k - 3] = r[f];
+        if (q + 1 < ch.y) {
+            corr_eval_t<BP>(a, x, k, kk, rn);
+            save(q + 1, rn);
+            if (q + 2 < ch.y) x = load(q + 2, k, c);
         }
-        // every load of this block (and of the next one's k = 2, 3) is
-        // done before any store of it; LDS written above is read after
-        __syncthreads();
-        if (active) {
+        lds_barrier();
+        {
             if (glo && q > 0)
-                for (int f = 0; f < 4; ++f) r[f] = carry[(q - 1) & 1][f][slot + k - 1];
-            if (ghi && q + 1 == ch.y)
-                for (int f = 0; f < 4; ++f) r[f] = first[f][slot + k - (kk - 1)];
-            corr_store(a, rsp[q], i, j, c, ii, jj, r);
+                for (int f = 0; f < 4; ++f) rc[f] = carry[(q - 1) & 3][f][slot + k - 1];
+            if (ghi)
+                for (int f = 0; f < 4; ++f)
+                    rc[f] = q + 1 < ch.y ? nxt[(q + 1) & 1][f][slot + k - (kk - 1)]
+                                         : first[f][slot + k - (kk - 1)];
+            // all four fields on every row (rows outside a field's range
+            // get back the value they hold): a fixed number of stores, so
+            // the wait for the next loads leaves these in flight
+            const int64_t o = rsp[q] + c;
+            a.p[o] = rc[0];
+            a.u[o] = rc[1];
+            a.v[o] = rc[2];
+            a.w[o] = rc[3];
+            for (int f = 0; f < 4; ++f) rc[f] = rn[f];
         }
     }
 }
@@ -829,7 +927,12 @@ void launch_mgpcorr(const Topo &t, double *u, double *v, double *w,
     MGP_RANGE("mgp:mgpcorr");
     const CorrArgs a{u, v, w, p, dp, rdx, rdy, rdz, bp, rfak};
     if (cc) {
-        k_mgpcorr_chain<<<dim3(cc->nx, cc->n), kThreads, 0, s>>>(t, cc->chain, cc->chains, a);
+        if (bp)
+            k_mgpcorr_chain<true><<<dim3(cc->nx, cc->n), kThreads, 0, s>>>(t, cc->chain,
+                                                                           cc->chains, a);
+        else
+            k_mgpcorr_chain<false><<<dim3(cc->nx, cc->n), kThreads, 0, s>>>(t, cc->chain,
+                                                                            cc->chains, a);
     } else {
         const int maxe = (t.max_ni + 2) * (t.max_nj + 2) * (t.max_nk + 4);
         k_mgpcorr<<<dim3((maxe + kThreads - 1) / kThreads, t.nlist), kThreads, 0, s>>>(t, a);

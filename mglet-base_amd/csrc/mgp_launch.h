@@ -133,7 +133,8 @@ void launch_sor(const Topo &t, const Fields<T> &f, Mask m, const T *fin,
 // 5: 8x64 (one wave); 6: 4x64 (4 waves), 7: 4x64 (8 waves), 8: 3x128
 // (2 waves), 9: 4x64 (2 waves)
 constexpr int kNumShapes = 9;
-void set_plan_shape(int v);
+void set_plan_shape(int v);                // every plane kernel
+void set_plan_shapes(int fwd, int bwd);    // the backward sweep apart
 int plan_shape();
 
 template <typename T>
@@ -162,6 +163,7 @@ void launch_wo_to_sp(const Topo &t, const T *src, double *dst,
 // workgroups per chain.
 constexpr int kCorrThreads = 256;   // workgroup of the correction kernels
 constexpr int kCorrChainMax = 256;  // longest chain
+constexpr int kCorrRows = 16;       // most rows per workgroup of the chain update
 struct CorrChains {
     const int *chain;
     const int2 *chains;

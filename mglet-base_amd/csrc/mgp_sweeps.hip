@@ -597,9 +597,16 @@ __global__ __launch_bounds__(kWave * NW) void k_outer(
 // instruction stream; the deep and the multi-wave shapes are for them.
 // SOR relaxes HILF in place in global memory, which a prefetching wave
 // must not race, so it runs one wave per block in every shape.
-static int g_shape = 1;
+// The backward sweep has a shape of its own (set_plan_shapes), tuned apart:
+// it holds fewer registers and no HILF window, so it can prefer deeper
+// pipelines than the forward sweep and the outer update.
+static int g_shape = 1, g_shape_bwd = 1;
 
-void set_plan_shape(int v) { g_shape = (v >= 1 && v <= kNumShapes) ? v : 1; }
+void set_plan_shape(int v) { set_plan_shapes(v, v); }
+void set_plan_shapes(int fwd, int bwd) {
+    g_shape = (fwd >= 1 && fwd <= kNumShapes) ? fwd : 1;
+    g_shape_bwd = (bwd >= 1 && bwd <= kNumShapes) ? bwd : 1;
+}
 int plan_shape() { return g_shape; }
 
 template <typename T>
@@ -665,7 +672,7 @@ static void bwd_launch(const Topo &t, const Fields<T> &f, const int *done,
 #define MGP_BWD(D, M, NW) \
     k_backward<T, CLASSIC, HZ, D, M, NW><<<t.nlist, kWave * NW, \
         lds + (size_t)NW * kWave * M * sizeof(T), s>>>(t.items, t, f, done)
-    switch (g_shape) {
+    switch (g_shape_bwd) {
     case 1: MGP_BWD(4, 1, 1); break;
     case 3: MGP_BWD(3, 2, 1); break;
     case 4: MGP_BWD(6, 2, 1); break;

@@ -298,6 +298,14 @@ class PressureSolver:
                    "mgp_plan_shape")
         return chosen.value, ms
 
+    def plan_shape_backward(self):
+        """The backward sweep's pipeline shape (the autotuner picks it apart
+        from the others; a set request sets both)."""
+        chosen = C.c_int32()
+        _lib.check(self.lib.mgp_plan_shape_backward(self.h, C.byref(chosen)),
+                   "mgp_plan_shape_backward")
+        return chosen.value
+
     def history(self):
         """max |RHS| after each outer iteration of the last mgpoisl call."""
         cap = max(1, self.cfg.nouter)

@@ -93,6 +93,7 @@ def load():
         "mgp_set_bound": [vp, _dp, _dp, _dp, _dp],
         "mgp_time_kernels": [vp, C.c_int32, _dp],
         "mgp_plan_shape": [vp, C.c_int32, _i32p, _dp],
+        "mgp_plan_shape_backward": [vp, _i32p],
         "mgp_tstle4_device": [vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_int32, _dp],
         "mgp_rkstep_device": [vp, vp, vp, vp, C.c_double, C.c_double],
         "mgp_flow_bc_device": [vp, C.c_int32, vp, vp, vp, vp],
