@@ -30,6 +30,8 @@ template <typename T, bool CLASSIC, bool MASKED, int NS>
 struct OutFwdK {
     static constexpr bool kFaces = true;
     static constexpr bool kSplit = false;
+    static constexpr bool kFar = false;
+    struct Far {};
     struct Data {
         uint32_t c;
         double ap, rhs, dp;

@@ -318,6 +318,14 @@ __device__ __forceinline__ void fill_tab(SrcTab<T> *tab, const Topo &t,
     }
 }
 
+// Row l of an item of n rows, lanes past the end repeating its last row
+// (row 0 when n = 0): min(l, max(n, 1) - 1), one VALU instruction with the
+// bound in a scalar register instead of a compare and two selects.
+__device__ __forceinline__ int clamp_row(int l, int n) {
+    const uint32_t lim = (uint32_t)__builtin_amdgcn_readfirstlane(n > 0 ? n - 1 : 0);
+    return (int)min((uint32_t)l, lim);
+}
+
 // Fields of a face-store entry (face_entry, mgp_internal.h).
 __device__ __forceinline__ int face_src(uint32_t f) { return f & 0xfff; }
 __device__ __forceinline__ int face_dst(uint32_t f) { return (f >> 12) & 0x1fff; }
@@ -325,7 +333,8 @@ __device__ __forceinline__ bool face_hi(uint32_t f) { return (f >> 25) & 1; }
 
 // The plan engine: runs the items of one block through functor K.  An
 // item covers up to 64 M cells (M per lane) and 64 M entries.  K provides
-// Data, load(Data&, s), compute(const Data&, s, p, l) (l = the cell's row
+// Data, load(Data&, s), kFar with Far, far_load(Far&, s) and merge(Data&,
+// const Far&), compute(const Data&, s, p, l) (l = the cell's row
 // in the item, lane + 64 m), store(entry, value, bp, p), and, with
 // K::kFaces, face_get(entry, p) and face_put(entry, value): the boundary
 // values the item computed, read back from LDS by the lanes of its
@@ -341,6 +350,11 @@ __device__ __forceinline__ bool face_hi(uint32_t f) { return (f >> 25) & 1; }
 //   * entry words of item q: loaded at issue(q - D), ring ew[q mod W];
 //   * data and entry values of item q: loaded at issue(q), ring q mod D,
 //     consumed by compute(q), which runs D items later;
+//   * with K::kFar, the fields of K::Far (the ones HBM serves, when the
+//     rest come from L2): loaded at issue(q - D), ring q mod W, merged into
+//     item q's Data (K::merge) just before compute(q), so they get twice
+//     the prefetch distance without the registers of a deeper ring for
+//     every field;
 //   * the LDS source pointers of item q's entries are looked up at the end
 //     of issue(q - 1).
 // All loads go through the in-order vector memory counter, so each item
@@ -368,6 +382,13 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
     T fvp = T(0);
     const T *pn[M];
     const uint8_t *bn[M];
+    typename K::Far fd[K::kFar ? W : 1][M];
+    if constexpr (K::kFar) {
+#pragma unroll
+        for (int r = 0; r < W; ++r)
+#pragma unroll
+            for (int m = 0; m < M; ++m) fd[r][m] = typename K::Far{};
+    }
     // this wave's items: every NW-th of the plan, from its own first one
     auto ldi = [&](int q) -> uint2 { return ld(items, ((q < 0 || q > last) ? last : q) * NW); };
     auto lookup = [&](int m, uint32_t e) {
@@ -385,7 +406,7 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
 #pragma unroll
         for (int m = 0; m < M; ++m) {
             const int l = lane + 64 * m;
-            k.load(cd[u][m], cpos + (l < cc ? l : (cc > 0 ? cc - 1 : 0)));
+            k.load(cd[u][m], cpos + clamp_row(l, cc));
         }
 #pragma unroll
         for (int m = 0; m < M; ++m) {
@@ -404,7 +425,16 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
 #pragma unroll
         for (int m = 0; m < M; ++m) {
             const int l = lane + 64 * m;
-            ew[r2][m] = ld(ents, ep2 + (l < ec2 ? l : (ec2 > 0 ? ec2 - 1 : 0)));
+            ew[r2][m] = ld(ents, ep2 + clamp_row(l, ec2));
+        }
+        if constexpr (K::kFar) {   // item q + D
+            const int cp2 = x2 & 0xffff, cc2 = (x2 >> 16) & 0xff;
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const int l = lane + 64 * m;
+                k.far_load(fd[r2 % (K::kFar ? W : 1)][m],
+                           cp2 + clamp_row(l, cc2));
+            }
         }
         iv[r] = ldi(q + W);
         const int r1 = (r + 1) % W;
@@ -435,6 +465,10 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
             const uint32_t x = ix[u], y = iy[u];
             const int cpos = x & 0xffff, cc = (x >> 16) & 0xff, ec = x >> 24;
             const int p = (int)(y >> 21) - 8;
+            if constexpr (K::kFar) {
+#pragma unroll
+                for (int m = 0; m < M; ++m) k.merge(cd[u][m], fd[w % (K::kFar ? W : 1)][m]);
+            }
             bool split = false;
             if constexpr (K::kSplit && M > 1) {
                 // every row holds cells: all LDS reads of the item first,

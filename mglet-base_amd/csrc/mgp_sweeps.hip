@@ -50,6 +50,8 @@ struct FwdK {
     static constexpr bool APC = LAP && !MASKED;
     static constexpr bool kFaces = true;
     static constexpr bool kSplit = true;
+    static constexpr bool kFar = false;
+    struct Far {};
     struct Data {
         uint32_t c;
         T rhs, lpr, lb, ls, lw, ap;
@@ -224,9 +226,15 @@ template <typename T, bool CLASSIC, bool HZ>
 struct BwdK {
     static constexpr bool kFaces = true;
     static constexpr bool kSplit = true;
+    // RES and HILF come from HBM, the cell words and factors (shared) from
+    // L2: the former get the engine's far prefetch
+    static constexpr bool kFar = true;
     struct Data {
         uint32_t c;
         T res, ue, un, ut, h;
+    };
+    struct Far {
+        T res, h;
     };
     struct G {
         T rt, rn, re;   // recurrence neighbours k+1, j+1, i+1
@@ -240,11 +248,17 @@ struct BwdK {
 
     __device__ __forceinline__ void load(Data &d, int s) const {
         d.c = ld(cells, s);
-        d.res = ld(res, s);
         d.ue = ld(ue, s);
         d.un = ld(un, s);
         d.ut = ld(ut, s);
-        if (!HZ) d.h = ld((const T *)hilf, s);
+    }
+    __device__ __forceinline__ void far_load(Far &f, int s) const {
+        f.res = ld(res, s);
+        f.h = HZ ? T(0) : ld((const T *)hilf, s);
+    }
+    __device__ __forceinline__ void merge(Data &d, const Far &f) const {
+        d.res = f.res;
+        d.h = f.h;
     }
     __device__ __forceinline__ G gather(const Data &d, int p) const {
         G g;
@@ -344,6 +358,8 @@ template <typename T, bool MASKED, bool HZ>
 struct SorK {
     static constexpr bool kFaces = true;
     static constexpr bool kSplit = false;
+    static constexpr bool kFar = false;
+    struct Far {};
     struct Data {
         uint32_t c;
         T rhs, rap;
@@ -480,6 +496,8 @@ template <typename T, bool MASKED, int NS>
 struct OutK {
     static constexpr bool kFaces = false;
     static constexpr bool kSplit = false;
+    static constexpr bool kFar = false;
+    struct Far {};
     struct Data {
         uint32_t c;
         double ap, rhs, dp;
