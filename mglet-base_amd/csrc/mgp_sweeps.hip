@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "mgp_device.h"
 #include "mgp_launch.h"
@@ -41,8 +42,13 @@ namespace mgp {
 // iteration (the reference's conn(..., forward=-1) then skips faces
 // 1/3/5), the lower planes always go to rlo for the backward sweep.
 // LAP = false is the first inner iteration of an outer iteration, where
-// HILF = 0 and the Laplacian term vanishes.
-template <typename T, bool CLASSIC, bool MASKED, bool LAP, int NS>
+// HILF = 0 and the Laplacian term vanishes.  LATE: the factors and GSAP,
+// which L2 serves when every block shares one copy, are loaded when the
+// item gathers its LDS values instead of D items ahead, which frees 10
+// registers per pipeline slot (the multi-wave FP32 shapes then run 2-2.5 %
+// faster on 32768 blocks of 24^3; from HBM, without sharing, they would
+// stall instead).
+template <typename T, bool CLASSIC, bool MASKED, bool LAP, int NS, bool LATE = false>
 struct FwdK {
     // launches without blocked cells take GSAP from the copy shared with the
     // factors (an L2 hit), or rebuild it from its separable part when no
@@ -52,9 +58,15 @@ struct FwdK {
     static constexpr bool kSplit = true;
     static constexpr bool kFar = false;
     struct Far {};
-    struct Data {
+    struct Early {
+        T lpr, lb, ls, lw, ap;
+    };
+    struct Late {
+        int s;
+    };
+    struct Data : std::conditional_t<LATE, Late, Early> {
         uint32_t c;
-        T rhs, lpr, lb, ls, lw, ap;
+        T rhs;
     };
     // LDS values of one cell (gather), then its arithmetic and stores
     // (finish), so the engine can issue two cells' reads before waiting
@@ -62,6 +74,7 @@ struct FwdK {
         Stencil<T, T, MASKED> st;
         T w, e, s, n, b, t, ap;   // 1-D coefficients, GSAP
         T rk, rj, ri;             // recurrence neighbours k-1, j-1, i-1
+        T lpr, lb, ls, lw;        // factors
         int c1;
     };
     const uint32_t *cells;
@@ -80,14 +93,32 @@ struct FwdK {
     __device__ __forceinline__ void load(Data &d, int s) const {
         d.c = ld(cells, s);
         d.rhs = ld(rhs, s);
-        d.lpr = ld(lpr, s);
-        d.lb = ld(lb, s);
-        d.ls = ld(ls, s);
-        d.lw = ld(lw, s);
-        if (LAP) d.ap = ld(ap, s);
+        if constexpr (LATE) {
+            d.s = s;
+        } else {
+            d.lpr = ld(lpr, s);
+            d.lb = ld(lb, s);
+            d.ls = ld(ls, s);
+            d.lw = ld(lw, s);
+            if (LAP) d.ap = ld(ap, s);
+        }
     }
     __device__ __forceinline__ G gather(const Data &d, int p) const {
         G g;
+        T apl = T(0);
+        if constexpr (LATE) {
+            g.lpr = ld(lpr, d.s);
+            g.lb = ld(lb, d.s);
+            g.ls = ld(ls, d.s);
+            g.lw = ld(lw, d.s);
+            if (LAP) apl = ld(ap, d.s);
+        } else {
+            g.lpr = d.lpr;
+            g.lb = d.lb;
+            g.ls = d.ls;
+            g.lw = d.lw;
+            if (LAP) apl = d.ap;
+        }
         int i, j, k;
         cell_ijk(d.c, i, j, k);
         if (LAP) {
@@ -97,7 +128,7 @@ struct FwdK {
                       sp = win_slot<NS>(p + 1) * mHS;
             const Coef6<T> co(cf, nr, i, j, k);
             g.w = co.w; g.e = co.e; g.s = co.s; g.n = co.n; g.b = co.b; g.t = co.t;
-            g.ap = (APC && !apsh) ? (T)ap_cell(ar, nr, i, j, k, cell_flags(d.c), wf) : d.ap;
+            g.ap = (APC && !apsh) ? (T)ap_cell(ar, nr, i, j, k, cell_flags(d.c), wf) : apl;
             g.st.read(H + sm, H + s0, H + sp, BH + sm, BH + s0, BH + sp, c0, HW,
                       MASKED && masked);
         }
@@ -112,15 +143,15 @@ struct FwdK {
                                            bool ok) const {
         T r = T(0);
         if (LAP) r = g.st.eval(g.w, g.e, g.s, g.n, g.b, g.t, g.ap);
-        r = (d.rhs + r) * d.lpr;
+        r = (d.rhs + r) * g.lpr;
         if (CLASSIC) {
-            r = fms(r, d.lw, g.ri);
-            r = fms(r, d.ls, g.rj);
-            r = fms(r, d.lb, g.rk);
+            r = fms(r, g.lw, g.ri);
+            r = fms(r, g.ls, g.rj);
+            r = fms(r, g.lb, g.rk);
         } else {
-            r = fms(r, d.lb, g.rk);
-            r = fms(r, d.ls, g.rj);
-            r = fms(r, d.lw, g.ri);
+            r = fms(r, g.lb, g.rk);
+            r = fms(r, g.ls, g.rj);
+            r = fms(r, g.lw, g.ri);
         }
         if (ok) {
             R[(p & 1) * RS + g.c1] = r;
@@ -152,7 +183,8 @@ struct FwdK {
     }
 };
 
-template <typename T, bool CLASSIC, bool MASKED, bool LAP, int D, int M, int NW>
+template <typename T, bool CLASSIC, bool MASKED, bool LAP, int D, int M, int NW,
+          bool LATE = false>
 __global__ __launch_bounds__(kWave * NW) void k_forward(
     const uint2 *__restrict__ items, Topo t, Fields<T> F, Mask m, int rd,
     int write_hi, const int *done) {
@@ -180,7 +212,7 @@ __global__ __launch_bounds__(kWave * NW) void k_forward(
     if (NW > 1) __syncthreads(); else wave_sync();
     const int NI = B.ni, NJ = B.nj, NK = B.nk;
     const int64_t wo = B.wo, fc = B.fc;
-    FwdK<T, CLASSIC, MASKED, LAP, NS> k;
+    FwdK<T, CLASSIC, MASKED, LAP, NS, LATE> k;
     k.cells = t.cellx + S.cell_off;
     k.rhs = F.rhs + wo;
     k.lw = F.f[0] + B.fo;
@@ -639,17 +671,30 @@ static void fwd_launch(const Topo &t, const Fields<T> &f, Mask m, int rd,
 #define MGP_FWD1(D, M, NW) \
     k_forward<T, CLASSIC, MASKED, LAP, D, M, NW><<<t.nlist, kWave * NW, \
         lds + win_slots<T, NW>() * wslot, s>>>(t.items, t, f, m, rd, write_hi, done)
+    // late factor loads for the FP32 multi-wave shapes when every block's
+    // factors come from a shared copy (L2); measured slower for 8 waves
+#define MGP_FWDL(D, M, NW) \
+    if constexpr (kLate) { \
+        if (f.aps) { \
+            k_forward<T, CLASSIC, MASKED, LAP, D, M, NW, true><<<t.nlist, kWave * NW, \
+                lds + win_slots<T, NW>() * wslot, s>>>(t.items, t, f, m, rd, write_hi, done); \
+            break; \
+        } \
+    } \
+    MGP_FWD1(D, M, NW)
+    constexpr bool kLate = sizeof(T) == 4 && !MASKED && LAP;
     switch (g_shape) {
     case 1: MGP_FWD1(4, 1, 1); break;
     case 3: MGP_FWD1(3, 2, 1); break;
     case 4: MGP_FWD1(6, 2, 1); break;
     case 5: MGP_FWD1(8, 1, 1); break;
-    case 6: MGP_FWD1(4, 1, 4); break;
+    case 6: MGP_FWDL(4, 1, 4); break;
     case 7: MGP_FWD1(4, 1, 8); break;
-    case 8: MGP_FWD1(3, 2, 2); break;
-    case 9: MGP_FWD1(4, 1, 2); break;
+    case 8: MGP_FWDL(3, 2, 2); break;
+    case 9: MGP_FWDL(4, 1, 2); break;
     default: MGP_FWD1(2, 2, 1); break;
     }
+#undef MGP_FWDL
 #undef MGP_FWD1
 }
 
