@@ -355,8 +355,9 @@ __device__ __forceinline__ bool face_hi(uint32_t f) { return (f >> 25) & 1; }
 //     item q's Data (K::merge) just before compute(q), so they get twice
 //     the prefetch distance without the registers of a deeper ring for
 //     every field;
-//   * the LDS source pointers of item q's entries are looked up at the end
-//     of issue(q - 1).
+//   * the LDS source pointers of item q's entries are looked up in
+//     issue(q), just before its entry loads (held through a compute they
+//     would cost 2 M registers).
 // All loads go through the in-order vector memory counter, so each item
 // waits only for its own loads.  The loop starts W items early with no-op
 // items instead of a prologue, so it is entered with no load outstanding
@@ -410,6 +411,7 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
         }
 #pragma unroll
         for (int m = 0; m < M; ++m) {
+            lookup(m, ew[r][m]);
             const int off = (int)(ew[r][m] >> 16);
             ev[u][m] = gld(pn[m] + off);
             if (EBP) eb[u][m] = gld(bn[m] + off);
@@ -437,9 +439,6 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
             }
         }
         iv[r] = ldi(q + W);
-        const int r1 = (r + 1) % W;
-#pragma unroll
-        for (int m = 0; m < M; ++m) lookup(m, ew[r1][m]);
     };
 #pragma unroll
     for (int r = 0; r < W; ++r) {
@@ -456,8 +455,6 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
         iy[u] = 8u << 21;
         fw[u] = kFaceNone;
     }
-#pragma unroll
-    for (int m = 0; m < M; ++m) lookup(m, 0u);
     for (int q0 = -W; q0 < nitem; q0 += W) {
 #pragma unroll
         for (int w = 0; w < W; ++w) {
@@ -477,7 +474,7 @@ __device__ __forceinline__ void run_plan(K &k, const uint2 *__restrict__ items,
                 if (split) {
                     typename K::G g[M];
 #pragma unroll
-                    for (int m = 0; m < M; ++m) g[m] = k.gather(cd[u][m], p);
+                    for (int m = 0; m < M; ++m) g[m] = k.gather(cd[u][m], cpos + lane + 64 * m, p);
 #pragma unroll
                     for (int m = 0; m < M; ++m)
                         k.finish(cd[u][m], g[m], cpos + lane + 64 * m, p, lane + 64 * m,

@@ -61,9 +61,7 @@ struct FwdK {
     struct Early {
         T lpr, lb, ls, lw, ap;
     };
-    struct Late {
-        int s;
-    };
+    struct Late {};
     struct Data : std::conditional_t<LATE, Late, Early> {
         uint32_t c;
         T rhs;
@@ -93,9 +91,7 @@ struct FwdK {
     __device__ __forceinline__ void load(Data &d, int s) const {
         d.c = ld(cells, s);
         d.rhs = ld(rhs, s);
-        if constexpr (LATE) {
-            d.s = s;
-        } else {
+        if constexpr (!LATE) {
             d.lpr = ld(lpr, s);
             d.lb = ld(lb, s);
             d.ls = ld(ls, s);
@@ -103,15 +99,15 @@ struct FwdK {
             if (LAP) d.ap = ld(ap, s);
         }
     }
-    __device__ __forceinline__ G gather(const Data &d, int p) const {
+    __device__ __forceinline__ G gather(const Data &d, int s, int p) const {
         G g;
         T apl = T(0);
         if constexpr (LATE) {
-            g.lpr = ld(lpr, d.s);
-            g.lb = ld(lb, d.s);
-            g.ls = ld(ls, d.s);
-            g.lw = ld(lw, d.s);
-            if (LAP) apl = ld(ap, d.s);
+            g.lpr = ld(lpr, s);
+            g.lb = ld(lb, s);
+            g.ls = ld(ls, s);
+            g.lw = ld(lw, s);
+            if (LAP) apl = ld(ap, s);
         } else {
             g.lpr = d.lpr;
             g.lb = d.lb;
@@ -159,7 +155,7 @@ struct FwdK {
         }
     }
     __device__ __forceinline__ void compute(const Data &d, int s, int p, int l) const {
-        finish(d, gather(d, p), s, p, l, true);
+        finish(d, gather(d, s, p), s, p, l, true);
     }
     __device__ __forceinline__ void store(uint32_t e, T v, uint8_t b, int p) const {
         const int x = e & 0xfff;
@@ -292,7 +288,7 @@ struct BwdK {
         d.res = f.res;
         d.h = f.h;
     }
-    __device__ __forceinline__ G gather(const Data &d, int p) const {
+    __device__ __forceinline__ G gather(const Data &d, int, int p) const {
         G g;
         int i, j, k;
         cell_ijk(d.c, i, j, k);
@@ -323,7 +319,7 @@ struct BwdK {
         }
     }
     __device__ __forceinline__ void compute(const Data &d, int s, int p, int l) const {
-        finish(d, gather(d, p), s, p, l, true);
+        finish(d, gather(d, s, p), s, p, l, true);
     }
     __device__ __forceinline__ void store(uint32_t e, T v, uint8_t, int p) const {
         R[(p & 1) * RS + (e & 0xfff)] = v;
