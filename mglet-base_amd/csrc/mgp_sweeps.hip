@@ -45,9 +45,10 @@ namespace mgp {
 // HILF = 0 and the Laplacian term vanishes.  LATE: the factors and GSAP,
 // which L2 serves when every block shares one copy, are loaded when the
 // item gathers its LDS values instead of D items ahead, which frees 10
-// registers per pipeline slot (the multi-wave FP32 shapes then run 2-2.5 %
-// faster on 32768 blocks of 24^3; from HBM, without sharing, they would
-// stall instead).
+// (FP32) or 20 (FP64) registers per pipeline slot (on 32768 blocks of
+// 24^3 the multi-wave FP32 shapes run 2-2.5 % faster, FP64 with 4 waves of
+// 64-cell items or 2 waves of 128-cell items 16-19 %; from HBM, without
+// sharing, they would stall instead).
 template <typename T, bool CLASSIC, bool MASKED, bool LAP, int NS, bool LATE = false>
 struct FwdK {
     // launches without blocked cells take GSAP from the copy shared with the
@@ -667,10 +668,11 @@ static void fwd_launch(const Topo &t, const Fields<T> &f, Mask m, int rd,
 #define MGP_FWD1(D, M, NW) \
     k_forward<T, CLASSIC, MASKED, LAP, D, M, NW><<<t.nlist, kWave * NW, \
         lds + win_slots<T, NW>() * wslot, s>>>(t.items, t, f, m, rd, write_hi, done)
-    // late factor loads for the FP32 multi-wave shapes when every block's
-    // factors come from a shared copy (L2); measured slower for 8 waves
-#define MGP_FWDL(D, M, NW) \
-    if constexpr (kLate) { \
+    // late factor loads for the multi-wave shapes when every block's
+    // factors come from a shared copy (L2); measured slower for 8 waves,
+    // and in FP64 for 64-cell items with 2 waves
+#define MGP_FWDL(D, M, NW, OK) \
+    if constexpr (OK) { \
         if (f.aps) { \
             k_forward<T, CLASSIC, MASKED, LAP, D, M, NW, true><<<t.nlist, kWave * NW, \
                 lds + win_slots<T, NW>() * wslot, s>>>(t.items, t, f, m, rd, write_hi, done); \
@@ -678,16 +680,17 @@ static void fwd_launch(const Topo &t, const Fields<T> &f, Mask m, int rd,
         } \
     } \
     MGP_FWD1(D, M, NW)
-    constexpr bool kLate = sizeof(T) == 4 && !MASKED && LAP;
+    constexpr bool kLate = !MASKED && LAP;
+    constexpr bool kLate32 = kLate && sizeof(T) == 4;
     switch (g_shape) {
     case 1: MGP_FWD1(4, 1, 1); break;
     case 3: MGP_FWD1(3, 2, 1); break;
     case 4: MGP_FWD1(6, 2, 1); break;
     case 5: MGP_FWD1(8, 1, 1); break;
-    case 6: MGP_FWDL(4, 1, 4); break;
+    case 6: MGP_FWDL(4, 1, 4, kLate); break;
     case 7: MGP_FWD1(4, 1, 8); break;
-    case 8: MGP_FWDL(3, 2, 2); break;
-    case 9: MGP_FWDL(4, 1, 2); break;
+    case 8: MGP_FWDL(3, 2, 2, kLate); break;
+    case 9: MGP_FWDL(4, 1, 2, kLate32); break;
     default: MGP_FWD1(2, 2, 1); break;
     }
 #undef MGP_FWDL
