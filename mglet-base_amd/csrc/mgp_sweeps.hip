@@ -251,22 +251,30 @@ __global__ __launch_bounds__(kWave * NW) void k_forward(
 // this iteration's forward sweep.  Also writes HILF's boundary planes,
 // which stand in for the reference's conn(ilevel, 1, s1=dp) (:414).
 // HZ: HILF is zero on entry (first inner iteration of an outer iteration).
-template <typename T, bool CLASSIC, bool HZ>
+template <typename T, bool CLASSIC, bool HZ, bool LATE = false>
 struct BwdK {
     static constexpr bool kFaces = true;
     static constexpr bool kSplit = true;
     // RES and HILF come from HBM, the cell words and factors (shared) from
-    // L2: the former get the engine's far prefetch
+    // L2: the former get the engine's far prefetch.  LATE (FP64 with
+    // factors shared between blocks): the factors load when the item
+    // gathers (best FP64 shape 3.89 -> 3.80 ms on 32768 blocks of 24^3;
+    // FP32 measured slower that way)
     static constexpr bool kFar = true;
-    struct Data {
+    struct Early {
+        T ue, un, ut;
+    };
+    struct Late {};
+    struct Data : std::conditional_t<LATE, Late, Early> {
         uint32_t c;
-        T res, ue, un, ut, h;
+        T res, h;
     };
     struct Far {
         T res, h;
     };
     struct G {
         T rt, rn, re;   // recurrence neighbours k+1, j+1, i+1
+        T ue, un, ut;   // factors
         int c1;
     };
     const uint32_t *cells;
@@ -277,9 +285,11 @@ struct BwdK {
 
     __device__ __forceinline__ void load(Data &d, int s) const {
         d.c = ld(cells, s);
-        d.ue = ld(ue, s);
-        d.un = ld(un, s);
-        d.ut = ld(ut, s);
+        if constexpr (!LATE) {
+            d.ue = ld(ue, s);
+            d.un = ld(un, s);
+            d.ut = ld(ut, s);
+        }
     }
     __device__ __forceinline__ void far_load(Far &f, int s) const {
         f.res = ld(res, s);
@@ -289,8 +299,17 @@ struct BwdK {
         d.res = f.res;
         d.h = f.h;
     }
-    __device__ __forceinline__ G gather(const Data &d, int, int p) const {
+    __device__ __forceinline__ G gather(const Data &d, int s, int p) const {
         G g;
+        if constexpr (LATE) {
+            g.ue = ld(ue, s);
+            g.un = ld(un, s);
+            g.ut = ld(ut, s);
+        } else {
+            g.ue = d.ue;
+            g.un = d.un;
+            g.ut = d.ut;
+        }
         int i, j, k;
         cell_ijk(d.c, i, j, k);
         g.c1 = j + __mul24(RW, i);
@@ -304,13 +323,13 @@ struct BwdK {
                                            bool ok) const {
         T r = d.res;
         if (CLASSIC) {
-            r = fms(r, d.un, g.rn);
-            r = fms(r, d.ue, g.re);
-            r = fms(r, d.ut, g.rt);
+            r = fms(r, g.un, g.rn);
+            r = fms(r, g.ue, g.re);
+            r = fms(r, g.ut, g.rt);
         } else {
-            r = fms(r, d.ut, g.rt);
-            r = fms(r, d.un, g.rn);
-            r = fms(r, d.ue, g.re);
+            r = fms(r, g.ut, g.rt);
+            r = fms(r, g.un, g.rn);
+            r = fms(r, g.ue, g.re);
         }
         const T h = (HZ ? T(0) : d.h) + r;
         if (ok) {
@@ -330,7 +349,7 @@ struct BwdK {
     __device__ __forceinline__ void face_put(uint32_t f, T v) const { st(hface, face_dst(f), v); }
 };
 
-template <typename T, bool CLASSIC, bool HZ, int D, int M, int NW>
+template <typename T, bool CLASSIC, bool HZ, int D, int M, int NW, bool LATE = false>
 __global__ __launch_bounds__(kWave * NW) void k_backward(
     const uint2 *__restrict__ items, Topo t, Fields<T> F, const int *done) {
     if (*done) return;
@@ -346,7 +365,7 @@ __global__ __launch_bounds__(kWave * NW) void k_backward(
     if (NW > 1) __syncthreads(); else wave_sync();
     const int NI = B.ni, NJ = B.nj, NK = B.nk;
     const int64_t wo = B.wo;
-    BwdK<T, CLASSIC, HZ> k;
+    BwdK<T, CLASSIC, HZ, LATE> k;
     k.cells = t.cellx + S.cell_off;
     k.res = F.res + wo;
     k.ue = F.f[4] + B.fo;
@@ -731,9 +750,18 @@ template <typename T, bool CLASSIC, bool HZ>
 static void bwd_launch(const Topo &t, const Fields<T> &f, const int *done,
                        size_t lds, hipStream_t s) {
     // each wave stages its item's HILF values (64 M) for the face stores
-#define MGP_BWD(D, M, NW) \
-    k_backward<T, CLASSIC, HZ, D, M, NW><<<t.nlist, kWave * NW, \
+#define MGP_BWD0(D, M, NW, L) \
+    k_backward<T, CLASSIC, HZ, D, M, NW, L><<<t.nlist, kWave * NW, \
         lds + (size_t)NW * kWave * M * sizeof(T), s>>>(t.items, t, f, done)
+    // FP64 with factors shared between blocks: late factor loads
+#define MGP_BWD(D, M, NW) \
+    if constexpr (sizeof(T) == 8) { \
+        if (f.aps) { \
+            MGP_BWD0(D, M, NW, true); \
+            break; \
+        } \
+    } \
+    MGP_BWD0(D, M, NW, false)
     switch (g_shape_bwd) {
     case 1: MGP_BWD(4, 1, 1); break;
     case 3: MGP_BWD(3, 2, 1); break;
@@ -746,6 +774,7 @@ static void bwd_launch(const Topo &t, const Fields<T> &f, const int *done,
     default: MGP_BWD(2, 2, 1); break;
     }
 #undef MGP_BWD
+#undef MGP_BWD0
 }
 
 template <typename T>
