@@ -7,8 +7,8 @@
 !   finish_pressuresolver()                         (:148-157)
 ! Inside MGLET, init_pressuresolver takes its arguments from the grid
 ! module, the pointer module and fort7; here they are explicit so the module
-! builds on its own (the MGLET-side glue that fills them is shown in
-! INTEGRATION.md).  Fields are the flat `field_t%arr` arrays in the
+! builds on its own (the MGLET-side glue fills them from those modules).
+! Fields are the flat `field_t%arr` arrays in the
 ! reference layout; errors stop the program through `errr`-like handling.
 MODULE mgp_pressuresolver_mod
     USE, INTRINSIC :: ISO_C_BINDING
