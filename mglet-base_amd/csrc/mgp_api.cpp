@@ -721,6 +721,25 @@ void exchange(mgp_handle &h, int level, int layers, int corners, int forward,
     }
 }
 
+// LDS per workgroup that the factorisation and the plane kernels need for
+// the handle's largest block (the sums of their launchers, the HILF window
+// at its four-slot maximum).
+size_t plane_lds_need(const mgp_handle &h) {
+    const size_t T = h.tsz();
+    const size_t mRS = (size_t)(h.max_nj + 1) * (h.max_ni + 1);
+    const size_t mHS = (size_t)(h.max_nj + 2) * (h.max_ni + 2);
+    const int m = std::max(std::max(h.max_ni, h.max_nj), h.max_nk);
+    const size_t nr = (size_t)((m + 1) & ~1);
+    const size_t tab = 24 * sizeof(void *);   // source table: three rows of 8 pointers
+    const size_t mb = h.masked ? mHS : 0;
+    size_t need = 6 * mRS * sizeof(double);                                          // k_lu
+    need = std::max(need, tab + 6 * nr * T + 2 * mRS * T + 3 * nr * 8 + 4 * (mHS * T + mb));  // forward
+    need = std::max(need, tab + 9 * nr * 8 + 2 * mRS * T + 4 * (mHS * T + mb));     // fused outer
+    if (h.cfg.ityp == 1)
+        need = std::max(need, tab + 6 * nr * T + 8 * mHS * T + (h.masked ? 8 * mHS : 3 * nr * 8));
+    return need;
+}
+
 // Decided once per handle: whether mgpoisl's velocity and pressure
 // correction runs along z-chains and writes the k-face ghosts of interior
 // rows for the closing exchange (k_mgpcorr_chain).  It does on single-level
@@ -1307,12 +1326,17 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
         h->ip1y.push_back(G->ip1y[g] - 1);
         h->ip1z.push_back(G->ip1z[g] - 1);
         const int ni = h->ii[g] - 4, nj = h->jj[g] - 4, nk = h->kk[g] - 4;
-        // plane kernels: 1-D coefficient rows of kMaxN1 entries, pencil
-        // arrays of (ni+2)(nj+2) cells in LDS, 16-bit plan offsets
+        // plane kernels: 1-D coefficient rows of kMaxN1 entries, 16-bit
+        // cell positions in plan items and entries, 12-bit LDS positions in
+        // the pencil and window planes of (ni+2)(nj+2) cells, 14-bit
+        // offsets into the block's face region (the LDS the kernels need
+        // is checked against the device once the shapes are known)
         if (ni < 2 || nj < 2 || nk < 2 || ni > kMaxN1 || nj > kMaxN1 || nk > kMaxN1 ||
-            ni * nj > 1024)
+            ni * nj * nk > 65536 || (ni + 2) * (nj + 2) > 4096 ||
+            2 * (nk * nj + nk * ni + nj * ni) > 16384)
             return fail(MGP_ERR_TOPOLOGY, "grid %d: interior %dx%dx%d outside the supported "
-                        "block sizes (each <= %d, ni*nj <= 1024)", g + 1, ni, nj, nk, kMaxN1);
+                        "block sizes (each 2..%d, at most 65536 cells)", g + 1, ni, nj, nk,
+                        kMaxN1);
         if (h->level[g] < h->minlevel || h->level[g] > h->maxlevel)
             return fail(MGP_ERR_TOPOLOGY, "grid %d: level out of range", g + 1);
         if (g > 0 && h->level[g] < h->level[g - 1])
@@ -1427,6 +1451,16 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
             for (auto &B : h->blocks) B.flags |= 1;
         }
 
+        {
+            int dev = 0, lim = 0;
+            HC(hipGetDevice(&dev));
+            HC(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+            const size_t need = plane_lds_need(*h);
+            if (need > (size_t)lim)
+                return fail(MGP_ERR_TOPOLOGY, "blocks of %dx%d faces need %zu bytes of LDS per "
+                            "workgroup in the plane kernels; the device allows %d",
+                            h->max_ni, h->max_nj, need, lim);
+        }
         upload(h->d_blocks, h->blocks);
         upload(h->d_shapes, h->shapes);
         upload(h->d_cells, cells);

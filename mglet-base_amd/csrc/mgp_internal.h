@@ -68,14 +68,14 @@ constexpr int kPlanWaves[4] = {1, 2, 4, 8};
 // Face-store entry of the plane kernels (kFaceSlots per plan item, one per
 // lane, built on the host): the LDS position of a value the item computed
 // (12 bits: in the step's recurrence or window slot, or the item's staging
-// row), the value's offset in the block's face region (13 bits) and bit 25
+// row), the value's offset in the block's face region (14 bits) and bit 26
 // for an upper face (the forward sweep's second destination); kFaceNone
-// where the lane stores nothing.  No item of any supported block shape has
-// more than 45 face cells.
+// where the lane stores nothing.  The plan builder rejects an item with
+// more face cells than lanes (none of 24^3 has more than 45).
 constexpr int kFaceSlots = 64;
 constexpr uint32_t kFaceNone = 0xffffffffu;
 __host__ __device__ inline uint32_t face_entry(int src, int dst, int hi) {
-    return (uint32_t)src | (uint32_t)dst << 12 | (uint32_t)hi << 25;
+    return (uint32_t)src | (uint32_t)dst << 12 | (uint32_t)hi << 26;
 }
 __host__ __device__ inline uint32_t plan_entry(int lds, int kind, int tgt, int off) {
     return (uint32_t)lds | ((uint32_t)kind << 12) | ((uint32_t)tgt << 15) |

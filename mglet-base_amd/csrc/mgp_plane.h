@@ -328,8 +328,8 @@ __device__ __forceinline__ int clamp_row(int l, int n) {
 
 // Fields of a face-store entry (face_entry, mgp_internal.h).
 __device__ __forceinline__ int face_src(uint32_t f) { return f & 0xfff; }
-__device__ __forceinline__ int face_dst(uint32_t f) { return (f >> 12) & 0x1fff; }
-__device__ __forceinline__ bool face_hi(uint32_t f) { return (f >> 25) & 1; }
+__device__ __forceinline__ int face_dst(uint32_t f) { return (f >> 12) & 0x3fff; }
+__device__ __forceinline__ bool face_hi(uint32_t f) { return (f >> 26) & 1; }
 
 // The plan engine: runs the items of one block through functor K.  An
 // item covers up to 64 M cells (M per lane) and 64 M entries.  K provides

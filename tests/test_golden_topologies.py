@@ -75,13 +75,21 @@ def test_exchange_on_reference_topology(oracle_mod, name, layers, corners):
                 assert (v[tuple(sl)] != 0).all(), (name, b, f + 1, layer)
 
 
+def _block_ok(ni, nj, nk):
+    """The block sizes mgp_init takes (before its LDS check against the
+    device): interior counts 2..64, at most 65536 cells (16-bit cell
+    positions), (ni+2)(nj+2) <= 4096 (12-bit LDS positions) and a face
+    region of at most 16384 values (14-bit offsets)."""
+    return (min(ni, nj, nk) >= 2 and max(ni, nj, nk) <= 64 and ni * nj * nk <= 65536 and
+            (ni + 2) * (nj + 2) <= 4096 and 2 * (nk * nj + nk * ni + nj * ni) <= 16384)
+
+
 def test_unsupported_reference_topologies_are_named():
     """The cases outside SUPPORTED_TOPOLOGIES have a block the plane kernels
-    do not take: one interior cell along an axis (2-D cases) or a face of
-    more than 1024 cells."""
+    do not take: one interior cell along an axis (the 2-D cases)."""
     for name, case in golden_topologies().items():
         dims = [(r["II"] - 4, r["JJ"] - 4, r["KK"] - 4) for r in case["grids"]]
-        ok = all(min(d) >= 2 and d[0] * d[1] <= 1024 and max(d) <= 64 for d in dims)
+        ok = all(_block_ok(*d) for d in dims)
         assert ok == (name in SUPPORTED_TOPOLOGIES), name
 
 

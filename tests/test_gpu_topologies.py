@@ -66,7 +66,8 @@ def test_mgpoisl_reference_topology_sor_bitwise(oracle_mod, name):
 
 
 @pytest.mark.parametrize("name", ["Parker_micro", "Santarelli", "Checkblock",
-                                  "HotAndColdSpheres", "TaylorGreen", "GeostrophicWind"])
+                                  "HotAndColdSpheres", "TaylorGreen", "GeostrophicWind",
+                                  "sphere"])
 def test_fp32_refined_converged_reference_topology(oracle_mod, name):
     """FP32 sweeps inside the FP64 correction reach 1e-10 on the reference's
     grids and agree with the FP64 solve (the pressure modulo its mean where
