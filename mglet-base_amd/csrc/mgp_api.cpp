@@ -21,6 +21,7 @@
 #include <memory>
 #include <string>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/mgp.h"
@@ -608,9 +609,114 @@ mgp_handle::TaskSet &task_set(mgp_handle &h, int level, int layers,
             for (const Box &b : pushed[q.sg]) valid = valid && !overlap(q.sb, b);
             for (const Box &b : pushed[q.dg]) valid = valid && !overlap(q.db, b);
         }
+    // A source box that reaches into a destination box of its own grid (a
+    // two-layer exchange of blocks one cell thick along an axis, whose
+    // source boxes take a ghost layer, also a copy's own destination) makes
+    // the result depend on the element order inside a copy.
+    std::vector<std::vector<int>> wr(h.ngrid + 1), rdl(h.ngrid + 1);
+    bool hazard = false;
+    for (size_t q = 0; q < list.size(); ++q) wr[list[q].dg].push_back((int)q);
+    for (size_t q = 0; q < list.size() && !hazard; ++q)
+        for (int u : wr[list[q].sg])
+            if (overlap(list[u].db, list[q].sb)) { hazard = true; break; }
+    if (hazard) {
+        // Then the copies run one element at a time in the reference's
+        // order (connections in turn; i, j, k with k fastest) on the host:
+        // each written position ends with the value of one original
+        // position, and an entry that reads a position another entry writes
+        // runs in an earlier stage (longest chain).
+        struct Orig { int64_t pos; int grid; };
+        std::unordered_map<int64_t, Orig> cur;
+        std::vector<std::pair<int64_t, int>> written;   // position, receiving grid
+        for (const T2 &q : list) {
+            const CopyTask &t = q.t;
+            for (int i = 0; i < t.ni; ++i)
+                for (int j = 0; j < t.nj; ++j)
+                    for (int k = 0; k < t.nk; ++k) {
+                        const int64_t d = t.dst + k + (int64_t)j * t.d_sj + (int64_t)i * t.d_si;
+                        const int64_t so = t.src + k + (int64_t)j * t.s_sj + (int64_t)i * t.s_si;
+                        const auto f = cur.find(so);
+                        const Orig o = f == cur.end() ? Orig{so, q.sg} : f->second;
+                        if (cur.insert_or_assign(d, o).second) written.push_back({d, q.dg});
+                    }
+        }
+        std::vector<int64_t> ed, eo;
+        std::vector<int> eg, esg;
+        std::unordered_map<int64_t, int> at;
+        for (const auto &w : written) {
+            const Orig o = cur.at(w.first);
+            if (o.pos == w.first) continue;   // ends with its own value
+            at[w.first] = (int)ed.size();
+            ed.push_back(w.first);
+            eo.push_back(o.pos);
+            eg.push_back(w.second);
+            esg.push_back(o.grid);
+        }
+        const size_t ne = ed.size();
+        std::vector<int> nxt(ne, -1), indeg(ne, 0), lev(ne, 0), todo;
+        for (size_t e = 0; e < ne; ++e) {
+            const auto f = at.find(eo[e]);
+            if (f != at.end()) { nxt[e] = f->second; ++indeg[f->second]; }
+        }
+        for (size_t e = 0; e < ne; ++e)
+            if (!indeg[e]) todo.push_back((int)e);
+        size_t done = 0;
+        int nst = ne ? 1 : 0;
+        while (!todo.empty()) {
+            const int e = todo.back();
+            todo.pop_back();
+            ++done;
+            const int x = nxt[e];
+            if (x < 0) continue;
+            lev[x] = std::max(lev[x], lev[e] + 1);
+            nst = std::max(nst, lev[x] + 1);
+            if (--indeg[x] == 0) todo.push_back(x);
+        }
+        if (done != ne) throw HipError{"cyclic ghost copies"};
+        auto ts = std::make_unique<mgp_handle::TaskSet>();
+        ts->valid = valid;
+        ts->pairs = false;
+        std::vector<PullBlock> pbs;
+        std::vector<uint2> ents;
+        for (int st = 0; st < nst; ++st) {
+            ts->stage_start.push_back((int)pbs.size());
+            std::map<int, std::vector<int>> byrecv;
+            for (size_t e = 0; e < ne; ++e)
+                if (lev[e] == st) byrecv[eg[e]].push_back((int)e);
+            for (const auto &kv : byrecv) {
+                PullBlock pb{};
+                pb.dst = h.ip3[kv.first - 1];
+                pb.tab = (int)ents.size();
+                std::vector<int> slots;   // source grids
+                std::vector<uint2> e2;
+                for (const int e : kv.second) {
+                    const int sl = (int)(std::find(slots.begin(), slots.end(), esg[e]) - slots.begin());
+                    if (sl == (int)slots.size()) {
+                        if (sl == 26) throw HipError{"more than 26 ghost copies into one block"};
+                        slots.push_back(esg[e]);
+                        pb.src[sl] = h.ip3[esg[e] - 1];
+                    }
+                    const int64_t d = ed[e] - pb.dst, so = eo[e] - pb.src[sl];
+                    if (so < 0 || so >= (1 << kPullSlotBits) || d < 0 || d > INT32_MAX)
+                        throw HipError{"ghost copy offsets beyond the entry format"};
+                    e2.push_back(make_uint2((uint32_t)d, (uint32_t)so | (uint32_t)sl << kPullSlotBits));
+                }
+                std::sort(e2.begin(), e2.end(), [](const uint2 &a, const uint2 &b) { return a.x < b.x; });
+                ents.insert(ents.end(), e2.begin(), e2.end());
+                pb.n = (int)e2.size();
+                pbs.push_back(pb);
+            }
+        }
+        ts->stage_start.push_back((int)pbs.size());
+        upload(ts->blocks, pbs);
+        upload(ts->ents, ents);
+        auto &ref = *ts;
+        h.tasks[key] = std::move(ts);
+        return ref;
+    }
+    for (auto &w : wr) w.clear();
     // staging
     std::vector<int> stage(list.size(), 0);
-    std::vector<std::vector<int>> wr(h.ngrid + 1), rdl(h.ngrid + 1);
     int nst = list.empty() ? 0 : 1;
     for (size_t q = 0; q < list.size(); ++q) {
         int st = 0;
@@ -957,14 +1063,39 @@ void build_plans(mgp_handle &h, const std::vector<uint32_t> &cells,
             // same step where the step's compute items cannot take them.
             const bool three = (kind == kPlanFwdLap || kind == kPlanOut) && !h.fp32();
             const int HS = (NI + 2) * HW;
+            // cell chunks (first, count) of a step: at most W cells and
+            // kFaceSlots face entries each (a chunk ends early only where
+            // a block is one cell thick along an axis and all its cells
+            // lie on two faces)
+            auto chunks = [&](const Step &st, int W) {
+                std::vector<std::pair<int, int>> ch;
+                for (int t = 0; t < st.ccnt;) {
+                    int n = 0, f = 0;
+                    while (t + n < st.ccnt && n < W) {
+                        const uint32_t w = cl[st.cpos + t + n];
+                        const int i = w & 1023, j = (w >> 10) & 1023, k = w >> 20;
+                        const int nf = (i == 0) + (i == NI - 1) + (j == 0) + (j == NJ - 1) +
+                                       (k == 0) + (k == NK - 1);
+                        if (f + nf > kFaceSlots) break;
+                        f += nf;
+                        ++n;
+                    }
+                    ch.push_back({t, n});
+                    t += n;
+                }
+                return ch;
+            };
             auto scheduled = [&](int W) {
                 std::vector<int> cellitem(HS);
                 for (const Step &st : steps) {
                     std::fill(cellitem.begin(), cellitem.end(), -1);
-                    for (int t = 0; t < st.ccnt; ++t) {
-                        const uint32_t w = cl[st.cpos + t];
-                        cellitem[((w >> 10) & 1023) + 1 + HW * ((w & 1023) + 1)] = t / W;
-                    }
+                    const auto ch = chunks(st, W);
+                    const int nc = (int)ch.size();
+                    for (int c = 0; c < nc; ++c)
+                        for (int t = ch[c].first; t < ch[c].first + ch[c].second; ++t) {
+                            const uint32_t w = cl[st.cpos + t];
+                            cellitem[((w >> 10) & 1023) + 1 + HW * ((w & 1023) + 1)] = c;
+                        }
                     std::vector<std::pair<int, uint32_t>> es;
                     for (int q = 0; q < st.ecnt; ++q) {
                         const uint32_t e = ents[ebase0 + st.epos + q];
@@ -978,7 +1109,6 @@ void build_plans(mgp_handle &h, const std::vector<uint32_t> &cells,
                     }
                     std::stable_sort(es.begin(), es.end(),
                                      [](const auto &a, const auto &b) { return a.first < b.first; });
-                    const int nc = (st.ccnt + W - 1) / W;
                     size_t idx = 0;
                     for (int c = 0; c < nc || idx < es.size(); ++c) {
                         const int epos = (int)ents.size() - ebase0;
@@ -987,8 +1117,8 @@ void build_plans(mgp_handle &h, const std::vector<uint32_t> &cells,
                             ents.push_back(es[idx++].second);
                             ++ec;
                         }
-                        const int cc = c < nc ? std::min(st.ccnt - W * c, W) : 0;
-                        items.push_back(pack(cc ? st.cpos + W * c : 0, cc, ec, ec ? epos : 0, st.p));
+                        const int cc = c < nc ? ch[c].second : 0;
+                        items.push_back(pack(cc ? st.cpos + ch[c].first : 0, cc, ec, ec ? epos : 0, st.p));
                     }
                 }
             };
@@ -1000,16 +1130,18 @@ void build_plans(mgp_handle &h, const std::vector<uint32_t> &cells,
                     S.plan_item[nwi][wi][kind] = (int)base;
                     if (three && NW == 1) scheduled(W);
                     else for (const Step &st : steps) {
-                        const int n = std::max((st.ccnt + W - 1) / W, (st.ecnt + W - 1) / W);
+                        const auto ch = chunks(st, W);
+                        const int nc = (int)ch.size();
+                        const int n = std::max(nc, (st.ecnt + W - 1) / W);
                         const int rows = (n + NW - 1) / NW;
                         for (int c = 0; c < rows * NW; ++c) {
                             if (c >= n) {   // padding: a no-op of the same step
                                 items.push_back(pack(0, 0, 0, 0, st.p));
                                 continue;
                             }
-                            const int cc = std::min(std::max(st.ccnt - W * c, 0), W);
+                            const int cc = c < nc ? ch[c].second : 0;
                             const int ec = std::min(std::max(st.ecnt - W * c, 0), W);
-                            items.push_back(pack(cc ? st.cpos + W * c : 0, cc, ec,
+                            items.push_back(pack(cc ? st.cpos + ch[c].first : 0, cc, ec,
                                                  ec ? st.epos + W * c : 0, st.p));
                         }
                     }
@@ -1331,11 +1463,11 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
         // the pencil and window planes of (ni+2)(nj+2) cells, 14-bit
         // offsets into the block's face region (the LDS the kernels need
         // is checked against the device once the shapes are known)
-        if (ni < 2 || nj < 2 || nk < 2 || ni > kMaxN1 || nj > kMaxN1 || nk > kMaxN1 ||
+        if (ni < 1 || nj < 1 || nk < 1 || ni > kMaxN1 || nj > kMaxN1 || nk > kMaxN1 ||
             ni * nj * nk > 65536 || (ni + 2) * (nj + 2) > 4096 ||
             2 * (nk * nj + nk * ni + nj * ni) > 16384)
             return fail(MGP_ERR_TOPOLOGY, "grid %d: interior %dx%dx%d outside the supported "
-                        "block sizes (each 2..%d, at most 65536 cells)", g + 1, ni, nj, nk,
+                        "block sizes (each 1..%d, at most 65536 cells)", g + 1, ni, nj, nk,
                         kMaxN1);
         if (h->level[g] < h->minlevel || h->level[g] > h->maxlevel)
             return fail(MGP_ERR_TOPOLOGY, "grid %d: level out of range", g + 1);

@@ -235,11 +235,13 @@ def mask_levels(g, o, blocked):
 BC_CODES = {"FIX": 2, "OP1": 3, "NOS": 5, "SLI": 6, "CON": 7, "PAR": 8, "GRA": 15,
             "SCA": 16, "REF": 17, "NRE": 18, "CO1": 19, "NIX": 99}
 # cases of the reference's tests/*/grids.h5 whose blocks the plane kernels
-# take (every interior count 2..64, at most 65536 cells, (ni+2)(nj+2) <= 4096,
+# take (every interior count 1..64, at most 65536 cells, (ni+2)(nj+2) <= 4096,
 # face region <= 16384; sphere's 40^3 blocks need 87 KB of LDS per workgroup
-# in FP64, which the MI355X's 160 KB allows)
-SUPPORTED_TOPOLOGIES = ["Checkblock", "GeostrophicWind", "HotAndColdSpheres", "Parker_micro",
-                        "Santarelli", "ScalarDecay", "Statistics", "TaylorGreen", "sphere"]
+# in FP64, which the MI355X's 160 KB allows; BlasiusBL, Inflow-ramp and
+# NaturalConvection are one cell thick along y, or along y and z)
+SUPPORTED_TOPOLOGIES = ["BlasiusBL", "Checkblock", "GeostrophicWind", "HotAndColdSpheres",
+                        "Inflow-ramp", "NaturalConvection", "Parker_micro", "Santarelli",
+                        "ScalarDecay", "Statistics", "TaylorGreen", "sphere"]
 
 
 def golden_topologies():

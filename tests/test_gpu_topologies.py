@@ -57,7 +57,7 @@ def test_mgpoisl_reference_topology_bitwise(oracle_mod, name, ityp):
     _solve(oracle_mod, g, 92 + ityp, ityp=ityp)
 
 
-@pytest.mark.parametrize("name", ["Parker_micro", "Checkblock", "Statistics"])
+@pytest.mark.parametrize("name", ["Parker_micro", "Checkblock", "Statistics", "BlasiusBL"])
 def test_mgpoisl_reference_topology_sor_bitwise(oracle_mod, name):
     """type "sor": SOR on the refined levels of the reference's nested grids."""
     g = reference_topology(name)
@@ -67,7 +67,7 @@ def test_mgpoisl_reference_topology_sor_bitwise(oracle_mod, name):
 
 @pytest.mark.parametrize("name", ["Parker_micro", "Santarelli", "Checkblock",
                                   "HotAndColdSpheres", "TaylorGreen", "GeostrophicWind",
-                                  "sphere"])
+                                  "sphere", "NaturalConvection"])
 def test_fp32_refined_converged_reference_topology(oracle_mod, name):
     """FP32 sweeps inside the FP64 correction reach 1e-10 on the reference's
     grids and agree with the FP64 solve (the pressure modulo its mean where
