@@ -838,7 +838,9 @@ size_t plane_lds_need(const mgp_handle &h) {
     const size_t nr = (size_t)((m + 1) & ~1);
     const size_t tab = 24 * sizeof(void *);   // source table: three rows of 8 pointers
     const size_t mb = h.masked ? mHS : 0;
-    size_t need = 6 * mRS * sizeof(double);                                          // k_lu
+    const size_t maxc = (size_t)h.max_ni * h.max_nj * h.max_nk;
+    size_t need = std::min(maxc, (size_t)kStage) * sizeof(double);                    // divcal
+    need = std::max(need, 6 * mRS * sizeof(double));                                 // k_lu
     need = std::max(need, tab + 6 * nr * T + 2 * mRS * T + 3 * nr * 8 + 4 * (mHS * T + mb));  // forward
     need = std::max(need, tab + 9 * nr * 8 + 2 * mRS * T + 4 * (mHS * T + mb));     // fused outer
     if (h.cfg.ityp == 1)
@@ -1590,7 +1592,7 @@ int init_common(const mgp_grids *G, const mgp_config *cfg, const double *dx,
             const size_t need = plane_lds_need(*h);
             if (need > (size_t)lim)
                 return fail(MGP_ERR_TOPOLOGY, "blocks of %dx%d faces need %zu bytes of LDS per "
-                            "workgroup in the plane kernels; the device allows %d",
+                            "workgroup in the plane and layout kernels; the device allows %d",
                             h->max_ni, h->max_nj, need, lim);
         }
         upload(h->d_blocks, h->blocks);

@@ -397,7 +397,6 @@ MGP_INST_FSHARE(float)
 // up to kStage WO slots of the block at a time (blocks with more cells take
 // several passes).
 constexpr int kStageThreads = 1024;
-constexpr int kStage = 8192;   // 64 KB of doubles
 
 // RHS = fak * div(u, v, w): src/ib/noib_mod.F90:304-354 (noib) and the
 // ghostcell variant src/ib/gc_mod.F90:278-347 (+ fak*SDIV, the caller's

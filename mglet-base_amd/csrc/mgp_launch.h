@@ -161,6 +161,14 @@ void launch_wo_to_sp(const Topo &t, const T *src, double *dst,
 // block indices, chains[c] = (first entry, length) of chain c, each a cycle
 // along +z of at most kCorrChainMax blocks of one size with nk >= 2; nx
 // workgroups per chain.
+// WO slots the layout-change kernels (divcal, WO -> reference copy) stage
+// in LDS per pass: 16384 doubles, 128 KB of the MI355X's 160 KB per
+// workgroup, so a block of up to 16384 cells (24^3 = 13824) takes one
+// pass.  Each pass reads (divcal) or writes (the copy) the parts of the
+// reference layout's k-lines whose cells fall in its slot range, so two
+// passes move every line twice (8192 slots: divcal 5.74 ms, 4.42 ms with
+// one pass on 32768 blocks of 24^3).
+constexpr int kStage = 16384;
 constexpr int kCorrThreads = 256;   // workgroup of the correction kernels
 constexpr int kCorrChainMax = 256;  // longest chain
 constexpr int kCorrRows = 16;       // most rows per workgroup of the chain update
