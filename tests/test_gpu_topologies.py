@@ -31,16 +31,20 @@ def test_connect_reference_topology(oracle_mod, name):
     s.close()
 
 
-def _solve(oracle_mod, g, seed, ityp=0, ninner=3, nouter=2, omega=1.1):
+def _solve(oracle_mod, g, seed, ityp=0, ninner=3, nouter=2, omega=1.1, shape=None,
+           sor_all_levels=0):
     import mgp
     o = oracle_mod.Oracle(g)
     st = o.setup(ityp=ityp)
     u, v, w = velocity_levels(g, o, seed)
     p = np.zeros(g.n3d)
     got = [a.copy() for a in (u, v, w, p)]
-    so = o.mgpoisl(st, u, v, w, p, 1e-3, ityp=ityp, ninner=ninner, nouter=nouter, omega=omega)
+    so = o.mgpoisl(st, u, v, w, p, 1e-3, ityp=ityp, ninner=ninner, nouter=nouter, omega=omega,
+                   sor_all_levels=sor_all_levels)
     s = mgp.PressureSolver(g, type={0: "sip", 1: "sor", 2: "classic"}[ityp], ninner=ninner,
-                           nouter=nouter, omega=omega)
+                           nouter=nouter, omega=omega, sor_all_levels=sor_all_levels)
+    if shape is not None:
+        assert s.plan_shape(shape)[0] == shape
     sg = s.mgpoisl(*got, 1e-3)
     assert sg["maxrhs0"] == so["maxrhs0"] and sg["maxrhs"] == so["maxrhs"]
     assert np.array_equal(s.history(), so["history"])
@@ -63,6 +67,19 @@ def test_mgpoisl_reference_topology_sor_bitwise(oracle_mod, name):
     g = reference_topology(name)
     print(f"[topology] {name} sor", flush=True)
     _solve(oracle_mod, g, 95, ityp=1, ninner=4, omega=1.2)
+
+
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("ityp", [0, 1, 2])
+def test_thin_block_plan_shapes_bitwise(oracle_mod, shape, ityp):
+    """Every pipeline shape on NaturalConvection's 32 x 1 x 32 block, whose
+    plan items end early (every cell lies on both y faces, so 64 cells would
+    need more than an item's 64 face-store entries): SIP, SOR (on its only
+    level) and classic SIP, bitwise against the oracle."""
+    g = reference_topology("NaturalConvection")
+    print(f"[topology] NaturalConvection shape {shape} ityp={ityp}", flush=True)
+    _solve(oracle_mod, g, 97, ityp=ityp, ninner=3, nouter=2, omega=1.2, shape=shape,
+           sor_all_levels=int(ityp == 1))
 
 
 @pytest.mark.parametrize("name", ["Parker_micro", "Santarelli", "Checkblock",
