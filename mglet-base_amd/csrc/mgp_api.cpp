@@ -2056,8 +2056,7 @@ void dp_to_reference(mgp_handle &h) {
     const hipStream_t s = h.stream;
     Topo tall = h.topo(-1);
     if (h.d_dpsp.n < (size_t)h.n3d * 8) h.d_dpsp.alloc((size_t)h.n3d * 8);
-    launch_fill<double>(h.d_dpsp.as<double>(), h.n3d, 0.0, s);
-    launch_wo_to_sp<double>(tall, h.d_dp64.as<double>(), h.d_dpsp.as<double>(), s);
+    launch_wo_to_sp<double>(tall, h.d_dp64.as<double>(), h.d_dpsp.as<double>(), s, true);
     if (h.d_dsh.p) launch_shell_to_sp(tall, h.d_dsh.as<double>(), h.d_dpsp.as<double>(), s);
     double *fd[1] = {h.d_dpsp.as<double>()};
     exchange(h, -1, 1, 0, 0, fd, 1);

@@ -601,9 +601,11 @@ void launch_check(Outer o, int nouter, int nouter_min, double epcorr,
 }
 
 // ---------------------------------------------------------------------
-// WO -> reference layout (interior only; ghosts are left as they are),
-// through LDS as k_divcal.
-template <typename T>
+// WO -> reference layout, through LDS as k_divcal: the interior, with the
+// ghosts left as they are (ZG false) or set to zero (ZG true: the block's
+// whole region is written in reference order, in whole lines, where the
+// block fits one stage).
+template <typename T, bool ZG>
 __global__ __launch_bounds__(kStageThreads) void k_wo_to_sp(Topo t, const T *src,
                                                             double *dst) {
     extern __shared__ double st_ws[];
@@ -611,8 +613,24 @@ __global__ __launch_bounds__(kStageThreads) void k_wo_to_sp(Topo t, const T *src
     const Block &B = t.blocks[b];
     const Shape &S = t.shapes[B.shape];
     const int *slot_of = t.slot_of + S.slot_off;
-    const int NJ = B.nj, NK = B.nk, n = S.ncell;
+    const int NI = B.ni, NJ = B.nj, NK = B.nk, n = S.ncell;
     const int64_t kk = B.nk + 4, jj = B.nj + 4;
+    if (ZG) {
+        const int tot = (int)(kk * jj) * (NI + 4);
+        const bool one = n <= kStage;
+        if (one) {
+            for (int q = threadIdx.x; q < n; q += blockDim.x)
+                st_ws[q] = (double)src[B.wo + q];
+            __syncthreads();
+        }
+        for (int p = threadIdx.x; p < tot; p += blockDim.x) {
+            const int k = p % (int)kk, r = p / (int)kk, j = r % (int)jj, i = r / (int)jj;
+            const bool in = k >= 2 && k < NK + 2 && j >= 2 && j < NJ + 2 && i >= 2 && i < NI + 2;
+            if (in && !one) continue;   // the passes below
+            dst[B.sp + p] = in ? st_ws[slot_of[(k - 2) + NK * ((j - 2) + NJ * (i - 2))]] : 0.0;
+        }
+        if (one) return;
+    }
     for (int base = 0; base < n; base += kStage) {
         const int hi = n < base + kStage ? n : base + kStage;
         for (int q = threadIdx.x; q < hi - base; q += blockDim.x)
@@ -629,16 +647,19 @@ __global__ __launch_bounds__(kStageThreads) void k_wo_to_sp(Topo t, const T *src
 }
 template <typename T>
 void launch_wo_to_sp(const Topo &t, const T *src, double *dst,
-                     hipStream_t s) {
+                     hipStream_t s, bool zero_ghosts) {
     MGP_RANGE("mgp:wo_to_sp");
-    k_wo_to_sp<T><<<t.nlist, kStageThreads, stage_bytes(t), s>>>(t, src, dst);
+    if (zero_ghosts)
+        k_wo_to_sp<T, true><<<t.nlist, kStageThreads, stage_bytes(t), s>>>(t, src, dst);
+    else
+        k_wo_to_sp<T, false><<<t.nlist, kStageThreads, stage_bytes(t), s>>>(t, src, dst);
 }
 template void launch_wo_to_sp<double>(const Topo &, const double *, double *,
-                                      hipStream_t);
+                                      hipStream_t, bool);
 template void launch_wo_to_sp<float>(const Topo &, const float *, double *,
-                                     hipStream_t);
+                                     hipStream_t, bool);
 template void launch_wo_to_sp<uint8_t>(const Topo &, const uint8_t *,
-                                       double *, hipStream_t);
+                                       double *, hipStream_t, bool);
 
 // Factors back to the reference's storage: hyperplane access order in the
 // first ncell entries of the block (src/flow/sip_hyperplane_mod.F90:

@@ -152,9 +152,10 @@ template <typename T>
 void launch_outer_fwd(const Topo &t, const Fields<T> &f, Mask m, Outer o, T *rhs_t,
                       int rd, int write_hi, int classic, hipStream_t s);
 
+// zero_ghosts: also set every non-interior position of each block to 0
 template <typename T>
 void launch_wo_to_sp(const Topo &t, const T *src, double *dst,
-                     hipStream_t s);
+                     hipStream_t s, bool zero_ghosts = false);
 
 // z-chains of blocks for the velocity and pressure correction that also
 // writes the k-face ghosts of interior rows (k_mgpcorr_chain): chain[] holds
